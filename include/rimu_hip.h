@@ -1,0 +1,166 @@
+/*
+ * rimu_hip.h -- C ABI of the MI355X (gfx950) FCIQMC step engine.
+ *
+ * The library replaces the vector-matrix product of one FCIQMC step of Rimu.jl
+ * (apply_operator!, Interfaces/dictvectors.jl:112-154) together with the reductions that
+ * advance! performs right after it (walkernumber_and_length, fciqmc.jl:144; the
+ * ProjectedEnergy post-step, poststepstrategy.jl:110-121).  Vectors live in device memory
+ * (HBM) between calls; the host keeps the step loop and the shift strategy.
+ *
+ * Conventions: every function returns RIMU_OK (0) or an error code; rimu_last_error()
+ * gives a message for the calling thread.  Host pointers are borrowed for the duration of
+ * the call and copied.  No function throws or unwinds across the ABI.  An engine is driven
+ * by one host thread at a time (its work is ordered on one HIP stream).
+ *
+ * Addresses are BoseFS bit strings on one 64-bit chunk (B = N + M - 1 <= 64 bits), in
+ * the layout of BitStringAddresses/bitstring.jl:464-471: mode 1 in the least significant
+ * bits, mode i written as n_i one-bits followed by a zero separator.
+ */
+#ifndef RIMU_HIP_H
+#define RIMU_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RIMU_OK = 0,
+    RIMU_ERR_ARG = 1,        /* invalid argument (the Julia side throws ArgumentError) */
+    RIMU_ERR_HIP = 2,        /* HIP runtime failure */
+    RIMU_ERR_CAPACITY = 3,   /* a device buffer would overflow (maps to max_length abort) */
+    RIMU_ERR_NODEVICE = 4,   /* no usable gfx950 device */
+    RIMU_ERR_INTERNAL = 5
+};
+
+/* Hamiltonian kinds: HubbardReal1D (Hamiltonians/HubbardReal1D.jl:22-62) and
+ * HubbardMom1D (Hamiltonians/HubbardMom1D.jl:43-239, BoseFS branch). */
+enum { RIMU_MODEL_HUBBARD_REAL_1D = 1, RIMU_MODEL_HUBBARD_MOM_1D = 2 };
+
+/* Stochastic styles: IsStochasticInteger (StochasticStyles/styles.jl:11-25),
+ * IsDeterministic (:76-105), IsDynamicSemistochastic (:175-214). */
+enum {
+    RIMU_STYLE_STOCHASTIC_INTEGER = 1,
+    RIMU_STYLE_DETERMINISTIC = 2,
+    RIMU_STYLE_DYNAMIC_SEMISTOCHASTIC = 3
+};
+
+/* Plain data of a Hamiltonian: the fields of HubbardReal1D{TT,A,U,T} (type parameters
+ * u, t; HubbardReal1D.jl:46-49) or HubbardMom1D (u, t and the dispersion table kes,
+ * HubbardMom1D.jl:43-47, :80-85).  kes[j-1] is epsilon of mode j. */
+typedef struct rimu_model {
+    int32_t kind;
+    int32_t M;
+    int32_t N;
+    int32_t reserved;
+    double u;
+    double t;
+    double kes[64];
+} rimu_model;
+
+/* Plain data of a StochasticStyle and its CompressionStrategy.
+ * compression: 0 NoCompression, 1 ThresholdCompression (compression.jl:8-42).
+ * proj_threshold: projection threshold of deposits (spawning.jl:13-31); 0 = exact.
+ * rel_threshold / abs_threshold: DynamicSemistochastic (spawning.jl:358-378). */
+typedef struct rimu_style {
+    int32_t kind;
+    int32_t compression;
+    double compression_threshold;
+    double proj_threshold;
+    double rel_threshold;
+    double abs_threshold;
+} rimu_style;
+
+/* Step results.  The first eight fields are the step_stats of the styles
+ * (styles.jl:14-20, :94-96, :203-209) and ThresholdCompression's len_before
+ * (compression.jl:16); len and norm are walkernumber_and_length (abstractdvec.jl:270);
+ * vproj and hproj are the ProjectedEnergy columns (poststepstrategy.jl:110-121). */
+typedef struct rimu_step_stats {
+    double spawn_attempts;
+    double exact_steps;
+    double inexact_steps;
+    double spawns;
+    double deaths;
+    double clones;
+    double zombies;
+    double len_before;
+    double len;
+    double norm;
+    double vproj;
+    double hproj;
+    int32_t overflow;   /* 1 when a buffer capacity was hit: the step is void */
+    int32_t reserved;
+} rimu_step_stats;
+
+typedef struct rimu_engine rimu_engine;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rimu_last_error(void);
+
+/* Number of visible devices (0 when there is no GPU). */
+int rimu_device_count(int32_t* count);
+
+/* Create an engine: a device-resident vector plus working memory for one FCIQMC state.
+ * Replaces the DVec/PDVec + working_memory pair (Interfaces/dictvectors.jl:87,
+ * DictVectors/pdworkingmemory.jl:86).  max_entries bounds the vector length (Rimu's
+ * max_length, projector_monte_carlo_problem.jl:243-253); max_spawns bounds the spawn list
+ * of one step.  seed keys the Philox streams of this engine. */
+int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_t seed,
+                       int64_t max_entries, int64_t max_spawns, rimu_engine** out);
+int rimu_engine_destroy(rimu_engine* e);
+/* Grow the capacities (never shrinks; the vector is kept).  A step that failed with
+ * RIMU_ERR_CAPACITY left the vector unchanged and, because every random draw is a function
+ * of (seed, address, attempt, step), gives the same result when repeated. */
+int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t max_spawns);
+
+/* Replace the vector.  keys must be strictly ascending; vals are int64 for
+ * RIMU_STYLE_STOCHASTIC_INTEGER and double otherwise; zero values are dropped.
+ * (copyto!/setindex! on the DVec, dvec.jl:129-136.) */
+int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n);
+/* Current length (length(v)). */
+int rimu_vector_length(rimu_engine* e, int64_t* n);
+/* Copy the vector out, ascending keys (pairs(v)).  cap is the capacity of keys/vals. */
+int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap, int64_t* n);
+
+/* Projector of the ProjectedEnergy post-step (poststepstrategy.jl:82-121): vproj is
+ * sum_i pc_i * v[pk_i], hproj is sum_i hc_i * v[hk_i] (hproj = freeze(H' * projector)).
+ * keys need not be sorted.  n = nh = 0 disables the projection. */
+int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int64_t n,
+                       const uint64_t* hk, const double* hc, int64_t nh);
+
+/* One step v <- compress(T v) with T = 1 + dtau (shift - H) (fciqmc.jl:78-112, :139),
+ * apply_operator! with boost (Interfaces/dictvectors.jl:112), then len, norm and the
+ * projections.  step numbers the Philox streams.  On overflow (stats->overflow = 1,
+ * return RIMU_ERR_CAPACITY) the vector is left unchanged. */
+int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
+              rimu_step_stats* stats);
+
+/* Per-kernel device time accumulated since the last reset, measured with HIP events on
+ * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */
+int rimu_engine_set_timing(rimu_engine* e, int32_t enabled);
+int rimu_engine_kernel_times(rimu_engine* e, char* names, int32_t name_len, double* ms,
+                             int64_t* launches, int32_t cap, int32_t* n);
+/* Bytes moved per step by the last step, by the algorithmic byte count of the merge:
+ * len_in*(k+v) + 2*S*(k+v) + len_out*(k+v).  Also returns S (records emitted, diagonal
+ * survivors included). */
+int rimu_engine_last_traffic(rimu_engine* e, double* alg_bytes, double* records);
+
+/* Annihilation of a spawn list on its own (deposit! folded over the list,
+ * Interfaces/dictvectors.jl:49-51 with dvec.jl:129-136): equal keys are summed in list
+ * order, exact zeros dropped; output ascending.  vtype 0: int64 values, 1: double. */
+int rimu_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
+                    uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out);
+
+/* Hamiltonian interface on the device for a batch of addresses (for tests and
+ * ProjectedEnergy set-up): diagonal_element and num_offdiagonals
+ * (Interfaces/hamiltonians.jl), and get_offdiagonal for one chosen index per address. */
+int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const int64_t* chosen,
+                          int64_t n, double* diag, int64_t* num_offdiag, uint64_t* new_addr,
+                          double* offdiag_value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIMU_HIP_H */

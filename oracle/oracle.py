@@ -1,0 +1,329 @@
+"""ctypes front end of the CPU oracle (``liboracle.so``) plus project-owned exact
+diagonalisation.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``.  The product package never imports it.
+
+Reference files restated here (paths relative to the Rimu.jl v0.14.0 checkout):
+
+* ``HubbardMom1D`` momentum grid and dispersion: ``src/Hamiltonians/HubbardMom1D.jl:49-65``
+  and ``:7`` (``hubbard_dispersion``).
+* Basis by breadth-first search from a starting address:
+  ``src/ExactDiagonalization/basis_breadth_first_search.jl:216,390``.
+* Exact diagonalisation (``solve(ExactDiagonalizationProblem(H))``):
+  ``src/ExactDiagonalization/solve.jl:72-112`` -- restated with scipy's dense ``eigh`` for
+  small bases and ARPACK ``eigsh`` (Lanczos) for large ones.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import pathlib
+from fractions import Fraction
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB = None
+
+MODEL_REAL1D = 1
+MODEL_MOM1D = 2
+STYLE_INTEGER = 1
+STYLE_DETERMINISTIC = 2
+STYLE_DYNAMIC = 3
+
+
+class Model(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("M", ctypes.c_int32), ("N", ctypes.c_int32),
+                ("u", ctypes.c_double), ("t", ctypes.c_double), ("kes", ctypes.c_double * 64)]
+
+
+class Style(ctypes.Structure):
+    _fields_ = [("style", ctypes.c_int32), ("compression", ctypes.c_int32),
+                ("compression_threshold", ctypes.c_double), ("proj_threshold", ctypes.c_double),
+                ("rel_threshold", ctypes.c_double), ("abs_threshold", ctypes.c_double)]
+
+
+def lib():
+    """Load ``liboracle.so`` (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = ctypes.CDLL(str(path))
+        P = ctypes.POINTER
+        u64p, i32p, dp, vp = P(ctypes.c_uint64), P(ctypes.c_int32), P(ctypes.c_double), ctypes.c_void_p
+        L.or_philox4x32_10.argtypes = [P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]
+        L.or_from_onr.argtypes = [i32p, ctypes.c_int32]
+        L.or_from_onr.restype = ctypes.c_uint64
+        L.or_to_onr.argtypes = [ctypes.c_uint64, ctypes.c_int32, i32p]
+        L.or_occupied_modes.argtypes = [ctypes.c_uint64, i32p, i32p, i32p]
+        L.or_occupied_modes.restype = ctypes.c_int32
+        L.or_bose_hubbard_interaction.argtypes = [ctypes.c_uint64]
+        L.or_bose_hubbard_interaction.restype = ctypes.c_int64
+        L.or_num_offdiagonals.argtypes = [P(Model), ctypes.c_uint64]
+        L.or_num_offdiagonals.restype = ctypes.c_int64
+        L.or_diagonal_element.argtypes = [P(Model), ctypes.c_uint64]
+        L.or_diagonal_element.restype = ctypes.c_double
+        L.or_get_offdiagonal.argtypes = [P(Model), ctypes.c_uint64, ctypes.c_int64, dp]
+        L.or_get_offdiagonal.restype = ctypes.c_uint64
+        L.or_step.argtypes = [P(Model), P(Style), ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                              u64p, vp, ctypes.c_int64, u64p, vp, ctypes.c_int64, dp]
+        L.or_step.restype = ctypes.c_int64
+        L.or_step_threaded.argtypes = [P(Model), P(Style), ctypes.c_uint64, ctypes.c_int64,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int32, ctypes.c_int32, u64p, vp, ctypes.c_int64,
+                                       u64p, vp, ctypes.c_int64, dp]
+        L.or_step_threaded.restype = ctypes.c_int64
+        L.or_annihilate.argtypes = [ctypes.c_int32, u64p, vp, ctypes.c_int64, u64p, vp, ctypes.c_int64]
+        L.or_annihilate.restype = ctypes.c_int64
+        L.or_double_log_update.argtypes = [ctypes.c_double] * 7
+        L.or_double_log_update.restype = ctypes.c_double
+        L.or_build_basis.argtypes = [ctypes.c_int32, ctypes.c_int32, u64p, ctypes.c_int64]
+        L.or_build_basis.restype = ctypes.c_int64
+        L.or_apply_affine.argtypes = [P(Model), ctypes.c_int32, u64p, ctypes.c_int64, dp, dp, dp, dp]
+        L.or_apply_affine.restype = ctypes.c_int32
+        L.or_rank.argtypes = [ctypes.c_uint64]
+        L.or_rank.restype = ctypes.c_int64
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+# ----------------------------------------------------------------------------------------
+# Models
+# ----------------------------------------------------------------------------------------
+def mom1d_grid(M: int, t: float):
+    """``ks`` and ``kes`` of ``HubbardMom1D`` (HubbardMom1D.jl:53-63).
+
+    ``ks`` is Julia's ``range(start; step, length=M)`` on Float64 values: element j is the
+    correctly rounded value of ``start + (j-1)*step``, computed here in exact rational
+    arithmetic from the two Float64 inputs."""
+    step = 2 * math.pi / M
+    if M % 2 == 1:
+        start = -math.pi * (1 + 1 / M) + step
+    else:
+        start = -math.pi + step
+    ks = [float(Fraction(start) + j * Fraction(step)) for j in range(M)]
+    kes = [(-2 * float(t)) * math.cos(k) for k in ks]
+    return np.array(ks), np.array(kes)
+
+
+def real1d(M: int, N: int, u: float = 1.0, t: float = 1.0) -> Model:
+    m = Model()
+    m.model, m.M, m.N, m.u, m.t = MODEL_REAL1D, M, N, float(u), float(t)
+    return m
+
+
+def mom1d(M: int, N: int, u: float = 1.0, t: float = 1.0) -> Model:
+    m = Model()
+    m.model, m.M, m.N, m.u, m.t = MODEL_MOM1D, M, N, float(u), float(t)
+    _, kes = mom1d_grid(M, t)
+    for j in range(M):
+        m.kes[j] = kes[j]
+    return m
+
+
+def style(kind: str, compression_threshold: float = 1.0, proj_threshold: float = 0.0,
+          rel_threshold: float = 1.0, abs_threshold: float = math.inf, compression=None) -> Style:
+    s = Style()
+    s.style = {"integer": STYLE_INTEGER, "deterministic": STYLE_DETERMINISTIC,
+               "dynamic": STYLE_DYNAMIC}[kind]
+    if compression is None:
+        compression = 1 if kind == "dynamic" else 0
+    s.compression = compression
+    s.compression_threshold = compression_threshold
+    s.proj_threshold = proj_threshold
+    s.rel_threshold = rel_threshold
+    s.abs_threshold = abs_threshold
+    return s
+
+
+# ----------------------------------------------------------------------------------------
+# Addresses and matrix elements
+# ----------------------------------------------------------------------------------------
+def from_onr(onr) -> int:
+    a = np.ascontiguousarray(onr, dtype=np.int32)
+    return int(lib().or_from_onr(_ptr(a, ctypes.c_int32), len(a)))
+
+
+def to_onr(bits: int, M: int):
+    out = np.zeros(M, dtype=np.int32)
+    lib().or_to_onr(ctypes.c_uint64(bits), M, _ptr(out, ctypes.c_int32))
+    return tuple(int(x) for x in out)
+
+
+def occupied_modes(bits: int):
+    o, m, f = (np.zeros(64, dtype=np.int32) for _ in range(3))
+    n = lib().or_occupied_modes(ctypes.c_uint64(bits), _ptr(o, ctypes.c_int32),
+                                _ptr(m, ctypes.c_int32), _ptr(f, ctypes.c_int32))
+    return [(int(o[i]), int(m[i]), int(f[i])) for i in range(n)]
+
+
+def bose_hubbard_interaction(bits: int) -> int:
+    return int(lib().or_bose_hubbard_interaction(ctypes.c_uint64(bits)))
+
+
+def num_offdiagonals(model: Model, bits: int) -> int:
+    return int(lib().or_num_offdiagonals(ctypes.byref(model), ctypes.c_uint64(bits)))
+
+
+def diagonal_element(model: Model, bits: int) -> float:
+    return float(lib().or_diagonal_element(ctypes.byref(model), ctypes.c_uint64(bits)))
+
+
+def get_offdiagonal(model: Model, bits: int, chosen: int):
+    v = ctypes.c_double()
+    nb = lib().or_get_offdiagonal(ctypes.byref(model), ctypes.c_uint64(bits), chosen, ctypes.byref(v))
+    return int(nb), float(v.value)
+
+
+def offdiagonals(model: Model, bits: int):
+    return [get_offdiagonal(model, bits, c) for c in range(1, num_offdiagonals(model, bits) + 1)]
+
+
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return tuple(int(x) for x in o)
+
+
+# ----------------------------------------------------------------------------------------
+# Steps and annihilation
+# ----------------------------------------------------------------------------------------
+STAT_NAMES = ("attempts", "exact_steps", "inexact_steps", "spawns", "deaths", "clones",
+              "zombies", "len_before", "len", "norm")
+
+
+def step(model: Model, sty: Style, keys, vals, *, seed: int, step: int, shift: float,
+         dtau: float, boost: float = 1.0, order: int = 0, threads: int = 0):
+    """One ``apply_operator!`` + ``compress!`` of (keys, vals).  ``threads > 0`` uses the
+    threaded PDVec-style restatement (output unsorted)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    vtype = 0 if sty.style == STYLE_INTEGER else 1
+    vals = np.ascontiguousarray(vals, dtype=np.int64 if vtype == 0 else np.float64)
+    cap = max(64, 4 * len(keys) + 1024)
+    while True:
+        ok = np.zeros(cap, dtype=np.uint64)
+        ov = np.zeros(cap, dtype=vals.dtype)
+        st = np.zeros(16, dtype=np.float64)
+        if threads:
+            n = lib().or_step_threaded(ctypes.byref(model), ctypes.byref(sty), seed, step, shift,
+                                       dtau, boost, vtype, threads, _ptr(keys, ctypes.c_uint64),
+                                       vals.ctypes.data, len(keys), _ptr(ok, ctypes.c_uint64),
+                                       ov.ctypes.data, cap, _ptr(st, ctypes.c_double))
+        else:
+            n = lib().or_step(ctypes.byref(model), ctypes.byref(sty), seed, step, shift, dtau,
+                              boost, order, vtype, _ptr(keys, ctypes.c_uint64), vals.ctypes.data,
+                              len(keys), _ptr(ok, ctypes.c_uint64), ov.ctypes.data, cap,
+                              _ptr(st, ctypes.c_double))
+        if n >= 0:
+            return ok[:n], ov[:n], dict(zip(STAT_NAMES, st[:10].tolist()))
+        cap = -n + 16
+
+
+def annihilate(keys, vals):
+    """Left fold of ``deposit!`` over a spawn list; output sorted by key."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    vals = np.ascontiguousarray(vals)
+    vtype = 0 if vals.dtype == np.int64 else 1
+    if vtype == 1:
+        vals = vals.astype(np.float64, copy=False)
+    cap = len(keys) + 1
+    ok = np.zeros(cap, dtype=np.uint64)
+    ov = np.zeros(cap, dtype=vals.dtype)
+    n = lib().or_annihilate(vtype, _ptr(keys, ctypes.c_uint64), vals.ctypes.data, len(keys),
+                            _ptr(ok, ctypes.c_uint64), ov.ctypes.data, cap)
+    return ok[:n], ov[:n]
+
+
+def double_log_update(shift, tnorm, pnorm, target, zeta=0.08, xi=None, dtau=0.01):
+    xi = zeta ** 2 / 4 if xi is None else xi
+    return float(lib().or_double_log_update(shift, tnorm, pnorm, target, zeta, xi, dtau))
+
+
+# ----------------------------------------------------------------------------------------
+# Bases, deterministic products and exact diagonalisation
+# ----------------------------------------------------------------------------------------
+def build_basis(M: int, N: int) -> np.ndarray:
+    """All N-boson M-mode addresses, ascending."""
+    cap = math.comb(N + M - 1, N)
+    out = np.zeros(cap, dtype=np.uint64)
+    n = lib().or_build_basis(M, N, _ptr(out, ctypes.c_uint64), cap)
+    assert n == cap
+    return out
+
+
+def rank(bits: int) -> int:
+    return int(lib().or_rank(ctypes.c_uint64(bits)))
+
+
+def apply_affine(models, basis: np.ndarray, a, b, v: np.ndarray) -> np.ndarray:
+    """w_r = (a_r + b_r H_r) v_r for nrhs models; v has shape (D, nrhs)."""
+    models = list(models)
+    nrhs = len(models)
+    arr = (Model * nrhs)(*models)
+    basis = np.ascontiguousarray(basis, dtype=np.uint64)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(len(basis), nrhs)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    w = np.zeros_like(v)
+    rc = lib().or_apply_affine(arr, nrhs, _ptr(basis, ctypes.c_uint64), len(basis),
+                               _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double),
+                               _ptr(v, ctypes.c_double), _ptr(w, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("neighbour outside basis")
+    return w
+
+
+def bfs_basis(model: Model, start: int, limit: int = 5_000_000) -> np.ndarray:
+    """Addresses connected to ``start`` by nonzero off-diagonal elements, ascending."""
+    seen = {start}
+    frontier = [start]
+    while frontier:
+        nxt = []
+        for a in frontier:
+            for nb, val in offdiagonals(model, a):
+                if val != 0 and nb not in seen:
+                    seen.add(nb)
+                    nxt.append(nb)
+        if len(seen) > limit:
+            raise RuntimeError("basis too large")
+        frontier = nxt
+    return np.array(sorted(seen), dtype=np.uint64)
+
+
+def hamiltonian_operator(model: Model, basis: np.ndarray):
+    import scipy.sparse.linalg as sla
+    D = len(basis)
+    one, zero = np.ones(1), np.zeros(1)
+
+    def mv(x):
+        x = np.asarray(x, dtype=np.float64).reshape(D, 1)
+        return apply_affine([model], basis, zero, one, x).reshape(D)
+
+    return sla.LinearOperator((D, D), matvec=mv, rmatvec=mv, dtype=np.float64)
+
+
+def ed_energies(model: Model, basis: np.ndarray, k: int = 1) -> np.ndarray:
+    """Lowest ``k`` eigenvalues of H on ``basis`` (dense ``eigh`` when small, else ARPACK)."""
+    D = len(basis)
+    if D <= 3000:
+        cols = np.zeros((D, D))
+        for j in range(D):
+            e = np.zeros((D, 1))
+            e[j, 0] = 1.0
+            cols[:, j] = apply_affine([model], basis, np.zeros(1), np.ones(1), e)[:, 0]
+        return np.linalg.eigvalsh(0.5 * (cols + cols.T))[:k]
+    import scipy.sparse.linalg as sla
+    vals = sla.eigsh(hamiltonian_operator(model, basis), k=k, which="SA", tol=1e-12,
+                     return_eigenvectors=False)
+    return np.sort(vals)

@@ -1,0 +1,808 @@
+// rimu_oracle.cpp -- CPU restatement of Rimu.jl's FCIQMC step path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle that the GPU engine is checked
+// against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it.  The product library (librimu_hip.so) never links or calls it.
+//
+// Each function cites the reference file:line it restates (paths relative to the
+// Rimu.jl v0.14.0 checkout).  Where the reference draws from Julia's task-local
+// Xoshiro stream (spawning.jl:20,40; compression.jl:22; Interfaces/hamiltonians.jl:363)
+// this restatement draws from counter-based Philox4x32-10 streams instead, keyed by the
+// run seed and addressed by (source address, attempt, step, purpose).  The GPU engine
+// uses the same streams, so a single step can be compared record for record.
+//
+// Addresses are BoseFS on one BitString chunk (B = N + M - 1 <= 64 bits), layout of
+// bitstring.jl:464-471: mode 1 occupies the least significant bits, each mode is
+// written as n_i one-bits followed by a zero separator.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11).  Round function and constants as published there.
+// ---------------------------------------------------------------------------------
+struct P4 { uint32_t v[4]; };
+
+inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k[0];
+    uint32_t n2 = hi0 ^ c[3] ^ k[1];
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+}
+
+inline P4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                        uint32_t k0, uint32_t k1) {
+    uint32_t c[4] = {c0, c1, c2, c3};
+    uint32_t k[2] = {k0, k1};
+    for (int r = 0; r < 10; ++r) {
+        philox_round(c, k);
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    P4 out;
+    std::memcpy(out.v, c, sizeof(c));
+    return out;
+}
+
+// Stream purposes, stored in the top two bits of counter word 3.
+enum { TAG_SPAWN = 0u, TAG_DIAG = 1u, TAG_COMPRESS = 2u };
+
+inline P4 draw(uint64_t seed, uint64_t addr, uint32_t attempt, int64_t step, uint32_t tag) {
+    const uint32_t c3 = ((uint32_t)step & 0x3FFFFFFFu) | (tag << 30);
+    return philox4x32_10((uint32_t)addr, (uint32_t)(addr >> 32), attempt, c3,
+                         (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// 53-bit uniform double in [0, 1) from two 32-bit words.
+inline double u53(uint32_t hi, uint32_t lo) {
+    const uint64_t x = ((((uint64_t)hi) << 32) | lo) >> 11;
+    return (double)x * 0x1.0p-53;
+}
+
+// Uniform index in 1:n from one 32-bit word (multiply-shift; bias <= n / 2^32).
+inline int64_t pick1(uint32_t r, int64_t n) {
+    return 1 + (int64_t)(((uint64_t)r * (uint64_t)n) >> 32);
+}
+
+// ---------------------------------------------------------------------------------
+// BoseFS on one chunk.
+// ---------------------------------------------------------------------------------
+struct BoseIndex { int occnum, mode, offset; };  // BoseFSIndex, bosefs.jl / fockaddress.jl
+
+inline int tz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
+inline int to64(uint64_t x) { return tz64(~x); }
+
+// occupied_modes iterator, bitstring.jl:636-655 (single chunk).
+inline int occupied_modes(uint64_t chunk, BoseIndex* out) {
+    int n = 0;
+    int empty = tz64(chunk);
+    if (empty >= 64) return 0;
+    chunk >>= empty;
+    int bit = empty, mode = 1 + empty;
+    while (chunk) {
+        int bosons = to64(chunk);
+        chunk = bosons >= 64 ? 0 : (chunk >> bosons);
+        int e = chunk ? tz64(chunk) : 0;
+        chunk = e >= 64 ? 0 : (chunk >> e);
+        out[n++] = BoseIndex{bosons, mode, bit};
+        bit += bosons + e;
+        mode += e;
+    }
+    return n;
+}
+
+// from_bose_onr, bitstring.jl:464-471.
+inline uint64_t from_onr(const int* onr, int M) {
+    uint64_t r = 0;
+    for (int i = M - 1; i >= 0; --i) {
+        const int n = onr[i];
+        r = (n + 1 >= 64) ? 0 : (r << (n + 1));
+        r |= (n >= 64 ? ~0ull : ((1ull << n) - 1));
+    }
+    return r;
+}
+
+// to_bose_onr, bitstring.jl:511-520.
+inline void to_onr(uint64_t bs, int M, int* onr) {
+    for (int i = 0; i < M; ++i) onr[i] = 0;
+    for (int mode = 0; mode < M; ++mode) {
+        int b = to64(bs);
+        onr[mode] = b;
+        bs = (b + 1 >= 64) ? 0 : (bs >> (b + 1));
+        if (!bs) break;
+    }
+}
+
+// find_mode, bosefs.jl:188-203.
+inline BoseIndex find_mode(const BoseIndex* occ, int nocc, int index) {
+    int last_occnum = 0, last_mode = 0, last_offset = 0;
+    for (int i = 0; i < nocc; ++i) {
+        const int dist = index - occ[i].mode;
+        if (dist == 0) return BoseIndex{occ[i].occnum, index, occ[i].offset};
+        if (dist < 0) return BoseIndex{0, index, occ[i].offset + dist};
+        last_occnum = occ[i].occnum; last_mode = occ[i].mode; last_offset = occ[i].offset;
+    }
+    return BoseIndex{0, index, last_offset + last_occnum + index - last_mode};
+}
+
+// bose_excitation_value, fockaddress.jl:544-567: destructions are applied first (in the
+// order given), then creations; each operator sees the occupation numbers left by the
+// operators before it.  Arguments are already reversed as in bitstring.jl:586-587.
+inline int64_t excitation_value(const BoseIndex* cre, int nc, const BoseIndex* des, int nd) {
+    int cocc[4], docc[4];
+    for (int i = 0; i < nc; ++i) cocc[i] = cre[i].occnum;
+    for (int i = 0; i < nd; ++i) docc[i] = des[i].occnum;
+    int64_t value = 1;
+    for (int a = 0; a < nd; ++a) {
+        value *= docc[a];
+        for (int i = 0; i < nc; ++i) cocc[i] -= (des[a].mode == cre[i].mode);
+        for (int b = a + 1; b < nd; ++b) docc[b] -= (des[a].mode == des[b].mode);
+    }
+    for (int a = 0; a < nc; ++a) {
+        value *= (cocc[a] + 1);
+        for (int b = a + 1; b < nc; ++b) cocc[b] += (cre[a].mode == cre[b].mode);
+    }
+    return value;
+}
+
+// excitation(b, creations, destructions), bosefs.jl:252-255 + bitstring.jl:581-595.
+// The particle moves are restated on the occupation-number representation: the resulting
+// address is the one with the destroyed modes decremented and created modes incremented.
+inline uint64_t excitation(uint64_t bs, int M, const BoseIndex* cre, const BoseIndex* des,
+                           int n, double* val) {
+    BoseIndex cre_rev[4], des_rev[4];
+    for (int i = 0; i < n; ++i) { cre_rev[i] = cre[n - 1 - i]; des_rev[i] = des[n - 1 - i]; }
+    const int64_t v = excitation_value(cre_rev, n, des_rev, n);
+    if (v == 0) { *val = 0.0; return bs; }
+    int onr[64];
+    to_onr(bs, M, onr);
+    for (int i = 0; i < n; ++i) onr[des[i].mode - 1] -= 1;
+    for (int i = 0; i < n; ++i) onr[cre[i].mode - 1] += 1;
+    *val = std::sqrt((double)v);
+    return from_onr(onr, M);
+}
+
+inline int mod1(int x, int m) { int r = ((x - 1) % m + m) % m; return r + 1; }
+inline void fldmod1(int64_t x, int64_t y, int64_t* f, int64_t* m) {
+    // Julia fldmod1 for positive x, y
+    *m = (x - 1) % y + 1;
+    *f = (x - *m) / y + 1;
+}
+
+// bose_hubbard_interaction, bosefs.jl:407-413.
+inline int64_t bose_hubbard_interaction(const BoseIndex* occ, int nocc) {
+    int64_t r = 0;
+    for (int i = 0; i < nocc; ++i) r += (int64_t)occ[i].occnum * (occ[i].occnum - 1);
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+    int32_t model;  // 1 = HubbardReal1D, 2 = HubbardMom1D
+    int32_t M, N;
+    double u, t;
+    double kes[64];  // HubbardMom1D dispersion table, mode j at index j-1
+} or_model;
+
+typedef struct {
+    int32_t style;     // 1 IsStochasticInteger, 2 IsDeterministic, 3 IsDynamicSemistochastic
+    int32_t compression;  // 0 NoCompression, 1 ThresholdCompression
+    double compression_threshold;
+    double proj_threshold;  // projection threshold of diagonal/spawn deposits
+    double rel_threshold;   // DynamicSemistochastic rel_threshold
+    double abs_threshold;   // DynamicSemistochastic abs_threshold
+} or_style;
+
+void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    P4 r = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    std::memcpy(out, r.v, sizeof(r.v));
+}
+
+uint64_t or_from_onr(const int32_t* onr, int32_t M) {
+    int tmp[64];
+    for (int i = 0; i < M; ++i) tmp[i] = onr[i];
+    return from_onr(tmp, M);
+}
+
+void or_to_onr(uint64_t bs, int32_t M, int32_t* onr) {
+    int tmp[64];
+    to_onr(bs, M, tmp);
+    for (int i = 0; i < M; ++i) onr[i] = tmp[i];
+}
+
+int32_t or_occupied_modes(uint64_t bs, int32_t* occnum, int32_t* mode, int32_t* offset) {
+    BoseIndex occ[64];
+    const int n = occupied_modes(bs, occ);
+    for (int i = 0; i < n; ++i) { occnum[i] = occ[i].occnum; mode[i] = occ[i].mode; offset[i] = occ[i].offset; }
+    return n;
+}
+
+int64_t or_bose_hubbard_interaction(uint64_t bs) {
+    BoseIndex occ[64];
+    const int n = occupied_modes(bs, occ);
+    return bose_hubbard_interaction(occ, n);
+}
+
+// num_offdiagonals: HubbardReal1D.jl:51-53; HubbardMom1D.jl:131-140.
+int64_t or_num_offdiagonals(const or_model* h, uint64_t bs) {
+    BoseIndex occ[64];
+    const int s = occupied_modes(bs, occ);
+    if (h->model == 1) return 2 * (int64_t)s;
+    int d = 0;
+    for (int i = 0; i < s; ++i) d += occ[i].occnum > 1;
+    const int64_t M = h->M;
+    return (int64_t)s * (s - 1) * (M - 2) + (int64_t)d * (M - 1);
+}
+
+// diagonal_element: HubbardReal1D.jl:55-57; HubbardMom1D.jl:174-177 with
+// momentum_transfer_diagonal excitations.jl:126-137 and dot(kes, map) fockaddress.jl:316-321.
+double or_diagonal_element(const or_model* h, uint64_t bs) {
+    BoseIndex occ[64];
+    const int s = occupied_modes(bs, occ);
+    if (h->model == 1) {
+        return h->u * (double)bose_hubbard_interaction(occ, s) / 2;
+    }
+    double kin = 0.0;
+    for (int i = 0; i < s; ++i) kin += h->kes[occ[i].mode - 1] * occ[i].occnum;
+    int64_t onproduct = 0;
+    for (int i = 0; i < s; ++i) {
+        const int64_t oi = occ[i].occnum;
+        onproduct += oi * (oi - 1);
+        for (int j = 0; j < i; ++j) onproduct += 4 * oi * occ[j].occnum;
+    }
+    return kin + h->u / (2.0 * h->M) * (double)onproduct;
+}
+
+// hopnextneighbour generic form, bosefs.jl:347-353; value -t*sqrt(...), HubbardReal1D.jl:59-62.
+static uint64_t real1d_offdiagonal(const or_model* h, uint64_t bs, const BoseIndex* occ,
+                                   int nocc, int64_t chosen, double* value) {
+    const BoseIndex src = occ[(chosen + 1) / 2 - 1];       // find_occupied_mode
+    const int dir = (chosen & 1) ? 1 : -1;
+    const BoseIndex dst = find_mode(occ, nocc, mod1(src.mode + dir, h->M));
+    double v;
+    const uint64_t nb = excitation(bs, h->M, &dst, &src, 1, &v);
+    *value = -h->t * v;
+    return nb;
+}
+
+// momentum_transfer_excitation (fold=true), excitations.jl:26-80; value u/(2M)*onproduct,
+// HubbardMom1D.jl:189-194.
+static uint64_t mom1d_offdiagonal(const or_model* h, uint64_t bs, const BoseIndex* occ,
+                                  int nocc, int64_t chosen, double* value) {
+    const int M = h->M;
+    const int64_t singlies = nocc;
+    int64_t dbl = chosen - singlies * (singlies - 1) * (M - 2);
+    BoseIndex src[2];
+    int64_t mom_change;
+    if (dbl > 0) {
+        int64_t d;
+        fldmod1(dbl, M - 1, &d, &mom_change);
+        BoseIndex idx = occ[0];
+        for (int i = 0; i < nocc; ++i) {
+            d -= occ[i].occnum >= 2;
+            if (d == 0) { idx = occ[i]; break; }
+        }
+        src[0] = idx; src[1] = idx;
+    } else {
+        int64_t pair, fst, snd;
+        fldmod1(chosen, M - 2, &pair, &mom_change);
+        fldmod1(pair, singlies - 1, &fst, &snd);
+        int64_t f_hole, s_hole;
+        if (snd < fst) { f_hole = snd; s_hole = fst; }
+        else { f_hole = fst; s_hole = snd + 1; }
+        src[0] = occ[f_hole - 1]; src[1] = occ[s_hole - 1];
+        if (mom_change >= src[1].mode - src[0].mode) mom_change += 1;
+    }
+    const int d0 = mod1((int)(src[0].mode + mom_change), M);
+    const int d1 = mod1((int)(src[1].mode - mom_change), M);
+    BoseIndex dst[2] = {find_mode(occ, nocc, d0), find_mode(occ, nocc, d1)};
+    double v;
+    const uint64_t nb = excitation(bs, M, dst, src, 2, &v);
+    *value = h->u / (2 * M) * v;
+    return nb;
+}
+
+uint64_t or_get_offdiagonal(const or_model* h, uint64_t bs, int64_t chosen, double* value) {
+    BoseIndex occ[64];
+    const int nocc = occupied_modes(bs, occ);
+    if (h->model == 1) return real1d_offdiagonal(h, bs, occ, nocc, chosen, value);
+    return mom1d_offdiagonal(h, bs, occ, nocc, chosen, value);
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------------
+// One column of the transition operator T = 1 + dt(S - H), fciqmc.jl:78-112, applied
+// per style (styles.jl:21-25, :101-105, :210-214) into a deposit sink.
+// ---------------------------------------------------------------------------------
+struct Params {
+    const or_model* h;
+    const or_style* s;
+    uint64_t seed;
+    int64_t step;
+    double shift, dtau, boost;
+};
+
+struct Stats {
+    double attempts = 0, exact = 0, inexact = 0, spawns = 0;
+    double deaths = 0, clones = 0, zombies = 0;
+};
+
+// projected_deposit! for integer values, spawning.jl:33-45.
+inline int64_t project_int(double x, double U) {
+    const double sg = (x > 0) - (x < 0);
+    return (int64_t)sg * (int64_t)std::floor(std::fabs(x) + U);
+}
+// projected_deposit! for floating values, spawning.jl:13-31.
+inline double project_float(double x, double thresh, double U) {
+    if (std::fabs(x) < thresh) {
+        const double sg = (x > 0) - (x < 0);
+        return (U < std::fabs(x) / thresh) ? sg * thresh : 0.0;
+    }
+    return x;
+}
+
+// clones_deaths_zombies, spawning.jl:79-93.
+template <class V>
+inline void cdz(V res, V val, Stats& st) {
+    auto sgn = [](V x) { return (x > 0) - (x < 0); };
+    if (res > val) {
+        st.clones += std::fabs((double)(res - val));
+    } else if (sgn(res) != sgn(val)) {
+        st.deaths += std::fabs((double)val);
+        st.zombies += std::fabs((double)res);
+    } else {
+        st.deaths += std::fabs((double)(res - val));
+    }
+}
+
+// apply_column! of one entry.  emit(addr, value) receives every deposit in the order the
+// reference performs them: the diagonal deposit first, then the spawns.
+template <class V, class Emit>
+void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit) {
+    BoseIndex occ[64];
+    const int nocc = occupied_modes(addr, occ);
+    const or_model* h = p.h;
+    const int style = p.s->style;
+
+    // diagonal_step!, spawning.jl:73-77; diagonal_element(T, add) fciqmc.jl:93-96.
+    {
+        const double hd = or_diagonal_element(h, addr);
+        const double d = 1 - p.dtau * (hd - p.shift);
+        const double x = d * (double)val;
+        if constexpr (std::is_integral<V>::value) {
+            P4 r = draw(p.seed, addr, 0, p.step, TAG_DIAG);
+            const V nv = (V)project_int(x, u53(r.v[0], r.v[1]));
+            if (nv != 0) emit(addr, nv);
+            cdz<V>(nv, val, st);
+        } else {
+            double nv = x;
+            if (p.s->proj_threshold > 0) {
+                P4 r = draw(p.seed, addr, 0, p.step, TAG_DIAG);
+                nv = project_float(x, p.s->proj_threshold, u53(r.v[0], r.v[1]));
+            }
+            if (nv != 0) emit(addr, (V)nv);
+            cdz<V>((V)nv, val, st);
+        }
+    }
+
+    // offdiagonals(T, add), fciqmc.jl:104-112: matrix elements -dt * H_ba.
+    const int64_t n_od = or_num_offdiagonals(h, addr);
+    auto offdiag = [&](int64_t chosen, double* mel) {
+        double hv;
+        const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, chosen, &hv)
+                                            : mom1d_offdiagonal(h, addr, occ, nocc, chosen, &hv);
+        *mel = -hv * p.dtau;
+        return nb;
+    };
+    auto deposit = [&](uint64_t nb, double x, double U) {
+        if constexpr (std::is_integral<V>::value) {
+            const V nv = (V)project_int(x, U);
+            if (nv != 0) emit(nb, nv);
+            st.spawns += std::fabs((double)nv);
+        } else {
+            const double nv = (p.s->proj_threshold > 0) ? project_float(x, p.s->proj_threshold, U) : x;
+            if (nv != 0) emit(nb, (V)nv);
+            st.spawns += std::fabs(nv);
+        }
+    };
+    // spawn!(Exact), spawning.jl:174-182.
+    auto exact = [&]() {
+        for (int64_t j = 0; j < n_od; ++j) {
+            double mel;
+            const uint64_t nb = offdiag(j + 1, &mel);
+            double U = 0;
+            if (p.s->proj_threshold > 0) { P4 r = draw(p.seed, addr, (uint32_t)j, p.step, TAG_SPAWN); U = u53(r.v[1], r.v[2]); }
+            deposit(nb, (double)val * mel, U);
+        }
+        st.attempts += (double)n_od;
+    };
+    // spawn!(WithReplacement), spawning.jl:232-243, random_offdiagonal Interfaces/hamiltonians.jl:361-366.
+    auto with_replacement = [&]() {
+        const int64_t n_att = std::max<int64_t>((int64_t)std::floor(std::fabs((double)val) * p.boost), 1);
+        const double magnitude = (double)val / (double)n_att;
+        const double prob = 1.0 / (double)n_od;
+        for (int64_t j = 0; j < n_att; ++j) {
+            P4 r = draw(p.seed, addr, (uint32_t)j, p.step, TAG_SPAWN);
+            double mel;
+            const uint64_t nb = offdiag(pick1(r.v[0], n_od), &mel);
+            deposit(nb, mel * magnitude / prob, u53(r.v[1], r.v[2]));
+        }
+        st.attempts += (double)n_att;
+    };
+
+    if (n_od == 0) return;
+    if (style == 1) {
+        with_replacement();
+    } else if (style == 2) {
+        exact();
+        st.exact += 1;
+    } else {
+        // spawn!(DynamicSemistochastic), spawning.jl:364-378.
+        const double thresh = std::min(p.s->abs_threshold, (double)n_od);
+        const double amount = p.boost * std::fabs((double)val) * p.s->rel_threshold;
+        if (amount >= thresh) { exact(); st.exact += 1; }
+        else { with_replacement(); st.inexact += 1; }
+    }
+}
+
+// DVec storage: Dict semantics of dvec.jl:126-136 (a value that becomes zero is deleted).
+template <class V>
+struct DictVec {
+    std::unordered_map<uint64_t, V> m;
+    void deposit(uint64_t k, V v) {  // deposit!, Interfaces/dictvectors.jl:49-51
+        auto it = m.find(k);
+        if (it == m.end()) {
+            if (v != 0) m.emplace(k, V(0) + v);
+        } else {
+            it->second = it->second + v;
+            if (it->second == 0) m.erase(it);
+        }
+    }
+};
+
+template <class V>
+int64_t dump_sorted(const std::unordered_map<uint64_t, V>& m, uint64_t* keys, V* vals, int64_t cap) {
+    std::vector<std::pair<uint64_t, V>> items(m.begin(), m.end());
+    std::sort(items.begin(), items.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    const int64_t n = (int64_t)items.size();
+    if (n > cap) return -n;
+    for (int64_t i = 0; i < n; ++i) { keys[i] = items[i].first; vals[i] = items[i].second; }
+    return n;
+}
+
+// ThresholdCompression, compression.jl:18-33.
+template <class V>
+void compress(const Params& p, std::unordered_map<uint64_t, V>& m) {
+    if (p.s->compression != 1) return;
+    if constexpr (!std::is_integral<V>::value) {
+        const double th = p.s->compression_threshold;
+        for (auto it = m.begin(); it != m.end();) {
+            const double prob = std::fabs(it->second) / th;
+            if (prob < 1) {
+                P4 r = draw(p.seed, it->first, 0, p.step, TAG_COMPRESS);
+                const double sg = (it->second > 0) - (it->second < 0);
+                it->second = (prob > u53(r.v[0], r.v[1])) ? th * sg : 0.0;
+            }
+            if (it->second == 0) it = m.erase(it); else ++it;
+        }
+    }
+}
+
+template <class V>
+int64_t step_impl(const Params& p, int order, const uint64_t* keys, const V* vals, int64_t n,
+                  uint64_t* out_keys, V* out_vals, int64_t cap, double* stats) {
+    DictVec<V> w;
+    Stats st;
+    if (order == 0) {
+        // apply_operator!, Interfaces/dictvectors.jl:112-123: one column at a time.
+        for (int64_t i = 0; i < n; ++i)
+            apply_column<V>(p, keys[i], vals[i], st, [&](uint64_t k, V v) { w.deposit(k, v); });
+    } else {
+        // Same deposits, reordered into a list: every diagonal deposit first (entry order),
+        // then every spawn (entry, attempt order).  This is the list the GPU engine merges.
+        std::vector<std::pair<uint64_t, V>> diag, spawns;
+        for (int64_t i = 0; i < n; ++i) {
+            bool first = true;
+            const uint64_t self = keys[i];
+            apply_column<V>(p, keys[i], vals[i], st, [&](uint64_t k, V v) {
+                if (first && k == self) diag.emplace_back(k, v); else spawns.emplace_back(k, v);
+                first = false;
+            });
+        }
+        for (auto& kv : diag) w.deposit(kv.first, kv.second);
+        for (auto& kv : spawns) w.deposit(kv.first, kv.second);
+    }
+    const double len_before = (double)w.m.size();
+    compress<V>(p, w.m);
+    double norm = 0;
+    for (auto& kv : w.m) norm += std::fabs((double)kv.second);
+    stats[0] = st.attempts; stats[1] = st.exact; stats[2] = st.inexact; stats[3] = st.spawns;
+    stats[4] = st.deaths; stats[5] = st.clones; stats[6] = st.zombies; stats[7] = len_before;
+    stats[8] = (double)w.m.size(); stats[9] = norm;
+    return dump_sorted<V>(w.m, out_keys, out_vals, cap);
+}
+
+}  // namespace
+
+extern "C" {
+
+// One FCIQMC step (apply_operator! + compress!) of the source vector (keys, vals).
+// vtype 0: int64 values (IsStochasticInteger); 1: double values.
+// order 0: deposits in the reference's column order; 1: list order (diagonal first).
+// stats[0..9] = attempts, exact_steps, inexact_steps, spawns, deaths, clones, zombies,
+//               len_before, len, norm.  Returns the output length, or -(needed) if cap
+// is too small.  Output keys are sorted ascending.
+int64_t or_step(const or_model* h, const or_style* s, uint64_t seed, int64_t step,
+                double shift, double dtau, double boost, int32_t order, int32_t vtype,
+                const uint64_t* keys, const void* vals, int64_t n,
+                uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
+    Params p{h, s, seed, step, shift, dtau, boost};
+    if (vtype == 0)
+        return step_impl<int64_t>(p, order, keys, (const int64_t*)vals, n, out_keys, (int64_t*)out_vals, cap, stats);
+    return step_impl<double>(p, order, keys, (const double*)vals, n, out_keys, (double*)out_vals, cap, stats);
+}
+
+// Annihilation of a spawn list: a left fold of deposit! over the list, dvec.jl:126-136.
+int64_t or_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
+                      uint64_t* out_keys, void* out_vals, int64_t cap) {
+    if (vtype == 0) {
+        DictVec<int64_t> w;
+        const int64_t* v = (const int64_t*)vals;
+        for (int64_t i = 0; i < n; ++i) w.deposit(keys[i], v[i]);
+        return dump_sorted<int64_t>(w.m, out_keys, (int64_t*)out_vals, cap);
+    }
+    DictVec<double> w;
+    const double* v = (const double*)vals;
+    for (int64_t i = 0; i < n; ++i) w.deposit(keys[i], v[i]);
+    return dump_sorted<double>(w.m, out_keys, (double*)out_vals, cap);
+}
+
+// DoubleLogUpdate, shiftstrategy.jl:174-181.
+double or_double_log_update(double shift, double tnorm, double pnorm, double target,
+                            double zeta, double xi, double dtau) {
+    return shift - (xi / dtau * std::log(tnorm / target) + zeta / dtau * std::log(tnorm / pnorm));
+}
+
+// ---------------------------------------------------------------------------------
+// Full basis and deterministic products (IsDeterministic path, mul! abstractdvec.jl:275-284).
+// ---------------------------------------------------------------------------------
+
+// All BoseFS addresses of N bosons in M modes, sorted ascending (build_basis restated:
+// the number-conserving basis of basis_fock.jl).  Returns count or -(needed).
+namespace {
+void enumerate_compositions(int i, int left, int M, std::vector<int>& onr, std::vector<uint64_t>& out) {
+    if (i == M - 1) { onr[i] = left; out.push_back(from_onr(onr.data(), M)); onr[i] = 0; return; }
+    for (int k = 0; k <= left; ++k) { onr[i] = k; enumerate_compositions(i + 1, left - k, M, onr, out); }
+    onr[i] = 0;
+}
+}  // namespace
+
+int64_t or_build_basis(int32_t M, int32_t N, uint64_t* out, int64_t cap) {
+    std::vector<int> onr(M, 0);
+    std::vector<uint64_t> tmp;
+    enumerate_compositions(0, N, M, onr, tmp);
+    std::sort(tmp.begin(), tmp.end());
+    const int64_t count = (int64_t)tmp.size();
+    if (count > cap) return -count;
+    std::memcpy(out, tmp.data(), sizeof(uint64_t) * tmp.size());
+    return count;
+}
+
+// w = (a*1 + b*H) v on a sorted basis, accumulated column by column in basis order
+// (the DVec mul! / IsDeterministic deposit order).  With a = 1 + dt*S, b = -dt this is
+// T v (fciqmc.jl:93-112); with a = 0, b = 1 it is H v.  nrhs vectors stored row-major
+// [D][nrhs], each with its own model (u, t) in hs[r].
+int32_t or_apply_affine(const or_model* hs, int32_t nrhs, const uint64_t* basis, int64_t D,
+                        const double* a, const double* b, const double* v, double* w) {
+    std::memset(w, 0, sizeof(double) * D * nrhs);
+    for (int64_t j = 0; j < D; ++j) {
+        const uint64_t addr = basis[j];
+        BoseIndex occ[64];
+        const int nocc = occupied_modes(addr, occ);
+        for (int r = 0; r < nrhs; ++r) {
+            const or_model* h = &hs[r];
+            const double vj = v[j * nrhs + r];
+            const double tdiag = a[r] + b[r] * or_diagonal_element(h, addr);
+            w[j * nrhs + r] += tdiag * vj;
+            const int64_t n_od = or_num_offdiagonals(h, addr);
+            for (int64_t c = 1; c <= n_od; ++c) {
+                double hv;
+                const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
+                                                    : mom1d_offdiagonal(h, addr, occ, nocc, c, &hv);
+                const int64_t k = std::lower_bound(basis, basis + D, nb) - basis;
+                if (k >= D || basis[k] != nb) return -1;
+                w[k * nrhs + r] += (b[r] * hv) * vj;
+            }
+        }
+    }
+    return 0;
+}
+
+// Colex rank of an address among all N-boson M-mode addresses: the address is an
+// N-subset (positions of the one-bits) of B = N + M - 1 bits; rank = sum_j C(p_j, j+1).
+// This ordering equals ascending numeric order of the bit strings.
+int64_t or_rank(uint64_t bs) {
+    int64_t r = 0;
+    int j = 1;
+    while (bs) {
+        const int p = __builtin_ctzll(bs);
+        // C(p, j)
+        int64_t c = 1;
+        if (p < j) c = 0;
+        else for (int i = 1; i <= j; ++i) c = c * (p - j + i) / i;
+        r += c;
+        ++j;
+        bs &= bs - 1;
+    }
+    return r;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Threaded restatement of PDVec's step (the timed CPU baseline).
+// PDVec hash segments, pdvec.jl:43-50, fastrange_hash :6-9; per-thread spawn columns,
+// perform_spawns! pdworkingmemory.jl:208-217; collect_local! :228-235;
+// move_and_compress! :262-273.  Each thread owns one segment; each thread spawns into
+// its own column of per-target-segment maps; then each thread merges column entries for
+// its segment.  Hash maps are open-addressing tables (like Julia's Dict).
+// ---------------------------------------------------------------------------------
+namespace {
+
+inline uint64_t mix64(uint64_t x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+    return x;
+}
+inline int segment_of(uint64_t key, int nseg) {
+    return (int)(((unsigned __int128)mix64(key) * (unsigned __int128)nseg) >> 64);
+}
+
+template <class V>
+struct OpenMap {
+    std::vector<uint64_t> k;
+    std::vector<V> v;
+    std::vector<uint8_t> used;
+    size_t mask = 0, count = 0;
+    void init(size_t cap) {
+        size_t c = 16;
+        while (c < cap * 2) c <<= 1;
+        k.assign(c, 0); v.assign(c, V(0)); used.assign(c, 0); mask = c - 1; count = 0;
+    }
+    void grow() {
+        std::vector<uint64_t> ok; std::vector<V> ov; std::vector<uint8_t> ou;
+        ok.swap(k); ov.swap(v); ou.swap(used);
+        init((ou.size()));
+        for (size_t i = 0; i < ou.size(); ++i) if (ou[i]) add(ok[i], ov[i]);
+    }
+    inline void add(uint64_t key, V val) {
+        if ((count + 1) * 2 > mask + 1) grow();
+        size_t i = mix64(key) & mask;
+        while (used[i]) {
+            if (k[i] == key) { v[i] = v[i] + val; return; }
+            i = (i + 1) & mask;
+        }
+        used[i] = 1; k[i] = key; v[i] = V(0) + val; ++count;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// One step of the threaded PDVec-style restatement, with nthreads segments.  Input and
+// output are flat arrays (the output is not sorted).  Returns output length or -needed.
+int64_t or_step_threaded(const or_model* h, const or_style* s, uint64_t seed, int64_t step,
+                         double shift, double dtau, double boost, int32_t vtype, int32_t nthreads,
+                         const uint64_t* keys, const void* vals, int64_t n,
+                         uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    Params p{h, s, seed, step, shift, dtau, boost};
+    int64_t result = 0;
+    auto run = [&](auto tag) {
+        using V = decltype(tag);
+        const V* vv = (const V*)vals;
+        V* ov = (V*)out_vals;
+        const int T = nthreads;
+        std::vector<OpenMap<V>> cols((size_t)T * T);
+        std::vector<Stats> sts(T);
+        const int64_t per = n / T + 1;
+        for (auto& c : cols) c.init((size_t)(2 * per / T + 64));
+#pragma omp parallel num_threads(T)
+        {
+#ifdef _OPENMP
+            const int t = omp_get_thread_num();
+#else
+            const int t = 0;
+#endif
+            // perform_spawns!: every thread processes a contiguous share of the source
+            // entries and deposits into its own column, split by target segment.
+            const int64_t lo = (int64_t)t * n / T, hi = (int64_t)(t + 1) * n / T;
+            for (int64_t i = lo; i < hi; ++i) {
+                apply_column<V>(p, keys[i], vv[i], sts[t], [&](uint64_t k, V v) {
+                    cols[(size_t)t * T + segment_of(k, T)].add(k, v);
+                });
+            }
+#pragma omp barrier
+            // collect_local!: thread t merges column entries of segment t.
+            OpenMap<V>& seg = cols[(size_t)0 * T + t];
+            for (int c = 1; c < T; ++c) {
+                OpenMap<V>& src = cols[(size_t)c * T + t];
+                for (size_t i = 0; i <= src.mask; ++i) if (src.used[i]) seg.add(src.k[i], src.v[i]);
+            }
+        }
+        // move_and_compress! + gather (serial gather of the segments into flat output).
+        Stats st;
+        for (auto& x : sts) {
+            st.attempts += x.attempts; st.exact += x.exact; st.inexact += x.inexact; st.spawns += x.spawns;
+            st.deaths += x.deaths; st.clones += x.clones; st.zombies += x.zombies;
+        }
+        int64_t len_before = 0, len = 0;
+        double norm = 0;
+        std::vector<int64_t> seg_count(T, 0), seg_before(T, 0);
+        std::vector<double> seg_norm(T, 0);
+#pragma omp parallel for num_threads(T)
+        for (int t = 0; t < T; ++t) {
+            OpenMap<V>& seg = cols[(size_t)0 * T + t];
+            for (size_t i = 0; i <= seg.mask; ++i) {
+                if (!seg.used[i] || seg.v[i] == 0) { seg.used[i] = 0; continue; }
+                seg_before[t]++;
+                if constexpr (!std::is_integral<V>::value) {
+                    if (s->compression == 1) {
+                        const double th = s->compression_threshold;
+                        const double prob = std::fabs(seg.v[i]) / th;
+                        if (prob < 1) {
+                            P4 r = draw(seed, seg.k[i], 0, step, TAG_COMPRESS);
+                            const double sg = (seg.v[i] > 0) - (seg.v[i] < 0);
+                            seg.v[i] = (prob > u53(r.v[0], r.v[1])) ? th * sg : 0.0;
+                        }
+                        if (seg.v[i] == 0) { seg.used[i] = 0; continue; }
+                    }
+                }
+                seg_count[t]++;
+                seg_norm[t] += std::fabs((double)seg.v[i]);
+            }
+        }
+        for (int t = 0; t < T; ++t) { len_before += seg_before[t]; len += seg_count[t]; norm += seg_norm[t]; }
+        stats[0] = st.attempts; stats[1] = st.exact; stats[2] = st.inexact; stats[3] = st.spawns;
+        stats[4] = st.deaths; stats[5] = st.clones; stats[6] = st.zombies; stats[7] = (double)len_before;
+        stats[8] = (double)len; stats[9] = norm;
+        if (len > cap) { result = -len; return; }
+        std::vector<int64_t> off(T + 1, 0);
+        for (int t = 0; t < T; ++t) off[t + 1] = off[t] + seg_count[t];
+#pragma omp parallel for num_threads(T)
+        for (int t = 0; t < T; ++t) {
+            OpenMap<V>& seg = cols[(size_t)0 * T + t];
+            int64_t o = off[t];
+            for (size_t i = 0; i <= seg.mask; ++i)
+                if (seg.used[i]) { out_keys[o] = seg.k[i]; ov[o] = seg.v[i]; ++o; }
+        }
+        result = len;
+    };
+    if (vtype == 0) run((int64_t)0); else run((double)0);
+    return result;
+}
+
+}  // extern "C"

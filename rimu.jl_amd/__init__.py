@@ -1,0 +1,26 @@
+"""rimu_amd -- an MI355X (gfx950) FCIQMC step engine behind Rimu.jl's interfaces.
+
+The package mirrors the part of Rimu.jl's API that drives the FCIQMC step:
+``ProjectorMonteCarloProblem`` / ``init`` / ``solve`` / ``lomc`` (``lomc!``), the
+``AbstractDVec`` and ``StochasticStyle`` contracts, and the Hamiltonian interface of
+``HubbardReal1D`` and ``HubbardMom1D``.  All compute goes through ``librimu_hip.so``
+(``include/rimu_hip.h``); the package has no CPU fallback.
+
+Julia names with ``!`` map to: ``solve!`` -> ``PMCSimulation.solve_``, ``step!`` ->
+``PMCSimulation.step_``, ``lomc!`` -> ``lomc``.
+"""
+from . import _native
+from ._native import RimuError, CapacityError, annihilate, device_count
+from .addresses import BoseFS, near_uniform
+from .hamiltonians import (HubbardMom1D, HubbardReal1D, diagonal_element, dimension,
+                           get_offdiagonal, num_offdiagonals, offdiagonals, starting_address,
+                           hubbard_dispersion)
+from .styles import (IsDeterministic, IsDynamicSemistochastic, IsStochasticInteger,
+                     NoCompression, ThresholdCompression, WithReplacement, default_style)
+from .dictvectors import DVec, DeviceDVec, FrozenDVec, freeze
+from .strategies import (ConstantTimeStep, DoubleLogUpdate, DoubleLogUpdateAfterTargetWalkers,
+                         DontUpdate, LogUpdate, LogUpdateAfterTargetWalkers, ProjectedEnergy)
+from .pmc import (PMCSimulation, ProjectorMonteCarloProblem, default_starting_vector, init,
+                  lomc, solve)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

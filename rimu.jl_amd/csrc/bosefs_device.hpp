@@ -1,0 +1,271 @@
+// bosefs_device.hpp -- BoseFS addresses, Hubbard matrix elements and Philox streams,
+// written as register-only bit arithmetic for one wavefront lane.
+//
+// Address layout (BitStringAddresses/bitstring.jl:464-471): one 64-bit chunk holds
+// B = N + M - 1 bits; mode 1 sits in the least significant bits and every mode is n_i
+// one-bits followed by a zero separator.  A "run" is the block of ones of one mode.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rimu {
+
+// ---------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1), ten rounds.
+// ---------------------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// Stream purposes live in the top two bits of counter word 3, the step in the rest.
+enum : uint32_t { TAG_SPAWN = 0u, TAG_DIAG = 1u, TAG_COMPRESS = 2u };
+
+__host__ __device__ inline U4 draw(uint64_t seed, uint64_t addr, uint32_t attempt, int64_t step,
+                                   uint32_t tag) {
+    const uint32_t c3 = ((uint32_t)step & 0x3FFFFFFFu) | (tag << 30);
+    return philox4x32_10((uint32_t)addr, (uint32_t)(addr >> 32), attempt, c3, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+}
+
+// 53-bit uniform double in [0, 1).
+__host__ __device__ inline double u53(uint32_t hi, uint32_t lo) {
+    const uint64_t x = ((((uint64_t)hi) << 32) | lo) >> 11;
+    return (double)x * 0x1.0p-53;
+}
+
+// Uniform index in 1..n (multiply-shift).
+__host__ __device__ inline int64_t pick1(uint32_t r, int64_t n) {
+    return 1 + (int64_t)(((uint64_t)r * (uint64_t)n) >> 32);
+}
+
+// ---------------------------------------------------------------------------------------
+// Bit helpers
+// ---------------------------------------------------------------------------------------
+__device__ inline int popc64(uint64_t x) { return __popcll(x); }
+__device__ inline int ctz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
+__device__ inline int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+__device__ inline uint64_t low_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+__device__ inline uint64_t shr(uint64_t x, int n) { return n >= 64 ? 0ull : (x >> n); }
+__device__ inline uint64_t shl(uint64_t x, int n) { return n >= 64 ? 0ull : (x << n); }
+
+// Position of the k-th (0-based) set bit of x; x must have more than k set bits.
+__device__ inline int select_bit(uint64_t x, int k) {
+    int pos = 0;
+    int c = __popc((uint32_t)x);
+    if (k >= c) { k -= c; x >>= 32; pos += 32; }
+    uint32_t v = (uint32_t)x;
+    c = __popc(v & 0xFFFFu);
+    if (k >= c) { k -= c; v >>= 16; pos += 16; }
+    c = __popc(v & 0xFFu);
+    if (k >= c) { k -= c; v >>= 8; pos += 8; }
+    c = __popc(v & 0xFu);
+    if (k >= c) { k -= c; v >>= 4; pos += 4; }
+    c = __popc(v & 0x3u);
+    if (k >= c) { k -= c; v >>= 2; pos += 2; }
+    c = (int)(v & 1u);
+    if (k >= c) { pos += 1; }
+    return pos;
+}
+
+// Number of consecutive ones starting at bit p upwards.
+__device__ inline int ones_from(uint64_t bits, int p) { return ctz64(~shr(bits, p)); }
+
+// First bit of the run of mode m (1-based): modes are separated by zeros, so the run of
+// mode m starts right after the (m-1)-th zero.
+__device__ inline int mode_start(uint64_t bits, int m) {
+    return m == 1 ? 0 : select_bit(~bits, m - 2) + 1;
+}
+
+// Run starts: a one whose lower neighbour is zero (or bit 0).
+__device__ inline uint64_t run_starts(uint64_t bits) { return bits & ~(bits << 1); }
+
+// Delete the bit at position p (higher bits move down).
+__device__ inline uint64_t delete_bit(uint64_t x, int p) {
+    return (x & low_mask(p)) | shl(shr(x, p + 1), p);
+}
+// Insert a one at position p (bits at p and above move up).
+__device__ inline uint64_t insert_one(uint64_t x, int p) {
+    return (x & low_mask(p)) | (1ull << p) | shl(shr(x, p), p + 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// Model data passed to kernels by value.
+// ---------------------------------------------------------------------------------------
+struct ModelDev {
+    int kind;   // 1 HubbardReal1D, 2 HubbardMom1D
+    int M, N, B;
+    double u, t;
+    double kes[64];
+};
+
+// Sum over occupied modes of n(n-1) (bose_hubbard_interaction, bosefs.jl:415-428).
+__device__ inline int64_t bose_hubbard_interaction(uint64_t bits) {
+    int64_t acc = 0;
+    uint64_t x = bits;
+    while (x) {
+        x = shr(x, ctz64(x));
+        const int n = ctz64(~x);
+        acc += (int64_t)n * (n - 1);
+        x = shr(x, n);
+    }
+    return acc;
+}
+
+__device__ inline int num_occupied(uint64_t bits) { return popc64(run_starts(bits)); }
+
+// num_offdiagonals: HubbardReal1D.jl:51-53 (2 * occupied modes);
+// HubbardMom1D.jl:131-140 (s(s-1)(M-2) + d(M-1)).
+__device__ inline int64_t num_offdiagonals(const ModelDev& h, uint64_t bits) {
+    const uint64_t st = run_starts(bits);
+    const int64_t s = popc64(st);
+    if (h.kind == 1) return 2 * s;
+    const int64_t d = popc64(st & (bits >> 1));  // runs of length >= 2
+    return s * (s - 1) * (h.M - 2) + d * (h.M - 1);
+}
+
+// diagonal_element: HubbardReal1D.jl:55-57; HubbardMom1D.jl:174-177 (kinetic sum over
+// occupied modes in ascending order, then u/(2M) * onproduct, excitations.jl:126-137).
+__device__ inline double diagonal_element(const ModelDev& h, uint64_t bits) {
+    if (h.kind == 1) return h.u * (double)bose_hubbard_interaction(bits) / 2;
+    double kin = 0.0;
+    int64_t sumsq = 0;
+    uint64_t x = bits;
+    int mode = 1;
+    while (x) {
+        const int z = ctz64(x);
+        mode += z;
+        x = shr(x, z);
+        const int n = ctz64(~x);
+        kin += h.kes[mode - 1] * n;
+        sumsq += (int64_t)n * n;
+        x = shr(x, n);
+        // the separator after this run is consumed by the next ctz (counts as one mode step)
+    }
+    const int64_t N = h.N;
+    const int64_t onproduct = 2 * N * N - N - sumsq;  // sum n(n-1) + 4 sum_{i<j} n_i n_j
+    return kin + h.u / (2.0 * h.M) * (double)onproduct;
+}
+
+// hopnextneighbour (bosefs.jl:297-345) with value -t*sqrt(prod) (HubbardReal1D.jl:59-62).
+// chosen in 1..2*s: site (chosen+1)/2 is the site-th occupied mode; odd hops right.
+__device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
+                                              double* value) {
+    const int site = (int)((chosen + 1) >> 1);
+    const uint64_t st = run_starts(bits);
+    const int p = select_bit(st, site - 1);
+    const int n = ones_from(bits, p);
+    const int m = p - popc64(bits & low_mask(p)) + 1;  // mode = zeros below p, plus one
+    const int B = h.B;
+    const uint64_t maskB = low_mask(B);
+    uint64_t nb;
+    int64_t prod;
+    if (chosen & 1) {  // right
+        if (m == h.M) {
+            nb = ((bits << 1) | 1ull) & maskB;
+            prod = (int64_t)n * (ones_from(bits, 0) + 1);
+        } else {
+            const int q = p + n - 1;
+            nb = bits ^ (3ull << q);
+            prod = (int64_t)n * (ones_from(bits, q + 2) + 1);
+        }
+    } else {  // left
+        if (m == 1) {
+            nb = (bits >> 1) | (1ull << (B - 1));
+            const int hz = 63 - clz64(~bits & maskB);  // highest zero inside the B bits
+            prod = (int64_t)n * ((B - 1 - hz) + 1);
+        } else {
+            nb = bits ^ (3ull << (p - 1));
+            int nprev = 0;
+            if (p >= 2) {
+                const uint64_t below = bits & low_mask(p - 1);  // bits 0 .. p-2
+                const uint64_t z = ~below & low_mask(p - 1);
+                nprev = z ? (p - 2 - (63 - clz64(z))) : (p - 1);
+            }
+            prod = (int64_t)n * (nprev + 1);
+        }
+    }
+    *value = -h.t * sqrt((double)prod);
+    return nb;
+}
+
+__device__ inline int imod1(int x, int m) { int r = ((x - 1) % m + m) % m; return r + 1; }
+
+// Occupation number of mode m.
+__device__ inline int occupation(uint64_t bits, int m) { return ones_from(bits, mode_start(bits, m)); }
+
+// momentum_transfer_excitation (fold=true), excitations.jl:26-80, value u/(2M)*sqrt(...)
+// (HubbardMom1D.jl:189-194).  The map of occupied modes is read from the run starts.
+__device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
+                                             double* value) {
+    const int M = h.M;
+    const uint64_t st = run_starts(bits);
+    const int64_t s = popc64(st);
+    const int64_t dbl = chosen - s * (s - 1) * (M - 2);
+    int p_mode, q_mode;  // source modes (p <= q)
+    int64_t mom;
+    if (dbl > 0) {
+        mom = (dbl - 1) % (M - 1) + 1;
+        const int64_t d = (dbl - mom) / (M - 1) + 1;
+        const uint64_t dstarts = st & (bits >> 1);
+        const int pos = select_bit(dstarts, (int)d - 1);
+        p_mode = q_mode = pos - popc64(bits & low_mask(pos)) + 1;
+    } else {
+        mom = (chosen - 1) % (M - 2) + 1;
+        const int64_t pair = (chosen - mom) / (M - 2) + 1;
+        const int64_t snd = (pair - 1) % (s - 1) + 1;
+        const int64_t fst = (pair - snd) / (s - 1) + 1;
+        int64_t f_hole, s_hole;
+        if (snd < fst) { f_hole = snd; s_hole = fst; }
+        else { f_hole = fst; s_hole = snd + 1; }
+        const int pf = select_bit(st, (int)f_hole - 1);
+        const int ps = select_bit(st, (int)s_hole - 1);
+        p_mode = pf - popc64(bits & low_mask(pf)) + 1;
+        q_mode = ps - popc64(bits & low_mask(ps)) + 1;
+        if (mom >= q_mode - p_mode) mom += 1;
+    }
+    const int r_mode = imod1((int)(p_mode + mom), M);
+    const int s_mode = imod1((int)(q_mode - mom), M);
+    // bose_excitation_value (fockaddress.jl:559-567) with reversed operator lists:
+    // destroy q, destroy p, create s, create r -- each sees the previous operators.
+    const int np = occupation(bits, p_mode);
+    const int nq = occupation(bits, q_mode);
+    const int nr = occupation(bits, r_mode);
+    const int ns = occupation(bits, s_mode);
+    int64_t v = nq;
+    v *= (np - (p_mode == q_mode));
+    v *= (ns - (s_mode == q_mode) - (s_mode == p_mode) + 1);
+    v *= (nr - (r_mode == q_mode) - (r_mode == p_mode) + (r_mode == s_mode) + 1);
+    // new address: remove one boson from p and q, add one to r and s
+    uint64_t nb = bits;
+    nb = delete_bit(nb, mode_start(nb, q_mode));
+    nb = delete_bit(nb, mode_start(nb, p_mode));
+    nb = insert_one(nb, mode_start(nb, s_mode));
+    nb = insert_one(nb, mode_start(nb, r_mode));
+    *value = h.u / (2 * M) * sqrt((double)v);
+    return nb;
+}
+
+__device__ inline uint64_t get_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
+                                           double* value) {
+    return h.kind == 1 ? real1d_offdiagonal(h, bits, chosen, value)
+                       : mom1d_offdiagonal(h, bits, chosen, value);
+}
+
+}  // namespace rimu

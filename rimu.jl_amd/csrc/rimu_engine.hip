@@ -1,0 +1,748 @@
+// rimu_engine.hip -- host side of the FCIQMC step engine and the C ABI of include/rimu_hip.h.
+//
+// One engine = one FCIQMC state: a sorted device vector (keys u64, values int64 or f64),
+// double buffered, plus the working memory of a step (diagonal values, work offsets, the
+// spawn list and its sort buffers, look-back status words, per-tile partial sums).
+// All work of an engine is ordered on its own HIP stream.  A step synchronises with the
+// host three times (work-item count, spawn count, statistics) so that every grid is sized
+// to the data it covers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rimu_hip.h"
+#include "step_kernels.hpp"
+
+using namespace rimu;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(RIMU_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e));  \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        if (count <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct KernelTimer {
+    bool enabled = false;
+    struct Rec { double ms = 0; int64_t launches = 0; };
+    std::map<std::string, Rec> totals;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void begin(const char* name, hipStream_t s, hipEvent_t* a) {
+        if (!enabled) return;
+        *a = get();
+        (void)hipEventRecord(*a, s);
+        pending.push_back({name, {*a, nullptr}});
+    }
+    void end(hipStream_t s) {
+        if (!enabled) return;
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, s);
+        pending.back().second.second = b;
+    }
+    void collect() {  // after a stream synchronisation
+        for (auto& p : pending) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
+            auto& r = totals[p.first];
+            r.ms += ms;
+            r.launches += 1;
+            pool.push_back(p.second.first);
+            pool.push_back(p.second.second);
+        }
+        pending.clear();
+    }
+    ~KernelTimer() {
+        for (auto& e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct rimu_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rimu_model model{};
+    rimu_style style{};
+    ModelDev hdev{};
+    uint64_t seed = 0;
+    bool int_vals = false;
+    size_t vsize = 8;
+    int64_t max_entries = 0, max_spawns = 0;
+
+    // vector (double buffered)
+    DevBuf<uint64_t> vkeys[2];
+    DevBuf<unsigned char> vvals[2];
+    int cur = 0;
+    int64_t len = 0;
+
+    // step working memory
+    DevBuf<unsigned char> dvals;
+    DevBuf<int64_t> woff;
+    DevBuf<int64_t> scan_tmp;
+    DevBuf<uint32_t> rcounts, rtmp;
+    DevBuf<uint64_t> skeys[2];
+    DevBuf<unsigned char> svals[2];
+    DevBuf<uint64_t> status;
+    DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev;
+    DevBuf<unsigned char> misc;  // counters and totals (see layout below)
+    // projector
+    DevBuf<uint64_t> pkeys;
+    DevBuf<double> pcoef;
+    int64_t np = 0, nh = 0;
+
+    // host staging
+    struct Misc {
+        uint32_t tile_counter;
+        int32_t flags;
+        int64_t W;
+        int64_t S;
+        int64_t L_new;
+        unsigned long long key_or;
+        unsigned long long key_and;
+    };
+    Misc* h_misc = nullptr;    // pinned
+    double* h_stats = nullptr; // pinned, 12 doubles
+
+    KernelTimer timer;
+    double last_alg_bytes = 0, last_records = 0;
+};
+
+namespace {
+
+ModelDev to_dev(const rimu_model& m) {
+    ModelDev h{};
+    h.kind = m.kind;
+    h.M = m.M;
+    h.N = m.N;
+    h.B = m.M + m.N - 1;
+    h.u = m.u;
+    h.t = m.t;
+    for (int j = 0; j < 64; ++j) h.kes[j] = m.kes[j];
+    return h;
+}
+
+int check_model(const rimu_model* m) {
+    if (!m) return fail(RIMU_ERR_ARG, "model is null");
+    if (m->kind != RIMU_MODEL_HUBBARD_REAL_1D && m->kind != RIMU_MODEL_HUBBARD_MOM_1D)
+        return fail(RIMU_ERR_ARG, "unknown model kind");
+    if (m->M < 1 || m->N < 1 || m->M + m->N - 1 > 64)
+        return fail(RIMU_ERR_ARG, "address must fit one 64-bit chunk: need N + M - 1 <= 64");
+    if (m->kind == RIMU_MODEL_HUBBARD_MOM_1D && m->M < 3)
+        return fail(RIMU_ERR_ARG, "HubbardMom1D needs M >= 3");
+    if (m->kind == RIMU_MODEL_HUBBARD_REAL_1D && m->M < 2)
+        return fail(RIMU_ERR_ARG, "HubbardReal1D needs M >= 2");
+    return RIMU_OK;
+}
+
+StepArgs make_args(rimu_engine* e, int64_t step, double shift, double dtau, double boost) {
+    StepArgs a{};
+    a.h = e->hdev;
+    a.seed = e->seed;
+    a.step = step;
+    a.shift = shift;
+    a.dtau = dtau;
+    a.boost = boost;
+    a.style = e->style.kind;
+    a.compression = e->style.compression;
+    a.comp_threshold = e->style.compression_threshold;
+    a.proj_threshold = e->style.proj_threshold;
+    a.rel_threshold = e->style.rel_threshold;
+    a.abs_threshold = e->style.abs_threshold;
+    return a;
+}
+
+// exclusive scan of n int64 values in place (data has n entries)
+int scan_inplace_i64(rimu_engine* e, int64_t* data, int64_t n) {
+    const int64_t tiles = ceil_div(n, SCAN_TILE);
+    HIP_TRY(e->scan_tmp.alloc(tiles + 1));
+    scan_tiles_kernel<int64_t><<<tiles, NT, 0, e->stream>>>(data, data, n, e->scan_tmp.p);
+    scan_sums_kernel<int64_t><<<1, 1024, 0, e->stream>>>(e->scan_tmp.p, tiles, nullptr);
+    scan_add_kernel<int64_t><<<tiles, NT, 0, e->stream>>>(data, n, e->scan_tmp.p);
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
+int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
+    const int64_t tiles = ceil_div(n, SCAN_TILE);
+    HIP_TRY(e->rtmp.alloc(tiles + 1));
+    scan_tiles_kernel<uint32_t><<<tiles, NT, 0, e->stream>>>(in, out, n, e->rtmp.p);
+    scan_sums_kernel<uint32_t><<<1, 1024, 0, e->stream>>>(e->rtmp.p, tiles, nullptr);
+    scan_add_kernel<uint32_t><<<tiles, NT, 0, e->stream>>>(out, n, e->rtmp.p);
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
+// Stable LSD radix sort of n (key, value) records on key bits [0, nbits), skipping passes
+// whose digit is the same for every key.  Input in buffer `in`; returns the buffer index
+// holding the result.
+template <class V>
+int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, int* result) {
+    int src = in;
+    if (n <= 1) { *result = src; return RIMU_OK; }
+    const int64_t tiles = ceil_div(n, SORT_TILE);
+    HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
+    for (int shift = 0; shift < nbits; shift += RADIX_BITS) {
+        if (((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
+        hipEvent_t ev{};
+        e->timer.begin("radix_count", e->stream, &ev);
+        radix_count_kernel<<<tiles, NT, 0, e->stream>>>(e->skeys[src].p, n, shift, e->rcounts.p, tiles);
+        e->timer.end(e->stream);
+        e->timer.begin("radix_scan", e->stream, &ev);
+        int rc = scan_u32(e, e->rcounts.p, e->rcounts.p, tiles * RADIX);
+        e->timer.end(e->stream);
+        if (rc) return rc;
+        e->timer.begin("radix_scatter", e->stream, &ev);
+        radix_scatter_kernel<V><<<tiles, NT, 0, e->stream>>>(
+            e->skeys[src].p, (const V*)e->svals[src].p, n, shift, e->rcounts.p, tiles,
+            e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
+        e->timer.end(e->stream);
+        HIP_TRY(hipGetLastError());
+        src = 1 - src;
+    }
+    *result = src;
+    return RIMU_OK;
+}
+
+template <class V>
+int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
+              rimu_step_stats* out) {
+    std::memset(out, 0, sizeof(*out));
+    const StepArgs a = make_args(e, step, shift, dtau, boost);
+    const int64_t L = e->len;
+    const uint64_t* vk = e->vkeys[e->cur].p;
+    const V* vv = (const V*)e->vvals[e->cur].p;
+    rimu_engine::Misc* dm = (rimu_engine::Misc*)e->misc.p;
+    hipEvent_t ev{};
+
+    HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
+    {   // key_and starts at all ones
+        const unsigned long long ones = ~0ull;
+        HIP_TRY(hipMemcpyAsync(&dm->key_and, &ones, sizeof(ones), hipMemcpyHostToDevice, e->stream));
+    }
+
+    // 1. plan
+    const int64_t plan_tiles = ceil_div(L + 1, NT);
+    HIP_TRY(e->plan_p.alloc((size_t)plan_tiles * 6));
+    e->timer.begin("plan", e->stream, &ev);
+    plan_kernel<V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p, e->plan_p.p);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    e->timer.begin("scan", e->stream, &ev);
+    if (int rc = scan_inplace_i64(e, e->woff.p, L + 1)) return rc;
+    e->timer.end(e->stream);
+    HIP_TRY(hipMemcpyAsync(&e->h_misc->W, e->woff.p + L, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const int64_t W = e->h_misc->W;
+
+    // 2. spawn
+    int64_t S = 0;
+    int64_t spawn_tiles = 0;
+    uint64_t kor = 0, kand = ~0ull;
+    if (W > 0) {
+        spawn_tiles = ceil_div(W, NT);
+        HIP_TRY(e->status.alloc(std::max<int64_t>(spawn_tiles, 1)));
+        HIP_TRY(e->spawn_p.alloc(spawn_tiles));
+        HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
+        e->timer.begin("spawn", e->stream, &ev);
+        spawn_kernel<V><<<spawn_tiles, NT, 0, e->stream>>>(
+            a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
+            e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, &dm->key_or, &dm->key_and);
+        e->timer.end(e->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "spawn look-back did not complete");
+        if (e->h_misc->flags & FLAG_SPAWN_OVERFLOW) {
+            e->timer.collect();
+            out->overflow = 1;
+            return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
+        }
+        S = e->h_misc->S;
+        kor = e->h_misc->key_or;
+        kand = e->h_misc->key_and;
+    }
+
+    // 3. sort the spawn records
+    int sorted = 0;
+    if (S > 1) {
+        const int nbits = e->hdev.B;
+        if (int rc = radix_sort<V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
+    }
+
+    // 4. merge with the diagonal survivors, fold, compress, count
+    const int nxt = 1 - e->cur;
+    const int64_t merge_tiles = std::max<int64_t>(ceil_div(S + L, NT * MERGE_IPT), 1);
+    HIP_TRY(e->status.alloc(merge_tiles));
+    HIP_TRY(e->merge_p.alloc((size_t)merge_tiles * 3));
+    HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * merge_tiles, e->stream));
+    HIP_TRY(hipMemsetAsync(&dm->tile_counter, 0, sizeof(uint32_t), e->stream));
+    e->timer.begin("merge", e->stream, &ev);
+    merge_kernel<V><<<merge_tiles, NT, 0, e->stream>>>(
+        a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk, (const V*)e->dvals.p, L,
+        e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, e->status.p, &dm->tile_counter,
+        &dm->L_new, &dm->flags, e->merge_p.p);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+
+    // 5. projections and statistics
+    if (e->np + e->nh > 0) {
+        project_kernel<V><<<ceil_div(e->np + e->nh, NT), NT, 0, e->stream>>>(
+            e->vkeys[nxt].p, (const V*)e->vvals[nxt].p, &dm->L_new, e->pkeys.p, e->pcoef.p,
+            e->np + e->nh, e->terms.p);
+        HIP_TRY(hipGetLastError());
+    }
+    finalize_kernel<<<1, NT, 0, e->stream>>>(e->plan_p.p, plan_tiles, e->spawn_p.p, spawn_tiles,
+                                             e->merge_p.p, merge_tiles, e->terms.p, e->np, e->nh,
+                                             e->stats_dev.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, 12 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->timer.collect();
+    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "merge look-back did not complete");
+    if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) {
+        out->overflow = 1;
+        return fail(RIMU_ERR_CAPACITY, "vector capacity (max_entries) exceeded");
+    }
+    const double* s = e->h_stats;
+    out->spawn_attempts = s[0];
+    out->exact_steps = s[1];
+    out->inexact_steps = s[2];
+    out->spawns = s[3];
+    out->deaths = s[4];
+    out->clones = s[5];
+    out->zombies = s[6];
+    out->len_before = s[7];
+    out->len = s[8];
+    out->norm = s[9];
+    out->vproj = s[10];
+    out->hproj = s[11];
+    const int64_t L_new = e->h_misc->L_new;
+    e->cur = nxt;
+    e->len = L_new;
+    // algorithmic bytes: read v, write S records, read S records into the merge, write v'
+    const double rec = 8.0 + (double)e->vsize;
+    const double records = (double)(S + L);  // diagonal survivors included
+    e->last_records = records;
+    e->last_alg_bytes = (double)L * rec + 2.0 * records * rec + (double)L_new * rec;
+    return RIMU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rimu_last_error(void) { return g_last_error.c_str(); }
+
+int rimu_device_count(int32_t* count) {
+    int n = 0;
+    hipError_t err = hipGetDeviceCount(&n);
+    if (err != hipSuccess) n = 0;
+    if (count) *count = n;
+    return RIMU_OK;
+}
+
+int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_t seed,
+                       int64_t max_entries, int64_t max_spawns, rimu_engine** out) {
+    if (!out) return fail(RIMU_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (int rc = check_model(model)) return rc;
+    if (!style) return fail(RIMU_ERR_ARG, "style is null");
+    if (style->kind < 1 || style->kind > 3) return fail(RIMU_ERR_ARG, "unknown style kind");
+    if (style->kind == RIMU_STYLE_STOCHASTIC_INTEGER && style->proj_threshold != 0)
+        return fail(RIMU_ERR_ARG, "Thresholding not supported for integer spawns");
+    if (max_entries < 1 || max_spawns < 1) return fail(RIMU_ERR_ARG, "capacities must be positive");
+    if (max_spawns >= (int64_t)UINT32_MAX) return fail(RIMU_ERR_ARG, "max_spawns must be below 2^32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
+    rimu_engine* e = new rimu_engine();
+    e->model = *model;
+    e->style = *style;
+    e->hdev = to_dev(*model);
+    e->seed = seed;
+    e->int_vals = style->kind == RIMU_STYLE_STOCHASTIC_INTEGER;
+    e->vsize = 8;
+    e->max_entries = max_entries;
+    e->max_spawns = max_spawns;
+    auto bad = [&](const std::string& what, hipError_t err) {
+        std::string m = what + ": " + hipGetErrorString(err);
+        rimu_engine_destroy(e);
+        return fail(RIMU_ERR_HIP, m);
+    };
+    hipError_t err;
+    if ((err = hipGetDevice(&e->device)) != hipSuccess) return bad("hipGetDevice", err);
+    if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bad("stream", err);
+    for (int b = 0; b < 2; ++b) {
+        if ((err = e->vkeys[b].alloc(max_entries)) != hipSuccess) return bad("vector keys", err);
+        if ((err = e->vvals[b].alloc(max_entries * 8)) != hipSuccess) return bad("vector values", err);
+        if ((err = e->skeys[b].alloc(max_spawns)) != hipSuccess) return bad("spawn keys", err);
+        if ((err = e->svals[b].alloc(max_spawns * 8)) != hipSuccess) return bad("spawn values", err);
+    }
+    if ((err = e->dvals.alloc(max_entries * 8)) != hipSuccess) return bad("diagonal values", err);
+    if ((err = e->woff.alloc(max_entries + 1)) != hipSuccess) return bad("work offsets", err);
+    if ((err = e->misc.alloc(sizeof(rimu_engine::Misc))) != hipSuccess) return bad("misc", err);
+    if ((err = e->stats_dev.alloc(16)) != hipSuccess) return bad("stats", err);
+    if ((err = e->terms.alloc(16)) != hipSuccess) return bad("terms", err);
+    if ((err = e->spawn_p.alloc(16)) != hipSuccess) return bad("partials", err);
+    if ((err = hipHostMalloc((void**)&e->h_misc, sizeof(rimu_engine::Misc), hipHostMallocDefault)) != hipSuccess)
+        return bad("pinned misc", err);
+    if ((err = hipHostMalloc((void**)&e->h_stats, 16 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
+        return bad("pinned stats", err);
+    *out = e;
+    return RIMU_OK;
+}
+
+int rimu_engine_destroy(rimu_engine* e) {
+    if (!e) return RIMU_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (int b = 0; b < 2; ++b) {
+        e->vkeys[b].release();
+        e->vvals[b].release();
+        e->skeys[b].release();
+        e->svals[b].release();
+    }
+    e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release();
+    e->rtmp.release(); e->status.release(); e->plan_p.release(); e->spawn_p.release();
+    e->merge_p.release(); e->terms.release(); e->stats_dev.release(); e->misc.release();
+    e->pkeys.release(); e->pcoef.release();
+    if (e->h_misc) (void)hipHostFree(e->h_misc);
+    if (e->h_stats) (void)hipHostFree(e->h_stats);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return RIMU_OK;
+}
+
+int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (n < 0 || (n > 0 && (!keys || !vals))) return fail(RIMU_ERR_ARG, "bad vector arguments");
+    const uint64_t maskB = (e->hdev.B >= 64) ? ~0ull : ((1ull << e->hdev.B) - 1);
+    std::vector<uint64_t> k;
+    std::vector<int64_t> vi;
+    std::vector<double> vd;
+    k.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && keys[i] <= keys[i - 1]) return fail(RIMU_ERR_ARG, "keys must be strictly ascending");
+        if ((keys[i] & ~maskB) != 0 || __builtin_popcountll(keys[i]) != e->hdev.N)
+            return fail(RIMU_ERR_ARG, "key is not an N-boson M-mode BoseFS address");
+        if (e->int_vals) {
+            const int64_t v = ((const int64_t*)vals)[i];
+            if (v == 0) continue;
+            k.push_back(keys[i]);
+            vi.push_back(v);
+        } else {
+            const double v = ((const double*)vals)[i];
+            if (v == 0) continue;
+            k.push_back(keys[i]);
+            vd.push_back(v);
+        }
+    }
+    const int64_t m = (int64_t)k.size();
+    if (m > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
+    if (m > 0) {
+        HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, k.data(), m * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->vvals[e->cur].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
+                               m * 8, hipMemcpyHostToDevice, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->len = m;
+    return RIMU_OK;
+}
+
+int rimu_vector_length(rimu_engine* e, int64_t* n) {
+    if (!e || !n) return fail(RIMU_ERR_ARG, "null argument");
+    *n = e->len;
+    return RIMU_OK;
+}
+
+int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap, int64_t* n) {
+    if (!e || !n) return fail(RIMU_ERR_ARG, "null argument");
+    *n = e->len;
+    if (e->len > cap) return fail(RIMU_ERR_CAPACITY, "download capacity too small");
+    if (e->len > 0) {
+        HIP_TRY(hipMemcpyAsync(keys, e->vkeys[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(vals, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RIMU_OK;
+}
+
+int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int64_t n,
+                       const uint64_t* hk, const double* hc, int64_t nh) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (n < 0 || nh < 0) return fail(RIMU_ERR_ARG, "negative projector length");
+    const int64_t tot = n + nh;
+    HIP_TRY(e->pkeys.alloc(std::max<int64_t>(tot, 1)));
+    HIP_TRY(e->pcoef.alloc(std::max<int64_t>(tot, 1)));
+    HIP_TRY(e->terms.alloc(std::max<int64_t>(tot, 16)));
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(e->pkeys.p, pk, n * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->pcoef.p, pc, n * 8, hipMemcpyHostToDevice, e->stream));
+    }
+    if (nh > 0) {
+        HIP_TRY(hipMemcpyAsync(e->pkeys.p + n, hk, nh * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->pcoef.p + n, hc, nh * 8, hipMemcpyHostToDevice, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->np = n;
+    e->nh = nh;
+    return RIMU_OK;
+}
+
+int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
+              rimu_step_stats* stats) {
+    if (!e || !stats) return fail(RIMU_ERR_ARG, "null argument");
+    if (step < 0 || step >= (1ll << 30)) return fail(RIMU_ERR_ARG, "step must be in [0, 2^30)");
+    if (!(dtau > 0) || !(boost > 0)) return fail(RIMU_ERR_ARG, "dtau and boost must be positive");
+    if (e->len == 0) {
+        std::memset(stats, 0, sizeof(*stats));
+        return RIMU_OK;
+    }
+    if (e->len > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
+    return e->int_vals ? step_impl<int64_t>(e, step, shift, dtau, boost, stats)
+                       : step_impl<double>(e, step, shift, dtau, boost, stats);
+}
+
+int rimu_engine_set_timing(rimu_engine* e, int32_t enabled) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    e->timer.enabled = enabled != 0;
+    e->timer.totals.clear();
+    return RIMU_OK;
+}
+
+int rimu_engine_kernel_times(rimu_engine* e, char* names, int32_t name_len, double* ms,
+                             int64_t* launches, int32_t cap, int32_t* n) {
+    if (!e || !n) return fail(RIMU_ERR_ARG, "null argument");
+    int i = 0;
+    for (auto& kv : e->timer.totals) {
+        if (i >= cap) break;
+        if (names && name_len > 0) {
+            std::strncpy(names + (size_t)i * name_len, kv.first.c_str(), name_len - 1);
+            names[(size_t)i * name_len + name_len - 1] = 0;
+        }
+        if (ms) ms[i] = kv.second.ms;
+        if (launches) launches[i] = kv.second.launches;
+        ++i;
+    }
+    *n = i;
+    return RIMU_OK;
+}
+
+int rimu_engine_last_traffic(rimu_engine* e, double* alg_bytes, double* records) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (alg_bytes) *alg_bytes = e->last_alg_bytes;
+    if (records) *records = e->last_records;
+    return RIMU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Stand-alone annihilation and Hamiltonian evaluation
+// ---------------------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(NT) void ham_eval_kernel(ModelDev h, const uint64_t* __restrict__ addrs,
+                                                      const int64_t* __restrict__ chosen, int64_t n,
+                                                      double* __restrict__ diag, int64_t* __restrict__ nod,
+                                                      uint64_t* __restrict__ naddr,
+                                                      double* __restrict__ oval) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = addrs[i];
+    diag[i] = diagonal_element(h, k);
+    const int64_t c = num_offdiagonals(h, k);
+    nod[i] = c;
+    const int64_t ch = chosen[i];
+    if (ch >= 1 && ch <= c) {
+        double v;
+        naddr[i] = get_offdiagonal(h, k, ch, &v);
+        oval[i] = v;
+    } else {
+        naddr[i] = k;
+        oval[i] = 0.0;
+    }
+}
+
+template <class V>
+int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n, int nbits,
+                    uint64_t varying, uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out) {
+    rimu_engine::Misc* dm = (rimu_engine::Misc*)e->misc.p;
+    HIP_TRY(hipMemcpyAsync(e->skeys[0].p, keys, n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->svals[0].p, vals, n * 8, hipMemcpyHostToDevice, e->stream));
+    int sorted = 0;
+    if (int rc = radix_sort<V>(e, n, nbits, varying, 0, &sorted)) return rc;
+    StepArgs a = make_args(e, 0, 0.0, 1.0, 1.0);
+    a.compression = 0;
+    const int64_t tiles = std::max<int64_t>(ceil_div(n, NT * MERGE_IPT), 1);
+    HIP_TRY(e->status.alloc(tiles));
+    HIP_TRY(e->merge_p.alloc((size_t)tiles * 3));
+    HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * tiles, e->stream));
+    HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
+    merge_kernel<V><<<tiles, NT, 0, e->stream>>>(a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n,
+                                                 e->vkeys[0].p, (const V*)e->dvals.p, 0, e->vkeys[1].p,
+                                                 (V*)e->vvals[1].p, e->max_entries, e->status.p,
+                                                 &dm->tile_counter, &dm->L_new, &dm->flags, e->merge_p.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "look-back did not complete");
+    const int64_t m = e->h_misc->L_new;
+    *n_out = m;
+    if (m > cap) return fail(RIMU_ERR_CAPACITY, "output capacity too small");
+    if (m > 0) {
+        HIP_TRY(hipMemcpyAsync(out_keys, e->vkeys[1].p, m * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(out_vals, e->vvals[1].p, m * 8, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RIMU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rimu_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
+                    uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out) {
+    if (!n_out) return fail(RIMU_ERR_ARG, "n_out is null");
+    *n_out = 0;
+    if (vtype != 0 && vtype != 1) return fail(RIMU_ERR_ARG, "vtype must be 0 (int64) or 1 (double)");
+    if (n < 0 || (n > 0 && (!keys || !vals))) return fail(RIMU_ERR_ARG, "bad list arguments");
+    if (n == 0) return RIMU_OK;
+    if (n >= (int64_t)UINT32_MAX) return fail(RIMU_ERR_ARG, "list longer than 2^32 - 2");
+    uint64_t kor = 0, kand = ~0ull;
+    for (int64_t i = 0; i < n; ++i) { kor |= keys[i]; kand &= keys[i]; }
+    const int nbits = kor ? 64 - __builtin_clzll(kor) : 1;
+    rimu_model m{};
+    m.kind = RIMU_MODEL_HUBBARD_REAL_1D;
+    m.M = 2;
+    m.N = 1;
+    rimu_style s{};
+    s.kind = vtype == 0 ? RIMU_STYLE_STOCHASTIC_INTEGER : RIMU_STYLE_DETERMINISTIC;
+    rimu_engine* e = nullptr;
+    if (int rc = rimu_engine_create(&m, &s, 0, n, n, &e)) return rc;
+    const int rc = vtype == 0
+        ? annihilate_impl<int64_t>(e, keys, vals, n, nbits, kor ^ kand, out_keys, out_vals, cap, n_out)
+        : annihilate_impl<double>(e, keys, vals, n, nbits, kor ^ kand, out_keys, out_vals, cap, n_out);
+    std::string err = g_last_error;
+    rimu_engine_destroy(e);
+    g_last_error = err;
+    return rc;
+}
+
+int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const int64_t* chosen,
+                          int64_t n, double* diag, int64_t* num_offdiag, uint64_t* new_addr,
+                          double* offdiag_value) {
+    if (int rc = check_model(model)) return rc;
+    if (n < 0) return fail(RIMU_ERR_ARG, "negative count");
+    if (n == 0) return RIMU_OK;
+    if (!addrs || !chosen || !diag || !num_offdiag || !new_addr || !offdiag_value)
+        return fail(RIMU_ERR_ARG, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
+    const ModelDev h = to_dev(*model);
+    const uint64_t maskB = (h.B >= 64) ? ~0ull : ((1ull << h.B) - 1);
+    for (int64_t i = 0; i < n; ++i)
+        if ((addrs[i] & ~maskB) != 0 || __builtin_popcountll(addrs[i]) != h.N)
+            return fail(RIMU_ERR_ARG, "address is not an N-boson M-mode BoseFS address");
+    DevBuf<uint64_t> a, na;
+    DevBuf<int64_t> c, nod;
+    DevBuf<double> d, ov;
+    auto cleanup = [&]() { a.release(); na.release(); c.release(); nod.release(); d.release(); ov.release(); };
+    if (a.alloc(n) || na.alloc(n) || c.alloc(n) || nod.alloc(n) || d.alloc(n) || ov.alloc(n)) {
+        cleanup();
+        return fail(RIMU_ERR_HIP, "allocation failed");
+    }
+    hipError_t err = hipMemcpy(a.p, addrs, n * 8, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(c.p, chosen, n * 8, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        ham_eval_kernel<<<ceil_div(n, NT), NT>>>(h, a.p, c.p, n, d.p, nod.p, na.p, ov.p);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipMemcpy(diag, d.p, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(num_offdiag, nod.p, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(new_addr, na.p, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(offdiag_value, ov.p, n * 8, hipMemcpyDeviceToHost);
+    cleanup();
+    if (err != hipSuccess) return fail(RIMU_ERR_HIP, std::string("hamiltonian_eval: ") + hipGetErrorString(err));
+    return RIMU_OK;
+}
+
+}  // extern "C"
+
+extern "C" int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t max_spawns) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (max_spawns >= (int64_t)UINT32_MAX) return fail(RIMU_ERR_ARG, "max_spawns must be below 2^32");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (max_spawns > e->max_spawns) {
+        for (int b = 0; b < 2; ++b) {
+            HIP_TRY(e->skeys[b].alloc(max_spawns));
+            HIP_TRY(e->svals[b].alloc(max_spawns * 8));
+        }
+        e->max_spawns = max_spawns;
+    }
+    if (max_entries > e->max_entries) {
+        for (int b = 0; b < 2; ++b) {
+            DevBuf<uint64_t> nk;
+            DevBuf<unsigned char> nv;
+            HIP_TRY(nk.alloc(max_entries));
+            HIP_TRY(nv.alloc(max_entries * 8));
+            if (b == e->cur && e->len > 0) {
+                HIP_TRY(hipMemcpyAsync(nk.p, e->vkeys[b].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipMemcpyAsync(nv.p, e->vvals[b].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipStreamSynchronize(e->stream));
+            }
+            e->vkeys[b].release();
+            e->vvals[b].release();
+            e->vkeys[b] = nk;
+            e->vvals[b] = nv;
+        }
+        HIP_TRY(e->dvals.alloc(max_entries * 8));
+        HIP_TRY(e->woff.alloc(max_entries + 1));
+        e->max_entries = max_entries;
+    }
+    return RIMU_OK;
+}

@@ -1,0 +1,596 @@
+// step_kernels.hpp -- the kernels of one FCIQMC step on a sorted device vector.
+//
+//   plan     : diagonal step per entry (death/clone with stochastic rounding) and the
+//              number of spawn work items of each entry          (spawning.jl:73-93)
+//   spawn    : all spawn attempts, Philox-seeded, compacted in (entry, attempt) order
+//              with a workgroup scan and a decoupled look-back    (spawning.jl:174-243,
+//                                                                  :364-378)
+//   radix    : LSD radix sort of the spawn records on the address key, ranks staged in
+//              LDS, stable                                         (deposit!, dvec.jl:129-136)
+//   merge    : merge-path union of the sorted spawns with the diagonal survivors, a
+//              sequential left fold per address (diagonal value first, then spawns in list
+//              order), ThresholdCompression, norm and length      (compression.jl:18-33)
+//   project  : vproj / hproj of ProjectedEnergy by binary search (poststepstrategy.jl:110-121)
+#pragma once
+
+#include "bosefs_device.hpp"
+#include "device_common.hpp"
+
+namespace rimu {
+
+constexpr int NT = 256;          // workgroup size of every step kernel
+constexpr int SPAWN_CHUNK = 8;   // attempts per spawn work item
+constexpr int MERGE_IPT = 8;     // merged positions per thread
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int SORT_IPT = 16;     // keys per thread per radix tile
+constexpr int SORT_TILE = NT * SORT_IPT;
+
+enum { FLAG_SPAWN_OVERFLOW = 1, FLAG_LOOKBACK = 2, FLAG_VECTOR_OVERFLOW = 4 };
+
+struct StepArgs {
+    ModelDev h;
+    uint64_t seed;
+    int64_t step;
+    double shift, dtau, boost;
+    int style;        // 1 integer, 2 deterministic, 3 dynamic semistochastic
+    int compression;  // 0 none, 1 threshold
+    double comp_threshold, proj_threshold, rel_threshold, abs_threshold;
+};
+
+template <class V> struct is_int_v { static constexpr bool value = false; };
+template <> struct is_int_v<int64_t> { static constexpr bool value = true; };
+
+__device__ inline double sgn(double x) { return (double)((x > 0) - (x < 0)); }
+
+// projected_deposit!, spawning.jl:13-45: integers are rounded stochastically, floats are
+// projected to +-threshold when smaller than a nonzero threshold.
+template <class V>
+__device__ inline V project(double x, double thresh, double U) {
+    if constexpr (is_int_v<V>::value) {
+        return (V)sgn(x) * (V)floor(fabs(x) + U);
+    } else {
+        if (thresh > 0 && fabs(x) < thresh) return (U < fabs(x) / thresh) ? sgn(x) * thresh : 0.0;
+        return x;
+    }
+}
+
+// Spawn-side decisions of one entry: exact (all n_od neighbours) or stochastic
+// (n_att draws with replacement), spawning.jl:232-243 and :364-378.
+struct ColumnPlan {
+    int64_t n_od;
+    int64_t n_items;  // neighbours (exact) or attempts (stochastic)
+    bool exact;
+};
+
+template <class V>
+__device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
+    ColumnPlan c;
+    c.n_od = num_offdiagonals(a.h, k);
+    const double av = fabs((double)v);
+    if (a.style == 2) {
+        c.exact = true;
+    } else if (a.style == 3) {
+        const double thresh = fmin(a.abs_threshold, (double)c.n_od);
+        c.exact = (a.boost * av * a.rel_threshold) >= thresh;
+    } else {
+        c.exact = false;
+    }
+    if (c.exact) c.n_items = c.n_od;
+    else c.n_items = max((int64_t)floor(av * a.boost), (int64_t)1);
+    if (c.n_od == 0) c.n_items = 0;
+    return c;
+}
+
+// ---------------------------------------------------------------------------------------
+// plan: one thread per entry (thread L writes the trailing zero of the count array).
+// partials[tile*6 + {0..5}] = deaths, clones, zombies, exact, inexact, attempts.
+// ---------------------------------------------------------------------------------------
+template <class V>
+__global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __restrict__ keys,
+                                                  const V* __restrict__ vals, int64_t L,
+                                                  V* __restrict__ dvals, int64_t* __restrict__ wcount,
+                                                  double* __restrict__ partials) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    double st[6] = {0, 0, 0, 0, 0, 0};
+    if (i < L) {
+        const uint64_t k = keys[i];
+        const V v = vals[i];
+        // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
+        const double hd = diagonal_element(a.h, k);
+        const double d = 1 - a.dtau * (hd - a.shift);
+        const double x = d * (double)v;
+        double U = 0;
+        if (is_int_v<V>::value || a.proj_threshold > 0) {
+            const U4 r = draw(a.seed, k, 0, a.step, TAG_DIAG);
+            U = u53(r.x, r.y);
+        }
+        const V nv = project<V>(x, a.proj_threshold, U);
+        dvals[i] = nv;
+        // clones_deaths_zombies, spawning.jl:79-93
+        const double res = (double)nv, val = (double)v;
+        if (res > val) st[1] = fabs(res - val);
+        else if (sgn(res) != sgn(val)) { st[0] = fabs(val); st[2] = fabs(res); }
+        else st[0] = fabs(res - val);
+        const ColumnPlan c = plan_column<V>(a, k, v);
+        if (c.n_od > 0) {
+            if (a.style != 1) { if (c.exact) st[3] = 1; else st[4] = 1; }
+            st[5] = (double)c.n_items;
+        }
+        wcount[i] = (c.n_items + SPAWN_CHUNK - 1) / SPAWN_CHUNK;
+    } else if (i == L) {
+        wcount[i] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const double s = block_sum<NT>(st[j]);
+        if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * 6 + j] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Exclusive scan of int64 counts (three kernels; deterministic).
+// ---------------------------------------------------------------------------------------
+constexpr int SCAN_IPT = 8;
+constexpr int SCAN_TILE = NT * SCAN_IPT;
+
+template <class T>
+__global__ __launch_bounds__(NT) void scan_tiles_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                        int64_t n, T* __restrict__ tile_sums) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+    T loc[SCAN_IPT];
+    T sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; ++j) {
+        const int64_t idx = base + j;
+        loc[j] = idx < n ? in[idx] : (T)0;
+        sum += loc[j];
+    }
+    T total;
+    T run = block_exclusive_scan<T, NT>(sum, &total);
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; ++j) {
+        const int64_t idx = base + j;
+        if (idx < n) out[idx] = run;
+        run += loc[j];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void scan_sums_kernel(T* __restrict__ sums, int64_t n,
+                                                         T* __restrict__ grand_total) {
+    T carry = 0;
+    for (int64_t base = 0; base < n; base += 1024) {
+        const int64_t idx = base + threadIdx.x;
+        const T v = idx < n ? sums[idx] : (T)0;
+        T tot;
+        const T ex = block_exclusive_scan<T, 1024>(v, &tot);
+        if (idx < n) sums[idx] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && grand_total) *grand_total = carry;
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64_t n,
+                                                      const T* __restrict__ tile_sums) {
+    const T add = tile_sums[blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; ++j) {
+        const int64_t idx = base + (int64_t)j * NT + threadIdx.x;
+        if (idx < n) out[idx] += add;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// spawn: one thread per work item (entry, chunk of SPAWN_CHUNK attempts).  Records are
+// staged per thread in LDS, compacted in (thread, attempt) order, placed after all earlier
+// tiles by the look-back, and written with consecutive lanes on consecutive records.
+// Also reduces the key OR/AND (which radix digits vary) and sum |spawn| per tile.
+// ---------------------------------------------------------------------------------------
+template <class V>
+__global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* __restrict__ keys,
+                                                   const V* __restrict__ vals, int64_t L,
+                                                   const int64_t* __restrict__ woff, int64_t W,
+                                                   uint64_t* __restrict__ out_keys,
+                                                   V* __restrict__ out_vals, int64_t cap,
+                                                   uint64_t* status, uint32_t* tile_counter,
+                                                   int64_t* total_out, int* flags,
+                                                   double* __restrict__ partials,
+                                                   unsigned long long* key_or,
+                                                   unsigned long long* key_and) {
+    __shared__ uint64_t s_k[NT * SPAWN_CHUNK];
+    __shared__ V s_v[NT * SPAWN_CHUNK];
+    __shared__ int s_off[NT + 1];
+    __shared__ uint64_t s_base;
+
+    const uint32_t tile = next_tile_id(tile_counter);
+    const int64_t w = (int64_t)tile * NT + threadIdx.x;
+    int count = 0;
+    double spawned = 0;
+    uint64_t kor = 0, kand = ~0ull;
+    if (w < W) {
+        // entry of this work item: last i with woff[i] <= w
+        int64_t lo = 0, hi = L;  // woff[L] = W > w
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (woff[mid] <= w) lo = mid; else hi = mid;
+        }
+        const int64_t i = lo;
+        const int64_t chunk = w - woff[i];
+        const uint64_t k = keys[i];
+        const V v = vals[i];
+        const ColumnPlan c = plan_column<V>(a, k, v);
+        const double magnitude = (double)v / (double)c.n_items;
+        const double prob = 1.0 / (double)c.n_od;
+        const int64_t j0 = chunk * SPAWN_CHUNK;
+        const int64_t j1 = min(j0 + (int64_t)SPAWN_CHUNK, c.n_items);
+        const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
+        for (int64_t j = j0; j < j1; ++j) {
+            int64_t chosen;
+            U4 r = {0, 0, 0, 0};
+            if (!c.exact || need_u) r = draw(a.seed, k, (uint32_t)j, a.step, TAG_SPAWN);
+            chosen = c.exact ? j + 1 : pick1(r.x, c.n_od);
+            double hv;
+            const uint64_t nb = get_offdiagonal(a.h, k, chosen, &hv);
+            const double mel = -hv * a.dtau;  // FirstOrderOffdiagonals, fciqmc.jl:109-112
+            const double x = c.exact ? (double)v * mel : mel * magnitude / prob;
+            const V nv = project<V>(x, a.proj_threshold, need_u ? u53(r.y, r.z) : 0.0);
+            spawned += fabs((double)nv);
+            if (nv != 0) {
+                s_k[threadIdx.x * SPAWN_CHUNK + count] = nb;
+                s_v[threadIdx.x * SPAWN_CHUNK + count] = nv;
+                ++count;
+                kor |= nb;
+                kand &= nb;
+            }
+        }
+    }
+    int total;
+    const int ex = block_exclusive_scan<int, NT>(count, &total);
+    s_off[threadIdx.x] = ex;
+    if (threadIdx.x == 0) {
+        s_off[NT] = total;
+        s_base = lookback_exclusive(status, tile, (uint64_t)total, flags);
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)s_base;
+    if (base + total > cap) {
+        if (threadIdx.x == 0) atomicOr(flags, FLAG_SPAWN_OVERFLOW);
+    } else {
+        for (int idx = threadIdx.x; idx < total; idx += NT) {
+            int lo = 0, hi = NT;  // s_off[lo] <= idx < s_off[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= idx) lo = mid; else hi = mid;
+            }
+            const int slot = lo * SPAWN_CHUNK + (idx - s_off[lo]);
+            out_keys[base + idx] = s_k[slot];
+            out_vals[base + idx] = s_v[slot];
+        }
+    }
+    const double s = block_sum<NT>(spawned);
+    if (threadIdx.x == 0) {
+        partials[tile] = s;
+        if (tile == gridDim.x - 1) *total_out = base + total;
+    }
+    // key OR/AND for the radix pass selection (bitwise: order independent)
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        kor |= __shfl_xor(kor, off, WAVE);
+        kand &= __shfl_xor(kand, off, WAVE);
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        atomicOr(key_or, (unsigned long long)kor);
+        atomicAnd(key_and, (unsigned long long)kand);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Radix sort pass: count (per-tile digit histogram, digit-major) and scatter (stable
+// ranks within the tile from wavefront ballots, staged in LDS in digit order, then
+// written to the global digit offsets).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                         int shift, uint32_t* __restrict__ counts,
+                                                         int64_t num_tiles) {
+    __shared__ uint32_t hist[RADIX];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+#pragma unroll
+    for (int r = 0; r < SORT_IPT; ++r) {
+        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        if (idx < n) atomicAdd(&hist[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    counts[(int64_t)threadIdx.x * num_tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <class V>
+__global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __restrict__ keys_in,
+                                                           const V* __restrict__ vals_in, int64_t n,
+                                                           int shift, const uint32_t* __restrict__ offs,
+                                                           int64_t num_tiles,
+                                                           uint64_t* __restrict__ keys_out,
+                                                           V* __restrict__ vals_out) {
+    __shared__ uint64_t s_k[SORT_TILE];
+    __shared__ V s_v[SORT_TILE];
+    __shared__ uint32_t s_wcnt[NT / WAVE][RADIX];
+    __shared__ uint32_t s_run[RADIX];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+
+    s_run[threadIdx.x] = 0;
+    s_goff[threadIdx.x] = offs[(int64_t)threadIdx.x * num_tiles + blockIdx.x];
+
+    uint64_t k[SORT_IPT];
+    V v[SORT_IPT];
+    uint32_t rank[SORT_IPT];
+#pragma unroll
+    for (int r = 0; r < SORT_IPT; ++r) {
+        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        const bool valid = idx < n;
+        k[r] = valid ? keys_in[idx] : 0ull;
+        v[r] = valid ? vals_in[idx] : (V)0;
+        const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
+        // peers: lanes of this wavefront holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; ++b) {
+            const uint64_t bb = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+        }
+        for (int j = threadIdx.x; j < (NT / WAVE) * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
+        __syncthreads();
+        const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && in_wave == 0) s_wcnt[wid][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t before = s_run[digit];
+            for (int w2 = 0; w2 < wid; ++w2) before += s_wcnt[w2][digit];
+            rank[r] = before + in_wave;
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+#pragma unroll
+            for (int w2 = 0; w2 < NT / WAVE; ++w2) add += s_wcnt[w2][threadIdx.x];
+            s_run[threadIdx.x] += add;
+        }
+        __syncthreads();
+    }
+    // tile-local digit starts
+    uint32_t tot;
+    const uint32_t start = block_exclusive_scan<uint32_t, NT>(s_run[threadIdx.x], &tot);
+    s_start[threadIdx.x] = start;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_IPT; ++r) {
+        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        if (idx < n) {
+            const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
+            const uint32_t pos = s_start[digit] + rank[r];
+            s_k[pos] = k[r];
+            s_v[pos] = v[r];
+        }
+    }
+    __syncthreads();
+    const int64_t cnt = min((int64_t)SORT_TILE, n - base);
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+        const uint64_t kk = s_k[i];
+        const uint32_t digit = (uint32_t)((kk >> shift) & (RADIX - 1));
+        const int64_t g = (int64_t)s_goff[digit] + (i - (int64_t)s_start[digit]);
+        keys_out[g] = kk;
+        vals_out[g] = s_v[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// merge: union of A (sorted spawns, duplicates) and B (vector keys with diagonal values,
+// unique).  Merged order: by key, B before A on equal keys.  Each address is one segment;
+// the thread owning the segment's first position folds it sequentially:
+//   acc = 0 (+ diag) + spawn_1 + spawn_2 + ...
+// then applies ThresholdCompression and emits the survivor.  Outputs are compacted in
+// merged order by the look-back.  partials[tile*3 + {0,1,2}] = len_before, len, norm.
+// ---------------------------------------------------------------------------------------
+template <class V>
+__global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
+                                                   const V* __restrict__ av, int64_t S,
+                                                   const uint64_t* __restrict__ bk,
+                                                   const V* __restrict__ bv, int64_t L,
+                                                   uint64_t* __restrict__ ok, V* __restrict__ ov,
+                                                   int64_t cap, uint64_t* status,
+                                                   uint32_t* tile_counter, int64_t* total_out,
+                                                   int* flags, double* __restrict__ partials) {
+    constexpr int TILE = NT * MERGE_IPT;
+    __shared__ uint64_t s_a[TILE];
+    __shared__ uint64_t s_b[TILE];
+    __shared__ int64_t s_split[4];  // a0, b0, a1, b1
+    __shared__ uint64_t s_base;
+
+    const uint32_t tile = next_tile_id(tile_counter);
+    const int64_t total = S + L;
+    const int64_t d0 = (int64_t)tile * TILE;
+    const int64_t d1 = min(d0 + TILE, total);
+
+    // merge-path split of a diagonal d: number of A elements among the first d
+    auto split = [&](int64_t d) -> int64_t {
+        int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (ak[mid] < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    if (threadIdx.x == 0) { const int64_t x = split(d0); s_split[0] = x; s_split[1] = d0 - x; }
+    if (threadIdx.x == 1) { const int64_t x = split(d1); s_split[2] = x; s_split[3] = d1 - x; }
+    __syncthreads();
+    const int64_t a0 = s_split[0], b0 = s_split[1], a1 = s_split[2], b1 = s_split[3];
+    const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+    for (int i = threadIdx.x; i < na; i += NT) s_a[i] = ak[a0 + i];
+    for (int i = threadIdx.x; i < nb; i += NT) s_b[i] = bk[b0 + i];
+    __syncthreads();
+
+    // this thread's positions within the tile
+    const int dd = (int)threadIdx.x * MERGE_IPT;
+    const int n_here = (int)max((int64_t)0, min((int64_t)MERGE_IPT, (d1 - d0) - dd));
+    int ia = 0;
+    {
+        int lo = max(0, dd - nb), hi = min(dd, na);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        ia = lo;
+    }
+    int ib = dd - ia;
+    // key of the merged element just before this thread's first position
+    bool have_prev = false;
+    uint64_t prev = 0;
+    if (dd > 0) {
+        have_prev = true;
+        const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0;
+        const uint64_t pb = ib > 0 ? s_b[ib - 1] : 0;
+        prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
+    } else if (d0 > 0) {
+        have_prev = true;
+        const uint64_t pa = a0 > 0 ? ak[a0 - 1] : 0;
+        const uint64_t pb = b0 > 0 ? bk[b0 - 1] : 0;
+        prev = (a0 > 0 && b0 > 0) ? max(pa, pb) : (a0 > 0 ? pa : pb);
+    }
+
+    uint64_t out_k[MERGE_IPT];
+    V out_v[MERGE_IPT];
+    int count = 0;
+    double lb = 0, ln = 0, nrm = 0;
+#pragma unroll
+    for (int j = 0; j < MERGE_IPT; ++j) {
+        if (j < n_here) {
+            const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
+            const uint64_t key = take_b ? s_b[ib] : s_a[ia];
+            if (!have_prev || key != prev) {
+                // segment head: fold diag (if any) then the run of spawns in A
+                V acc = (V)0;
+                int64_t ga = a0 + ia;
+                if (take_b) acc = acc + bv[b0 + ib];
+                while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
+                if (acc != (V)0) {
+                    lb += 1;
+                    if constexpr (!is_int_v<V>::value) {
+                        if (a.compression == 1) {
+                            const double th = a.comp_threshold;
+                            const double prob = fabs((double)acc) / th;
+                            if (prob < 1) {
+                                const U4 r = draw(a.seed, key, 0, a.step, TAG_COMPRESS);
+                                acc = (prob > u53(r.x, r.y)) ? th * sgn((double)acc) : 0.0;
+                            }
+                        }
+                    }
+                    if (acc != (V)0) {
+                        out_k[count] = key;
+                        out_v[count] = acc;
+                        ++count;
+                        ln += 1;
+                        nrm += fabs((double)acc);
+                    }
+                }
+            }
+            prev = key;
+            have_prev = true;
+            if (take_b) ++ib; else ++ia;
+        }
+    }
+    int btotal;
+    const int ex = block_exclusive_scan<int, NT>(count, &btotal);
+    if (threadIdx.x == 0) s_base = lookback_exclusive(status, tile, (uint64_t)btotal, flags);
+    __syncthreads();
+    const int64_t base = (int64_t)s_base;
+    if (base + btotal > cap) {
+        if (threadIdx.x == 0) atomicOr(flags, FLAG_VECTOR_OVERFLOW);
+    } else {
+#pragma unroll
+        for (int j = 0; j < MERGE_IPT; ++j) {
+            if (j < count) {
+                ok[base + ex + j] = out_k[j];
+                ov[base + ex + j] = out_v[j];
+            }
+        }
+    }
+    const double s0 = block_sum<NT>(lb);
+    const double s1 = block_sum<NT>(ln);
+    const double s2 = block_sum<NT>(nrm);
+    if (threadIdx.x == 0) {
+        partials[(int64_t)tile * 3 + 0] = s0;
+        partials[(int64_t)tile * 3 + 1] = s1;
+        partials[(int64_t)tile * 3 + 2] = s2;
+        if (tile == gridDim.x - 1) *total_out = base + btotal;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// project: terms[i] = coef_i * v[key_i] (0 if absent) by binary search in the new vector.
+// ---------------------------------------------------------------------------------------
+template <class V>
+__global__ __launch_bounds__(NT) void project_kernel(const uint64_t* __restrict__ keys,
+                                                     const V* __restrict__ vals, const int64_t* len_p,
+                                                     const uint64_t* __restrict__ pk,
+                                                     const double* __restrict__ pc, int64_t np,
+                                                     double* __restrict__ terms) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= np) return;
+    const int64_t L = *len_p;
+    const uint64_t key = pk[i];
+    int64_t lo = 0, hi = L;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    const double x = (lo < L && keys[lo] == key) ? (double)vals[lo] : 0.0;
+    terms[i] = x * pc[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// finalize: sums the per-tile partials in tile order into the step statistics.
+// out: [0] attempts [1] exact [2] inexact [3] spawns [4] deaths [5] clones [6] zombies
+//      [7] len_before [8] len [9] norm [10] vproj [11] hproj
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void finalize_kernel(const double* __restrict__ plan_p, int64_t n_plan,
+                                                      const double* __restrict__ spawn_p, int64_t n_spawn,
+                                                      const double* __restrict__ merge_p, int64_t n_merge,
+                                                      const double* __restrict__ terms, int64_t np,
+                                                      int64_t nh, double* __restrict__ out) {
+    double acc[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) acc[j] = 0;
+    for (int64_t t = threadIdx.x; t < n_plan; t += NT) {
+        acc[4] += plan_p[t * 6 + 0];
+        acc[5] += plan_p[t * 6 + 1];
+        acc[6] += plan_p[t * 6 + 2];
+        acc[1] += plan_p[t * 6 + 3];
+        acc[2] += plan_p[t * 6 + 4];
+        acc[0] += plan_p[t * 6 + 5];
+    }
+    for (int64_t t = threadIdx.x; t < n_spawn; t += NT) acc[3] += spawn_p[t];
+    for (int64_t t = threadIdx.x; t < n_merge; t += NT) {
+        acc[7] += merge_p[t * 3 + 0];
+        acc[8] += merge_p[t * 3 + 1];
+        acc[9] += merge_p[t * 3 + 2];
+    }
+    for (int64_t t = threadIdx.x; t < np; t += NT) acc[10] += terms[t];
+    for (int64_t t = threadIdx.x; t < nh; t += NT) acc[11] += terms[np + t];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        const double s = block_sum<NT>(acc[j]);
+        if (threadIdx.x == 0) out[j] = s;
+    }
+}
+
+}  // namespace rimu

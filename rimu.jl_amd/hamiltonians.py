@@ -1,0 +1,162 @@
+"""``HubbardReal1D`` and ``HubbardMom1D`` host descriptors with the Hamiltonian interface
+(``diagonal_element``, ``num_offdiagonals``, ``get_offdiagonal``, ``offdiagonals``,
+``starting_address``, ``dimension``) evaluated on the device by ``rimu_hamiltonian_eval``.
+
+References: ``src/Hamiltonians/HubbardReal1D.jl:22-62``, ``src/Hamiltonians/HubbardMom1D.jl:1-65``
+(momentum grid and dispersion), ``src/Interfaces/hamiltonians.jl`` (interface).
+"""
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+import numpy as np
+
+from . import _native
+from .addresses import BoseFS
+
+
+class AbstractHamiltonian:
+    address: BoseFS
+    u: float
+    t: float
+
+    def starting_address(self) -> BoseFS:
+        return self.address
+
+    @property
+    def M(self) -> int:
+        return self.address.M
+
+    @property
+    def N(self) -> int:
+        return self.address.N
+
+    def dimension(self) -> int:
+        """``number_conserving_bose_dimension`` (Hamiltonians/abstract.jl:110)."""
+        return math.comb(self.N + self.M - 1, self.N)
+
+    def descriptor(self) -> _native.Model:
+        raise NotImplementedError
+
+    def _check(self, addr: BoseFS):
+        if addr.M != self.M or addr.N != self.N:
+            raise ValueError("address does not match the Hamiltonian's N and M")
+
+    # Interface -------------------------------------------------------------------------
+    def diagonal_element(self, addr: BoseFS) -> float:
+        self._check(addr)
+        d, _, _, _ = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [0])
+        return float(d[0])
+
+    def num_offdiagonals(self, addr: BoseFS) -> int:
+        self._check(addr)
+        _, n, _, _ = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [0])
+        return int(n[0])
+
+    def get_offdiagonal(self, addr: BoseFS, chosen: int):
+        self._check(addr)
+        _, n, na, ov = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [chosen])
+        if not 1 <= chosen <= n[0]:
+            raise IndexError(f"chosen={chosen} outside 1:{n[0]}")
+        return BoseFS(M=self.M, N=self.N, bits=int(na[0])), float(ov[0])
+
+    def offdiagonals(self, addr: BoseFS):
+        self._check(addr)
+        n = self.num_offdiagonals(addr)
+        if n == 0:
+            return []
+        _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), [addr.bits] * n,
+                                                list(range(1, n + 1)))
+        return [(BoseFS(M=self.M, N=self.N, bits=int(a)), float(v)) for a, v in zip(na, ov)]
+
+    def column(self, addrs_bits):
+        """Diagonal elements and all off-diagonal (row, value) pairs of the columns of a batch
+        of addresses (given as bit patterns): returns (diag, rows, cols, vals)."""
+        addrs_bits = np.asarray(addrs_bits, dtype=np.uint64)
+        d, nod, _, _ = _native.hamiltonian_eval(self.descriptor(), addrs_bits,
+                                                np.zeros(len(addrs_bits), dtype=np.int64))
+        cols = np.repeat(np.arange(len(addrs_bits)), nod)
+        chosen = np.concatenate([np.arange(1, n + 1) for n in nod]) if len(nod) else np.zeros(0, np.int64)
+        if len(cols) == 0:
+            return d, np.zeros(0, np.uint64), cols, np.zeros(0)
+        _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), addrs_bits[cols], chosen)
+        return d, na, cols, ov
+
+
+class HubbardReal1D(AbstractHamiltonian):
+    """``HubbardReal1D(address; u=1.0, t=1.0)``, periodic chain (HubbardReal1D.jl:22-29)."""
+
+    def __init__(self, address: BoseFS, u: float = 1.0, t: float = 1.0):
+        if address.M < 2:
+            raise ValueError("HubbardReal1D needs at least two sites")
+        self.address = address
+        self.u = float(u)
+        self.t = float(t)
+
+    def descriptor(self) -> _native.Model:
+        m = _native.Model()
+        m.kind, m.M, m.N, m.u, m.t = _native.MODEL_REAL1D, self.M, self.N, self.u, self.t
+        return m
+
+    def __repr__(self):
+        return f"HubbardReal1D({self.address!r}; u={self.u}, t={self.t})"
+
+    def with_params(self, u=None, t=None) -> "HubbardReal1D":
+        return HubbardReal1D(self.address, self.u if u is None else u, self.t if t is None else t)
+
+
+def hubbard_dispersion(t: float, k: float) -> float:
+    """``hubbard_dispersion`` (HubbardMom1D.jl:7): -2t cos(k)."""
+    return (-2 * t) * math.cos(k)
+
+
+class HubbardMom1D(AbstractHamiltonian):
+    """``HubbardMom1D(address; u=1.0, t=1.0)`` (HubbardMom1D.jl:49-65)."""
+
+    def __init__(self, address: BoseFS, u: float = 1.0, t: float = 1.0, dispersion=hubbard_dispersion):
+        M = address.M
+        if M < 3:
+            raise ValueError("HubbardMom1D needs at least three modes")
+        self.address = address
+        self.u = float(u)
+        self.t = float(t)
+        step = 2 * math.pi / M
+        start = (-math.pi * (1 + 1 / M) + step) if M % 2 == 1 else (-math.pi + step)
+        # Julia's range(start; step, length=M) for Float64: correctly rounded start + j*step
+        self.ks = np.array([float(Fraction(start) + j * Fraction(step)) for j in range(M)])
+        self.kes = np.array([dispersion(self.t, k) for k in self.ks])
+
+    def descriptor(self) -> _native.Model:
+        m = _native.Model()
+        m.kind, m.M, m.N, m.u, m.t = _native.MODEL_MOM1D, self.M, self.N, self.u, self.t
+        for j, e in enumerate(self.kes):
+            m.kes[j] = float(e)
+        return m
+
+    def __repr__(self):
+        return f"HubbardMom1D({self.address!r}; u={self.u}, t={self.t})"
+
+
+def diagonal_element(h: AbstractHamiltonian, addr: BoseFS) -> float:
+    return h.diagonal_element(addr)
+
+
+def num_offdiagonals(h: AbstractHamiltonian, addr: BoseFS) -> int:
+    return h.num_offdiagonals(addr)
+
+
+def get_offdiagonal(h: AbstractHamiltonian, addr: BoseFS, chosen: int):
+    return h.get_offdiagonal(addr, chosen)
+
+
+def offdiagonals(h: AbstractHamiltonian, addr: BoseFS):
+    return h.offdiagonals(addr)
+
+
+def starting_address(h: AbstractHamiltonian) -> BoseFS:
+    return h.starting_address()
+
+
+def dimension(h: AbstractHamiltonian) -> int:
+    return h.dimension()

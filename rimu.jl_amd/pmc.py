@@ -1,0 +1,276 @@
+"""``ProjectorMonteCarloProblem`` / ``init`` / ``solve`` / ``step!`` / ``lomc!`` over the
+device engine.
+
+The host keeps Rimu's step loop (``advance!``, ``src/fciqmc.jl:126-181``; ``step!`` and
+``solve!``, ``src/pmc_simulation.jl:265-310``, ``:350-452``), the shift strategy and the
+report; one ``rimu_step`` call replaces ``apply_operator!`` + ``walkernumber_and_length``
++ the ProjectedEnergy post-step.  Keyword defaults follow
+``src/projector_monte_carlo_problem.jl:150-290`` and ``src/lomc.jl:95-195``.
+"""
+from __future__ import annotations
+
+import math
+import secrets
+import time
+
+import numpy as np
+
+from . import _native
+from .addresses import BoseFS
+from .dictvectors import DVec, DeviceDVec
+from .hamiltonians import AbstractHamiltonian
+from .strategies import (ConstantTimeStep, DefaultShiftParameters, DoubleLogUpdate,
+                         ProjectedEnergy)
+from .styles import IsDynamicSemistochastic, IsStochasticInteger, StochasticStyle
+
+_seed_counter = [0x5EED_0000_0000]
+
+
+class ProjectorMonteCarloProblem:
+    def __init__(self, hamiltonian: AbstractHamiltonian, *, start_at=None, shift=None,
+                 style: StochasticStyle | None = None, initiator=False, threading=None,
+                 time_step: float = 0.01, starting_step: int = 0, last_step: int = 100,
+                 wall_time: float = math.inf, target_walkers: float = 1000, zeta: float = 0.08,
+                 xi: float | None = None, shift_strategy=None, time_step_strategy=None,
+                 post_step_strategy=(), max_length: int | None = None, metadata=None,
+                 display_name: str = "PMCSimulation", random_seed=True, n_replicas: int = 1,
+                 reporting_interval: int = 1):
+        if initiator not in (False, None):
+            raise NotImplementedError("initiator rules are not part of this engine")
+        if n_replicas != 1:
+            raise NotImplementedError("one replica per problem")
+        self.hamiltonian = hamiltonian
+        self.start_at = hamiltonian.starting_address() if start_at is None else start_at
+        self.style = IsDynamicSemistochastic() if style is None else style
+        self.time_step = float(time_step)
+        self.shift = shift
+        self.starting_step = int(starting_step)
+        self.last_step = int(last_step)
+        self.wall_time = float(wall_time)
+        if shift_strategy is None:
+            shift_strategy = DoubleLogUpdate(target_walkers=target_walkers, zeta=zeta, xi=xi)
+        self.shift_strategy = shift_strategy
+        self.time_step_strategy = time_step_strategy or ConstantTimeStep()
+        if isinstance(post_step_strategy, ProjectedEnergy):
+            post_step_strategy = (post_step_strategy,)
+        self.post_step_strategy = tuple(post_step_strategy)
+        if max_length is None:
+            tw = getattr(shift_strategy, "target_walkers", target_walkers)
+            max_length = round(2 * abs(tw) + 100)
+        self.max_length = int(max_length)
+        self.metadata = dict(metadata or {})
+        self.metadata["display_name"] = display_name
+        if random_seed is True:
+            random_seed = secrets.randbits(64)
+        elif random_seed is False:
+            random_seed = None
+        self.random_seed = None if random_seed is None else int(random_seed) & (2 ** 64 - 1)
+        self.reporting_interval = int(reporting_interval)
+
+    def __repr__(self):
+        return (f"ProjectorMonteCarloProblem(hamiltonian={self.hamiltonian}, style={self.style}, "
+                f"time_step={self.time_step}, last_step={self.last_step}, "
+                f"shift_strategy={self.shift_strategy}, max_length={self.max_length})")
+
+
+def default_starting_vector(address: BoseFS, style=None, population=10) -> DVec:
+    """``default_starting_vector`` (qmc_states.jl:229-260): address => population."""
+    style = style or IsDynamicSemistochastic()
+    return DVec((address, population), style=style)
+
+
+def _start_vector(problem: ProjectorMonteCarloProblem) -> DVec:
+    s = problem.start_at
+    if isinstance(s, BoseFS):
+        return default_starting_vector(s, problem.style)
+    if isinstance(s, DVec):
+        return DVec(dict(s.storage), style=problem.style)
+    raise TypeError("start_at must be a BoseFS or a DVec")
+
+
+def rayleigh_quotient_dvec(h: AbstractHamiltonian, v: DVec) -> float:
+    """dot(v, H, v) / dot(v, v) (fciqmc.jl:51-61, abstractdvec.jl:313-324)."""
+    items = list(v.pairs())
+    keys = np.array([k.bits for k, _ in items], dtype=np.uint64)
+    vals = np.array([float(x) for _, x in items])
+    diag, rows, cols, offv = h.column(keys)
+    index = {int(k): i for i, k in enumerate(keys.tolist())}
+    num = float(np.dot(vals * diag, vals))
+    for r, c, x in zip(rows.tolist(), cols.tolist(), offv.tolist()):
+        i = index.get(int(r))
+        if i is not None:
+            num += vals[i] * x * vals[c]
+    return num / float(np.dot(vals, vals))
+
+
+class PMCSimulation:
+    """State, report and status of a run (pmc_simulation.jl:22-31)."""
+
+    def __init__(self, problem: ProjectorMonteCarloProblem):
+        self.problem = problem
+        h = problem.hamiltonian
+        v0 = _start_vector(problem)
+        if problem.random_seed is not None:
+            seed = problem.random_seed
+        else:
+            _seed_counter[0] += 0x9E3779B97F4A7C15
+            seed = _seed_counter[0] & (2 ** 64 - 1)
+        self.seed = seed
+        tw = abs(getattr(problem.shift_strategy, "target_walkers", 1000))
+        max_entries = problem.max_length + 1
+        max_spawns = int(min(max(4 * tw + 4096, 4 * len(v0) + 4096), 2 ** 32 - 2))
+        self.engine = _native.Engine(h.descriptor(), problem.style.descriptor(), seed,
+                                     max_entries, max_spawns)
+        keys, vals = v0.to_arrays()
+        self.engine.upload(keys, vals)
+        self.v = DeviceDVec(self.engine, h, problem.style)
+        # initial shift: Rayleigh quotient of the start vector unless given
+        shift = problem.shift
+        if shift is None:
+            shift = rayleigh_quotient_dvec(h, v0)
+        self.shift_parameters = DefaultShiftParameters(float(shift), v0.walkernumber(),
+                                                       problem.time_step)
+        # projections (only ProjectedEnergy is evaluated on the device)
+        self.projected = [p for p in problem.post_step_strategy if isinstance(p, ProjectedEnergy)]
+        if len(self.projected) > 1:
+            raise NotImplementedError("one ProjectedEnergy per problem")
+        if self.projected:
+            p = self.projected[0]
+            self.engine.set_projector(p.vproj.keys_bits(), p.vproj.coefs(),
+                                      p.hproj.keys_bits(), p.hproj.coefs())
+        self.step = problem.starting_step
+        self.last_step = problem.last_step
+        self.wall_time = problem.wall_time
+        self.columns = {}
+        self.modified = False
+        self.aborted = False
+        self.success = False
+        self.message = ""
+        self.elapsed_time = 0.0
+
+    # report ---------------------------------------------------------------------------
+    def _report(self, name, value):
+        self.columns.setdefault(name, []).append(value)
+
+    @property
+    def df(self):
+        import pandas as pd
+        n = max((len(c) for c in self.columns.values()), default=0)
+        cols = {k: v for k, v in self.columns.items() if len(v) == n}
+        df = pd.DataFrame(cols)
+        df.attrs.update(self.problem.metadata)
+        df.attrs.update({"aborted": self.aborted, "success": self.success,
+                         "message": self.message, "elapsed_time": self.elapsed_time})
+        return df
+
+    def DataFrame(self):
+        return self.df
+
+    # stepping -------------------------------------------------------------------------
+    def advance(self) -> bool:
+        """``advance!`` (fciqmc.jl:126-181)."""
+        p = self.problem
+        sp = self.shift_parameters
+        st = None
+        try:
+            st = self.engine.step(self.step, sp.shift, sp.time_step)
+        except _native.CapacityError:
+            self.message_detail = "`max_length` reached. Aborting."
+            return False
+        tnorm, length = st.norm, int(st.len)
+        self.v._norm = tnorm
+        shift_stats, proceed = p.shift_strategy.update(sp, tnorm)
+        if self.step % p.reporting_interval == 0:
+            self._report("len", length)
+            for k, val in shift_stats.items():
+                self._report(k, val)
+            for name, val in zip(p.style.step_stat_names(), p.style.stat_values(st)):
+                self._report(name, val)
+            for pe in self.projected:
+                self._report(pe.vproj_name, st.vproj)
+                self._report(pe.hproj_name, st.hproj)
+        self.last_stats = st
+        if length == 0:
+            self.message_detail = "Population is dead. Aborting."
+            return False
+        if length > p.max_length:
+            self.message_detail = "`max_length` reached. Aborting."
+            return False
+        return proceed
+
+    def step_(self):
+        """``step!`` (pmc_simulation.jl:265-310)."""
+        if self.aborted or self.success or self.step >= self.last_step:
+            return self
+        self.step += 1
+        if self.step % self.problem.reporting_interval == 0:
+            self._report("step", self.step)
+        proceed = self.advance()
+        self.modified = True
+        if not proceed:
+            self.aborted = True
+            self.message = f"Aborted in step {self.step}."
+            return self
+        if self.step == self.last_step:
+            self.success = True
+        return self
+
+    def solve_(self, last_step=None, wall_time=None):
+        """``solve!`` (pmc_simulation.jl:350-452)."""
+        if last_step is not None:
+            self.last_step = int(last_step)
+            self.aborted = self.success = False
+        if wall_time is not None:
+            self.wall_time = float(wall_time)
+            self.aborted = self.success = False
+        t0 = time.time() - self.elapsed_time
+        while not self.aborted and not self.success:
+            if self.step >= self.last_step:
+                self.success = True
+                break
+            if time.time() - t0 > self.wall_time:
+                self.aborted = True
+                self.message = "Wall time limit reached."
+                break
+            self.step_()
+        self.elapsed_time = time.time() - t0
+        return self
+
+    @property
+    def state_vector(self) -> DeviceDVec:
+        return self.v
+
+    def close(self):
+        self.engine.close()
+
+
+def init(problem: ProjectorMonteCarloProblem) -> PMCSimulation:
+    return PMCSimulation(problem)
+
+
+def solve(problem: ProjectorMonteCarloProblem) -> PMCSimulation:
+    return init(problem).solve_()
+
+
+def lomc(ham: AbstractHamiltonian, v=None, *, style=None, targetwalkers=1000, s_strat=None,
+         laststep=100, dtau=0.01, shift=None, address=None, post_step=(), maxlength=None,
+         step=0, random_seed=None):
+    """``lomc!`` (lomc.jl:95-195): default style ``IsStochasticInteger``, default shift the
+    diagonal element of the starting address, no re-seeding.  Returns (df, simulation)."""
+    address = ham.starting_address() if address is None else address
+    if v is None:
+        v = default_starting_vector(address, style or IsStochasticInteger())
+    style = v.style if style is None else style
+    if s_strat is None:
+        s_strat = DoubleLogUpdate(target_walkers=targetwalkers)
+    if shift is None:
+        shift = ham.diagonal_element(address)
+    tw = getattr(s_strat, "target_walkers", targetwalkers)
+    maxlength = round(2 * abs(tw) + 100) if maxlength is None else maxlength
+    p = ProjectorMonteCarloProblem(ham, start_at=v, style=style, shift=shift, time_step=dtau,
+                                   starting_step=step, last_step=laststep, shift_strategy=s_strat,
+                                   post_step_strategy=post_step, max_length=maxlength,
+                                   random_seed=False if random_seed is None else random_seed,
+                                   display_name="lomc!")
+    sim = init(p).solve_()
+    return sim.df, sim

@@ -1,0 +1,184 @@
+"""GPU parity of the step path against the CPU oracle (through the C ABI).
+
+Bars: integer values and addresses bit-exact; floating values bit-exact where the oracle
+replays the engine's fold order (list order: diagonal value first, then spawns in
+(entry, attempt) order), and within 1e-12 relative of the reference column order.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def random_vector(M, N, n, rng, kind, scale=5.0):
+    basis = O.build_basis(M, N)
+    n = min(n, len(basis))
+    keys = np.sort(rng.choice(basis, n, replace=False))
+    if kind == "int":
+        vals = rng.integers(-int(scale), int(scale) + 1, size=n).astype(np.int64)
+        vals[vals == 0] = 1
+    else:
+        vals = rng.normal(scale=scale, size=n)
+    return keys, vals
+
+
+def oracle_model(kind, M, N, u, t):
+    return O.real1d(M, N, u, t) if kind == "real" else O.mom1d(M, N, u, t)
+
+
+def device_model(rimu, kind, M, N, u, t):
+    addr = rimu.near_uniform(N, M)
+    return (rimu.HubbardReal1D(addr, u=u, t=t) if kind == "real"
+            else rimu.HubbardMom1D(addr, u=u, t=t))
+
+
+@pytest.mark.parametrize("kind,M,N", [("real", 6, 6), ("real", 10, 10), ("real", 20, 20),
+                                      ("mom", 6, 6), ("mom", 16, 16), ("mom", 9, 5)])
+def test_hamiltonian_elements(gpu, kind, M, N):
+    rng = np.random.default_rng(M * 100 + N)
+    h = device_model(gpu, kind, M, N, 4.0, 1.3)
+    om = oracle_model(kind, M, N, 4.0, 1.3)
+    if M + N - 1 <= 23:
+        addrs = O.build_basis(M, N)
+        addrs = addrs[rng.permutation(len(addrs))[:300]]
+    else:  # random compositions
+        addrs = []
+        for _ in range(300):
+            cuts = np.sort(rng.choice(N + M - 1, M - 1, replace=False))
+            parts = np.diff(np.concatenate([[-1], cuts, [N + M - 1]])) - 1
+            addrs.append(O.from_onr(parts))
+        addrs = np.array(addrs, dtype=np.uint64)
+    diag, nod, _, _ = gpu._native.hamiltonian_eval(h.descriptor(), addrs, np.zeros(len(addrs), np.int64))
+    rows, ch = [], []
+    for a, n in zip(addrs.tolist(), nod.tolist()):
+        assert n == O.num_offdiagonals(om, a)
+        rows += [a] * n
+        ch += list(range(1, n + 1))
+    for a, d in zip(addrs.tolist(), diag.tolist()):
+        assert d == O.diagonal_element(om, a)
+    _, _, na, ov = gpu._native.hamiltonian_eval(h.descriptor(), np.array(rows, np.uint64), np.array(ch))
+    for a, c, b, v in zip(rows, ch, na.tolist(), ov.tolist()):
+        ob, oval = O.get_offdiagonal(om, a, c)
+        assert b == ob and v == oval, (a, c)
+
+
+@pytest.mark.parametrize("vtype", ["int", "float"])
+@pytest.mark.parametrize("n,nkeys", [(0, 1), (1, 1), (7, 3), (5000, 300), (200_000, 50_000),
+                                     (300_000, 7)])
+def test_annihilate_bit_exact(gpu, vtype, n, nkeys):
+    rng = np.random.default_rng(n + nkeys)
+    pool = rng.choice(2 ** 39, nkeys, replace=False).astype(np.uint64)
+    keys = pool[rng.integers(0, nkeys, size=n)]
+    if vtype == "int":
+        vals = rng.integers(-3, 4, size=n).astype(np.int64)
+    else:
+        vals = rng.normal(size=n) * 10.0 ** rng.integers(-8, 8, size=n)
+        # exact cancellations: add the negated value of a few records to the same key
+        m = n // 10
+        if m:
+            idx = rng.integers(0, n, size=m)
+            keys = np.concatenate([keys, keys[idx]])
+            vals = np.concatenate([vals, -vals[idx]])
+    gk, gv = gpu.annihilate(keys, vals)
+    ok, ov = O.annihilate(keys, vals)
+    assert np.array_equal(gk, ok)
+    assert gv.dtype == ov.dtype
+    assert np.array_equal(gv.view(np.uint64) if vtype == "float" else gv,
+                          ov.view(np.uint64) if vtype == "float" else ov)
+
+
+STEP_CASES = [
+    # model, M, N, u, style, vector length, value scale, shift, dtau
+    ("real", 10, 10, 6.0, "integer", 2000, 6, -2.0, 0.01),
+    ("real", 20, 20, 6.0, "integer", 5000, 40, 0.5, 0.001),
+    ("mom", 8, 8, 4.0, "integer", 1500, 8, -3.0, 0.005),
+    ("mom", 16, 16, 4.0, "integer", 800, 30, -10.0, 0.0005),
+    ("real", 10, 10, 6.0, "dynamic", 2000, 4.0, -2.0, 0.01),
+    ("real", 20, 20, 6.0, "dynamic", 5000, 60.0, 0.5, 0.001),
+    ("mom", 8, 8, 4.0, "dynamic", 1500, 10.0, -3.0, 0.005),
+    ("real", 8, 8, 2.0, "deterministic", 1000, 1.0, 0.3, 0.01),
+    ("mom", 7, 6, 3.0, "deterministic", 600, 1.0, 0.3, 0.01),
+]
+
+
+def _engine(rimu, kind, M, N, u, style, seed):
+    h = device_model(rimu, kind, M, N, u, 1.0)
+    st = {"integer": rimu.IsStochasticInteger(), "dynamic": rimu.IsDynamicSemistochastic(),
+          "deterministic": rimu.IsDeterministic()}[style]
+    return rimu._native.Engine(h.descriptor(), st.descriptor(), seed, 4_000_000, 1 << 16)
+
+
+@pytest.mark.parametrize("case", STEP_CASES, ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[4]}")
+def test_step_matches_oracle(gpu, case):
+    kind, M, N, u, style, n, scale, shift, dtau = case
+    rng = np.random.default_rng(hash(case) % 2 ** 32)
+    keys, vals = random_vector(M, N, n, rng, "int" if style == "integer" else "float", scale)
+    seed, step = 0xC0FFEE + n, 37
+    eng = _engine(gpu, kind, M, N, u, style, seed)
+    eng.upload(keys, vals)
+    st = eng.step(step, shift, dtau)
+    gk, gv = eng.download()
+    eng.close()
+    om = oracle_model(kind, M, N, u, 1.0)
+    osty = O.style(style)
+    ok, ov, ostats = O.step(om, osty, keys, vals, seed=seed, step=step, shift=shift,
+                            dtau=dtau, order=1)
+    assert np.array_equal(gk, ok)
+    if style == "integer":
+        assert np.array_equal(gv, ov)
+        for name, field in [("attempts", "spawn_attempts"), ("spawns", "spawns"),
+                            ("deaths", "deaths"), ("clones", "clones"), ("zombies", "zombies"),
+                            ("len", "len"), ("norm", "norm")]:
+            assert getattr(st, field) == ostats[name], name
+    else:
+        assert np.array_equal(gv.view(np.uint64), ov.view(np.uint64))
+        assert st.spawn_attempts == ostats["attempts"]
+        assert st.exact_steps == ostats["exact_steps"]
+        assert st.inexact_steps == ostats["inexact_steps"]
+        assert st.len == ostats["len"] and st.len_before == ostats["len_before"]
+        assert st.norm == pytest.approx(ostats["norm"], rel=1e-13)
+        assert st.spawns == pytest.approx(ostats["spawns"], rel=1e-13)
+    # the reference's own column order differs only in floating-point association
+    rk, rv, _ = O.step(om, osty, keys, vals, seed=seed, step=step, shift=shift, dtau=dtau, order=0)
+    if style == "deterministic":
+        assert np.array_equal(rk, gk)
+        np.testing.assert_allclose(gv, rv, rtol=1e-12, atol=1e-12 * np.abs(rv).max())
+
+
+def test_step_sequence_reproducible(gpu):
+    """Same seed, same steps: bit-identical vectors (no order dependence on scheduling)."""
+    M = N = 12
+    rng = np.random.default_rng(5)
+    keys, vals = random_vector(M, N, 20000, rng, "float", 20.0)
+    outs = []
+    for _ in range(2):
+        eng = _engine(gpu, "real", M, N, 6.0, "dynamic", 99)
+        eng.upload(keys, vals)
+        for s in range(1, 6):
+            eng.step(s, -1.0, 0.01)
+        outs.append(eng.download())
+        eng.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
+
+
+def test_spawn_buffer_growth_repeats_step(gpu):
+    """A step that overflows the spawn buffer is repeated after growing it, with the
+    same result as a step that had room from the start."""
+    M = N = 10
+    rng = np.random.default_rng(11)
+    keys, vals = random_vector(M, N, 3000, rng, "int", 50)
+    h = device_model(gpu, "real", M, N, 6.0, 1.0)
+    sty = gpu.IsStochasticInteger().descriptor()
+    small = gpu._native.Engine(h.descriptor(), sty, 5, 100000, 64)
+    big = gpu._native.Engine(h.descriptor(), sty, 5, 100000, 1 << 22)
+    for e in (small, big):
+        e.upload(keys, vals)
+        e.step(3, 0.0, 0.01)
+    a, b = small.download(), big.download()
+    assert small.max_spawns > 64
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    small.close()
+    big.close()
