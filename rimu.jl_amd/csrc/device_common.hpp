@@ -109,6 +109,48 @@ __device__ inline uint64_t lookback_exclusive(uint64_t* status, uint32_t tile, u
     return prefix;
 }
 
+// Wavefront-parallel look-back, called by all 64 lanes of one wavefront of tile `tile`.
+// Each probe reads the status words of 64 predecessors at once (lane l reads tile
+// window_end - l); the nearest inclusive prefix ends the walk, aggregates in front of it
+// are added.  Returns the exclusive prefix in every lane.  Waits are bounded as above.
+__device__ inline uint64_t lookback_exclusive_wave(uint64_t* status, uint32_t tile, uint64_t agg,
+                                                   int* err_flag) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (tile == 0) {
+        if (lane == 0) lb_store(&status[0], LB_FLAG_PFX | agg);
+        return 0;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_FLAG_AGG | agg);
+    uint64_t prefix = 0;
+    int64_t window_end = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (true) {
+        const int64_t j = window_end - lane;
+        const uint64_t w = (j >= 0) ? lb_load(&status[j]) : LB_FLAG_PFX;  // before tile 0: prefix 0
+        const uint32_t f = (uint32_t)(w >> 62);
+        const uint64_t empty = __ballot(f == 0);
+        const uint64_t pfx = __ballot(f == 2);
+        const int first = pfx ? (__ffsll((long long)pfx) - 1) : WAVE;  // nearest prefix lane
+        const uint64_t needed = first >= WAVE - 1 ? ~0ull : ((2ull << first) - 1ull);
+        if (empty & needed) {
+            if (++spins > LB_SPIN_LIMIT) {
+                if (lane == 0) atomicOr(err_flag, 2);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t v = (lane <= first) ? (w & LB_MASK) : 0ull;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) v += __shfl_xor(v, off, WAVE);
+        prefix += v;
+        if (first < WAVE) break;
+        window_end -= WAVE;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_FLAG_PFX | (prefix + agg));
+    return prefix;
+}
+
 // Dynamic tile id shared through LDS.
 __device__ inline uint32_t next_tile_id(uint32_t* counter) {
     __shared__ uint32_t s_tile;

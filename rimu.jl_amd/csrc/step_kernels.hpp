@@ -206,19 +206,48 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     __shared__ int s_off[NT + 1];
     __shared__ uint64_t s_base;
 
+    __shared__ int64_t s_woff[2 * NT + 1];
+    __shared__ int64_t s_e0;
+
     const uint32_t tile = next_tile_id(tile_counter);
-    const int64_t w = (int64_t)tile * NT + threadIdx.x;
+    const int64_t w0 = (int64_t)tile * NT;
+    const int64_t w = w0 + threadIdx.x;
+    // entries of this tile's work items: one global search for the first, then a window
+    // of the offsets in LDS (every entry owns at least one work item unless it has no
+    // off-diagonals, so the window normally covers the tile; a miss searches globally)
+    if (threadIdx.x == 0) {
+        int64_t lo = 0, hi = L;  // woff[lo] <= w0 < woff[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (woff[mid] <= w0) lo = mid; else hi = mid;
+        }
+        s_e0 = lo;
+    }
+    __syncthreads();
+    const int64_t e0 = s_e0;
+    for (int t = threadIdx.x; t < 2 * NT + 1; t += NT) s_woff[t] = woff[min(e0 + t, L)];
+    __syncthreads();
     int count = 0;
     double spawned = 0;
     uint64_t kor = 0, kand = ~0ull;
     if (w < W) {
         // entry of this work item: last i with woff[i] <= w
-        int64_t lo = 0, hi = L;  // woff[L] = W > w
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (woff[mid] <= w) lo = mid; else hi = mid;
+        int64_t i;
+        if (w < s_woff[2 * NT]) {
+            int lo = 0, hi = 2 * NT;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_woff[mid] <= w) lo = mid; else hi = mid;
+            }
+            i = e0 + lo;
+        } else {
+            int64_t lo = e0, hi = L;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (woff[mid] <= w) lo = mid; else hi = mid;
+            }
+            i = lo;
         }
-        const int64_t i = lo;
         const int64_t chunk = w - woff[i];
         const uint64_t k = keys[i];
         const V v = vals[i];
@@ -251,9 +280,12 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     int total;
     const int ex = block_exclusive_scan<int, NT>(count, &total);
     s_off[threadIdx.x] = ex;
-    if (threadIdx.x == 0) {
-        s_off[NT] = total;
-        s_base = lookback_exclusive(status, tile, (uint64_t)total, flags);
+    if (threadIdx.x < WAVE) {
+        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
+        if (threadIdx.x == 0) {
+            s_off[NT] = total;
+            s_base = b;
+        }
     }
     __syncthreads();
     const int64_t base = (int64_t)s_base;
@@ -510,7 +542,10 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     }
     int btotal;
     const int ex = block_exclusive_scan<int, NT>(count, &btotal);
-    if (threadIdx.x == 0) s_base = lookback_exclusive(status, tile, (uint64_t)btotal, flags);
+    if (threadIdx.x < WAVE) {
+        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)btotal, flags);
+        if (threadIdx.x == 0) s_base = b;
+    }
     __syncthreads();
     const int64_t base = (int64_t)s_base;
     if (base + btotal > cap) {
