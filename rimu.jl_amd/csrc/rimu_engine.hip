@@ -127,6 +127,7 @@ struct rimu_engine {
     DevBuf<uint64_t> skeys[2];
     DevBuf<unsigned char> svals[2];
     DevBuf<uint64_t> status;
+    DevBuf<int64_t> tile_first, splits;
     DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev;
     DevBuf<unsigned char> misc;  // counters and totals (see layout below)
     // projector
@@ -286,11 +287,14 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         spawn_tiles = ceil_div(W, NT);
         HIP_TRY(e->status.alloc(std::max<int64_t>(spawn_tiles, 1)));
         HIP_TRY(e->spawn_p.alloc(spawn_tiles));
+        HIP_TRY(e->tile_first.alloc(spawn_tiles));
         HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
+        tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
         e->timer.begin("spawn", e->stream, &ev);
         spawn_kernel<V><<<spawn_tiles, NT, 0, e->stream>>>(
             a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
-            e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, &dm->key_or, &dm->key_and);
+            e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
+            &dm->key_or, &dm->key_and);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
@@ -320,11 +324,14 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(e->merge_p.alloc((size_t)merge_tiles * 3));
     HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * merge_tiles, e->stream));
     HIP_TRY(hipMemsetAsync(&dm->tile_counter, 0, sizeof(uint32_t), e->stream));
+    HIP_TRY(e->splits.alloc(merge_tiles + 1));
+    merge_split_kernel<<<ceil_div(merge_tiles + 1, NT), NT, 0, e->stream>>>(
+        e->skeys[sorted].p, S, vk, L, NT * MERGE_IPT, merge_tiles, e->splits.p);
     e->timer.begin("merge", e->stream, &ev);
     merge_kernel<V><<<merge_tiles, NT, 0, e->stream>>>(
         a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk, (const V*)e->dvals.p, L,
         e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, e->status.p, &dm->tile_counter,
-        &dm->L_new, &dm->flags, e->merge_p.p);
+        &dm->L_new, &dm->flags, e->merge_p.p, e->splits.p);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
 
@@ -445,6 +452,7 @@ int rimu_engine_destroy(rimu_engine* e) {
         e->skeys[b].release();
         e->svals[b].release();
     }
+    e->tile_first.release(); e->splits.release();
     e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release();
     e->rtmp.release(); e->status.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->stats_dev.release(); e->misc.release();
@@ -622,10 +630,14 @@ int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int6
     HIP_TRY(e->merge_p.alloc((size_t)tiles * 3));
     HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * tiles, e->stream));
     HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
+    HIP_TRY(e->splits.alloc(tiles + 1));
+    merge_split_kernel<<<ceil_div(tiles + 1, NT), NT, 0, e->stream>>>(
+        e->skeys[sorted].p, n, e->vkeys[0].p, 0, NT * MERGE_IPT, tiles, e->splits.p);
     merge_kernel<V><<<tiles, NT, 0, e->stream>>>(a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n,
                                                  e->vkeys[0].p, (const V*)e->dvals.p, 0, e->vkeys[1].p,
                                                  (V*)e->vvals[1].p, e->max_entries, e->status.p,
-                                                 &dm->tile_counter, &dm->L_new, &dm->flags, e->merge_p.p);
+                                                 &dm->tile_counter, &dm->L_new, &dm->flags, e->merge_p.p,
+                                                 e->splits.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));

@@ -199,32 +199,24 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
                                                    uint64_t* status, uint32_t* tile_counter,
                                                    int64_t* total_out, int* flags,
                                                    double* __restrict__ partials,
+                                                   const int64_t* __restrict__ tile_first,
                                                    unsigned long long* key_or,
                                                    unsigned long long* key_and) {
     __shared__ uint64_t s_k[NT * SPAWN_CHUNK];
     __shared__ V s_v[NT * SPAWN_CHUNK];
     __shared__ int s_off[NT + 1];
     __shared__ uint64_t s_base;
-
     __shared__ int64_t s_woff[2 * NT + 1];
-    __shared__ int64_t s_e0;
+    __shared__ uint64_t s_kor[NT / WAVE], s_kand[NT / WAVE];
 
     const uint32_t tile = next_tile_id(tile_counter);
     const int64_t w0 = (int64_t)tile * NT;
     const int64_t w = w0 + threadIdx.x;
-    // entries of this tile's work items: one global search for the first, then a window
-    // of the offsets in LDS (every entry owns at least one work item unless it has no
-    // off-diagonals, so the window normally covers the tile; a miss searches globally)
-    if (threadIdx.x == 0) {
-        int64_t lo = 0, hi = L;  // woff[lo] <= w0 < woff[hi]
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (woff[mid] <= w0) lo = mid; else hi = mid;
-        }
-        s_e0 = lo;
-    }
-    __syncthreads();
-    const int64_t e0 = s_e0;
+    // entries of this tile's work items: the first is precomputed (tile_first_kernel), the
+    // offsets of the following entries are staged in LDS (every entry owns at least one
+    // work item unless it has no off-diagonals; a work item beyond the window searches
+    // the offsets in global memory)
+    const int64_t e0 = tile_first[tile];
     for (int t = threadIdx.x; t < 2 * NT + 1; t += NT) s_woff[t] = woff[min(e0 + t, L)];
     __syncthreads();
     int count = 0;
@@ -308,16 +300,53 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         partials[tile] = s;
         if (tile == gridDim.x - 1) *total_out = base + total;
     }
-    // key OR/AND for the radix pass selection (bitwise: order independent)
+    // key OR/AND for the radix pass selection (bitwise: order independent).  Reduced over
+    // the workgroup; the global words are only touched when they would change.
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
         kor |= __shfl_xor(kor, off, WAVE);
         kand &= __shfl_xor(kand, off, WAVE);
     }
     if ((threadIdx.x & (WAVE - 1)) == 0) {
-        atomicOr(key_or, (unsigned long long)kor);
-        atomicAnd(key_and, (unsigned long long)kand);
+        s_kor[threadIdx.x / WAVE] = kor;
+        s_kand[threadIdx.x / WAVE] = kand;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t bo = 0, ba = ~0ull;
+        for (int q = 0; q < NT / WAVE; ++q) { bo |= s_kor[q]; ba &= s_kand[q]; }
+        const uint64_t cur_o = __hip_atomic_load(key_or, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t cur_a = __hip_atomic_load(key_and, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bo & ~cur_o) atomicOr(key_or, (unsigned long long)bo);
+        if (cur_a & ~ba) atomicAnd(key_and, (unsigned long long)ba);
+    }
+}
+
+// First entry of every spawn tile: entry i owns work items [woff[i], woff[i+1]); a tile of
+// NT items starting at t*NT belongs first to the entry whose range contains t*NT.
+__global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restrict__ woff, int64_t L,
+                                                        int64_t* __restrict__ tile_first) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= L) return;
+    const int64_t c0 = woff[i], c1 = woff[i + 1];
+    for (int64_t t = (c0 + NT - 1) / NT; t * NT < c1; ++t) tile_first[t] = i;
+}
+
+// Merge-path split of every merge tile boundary d = t*TILE (clamped to S+L): number of A
+// elements among the first d merged elements (B before A on equal keys).
+__global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restrict__ ak, int64_t S,
+                                                         const uint64_t* __restrict__ bk, int64_t L,
+                                                         int64_t tile_size, int64_t ntiles,
+                                                         int64_t* __restrict__ splits) {
+    const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (t > ntiles) return;
+    const int64_t d = min(t * tile_size, S + L);
+    int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (ak[mid] < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
+    }
+    splits[t] = lo;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -441,35 +470,26 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                                                    uint64_t* __restrict__ ok, V* __restrict__ ov,
                                                    int64_t cap, uint64_t* status,
                                                    uint32_t* tile_counter, int64_t* total_out,
-                                                   int* flags, double* __restrict__ partials) {
+                                                   int* flags, double* __restrict__ partials,
+                                                   const int64_t* __restrict__ splits) {
     constexpr int TILE = NT * MERGE_IPT;
-    __shared__ uint64_t s_a[TILE];
-    __shared__ uint64_t s_b[TILE];
-    __shared__ int64_t s_split[4];  // a0, b0, a1, b1
+    // the tile's A range followed by its B range (na + nb <= TILE)
+    __shared__ uint64_t s_key[TILE];
+    __shared__ V s_val[TILE];
     __shared__ uint64_t s_base;
 
     const uint32_t tile = next_tile_id(tile_counter);
     const int64_t total = S + L;
     const int64_t d0 = (int64_t)tile * TILE;
     const int64_t d1 = min(d0 + TILE, total);
-
-    // merge-path split of a diagonal d: number of A elements among the first d
-    auto split = [&](int64_t d) -> int64_t {
-        int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (ak[mid] < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
-        }
-        return lo;
-    };
-    if (threadIdx.x == 0) { const int64_t x = split(d0); s_split[0] = x; s_split[1] = d0 - x; }
-    if (threadIdx.x == 1) { const int64_t x = split(d1); s_split[2] = x; s_split[3] = d1 - x; }
-    __syncthreads();
-    const int64_t a0 = s_split[0], b0 = s_split[1], a1 = s_split[2], b1 = s_split[3];
+    const int64_t a0 = splits[tile], a1 = splits[tile + 1];
+    const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-    for (int i = threadIdx.x; i < na; i += NT) s_a[i] = ak[a0 + i];
-    for (int i = threadIdx.x; i < nb; i += NT) s_b[i] = bk[b0 + i];
+    for (int i = threadIdx.x; i < na; i += NT) { s_key[i] = ak[a0 + i]; s_val[i] = av[a0 + i]; }
+    for (int i = threadIdx.x; i < nb; i += NT) { s_key[na + i] = bk[b0 + i]; s_val[na + i] = bv[b0 + i]; }
     __syncthreads();
+    const uint64_t* s_a = s_key;
+    const uint64_t* s_b = s_key + na;
 
     // this thread's positions within the tile
     const int dd = (int)threadIdx.x * MERGE_IPT;
@@ -511,9 +531,13 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
             if (!have_prev || key != prev) {
                 // segment head: fold diag (if any) then the run of spawns in A
                 V acc = (V)0;
-                int64_t ga = a0 + ia;
-                if (take_b) acc = acc + bv[b0 + ib];
-                while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
+                if (take_b) acc = acc + s_val[na + ib];
+                int la = ia;
+                while (la < na && s_a[la] == key) { acc = acc + s_val[la]; ++la; }
+                if (la == na) {  // the run may continue past the tile
+                    int64_t ga = a0 + la;
+                    while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
+                }
                 if (acc != (V)0) {
                     lb += 1;
                     if constexpr (!is_int_v<V>::value) {
