@@ -279,22 +279,30 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(hipStreamSynchronize(e->stream));
     const int64_t W = e->h_misc->W;
 
-    // 2. spawn
+    // 2. spawn (W = number of attempts)
     int64_t S = 0;
     int64_t spawn_tiles = 0;
     uint64_t kor = 0, kand = ~0ull;
     if (W > 0) {
-        spawn_tiles = ceil_div(W, NT);
+        spawn_tiles = ceil_div(W, SPAWN_TILE);
+        // every attempt yields a record when values are floating and deposits unprojected
+        const bool dense = !e->int_vals && e->style.proj_threshold == 0;
         HIP_TRY(e->status.alloc(std::max<int64_t>(spawn_tiles, 1)));
         HIP_TRY(e->spawn_p.alloc(spawn_tiles));
         HIP_TRY(e->tile_first.alloc(spawn_tiles));
-        HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
+        if (!dense) HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
         tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
         e->timer.begin("spawn", e->stream, &ev);
-        spawn_kernel<V><<<spawn_tiles, NT, 0, e->stream>>>(
-            a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
-            e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
-            &dm->key_or, &dm->key_and);
+        if (dense)
+            spawn_kernel<V, true><<<spawn_tiles, NT, 0, e->stream>>>(
+                a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
+                e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
+                &dm->key_or, &dm->key_and);
+        else
+            spawn_kernel<V, false><<<spawn_tiles, NT, 0, e->stream>>>(
+                a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
+                e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
+                &dm->key_or, &dm->key_and);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));

@@ -19,7 +19,6 @@
 namespace rimu {
 
 constexpr int NT = 256;          // workgroup size of every step kernel
-constexpr int SPAWN_CHUNK = 8;   // attempts per spawn work item
 constexpr int MERGE_IPT = 8;     // merged positions per thread
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
@@ -83,7 +82,8 @@ __device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
 }
 
 // ---------------------------------------------------------------------------------------
-// plan: one thread per entry (thread L writes the trailing zero of the count array).
+// plan: one thread per entry: the diagonal step and the number of spawn attempts of the
+// entry (thread L writes the trailing zero of the count array).
 // partials[tile*6 + {0..5}] = deaths, clones, zombies, exact, inexact, attempts.
 // ---------------------------------------------------------------------------------------
 template <class V>
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __
             if (a.style != 1) { if (c.exact) st[3] = 1; else st[4] = 1; }
             st[5] = (double)c.n_items;
         }
-        wcount[i] = (c.n_items + SPAWN_CHUNK - 1) / SPAWN_CHUNK;
+        wcount[i] = c.n_items;
     } else if (i == L) {
         wcount[i] = 0;
     }
@@ -185,15 +185,23 @@ __global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64
 }
 
 // ---------------------------------------------------------------------------------------
-// spawn: one thread per work item (entry, chunk of SPAWN_CHUNK attempts).  Records are
-// staged per thread in LDS, compacted in (thread, attempt) order, placed after all earlier
-// tiles by the look-back, and written with consecutive lanes on consecutive records.
-// Also reduces the key OR/AND (which radix digits vary) and sum |spawn| per tile.
+// spawn: one thread per spawn attempt.  Attempts are numbered entry by entry (attempt j of
+// entry i is aoff[i] + j), so consecutive lanes hold consecutive attempts and the lane
+// order is the (entry, attempt) order.  Each thread handles SPAWN_ITEMS attempts, strided
+// by NT within the tile.  Nonzero records are compacted with a wavefront ballot and
+// popcount prefix; the (item, wavefront) counts are scanned across the workgroup and the
+// tile is placed after all earlier tiles by the look-back.  When every attempt yields a
+// record (DENSE: floating values without projection) the record of attempt a is simply
+// written at position a and no look-back is needed.
+// Also reduces sum |spawn| per tile and the key OR/AND (which radix digits vary).
 // ---------------------------------------------------------------------------------------
-template <class V>
+constexpr int SPAWN_ITEMS = 4;
+constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
+
+template <class V, bool DENSE>
 __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* __restrict__ keys,
                                                    const V* __restrict__ vals, int64_t L,
-                                                   const int64_t* __restrict__ woff, int64_t W,
+                                                   const int64_t* __restrict__ aoff, int64_t A,
                                                    uint64_t* __restrict__ out_keys,
                                                    V* __restrict__ out_vals, int64_t cap,
                                                    uint64_t* status, uint32_t* tile_counter,
@@ -202,103 +210,127 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
                                                    const int64_t* __restrict__ tile_first,
                                                    unsigned long long* key_or,
                                                    unsigned long long* key_and) {
-    __shared__ uint64_t s_k[NT * SPAWN_CHUNK];
-    __shared__ V s_v[NT * SPAWN_CHUNK];
-    __shared__ int s_off[NT + 1];
+    constexpr int NW = NT / WAVE;
+    constexpr int WIN = SPAWN_TILE + 1;
+    __shared__ int64_t s_aoff[WIN];
+    __shared__ int s_cnt[SPAWN_ITEMS * NW];
     __shared__ uint64_t s_base;
-    __shared__ int64_t s_woff[2 * NT + 1];
-    __shared__ uint64_t s_kor[NT / WAVE], s_kand[NT / WAVE];
+    __shared__ int s_total;
+    __shared__ uint64_t s_kor[NW], s_kand[NW];
 
-    const uint32_t tile = next_tile_id(tile_counter);
-    const int64_t w0 = (int64_t)tile * NT;
-    const int64_t w = w0 + threadIdx.x;
-    // entries of this tile's work items: the first is precomputed (tile_first_kernel), the
-    // offsets of the following entries are staged in LDS (every entry owns at least one
-    // work item unless it has no off-diagonals; a work item beyond the window searches
-    // the offsets in global memory)
+    const uint32_t tile = DENSE ? blockIdx.x : next_tile_id(tile_counter);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int64_t a0 = (int64_t)tile * SPAWN_TILE;
     const int64_t e0 = tile_first[tile];
-    for (int t = threadIdx.x; t < 2 * NT + 1; t += NT) s_woff[t] = woff[min(e0 + t, L)];
+    for (int t = threadIdx.x; t < WIN; t += NT) s_aoff[t] = aoff[min(e0 + t, L)];
     __syncthreads();
-    int count = 0;
+
+    uint64_t rk[SPAWN_ITEMS];
+    V rv[SPAWN_ITEMS];
+    int rpos[SPAWN_ITEMS];
+    bool rok[SPAWN_ITEMS];
     double spawned = 0;
     uint64_t kor = 0, kand = ~0ull;
-    if (w < W) {
-        // entry of this work item: last i with woff[i] <= w
-        int64_t i;
-        if (w < s_woff[2 * NT]) {
-            int lo = 0, hi = 2 * NT;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_woff[mid] <= w) lo = mid; else hi = mid;
+#pragma unroll
+    for (int it = 0; it < SPAWN_ITEMS; ++it) {
+        const int64_t att = a0 + (int64_t)it * NT + threadIdx.x;
+        bool ok = false;
+        uint64_t nb = 0;
+        V nv = (V)0;
+        if (att < A) {
+            // entry of this attempt: last i with aoff[i] <= att (every entry has at least
+            // one attempt unless it has no off-diagonals; outside the window: global search)
+            int64_t i, ai;
+            if (att < s_aoff[WIN - 1]) {
+                int lo = 0, hi = WIN - 1;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_aoff[mid] <= att) lo = mid; else hi = mid;
+                }
+                i = e0 + lo;
+                ai = s_aoff[lo];
+            } else {
+                int64_t lo = e0, hi = L;
+                while (hi - lo > 1) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (aoff[mid] <= att) lo = mid; else hi = mid;
+                }
+                i = lo;
+                ai = aoff[lo];
             }
-            i = e0 + lo;
-        } else {
-            int64_t lo = e0, hi = L;
-            while (hi - lo > 1) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (woff[mid] <= w) lo = mid; else hi = mid;
-            }
-            i = lo;
-        }
-        const int64_t chunk = w - woff[i];
-        const uint64_t k = keys[i];
-        const V v = vals[i];
-        const ColumnPlan c = plan_column<V>(a, k, v);
-        const double magnitude = (double)v / (double)c.n_items;
-        const double prob = 1.0 / (double)c.n_od;
-        const int64_t j0 = chunk * SPAWN_CHUNK;
-        const int64_t j1 = min(j0 + (int64_t)SPAWN_CHUNK, c.n_items);
-        const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
-        for (int64_t j = j0; j < j1; ++j) {
-            int64_t chosen;
+            const int64_t j = att - ai;
+            const uint64_t k = keys[i];
+            const V v = vals[i];
+            const ColumnPlan c = plan_column<V>(a, k, v);
+            const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
             U4 r = {0, 0, 0, 0};
             if (!c.exact || need_u) r = draw(a.seed, k, (uint32_t)j, a.step, TAG_SPAWN);
-            chosen = c.exact ? j + 1 : pick1(r.x, c.n_od);
+            const int64_t chosen = c.exact ? j + 1 : pick1(r.x, c.n_od);
             double hv;
-            const uint64_t nb = get_offdiagonal(a.h, k, chosen, &hv);
+            nb = get_offdiagonal(a.h, k, chosen, &hv);
             const double mel = -hv * a.dtau;  // FirstOrderOffdiagonals, fciqmc.jl:109-112
-            const double x = c.exact ? (double)v * mel : mel * magnitude / prob;
-            const V nv = project<V>(x, a.proj_threshold, need_u ? u53(r.y, r.z) : 0.0);
-            spawned += fabs((double)nv);
-            if (nv != 0) {
-                s_k[threadIdx.x * SPAWN_CHUNK + count] = nb;
-                s_v[threadIdx.x * SPAWN_CHUNK + count] = nv;
-                ++count;
-                kor |= nb;
-                kand &= nb;
+            double x;
+            if (c.exact) {
+                x = (double)v * mel;
+            } else {
+                const double magnitude = (double)v / (double)c.n_items;
+                const double prob = 1.0 / (double)c.n_od;
+                x = mel * magnitude / prob;
             }
+            nv = project<V>(x, a.proj_threshold, need_u ? u53(r.y, r.z) : 0.0);
+            spawned += fabs((double)nv);
+            ok = DENSE || nv != (V)0;
+            if (ok) { kor |= nb; kand &= nb; }
+        }
+        rk[it] = nb;
+        rv[it] = nv;
+        rok[it] = ok;
+        if (!DENSE) {
+            const uint64_t m = __ballot(ok);
+            rpos[it] = __popcll(m & lt_mask);
+            if (lane == 0) s_cnt[it * NW + wid] = __popcll(m);
         }
     }
-    int total;
-    const int ex = block_exclusive_scan<int, NT>(count, &total);
-    s_off[threadIdx.x] = ex;
-    if (threadIdx.x < WAVE) {
-        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
+
+    int64_t base = a0;
+    int total = 0;
+    if (!DENSE) {
+        __syncthreads();
+        // exclusive scan of the (item, wavefront) counts: attempt order within the tile
         if (threadIdx.x == 0) {
-            s_off[NT] = total;
-            s_base = b;
+            int run = 0;
+            for (int q = 0; q < SPAWN_ITEMS * NW; ++q) { const int c = s_cnt[q]; s_cnt[q] = run; run += c; }
+            s_total = run;
         }
+        __syncthreads();
+        total = s_total;
+        if (threadIdx.x < WAVE) {
+            const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
+            if (threadIdx.x == 0) s_base = b;
+        }
+        __syncthreads();
+        base = (int64_t)s_base;
     }
-    __syncthreads();
-    const int64_t base = (int64_t)s_base;
-    if (base + total > cap) {
+    const int64_t tile_end = DENSE ? min(a0 + (int64_t)SPAWN_TILE, A) : base + total;
+    if (tile_end > cap) {
         if (threadIdx.x == 0) atomicOr(flags, FLAG_SPAWN_OVERFLOW);
     } else {
-        for (int idx = threadIdx.x; idx < total; idx += NT) {
-            int lo = 0, hi = NT;  // s_off[lo] <= idx < s_off[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= idx) lo = mid; else hi = mid;
+#pragma unroll
+        for (int it = 0; it < SPAWN_ITEMS; ++it) {
+            if (rok[it]) {
+                const int64_t pos = DENSE ? a0 + (int64_t)it * NT + threadIdx.x
+                                          : base + s_cnt[it * NW + wid] + rpos[it];
+                out_keys[pos] = rk[it];
+                out_vals[pos] = rv[it];
             }
-            const int slot = lo * SPAWN_CHUNK + (idx - s_off[lo]);
-            out_keys[base + idx] = s_k[slot];
-            out_vals[base + idx] = s_v[slot];
         }
     }
     const double s = block_sum<NT>(spawned);
     if (threadIdx.x == 0) {
         partials[tile] = s;
-        if (tile == gridDim.x - 1) *total_out = base + total;
+        if (tile == gridDim.x - 1) *total_out = tile_end;
     }
     // key OR/AND for the radix pass selection (bitwise: order independent).  Reduced over
     // the workgroup; the global words are only touched when they would change.
@@ -307,14 +339,14 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         kor |= __shfl_xor(kor, off, WAVE);
         kand &= __shfl_xor(kand, off, WAVE);
     }
-    if ((threadIdx.x & (WAVE - 1)) == 0) {
-        s_kor[threadIdx.x / WAVE] = kor;
-        s_kand[threadIdx.x / WAVE] = kand;
+    if (lane == 0) {
+        s_kor[wid] = kor;
+        s_kand[wid] = kand;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t bo = 0, ba = ~0ull;
-        for (int q = 0; q < NT / WAVE; ++q) { bo |= s_kor[q]; ba &= s_kand[q]; }
+        for (int q = 0; q < NW; ++q) { bo |= s_kor[q]; ba &= s_kand[q]; }
         const uint64_t cur_o = __hip_atomic_load(key_or, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t cur_a = __hip_atomic_load(key_and, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (bo & ~cur_o) atomicOr(key_or, (unsigned long long)bo);
@@ -322,14 +354,14 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     }
 }
 
-// First entry of every spawn tile: entry i owns work items [woff[i], woff[i+1]); a tile of
-// NT items starting at t*NT belongs first to the entry whose range contains t*NT.
-__global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restrict__ woff, int64_t L,
+// First entry of every spawn tile: entry i owns attempts [aoff[i], aoff[i+1]); the tile
+// starting at attempt t*SPAWN_TILE belongs first to the entry whose range contains it.
+__global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restrict__ aoff, int64_t L,
                                                         int64_t* __restrict__ tile_first) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i >= L) return;
-    const int64_t c0 = woff[i], c1 = woff[i + 1];
-    for (int64_t t = (c0 + NT - 1) / NT; t * NT < c1; ++t) tile_first[t] = i;
+    const int64_t c0 = aoff[i], c1 = aoff[i + 1];
+    for (int64_t t = (c0 + SPAWN_TILE - 1) / SPAWN_TILE; t * SPAWN_TILE < c1; ++t) tile_first[t] = i;
 }
 
 // Merge-path split of every merge tile boundary d = t*TILE (clamped to S+L): number of A
