@@ -86,6 +86,9 @@ def lib():
         L.or_build_basis.restype = ctypes.c_int64
         L.or_apply_affine.argtypes = [P(Model), ctypes.c_int32, u64p, ctypes.c_int64, dp, dp, dp, dp]
         L.or_apply_affine.restype = ctypes.c_int32
+        L.or_matrix_triplets.argtypes = [P(Model), u64p, ctypes.c_int64, P(ctypes.c_int64),
+                                         P(ctypes.c_int64), dp, ctypes.c_int64]
+        L.or_matrix_triplets.restype = ctypes.c_int64
         L.or_rank.argtypes = [ctypes.c_uint64]
         L.or_rank.restype = ctypes.c_int64
         _LIB = L
@@ -301,29 +304,37 @@ def bfs_basis(model: Model, start: int, limit: int = 5_000_000) -> np.ndarray:
     return np.array(sorted(seen), dtype=np.uint64)
 
 
-def hamiltonian_operator(model: Model, basis: np.ndarray):
-    import scipy.sparse.linalg as sla
+def sparse_hamiltonian(model: Model, basis: np.ndarray):
+    """H on ``basis`` as a scipy CSR matrix (duplicate elements summed)."""
+    import scipy.sparse as sp
+    basis = np.ascontiguousarray(basis, dtype=np.uint64)
     D = len(basis)
-    one, zero = np.ones(1), np.zeros(1)
+    cap = D * 64
+    while True:
+        r = np.zeros(cap, dtype=np.int64)
+        c = np.zeros(cap, dtype=np.int64)
+        v = np.zeros(cap)
+        n = lib().or_matrix_triplets(ctypes.byref(model), _ptr(basis, ctypes.c_uint64), D,
+                                     _ptr(r, ctypes.c_int64), _ptr(c, ctypes.c_int64),
+                                     _ptr(v, ctypes.c_double), cap)
+        if n == -1 - cap:
+            raise RuntimeError("neighbour outside basis")
+        if n >= 0:
+            return sp.csr_matrix((v[:n], (r[:n], c[:n])), shape=(D, D))
+        cap = -n
 
-    def mv(x):
-        x = np.asarray(x, dtype=np.float64).reshape(D, 1)
-        return apply_affine([model], basis, zero, one, x).reshape(D)
 
-    return sla.LinearOperator((D, D), matvec=mv, rmatvec=mv, dtype=np.float64)
+def hamiltonian_operator(model: Model, basis: np.ndarray):
+    return sparse_hamiltonian(model, basis)
 
 
 def ed_energies(model: Model, basis: np.ndarray, k: int = 1) -> np.ndarray:
     """Lowest ``k`` eigenvalues of H on ``basis`` (dense ``eigh`` when small, else ARPACK)."""
     D = len(basis)
     if D <= 3000:
-        cols = np.zeros((D, D))
-        for j in range(D):
-            e = np.zeros((D, 1))
-            e[j, 0] = 1.0
-            cols[:, j] = apply_affine([model], basis, np.zeros(1), np.ones(1), e)[:, 0]
-        return np.linalg.eigvalsh(0.5 * (cols + cols.T))[:k]
+        H = sparse_hamiltonian(model, basis).toarray()
+        return np.linalg.eigvalsh(0.5 * (H + H.T))[:k]
     import scipy.sparse.linalg as sla
-    vals = sla.eigsh(hamiltonian_operator(model, basis), k=k, which="SA", tol=1e-12,
+    vals = sla.eigsh(sparse_hamiltonian(model, basis), k=k, which="SA", tol=1e-12,
                      return_eigenvectors=False)
     return np.sort(vals)

@@ -637,6 +637,34 @@ int32_t or_apply_affine(const or_model* hs, int32_t nrhs, const uint64_t* basis,
     return 0;
 }
 
+// All nonzero matrix elements of H on a sorted basis as (row, col, value) triplets, in
+// column order (duplicates of the same (row, col) are listed separately).  Returns the
+// number of triplets, or -(needed) if cap is too small, or -1 - cap if a neighbour is
+// outside the basis.
+int64_t or_matrix_triplets(const or_model* h, const uint64_t* basis, int64_t D, int64_t* rows,
+                           int64_t* cols, double* vals, int64_t cap) {
+    int64_t n = 0;
+    for (int64_t j = 0; j < D; ++j) {
+        const uint64_t addr = basis[j];
+        BoseIndex occ[64];
+        const int nocc = occupied_modes(addr, occ);
+        const double d = or_diagonal_element(h, addr);
+        if (n < cap) { rows[n] = j; cols[n] = j; vals[n] = d; }
+        ++n;
+        const int64_t n_od = or_num_offdiagonals(h, addr);
+        for (int64_t c = 1; c <= n_od; ++c) {
+            double hv;
+            const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
+                                                : mom1d_offdiagonal(h, addr, occ, nocc, c, &hv);
+            const int64_t k = std::lower_bound(basis, basis + D, nb) - basis;
+            if (k >= D || basis[k] != nb) return -1 - cap;
+            if (n < cap) { rows[n] = k; cols[n] = j; vals[n] = hv; }
+            ++n;
+        }
+    }
+    return n <= cap ? n : -n;
+}
+
 // Colex rank of an address among all N-boson M-mode addresses: the address is an
 // N-subset (positions of the one-bits) of B = N + M - 1 bits; rank = sum_j C(p_j, j+1).
 // This ordering equals ascending numeric order of the bit strings.

@@ -30,6 +30,17 @@ class BlockingResult:
     blocks: int
 
 
+def smoothen(noisy, b):
+    """Moving average over b elements with periodic wrap (StatsTools/growth_witness.jl:66-77)."""
+    noisy = np.asarray(noisy, dtype=np.float64)
+    n = len(noisy)
+    offset = b // 2 + 1
+    out = np.zeros(n)
+    for j in range(1, b + 1):
+        out += np.roll(noisy, -(j - offset)) / b
+    return out
+
+
 def _blocker(v):
     n = len(v) // 2
     return (v[0:2 * n:2] + v[1:2 * n:2]) / 2
@@ -46,7 +57,8 @@ def blocks_with_m(v, corrected=True):
         d = v - mean_v
         variance = float(np.dot(d, d) / (n - c))
         gamma = float(np.dot(d[:-1], d[1:]) / (n - c))
-        mj = n * ((n - 1) * variance / n ** 2 + gamma) ** 2 / variance ** 2 if variance > 0 else 0.0
+        # a constant series gives 0/0 = NaN, which never passes the M test (as in Julia)
+        mj = n * ((n - 1) * variance / n ** 2 + gamma) ** 2 / variance ** 2 if variance > 0 else np.nan
         se = np.sqrt(variance / n)
         out["blocks"].append(n)
         out["mean"].append(mean_v)
