@@ -62,6 +62,29 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
+// Three deterministic workgroup sums at once.  Results valid in thread 0.
+template <int NT>
+__device__ inline void block_sum3(double& a, double& b, double& c) {
+    __shared__ double s[3][NT];
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    __syncthreads();
+#pragma unroll
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + off];
+            s[1][threadIdx.x] += s[1][threadIdx.x + off];
+            s[2][threadIdx.x] += s[2][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    a = s[0][0];
+    b = s[1][0];
+    c = s[2][0];
+    __syncthreads();
+}
+
 // Status words of the look-back: top two bits flag (0 empty, 1 tile aggregate, 2 inclusive
 // prefix), low 62 bits the value.
 constexpr uint64_t LB_FLAG_AGG = 1ull << 62;

@@ -409,10 +409,13 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
                                                            int64_t num_tiles,
                                                            uint64_t* __restrict__ keys_out,
                                                            V* __restrict__ vals_out) {
+    // Wavefront w ranks the contiguous sub-tile [w*SORT_IPT*64, (w+1)*SORT_IPT*64) of the
+    // tile with private digit counters: round r handles keys w*SORT_IPT*64 + r*64 + lane,
+    // so (round, lane) is the element order and ranks are stable without barriers.
+    constexpr int NW = NT / WAVE;
     __shared__ uint64_t s_k[SORT_TILE];
     __shared__ V s_v[SORT_TILE];
-    __shared__ uint32_t s_wcnt[NT / WAVE][RADIX];
-    __shared__ uint32_t s_run[RADIX];
+    __shared__ uint32_t s_wcnt[NW][RADIX];  // per-wavefront digit counts, then prefixes
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_goff[RADIX];
 
@@ -420,64 +423,70 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    const int64_t wbase = base + (int64_t)wid * SORT_IPT * WAVE;
 
-    s_run[threadIdx.x] = 0;
+    for (int j = threadIdx.x; j < NW * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
     s_goff[threadIdx.x] = offs[(int64_t)threadIdx.x * num_tiles + blockIdx.x];
+    __syncthreads();
 
     uint64_t k[SORT_IPT];
     V v[SORT_IPT];
     uint32_t rank[SORT_IPT];
+    uint32_t* cnt = s_wcnt[wid];
 #pragma unroll
     for (int r = 0; r < SORT_IPT; ++r) {
-        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         const bool valid = idx < n;
         k[r] = valid ? keys_in[idx] : 0ull;
         v[r] = valid ? vals_in[idx] : (V)0;
+    }
+#pragma unroll
+    for (int r = 0; r < SORT_IPT; ++r) {
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
+        const bool valid = idx < n;
         const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
-        // peers: lanes of this wavefront holding the same digit
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < RADIX_BITS; ++b) {
             const uint64_t bb = __ballot((digit >> b) & 1u);
             peers &= ((digit >> b) & 1u) ? bb : ~bb;
         }
-        for (int j = threadIdx.x; j < (NT / WAVE) * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
-        __syncthreads();
+        const uint32_t before = valid ? cnt[digit] : 0u;
         const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && in_wave == 0) s_wcnt[wid][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t before = s_run[digit];
-            for (int w2 = 0; w2 < wid; ++w2) before += s_wcnt[w2][digit];
-            rank[r] = before + in_wave;
-        }
-        __syncthreads();
-        {
-            uint32_t add = 0;
-#pragma unroll
-            for (int w2 = 0; w2 < NT / WAVE; ++w2) add += s_wcnt[w2][threadIdx.x];
-            s_run[threadIdx.x] += add;
-        }
-        __syncthreads();
+        rank[r] = before + in_wave;
+        // the lowest lane of each peer group advances the counter after every lane read it
+        __builtin_amdgcn_wave_barrier();
+        if (valid && in_wave == 0) cnt[digit] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
-    // tile-local digit starts
-    uint32_t tot;
-    const uint32_t start = block_exclusive_scan<uint32_t, NT>(s_run[threadIdx.x], &tot);
-    s_start[threadIdx.x] = start;
+    __syncthreads();
+    // per digit: prefix over wavefronts (element order) and the tile's digit total
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = s_wcnt[w][d];
+            s_wcnt[w][d] = run;
+            run += c;
+        }
+        uint32_t tot;
+        s_start[d] = block_exclusive_scan<uint32_t, NT>(run, &tot);
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_IPT; ++r) {
-        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         if (idx < n) {
             const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
-            const uint32_t pos = s_start[digit] + rank[r];
+            const uint32_t pos = s_start[digit] + s_wcnt[wid][digit] + rank[r];
             s_k[pos] = k[r];
             s_v[pos] = v[r];
         }
     }
     __syncthreads();
-    const int64_t cnt = min((int64_t)SORT_TILE, n - base);
-    for (int i = threadIdx.x; i < cnt; i += NT) {
+    const int64_t cnt_tile = min((int64_t)SORT_TILE, n - base);
+    for (int i = threadIdx.x; i < cnt_tile; i += NT) {
         const uint64_t kk = s_k[i];
         const uint32_t digit = (uint32_t)((kk >> shift) & (RADIX - 1));
         const int64_t g = (int64_t)s_goff[digit] + (i - (int64_t)s_start[digit]);
@@ -597,31 +606,34 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         }
     }
     int btotal;
-    const int ex = block_exclusive_scan<int, NT>(count, &btotal);
+    const int ex = block_exclusive_scan<int, NT>(count, &btotal);  // ends with a barrier
     if (threadIdx.x < WAVE) {
         const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)btotal, flags);
         if (threadIdx.x == 0) s_base = b;
+    }
+    // stage the survivors in LDS (the input keys are no longer read) for coalesced writes
+#pragma unroll
+    for (int j = 0; j < MERGE_IPT; ++j) {
+        if (j < count) {
+            s_key[ex + j] = out_k[j];
+            s_val[ex + j] = out_v[j];
+        }
     }
     __syncthreads();
     const int64_t base = (int64_t)s_base;
     if (base + btotal > cap) {
         if (threadIdx.x == 0) atomicOr(flags, FLAG_VECTOR_OVERFLOW);
     } else {
-#pragma unroll
-        for (int j = 0; j < MERGE_IPT; ++j) {
-            if (j < count) {
-                ok[base + ex + j] = out_k[j];
-                ov[base + ex + j] = out_v[j];
-            }
+        for (int i = threadIdx.x; i < btotal; i += NT) {
+            ok[base + i] = s_key[i];
+            ov[base + i] = s_val[i];
         }
     }
-    const double s0 = block_sum<NT>(lb);
-    const double s1 = block_sum<NT>(ln);
-    const double s2 = block_sum<NT>(nrm);
+    block_sum3<NT>(lb, ln, nrm);
     if (threadIdx.x == 0) {
-        partials[(int64_t)tile * 3 + 0] = s0;
-        partials[(int64_t)tile * 3 + 1] = s1;
-        partials[(int64_t)tile * 3 + 2] = s2;
+        partials[(int64_t)tile * 3 + 0] = lb;
+        partials[(int64_t)tile * 3 + 1] = ln;
+        partials[(int64_t)tile * 3 + 2] = nrm;
         if (tile == gridDim.x - 1) *total_out = base + btotal;
     }
 }
