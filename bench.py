@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--cpu-sample-entries", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    ap.add_argument("--state-file", default=None,
+                    help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,17 +119,28 @@ def main():
     sp = rimu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, dtau)
 
     step = 0
-    t_eq = time.perf_counter()
-    last_log = t_eq
-    while step < args.equilibration:
-        step += 1
-        st = eng.step(step, sp.shift, dtau)
-        strat.update(sp, st.norm)
-        if time.perf_counter() - last_log > 10:
-            log(f"equilibration step {step}: len {st.len:.3g} norm {st.norm:.3g} shift {sp.shift:.4f}")
-            last_log = time.perf_counter()
-    t_eq = time.perf_counter() - t_eq
-    log(f"equilibrated after {step} steps in {t_eq:.1f} s: len {st.len:.4g} norm {st.norm:.4g} shift {sp.shift:.4f}")
+    if args.state_file and os.path.exists(args.state_file):
+        z = np.load(args.state_file)
+        eng.upload(z["keys"], z["vals"])
+        sp.shift, sp.pnorm, step = float(z["shift"]), float(z["pnorm"]), int(z["step"])
+        log(f"resumed equilibrated state at step {step}: len {len(z['keys'])} shift {sp.shift:.4f}")
+        st = None
+    else:
+        t_eq = time.perf_counter()
+        last_log = t_eq
+        while step < args.equilibration:
+            step += 1
+            st = eng.step(step, sp.shift, dtau)
+            strat.update(sp, st.norm)
+            if time.perf_counter() - last_log > 10:
+                log(f"equilibration step {step}: len {st.len:.3g} norm {st.norm:.3g} shift {sp.shift:.4f}")
+                last_log = time.perf_counter()
+        t_eq = time.perf_counter() - t_eq
+        log(f"equilibrated after {step} steps in {t_eq:.1f} s: len {st.len:.4g} norm {st.norm:.4g} shift {sp.shift:.4f}")
+        if args.state_file:
+            k, v = eng.download()
+            with open(args.state_file, "wb") as f:
+                np.savez(f, keys=k, vals=v, shift=sp.shift, pnorm=sp.pnorm, step=step)
 
     for _ in range(args.warmup):
         step += 1
@@ -138,7 +151,7 @@ def main():
     attempts = 0.0
     S_tot = L_tot = Lnew_tot = 0.0
     alg_bytes = 0.0
-    L_prev = st.len
+    L_prev = len(eng)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step += 1

@@ -99,43 +99,12 @@ __device__ inline void lb_store(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by one thread of tile `tile` with the tile's own count.  Returns the exclusive
-// prefix (sum of counts of tiles 0..tile-1).  Every wait is bounded: if a predecessor
-// never publishes (which the tile-id ordering rules out) the error flag is raised and
-// the loop ends.
-__device__ inline uint64_t lookback_exclusive(uint64_t* status, uint32_t tile, uint64_t agg,
-                                              int* err_flag) {
-    if (tile == 0) {
-        lb_store(&status[0], LB_FLAG_PFX | agg);
-        return 0;
-    }
-    lb_store(&status[tile], LB_FLAG_AGG | agg);
-    uint64_t prefix = 0;
-    int64_t j = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    while (j >= 0) {
-        const uint64_t w = lb_load(&status[j]);
-        const uint64_t f = w >> 62;
-        if (f == 0) {
-            if (++spins > LB_SPIN_LIMIT) {
-                atomicOr(err_flag, 2);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        prefix += w & LB_MASK;
-        if (f == 2) break;
-        --j;
-    }
-    lb_store(&status[tile], LB_FLAG_PFX | (prefix + agg));
-    return prefix;
-}
-
 // Wavefront-parallel look-back, called by all 64 lanes of one wavefront of tile `tile`.
 // Each probe reads the status words of 64 predecessors at once (lane l reads tile
 // window_end - l); the nearest inclusive prefix ends the walk, aggregates in front of it
-// are added.  Returns the exclusive prefix in every lane.  Waits are bounded as above.
+// are added.  Returns the exclusive prefix in every lane.  Every wait is bounded: if a
+// predecessor never publishes (which the tile-id ordering rules out) the error flag is raised
+// and the loop ends.
 __device__ inline uint64_t lookback_exclusive_wave(uint64_t* status, uint32_t tile, uint64_t agg,
                                                    int* err_flag) {
     const int lane = threadIdx.x & (WAVE - 1);

@@ -128,7 +128,7 @@ struct rimu_engine {
     DevBuf<unsigned char> svals[2];
     DevBuf<uint64_t> status;
     DevBuf<int64_t> tile_first, splits;
-    DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev;
+    DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev, chunk_sums;
     DevBuf<unsigned char> misc;  // counters and totals (see layout below)
     // projector
     DevBuf<uint64_t> pkeys;
@@ -350,9 +350,27 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
             e->np + e->nh, e->terms.p);
         HIP_TRY(hipGetLastError());
     }
-    finalize_kernel<<<1, NT, 0, e->stream>>>(e->plan_p.p, plan_tiles, e->spawn_p.p, spawn_tiles,
-                                             e->merge_p.p, merge_tiles, e->terms.p, e->np, e->nh,
-                                             e->stats_dev.p);
+    {
+        PartialSet ps{};
+        // plan partials rows: deaths, clones, zombies, exact, inexact, attempts
+        const int plan_map[6] = {4, 5, 6, 1, 2, 0};
+        for (int r = 0; r < 6; ++r) {
+            ps.p[plan_map[r]] = e->plan_p.p + (int64_t)r * plan_tiles;
+            ps.n[plan_map[r]] = plan_tiles;
+        }
+        ps.p[3] = e->spawn_p.p;
+        ps.n[3] = spawn_tiles;
+        for (int r = 0; r < 3; ++r) {
+            ps.p[7 + r] = e->merge_p.p + (int64_t)r * merge_tiles;
+            ps.n[7 + r] = merge_tiles;
+        }
+        ps.p[10] = e->terms.p;
+        ps.n[10] = e->np;
+        ps.p[11] = e->terms.p + e->np;
+        ps.n[11] = e->nh;
+        partial_sums_kernel<<<dim3(PARTIAL_BLOCKS, NSTAT), NT, 0, e->stream>>>(ps, e->chunk_sums.p);
+        finalize_kernel<<<1, 64, 0, e->stream>>>(e->chunk_sums.p, PARTIAL_BLOCKS, e->stats_dev.p);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, 12 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
@@ -442,6 +460,7 @@ int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_
     if ((err = e->misc.alloc(sizeof(rimu_engine::Misc))) != hipSuccess) return bad("misc", err);
     if ((err = e->stats_dev.alloc(16)) != hipSuccess) return bad("stats", err);
     if ((err = e->terms.alloc(16)) != hipSuccess) return bad("terms", err);
+    if ((err = e->chunk_sums.alloc(PARTIAL_BLOCKS * NSTAT)) != hipSuccess) return bad("chunk sums", err);
     if ((err = e->spawn_p.alloc(16)) != hipSuccess) return bad("partials", err);
     if ((err = hipHostMalloc((void**)&e->h_misc, sizeof(rimu_engine::Misc), hipHostMallocDefault)) != hipSuccess)
         return bad("pinned misc", err);
@@ -463,7 +482,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->tile_first.release(); e->splits.release();
     e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release();
     e->rtmp.release(); e->status.release(); e->plan_p.release(); e->spawn_p.release();
-    e->merge_p.release(); e->terms.release(); e->stats_dev.release(); e->misc.release();
+    e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);
     if (e->h_stats) (void)hipHostFree(e->h_stats);

@@ -84,7 +84,7 @@ __device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
 // ---------------------------------------------------------------------------------------
 // plan: one thread per entry: the diagonal step and the number of spawn attempts of the
 // entry (thread L writes the trailing zero of the count array).
-// partials[tile*6 + {0..5}] = deaths, clones, zombies, exact, inexact, attempts.
+// partials[j * gridDim.x + tile], j = 0..5: deaths, clones, zombies, exact, inexact, attempts.
 // ---------------------------------------------------------------------------------------
 template <class V>
 __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __restrict__ keys,
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         const double s = block_sum<NT>(st[j]);
-        if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * 6 + j] = s;
+        if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -501,7 +501,8 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
 // the thread owning the segment's first position folds it sequentially:
 //   acc = 0 (+ diag) + spawn_1 + spawn_2 + ...
 // then applies ThresholdCompression and emits the survivor.  Outputs are compacted in
-// merged order by the look-back.  partials[tile*3 + {0,1,2}] = len_before, len, norm.
+// merged order by the look-back.  partials[j * gridDim.x + tile], j = 0..2: len_before,
+// len, norm.
 // ---------------------------------------------------------------------------------------
 template <class V>
 __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
@@ -631,9 +632,9 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     }
     block_sum3<NT>(lb, ln, nrm);
     if (threadIdx.x == 0) {
-        partials[(int64_t)tile * 3 + 0] = lb;
-        partials[(int64_t)tile * 3 + 1] = ln;
-        partials[(int64_t)tile * 3 + 2] = nrm;
+        partials[(int64_t)0 * gridDim.x + tile] = lb;
+        partials[(int64_t)1 * gridDim.x + tile] = ln;
+        partials[(int64_t)2 * gridDim.x + tile] = nrm;
         if (tile == gridDim.x - 1) *total_out = base + btotal;
     }
 }
@@ -661,39 +662,37 @@ __global__ __launch_bounds__(NT) void project_kernel(const uint64_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------
-// finalize: sums the per-tile partials in tile order into the step statistics.
+// Statistics: per-tile partial sums are reduced in two deterministic levels.  Level one
+// (grid PARTIAL_BLOCKS x 12) sums fixed contiguous chunks of each partial array; level
+// two (one workgroup) adds the chunk sums in order.
 // out: [0] attempts [1] exact [2] inexact [3] spawns [4] deaths [5] clones [6] zombies
 //      [7] len_before [8] len [9] norm [10] vproj [11] hproj
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void finalize_kernel(const double* __restrict__ plan_p, int64_t n_plan,
-                                                      const double* __restrict__ spawn_p, int64_t n_spawn,
-                                                      const double* __restrict__ merge_p, int64_t n_merge,
-                                                      const double* __restrict__ terms, int64_t np,
-                                                      int64_t nh, double* __restrict__ out) {
-    double acc[12];
-#pragma unroll
-    for (int j = 0; j < 12; ++j) acc[j] = 0;
-    for (int64_t t = threadIdx.x; t < n_plan; t += NT) {
-        acc[4] += plan_p[t * 6 + 0];
-        acc[5] += plan_p[t * 6 + 1];
-        acc[6] += plan_p[t * 6 + 2];
-        acc[1] += plan_p[t * 6 + 3];
-        acc[2] += plan_p[t * 6 + 4];
-        acc[0] += plan_p[t * 6 + 5];
-    }
-    for (int64_t t = threadIdx.x; t < n_spawn; t += NT) acc[3] += spawn_p[t];
-    for (int64_t t = threadIdx.x; t < n_merge; t += NT) {
-        acc[7] += merge_p[t * 3 + 0];
-        acc[8] += merge_p[t * 3 + 1];
-        acc[9] += merge_p[t * 3 + 2];
-    }
-    for (int64_t t = threadIdx.x; t < np; t += NT) acc[10] += terms[t];
-    for (int64_t t = threadIdx.x; t < nh; t += NT) acc[11] += terms[np + t];
-#pragma unroll
-    for (int j = 0; j < 12; ++j) {
-        const double s = block_sum<NT>(acc[j]);
-        if (threadIdx.x == 0) out[j] = s;
-    }
+constexpr int NSTAT = 12;
+constexpr int PARTIAL_BLOCKS = 64;
+
+struct PartialSet {
+    const double* p[NSTAT];
+    int64_t n[NSTAT];
+};
+
+__global__ __launch_bounds__(NT) void partial_sums_kernel(PartialSet ps, double* __restrict__ chunk_sums) {
+    const int j = blockIdx.y;
+    const int64_t n = ps.n[j];
+    const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
+    double acc = 0;
+    for (int64_t t = lo + threadIdx.x; t < hi; t += NT) acc += ps.p[j][t];
+    const double s = block_sum<NT>(acc);
+    if (threadIdx.x == 0) chunk_sums[(int64_t)j * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(64) void finalize_kernel(const double* __restrict__ chunk_sums, int nchunks,
+                                                      double* __restrict__ out) {
+    const int j = threadIdx.x;
+    if (j >= NSTAT) return;
+    double acc = 0;
+    for (int b = 0; b < nchunks; ++b) acc += chunk_sums[j * nchunks + b];
+    out[j] = acc;
 }
 
 }  // namespace rimu
