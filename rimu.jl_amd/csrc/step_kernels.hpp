@@ -500,13 +500,9 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
 // unique).  Merged order: by key, B before A on equal keys.  Each address is one segment;
 // the thread owning the segment's first position folds it sequentially:
 //   acc = 0 (+ diag) + spawn_1 + spawn_2 + ...
-// then applies ThresholdCompression and emits the survivor.
-// A tile covers TILE merged positions (split precomputed by merge_split_kernel).  Every
-// element of the tile finds its merged position by a binary search in the other list
-// (lanes take consecutive elements), and is scattered into a merged LDS array; heads then
-// fold their runs, lanes again on consecutive positions.  Survivors are compacted in
-// position order (wavefront ballots, then the look-back across tiles).
-// partials[j * gridDim.x + tile], j = 0..2: len_before, len, norm.
+// then applies ThresholdCompression and emits the survivor.  Outputs are compacted in
+// merged order by the look-back.  partials[j * gridDim.x + tile], j = 0..2: len_before,
+// len, norm.
 // ---------------------------------------------------------------------------------------
 template <class V>
 __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
@@ -519,83 +515,69 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                                                    int* flags, double* __restrict__ partials,
                                                    const int64_t* __restrict__ splits) {
     constexpr int TILE = NT * MERGE_IPT;
-    constexpr int NW = NT / WAVE;
-    __shared__ uint64_t s_key[TILE];  // inputs: A range then B range; later the survivors
+    // the tile's A range followed by its B range (na + nb <= TILE)
+    __shared__ uint64_t s_key[TILE];
     __shared__ V s_val[TILE];
-    __shared__ uint64_t m_key[TILE];  // merged order
-    __shared__ V m_val[TILE];
-    __shared__ int s_cnt[MERGE_IPT * NW];
-    __shared__ int s_total;
     __shared__ uint64_t s_base;
 
     const uint32_t tile = next_tile_id(tile_counter);
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = threadIdx.x / WAVE;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t total = S + L;
     const int64_t d0 = (int64_t)tile * TILE;
     const int64_t d1 = min(d0 + TILE, total);
     const int64_t a0 = splits[tile], a1 = splits[tile + 1];
     const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-    const int nm = na + nb;
     for (int i = threadIdx.x; i < na; i += NT) { s_key[i] = ak[a0 + i]; s_val[i] = av[a0 + i]; }
     for (int i = threadIdx.x; i < nb; i += NT) { s_key[na + i] = bk[b0 + i]; s_val[na + i] = bv[b0 + i]; }
     __syncthreads();
-    // merged position of every element: A[i] follows the B elements with key <= A[i];
-    // B[j] follows the A elements with key < B[j]
-    for (int e = threadIdx.x; e < nm; e += NT) {
-        const uint64_t key = s_key[e];
-        int rank;
-        if (e < na) {
-            int lo = na, hi = nm;  // upper bound of key in B
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_key[mid] <= key) lo = mid + 1; else hi = mid;
-            }
-            rank = e + (lo - na);
-        } else {
-            int lo = 0, hi = na;  // lower bound of key in A
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_key[mid] < key) lo = mid + 1; else hi = mid;
-            }
-            rank = (e - na) + lo;
+    const uint64_t* s_a = s_key;
+    const uint64_t* s_b = s_key + na;
+
+    // this thread's positions within the tile
+    const int dd = (int)threadIdx.x * MERGE_IPT;
+    const int n_here = (int)max((int64_t)0, min((int64_t)MERGE_IPT, (d1 - d0) - dd));
+    int ia = 0;
+    {
+        int lo = max(0, dd - nb), hi = min(dd, na);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
         }
-        m_key[rank] = key;
-        m_val[rank] = s_val[e];
+        ia = lo;
     }
-    // key of the merged element just before the tile
+    int ib = dd - ia;
+    // key of the merged element just before this thread's first position
     bool have_prev = false;
     uint64_t prev = 0;
-    if (d0 > 0) {
+    if (dd > 0) {
+        have_prev = true;
+        const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0;
+        const uint64_t pb = ib > 0 ? s_b[ib - 1] : 0;
+        prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
+    } else if (d0 > 0) {
         have_prev = true;
         const uint64_t pa = a0 > 0 ? ak[a0 - 1] : 0;
         const uint64_t pb = b0 > 0 ? bk[b0 - 1] : 0;
         prev = (a0 > 0 && b0 > 0) ? max(pa, pb) : (a0 > 0 ? pa : pb);
     }
-    __syncthreads();
 
     uint64_t out_k[MERGE_IPT];
     V out_v[MERGE_IPT];
-    int out_pos[MERGE_IPT];
-    bool out_ok[MERGE_IPT];
+    int count = 0;
     double lb = 0, ln = 0, nrm = 0;
 #pragma unroll
-    for (int it = 0; it < MERGE_IPT; ++it) {
-        const int m = it * NT + threadIdx.x;
-        bool ok_here = false;
-        uint64_t key = 0;
-        V acc = (V)0;
-        if (m < nm) {
-            key = m_key[m];
-            const bool head = (m == 0) ? (!have_prev || key != prev) : (key != m_key[m - 1]);
-            if (head) {
-                // the run: the diagonal value (B sorts first) then the spawns in list order
-                int p = m;
-                while (p < nm && m_key[p] == key) { acc = acc + m_val[p]; ++p; }
-                if (p == nm) {  // the run may continue in A past the tile
-                    int64_t ga = a1;
+    for (int j = 0; j < MERGE_IPT; ++j) {
+        if (j < n_here) {
+            const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
+            const uint64_t key = take_b ? s_b[ib] : s_a[ia];
+            if (!have_prev || key != prev) {
+                // segment head: fold diag (if any) then the run of spawns in A
+                V acc = (V)0;
+                if (take_b) acc = acc + s_val[na + ib];
+                int la = ia;
+                while (la < na && s_a[la] == key) { acc = acc + s_val[la]; ++la; }
+                if (la == na) {  // the run may continue past the tile
+                    int64_t ga = a0 + la;
                     while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
                 }
                 if (acc != (V)0) {
@@ -611,39 +593,31 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                         }
                     }
                     if (acc != (V)0) {
-                        ok_here = true;
+                        out_k[count] = key;
+                        out_v[count] = acc;
+                        ++count;
                         ln += 1;
                         nrm += fabs((double)acc);
                     }
                 }
             }
+            prev = key;
+            have_prev = true;
+            if (take_b) ++ib; else ++ia;
         }
-        out_k[it] = key;
-        out_v[it] = acc;
-        out_ok[it] = ok_here;
-        const uint64_t bal = __ballot(ok_here);
-        out_pos[it] = __popcll(bal & lt_mask);
-        if (lane == 0) s_cnt[it * NW + wid] = __popcll(bal);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int q = 0; q < MERGE_IPT * NW; ++q) { const int c = s_cnt[q]; s_cnt[q] = run; run += c; }
-        s_total = run;
-    }
-    __syncthreads();
-    const int btotal = s_total;
+    int btotal;
+    const int ex = block_exclusive_scan<int, NT>(count, &btotal);  // ends with a barrier
     if (threadIdx.x < WAVE) {
         const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)btotal, flags);
         if (threadIdx.x == 0) s_base = b;
     }
-    // stage the survivors in LDS (inputs are no longer read) for coalesced writes
+    // stage the survivors in LDS (the input keys are no longer read) for coalesced writes
 #pragma unroll
-    for (int it = 0; it < MERGE_IPT; ++it) {
-        if (out_ok[it]) {
-            const int q = s_cnt[it * NW + wid] + out_pos[it];
-            s_key[q] = out_k[it];
-            s_val[q] = out_v[it];
+    for (int j = 0; j < MERGE_IPT; ++j) {
+        if (j < count) {
+            s_key[ex + j] = out_k[j];
+            s_val[ex + j] = out_v[j];
         }
     }
     __syncthreads();
