@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--cpu-sample-entries", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for HBM byte counters)")
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -101,6 +103,16 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 2 ** 27  # 2^27 doubles = 1 GiB read + 1 GiB written per copy
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": 5}))
+        return
 
     rimu = load_package()
     if rimu.device_count() < 1:
