@@ -30,6 +30,12 @@ from __graft_entry__ import load_package  # noqa: E402
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
 
+# HBM bytes per launch measured on this workload (target 1e8 walkers) with rocprofv3
+# FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a plain 1 GiB copy (fetch x2.0,
+# write x1.0), expressed as a ratio to the kernel's algorithmic bytes.
+MEASURED_TRAFFIC_RATIO = {"radix_scatter": 2.462 / 2.283, "merge": 2.290 / 2.295,
+                          "spawn": 2.258 / 1.962, "plan": 1.127 / 1.096}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -194,8 +200,11 @@ def main():
         bytes_per_launch = kernel_alg_bytes(dom, S_tot / K, L_tot / K, Lnew_tot / K)
         avg_s = ms / launches / 1e3
         achieved = bytes_per_launch / avg_s
+        traffic = None
+        if dom in MEASURED_TRAFFIC_RATIO and args.target == 1e8:
+            traffic = MEASURED_TRAFFIC_RATIO[dom] * bytes_per_launch
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
                     "launches_per_step": per_step_launches}
         for name in kernels:
