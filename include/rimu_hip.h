@@ -159,6 +159,49 @@ int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const 
                           int64_t n, double* diag, int64_t* num_offdiag, uint64_t* new_addr,
                           double* offdiag_value);
 
+/* ---------------------------------------------------------------------------------------
+ * Deterministic products on the ranked basis (the IsDeterministic path at full dimension)
+ * --------------------------------------------------------------------------------------- */
+typedef struct rimu_dense rimu_dense;
+
+/* binomial(N+M-1, N): number of N-boson M-mode addresses (number_conserving_bose_dimension,
+ * Hamiltonians/abstract.jl:103-110). */
+int rimu_basis_size(int32_t M, int32_t N, int64_t* dim);
+
+/* The ranked basis on the device for nrhs HubbardReal1D Hamiltonians that share M and N
+ * (e.g. one per u).  Row r of every vector belongs to the address of colex rank r, which is
+ * the r-th address in ascending numeric order of the bit strings (the sorted basis of
+ * ExactDiagonalization/basis_fock.jl).  Vectors are row-major [dim][nrhs], nrhs <= 8. */
+int rimu_dense_create(const rimu_model* models, int32_t nrhs, rimu_dense** out);
+int rimu_dense_destroy(rimu_dense* d);
+int rimu_dense_dim(rimu_dense* d, int64_t* dim);
+/* The dim addresses, ascending. */
+int rimu_dense_basis(rimu_dense* d, uint64_t* keys);
+
+/* w_i = (a_i + b_i H_i) v_i for every right-hand side at once: mul!(w, H, v)
+ * (DictVectors/abstractdvec.jl:275-284) for a = 0, b = 1, and one IsDeterministic step with
+ * T = 1 + dt (S - H) (styles.jl:101-105, fciqmc.jl:93-112) for a = 1 + dt S, b = -dt. */
+int rimu_dense_apply(rimu_dense* d, const double* a, const double* b, const double* v, double* w);
+
+/* Pullback of y_i = (a_i + b_i H_i) v_i, with H_i = u_i D + t_i K (D = sum n(n-1)/2 on the
+ * diagonal, K the hopping with unit amplitude): given the cotangent ybar,
+ *   vbar_i = (a_i + b_i H_i) ybar_i   (H is real symmetric),
+ *   ubar_i = b_i <ybar_i|D|v_i>,   tbar_i = b_i <ybar_i|K|v_i>.
+ * No counterpart in the reference: it is the rrule a differentiable mul! needs
+ * (ChainRulesCore), for callers that differentiate through the projector. */
+int rimu_dense_pullback(rimu_dense* d, const double* a, const double* b, const double* v,
+                        const double* ybar, double* vbar, double* ubar, double* tbar);
+
+/* K deterministic steps v <- (1 + dt (S_i - H_i)) v from v0 with fixed shifts S_i, then the
+ * Rayleigh quotient E_i = <v|H_i|v>/<v|v> (rayleigh_quotient, Hamiltonians/abstract.jl:153)
+ * and dE_i/du_i, dE_i/dt_i by the adjoint of the whole projection: a reverse sweep that
+ * recomputes each segment of ckpt steps from a stored checkpoint.  hf_u, hf_t receive
+ * <v|D|v>/<v|v> and <v|K|v>/<v|v> of the final vector (Hellmann-Feynman values), v_final the
+ * final vector.  Output pointers may be null.  (No counterpart in the reference.) */
+int rimu_dense_projector(rimu_dense* d, int64_t K, double dtau, const double* shift, int64_t ckpt,
+                         const double* v0, double* energy, double* grad_u, double* grad_t,
+                         double* hf_u, double* hf_t, double* v_final);
+
 #ifdef __cplusplus
 }
 #endif

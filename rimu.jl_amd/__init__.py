@@ -22,5 +22,6 @@ from .strategies import (ConstantTimeStep, DoubleLogUpdate, DoubleLogUpdateAfter
                          DontUpdate, LogUpdate, LogUpdateAfterTargetWalkers, ProjectedEnergy)
 from .pmc import (PMCSimulation, ProjectorMonteCarloProblem, default_starting_vector, init,
                   lomc, solve)
+from .dense import DenseHamiltonians, basis_size
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -24,6 +24,8 @@ EXPORTS = (
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
     "rimu_set_projector", "rimu_step", "rimu_engine_set_timing", "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
+    "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
+    "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
 )
 
 
@@ -83,6 +85,14 @@ def lib():
         L.rimu_engine_last_traffic.argtypes = [vp, P(dbl), P(dbl)]
         L.rimu_annihilate.argtypes = [i32, vp, vp, i64, vp, vp, i64, P(i64)]
         L.rimu_hamiltonian_eval.argtypes = [P(Model), vp, vp, i64, vp, vp, vp, vp]
+        L.rimu_basis_size.argtypes = [i32, i32, P(i64)]
+        L.rimu_dense_create.argtypes = [P(Model), i32, P(vp)]
+        L.rimu_dense_destroy.argtypes = [vp]
+        L.rimu_dense_dim.argtypes = [vp, P(i64)]
+        L.rimu_dense_basis.argtypes = [vp, vp]
+        L.rimu_dense_apply.argtypes = [vp, vp, vp, vp, vp]
+        L.rimu_dense_pullback.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.rimu_dense_projector.argtypes = [vp, i64, dbl, vp, i64, vp, vp, vp, vp, vp, vp, vp]
         for name in EXPORTS:
             if name != "rimu_last_error":
                 getattr(L, name).restype = ctypes.c_int

@@ -17,47 +17,16 @@
 #include <vector>
 
 #include "../../include/rimu_hip.h"
+#include "capi_util.hpp"
 #include "step_kernels.hpp"
 
 using namespace rimu;
 
-namespace {
-
+namespace rimu {
 thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t _e = (expr);                                                           \
-        if (_e != hipSuccess)                                                             \
-            return fail(RIMU_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e));  \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t alloc(size_t count) {
-        if (count <= n) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) n = count;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+namespace {
 
 struct KernelTimer {
     bool enabled = false;
