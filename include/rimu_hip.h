@@ -202,6 +202,58 @@ int rimu_dense_projector(rimu_dense* d, int64_t K, double dtau, const double* sh
                          const double* v0, double* energy, double* grad_u, double* grad_t,
                          double* hf_u, double* hf_t, double* v_final);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched engine: many independent IsStochasticInteger FCIQMC runs of HubbardReal1D, one
+ * workgroup per run, one kernel launch per step for all runs (captured in a HIP graph).
+ * Each run is Rimu's lomc!/ProjectorMonteCarloProblem loop with DoubleLogUpdate
+ * (shiftstrategy.jl:174-181) and a ProjectedEnergy onto its start address
+ * (poststepstrategy.jl:82-121); the shift update happens on the device.
+ * --------------------------------------------------------------------------------------- */
+typedef struct rimu_batch rimu_batch;
+
+typedef struct rimu_batch_run {
+    int32_t M;
+    int32_t N;
+    double u;
+    double t;
+    uint64_t seed;
+    uint64_t start_address;     /* BoseFS bit pattern, population start_population */
+    int64_t start_population;
+    double shift;               /* initial shift */
+    double target_walkers;      /* DoubleLogUpdate target */
+    double zeta;                /* DoubleLogUpdate damping (0.08) */
+    double xi;                  /* DoubleLogUpdate restoring force (zeta^2/4) */
+    double dtau;
+    int64_t max_length;         /* vector capacity; exceeding it aborts the run */
+} rimu_batch_run;
+
+/* Per-step history of every run, row-major [steps][nruns]: shift, norm (walkers), len,
+ * vproj, hproj, spawn_attempts.  history_steps bounds the number of recorded steps. */
+int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history_steps,
+                      rimu_batch** out);
+int rimu_batch_destroy(rimu_batch* b);
+/* Launch configuration: workgroup size (256, 512 or 1024) and radix digit width (4..8). */
+int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits);
+/* Advance every live run by nsteps; steps_per_graph launches are captured once in a HIP
+ * graph and replayed.  A run whose population dies or whose vector or spawn buffer would
+ * overflow stops there (its status says why); the others continue. */
+int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph);
+/* Steps recorded so far and the history arrays (each [steps][nruns], may be null). */
+int rimu_batch_history(rimu_batch* b, int64_t* nsteps, double* shift, double* norm, double* len,
+                       double* vproj, double* hproj, double* attempts);
+/* Status of every run: 0 running, 1 population dead, 2 vector overflow (max_length),
+ * 3 spawn buffer overflow; and the current length of each vector. */
+int rimu_batch_status(rimu_batch* b, int32_t* status, int64_t* len);
+/* Time `trial_steps` steps for every (block size, radix width) candidate from a snapshot of
+ * the current state (restored after each trial; results do not depend on the choice),
+ * select the fastest.  ms receives the per-step time of each candidate in the order
+ * block in {256, 512, 1024} x radix in {4, 6, 8}; best_block/best_radix the choice. */
+int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
+                        int32_t* best_radix, double* ms);
+/* The vector of one run (ascending keys, int64 values). */
+int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals, int64_t cap,
+                      int64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

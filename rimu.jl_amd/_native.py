@@ -26,6 +26,8 @@ EXPORTS = (
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
     "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
+    "rimu_batch_create", "rimu_batch_destroy", "rimu_batch_set_config", "rimu_batch_step",
+    "rimu_batch_history", "rimu_batch_status", "rimu_batch_autotune", "rimu_batch_vector",
 )
 
 
@@ -49,6 +51,14 @@ class Style(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("compression", ctypes.c_int32),
                 ("compression_threshold", ctypes.c_double), ("proj_threshold", ctypes.c_double),
                 ("rel_threshold", ctypes.c_double), ("abs_threshold", ctypes.c_double)]
+
+
+class BatchRun(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int32), ("N", ctypes.c_int32), ("u", ctypes.c_double),
+                ("t", ctypes.c_double), ("seed", ctypes.c_uint64), ("start_address", ctypes.c_uint64),
+                ("start_population", ctypes.c_int64), ("shift", ctypes.c_double),
+                ("target_walkers", ctypes.c_double), ("zeta", ctypes.c_double),
+                ("xi", ctypes.c_double), ("dtau", ctypes.c_double), ("max_length", ctypes.c_int64)]
 
 
 class StepStats(ctypes.Structure):
@@ -93,6 +103,14 @@ def lib():
         L.rimu_dense_apply.argtypes = [vp, vp, vp, vp, vp]
         L.rimu_dense_pullback.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.rimu_dense_projector.argtypes = [vp, i64, dbl, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.rimu_batch_create.argtypes = [P(BatchRun), i32, i64, P(vp)]
+        L.rimu_batch_destroy.argtypes = [vp]
+        L.rimu_batch_set_config.argtypes = [vp, i32, i32]
+        L.rimu_batch_step.argtypes = [vp, i64, i32]
+        L.rimu_batch_history.argtypes = [vp, P(i64), vp, vp, vp, vp, vp, vp]
+        L.rimu_batch_status.argtypes = [vp, vp, vp]
+        L.rimu_batch_autotune.argtypes = [vp, i64, P(i32), P(i32), vp]
+        L.rimu_batch_vector.argtypes = [vp, i32, vp, vp, i64, P(i64)]
         for name in EXPORTS:
             if name != "rimu_last_error":
                 getattr(L, name).restype = ctypes.c_int

@@ -13,7 +13,7 @@ import sys
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "librimu_hip.so"
-SOURCES = [CSRC / "rimu_engine.hip", CSRC / "dense_hv.hip"]
+SOURCES = [CSRC / "rimu_engine.hip", CSRC / "dense_hv.hip", CSRC / "batched.hip"]
 DEPENDS = SOURCES + sorted(CSRC.glob("*.hpp")) + [HERE.parent / "include" / "rimu_hip.h"]
 
 # -ffp-contract=off: a*b+c stays two rounded operations (bit-reproducible against the

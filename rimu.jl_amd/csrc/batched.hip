@@ -1,0 +1,674 @@
+// batched.hip -- many independent IsStochasticInteger FCIQMC runs of HubbardReal1D, one
+// workgroup per run and one kernel launch per step for all runs (C ABI rimu_batch_*).
+//
+// A workgroup performs a whole step of its run with the same algorithm as the plain engine,
+// scaled down to one workgroup: diagonal step and spawn attempts (attempt-parallel, ballot
+// compaction into the run's spawn buffer), a stable LSD radix sort of the spawns
+// (histogram in LDS, ranks from wavefront ballots, RB-bit digits), a merge of the sorted
+// spawns with the diagonal survivors (sequential tiles, so output offsets need no
+// look-back), the norm, the ProjectedEnergy values and the DoubleLogUpdate shift update
+// (shiftstrategy.jl:174-181).  The step number, shift and lengths live in device memory, so
+// a sequence of launches can be captured once in a HIP graph and replayed.
+//
+// Random draws are addressed by (seed, address, attempt, step) and integer walker sums are
+// exact, so the result of a step does not depend on the workgroup size or radix width.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "bosefs_device.hpp"
+#include "capi_util.hpp"
+#include "device_common.hpp"
+
+using namespace rimu;
+
+namespace {
+
+constexpr int MIPT = 2;          // merged positions per thread per merge tile
+constexpr int MAXPROJ = 160;     // projector entries per run (start address and neighbours)
+constexpr int NHIST = 6;         // shift, norm, len, vproj, hproj, attempts
+
+enum { RUN_OK = 0, RUN_DEAD = 1, RUN_VEC_OVERFLOW = 2, RUN_SPAWN_OVERFLOW = 3 };
+
+struct RunState {
+    int M, N, B, status;
+    double u, t, dtau, shift, pnorm, target, zeta, xi;
+    uint64_t seed;
+    int64_t len, cur, step, nrec;     // nrec: history rows written
+    int64_t capV, capS, offV, offS, offP, nproj;
+};
+
+struct Pools {
+    uint64_t* vk[2];
+    int64_t* vv[2];
+    int64_t* dv;
+    uint64_t* sk[2];
+    int64_t* sv[2];
+    const uint64_t* pk;
+    const double* pc;    // coefficients of hproj (vproj uses the start address, pc index 0 row)
+    double* hist;        // [NHIST][hist_cap][nruns]
+    int64_t hist_cap;
+};
+
+__device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
+
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ runs, int nruns, Pools P,
+                                                        int radix_bits) {
+    constexpr int NW = NT / WAVE;
+    constexpr int TILE = NT * MIPT;
+    __shared__ int64_t s_aoff[NT + 1];
+    __shared__ int s_wc[NW];
+    __shared__ int64_t s_S;
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_wcnt[NW][256];
+    __shared__ uint64_t s_a[TILE];
+    __shared__ uint64_t s_b[TILE];
+    __shared__ int64_t s_split[2];
+    __shared__ int s_flag;
+
+    const int run = blockIdx.x;
+    RunState& st = runs[run];
+    if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+
+    const int M = st.M, B = st.B;
+    const double u = st.u, t = st.t, dtau = st.dtau, shift = st.shift;
+    const uint64_t seed = st.seed;
+    const int64_t L = st.len, capV = st.capV, capS = st.capS;
+    const int cur = (int)st.cur;
+    const int64_t step = st.step + 1;
+    const uint64_t* vk = P.vk[cur] + st.offV;
+    const int64_t* vv = P.vv[cur] + st.offV;
+    uint64_t* nk = P.vk[1 - cur] + st.offV;
+    int64_t* nv = P.vv[1 - cur] + st.offV;
+    int64_t* dv = P.dv + st.offV;
+    uint64_t* sk0 = P.sk[0] + st.offS;
+    int64_t* sv0 = P.sv[0] + st.offS;
+    uint64_t* sk1 = P.sk[1] + st.offS;
+    int64_t* sv1 = P.sv[1] + st.offS;
+
+    if (threadIdx.x == 0) { s_S = 0; s_flag = 0; }
+    __syncthreads();
+
+    // ---- diagonal step and spawning, NT entries at a time --------------------------------
+    double st_att = 0, st_spawn = 0, st_death = 0, st_clone = 0, st_zombie = 0;
+    for (int64_t c0 = 0; c0 < L; c0 += NT) {
+        const int64_t e = c0 + threadIdx.x;
+        int64_t n_att = 0;
+        if (e < L) {
+            const uint64_t k = vk[e];
+            const int64_t v = vv[e];
+            const double hd = u * (double)bose_hubbard_interaction(k) / 2;
+            const double x = (1 - dtau * (hd - shift)) * (double)v;
+            const U4 r = draw(seed, k, 0, step, TAG_DIAG);
+            const int64_t res = isgn(x) * (int64_t)floor(fabs(x) + u53(r.x, r.y));
+            dv[e] = res;
+            if (res > v) st_clone += (double)(res - v);
+            else if (isgn((double)res) != isgn((double)v)) { st_death += fabs((double)v); st_zombie += fabs((double)res); }
+            else st_death += fabs((double)(res - v));
+            n_att = v < 0 ? -v : v;
+        }
+        int64_t T;
+        const int64_t ex = block_exclusive_scan<int64_t, NT>(n_att, &T);
+        s_aoff[threadIdx.x] = ex;
+        if (threadIdx.x == 0) s_aoff[NT] = T;
+        st_att += (double)n_att;
+        __syncthreads();
+        for (int64_t a0 = 0; a0 < T; a0 += NT) {
+            const int64_t att = a0 + threadIdx.x;
+            bool ok = false;
+            uint64_t nbk = 0;
+            int64_t nbv = 0;
+            if (att < T) {
+                int lo = 0, hi = NT;  // s_aoff[lo] <= att < s_aoff[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_aoff[mid] <= att) lo = mid; else hi = mid;
+                }
+                const uint64_t k = vk[c0 + lo];
+                const int64_t v = vv[c0 + lo];
+                const int64_t j = att - s_aoff[lo];
+                const int64_t n_od = 2 * (int64_t)num_occupied(k);
+                const int64_t na = v < 0 ? -v : v;
+                const double magnitude = (double)v / (double)na;
+                const double prob = 1.0 / (double)n_od;
+                const U4 r = draw(seed, k, (uint32_t)j, step, TAG_SPAWN);
+                int64_t prod;
+                nbk = real1d_hop(k, pick1(r.x, n_od), M, B, &prod);
+                const double mel = -(-t * sqrt((double)prod)) * dtau;
+                const double x = mel * magnitude / prob;
+                nbv = isgn(x) * (int64_t)floor(fabs(x) + u53(r.y, r.z));
+                st_spawn += fabs((double)nbv);
+                ok = nbv != 0;
+            }
+            const uint64_t bal = __ballot(ok);
+            if (lane == 0) s_wc[wid] = __popcll(bal);
+            __syncthreads();
+            int64_t before = s_S;
+            int tot = 0;
+            for (int w = 0; w < NW; ++w) {
+                if (w < wid) before += s_wc[w];
+                tot += s_wc[w];
+            }
+            if (ok) {
+                const int64_t pos = before + __popcll(bal & lt_mask);
+                if (pos < capS) { sk0[pos] = nbk; sv0[pos] = nbv; }
+                else s_flag = RUN_SPAWN_OVERFLOW;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_S += tot;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    const int64_t S = min(s_S, capS);
+
+    // ---- stable LSD radix sort of the spawn records on bits [0, B) ------------------------
+    const int nbins = 1 << radix_bits;
+    uint64_t* srck = sk0;
+    int64_t* srcv = sv0;
+    uint64_t* dstk = sk1;
+    int64_t* dstv = sv1;
+    for (int sh = 0; sh < B; sh += radix_bits) {
+        const uint64_t dmask = (uint64_t)(nbins - 1);
+        for (int d = threadIdx.x; d < nbins; d += NT) s_hist[d] = 0;
+        for (int j = threadIdx.x; j < NW * nbins; j += NT) s_wcnt[j / nbins][j % nbins] = 0;
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < S; i += NT) atomicAdd(&s_hist[(srck[i] >> sh) & dmask], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {  // exclusive digit offsets (at most 256 bins)
+            uint32_t run_ = 0;
+            for (int d = 0; d < nbins; ++d) { const uint32_t c = s_hist[d]; s_hist[d] = run_; run_ += c; }
+        }
+        __syncthreads();
+        for (int64_t t0 = 0; t0 < S; t0 += NT) {
+            const int64_t i = t0 + threadIdx.x;
+            const bool valid = i < S;
+            const uint64_t key = valid ? srck[i] : 0;
+            const int64_t val = valid ? srcv[i] : 0;
+            const uint32_t digit = (uint32_t)((key >> sh) & dmask);
+            uint64_t peers = __ballot(valid);
+            for (int b = 0; b < radix_bits; ++b) {
+                const uint64_t bb = __ballot((digit >> b) & 1u);
+                peers &= ((digit >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
+            if (valid && in_wave == 0) s_wcnt[wid][digit] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = s_hist[digit] + in_wave;
+                for (int w = 0; w < wid; ++w) pos += s_wcnt[w][digit];
+                dstk[pos] = key;
+                dstv[pos] = val;
+            }
+            __syncthreads();
+            for (int d = threadIdx.x; d < nbins; d += NT) {
+                uint32_t add = 0;
+                for (int w = 0; w < NW; ++w) { add += s_wcnt[w][d]; s_wcnt[w][d] = 0; }
+                s_hist[d] += add;
+            }
+            __syncthreads();
+        }
+        uint64_t* tk = srck; srck = dstk; dstk = tk;
+        int64_t* tv = srcv; srcv = dstv; dstv = tv;
+    }
+
+    // ---- merge the sorted spawns (A) with the diagonal survivors (B), fold, count ------
+    // merged order: by key, B before A on equal keys; each address folds diag + spawns.
+    const uint64_t* ak = srck;
+    const int64_t* av = srcv;
+    int64_t out = 0;
+    double nrm = 0;
+    for (int64_t d0 = 0; d0 < S + L; d0 += TILE) {
+        const int64_t d1 = min(d0 + TILE, S + L);
+        if (threadIdx.x < 2) {
+            const int64_t d = threadIdx.x == 0 ? d0 : d1;
+            int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (ak[mid] < vk[d - mid - 1]) lo = mid + 1; else hi = mid;
+            }
+            s_split[threadIdx.x] = lo;
+        }
+        __syncthreads();
+        const int64_t a0 = s_split[0], a1 = s_split[1];
+        const int64_t b0 = d0 - a0, b1 = d1 - a1;
+        const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+        for (int i = threadIdx.x; i < na; i += NT) s_a[i] = ak[a0 + i];
+        for (int i = threadIdx.x; i < nb; i += NT) s_b[i] = vk[b0 + i];
+        __syncthreads();
+        const int dd = threadIdx.x * MIPT;
+        const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
+        int ia;
+        {
+            int lo = max(0, dd - nb), hi = min(dd, na);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
+            }
+            ia = lo;
+        }
+        int ib = dd - ia;
+        bool have_prev = false;
+        uint64_t prev = 0;
+        if (dd > 0 && n_here > 0) {
+            have_prev = true;
+            const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0, pb = ib > 0 ? s_b[ib - 1] : 0;
+            prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
+        } else if (d0 > 0 && n_here > 0) {
+            have_prev = true;
+            const uint64_t pa = a0 > 0 ? ak[a0 - 1] : 0, pb = b0 > 0 ? vk[b0 - 1] : 0;
+            prev = (a0 > 0 && b0 > 0) ? max(pa, pb) : (a0 > 0 ? pa : pb);
+        }
+        uint64_t ok_k[MIPT];
+        int64_t ok_v[MIPT];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < MIPT; ++j) {
+            if (j < n_here) {
+                const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
+                const uint64_t key = take_b ? s_b[ib] : s_a[ia];
+                if (!have_prev || key != prev) {
+                    int64_t acc = 0;
+                    if (take_b) acc += dv[b0 + ib];
+                    int64_t ga = a0 + ia;
+                    while (ga < S && ak[ga] == key) { acc += av[ga]; ++ga; }
+                    if (acc != 0) {
+                        ok_k[cnt] = key;
+                        ok_v[cnt] = acc;
+                        ++cnt;
+                        nrm += fabs((double)acc);
+                    }
+                }
+                prev = key;
+                have_prev = true;
+                if (take_b) ++ib; else ++ia;
+            }
+        }
+        int tot;
+        const int ex = block_exclusive_scan<int, NT>(cnt, &tot);
+        if (out + tot > capV) {
+            if (threadIdx.x == 0) s_flag = RUN_VEC_OVERFLOW;
+        } else {
+            for (int j = 0; j < cnt; ++j) {
+                nk[out + ex + j] = ok_k[j];
+                nv[out + ex + j] = ok_v[j];
+            }
+        }
+        out += tot;
+        __syncthreads();
+    }
+    const int64_t Lnew = min(out, capV);
+
+    // ---- ProjectedEnergy, statistics and the shift update -----------------------------
+    double vp = 0, hp = 0;
+    {
+        const int64_t np = st.nproj;
+        for (int64_t q = threadIdx.x; q < np; q += NT) {
+            const uint64_t key = P.pk[st.offP + q];
+            int64_t lo = 0, hi = Lnew;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (nk[mid] < key) lo = mid + 1; else hi = mid;
+            }
+            const double x = (lo < Lnew && nk[lo] == key) ? (double)nv[lo] : 0.0;
+            hp += P.pc[st.offP + q] * x;
+            if (q == 0) vp = x;   // entry 0 is the start address itself
+        }
+    }
+    block_sum3<NT>(nrm, vp, hp);
+    const double att_sum = block_sum<NT>(st_att);
+    (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie;
+    if (threadIdx.x == 0) {
+        const int64_t row = st.nrec;
+        if (row < P.hist_cap) {
+            const int64_t base = row * nruns + run;
+            const int64_t plane = P.hist_cap * nruns;
+            double sh = shift;
+            if (Lnew > 0) sh = shift - (st.xi / dtau * log(nrm / st.target) + st.zeta / dtau * log(nrm / st.pnorm));
+            P.hist[0 * plane + base] = sh;
+            P.hist[1 * plane + base] = nrm;
+            P.hist[2 * plane + base] = (double)Lnew;
+            P.hist[3 * plane + base] = vp;
+            P.hist[4 * plane + base] = hp;
+            P.hist[5 * plane + base] = att_sum;
+            st.nrec = row + 1;
+        }
+        if (s_flag != 0) {
+            st.status = s_flag;
+        } else if (Lnew == 0) {
+            st.status = RUN_DEAD;
+        } else {
+            st.shift = shift - (st.xi / dtau * log(nrm / st.target) + st.zeta / dtau * log(nrm / st.pnorm));
+            st.pnorm = nrm;
+            st.len = Lnew;
+            st.cur = 1 - cur;
+        }
+        st.step = step;
+    }
+}
+
+// Projector of every run: the start address (coefficient diag) and its neighbours.
+__global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, const uint64_t* __restrict__ start,
+                                       uint64_t* __restrict__ pk, double* __restrict__ pc) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    RunState& st = runs[r];
+    const uint64_t a = start[r];
+    uint64_t* k = pk + st.offP;
+    double* c = pc + st.offP;
+    k[0] = a;
+    c[0] = st.u * (double)bose_hubbard_interaction(a) / 2;  // hproj = H' * projector
+    const int64_t n_od = 2 * (int64_t)num_occupied(a);
+    int64_t n = 1;
+    for (int64_t ch = 1; ch <= n_od; ++ch) {
+        int64_t prod;
+        const uint64_t nb = real1d_hop(a, ch, st.M, st.B, &prod);
+        const double val = -st.t * sqrt((double)prod);
+        int64_t q = 1;
+        while (q < n && k[q] != nb) ++q;  // repeated targets (M = 2) are summed
+        if (q == n) { k[n] = nb; c[n] = val; ++n; }
+        else c[q] += val;
+    }
+    st.nproj = n;
+}
+
+}  // namespace
+
+struct rimu_batch {
+    int nruns = 0;
+    int block = 512, radix = 8;
+    hipStream_t stream = nullptr;
+    std::vector<RunState> host_init;
+    DevBuf<RunState> runs;
+    DevBuf<uint64_t> vk[2], sk[2], pk, start;
+    DevBuf<int64_t> vv[2], dv, sv[2];
+    DevBuf<double> pc, hist;
+    int64_t totV = 0, totS = 0, totP = 0, hist_cap = 0;
+    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
+    Pools pools() {
+        Pools P{};
+        for (int b = 0; b < 2; ++b) {
+            P.vk[b] = vk[b].p;
+            P.vv[b] = vv[b].p;
+            P.sk[b] = sk[b].p;
+            P.sv[b] = sv[b].p;
+        }
+        P.dv = dv.p;
+        P.pk = pk.p;
+        P.pc = pc.p;
+        P.hist = hist.p;
+        P.hist_cap = hist_cap;
+        return P;
+    }
+};
+
+namespace {
+
+int launch_step(rimu_batch* b) {
+    const Pools P = b->pools();
+    switch (b->block) {
+        case 256: batch_step_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
+        case 1024: batch_step_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
+        default: batch_step_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
+void clear_graphs(rimu_batch* b) {
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
+}
+
+int graph_for(rimu_batch* b, int steps, hipGraphExec_t* out) {
+    const auto key = std::make_pair(steps, std::make_pair(b->block, b->radix));
+    auto it = b->graphs.find(key);
+    if (it != b->graphs.end()) { *out = it->second; return RIMU_OK; }
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < steps; ++s) {
+        if (int rc = launch_step(b)) {
+            hipGraph_t dummy;
+            (void)hipStreamEndCapture(b->stream, &dummy);
+            return rc;
+        }
+    }
+    HIP_TRY(hipStreamEndCapture(b->stream, &g));
+    hipGraphExec_t exec;
+    HIP_TRY(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    b->graphs[key] = exec;
+    *out = exec;
+    return RIMU_OK;
+}
+
+int run_steps(rimu_batch* b, int64_t nsteps, int spg) {
+    if (spg <= 1) {
+        for (int64_t s = 0; s < nsteps; ++s)
+            if (int rc = launch_step(b)) return rc;
+        return RIMU_OK;
+    }
+    hipGraphExec_t exec;
+    if (int rc = graph_for(b, spg, &exec)) return rc;
+    int64_t done = 0;
+    for (; done + spg <= nsteps; done += spg) HIP_TRY(hipGraphLaunch(exec, b->stream));
+    for (; done < nsteps; ++done)
+        if (int rc = launch_step(b)) return rc;
+    return RIMU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history_steps, rimu_batch** out) {
+    if (!runs || !out) return fail(RIMU_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (nruns < 1 || history_steps < 0) return fail(RIMU_ERR_ARG, "need nruns >= 1 and history_steps >= 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
+    std::vector<RunState> st(nruns);
+    std::vector<uint64_t> start(nruns);
+    int64_t totV = 0, totS = 0, totP = 0;
+    for (int r = 0; r < nruns; ++r) {
+        const rimu_batch_run& q = runs[r];
+        if (q.M < 2 || q.N < 1 || q.M + q.N - 1 > 64) return fail(RIMU_ERR_ARG, "run: need M >= 2, N >= 1, N + M - 1 <= 64");
+        if (!(q.dtau > 0) || !(q.target_walkers > 0) || q.max_length < 1 || q.start_population == 0)
+            return fail(RIMU_ERR_ARG, "run: need dtau > 0, target_walkers > 0, max_length >= 1, population != 0");
+        const int B = q.M + q.N - 1;
+        if ((q.start_address >> B) != 0 || __builtin_popcountll(q.start_address) != q.N)
+            return fail(RIMU_ERR_ARG, "run: start_address is not an N-boson M-mode address");
+        if (2 * q.M + 1 > MAXPROJ) return fail(RIMU_ERR_ARG, "run: too many modes for the projector");
+        RunState& s = st[r];
+        std::memset(&s, 0, sizeof(s));
+        s.M = q.M; s.N = q.N; s.B = B; s.status = RUN_OK;
+        s.u = q.u; s.t = q.t; s.dtau = q.dtau; s.shift = q.shift;
+        s.pnorm = std::fabs((double)q.start_population);
+        s.target = q.target_walkers; s.zeta = q.zeta; s.xi = q.xi; s.seed = q.seed;
+        s.len = 1; s.cur = 0; s.step = 0; s.nrec = 0;
+        s.capV = q.max_length;
+        s.capS = (int64_t)std::min(4.0 * q.target_walkers + 4096.0, 4.0e9);
+        s.offV = totV; s.offS = totS; s.offP = totP; s.nproj = 0;
+        totV += s.capV; totS += s.capS; totP += 2 * q.M + 1;
+        start[r] = q.start_address;
+    }
+    rimu_batch* b = new rimu_batch();
+    b->nruns = nruns;
+    b->host_init = st;
+    b->totV = totV; b->totS = totS; b->totP = totP; b->hist_cap = history_steps;
+    auto bad = [&](const char* what, hipError_t e) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        rimu_batch_destroy(b);
+        return fail(RIMU_ERR_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bad("stream", e);
+    if ((e = b->runs.alloc(nruns)) != hipSuccess) return bad("runs", e);
+    for (int k = 0; k < 2; ++k) {
+        if ((e = b->vk[k].alloc(totV)) != hipSuccess) return bad("vector keys", e);
+        if ((e = b->vv[k].alloc(totV)) != hipSuccess) return bad("vector values", e);
+        if ((e = b->sk[k].alloc(totS)) != hipSuccess) return bad("spawn keys", e);
+        if ((e = b->sv[k].alloc(totS)) != hipSuccess) return bad("spawn values", e);
+    }
+    if ((e = b->dv.alloc(totV)) != hipSuccess) return bad("diagonal values", e);
+    if ((e = b->pk.alloc(totP)) != hipSuccess) return bad("projector keys", e);
+    if ((e = b->pc.alloc(totP)) != hipSuccess) return bad("projector coefficients", e);
+    if ((e = b->start.alloc(nruns)) != hipSuccess) return bad("start addresses", e);
+    if ((e = b->hist.alloc(std::max<int64_t>(1, NHIST * history_steps * nruns))) != hipSuccess) return bad("history", e);
+    if ((e = hipMemcpy(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpy(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
+        const int64_t pop = runs[r].start_population;
+        if ((e = hipMemcpy(b->vk[0].p + st[r].offV, &start[r], 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+        if ((e = hipMemcpy(b->vv[0].p + st[r].offV, &pop, 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    }
+    batch_projector_kernel<<<ceil_div(nruns, 64), 64, 0, b->stream>>>(b->runs.p, nruns, b->start.p, b->pk.p, b->pc.p);
+    if ((e = hipGetLastError()) != hipSuccess) return bad("projector", e);
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bad("projector", e);
+    *out = b;
+    return RIMU_OK;
+}
+
+int rimu_batch_destroy(rimu_batch* b) {
+    if (!b) return RIMU_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    clear_graphs(b);
+    b->runs.release();
+    for (int k = 0; k < 2; ++k) { b->vk[k].release(); b->vv[k].release(); b->sk[k].release(); b->sv[k].release(); }
+    b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return RIMU_OK;
+}
+
+int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (block_size != 256 && block_size != 512 && block_size != 1024)
+        return fail(RIMU_ERR_ARG, "block_size must be 256, 512 or 1024");
+    if (radix_bits < 4 || radix_bits > 8) return fail(RIMU_ERR_ARG, "radix_bits must be in 4..8");
+    b->block = block_size;
+    b->radix = radix_bits;
+    return RIMU_OK;
+}
+
+int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (nsteps < 0 || steps_per_graph < 1) return fail(RIMU_ERR_ARG, "need nsteps >= 0 and steps_per_graph >= 1");
+    if (int rc = run_steps(b, nsteps, steps_per_graph)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RIMU_OK;
+}
+
+int rimu_batch_history(rimu_batch* b, int64_t* nsteps, double* shift, double* norm, double* len,
+                       double* vproj, double* hproj, double* attempts) {
+    if (!b || !nsteps) return fail(RIMU_ERR_ARG, "null argument");
+    std::vector<RunState> st(b->nruns);
+    HIP_TRY(hipMemcpy(st.data(), b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToHost));
+    int64_t rows = 0;
+    for (auto& s : st) rows = std::max(rows, s.nrec);
+    *nsteps = rows;
+    double* outs[NHIST] = {shift, norm, len, vproj, hproj, attempts};
+    const int64_t plane = b->hist_cap * b->nruns;
+    for (int f = 0; f < NHIST; ++f)
+        if (outs[f] && rows > 0)
+            HIP_TRY(hipMemcpy(outs[f], b->hist.p + f * plane, rows * b->nruns * sizeof(double), hipMemcpyDeviceToHost));
+    return RIMU_OK;
+}
+
+int rimu_batch_status(rimu_batch* b, int32_t* status, int64_t* len) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    std::vector<RunState> st(b->nruns);
+    HIP_TRY(hipMemcpy(st.data(), b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToHost));
+    for (int r = 0; r < b->nruns; ++r) {
+        if (status) status[r] = st[r].status;
+        if (len) len[r] = st[r].len;
+    }
+    return RIMU_OK;
+}
+
+int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals, int64_t cap, int64_t* n) {
+    if (!b || !n) return fail(RIMU_ERR_ARG, "null argument");
+    if (run < 0 || run >= b->nruns) return fail(RIMU_ERR_ARG, "run index out of range");
+    RunState s;
+    HIP_TRY(hipMemcpy(&s, b->runs.p + run, sizeof(RunState), hipMemcpyDeviceToHost));
+    *n = s.len;
+    if (s.len > cap) return fail(RIMU_ERR_CAPACITY, "capacity too small");
+    HIP_TRY(hipMemcpy(keys, b->vk[s.cur].p + s.offV, s.len * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vals, b->vv[s.cur].p + s.offV, s.len * 8, hipMemcpyDeviceToHost));
+    return RIMU_OK;
+}
+
+int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix, double* ms) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (trial_steps < 1) return fail(RIMU_ERR_ARG, "trial_steps must be >= 1");
+    // snapshot: run states and both vector buffers (the step reads one, writes the other)
+    DevBuf<RunState> s_runs;
+    DevBuf<uint64_t> s_vk[2];
+    DevBuf<int64_t> s_vv[2];
+    auto cleanup = [&]() { s_runs.release(); for (int k = 0; k < 2; ++k) { s_vk[k].release(); s_vv[k].release(); } };
+    auto body = [&]() -> int {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(s_runs.alloc(b->nruns));
+        HIP_TRY(hipMemcpy(s_runs.p, b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice));
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(s_vk[k].alloc(b->totV));
+            HIP_TRY(s_vv[k].alloc(b->totV));
+            HIP_TRY(hipMemcpy(s_vk[k].p, b->vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(s_vv[k].p, b->vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+        }
+        auto restore = [&]() -> int {
+            HIP_TRY(hipMemcpy(b->runs.p, s_runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice));
+            for (int k = 0; k < 2; ++k) {
+                HIP_TRY(hipMemcpy(b->vk[k].p, s_vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+                HIP_TRY(hipMemcpy(b->vv[k].p, s_vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+            }
+            return RIMU_OK;
+        };
+        const int blocks[3] = {256, 512, 1024};
+        const int radixes[3] = {4, 6, 8};
+        double best = 1e300;
+        int bb = b->block, br = b->radix;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        int idx = 0;
+        for (int bi = 0; bi < 3; ++bi)
+            for (int ri = 0; ri < 3; ++ri, ++idx) {
+                b->block = blocks[bi];
+                b->radix = radixes[ri];
+                if (int rc = launch_step(b)) return rc;  // first launch outside the timing
+                HIP_TRY(hipStreamSynchronize(b->stream));
+                if (int rc = restore()) return rc;
+                HIP_TRY(hipEventRecord(e0, b->stream));
+                for (int64_t s = 0; s < trial_steps; ++s)
+                    if (int rc = launch_step(b)) return rc;
+                HIP_TRY(hipEventRecord(e1, b->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float t = 0;
+                HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+                const double per = t / (double)trial_steps;
+                if (ms) ms[idx] = per;
+                if (per < best) { best = per; bb = b->block; br = b->radix; }
+                if (int rc = restore()) return rc;
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->block = bb;
+        b->radix = br;
+        if (best_block) *best_block = bb;
+        if (best_radix) *best_radix = br;
+        return RIMU_OK;
+    };
+    const int rc = body();
+    cleanup();
+    return rc;
+}
+
+}  // extern "C"

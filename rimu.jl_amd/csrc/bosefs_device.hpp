@@ -163,21 +163,20 @@ __device__ inline double diagonal_element(const ModelDev& h, uint64_t bits) {
     return kin + h.u / (2.0 * h.M) * (double)onproduct;
 }
 
-// hopnextneighbour (bosefs.jl:297-345) with value -t*sqrt(prod) (HubbardReal1D.jl:59-62).
-// chosen in 1..2*s: site (chosen+1)/2 is the site-th occupied mode; odd hops right.
-__device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
-                                              double* value) {
+// hopnextneighbour (bosefs.jl:297-345): the neighbour address and the occupation product
+// under the square root.  chosen in 1..2*s: site (chosen+1)/2 is the site-th occupied mode;
+// odd hops right.  M modes, B = N + M - 1 bits.
+__device__ inline uint64_t real1d_hop(uint64_t bits, int64_t chosen, int M, int B, int64_t* prod_out) {
     const int site = (int)((chosen + 1) >> 1);
     const uint64_t st = run_starts(bits);
     const int p = select_bit(st, site - 1);
     const int n = ones_from(bits, p);
     const int m = p - popc64(bits & low_mask(p)) + 1;  // mode = zeros below p, plus one
-    const int B = h.B;
     const uint64_t maskB = low_mask(B);
     uint64_t nb;
     int64_t prod;
     if (chosen & 1) {  // right
-        if (m == h.M) {
+        if (m == M) {
             nb = ((bits << 1) | 1ull) & maskB;
             prod = (int64_t)n * (ones_from(bits, 0) + 1);
         } else {
@@ -201,6 +200,15 @@ __device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, 
             prod = (int64_t)n * (nprev + 1);
         }
     }
+    *prod_out = prod;
+    return nb;
+}
+
+// get_offdiagonal of HubbardReal1D: value -t*sqrt(prod) (HubbardReal1D.jl:59-62).
+__device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
+                                              double* value) {
+    int64_t prod;
+    const uint64_t nb = real1d_hop(bits, chosen, h.M, h.B, &prod);
     *value = -h.t * sqrt((double)prod);
     return nb;
 }
