@@ -177,10 +177,16 @@ __global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64
                                                       const T* __restrict__ tile_sums) {
     const T add = tile_sums[blockIdx.x];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    T x[SCAN_IPT];
 #pragma unroll
     for (int j = 0; j < SCAN_IPT; ++j) {
         const int64_t idx = base + (int64_t)j * NT + threadIdx.x;
-        if (idx < n) out[idx] += add;
+        x[j] = out[min(idx, n - 1)];  // all loads issued before the first store
+    }
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; ++j) {
+        const int64_t idx = base + (int64_t)j * NT + threadIdx.x;
+        if (idx < n) out[idx] = x[j] + add;
     }
 }
 
@@ -224,7 +230,20 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t a0 = (int64_t)tile * SPAWN_TILE;
     const int64_t e0 = tile_first[tile];
-    for (int t = threadIdx.x; t < WIN; t += NT) s_aoff[t] = aoff[min(e0 + t, L)];
+    {
+        constexpr int WR = (WIN + NT - 1) / NT;
+        int64_t w[WR];
+#pragma unroll
+        for (int r = 0; r < WR; ++r) {
+            const int t = (int)threadIdx.x + r * NT;
+            if (t < WIN) w[r] = aoff[min(e0 + t, L)];
+        }
+#pragma unroll
+        for (int r = 0; r < WR; ++r) {
+            const int t = (int)threadIdx.x + r * NT;
+            if (t < WIN) s_aoff[t] = w[r];
+        }
+    }
     __syncthreads();
 
     uint64_t rk[SPAWN_ITEMS];
@@ -233,16 +252,14 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     bool rok[SPAWN_ITEMS];
     double spawned = 0;
     uint64_t kor = 0, kand = ~0ull;
+    // entry of each attempt: last i with aoff[i] <= att (every entry has at least one
+    // attempt unless it has no off-diagonals; outside the window: global search)
+    int64_t ent[SPAWN_ITEMS], aidx[SPAWN_ITEMS];
 #pragma unroll
     for (int it = 0; it < SPAWN_ITEMS; ++it) {
         const int64_t att = a0 + (int64_t)it * NT + threadIdx.x;
-        bool ok = false;
-        uint64_t nb = 0;
-        V nv = (V)0;
+        int64_t i = e0, ai = att;
         if (att < A) {
-            // entry of this attempt: last i with aoff[i] <= att (every entry has at least
-            // one attempt unless it has no off-diagonals; outside the window: global search)
-            int64_t i, ai;
             if (att < s_aoff[WIN - 1]) {
                 int lo = 0, hi = WIN - 1;
                 while (hi - lo > 1) {
@@ -260,9 +277,29 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
                 i = lo;
                 ai = aoff[lo];
             }
-            const int64_t j = att - ai;
-            const uint64_t k = keys[i];
-            const V v = vals[i];
+        }
+        ent[it] = i;
+        aidx[it] = ai;
+    }
+    // the entries' keys and values, all loads in flight at once (e0 < L whenever A > 0)
+    uint64_t ekey[SPAWN_ITEMS];
+    V eval[SPAWN_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SPAWN_ITEMS; ++it) {
+        const int64_t i = min(ent[it], L - 1);
+        ekey[it] = keys[i];
+        eval[it] = vals[i];
+    }
+#pragma unroll
+    for (int it = 0; it < SPAWN_ITEMS; ++it) {
+        const int64_t att = a0 + (int64_t)it * NT + threadIdx.x;
+        bool ok = false;
+        uint64_t nb = 0;
+        V nv = (V)0;
+        if (att < A) {
+            const int64_t j = att - aidx[it];
+            const uint64_t k = ekey[it];
+            const V v = eval[it];
             const ColumnPlan c = plan_column<V>(a, k, v);
             const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
             U4 r = {0, 0, 0, 0};
@@ -393,10 +430,16 @@ __global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restr
     hist[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    uint64_t k[SORT_IPT];
 #pragma unroll
     for (int r = 0; r < SORT_IPT; ++r) {
         const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
-        if (idx < n) atomicAdd(&hist[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+        k[r] = keys[min(idx, n - 1)];  // all loads issued before the first atomic
+    }
+#pragma unroll
+    for (int r = 0; r < SORT_IPT; ++r) {
+        const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
+        if (idx < n) atomicAdd(&hist[(k[r] >> shift) & (RADIX - 1)], 1u);
     }
     __syncthreads();
     counts[(int64_t)threadIdx.x * num_tiles + blockIdx.x] = hist[threadIdx.x];
@@ -527,8 +570,28 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     const int64_t a0 = splits[tile], a1 = splits[tile + 1];
     const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-    for (int i = threadIdx.x; i < na; i += NT) { s_key[i] = ak[a0 + i]; s_val[i] = av[a0 + i]; }
-    for (int i = threadIdx.x; i < nb; i += NT) { s_key[na + i] = bk[b0 + i]; s_val[na + i] = bv[b0 + i]; }
+    // stage A then B: all loads of a thread are issued before the first LDS store, so a
+    // wave waits for one memory round trip instead of one per element
+    {
+        uint64_t rk[MERGE_IPT];
+        V rv[MERGE_IPT];
+#pragma unroll
+        for (int r = 0; r < MERGE_IPT; ++r) {
+            const int i = (int)threadIdx.x + r * NT;
+            if (i < na + nb) {
+                const bool in_a = i < na;
+                const uint64_t* kp = in_a ? ak + (a0 + i) : bk + (b0 + (i - na));
+                const V* vp = in_a ? av + (a0 + i) : bv + (b0 + (i - na));
+                rk[r] = *kp;
+                rv[r] = *vp;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < MERGE_IPT; ++r) {
+            const int i = (int)threadIdx.x + r * NT;
+            if (i < na + nb) { s_key[i] = rk[r]; s_val[i] = rv[r]; }
+        }
+    }
     __syncthreads();
     const uint64_t* s_a = s_key;
     const uint64_t* s_b = s_key + na;
