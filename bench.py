@@ -12,6 +12,16 @@ value = spawn attempts per second over the timed steps (Rimu's `spawn_attempts` 
 The roofline object refers to the kernel with the largest device time; its algorithmic
 bytes are the records it must read and write once.  cpu_baseline times the threaded C++
 restatement of Rimu's PDVec step (oracle/) on a bounded sample of the same state.
+
+Two more objects on the same line:
+* "batched": 256 HubbardReal1D runs of a U-sweep advanced together for 10,000 steps
+  (solves/s), against the plain engine running the same runs one at a time; each run's
+  energy estimates are compared with its own Lanczos ground-state energy.
+* "adjoint": dE/du and dE/dt of the K = 2000-step deterministic projector energy for
+  HubbardReal1D M = N = 12 at u = 2, 4, 6, 8 in one product (grads/s), against central
+  differences on the plain engine.
+* "momentum": the momentum-space case, HubbardMom1D M = N = 16, u = 4, integer walkers,
+  2e7 walkers, dtau = 5e-4: spawn attempts/s and per-kernel times at steady state.
 """
 from __future__ import annotations
 
@@ -87,6 +97,197 @@ def run_cpu_baseline(rimu_keys, rimu_vals, args, step_no, shift):
             "seconds": dt}
 
 
+def run_batched_leg(rimu, args):
+    """The batched U-sweep: 256 HubbardReal1D runs (M ~ U{6..14}, N ~ U{ceil(M/2)..M},
+    u ~ U[1,10], target walkers log-uniform in [1e3, 1e5], IsStochasticInteger, dtau 1e-3,
+    seeds 0..255) for `--batched-steps` steps in one launch per step, graph-replayed.
+    solves/s = runs completed per second of the whole sweep.  The plain engine's figure is
+    the same sweep run one instance at a time: each instance's steady-state step time on
+    the plain engine (from the batched run's final vector and shift) times the step count."""
+    specs = rimu.u_sweep_specs(args.batched_runs)
+    nsteps = args.batched_steps
+    # autotune on a copy that has grown to steady state (snapshot/restore inside)
+    tune = rimu.BatchedRuns(specs, history_steps=args.batched_tune_at)
+    tune.step(args.batched_tune_at, steps_per_graph=100)
+    blk, rad, table = tune.autotune(trial_steps=20)
+    tune.close()
+    log(f"batched autotune: block {blk} radix {rad}")
+    b = rimu.BatchedRuns(specs, history_steps=nsteps)
+    b.set_config(blk, rad)
+    t0 = time.perf_counter()
+    b.step(nsteps, steps_per_graph=100)
+    elapsed = time.perf_counter() - t0
+    st, ln = b.status()
+    hist = b.history()
+    attempts = float(hist["spawn_attempts"].sum())
+    log(f"batched sweep: {len(specs)} runs x {nsteps} steps in {elapsed:.2f} s")
+
+    ed = batched_ed_check(rimu, specs, hist, st, skip=nsteps // 4)
+
+    plain_s = 0.0
+    sample_steps = args.plain_sample_steps
+    for r, sp in enumerate(specs):
+        d = sp.descriptor()
+        k, v = b.vector(r)
+        if st[r] != 0 or len(k) == 0:
+            continue
+        eng = rimu._native.Engine(sp.hamiltonian.descriptor(), rimu.IsStochasticInteger().descriptor(),
+                                  d.seed, d.max_length + 1, int(min(4 * d.target_walkers + 4096, 2 ** 32 - 2)))
+        eng.upload(k, v)
+        shift = float(hist["shift"][-1, r])
+        for i in range(3):
+            eng.step(nsteps + 1 + i, shift, d.dtau)
+        t1 = time.perf_counter()
+        for i in range(sample_steps):
+            eng.step(nsteps + 4 + i, shift, d.dtau)
+        plain_s += (time.perf_counter() - t1) / sample_steps * nsteps
+        eng.close()
+    b.close()
+    solves = len(specs) / elapsed
+    plain = len(specs) / plain_s
+    return {"metric": "solves/s", "value": solves, "plain_value": plain, "speedup": solves / plain,
+            "runs": len(specs), "steps": nsteps, "seconds": elapsed,
+            "spawn_attempts_per_s": attempts / elapsed,
+            "status_counts": {rimu.batched.STATUS[c]: int((st == c).sum()) for c in np.unique(st)},
+            "ed_check": ed,
+            "autotune": {"block": blk, "radix_bits": rad, "tuned_after_steps": args.batched_tune_at,
+                         "ms_per_step": {f"{k[0]}x{k[1]}": v for k, v in table.items()}},
+            "plain_estimate": (f"each run's steady-state step time on the plain engine "
+                               f"({sample_steps} steps from the sweep's final vector) x {nsteps} steps, "
+                               f"summed over runs: {plain_s:.1f} s")}
+
+
+def batched_ed_check(rimu, specs, hist, status, skip):
+    """Each run's shift and projected-energy estimates (blocking analysis after `skip`
+    steps) against its own ground-state energy (Lanczos on the device, runs of equal M, N
+    grouped eight to a launch).  Not part of the timed sweep."""
+    ST = rimu.statstools
+    t0 = time.perf_counter()
+    e0 = np.full(len(specs), np.nan)
+    groups = {}
+    for r, sp in enumerate(specs):
+        a = sp.hamiltonian.starting_address()
+        groups.setdefault((a.M, a.N), []).append(r)
+    for (M, N), runs in groups.items():
+        for c in range(0, len(runs), 8):
+            chunk = runs[c:c + 8]
+            d = rimu.DenseHamiltonians([specs[r].hamiltonian for r in chunk])
+            e, _ = d.ground_state()
+            e0[chunk] = e
+            d.close()
+    t_ed = time.perf_counter() - t0
+    rows = []
+    for r in range(len(specs)):
+        if status[r] != 0:
+            continue
+        se = ST.blocking_analysis(hist["shift"][:, r], skip=skip)
+        pe = ST.ratio_of_means(hist["hproj"][:, r], hist["vproj"][:, r], skip=skip)
+        rows.append((se.mean - e0[r], se.err, pe.ratio - e0[r], pe.err))
+    z_s = np.array([abs(a) / b for a, b, _, _ in rows if b > 0 and np.isfinite(b)])
+    z_p = np.array([abs(c) / d for _, _, c, d in rows if d > 0 and np.isfinite(d)])
+    return {"runs_checked": len(rows), "ed_seconds": t_ed, "skip_steps": skip,
+            "shift_within_2sigma": int((z_s < 2).sum()), "shift_within_3sigma": int((z_s < 3).sum()),
+            "shift_blocking_failed": len(rows) - len(z_s),
+            "projected_within_2sigma": int((z_p < 2).sum()), "projected_within_3sigma": int((z_p < 3).sum()),
+            "projected_blocking_failed": len(rows) - len(z_p),
+            "median_abs_shift_error_over_E0": float(np.median([abs(a) / abs(e0[r]) for r, (a, _, _, _) in
+                                                              zip([i for i in range(len(specs)) if status[i] == 0], rows)]))
+            if rows else None}
+
+
+def run_momentum_leg(rimu, args):
+    """HubbardMom1D M = N = 16, u = 4, t = 1 from the zero-momentum condensate (16 bosons
+    in mode 8), IsStochasticInteger, DoubleLogUpdate to 2e7 walkers, dtau = 5e-4, seed 17:
+    grown to the target (untimed), then `--mom-steps` steady-state steps timed."""
+    M = N = 16
+    onr = [0] * M
+    onr[7] = N
+    H = rimu.HubbardMom1D(rimu.BoseFS(*onr), u=4.0, t=1.0)
+    style = rimu.IsStochasticInteger()
+    target, dtau = args.mom_target, 5e-4
+    eng = rimu._native.Engine(H.descriptor(), style.descriptor(), 17, int(2 * target + 101),
+                              int(min(4 * target + 4096, 2 ** 32 - 2)))
+    start = H.starting_address()
+    eng.upload(np.array([start.bits], dtype=np.uint64), np.array([10], dtype=np.int64))
+    strat = rimu.DoubleLogUpdate(target_walkers=target)
+    sp = rimu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, dtau)
+    step, settle = 0, None
+    t0 = time.perf_counter()
+    while step < args.mom_equilibration:
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        if settle is None and st.norm >= 0.9 * target:
+            settle = step
+        if settle is not None and step >= settle + args.mom_settle:
+            break
+    log(f"momentum: grown in {step} steps ({time.perf_counter() - t0:.1f} s), len {st.len:.3g} norm {st.norm:.3g}")
+    eng.set_timing(True)
+    attempts = 0.0
+    t1 = time.perf_counter()
+    for _ in range(args.mom_steps):
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        attempts += st.spawn_attempts
+    elapsed = time.perf_counter() - t1
+    times = eng.kernel_times()
+    eng.close()
+    kern = {n: {"ms_avg": ms / max(c, 1), "launches_per_step": c / args.mom_steps}
+            for n, (ms, c) in times.items()}
+    return {"workload": "HubbardMom1D M=N=16 u=4 t=1, condensate start, IsStochasticInteger, "
+                        "target_walkers=2e7, dtau=5e-4, seed 17",
+            "metric": "spawn_attempts/s", "value": attempts / elapsed,
+            "ms_per_step": 1e3 * elapsed / args.mom_steps, "steps": args.mom_steps,
+            "growth_steps": step - args.mom_steps, "walkers": st.norm, "len": st.len,
+            "shift": sp.shift, "kernels": kern}
+
+
+def run_adjoint_leg(rimu, args):
+    """The differentiable projector: HubbardReal1D M = N = 12 (dimension 1,352,078),
+    u in {2,4,6,8} as four right-hand sides of one product, K = 2000 steps of dtau 1e-3,
+    checkpoints every 45 steps.  grads/s = (dE/du, dE/dt) pairs per second.  The plain
+    engine has no adjoint: a user of it differentiates by central differences, four
+    K-step IsDeterministic runs per u, one instance per launch; that figure uses the
+    plain engine's step time on the full vector."""
+    M = N = 12
+    us = [2.0, 4.0, 6.0, 8.0]
+    K, dtau, ckpt = args.adjoint_steps, 1e-3, 45
+    hs = [rimu.HubbardReal1D(rimu.near_uniform(N, M), u=u, t=1.0) for u in us]
+    d = rimu.DenseHamiltonians(hs)
+    v0 = d.unit_vector(hs[0].starting_address())
+    shift = np.zeros(len(us))
+    d.projector_energy(20, dtau, shift, v0, ckpt=5)   # warm-up
+    t0 = time.perf_counter()
+    res = d.projector_energy(K, dtau, shift, v0, ckpt=ckpt)
+    elapsed = time.perf_counter() - t0
+    grads = len(us) / elapsed
+    basis = d.basis()
+    vf = res["v"]
+    d.close()
+    # plain engine, IsDeterministic, on the final vector of u = us[0]
+    h = hs[0]
+    eng = rimu._native.Engine(h.descriptor(), rimu.IsDeterministic().descriptor(), 1,
+                              len(basis) + 1, int(len(basis) * (2 * N + 1) + 4096))
+    eng.upload(basis, np.ascontiguousarray(vf[:, 0]))
+    for i in range(3):
+        eng.step(i + 1, 0.0, dtau)
+    t1 = time.perf_counter()
+    for i in range(args.plain_sample_steps):
+        eng.step(i + 4, 0.0, dtau)
+    t_step = (time.perf_counter() - t1) / args.plain_sample_steps
+    eng.close()
+    plain_s = len(us) * 4 * K * t_step
+    return {"metric": "grads/s", "value": grads, "plain_value": len(us) / plain_s,
+            "speedup": grads / (len(us) / plain_s), "seconds": elapsed, "dim": len(basis),
+            "K": K, "checkpoint_every": ckpt, "u": us,
+            "energy": res["energy"].tolist(), "dE_du": res["grad_u"].tolist(),
+            "dE_dt": res["grad_t"].tolist(),
+            "plain_estimate": (f"central differences on the plain engine: 4 runs x {K} "
+                               f"IsDeterministic steps per u at {1e3 * t_step:.3f} ms/step "
+                               f"(measured on the full vector), {plain_s:.1f} s")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +303,19 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for HBM byte counters)")
+    ap.add_argument("--batched-runs", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=10_000)
+    ap.add_argument("--batched-tune-at", type=int, default=1000)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--plain-sample-steps", type=int, default=20)
+    ap.add_argument("--mom-target", type=float, default=2e7)
+    ap.add_argument("--mom-equilibration", type=int, default=12_000,
+                    help="cap on growth steps of the momentum-space run")
+    ap.add_argument("--mom-settle", type=int, default=200)
+    ap.add_argument("--mom-steps", type=int, default=40)
+    ap.add_argument("--no-momentum", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -218,6 +432,20 @@ def main():
         cpu = run_cpu_baseline(keys, vals, args, step + 1, sp.shift)
         log(f"cpu baseline: {cpu['value']:.4g} attempts/s on {cpu['cores']} threads ({cpu['sample']})")
 
+    final_norm, final_len = st.norm, st.len
+    eng.close()
+    batched = None if args.no_batched else run_batched_leg(rimu, args)
+    if batched:
+        log(f"batched: {batched['value']:.3g} solves/s vs plain {batched['plain_value']:.3g} "
+            f"({batched['speedup']:.1f}x)")
+    momentum = None if args.no_momentum else run_momentum_leg(rimu, args)
+    if momentum:
+        log(f"momentum: {momentum['value']:.3g} attempts/s, {momentum['ms_per_step']:.2f} ms/step")
+    adjoint = None if args.no_adjoint else run_adjoint_leg(rimu, args)
+    if adjoint:
+        log(f"adjoint: {adjoint['value']:.3g} grads/s vs plain {adjoint['plain_value']:.3g} "
+            f"({adjoint['speedup']:.1f}x)")
+
     out = {
         "metric": "spawned walkers/s & HBM % roofline; batched-U solves/s + grads/s vs plain",
         "value": value,
@@ -235,14 +463,16 @@ def main():
                                "target_walkers=1e8, dtau=1e-3, seed 17",
                    "equilibration_steps": args.equilibration, "parallelism": "single GPU"},
         "step_alg_roofline_frac": alg_bytes / elapsed / HBM_PEAK,
-        "walkers": st.norm,
-        "len": st.len,
+        "walkers": final_norm,
+        "len": final_len,
         "spawn_records_per_step": S_tot / K,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "batched": batched,
+        "adjoint": adjoint,
+        "momentum": momentum,
     }
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

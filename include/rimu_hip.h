@@ -202,6 +202,17 @@ int rimu_dense_projector(rimu_dense* d, int64_t K, double dtau, const double* sh
                          const double* v0, double* energy, double* grad_u, double* grad_t,
                          double* hf_u, double* hf_t, double* v_final);
 
+/* Lowest eigenvalue of every right-hand side's H by Lanczos on the device, started from
+ * v0 (host, [dim][R]; its Krylov space must reach the ground state, e.g. the start
+ * address, which overlaps the nodeless ground state of HubbardReal1D).  Stops when every
+ * lowest Ritz value changes by at most tol * max(1, |E|) over 16 iterations, when a Krylov
+ * space is exhausted, or after max_iters.  iters (may be null) receives the iterations run.
+ * Replaces the ground-state energy of solve(ExactDiagonalizationProblem(H)) with the
+ * KrylovKit Lanczos solver (ext/KrylovKitExt.jl:36-78, ExactDiagonalization/solve.jl),
+ * the yardstick Rimu's tests hold FCIQMC energies against. */
+int rimu_dense_ground_state(rimu_dense* d, const double* v0, int64_t max_iters, double tol,
+                            double* energy, int64_t* iters);
+
 /* ---------------------------------------------------------------------------------------
  * Batched engine: many independent IsStochasticInteger FCIQMC runs of HubbardReal1D, one
  * workgroup per run, one kernel launch per step for all runs (captured in a HIP graph).

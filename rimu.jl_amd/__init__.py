@@ -9,7 +9,7 @@ The package mirrors the part of Rimu.jl's API that drives the FCIQMC step:
 Julia names with ``!`` map to: ``solve!`` -> ``PMCSimulation.solve_``, ``step!`` ->
 ``PMCSimulation.step_``, ``lomc!`` -> ``lomc``.
 """
-from . import _native
+from . import _native, statstools
 from ._native import RimuError, CapacityError, annihilate, device_count
 from .addresses import BoseFS, near_uniform
 from .hamiltonians import (HubbardMom1D, HubbardReal1D, diagonal_element, dimension,

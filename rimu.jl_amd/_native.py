@@ -26,6 +26,7 @@ EXPORTS = (
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
     "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
+    "rimu_dense_ground_state",
     "rimu_batch_create", "rimu_batch_destroy", "rimu_batch_set_config", "rimu_batch_step",
     "rimu_batch_history", "rimu_batch_status", "rimu_batch_autotune", "rimu_batch_vector",
 )
@@ -103,6 +104,7 @@ def lib():
         L.rimu_dense_apply.argtypes = [vp, vp, vp, vp, vp]
         L.rimu_dense_pullback.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.rimu_dense_projector.argtypes = [vp, i64, dbl, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.rimu_dense_ground_state.argtypes = [vp, vp, i64, dbl, vp, P(i64)]
         L.rimu_batch_create.argtypes = [P(BatchRun), i32, i64, P(vp)]
         L.rimu_batch_destroy.argtypes = [vp]
         L.rimu_batch_set_config.argtypes = [vp, i32, i32]

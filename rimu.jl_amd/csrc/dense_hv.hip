@@ -192,6 +192,43 @@ __global__ __launch_bounds__(DT) void rq_gradient_kernel(const double* __restric
     }
 }
 
+// One three-term Lanczos update per right-hand side: w -= alpha_j v_j + beta_j v_{j-1},
+// with alpha_j and beta_j^2 read from device memory, and the partial sums of |w|^2 of each
+// block written to partials[blockIdx][R] (reduced in order afterwards).
+template <int R>
+__global__ __launch_bounds__(DT) void lanczos_orth_kernel(double* __restrict__ w, const double* __restrict__ v,
+                                                          const double* __restrict__ vprev,
+                                                          const double* __restrict__ alpha,
+                                                          const double* __restrict__ beta2, int64_t D,
+                                                          double* __restrict__ partials) {
+    const int64_t r = (int64_t)blockIdx.x * DT + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        double x = 0;
+        if (r < D) {
+            x = w[r * R + i] - alpha[i] * v[r * R + i] - sqrt(beta2[i]) * vprev[r * R + i];
+            w[r * R + i] = x;
+        }
+        const double s = block_sum<DT>(x * x);
+        if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * R + i] = s;
+    }
+}
+
+// out = w / sqrt(norm2) per right-hand side; a right-hand side whose norm fell to
+// `floor2` or below (its Krylov space is exhausted) gets the zero vector.
+template <int R>
+__global__ __launch_bounds__(DT) void lanczos_scale_kernel(const double* __restrict__ w,
+                                                           const double* __restrict__ norm2, double floor2,
+                                                           int64_t D, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * DT + threadIdx.x;
+    if (r >= D) return;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const double n2 = norm2[i];
+        out[r * R + i] = n2 > floor2 ? w[r * R + i] / sqrt(n2) : 0.0;
+    }
+}
+
 }  // namespace
 
 struct rimu_dense {
@@ -305,6 +342,59 @@ int rq_gradient(rimu_dense* d, const double* v, const double* hv, const double* 
     }
     HIP_TRY(hipGetLastError());
     return RIMU_OK;
+}
+
+template <int R>
+int launch_lanczos(rimu_dense* d, double* w, const double* v, const double* vprev, const double* alpha,
+                   const double* beta2, double* norm2_out, double* vnext, double floor2) {
+    const int64_t blocks = ceil_div(d->D, DT);
+    HIP_TRY(d->partials.alloc((size_t)blocks * R));
+    lanczos_orth_kernel<R><<<blocks, DT, 0, d->stream>>>(w, v, vprev, alpha, beta2, d->D, d->partials.p);
+    reduce_rows_kernel<<<R, DT, 0, d->stream>>>(d->partials.p, blocks, R, norm2_out, 0);
+    lanczos_scale_kernel<R><<<blocks, DT, 0, d->stream>>>(w, norm2_out, floor2, d->D, vnext);
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
+int lanczos_update(rimu_dense* d, double* w, const double* v, const double* vprev, const double* alpha,
+                   const double* beta2, double* norm2_out, double* vnext, double floor2) {
+    switch (d->R) {
+        case 1: return launch_lanczos<1>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 2: return launch_lanczos<2>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 3: return launch_lanczos<3>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 4: return launch_lanczos<4>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 5: return launch_lanczos<5>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 6: return launch_lanczos<6>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        case 7: return launch_lanczos<7>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+        default: return launch_lanczos<8>(d, w, v, vprev, alpha, beta2, norm2_out, vnext, floor2);
+    }
+}
+
+// Lowest eigenvalue of the symmetric tridiagonal matrix with diagonal a[0..m) and squared
+// off-diagonal b2[1..m) (b2[k] couples k-1 and k), by bisection on the Sturm count: the
+// number of negative pivots of T - x I is the number of eigenvalues below x.
+double tridiag_lowest(const double* a, const double* b2, int64_t m) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t k = 0; k < m; ++k) {
+        const double off = (k > 0 ? sqrt(b2[k]) : 0.0) + (k + 1 < m ? sqrt(b2[k + 1]) : 0.0);
+        lo = std::min(lo, a[k] - off);
+        hi = std::max(hi, a[k] + off);
+    }
+    auto below = [&](double x) {
+        int64_t c = 0;
+        double piv = 1.0;
+        for (int64_t k = 0; k < m; ++k) {
+            piv = a[k] - x - (k > 0 ? b2[k] / piv : 0.0);
+            if (piv == 0.0) piv = -1e-300;
+            if (piv < 0) ++c;
+        }
+        return c;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (below(mid) >= 1) hi = mid; else lo = mid;
+    }
+    return 0.5 * (lo + hi);
 }
 
 bool binomial64(int n, int k, int64_t* out) {
@@ -513,6 +603,83 @@ int rimu_dense_projector(rimu_dense* d, int64_t K, double dtau, const double* sh
             if (grad_u) grad_u[i] = du + h[4 * R + i];
             if (grad_t) grad_t[i] = dt + h[5 * R + i];
         }
+        return RIMU_OK;
+    };
+    const int rc = run();
+    cleanup();
+    return rc;
+}
+
+int rimu_dense_ground_state(rimu_dense* d, const double* v0, int64_t max_iters, double tol,
+                            double* energy, int64_t* iters) {
+    if (!d || !v0 || !energy) return fail(RIMU_ERR_ARG, "null argument");
+    if (max_iters < 1 || !(tol > 0)) return fail(RIMU_ERR_ARG, "need max_iters >= 1 and tol > 0");
+    const int R = d->R;
+    const size_t n = (size_t)d->D * R;
+    const int64_t cap = std::min<int64_t>(max_iters, d->D);
+    constexpr int64_t CHECK = 16;
+    // beta^2 at or below this ends a right-hand side's Krylov space (rounding leaves
+    // beta ~ 1e-14 |H| there; a true beta this small would move the Ritz value by ~beta^2)
+    constexpr double FLOOR2 = 1e-20;
+    double a_h[MAXR], b_h[MAXR];
+    for (int i = 0; i < R; ++i) { a_h[i] = 0; b_h[i] = 1; }
+    const DenseParams pH = params(d, a_h, b_h);
+    DevBuf<double> v, vprev, w, hist;
+    // hist: alpha [cap][R], then beta^2 [cap + 1][R] (beta^2[0] = 0), then |v0|^2 [R]
+    auto cleanup = [&]() { v.release(); vprev.release(); w.release(); hist.release(); };
+    auto run = [&]() -> int {
+        HIP_TRY(v.alloc(n));
+        HIP_TRY(vprev.alloc(n));
+        HIP_TRY(w.alloc(n));
+        HIP_TRY(hist.alloc((size_t)(2 * cap + 2) * R));
+        double* alpha = hist.p;
+        double* beta2 = hist.p + cap * R;
+        double* n0 = hist.p + (2 * cap + 1) * R;
+        HIP_TRY(hipMemsetAsync(hist.p, 0, (size_t)(2 * cap + 2) * R * 8, d->stream));
+        HIP_TRY(hipMemsetAsync(vprev.p, 0, n * 8, d->stream));
+        HIP_TRY(hipMemcpyAsync(w.p, v0, n * 8, hipMemcpyHostToDevice, d->stream));
+        // v_0 = v0 / |v0| (lanczos_update with alpha = beta = 0 only normalises)
+        if (int rc = lanczos_update(d, w.p, vprev.p, vprev.p, beta2, beta2, n0, v.p, 0.0)) return rc;
+        std::vector<double> h_a, h_b2, h_n0(R), last(R, INFINITY), cur(R);
+        std::vector<int64_t> m_eff(R, -1);
+        int64_t j = 0;
+        for (; j < cap; ++j) {
+            if (int rc = apply(d, pH, v.p, w.p)) return rc;
+            if (int rc = dot(d, v.p, w.p, alpha + j * R)) return rc;
+            if (int rc = lanczos_update(d, w.p, v.p, vprev.p, alpha + j * R, beta2 + j * R,
+                                        beta2 + (j + 1) * R, vprev.p, FLOOR2)) return rc;
+            std::swap(v.p, vprev.p);
+            const bool last_iter = j + 1 == cap;
+            if ((j + 1) % CHECK != 0 && !last_iter) continue;
+            const int64_t m = j + 1;
+            h_a.resize((size_t)m * R);
+            h_b2.resize((size_t)(m + 1) * R);
+            HIP_TRY(hipMemcpyAsync(h_a.data(), alpha, (size_t)m * R * 8, hipMemcpyDeviceToHost, d->stream));
+            HIP_TRY(hipMemcpyAsync(h_b2.data(), beta2, (size_t)(m + 1) * R * 8, hipMemcpyDeviceToHost, d->stream));
+            HIP_TRY(hipMemcpyAsync(h_n0.data(), n0, R * 8, hipMemcpyDeviceToHost, d->stream));
+            HIP_TRY(hipStreamSynchronize(d->stream));
+            bool done = true;
+            std::vector<double> a(m), b2(m + 1);
+            for (int i = 0; i < R; ++i) {
+                if (!(h_n0[i] > 0)) return fail(RIMU_ERR_ARG, "start vector is zero");
+                int64_t mi = m;
+                for (int64_t k = 1; k <= m; ++k) {
+                    if (!(h_b2[k * R + i] > FLOOR2)) { mi = k; break; }  // exhausted after k vectors
+                }
+                for (int64_t k = 0; k < mi; ++k) {
+                    a[k] = h_a[k * R + i];
+                    b2[k] = h_b2[k * R + i];
+                }
+                cur[i] = tridiag_lowest(a.data(), b2.data(), mi);
+                const bool exhausted = !(h_b2[mi * R + i] > FLOOR2);
+                const bool conv = exhausted || std::fabs(cur[i] - last[i]) <= tol * std::max(1.0, std::fabs(cur[i]));
+                if (!conv) done = false;
+                last[i] = cur[i];
+            }
+            if (done) { ++j; break; }
+        }
+        for (int i = 0; i < R; ++i) energy[i] = cur[i];
+        if (iters) *iters = j;
         return RIMU_OK;
     };
     const int rc = run();

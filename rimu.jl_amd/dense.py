@@ -109,6 +109,19 @@ class DenseHamiltonians:
         outs["ckpt"] = ckpt
         return outs
 
+    def ground_state(self, v0=None, max_iters: int = 3000, tol: float = 1e-12):
+        """Lowest eigenvalue of each Hamiltonian by Lanczos on the device (the energy of
+        ``solve(ExactDiagonalizationProblem(H))``, ext/KrylovKitExt.jl:36-78), started from
+        ``v0`` (default: the first Hamiltonian's start address).  Returns (energies, iterations)."""
+        if v0 is None:
+            v0 = self.unit_vector(self.hamiltonians[0].starting_address())
+        v0 = self._vec(v0)
+        e = np.zeros(self.R)
+        it = ctypes.c_int64(0)
+        _native.check(_native.lib().rimu_dense_ground_state(self._h, _p(v0), int(max_iters), float(tol),
+                                                            _p(e), ctypes.byref(it)))
+        return e, int(it.value)
+
     def unit_vector(self, address) -> np.ndarray:
         """The start vector e_address in every right-hand side."""
         b = self.basis()

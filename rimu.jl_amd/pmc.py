@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native
 from .addresses import BoseFS
-from .dictvectors import DVec, DeviceDVec
+from .dictvectors import DVec, DeviceDVec, FrozenDVec
 from .hamiltonians import AbstractHamiltonian
 from .strategies import (ConstantTimeStep, DefaultShiftParameters, DoubleLogUpdate,
                          ProjectedEnergy)
@@ -80,12 +80,16 @@ def default_starting_vector(address: BoseFS, style=None, population=10) -> DVec:
 
 
 def _start_vector(problem: ProjectorMonteCarloProblem) -> DVec:
+    """``_set_up_starting_vectors`` (pmc_simulation.jl:33-85): an address or a frozen
+    vector gets the problem's style; a ``DVec`` is copied with its own style."""
     s = problem.start_at
     if isinstance(s, BoseFS):
         return default_starting_vector(s, problem.style)
+    if isinstance(s, FrozenDVec):
+        return DVec(list(s.pairs), style=problem.style)
     if isinstance(s, DVec):
-        return DVec(dict(s.storage), style=problem.style)
-    raise TypeError("start_at must be a BoseFS or a DVec")
+        return DVec(dict(s.storage), style=s.style)
+    raise TypeError("start_at must be a BoseFS, a FrozenDVec or a DVec")
 
 
 def rayleigh_quotient_dvec(h: AbstractHamiltonian, v: DVec) -> float:
@@ -119,11 +123,12 @@ class PMCSimulation:
         tw = abs(getattr(problem.shift_strategy, "target_walkers", 1000))
         max_entries = problem.max_length + 1
         max_spawns = int(min(max(4 * tw + 4096, 4 * len(v0) + 4096), 2 ** 32 - 2))
-        self.engine = _native.Engine(h.descriptor(), problem.style.descriptor(), seed,
+        self.style = v0.style
+        self.engine = _native.Engine(h.descriptor(), self.style.descriptor(), seed,
                                      max_entries, max_spawns)
         keys, vals = v0.to_arrays()
         self.engine.upload(keys, vals)
-        self.v = DeviceDVec(self.engine, h, problem.style)
+        self.v = DeviceDVec(self.engine, h, self.style)
         # initial shift: Rayleigh quotient of the start vector unless given
         shift = problem.shift
         if shift is None:
@@ -184,7 +189,7 @@ class PMCSimulation:
             self._report("len", length)
             for k, val in shift_stats.items():
                 self._report(k, val)
-            for name, val in zip(p.style.step_stat_names(), p.style.stat_values(st)):
+            for name, val in zip(self.style.step_stat_names(), self.style.stat_values(st)):
                 self._report(name, val)
             for pe in self.projected:
                 self._report(pe.vproj_name, st.vproj)

@@ -9,6 +9,8 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 from __graft_entry__ import load_package  # noqa: E402
 
+load_package()  # registers the package as ``rimu_amd`` (the native library loads lazily)
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through librimu_hip.so)")

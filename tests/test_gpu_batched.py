@@ -4,17 +4,18 @@ import numpy as np
 import pytest
 
 import oracle as O
-import statstools as ST
+from rimu_amd import statstools as ST
 
 pytestmark = pytest.mark.gpu
 
 
-def specs(rimu, cases, zeta=0.08, target=500, dtau=1e-3):
+def specs(rimu, cases, zeta=0.08, target=500, dtau=1e-3, max_length=None):
     out = []
     for i, (M, N, u) in enumerate(cases):
         h = rimu.HubbardReal1D(rimu.near_uniform(N, M), u=u, t=1.0)
         xi = 0.0 if zeta == 0 else None
-        out.append(rimu.RunSpec(h, target_walkers=target, dtau=dtau, seed=100 + i, zeta=zeta, xi=xi))
+        out.append(rimu.RunSpec(h, target_walkers=target, dtau=dtau, seed=100 + i, zeta=zeta, xi=xi,
+                                max_length=max_length))
     return out
 
 
@@ -24,7 +25,7 @@ CASES = [(6, 6, 4.0), (9, 5, 2.5), (12, 12, 8.0), (7, 4, 1.0)]
 def test_batch_matches_plain_engine_frozen_shift(gpu):
     """With zeta = xi = 0 the shift stays fixed, so every step of a batched run must equal
     the plain engine's step from the same state (integers: bit-exact)."""
-    sp = specs(gpu, CASES, zeta=0.0, dtau=0.01)
+    sp = specs(gpu, CASES, zeta=0.0, dtau=0.01, max_length=200_000)
     nsteps = 60
     b = gpu.BatchedRuns(sp, history_steps=nsteps)
     b.step(nsteps, steps_per_graph=1)
@@ -81,17 +82,23 @@ def test_autotune_keeps_state(gpu):
     np.testing.assert_array_equal(ha["shift"], hb["shift"])
 
 
-def test_batch_status_dead_population(gpu):
-    """A huge u with a very negative fixed shift kills the population; the run stops there
-    (Rimu: "Population ... is dead. Aborting.") while the others continue."""
-    h_dead = gpu.HubbardReal1D(gpu.near_uniform(6, 6), u=20.0)
-    h_ok = gpu.HubbardReal1D(gpu.near_uniform(6, 6), u=2.0)
-    sp = [gpu.RunSpec(h_dead, target_walkers=1000, dtau=0.1, seed=1, shift=-60.0, zeta=0.0, xi=0.0),
-          gpu.RunSpec(h_ok, target_walkers=1000, dtau=1e-3, seed=2)]
-    b = gpu.BatchedRuns(sp, history_steps=100)
+def test_batch_status_dead_population_and_max_length(gpu):
+    """The reference's abort set-ups (test/projector_monte_carlo_problem.jl:252-312): a dead
+    population with BoseFS(2,3), u = 20, shift 0; a max_length overflow with
+    BoseFS(1,3,5,2,1), dtau = 0.1, 100 target walkers.  Each run stops on its own; the
+    healthy run beside them continues."""
+    dead = gpu.RunSpec(gpu.HubbardReal1D(gpu.BoseFS(2, 3), u=20.0), target_walkers=1000,
+                       dtau=0.01, seed=7, shift=0.0, zeta=0.0, xi=0.0)
+    blow = gpu.RunSpec(gpu.HubbardReal1D(gpu.BoseFS(1, 3, 5, 2, 1)), target_walkers=100,
+                       dtau=0.1, seed=7)
+    ok = gpu.RunSpec(gpu.HubbardReal1D(gpu.BoseFS(2, 3), u=20.0), target_walkers=1000,
+                     dtau=0.01, seed=7)
+    b = gpu.BatchedRuns([dead, blow, ok], history_steps=100)
     b.step(100, steps_per_graph=10)
     st, ln = b.status()
-    assert st[0] == 1 and st[1] == 0 and ln[1] > 0
+    assert st[0] == 1 and st[1] == 2 and st[2] == 0 and ln[2] > 0
+    h = b.history()
+    assert h["norm"].shape[0] == 100
 
 
 @pytest.mark.parametrize("M,N,u", [(6, 6, 6.0), (8, 5, 3.0), (7, 7, 2.0)])

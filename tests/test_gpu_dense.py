@@ -138,3 +138,43 @@ def test_full_size_config_apply(gpu):
     w = d.apply(a, b, v)
     wo = O.apply_affine([O.real1d(M, N, u, 1.0) for u in us], basis, a, b, v)
     np.testing.assert_allclose(w, wo, rtol=1e-12, atol=1e-12 * np.abs(wo).max())
+
+
+def test_ground_state_known_answers(gpu):
+    """Lanczos on the device against the reference's ED constants: HubbardReal1D 5x5 u=1
+    (test/lomc.jl:546), 7x7 u=6 (test/mpi_runtests.jl:160)."""
+    for (M, N, u), E0 in [((5, 5, 1.0), -8.280991746582686), ((7, 7, 6.0), -4.628524493494574)]:
+        d, _ = make(gpu, M, N, [u], [1.0])
+        e, it = d.ground_state()
+        assert e[0] == pytest.approx(E0, abs=1e-9), (M, N, u, it)
+        d.close()
+
+
+def test_ground_state_against_scipy(gpu):
+    """Four (u, t) at once, and the M = N = 10, u = 6 case (dimension 92,378) against the
+    oracle's sparse matrix and scipy eigsh."""
+    M, N = 8, 5
+    us, ts = [1.0, 3.0, 6.5, 10.0], [1.0, 0.5, 1.0, 2.0]
+    d, _ = make(gpu, M, N, us, ts)
+    e, it = d.ground_state()
+    basis = d.basis()
+    for i in range(4):
+        ref = O.ed_energies(O.real1d(M, N, us[i], ts[i]), basis, 1)[0]
+        assert e[i] == pytest.approx(ref, abs=1e-9), (i, it)
+    d.close()
+    d, _ = make(gpu, 10, 10, [6.0], [1.0])
+    assert d.dim == 92378
+    e, it = d.ground_state()
+    ref = O.ed_energies(O.real1d(10, 10, 6.0, 1.0), d.basis(), 1)[0]
+    assert e[0] == pytest.approx(ref, abs=1e-9), it
+    d.close()
+
+
+def test_ground_state_exhausted_krylov_space(gpu):
+    """Dimension 6 (M = 3, N = 2): the symmetric Krylov space of the start address ends
+    after a few vectors; the Ritz value is then exact."""
+    d, _ = make(gpu, 3, 2, [2.0], [1.0])
+    e, it = d.ground_state(max_iters=100)
+    w = np.linalg.eigvalsh(O.sparse_hamiltonian(O.real1d(3, 2, 2.0, 1.0), d.basis()).toarray())
+    assert e[0] == pytest.approx(w[0], abs=1e-12)
+    assert it <= 6

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-import statstools as ST
+from rimu_amd import statstools as ST
 
 
 # ----------------------------------------------------------------------------------------
