@@ -61,6 +61,8 @@ def kernel_alg_bytes(name, S, L, Lnew, vsize=8):
         return S * 8
     if name == "merge":
         return S * rec + L * (8 + vsize) + Lnew * rec
+    if name == "compact":
+        return 2 * Lnew * rec
     if name == "spawn":
         return L * (rec + 8) + S * rec
     if name == "plan":

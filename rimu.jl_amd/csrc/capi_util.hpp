@@ -29,13 +29,16 @@ template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    // Grows to at least `count`; a buffer that already exists grows by a quarter more than
+    // asked, so sizes that creep upward from step to step do not reallocate every step.
     hipError_t alloc(size_t count) {
         if (count <= n) return hipSuccess;
+        const size_t want = n == 0 ? count : std::max(count, count + count / 4);
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) n = count;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
         return e;
     }
     void release() {

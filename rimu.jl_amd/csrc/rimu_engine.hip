@@ -96,7 +96,8 @@ struct rimu_engine {
     DevBuf<uint64_t> skeys[2];
     DevBuf<unsigned char> svals[2];
     DevBuf<uint64_t> status;
-    DevBuf<int64_t> tile_first, splits;
+    DevBuf<int64_t> tile_first, splits, tile_cnt;
+    DevBuf<uint64_t> prevk, slot_k, slot_v;  // merge: per-tile survivor slots (values as raw 8 bytes)
     DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev, chunk_sums;
     DevBuf<unsigned char> misc;  // counters and totals (see layout below)
     // projector
@@ -217,6 +218,41 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
     return RIMU_OK;
 }
 
+// Merge the sorted spawn records (ak, av; S) with the vector keys and diagonal values
+// (bk, bv; L) into (ok, ov) of capacity cap: tile splits, merge into per-tile slots, scan
+// of the tile counts, compaction.  The new length lands in dm->L_new; `tiles` receives the
+// number of merge tiles (rows of merge_p).
+template <class V>
+int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
+               const uint64_t* bk, const V* bv, int64_t L, uint64_t* ok, V* ov, int64_t cap,
+               rimu_engine::Misc* dm, int64_t* tiles) {
+    constexpr int64_t TILE = NT * MERGE_IPT;
+    const int64_t nt = std::max<int64_t>(ceil_div(S + L, TILE), 1);
+    *tiles = nt;
+    HIP_TRY(e->merge_p.alloc((size_t)nt * 3));
+    HIP_TRY(e->splits.alloc(nt + 1));
+    HIP_TRY(e->prevk.alloc(nt + 1));
+    HIP_TRY(e->tile_cnt.alloc(nt + 1));
+    HIP_TRY(e->slot_k.alloc((size_t)nt * TILE));
+    HIP_TRY(e->slot_v.alloc((size_t)nt * TILE));
+    HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nt, 0, sizeof(int64_t), e->stream));
+    merge_split_kernel<<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(ak, S, bk, L, TILE, nt, e->splits.p,
+                                                                    e->prevk.p);
+    hipEvent_t ev{};
+    e->timer.begin("merge", e->stream, &ev);
+    merge_kernel<V><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p, (V*)e->slot_v.p,
+                                              e->tile_cnt.p, e->merge_p.p, e->splits.p, e->prevk.p);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    e->timer.begin("compact", e->stream, &ev);
+    if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nt + 1)) return rc;
+    compact_kernel<V><<<nt, NT, 0, e->stream>>>(e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok,
+                                                ov, cap, &dm->L_new, &dm->flags);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
 template <class V>
 int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
               rimu_step_stats* out) {
@@ -296,21 +332,10 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
 
     // 4. merge with the diagonal survivors, fold, compress, count
     const int nxt = 1 - e->cur;
-    const int64_t merge_tiles = std::max<int64_t>(ceil_div(S + L, NT * MERGE_IPT), 1);
-    HIP_TRY(e->status.alloc(merge_tiles));
-    HIP_TRY(e->merge_p.alloc((size_t)merge_tiles * 3));
-    HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * merge_tiles, e->stream));
-    HIP_TRY(hipMemsetAsync(&dm->tile_counter, 0, sizeof(uint32_t), e->stream));
-    HIP_TRY(e->splits.alloc(merge_tiles + 1));
-    merge_split_kernel<<<ceil_div(merge_tiles + 1, NT), NT, 0, e->stream>>>(
-        e->skeys[sorted].p, S, vk, L, NT * MERGE_IPT, merge_tiles, e->splits.p);
-    e->timer.begin("merge", e->stream, &ev);
-    merge_kernel<V><<<merge_tiles, NT, 0, e->stream>>>(
-        a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk, (const V*)e->dvals.p, L,
-        e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, e->status.p, &dm->tile_counter,
-        &dm->L_new, &dm->flags, e->merge_p.p, e->splits.p);
-    e->timer.end(e->stream);
-    HIP_TRY(hipGetLastError());
+    int64_t merge_tiles = 0;
+    if (int rc = merge_into<V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
+                               (const V*)e->dvals.p, L, e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
+                               e->max_entries, dm, &merge_tiles)) return rc;
 
     // 5. projections and statistics
     if (e->np + e->nh > 0) {
@@ -345,7 +370,6 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->timer.collect();
-    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "merge look-back did not complete");
     if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) {
         out->overflow = 1;
         return fail(RIMU_ERR_CAPACITY, "vector capacity (max_entries) exceeded");
@@ -621,25 +645,16 @@ int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int6
     if (int rc = radix_sort<V>(e, n, nbits, varying, 0, &sorted)) return rc;
     StepArgs a = make_args(e, 0, 0.0, 1.0, 1.0);
     a.compression = 0;
-    const int64_t tiles = std::max<int64_t>(ceil_div(n, NT * MERGE_IPT), 1);
-    HIP_TRY(e->status.alloc(tiles));
-    HIP_TRY(e->merge_p.alloc((size_t)tiles * 3));
-    HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * tiles, e->stream));
     HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
-    HIP_TRY(e->splits.alloc(tiles + 1));
-    merge_split_kernel<<<ceil_div(tiles + 1, NT), NT, 0, e->stream>>>(
-        e->skeys[sorted].p, n, e->vkeys[0].p, 0, NT * MERGE_IPT, tiles, e->splits.p);
-    merge_kernel<V><<<tiles, NT, 0, e->stream>>>(a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n,
-                                                 e->vkeys[0].p, (const V*)e->dvals.p, 0, e->vkeys[1].p,
-                                                 (V*)e->vvals[1].p, e->max_entries, e->status.p,
-                                                 &dm->tile_counter, &dm->L_new, &dm->flags, e->merge_p.p,
-                                                 e->splits.p);
-    HIP_TRY(hipGetLastError());
+    int64_t tiles = 0;
+    if (int rc = merge_into<V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n, e->vkeys[0].p,
+                               (const V*)e->dvals.p, 0, e->vkeys[1].p, (V*)e->vvals[1].p, e->max_entries,
+                               dm, &tiles)) return rc;
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "look-back did not complete");
     const int64_t m = e->h_misc->L_new;
     *n_out = m;
+    if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) return fail(RIMU_ERR_CAPACITY, "annihilation buffer too small");
     if (m > cap) return fail(RIMU_ERR_CAPACITY, "output capacity too small");
     if (m > 0) {
         HIP_TRY(hipMemcpyAsync(out_keys, e->vkeys[1].p, m * 8, hipMemcpyDeviceToHost, e->stream));

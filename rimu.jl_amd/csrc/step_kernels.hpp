@@ -406,7 +406,8 @@ __global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restric
 __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restrict__ ak, int64_t S,
                                                          const uint64_t* __restrict__ bk, int64_t L,
                                                          int64_t tile_size, int64_t ntiles,
-                                                         int64_t* __restrict__ splits) {
+                                                         int64_t* __restrict__ splits,
+                                                         uint64_t* __restrict__ prevk) {
     const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (t > ntiles) return;
     const int64_t d = min(t * tile_size, S + L);
@@ -416,6 +417,11 @@ __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restr
         if (ak[mid] < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
     }
     splits[t] = lo;
+    // key of the merged element just before the tile (read only when the tile starts past 0)
+    const int64_t bi = d - lo;
+    const uint64_t pa = lo > 0 ? ak[lo - 1] : 0;
+    const uint64_t pb = bi > 0 ? bk[bi - 1] : 0;
+    prevk[t] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -543,35 +549,42 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
 // unique).  Merged order: by key, B before A on equal keys.  Each address is one segment;
 // the thread owning the segment's first position folds it sequentially:
 //   acc = 0 (+ diag) + spawn_1 + spawn_2 + ...
-// then applies ThresholdCompression and emits the survivor.  Outputs are compacted in
-// merged order by the look-back.  partials[j * gridDim.x + tile], j = 0..2: len_before,
-// len, norm.
+// then applies ThresholdCompression and keeps the survivor.  A tile writes its survivors,
+// in merged order, to its own slot range [tile * TILE, tile * TILE + count) of the slot
+// buffer and its count to tile_cnt[tile]; compact_kernel then moves them to their final
+// positions, so no tile waits on another.  partials[j * gridDim.x + tile], j = 0..2:
+// len_before, len, norm.
 // ---------------------------------------------------------------------------------------
+constexpr int MERGE_EXTRA = 64;  // spawn records staged past the tile for runs that cross it
+static_assert(MERGE_IPT <= 32, "merge candidate masks are 32-bit");
+static_assert(MERGE_EXTRA <= NT, "one extra record per thread at most");
+
 template <class V>
 __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
                                                    const V* __restrict__ av, int64_t S,
                                                    const uint64_t* __restrict__ bk,
                                                    const V* __restrict__ bv, int64_t L,
-                                                   uint64_t* __restrict__ ok, V* __restrict__ ov,
-                                                   int64_t cap, uint64_t* status,
-                                                   uint32_t* tile_counter, int64_t* total_out,
-                                                   int* flags, double* __restrict__ partials,
-                                                   const int64_t* __restrict__ splits) {
+                                                   uint64_t* __restrict__ slot_k, V* __restrict__ slot_v,
+                                                   int64_t* __restrict__ tile_cnt,
+                                                   double* __restrict__ partials,
+                                                   const int64_t* __restrict__ splits,
+                                                   const uint64_t* __restrict__ prevk) {
     constexpr int TILE = NT * MERGE_IPT;
-    // the tile's A range followed by its B range (na + nb <= TILE)
-    __shared__ uint64_t s_key[TILE];
-    __shared__ V s_val[TILE];
-    __shared__ uint64_t s_base;
+    // the tile's A range, its B range, then up to MERGE_EXTRA records of A past the tile
+    __shared__ uint64_t s_key[TILE + MERGE_EXTRA];
+    __shared__ V s_val[TILE + MERGE_EXTRA];
 
-    const uint32_t tile = next_tile_id(tile_counter);
+    const int64_t tile = blockIdx.x;
     const int64_t total = S + L;
-    const int64_t d0 = (int64_t)tile * TILE;
+    const int64_t d0 = tile * TILE;
     const int64_t d1 = min(d0 + TILE, total);
     const int64_t a0 = splits[tile], a1 = splits[tile + 1];
     const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-    // stage A then B: all loads of a thread are issued before the first LDS store, so a
-    // wave waits for one memory round trip instead of one per element
+    const int ne = (int)min((int64_t)MERGE_EXTRA, S - a1);
+    const uint64_t tile_prev = prevk[tile];
+    // stage A, B and the extra records: all loads of a thread are issued before the first
+    // LDS store, so a wave waits for one memory round trip instead of one per element
     {
         uint64_t rk[MERGE_IPT];
         V rv[MERGE_IPT];
@@ -586,11 +599,16 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                 rv[r] = *vp;
             }
         }
+        uint64_t ek = 0;
+        V ev = (V)0;
+        const int x = (int)threadIdx.x;
+        if (x < ne) { ek = ak[a1 + x]; ev = av[a1 + x]; }
 #pragma unroll
         for (int r = 0; r < MERGE_IPT; ++r) {
             const int i = (int)threadIdx.x + r * NT;
             if (i < na + nb) { s_key[i] = rk[r]; s_val[i] = rv[r]; }
         }
+        if (x < ne) { s_key[na + nb + x] = ek; s_val[na + nb + x] = ev; }
     }
     __syncthreads();
     const uint64_t* s_a = s_key;
@@ -619,17 +637,19 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
     } else if (d0 > 0) {
         have_prev = true;
-        const uint64_t pa = a0 > 0 ? ak[a0 - 1] : 0;
-        const uint64_t pb = b0 > 0 ? bk[b0 - 1] : 0;
-        prev = (a0 > 0 && b0 > 0) ? max(pa, pb) : (a0 > 0 ? pa : pb);
+        prev = tile_prev;
     }
 
-    uint64_t out_k[MERGE_IPT];
-    V out_v[MERGE_IPT];
-    int count = 0;
+    // candidates stay in the slot of their position j (static register indices); bit j of
+    // cvalid marks a segment head with a nonzero folded value
+    uint64_t ck[MERGE_IPT];
+    V cv[MERGE_IPT];
+    uint32_t cvalid = 0;
     double lb = 0, ln = 0, nrm = 0;
 #pragma unroll
     for (int j = 0; j < MERGE_IPT; ++j) {
+        ck[j] = 0;
+        cv[j] = (V)0;
         if (j < n_here) {
             const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
             const uint64_t key = take_b ? s_b[ib] : s_a[ia];
@@ -639,29 +659,19 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                 if (take_b) acc = acc + s_val[na + ib];
                 int la = ia;
                 while (la < na && s_a[la] == key) { acc = acc + s_val[la]; ++la; }
-                if (la == na) {  // the run may continue past the tile
-                    int64_t ga = a0 + la;
-                    while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
+                if (la == na) {  // the run may continue past the tile: staged records first
+                    int x = 0;
+                    while (x < ne && s_key[na + nb + x] == key) { acc = acc + s_val[na + nb + x]; ++x; }
+                    if (x == ne) {
+                        int64_t ga = a1 + ne;
+                        while (ga < S && ak[ga] == key) { acc = acc + av[ga]; ++ga; }
+                    }
                 }
                 if (acc != (V)0) {
                     lb += 1;
-                    if constexpr (!is_int_v<V>::value) {
-                        if (a.compression == 1) {
-                            const double th = a.comp_threshold;
-                            const double prob = fabs((double)acc) / th;
-                            if (prob < 1) {
-                                const U4 r = draw(a.seed, key, 0, a.step, TAG_COMPRESS);
-                                acc = (prob > u53(r.x, r.y)) ? th * sgn((double)acc) : 0.0;
-                            }
-                        }
-                    }
-                    if (acc != (V)0) {
-                        out_k[count] = key;
-                        out_v[count] = acc;
-                        ++count;
-                        ln += 1;
-                        nrm += fabs((double)acc);
-                    }
+                    ck[j] = key;
+                    cv[j] = acc;
+                    cvalid |= 1u << j;
                 }
             }
             prev = key;
@@ -669,36 +679,117 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
             if (take_b) ++ib; else ++ia;
         }
     }
+    if constexpr (!is_int_v<V>::value) {
+        if (a.compression == 1) {
+            // ThresholdCompression: the values below the threshold of a wavefront are listed
+            // in LDS and drawn with every lane busy (a draw depends only on the address)
+            const double th = a.comp_threshold;
+            uint32_t need = 0;
+#pragma unroll
+            for (int j = 0; j < MERGE_IPT; ++j)
+                if (((cvalid >> j) & 1u) && fabs((double)cv[j]) / th < 1) need |= 1u << j;
+            const int lane = threadIdx.x & (WAVE - 1);
+            const int wid = threadIdx.x / WAVE;
+            const int nneed = __popc(need);
+            const int incl = wave_inclusive_scan(nneed);
+            const int wtot = __shfl(incl, WAVE - 1, WAVE);
+            const int mybase = incl - nneed;
+            uint64_t* wk = s_key + wid * (WAVE * MERGE_IPT);
+            V* wv = s_val + wid * (WAVE * MERGE_IPT);
+            __syncthreads();  // the tile's inputs are no longer read
+#pragma unroll
+            for (int j = 0; j < MERGE_IPT; ++j) {
+                if ((need >> j) & 1u) {
+                    const int q = mybase + __popc(need & ((1u << j) - 1u));
+                    wk[q] = ck[j];
+                    wv[q] = cv[j];
+                }
+            }
+            __syncthreads();
+            for (int q = lane; q < wtot; q += WAVE) {
+                const uint64_t key = wk[q];
+                const double x = (double)wv[q];
+                const U4 r = draw(a.seed, key, 0, a.step, TAG_COMPRESS);
+                wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < MERGE_IPT; ++j) {
+                if ((need >> j) & 1u) {
+                    const int q = mybase + __popc(need & ((1u << j) - 1u));
+                    cv[j] = wv[q];
+                    if (cv[j] == (V)0) cvalid &= ~(1u << j);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MERGE_IPT; ++j) {
+        if ((cvalid >> j) & 1u) {
+            ln += 1;
+            nrm += fabs((double)cv[j]);
+        }
+    }
+    const int count = __popc(cvalid);
     int btotal;
     const int ex = block_exclusive_scan<int, NT>(count, &btotal);  // ends with a barrier
-    if (threadIdx.x < WAVE) {
-        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)btotal, flags);
-        if (threadIdx.x == 0) s_base = b;
-    }
     // stage the survivors in LDS (the input keys are no longer read) for coalesced writes
 #pragma unroll
     for (int j = 0; j < MERGE_IPT; ++j) {
-        if (j < count) {
-            s_key[ex + j] = out_k[j];
-            s_val[ex + j] = out_v[j];
+        if ((cvalid >> j) & 1u) {
+            const int q = ex + __popc(cvalid & ((1u << j) - 1u));
+            s_key[q] = ck[j];
+            s_val[q] = cv[j];
         }
     }
     __syncthreads();
-    const int64_t base = (int64_t)s_base;
-    if (base + btotal > cap) {
-        if (threadIdx.x == 0) atomicOr(flags, FLAG_VECTOR_OVERFLOW);
-    } else {
-        for (int i = threadIdx.x; i < btotal; i += NT) {
-            ok[base + i] = s_key[i];
-            ov[base + i] = s_val[i];
-        }
+    for (int i = threadIdx.x; i < btotal; i += NT) {
+        slot_k[d0 + i] = s_key[i];
+        slot_v[d0 + i] = s_val[i];
     }
     block_sum3<NT>(lb, ln, nrm);
     if (threadIdx.x == 0) {
+        tile_cnt[tile] = btotal;
         partials[(int64_t)0 * gridDim.x + tile] = lb;
         partials[(int64_t)1 * gridDim.x + tile] = ln;
         partials[(int64_t)2 * gridDim.x + tile] = nrm;
-        if (tile == gridDim.x - 1) *total_out = base + btotal;
+    }
+}
+
+// compact: tile t's survivors move from its slot range to [offs[t], offs[t+1]) (offs is the
+// exclusive scan of the tile counts, offs[ntiles] the new length).  If the new length
+// exceeds cap nothing is written and the overflow flag is raised.
+template <class V>
+__global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict__ slot_k,
+                                                     const V* __restrict__ slot_v,
+                                                     const int64_t* __restrict__ offs, int64_t ntiles,
+                                                     uint64_t* __restrict__ ok, V* __restrict__ ov,
+                                                     int64_t cap, int64_t* total_out, int* flags) {
+    constexpr int TILE = NT * MERGE_IPT;
+    const int64_t tile = blockIdx.x;
+    const int64_t total = offs[ntiles];
+    if (total > cap) {
+        if (tile == 0 && threadIdx.x == 0) {
+            atomicOr(flags, FLAG_VECTOR_OVERFLOW);
+            *total_out = total;
+        }
+        return;
+    }
+    if (tile == 0 && threadIdx.x == 0) *total_out = total;
+    const int64_t off = offs[tile];
+    const int n = (int)(offs[tile + 1] - off);
+    const int64_t src = tile * TILE;
+    uint64_t k[MERGE_IPT];
+    V v[MERGE_IPT];
+#pragma unroll
+    for (int r = 0; r < MERGE_IPT; ++r) {
+        const int i = (int)threadIdx.x + r * NT;
+        if (i < n) { k[r] = slot_k[src + i]; v[r] = slot_v[src + i]; }
+    }
+#pragma unroll
+    for (int r = 0; r < MERGE_IPT; ++r) {
+        const int i = (int)threadIdx.x + r * NT;
+        if (i < n) { ok[off + i] = k[r]; ov[off + i] = v[r]; }
     }
 }
 
