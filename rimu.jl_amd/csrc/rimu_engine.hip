@@ -92,7 +92,7 @@ struct rimu_engine {
     DevBuf<unsigned char> dvals;
     DevBuf<int64_t> woff;
     DevBuf<int64_t> scan_tmp;
-    DevBuf<uint32_t> rcounts, rtmp;
+    DevBuf<uint32_t> rcounts, rchunks, rtmp;
     DevBuf<uint64_t> skeys[2];
     DevBuf<unsigned char> svals[2];
     DevBuf<uint64_t> status;
@@ -195,7 +195,9 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
     int src = in;
     if (n <= 1) { *result = src; return RIMU_OK; }
     const int64_t tiles = ceil_div(n, SORT_TILE);
+    const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
+    HIP_TRY(e->rchunks.alloc((size_t)nchunks * RADIX));
     for (int shift = 0; shift < nbits; shift += RADIX_BITS) {
         if (((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
         hipEvent_t ev{};
@@ -203,9 +205,10 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
         radix_count_kernel<<<tiles, NT, 0, e->stream>>>(e->skeys[src].p, n, shift, e->rcounts.p, tiles);
         e->timer.end(e->stream);
         e->timer.begin("radix_scan", e->stream, &ev);
-        int rc = scan_u32(e, e->rcounts.p, e->rcounts.p, tiles * RADIX);
+        radix_chunk_sums_kernel<<<nchunks, RADIX, 0, e->stream>>>(e->rcounts.p, tiles, nchunks, e->rchunks.p);
+        if (int rc = scan_u32(e, e->rchunks.p, e->rchunks.p, nchunks * RADIX)) return rc;
+        radix_chunk_offsets_kernel<<<nchunks, RADIX, 0, e->stream>>>(e->rcounts.p, tiles, nchunks, e->rchunks.p);
         e->timer.end(e->stream);
-        if (rc) return rc;
         e->timer.begin("radix_scatter", e->stream, &ev);
         radix_scatter_kernel<V><<<tiles, NT, 0, e->stream>>>(
             e->skeys[src].p, (const V*)e->svals[src].p, n, shift, e->rcounts.p, tiles,
@@ -472,8 +475,9 @@ int rimu_engine_destroy(rimu_engine* e) {
         e->skeys[b].release();
         e->svals[b].release();
     }
-    e->tile_first.release(); e->splits.release();
-    e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release();
+    e->tile_first.release(); e->splits.release(); e->tile_cnt.release();
+    e->prevk.release(); e->slot_k.release(); e->slot_v.release();
+    e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release(); e->rchunks.release();
     e->rtmp.release(); e->status.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
@@ -758,10 +762,8 @@ extern "C" int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t 
                 HIP_TRY(hipMemcpyAsync(nv.p, e->vvals[b].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
                 HIP_TRY(hipStreamSynchronize(e->stream));
             }
-            e->vkeys[b].release();
-            e->vvals[b].release();
-            e->vkeys[b] = nk;
-            e->vvals[b] = nv;
+            e->vkeys[b] = std::move(nk);
+            e->vvals[b] = std::move(nv);
         }
         HIP_TRY(e->dvals.alloc(max_entries * 8));
         HIP_TRY(e->woff.alloc(max_entries + 1));

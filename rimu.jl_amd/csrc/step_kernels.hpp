@@ -24,6 +24,7 @@ constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_IPT = 16;     // keys per thread per radix tile
 constexpr int SORT_TILE = NT * SORT_IPT;
+static_assert(RADIX == NT, "radix kernels use one thread per digit");
 
 enum { FLAG_SPAWN_OVERFLOW = 1, FLAG_LOOKBACK = 2, FLAG_VECTOR_OVERFLOW = 4 };
 
@@ -448,7 +449,47 @@ __global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restr
         if (idx < n) atomicAdd(&hist[(k[r] >> shift) & (RADIX - 1)], 1u);
     }
     __syncthreads();
-    counts[(int64_t)threadIdx.x * num_tiles + blockIdx.x] = hist[threadIdx.x];
+    counts[(int64_t)blockIdx.x * RADIX + threadIdx.x] = hist[threadIdx.x];  // tile-major row
+}
+
+// Offsets of every (tile, digit) for the stable scatter, from tile-major counts:
+//   offs[t][d] = sum over d' < d of all counts of d'  +  sum over t' < t of counts[t'][d].
+// Tiles are grouped in chunks of RADIX_CHUNK.  radix_chunk_sums_kernel writes the chunk
+// sums digit-major (sums[d * nchunks + c]); an exclusive scan of that small array gives
+// each (digit, chunk) its base; radix_chunk_offsets_kernel walks the tiles of a chunk and
+// turns the counts into offsets in place.  Every pass reads and writes whole rows.
+constexpr int RADIX_CHUNK = 64;
+
+__global__ __launch_bounds__(RADIX) void radix_chunk_sums_kernel(const uint32_t* __restrict__ counts,
+                                                                 int64_t num_tiles, int64_t nchunks,
+                                                                 uint32_t* __restrict__ sums) {
+    const int64_t c = blockIdx.x;
+    const int d = threadIdx.x;
+    const int64_t t0 = c * RADIX_CHUNK, t1 = min(t0 + RADIX_CHUNK, num_tiles);
+    uint32_t x[RADIX_CHUNK];
+#pragma unroll
+    for (int i = 0; i < RADIX_CHUNK; ++i) x[i] = t0 + i < t1 ? counts[(t0 + i) * RADIX + d] : 0u;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < RADIX_CHUNK; ++i) acc += x[i];
+    sums[(int64_t)d * nchunks + c] = acc;
+}
+
+__global__ __launch_bounds__(RADIX) void radix_chunk_offsets_kernel(uint32_t* __restrict__ counts,
+                                                                    int64_t num_tiles, int64_t nchunks,
+                                                                    const uint32_t* __restrict__ base) {
+    const int64_t c = blockIdx.x;
+    const int d = threadIdx.x;
+    const int64_t t0 = c * RADIX_CHUNK, t1 = min(t0 + RADIX_CHUNK, num_tiles);
+    uint32_t x[RADIX_CHUNK];
+#pragma unroll
+    for (int i = 0; i < RADIX_CHUNK; ++i) x[i] = t0 + i < t1 ? counts[(t0 + i) * RADIX + d] : 0u;
+    uint32_t run = base[(int64_t)d * nchunks + c];
+#pragma unroll
+    for (int i = 0; i < RADIX_CHUNK; ++i) {
+        if (t0 + i < t1) counts[(t0 + i) * RADIX + d] = run;
+        run += x[i];
+    }
 }
 
 template <class V>
@@ -475,7 +516,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     const int64_t wbase = base + (int64_t)wid * SORT_IPT * WAVE;
 
     for (int j = threadIdx.x; j < NW * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
-    s_goff[threadIdx.x] = offs[(int64_t)threadIdx.x * num_tiles + blockIdx.x];
+    s_goff[threadIdx.x] = offs[(int64_t)blockIdx.x * RADIX + threadIdx.x];
     __syncthreads();
 
     uint64_t k[SORT_IPT];
