@@ -43,8 +43,8 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 # HBM bytes per launch measured on this workload (target 1e8 walkers) with rocprofv3
 # FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a plain 1 GiB copy (fetch x2.0,
 # write x1.0), expressed as a ratio to the kernel's algorithmic bytes.
-MEASURED_TRAFFIC_RATIO = {"radix_scatter": 2.462 / 2.283, "merge": 2.290 / 2.295,
-                          "spawn": 2.258 / 1.962, "plan": 1.127 / 1.096}
+MEASURED_TRAFFIC_RATIO = {"radix_scatter": 1.047, "merge": 1.042, "spawn": 1.159, "plan": 1.018,
+                          "compact": 1.011, "radix_count": 1.031}
 
 
 def log(*a):
@@ -124,7 +124,7 @@ def run_batched_leg(rimu, args):
     attempts = float(hist["spawn_attempts"].sum())
     log(f"batched sweep: {len(specs)} runs x {nsteps} steps in {elapsed:.2f} s")
 
-    ed = batched_ed_check(rimu, specs, hist, st, skip=nsteps // 4)
+    ed = batched_ed_check(rimu, specs, hist, st, skip=nsteps // 4, details=args.batched_details)
 
     plain_s = 0.0
     sample_steps = args.plain_sample_steps
@@ -159,7 +159,7 @@ def run_batched_leg(rimu, args):
                                f"summed over runs: {plain_s:.1f} s")}
 
 
-def batched_ed_check(rimu, specs, hist, status, skip):
+def batched_ed_check(rimu, specs, hist, status, skip, details=None):
     """Each run's shift and projected-energy estimates (blocking analysis after `skip`
     steps) against its own ground-state energy (Lanczos on the device, runs of equal M, N
     grouped eight to a launch).  Not part of the timed sweep."""
@@ -179,12 +179,22 @@ def batched_ed_check(rimu, specs, hist, status, skip):
             d.close()
     t_ed = time.perf_counter() - t0
     rows = []
+    per_run = []
     for r in range(len(specs)):
         if status[r] != 0:
             continue
         se = ST.blocking_analysis(hist["shift"][:, r], skip=skip)
         pe = ST.ratio_of_means(hist["hproj"][:, r], hist["vproj"][:, r], skip=skip)
         rows.append((se.mean - e0[r], se.err, pe.ratio - e0[r], pe.err))
+        h = specs[r].hamiltonian
+        a = h.starting_address()
+        per_run.append({"run": r, "M": a.M, "N": a.N, "u": h.u, "target_walkers": specs[r].target_walkers,
+                        "E0": float(e0[r]), "shift": se.mean, "shift_err": se.err, "shift_k": se.k,
+                        "projected": pe.ratio, "projected_err": pe.err,
+                        "mean_walkers": float(hist["norm"][skip:, r].mean())})
+    if details:
+        with open(details, "w") as f:
+            json.dump(per_run, f)
     z_s = np.array([abs(a) / b for a, b, _, _ in rows if b > 0 and np.isfinite(b)])
     z_p = np.array([abs(c) / d for _, _, c, d in rows if d > 0 and np.isfinite(d)])
     return {"runs_checked": len(rows), "ed_seconds": t_ed, "skip_steps": skip,
@@ -317,6 +327,7 @@ def main():
     ap.add_argument("--mom-steps", type=int, default=40)
     ap.add_argument("--no-momentum", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-details", default=None, help="write the per-run energy check to this JSON file")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(DT) void dense_apply_kernel(const uint64_t* __restr
 }
 
 // Pullback: vbar = (a + b H) ybar; per-block partial sums of b <ybar|D|v> and b <ybar|K|v>
-// written to partials[blockIdx][2R].
+// written to partials[blockIdx][2R].  K is real symmetric, so <ybar|K|v> = sum_r v_r (K ybar)_r
+// and only ybar is gathered at the neighbours.
 template <int R>
 __global__ __launch_bounds__(DT) void dense_pullback_kernel(const uint64_t* __restrict__ basis, int64_t D,
                                                             DenseParams p, const int64_t* __restrict__ binom,
@@ -125,24 +126,21 @@ __global__ __launch_bounds__(DT) void dense_pullback_kernel(const uint64_t* __re
     if (r < D) {
         const uint64_t bits = basis[r];
         const double dr = (double)bose_hubbard_interaction(bits) / 2;
-        double ky[R], kv[R];
+        double ky[R];
 #pragma unroll
-        for (int i = 0; i < R; ++i) { ky[i] = 0.0; kv[i] = 0.0; }
+        for (int i = 0; i < R; ++i) ky[i] = 0.0;
         for_each_hop(bits, r, p.M, p.B, binom, [&](int64_t k, double sp) {
             const double* yk = ybar + k * R;
-            const double* vk = v + k * R;
 #pragma unroll
-            for (int i = 0; i < R; ++i) {
-                ky[i] += -sp * yk[i];
-                kv[i] += -sp * vk[i];
-            }
+            for (int i = 0; i < R; ++i) ky[i] += -sp * yk[i];
         });
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             const double yr = ybar[r * R + i];
+            const double vr = v[r * R + i];
             vbar[r * R + i] = (p.a[i] + p.b[i] * (p.u[i] * dr)) * yr + p.b[i] * (p.t[i] * ky[i]);
-            pu[i] = p.b[i] * (yr * (dr * v[r * R + i]));
-            pt[i] = p.b[i] * (yr * kv[i]);
+            pu[i] = p.b[i] * (yr * (dr * vr));
+            pt[i] = p.b[i] * (vr * ky[i]);
         }
     }
 #pragma unroll
