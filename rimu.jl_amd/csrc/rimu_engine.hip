@@ -95,7 +95,7 @@ struct rimu_engine {
     DevBuf<uint32_t> rcounts, rchunks, rtmp;
     DevBuf<uint64_t> skeys[2];
     DevBuf<unsigned char> svals[2];
-    DevBuf<uint64_t> status;
+    DevBuf<uint64_t> status, plan_status;
     DevBuf<int64_t> tile_first, splits, tile_cnt;
     DevBuf<uint64_t> prevk, slot_k, slot_v;  // merge: per-tile survivor slots (values as raw 8 bytes)
     DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev, chunk_sums;
@@ -109,6 +109,8 @@ struct rimu_engine {
     struct Misc {
         uint32_t tile_counter;
         int32_t flags;
+        uint32_t plan_counter;
+        uint32_t reserved;
         int64_t W;
         int64_t S;
         int64_t L_new;
@@ -273,18 +275,22 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         HIP_TRY(hipMemcpyAsync(&dm->key_and, &ones, sizeof(ones), hipMemcpyHostToDevice, e->stream));
     }
 
-    // 1. plan
-    const int64_t plan_tiles = ceil_div(L + 1, NT);
+    // 1. plan: diagonal step, attempt offsets and the first entry of each spawn tile (as
+    // many as the current tile_first buffer holds; the rest below if W needs more)
+    const int64_t plan_tiles = ceil_div(L + 1, PLAN_TILE);
     HIP_TRY(e->plan_p.alloc((size_t)plan_tiles * 6));
+    HIP_TRY(e->plan_status.alloc(plan_tiles));
+    HIP_TRY(hipMemsetAsync(e->plan_status.p, 0, sizeof(uint64_t) * plan_tiles, e->stream));
+    const int64_t tf_cap = (int64_t)e->tile_first.n;
     e->timer.begin("plan", e->stream, &ev);
-    plan_kernel<V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p, e->plan_p.p);
+    plan_kernel<V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
+                                                     e->tile_first.p, tf_cap, e->plan_status.p,
+                                                     &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
-    e->timer.begin("scan", e->stream, &ev);
-    if (int rc = scan_inplace_i64(e, e->woff.p, L + 1)) return rc;
-    e->timer.end(e->stream);
-    HIP_TRY(hipMemcpyAsync(&e->h_misc->W, e->woff.p + L, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "plan look-back did not complete");
     const int64_t W = e->h_misc->W;
 
     // 2. spawn (W = number of attempts)
@@ -297,9 +303,11 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         const bool dense = !e->int_vals && e->style.proj_threshold == 0;
         HIP_TRY(e->status.alloc(std::max<int64_t>(spawn_tiles, 1)));
         HIP_TRY(e->spawn_p.alloc(spawn_tiles));
-        HIP_TRY(e->tile_first.alloc(spawn_tiles));
         if (!dense) HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
-        tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
+        if (spawn_tiles > tf_cap) {  // plan filled only the first tf_cap tile heads
+            HIP_TRY(e->tile_first.alloc(spawn_tiles));
+            tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
+        }
         e->timer.begin("spawn", e->stream, &ev);
         if (dense)
             spawn_kernel<V, true><<<spawn_tiles, NT, 0, e->stream>>>(
@@ -478,7 +486,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->tile_first.release(); e->splits.release(); e->tile_cnt.release();
     e->prevk.release(); e->slot_k.release(); e->slot_v.release();
     e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release(); e->rchunks.release();
-    e->rtmp.release(); e->status.release(); e->plan_p.release(); e->spawn_p.release();
+    e->rtmp.release(); e->status.release(); e->plan_status.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);

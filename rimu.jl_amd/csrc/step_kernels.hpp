@@ -20,6 +20,7 @@ namespace rimu {
 
 constexpr int NT = 256;          // workgroup size of every step kernel
 constexpr int MERGE_IPT = 8;     // merged positions per thread
+constexpr int SPAWN_TILE_SIZE = 1024;  // attempts per spawn tile (NT * SPAWN_ITEMS)
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_IPT = 16;     // keys per thread per radix tile
@@ -83,49 +84,109 @@ __device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
 }
 
 // ---------------------------------------------------------------------------------------
-// plan: one thread per entry: the diagonal step and the number of spawn attempts of the
-// entry (thread L writes the trailing zero of the count array).
+// plan: the diagonal step and the number of spawn attempts of every entry, the exclusive
+// scan of the attempt counts (aoff, L + 1 values; aoff[L] = W, also written to *W_out), and
+// the first entry of every spawn tile (tile_first[t] = entry holding attempt t*SPAWN_TILE,
+// for t < tf_cap).  Thread x of tile t handles the PLAN_IPT consecutive entries
+// t*PLAN_TILE + x*PLAN_IPT + j; tiles are placed by the decoupled look-back (tile ids in
+// start order, so every tile waited on is running).
 // partials[j * gridDim.x + tile], j = 0..5: deaths, clones, zombies, exact, inexact, attempts.
 // ---------------------------------------------------------------------------------------
+constexpr int PLAN_IPT = 8;
+constexpr int PLAN_TILE = NT * PLAN_IPT;
+
 template <class V>
 __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __restrict__ keys,
                                                   const V* __restrict__ vals, int64_t L,
-                                                  V* __restrict__ dvals, int64_t* __restrict__ wcount,
-                                                  double* __restrict__ partials) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+                                                  V* __restrict__ dvals, int64_t* __restrict__ aoff,
+                                                  int64_t* __restrict__ tile_first, int64_t tf_cap,
+                                                  uint64_t* status, uint32_t* tile_counter, int* flags,
+                                                  double* __restrict__ partials, int64_t* W_out) {
+    __shared__ int64_t s_cnt[PLAN_TILE];  // counts, then exclusive offsets, in entry order
+    __shared__ int64_t s_prefix;
+    const uint32_t tile = next_tile_id(tile_counter);
+    const int64_t t0 = (int64_t)tile * PLAN_TILE;
     double st[6] = {0, 0, 0, 0, 0, 0};
-    if (i < L) {
-        const uint64_t k = keys[i];
-        const V v = vals[i];
-        // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
-        const double hd = diagonal_element(a.h, k);
-        const double d = 1 - a.dtau * (hd - a.shift);
-        const double x = d * (double)v;
-        double U = 0;
-        if (is_int_v<V>::value || a.proj_threshold > 0) {
-            const U4 r = draw(a.seed, k, 0, a.step, TAG_DIAG);
-            U = u53(r.x, r.y);
+    // striped: entry t0 + j*NT + x (coalesced loads and stores)
+    uint64_t kk[PLAN_IPT];
+    V vv[PLAN_IPT];
+#pragma unroll
+    for (int j = 0; j < PLAN_IPT; ++j) {  // all loads in flight before any is used
+        const int64_t i = min(t0 + j * NT + threadIdx.x, max(L - 1, (int64_t)0));
+        kk[j] = L > 0 ? keys[i] : 0;
+        vv[j] = L > 0 ? vals[i] : (V)0;
+    }
+#pragma unroll
+    for (int j = 0; j < PLAN_IPT; ++j) {
+        const int64_t i = t0 + j * NT + threadIdx.x;
+        int64_t n_items = 0;
+        if (i < L) {
+            const uint64_t k = kk[j];
+            const V v = vv[j];
+            // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
+            const double hd = diagonal_element(a.h, k);
+            const double d = 1 - a.dtau * (hd - a.shift);
+            const double x = d * (double)v;
+            double U = 0;
+            if (is_int_v<V>::value || a.proj_threshold > 0) {
+                const U4 r = draw(a.seed, k, 0, a.step, TAG_DIAG);
+                U = u53(r.x, r.y);
+            }
+            const V nv = project<V>(x, a.proj_threshold, U);
+            dvals[i] = nv;
+            // clones_deaths_zombies, spawning.jl:79-93
+            const double res = (double)nv, val = (double)v;
+            if (res > val) st[1] += fabs(res - val);
+            else if (sgn(res) != sgn(val)) { st[0] += fabs(val); st[2] += fabs(res); }
+            else st[0] += fabs(res - val);
+            const ColumnPlan c = plan_column<V>(a, k, v);
+            if (c.n_od > 0) {
+                if (a.style != 1) { if (c.exact) st[3] += 1; else st[4] += 1; }
+                st[5] += (double)c.n_items;
+            }
+            n_items = c.n_items;
         }
-        const V nv = project<V>(x, a.proj_threshold, U);
-        dvals[i] = nv;
-        // clones_deaths_zombies, spawning.jl:79-93
-        const double res = (double)nv, val = (double)v;
-        if (res > val) st[1] = fabs(res - val);
-        else if (sgn(res) != sgn(val)) { st[0] = fabs(val); st[2] = fabs(res); }
-        else st[0] = fabs(res - val);
-        const ColumnPlan c = plan_column<V>(a, k, v);
-        if (c.n_od > 0) {
-            if (a.style != 1) { if (c.exact) st[3] = 1; else st[4] = 1; }
-            st[5] = (double)c.n_items;
+        s_cnt[j * NT + threadIdx.x] = n_items;
+    }
+    __syncthreads();
+    // blocked: thread x scans the PLAN_IPT consecutive entries x*PLAN_IPT + j
+    const int b0 = (int)threadIdx.x * PLAN_IPT;
+    int64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < PLAN_IPT; ++j) sum += s_cnt[b0 + j];
+    int64_t total;
+    const int64_t ex = block_exclusive_scan<int64_t, NT>(sum, &total);  // ends with a barrier
+    if (threadIdx.x < WAVE) {
+        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
+        if (threadIdx.x == 0) s_prefix = (int64_t)b;
+    }
+    __syncthreads();
+    int64_t run = s_prefix + ex;
+#pragma unroll
+    for (int j = 0; j < PLAN_IPT; ++j) {
+        const int64_t i = t0 + b0 + j;
+        const int64_t c = s_cnt[b0 + j];
+        s_cnt[b0 + j] = run;
+        if (i < L) {
+            const int64_t end = run + c;
+            for (int64_t t = (run + SPAWN_TILE_SIZE - 1) / SPAWN_TILE_SIZE; t * SPAWN_TILE_SIZE < end && t < tf_cap; ++t)
+                tile_first[t] = i;
+        } else if (i == L) {
+            aoff[L] = run;
+            *W_out = run;
         }
-        wcount[i] = c.n_items;
-    } else if (i == L) {
-        wcount[i] = 0;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PLAN_IPT; ++j) {
+        const int64_t i = t0 + j * NT + threadIdx.x;
+        if (i < L) aoff[i] = s_cnt[j * NT + threadIdx.x];
     }
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         const double s = block_sum<NT>(st[j]);
-        if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + tile] = s;
     }
 }
 
@@ -204,6 +265,7 @@ __global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64
 // ---------------------------------------------------------------------------------------
 constexpr int SPAWN_ITEMS = 4;
 constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
+static_assert(SPAWN_TILE == SPAWN_TILE_SIZE, "plan and spawn agree on the spawn tile");
 
 template <class V, bool DENSE>
 __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* __restrict__ keys,
