@@ -265,6 +265,7 @@ __global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64
 // ---------------------------------------------------------------------------------------
 constexpr int SPAWN_ITEMS = 4;
 constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
+static_assert(SPAWN_TILE % NT == 0, "whole slots per thread in the entry scan");
 static_assert(SPAWN_TILE == SPAWN_TILE_SIZE, "plan and spawn agree on the spawn tile");
 
 template <class V, bool DENSE>
@@ -282,6 +283,8 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     constexpr int NW = NT / WAVE;
     constexpr int WIN = SPAWN_TILE + 1;
     __shared__ int64_t s_aoff[WIN];
+    __shared__ int s_head[SPAWN_TILE];  // per attempt slot: window index of its entry
+    __shared__ int s_wmax[NW];
     __shared__ int s_cnt[SPAWN_ITEMS * NW];
     __shared__ uint64_t s_base;
     __shared__ int s_total;
@@ -306,6 +309,48 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
             const int t = (int)threadIdx.x + r * NT;
             if (t < WIN) s_aoff[t] = w[r];
         }
+#pragma unroll
+        for (int r = 0; r < SPAWN_ITEMS; ++r) s_head[(int)threadIdx.x + r * NT] = 0;
+        __syncthreads();
+        // entry t of the window (0 < t < WIN-1) starts at attempt s_aoff[t]: mark that slot.
+        // Entries without attempts share the start of the next one; the larger index wins,
+        // which is the entry that owns the attempt.
+#pragma unroll
+        for (int r = 0; r < WR; ++r) {
+            const int t = (int)threadIdx.x + r * NT;
+            if (t >= 1 && t < WIN - 1 && e0 + t < L) {
+                const int64_t p = w[r] - a0;
+                if (p >= 0 && p < SPAWN_TILE) atomicMax(&s_head[p], t);
+            }
+        }
+        __syncthreads();
+        // inclusive max-scan over the slots: slot q then holds the entry of attempt a0 + q
+        {
+            constexpr int Q = SPAWN_TILE / NT;
+            const int q0 = (int)threadIdx.x * Q;
+            int m[Q];
+            int run = 0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                run = max(run, s_head[q0 + q]);
+                m[q] = run;
+            }
+            int incl = m[Q - 1];
+#pragma unroll
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const int o = __shfl_up(incl, off, WAVE);
+                if (lane >= off) incl = max(incl, o);
+            }
+            int before = __shfl_up(incl, 1, WAVE);
+            if (lane == 0) before = 0;
+            if (lane == WAVE - 1) s_wmax[wid] = incl;
+            __syncthreads();
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww)
+                if (ww < wid) before = max(before, s_wmax[ww]);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) s_head[q0 + q] = max(m[q], before);
+        }
     }
     __syncthreads();
 
@@ -324,13 +369,9 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         int64_t i = e0, ai = att;
         if (att < A) {
             if (att < s_aoff[WIN - 1]) {
-                int lo = 0, hi = WIN - 1;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_aoff[mid] <= att) lo = mid; else hi = mid;
-                }
-                i = e0 + lo;
-                ai = s_aoff[lo];
+                const int w = s_head[it * NT + threadIdx.x];
+                i = e0 + w;
+                ai = s_aoff[w];
             } else {
                 int64_t lo = e0, hi = L;
                 while (hi - lo > 1) {
