@@ -222,24 +222,30 @@ __device__ inline int occupation(uint64_t bits, int m) { return ones_from(bits, 
 // (HubbardMom1D.jl:189-194).  The map of occupied modes is read from the run starts.
 __device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
                                              double* value) {
+    // chosen <= s(s-1)(M-2) + d(M-1) < 2^31 for M <= 64: the index arithmetic runs in 32-bit
+    // unsigned integers (64-bit division is a long software sequence on the GPU)
     const int M = h.M;
     const uint64_t st = run_starts(bits);
-    const int64_t s = popc64(st);
-    const int64_t dbl = chosen - s * (s - 1) * (M - 2);
+    const int s = popc64(st);
+    const uint32_t c = (uint32_t)chosen;
+    const uint32_t single = (uint32_t)(s * (s - 1) * (M - 2));
     int p_mode, q_mode;  // source modes (p <= q)
-    int64_t mom;
-    if (dbl > 0) {
-        mom = (dbl - 1) % (M - 1) + 1;
-        const int64_t d = (dbl - mom) / (M - 1) + 1;
+    int mom;
+    if (c > single) {
+        const uint32_t dbl = c - single;
+        const uint32_t m1 = (uint32_t)(M - 1);
+        mom = (int)((dbl - 1) % m1) + 1;
+        const uint32_t d = (dbl - 1) / m1 + 1;
         const uint64_t dstarts = st & (bits >> 1);
         const int pos = select_bit(dstarts, (int)d - 1);
         p_mode = q_mode = pos - popc64(bits & low_mask(pos)) + 1;
     } else {
-        mom = (chosen - 1) % (M - 2) + 1;
-        const int64_t pair = (chosen - mom) / (M - 2) + 1;
-        const int64_t snd = (pair - 1) % (s - 1) + 1;
-        const int64_t fst = (pair - snd) / (s - 1) + 1;
-        int64_t f_hole, s_hole;
+        const uint32_t m2 = (uint32_t)(M - 2), s1 = (uint32_t)(s - 1);
+        mom = (int)((c - 1) % m2) + 1;
+        const uint32_t pair = (c - 1) / m2 + 1;
+        const int snd = (int)((pair - 1) % s1) + 1;
+        const int fst = (int)((pair - 1) / s1) + 1;
+        int f_hole, s_hole;
         if (snd < fst) { f_hole = snd; s_hole = fst; }
         else { f_hole = fst; s_hole = snd + 1; }
         const int pf = select_bit(st, (int)f_hole - 1);

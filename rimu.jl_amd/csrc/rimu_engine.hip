@@ -95,7 +95,7 @@ struct rimu_engine {
     DevBuf<uint32_t> rcounts, rchunks, rtmp;
     DevBuf<uint64_t> skeys[2];
     DevBuf<unsigned char> svals[2];
-    DevBuf<uint64_t> status, plan_status;
+    DevBuf<uint64_t> plan_status, tile_or, tile_and;
     DevBuf<int64_t> tile_first, splits, tile_cnt;
     DevBuf<uint64_t> prevk, slot_k, slot_v;  // merge: per-tile survivor slots (values as raw 8 bytes)
     DevBuf<double> plan_p, spawn_p, merge_p, terms, stats_dev, chunk_sums;
@@ -107,10 +107,8 @@ struct rimu_engine {
 
     // host staging
     struct Misc {
-        uint32_t tile_counter;
         int32_t flags;
         uint32_t plan_counter;
-        uint32_t reserved;
         int64_t W;
         int64_t S;
         int64_t L_new;
@@ -251,8 +249,8 @@ int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* a
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
     if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nt + 1)) return rc;
-    compact_kernel<V><<<nt, NT, 0, e->stream>>>(e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok,
-                                                ov, cap, &dm->L_new, &dm->flags);
+    compact_kernel<V, MERGE_IPT, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
+        e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok, ov, cap, &dm->L_new, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     return RIMU_OK;
@@ -270,10 +268,6 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     hipEvent_t ev{};
 
     HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
-    {   // key_and starts at all ones
-        const unsigned long long ones = ~0ull;
-        HIP_TRY(hipMemcpyAsync(&dm->key_and, &ones, sizeof(ones), hipMemcpyHostToDevice, e->stream));
-    }
 
     // 1. plan: diagonal step, attempt offsets and the first entry of each spawn tile (as
     // many as the current tile_first buffer holds; the rest below if W needs more)
@@ -293,7 +287,9 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "plan look-back did not complete");
     const int64_t W = e->h_misc->W;
 
-    // 2. spawn (W = number of attempts)
+    // 2. spawn (W = number of attempts).  Dense spawning (every attempt yields a record)
+    // writes the record of attempt a at position a; otherwise tiles write to their own slot
+    // ranges and a scan of the tile counts plus a compaction place the records.
     int64_t S = 0;
     int64_t spawn_tiles = 0;
     uint64_t kor = 0, kand = ~0ull;
@@ -301,35 +297,57 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         spawn_tiles = ceil_div(W, SPAWN_TILE);
         // every attempt yields a record when values are floating and deposits unprojected
         const bool dense = !e->int_vals && e->style.proj_threshold == 0;
-        HIP_TRY(e->status.alloc(std::max<int64_t>(spawn_tiles, 1)));
+        if (dense && W > e->max_spawns) {
+            out->overflow = 1;
+            return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
+        }
         HIP_TRY(e->spawn_p.alloc(spawn_tiles));
-        if (!dense) HIP_TRY(hipMemsetAsync(e->status.p, 0, sizeof(uint64_t) * spawn_tiles, e->stream));
+        HIP_TRY(e->tile_or.alloc(spawn_tiles));
+        HIP_TRY(e->tile_and.alloc(spawn_tiles));
         if (spawn_tiles > tf_cap) {  // plan filled only the first tf_cap tile heads
             HIP_TRY(e->tile_first.alloc(spawn_tiles));
             tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
         }
+        uint64_t* sk = e->skeys[0].p;
+        V* sv = (V*)e->svals[0].p;
+        if (!dense) {
+            HIP_TRY(e->slot_k.alloc((size_t)spawn_tiles * SPAWN_TILE));
+            HIP_TRY(e->slot_v.alloc((size_t)spawn_tiles * SPAWN_TILE));
+            HIP_TRY(e->tile_cnt.alloc(spawn_tiles + 1));
+            HIP_TRY(hipMemsetAsync(e->tile_cnt.p + spawn_tiles, 0, sizeof(int64_t), e->stream));
+            sk = e->slot_k.p;
+            sv = (V*)e->slot_v.p;
+        }
         e->timer.begin("spawn", e->stream, &ev);
         if (dense)
             spawn_kernel<V, true><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
-                e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
-                &dm->key_or, &dm->key_and);
+                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
+                e->tile_or.p, e->tile_and.p);
         else
             spawn_kernel<V, false><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, e->skeys[0].p, (V*)e->svals[0].p, e->max_spawns,
-                e->status.p, &dm->tile_counter, &dm->S, &dm->flags, e->spawn_p.p, e->tile_first.p,
-                &dm->key_or, &dm->key_and);
+                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
+                e->tile_or.p, e->tile_and.p);
         e->timer.end(e->stream);
+        HIP_TRY(hipGetLastError());
+        key_bits_kernel<<<1, 1024, 0, e->stream>>>(e->tile_or.p, e->tile_and.p, spawn_tiles, &dm->key_or,
+                                                   &dm->key_and);
+        if (!dense) {
+            e->timer.begin("spawn_compact", e->stream, &ev);
+            if (int rc = scan_inplace_i64(e, e->tile_cnt.p, spawn_tiles + 1)) return rc;
+            compact_kernel<V, SPAWN_ITEMS, FLAG_SPAWN_OVERFLOW><<<spawn_tiles, NT, 0, e->stream>>>(
+                e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, spawn_tiles, e->skeys[0].p,
+                (V*)e->svals[0].p, e->max_spawns, &dm->S, &dm->flags);
+            e->timer.end(e->stream);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
-        if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "spawn look-back did not complete");
         if (e->h_misc->flags & FLAG_SPAWN_OVERFLOW) {
             e->timer.collect();
             out->overflow = 1;
             return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
         }
-        S = e->h_misc->S;
+        S = dense ? W : e->h_misc->S;
         kor = e->h_misc->key_or;
         kand = e->h_misc->key_and;
     }
@@ -486,7 +504,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->tile_first.release(); e->splits.release(); e->tile_cnt.release();
     e->prevk.release(); e->slot_k.release(); e->slot_v.release();
     e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release(); e->rchunks.release();
-    e->rtmp.release(); e->status.release(); e->plan_status.release(); e->plan_p.release(); e->spawn_p.release();
+    e->rtmp.release(); e->plan_status.release(); e->tile_or.release(); e->tile_and.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);

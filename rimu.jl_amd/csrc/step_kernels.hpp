@@ -258,10 +258,10 @@ __global__ __launch_bounds__(NT) void scan_add_kernel(T* __restrict__ out, int64
 // order is the (entry, attempt) order.  Each thread handles SPAWN_ITEMS attempts, strided
 // by NT within the tile.  Nonzero records are compacted with a wavefront ballot and
 // popcount prefix; the (item, wavefront) counts are scanned across the workgroup and the
-// tile is placed after all earlier tiles by the look-back.  When every attempt yields a
-// record (DENSE: floating values without projection) the record of attempt a is simply
-// written at position a and no look-back is needed.
-// Also reduces sum |spawn| per tile and the key OR/AND (which radix digits vary).
+// tile's records go to its own slot range with their count in tile_cnt (compact_kernel
+// places them).  When every attempt yields a record (DENSE: floating values without
+// projection) the record of attempt a is simply written at position a.
+// Also writes sum |spawn| and the key OR/AND (which radix digits vary) of the tile.
 // ---------------------------------------------------------------------------------------
 constexpr int SPAWN_ITEMS = 4;
 constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
@@ -273,24 +273,22 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
                                                    const V* __restrict__ vals, int64_t L,
                                                    const int64_t* __restrict__ aoff, int64_t A,
                                                    uint64_t* __restrict__ out_keys,
-                                                   V* __restrict__ out_vals, int64_t cap,
-                                                   uint64_t* status, uint32_t* tile_counter,
-                                                   int64_t* total_out, int* flags,
+                                                   V* __restrict__ out_vals,
+                                                   int64_t* __restrict__ tile_cnt,
                                                    double* __restrict__ partials,
                                                    const int64_t* __restrict__ tile_first,
-                                                   unsigned long long* key_or,
-                                                   unsigned long long* key_and) {
+                                                   uint64_t* __restrict__ tile_or,
+                                                   uint64_t* __restrict__ tile_and) {
     constexpr int NW = NT / WAVE;
     constexpr int WIN = SPAWN_TILE + 1;
     __shared__ int64_t s_aoff[WIN];
     __shared__ int s_head[SPAWN_TILE];  // per attempt slot: window index of its entry
     __shared__ int s_wmax[NW];
     __shared__ int s_cnt[SPAWN_ITEMS * NW];
-    __shared__ uint64_t s_base;
     __shared__ int s_total;
     __shared__ uint64_t s_kor[NW], s_kand[NW];
 
-    const uint32_t tile = DENSE ? blockIdx.x : next_tile_id(tile_counter);
+    const uint32_t tile = blockIdx.x;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -435,7 +433,8 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         }
     }
 
-    int64_t base = a0;
+    // DENSE: the record of attempt a goes to position a.  Otherwise the tile's records go,
+    // in attempt order, to its own slot range [a0, a0 + total); compact_kernel places them.
     int total = 0;
     if (!DENSE) {
         __syncthreads();
@@ -447,34 +446,18 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         }
         __syncthreads();
         total = s_total;
-        if (threadIdx.x < WAVE) {
-            const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
-            if (threadIdx.x == 0) s_base = b;
-        }
-        __syncthreads();
-        base = (int64_t)s_base;
     }
-    const int64_t tile_end = DENSE ? min(a0 + (int64_t)SPAWN_TILE, A) : base + total;
-    if (tile_end > cap) {
-        if (threadIdx.x == 0) atomicOr(flags, FLAG_SPAWN_OVERFLOW);
-    } else {
 #pragma unroll
-        for (int it = 0; it < SPAWN_ITEMS; ++it) {
-            if (rok[it]) {
-                const int64_t pos = DENSE ? a0 + (int64_t)it * NT + threadIdx.x
-                                          : base + s_cnt[it * NW + wid] + rpos[it];
-                out_keys[pos] = rk[it];
-                out_vals[pos] = rv[it];
-            }
+    for (int it = 0; it < SPAWN_ITEMS; ++it) {
+        if (rok[it]) {
+            const int64_t pos = DENSE ? a0 + (int64_t)it * NT + threadIdx.x
+                                      : a0 + s_cnt[it * NW + wid] + rpos[it];
+            out_keys[pos] = rk[it];
+            out_vals[pos] = rv[it];
         }
     }
     const double s = block_sum<NT>(spawned);
-    if (threadIdx.x == 0) {
-        partials[tile] = s;
-        if (tile == gridDim.x - 1) *total_out = tile_end;
-    }
-    // key OR/AND for the radix pass selection (bitwise: order independent).  Reduced over
-    // the workgroup; the global words are only touched when they would change.
+    // key OR/AND of the tile (which radix digits vary; reduced over the tiles afterwards)
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
         kor |= __shfl_xor(kor, off, WAVE);
@@ -488,10 +471,10 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     if (threadIdx.x == 0) {
         uint64_t bo = 0, ba = ~0ull;
         for (int q = 0; q < NW; ++q) { bo |= s_kor[q]; ba &= s_kand[q]; }
-        const uint64_t cur_o = __hip_atomic_load(key_or, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t cur_a = __hip_atomic_load(key_and, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (bo & ~cur_o) atomicOr(key_or, (unsigned long long)bo);
-        if (cur_a & ~ba) atomicAnd(key_and, (unsigned long long)ba);
+        partials[tile] = s;
+        tile_or[tile] = bo;
+        tile_and[tile] = ba;
+        if (!DENSE) tile_cnt[tile] = total;
     }
 }
 
@@ -526,6 +509,41 @@ __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restr
     const uint64_t pa = lo > 0 ? ak[lo - 1] : 0;
     const uint64_t pb = bi > 0 ? bk[bi - 1] : 0;
     prevk[t] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
+}
+
+// OR and AND of the per-tile key words (one workgroup).
+__global__ __launch_bounds__(1024) void key_bits_kernel(const uint64_t* __restrict__ tile_or,
+                                                        const uint64_t* __restrict__ tile_and,
+                                                        int64_t ntiles, unsigned long long* key_or,
+                                                        unsigned long long* key_and) {
+    __shared__ uint64_t s_o[1024 / WAVE], s_a[1024 / WAVE];
+    uint64_t o = 0, an = ~0ull;
+    for (int64_t base = 0; base < ntiles; base += 4 * 1024) {
+        uint64_t xo[4], xa[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t t = base + r * 1024 + threadIdx.x;
+            xo[r] = t < ntiles ? tile_or[t] : 0ull;
+            xa[r] = t < ntiles ? tile_and[t] : ~0ull;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { o |= xo[r]; an &= xa[r]; }
+    }
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        o |= __shfl_xor(o, off, WAVE);
+        an &= __shfl_xor(an, off, WAVE);
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        s_o[threadIdx.x / WAVE] = o;
+        s_a[threadIdx.x / WAVE] = an;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 1024 / WAVE; ++q) { o |= s_o[q]; an &= s_a[q]; }
+        *key_or = o;
+        *key_and = an;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -900,21 +918,22 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     }
 }
 
-// compact: tile t's survivors move from its slot range to [offs[t], offs[t+1]) (offs is the
-// exclusive scan of the tile counts, offs[ntiles] the new length).  If the new length
-// exceeds cap nothing is written and the overflow flag is raised.
-template <class V>
+// compact: tile t's records (merge survivors or spawns; NT * IPT slots per tile) move from
+// its slot range to [offs[t], offs[t+1]) (offs is the exclusive scan of the tile counts,
+// offs[ntiles] the new length).  If the new length exceeds cap nothing is written and
+// FLAG is raised.
+template <class V, int IPT, int FLAG>
 __global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict__ slot_k,
                                                      const V* __restrict__ slot_v,
                                                      const int64_t* __restrict__ offs, int64_t ntiles,
                                                      uint64_t* __restrict__ ok, V* __restrict__ ov,
                                                      int64_t cap, int64_t* total_out, int* flags) {
-    constexpr int TILE = NT * MERGE_IPT;
+    constexpr int TILE = NT * IPT;
     const int64_t tile = blockIdx.x;
     const int64_t total = offs[ntiles];
     if (total > cap) {
         if (tile == 0 && threadIdx.x == 0) {
-            atomicOr(flags, FLAG_VECTOR_OVERFLOW);
+            atomicOr(flags, FLAG);
             *total_out = total;
         }
         return;
@@ -923,15 +942,15 @@ __global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict_
     const int64_t off = offs[tile];
     const int n = (int)(offs[tile + 1] - off);
     const int64_t src = tile * TILE;
-    uint64_t k[MERGE_IPT];
-    V v[MERGE_IPT];
+    uint64_t k[IPT];
+    V v[IPT];
 #pragma unroll
-    for (int r = 0; r < MERGE_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int i = (int)threadIdx.x + r * NT;
         if (i < n) { k[r] = slot_k[src + i]; v[r] = slot_v[src + i]; }
     }
 #pragma unroll
-    for (int r = 0; r < MERGE_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int i = (int)threadIdx.x + r * NT;
         if (i < n) { ok[off + i] = k[r]; ov[off + i] = v[r]; }
     }
