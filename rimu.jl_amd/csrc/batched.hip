@@ -28,6 +28,10 @@ using namespace rimu;
 namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
+constexpr int EPT = 4;           // entries per thread per spawn chunk
+constexpr int SPLITS = 1024;     // merge tile boundaries located per pass
+constexpr double WALKERS_PER_SLICE = 8192;  // target walkers per spawn workgroup of a run
+constexpr int64_t MAX_SLICES = 16;
 constexpr int MAXPROJ = 160;     // projector entries per run (start address and neighbours)
 constexpr int NHIST = 6;         // shift, norm, len, vproj, hproj, attempts
 
@@ -39,6 +43,7 @@ struct RunState {
     uint64_t seed;
     int64_t len, cur, step, nrec;     // nrec: history rows written
     int64_t capV, capS, offV, offS, offP, nproj;
+    int64_t K, slice0, offX;          // spawn slices, first slice's global index, slice pool offset
 };
 
 struct Pools {
@@ -51,28 +56,34 @@ struct Pools {
     const double* pc;    // coefficients of hproj (vproj uses the start address, pc index 0 row)
     double* hist;        // [NHIST][hist_cap][nruns]
     int64_t hist_cap;
+    uint64_t* xk;        // spawn records of slices 1..K-1 (capS each; slice 0 writes to sk[0])
+    int64_t* xv;
+    int64_t* xcnt;       // records of every slice
+    double* xatt;        // spawn attempts of every slice
+    const int* wg_run;   // run of every spawn workgroup
 };
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
 
+// Diagonal step and spawning of one slice of a run: entries [L*s/K, L*(s+1)/K).  Records go,
+// in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
+// region; batch_step_kernel gathers the slices in order.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ runs, int nruns, Pools P,
-                                                        int radix_bits) {
+__global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ runs, Pools P) {
     constexpr int NW = NT / WAVE;
-    constexpr int TILE = NT * MIPT;
-    __shared__ int64_t s_aoff[NT + 1];
+    constexpr int CH = NT * EPT;          // entries per spawn chunk
     __shared__ int s_wc[NW];
     __shared__ int64_t s_S;
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_wcnt[NW][256];
-    __shared__ uint64_t s_a[TILE];
-    __shared__ uint64_t s_b[TILE];
-    __shared__ int64_t s_split[2];
-    __shared__ int s_flag;
+    __shared__ struct {
+        uint64_t k[CH];        // the chunk's keys
+        int64_t v[CH];         // and values
+        int64_t aoff[CH + 1];  // attempt offsets within the chunk (counts first)
+    } sp;
 
-    const int run = blockIdx.x;
-    RunState& st = runs[run];
+    const int run = P.wg_run[blockIdx.x];
+    const RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const int64_t slice = (int64_t)blockIdx.x - st.slice0;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -80,45 +91,72 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
     const int M = st.M, B = st.B;
     const double u = st.u, t = st.t, dtau = st.dtau, shift = st.shift;
     const uint64_t seed = st.seed;
-    const int64_t L = st.len, capV = st.capV, capS = st.capS;
+    const int64_t Lrun = st.len, capS = st.capS;
     const int cur = (int)st.cur;
     const int64_t step = st.step + 1;
-    const uint64_t* vk = P.vk[cur] + st.offV;
-    const int64_t* vv = P.vv[cur] + st.offV;
-    uint64_t* nk = P.vk[1 - cur] + st.offV;
-    int64_t* nv = P.vv[1 - cur] + st.offV;
-    int64_t* dv = P.dv + st.offV;
-    uint64_t* sk0 = P.sk[0] + st.offS;
-    int64_t* sv0 = P.sv[0] + st.offS;
-    uint64_t* sk1 = P.sk[1] + st.offS;
-    int64_t* sv1 = P.sv[1] + st.offS;
+    const int64_t e_lo = Lrun * slice / st.K, e_hi = Lrun * (slice + 1) / st.K;
+    const uint64_t* vk = P.vk[cur] + st.offV + e_lo;
+    const int64_t* vv = P.vv[cur] + st.offV + e_lo;
+    int64_t* dv = P.dv + st.offV + e_lo;
+    const int64_t L = e_hi - e_lo;
+    uint64_t* sk0 = slice == 0 ? P.sk[0] + st.offS : P.xk + st.offX + (slice - 1) * capS;
+    int64_t* sv0 = slice == 0 ? P.sv[0] + st.offS : P.xv + st.offX + (slice - 1) * capS;
 
-    if (threadIdx.x == 0) { s_S = 0; s_flag = 0; }
+    if (threadIdx.x == 0) s_S = 0;
     __syncthreads();
 
-    // ---- diagonal step and spawning, NT entries at a time --------------------------------
+    // ---- diagonal step and spawning, CH entries at a time -------------------------------
+    // Entries are loaded striped (coalesced) and kept in LDS; attempts are numbered entry by
+    // entry, so the records come out in (entry, attempt) order.
     double st_att = 0, st_spawn = 0, st_death = 0, st_clone = 0, st_zombie = 0;
-    for (int64_t c0 = 0; c0 < L; c0 += NT) {
-        const int64_t e = c0 + threadIdx.x;
-        int64_t n_att = 0;
-        if (e < L) {
-            const uint64_t k = vk[e];
-            const int64_t v = vv[e];
-            const double hd = u * (double)bose_hubbard_interaction(k) / 2;
-            const double x = (1 - dtau * (hd - shift)) * (double)v;
-            const U4 r = draw(seed, k, 0, step, TAG_DIAG);
-            const int64_t res = isgn(x) * (int64_t)floor(fabs(x) + u53(r.x, r.y));
-            dv[e] = res;
-            if (res > v) st_clone += (double)(res - v);
-            else if (isgn((double)res) != isgn((double)v)) { st_death += fabs((double)v); st_zombie += fabs((double)res); }
-            else st_death += fabs((double)(res - v));
-            n_att = v < 0 ? -v : v;
+    for (int64_t c0 = 0; c0 < L; c0 += CH) {
+        const int n_ch = (int)min((int64_t)CH, L - c0);
+        uint64_t kq[EPT];
+        int64_t vq[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {  // all loads in flight before any is used
+            const int64_t e = min(c0 + j * NT + (int64_t)threadIdx.x, L - 1);
+            kq[j] = vk[e];
+            vq[j] = vv[e];
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int loc = j * NT + (int)threadIdx.x;
+            if (loc < n_ch) {
+                const uint64_t k = kq[j];
+                const int64_t v = vq[j];
+                const double hd = u * (double)bose_hubbard_interaction(k) / 2;
+                const double x = (1 - dtau * (hd - shift)) * (double)v;
+                const U4 r = draw(seed, k, 0, step, TAG_DIAG);
+                const int64_t res = isgn(x) * (int64_t)floor(fabs(x) + u53(r.x, r.y));
+                dv[c0 + loc] = res;
+                if (res > v) st_clone += (double)(res - v);
+                else if (isgn((double)res) != isgn((double)v)) { st_death += fabs((double)v); st_zombie += fabs((double)res); }
+                else st_death += fabs((double)(res - v));
+                sp.k[loc] = k;
+                sp.v[loc] = v;
+                sp.aoff[loc] = v < 0 ? -v : v;   // attempts: |v| (every entry has one at least)
+            }
+        }
+        __syncthreads();
+        // blocked scan: thread x owns chunk entries x*EPT .. x*EPT+EPT-1
+        int64_t c[EPT];
+        int64_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int loc = (int)threadIdx.x * EPT + j;
+            c[j] = loc < n_ch ? sp.aoff[loc] : 0;
+            sum += c[j];
         }
         int64_t T;
-        const int64_t ex = block_exclusive_scan<int64_t, NT>(n_att, &T);
-        s_aoff[threadIdx.x] = ex;
-        if (threadIdx.x == 0) s_aoff[NT] = T;
-        st_att += (double)n_att;
+        int64_t run_ = block_exclusive_scan<int64_t, NT>(sum, &T);  // ends with a barrier
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            sp.aoff[(int)threadIdx.x * EPT + j] = run_;
+            run_ += c[j];
+        }
+        if (threadIdx.x == 0) sp.aoff[CH] = T;
+        st_att += (double)sum;
         __syncthreads();
         for (int64_t a0 = 0; a0 < T; a0 += NT) {
             const int64_t att = a0 + threadIdx.x;
@@ -126,14 +164,14 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
             uint64_t nbk = 0;
             int64_t nbv = 0;
             if (att < T) {
-                int lo = 0, hi = NT;  // s_aoff[lo] <= att < s_aoff[hi]
+                int lo = 0, hi = CH;  // aoff[lo] <= att < aoff[hi]
                 while (hi - lo > 1) {
                     const int mid = (lo + hi) >> 1;
-                    if (s_aoff[mid] <= att) lo = mid; else hi = mid;
+                    if (sp.aoff[mid] <= att) lo = mid; else hi = mid;
                 }
-                const uint64_t k = vk[c0 + lo];
-                const int64_t v = vv[c0 + lo];
-                const int64_t j = att - s_aoff[lo];
+                const uint64_t k = sp.k[lo];
+                const int64_t v = sp.v[lo];
+                const int64_t j = att - sp.aoff[lo];
                 const int64_t n_od = 2 * (int64_t)num_occupied(k);
                 const int64_t na = v < 0 ? -v : v;
                 const double magnitude = (double)v / (double)na;
@@ -159,15 +197,87 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
             if (ok) {
                 const int64_t pos = before + __popcll(bal & lt_mask);
                 if (pos < capS) { sk0[pos] = nbk; sv0[pos] = nbv; }
-                else s_flag = RUN_SPAWN_OVERFLOW;
             }
             __syncthreads();
             if (threadIdx.x == 0) s_S += tot;
             __syncthreads();
         }
     }
+    // record count (it may exceed capS: the gather then reports the overflow) and attempts
+    const double att_sum = block_sum<NT>(st_att);
+    if (threadIdx.x == 0) {
+        P.xcnt[st.slice0 + slice] = s_S;
+        P.xatt[st.slice0 + slice] = att_sum;
+    }
+    (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie; (void)u;
+
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ runs, int nruns, Pools P,
+                                                        int radix_bits) {
+    constexpr int NW = NT / WAVE;
+    constexpr int TILE = NT * MIPT;
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_wcnt[NW][256];
+    __shared__ int s_flag;
+    __shared__ struct {
+        struct {
+            uint64_t a[TILE], b[TILE];    // tile keys: spawns, vector entries
+            int64_t av[TILE], bv[TILE];   // their values (diagonal survivors for b)
+            int64_t split[SPLITS + 1];    // spawns before each tile boundary
+            uint64_t prev[SPLITS + 1];    // merged key just before each tile
+        } mg;
+    } sh;
+
+    const int run = blockIdx.x;
+    RunState& st = runs[run];
+    if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+
+    const int B = st.B;
+    const double dtau = st.dtau, shift = st.shift;
+    const int64_t L = st.len, capV = st.capV, capS = st.capS;
+    const int cur = (int)st.cur;
+    const int64_t step = st.step + 1;
+    const uint64_t* vk = P.vk[cur] + st.offV;
+    uint64_t* nk = P.vk[1 - cur] + st.offV;
+    int64_t* nv = P.vv[1 - cur] + st.offV;
+    int64_t* dv = P.dv + st.offV;
+    uint64_t* sk0 = P.sk[0] + st.offS;
+    int64_t* sv0 = P.sv[0] + st.offS;
+    uint64_t* sk1 = P.sk[1] + st.offS;
+    int64_t* sv1 = P.sv[1] + st.offS;
+
+    if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
-    const int64_t S = min(s_S, capS);
+
+    // ---- the slices' spawn records, gathered after slice 0's in slice order -------------
+    int64_t S = 0;
+    double att_sum = 0;
+    for (int64_t q = 0; q < st.K; ++q) {
+        S += P.xcnt[st.slice0 + q];
+        att_sum += P.xatt[st.slice0 + q];
+    }
+    if (S > capS) {
+        if (threadIdx.x == 0) s_flag = RUN_SPAWN_OVERFLOW;
+        S = capS;
+    } else {
+        int64_t at = P.xcnt[st.slice0];
+        for (int64_t q = 1; q < st.K; ++q) {
+            const int64_t n = P.xcnt[st.slice0 + q];
+            const uint64_t* xk = P.xk + st.offX + (q - 1) * capS;
+            const int64_t* xv = P.xv + st.offX + (q - 1) * capS;
+            for (int64_t i = threadIdx.x; i < n; i += NT) {
+                sk0[at + i] = xk[i];
+                sv0[at + i] = xv[i];
+            }
+            at += n;
+        }
+    }
+    __syncthreads();
 
     // ---- stable LSD radix sort of the spawn records on bits [0, B) ------------------------
     const int nbins = 1 << radix_bits;
@@ -221,88 +331,120 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
 
     // ---- merge the sorted spawns (A) with the diagonal survivors (B), fold, count ------
     // merged order: by key, B before A on equal keys; each address folds diag + spawns.
+    // The tile boundaries of up to SPLITS tiles are located at once (one thread each).
     const uint64_t* ak = srck;
     const int64_t* av = srcv;
     int64_t out = 0;
     double nrm = 0;
-    for (int64_t d0 = 0; d0 < S + L; d0 += TILE) {
-        const int64_t d1 = min(d0 + TILE, S + L);
-        if (threadIdx.x < 2) {
-            const int64_t d = threadIdx.x == 0 ? d0 : d1;
+    const int64_t ntl = (S + L + TILE - 1) / TILE;
+    for (int64_t tb = 0; tb < ntl; tb += SPLITS) {
+        const int64_t te = min(tb + SPLITS, ntl);
+        __syncthreads();
+        for (int64_t q = threadIdx.x; q <= te - tb; q += NT) {
+            const int64_t d = min((tb + q) * TILE, S + L);
             int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
             while (lo < hi) {
                 const int64_t mid = (lo + hi) >> 1;
                 if (ak[mid] < vk[d - mid - 1]) lo = mid + 1; else hi = mid;
             }
-            s_split[threadIdx.x] = lo;
+            sh.mg.split[q] = lo;
+            const int64_t bi = d - lo;
+            const uint64_t pa = lo > 0 ? ak[lo - 1] : 0, pb = bi > 0 ? vk[bi - 1] : 0;
+            sh.mg.prev[q] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
         }
         __syncthreads();
-        const int64_t a0 = s_split[0], a1 = s_split[1];
-        const int64_t b0 = d0 - a0, b1 = d1 - a1;
-        const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-        for (int i = threadIdx.x; i < na; i += NT) s_a[i] = ak[a0 + i];
-        for (int i = threadIdx.x; i < nb; i += NT) s_b[i] = vk[b0 + i];
-        __syncthreads();
-        const int dd = threadIdx.x * MIPT;
-        const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
-        int ia;
-        {
-            int lo = max(0, dd - nb), hi = min(dd, na);
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
-            }
-            ia = lo;
-        }
-        int ib = dd - ia;
-        bool have_prev = false;
-        uint64_t prev = 0;
-        if (dd > 0 && n_here > 0) {
-            have_prev = true;
-            const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0, pb = ib > 0 ? s_b[ib - 1] : 0;
-            prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
-        } else if (d0 > 0 && n_here > 0) {
-            have_prev = true;
-            const uint64_t pa = a0 > 0 ? ak[a0 - 1] : 0, pb = b0 > 0 ? vk[b0 - 1] : 0;
-            prev = (a0 > 0 && b0 > 0) ? max(pa, pb) : (a0 > 0 ? pa : pb);
-        }
-        uint64_t ok_k[MIPT];
-        int64_t ok_v[MIPT];
-        int cnt = 0;
+        for (int64_t tl = tb; tl < te; ++tl) {
+            const int64_t d0 = tl * TILE;
+            const int64_t d1 = min(d0 + TILE, S + L);
+            const int64_t a0 = sh.mg.split[tl - tb], a1 = sh.mg.split[tl - tb + 1];
+            const uint64_t tile_prev = sh.mg.prev[tl - tb];
+            const int64_t b0 = d0 - a0, b1 = d1 - a1;
+            const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+            {
+                uint64_t rk[MIPT];
+                int64_t rv[MIPT];
 #pragma unroll
-        for (int j = 0; j < MIPT; ++j) {
-            if (j < n_here) {
-                const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
-                const uint64_t key = take_b ? s_b[ib] : s_a[ia];
-                if (!have_prev || key != prev) {
-                    int64_t acc = 0;
-                    if (take_b) acc += dv[b0 + ib];
-                    int64_t ga = a0 + ia;
-                    while (ga < S && ak[ga] == key) { acc += av[ga]; ++ga; }
-                    if (acc != 0) {
-                        ok_k[cnt] = key;
-                        ok_v[cnt] = acc;
-                        ++cnt;
-                        nrm += fabs((double)acc);
+                for (int r = 0; r < MIPT; ++r) {  // all loads in flight before any is used
+                    const int i = (int)threadIdx.x + r * NT;
+                    if (i < na + nb) {
+                        const bool in_a = i < na;
+                        const uint64_t* kp = in_a ? ak + (a0 + i) : vk + (b0 + (i - na));
+                        const int64_t* vp = in_a ? av + (a0 + i) : dv + (b0 + (i - na));
+                        rk[r] = *kp;
+                        rv[r] = *vp;
                     }
                 }
-                prev = key;
+#pragma unroll
+                for (int r = 0; r < MIPT; ++r) {
+                    const int i = (int)threadIdx.x + r * NT;
+                    if (i < na) { sh.mg.a[i] = rk[r]; sh.mg.av[i] = rv[r]; }
+                    else if (i < na + nb) { sh.mg.b[i - na] = rk[r]; sh.mg.bv[i - na] = rv[r]; }
+                }
+            }
+            __syncthreads();
+            const int dd = threadIdx.x * MIPT;
+            const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
+            int ia;
+            {
+                int lo = max(0, dd - nb), hi = min(dd, na);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sh.mg.a[mid] < sh.mg.b[dd - mid - 1]) lo = mid + 1; else hi = mid;
+                }
+                ia = lo;
+            }
+            int ib = dd - ia;
+            bool have_prev = false;
+            uint64_t prev = 0;
+            if (dd > 0 && n_here > 0) {
                 have_prev = true;
-                if (take_b) ++ib; else ++ia;
+                const uint64_t pa = ia > 0 ? sh.mg.a[ia - 1] : 0, pb = ib > 0 ? sh.mg.b[ib - 1] : 0;
+                prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
+            } else if (d0 > 0 && n_here > 0) {
+                have_prev = true;
+                prev = tile_prev;
             }
-        }
-        int tot;
-        const int ex = block_exclusive_scan<int, NT>(cnt, &tot);
-        if (out + tot > capV) {
-            if (threadIdx.x == 0) s_flag = RUN_VEC_OVERFLOW;
-        } else {
-            for (int j = 0; j < cnt; ++j) {
-                nk[out + ex + j] = ok_k[j];
-                nv[out + ex + j] = ok_v[j];
+            uint64_t ok_k[MIPT];
+            int64_t ok_v[MIPT];
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < MIPT; ++j) {
+                if (j < n_here) {
+                    const bool take_b = ib < nb && (ia >= na || sh.mg.b[ib] <= sh.mg.a[ia]);
+                    const uint64_t key = take_b ? sh.mg.b[ib] : sh.mg.a[ia];
+                    if (!have_prev || key != prev) {
+                        int64_t acc = 0;
+                        if (take_b) acc += sh.mg.bv[ib];
+                        int la = ia;
+                        while (la < na && sh.mg.a[la] == key) { acc += sh.mg.av[la]; ++la; }
+                        if (la == na) {  // the run may continue past the tile
+                            int64_t ga = a0 + la;
+                            while (ga < S && ak[ga] == key) { acc += av[ga]; ++ga; }
+                        }
+                        if (acc != 0) {
+                            ok_k[cnt] = key;
+                            ok_v[cnt] = acc;
+                            ++cnt;
+                            nrm += fabs((double)acc);
+                        }
+                    }
+                    prev = key;
+                    have_prev = true;
+                    if (take_b) ++ib; else ++ia;
+                }
             }
+            int tot;
+            const int ex = block_exclusive_scan<int, NT>(cnt, &tot);  // ends with a barrier
+            if (out + tot > capV) {
+                if (threadIdx.x == 0) s_flag = RUN_VEC_OVERFLOW;
+            } else {
+                for (int j = 0; j < cnt; ++j) {
+                    nk[out + ex + j] = ok_k[j];
+                    nv[out + ex + j] = ok_v[j];
+                }
+            }
+            out += tot;
         }
-        out += tot;
-        __syncthreads();
     }
     const int64_t Lnew = min(out, capV);
 
@@ -323,8 +465,6 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
         }
     }
     block_sum3<NT>(nrm, vp, hp);
-    const double att_sum = block_sum<NT>(st_att);
-    (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie;
     if (threadIdx.x == 0) {
         const int64_t row = st.nrec;
         if (row < P.hist_cap) {
@@ -390,7 +530,11 @@ struct rimu_batch {
     DevBuf<uint64_t> vk[2], sk[2], pk, start;
     DevBuf<int64_t> vv[2], dv, sv[2];
     DevBuf<double> pc, hist;
-    int64_t totV = 0, totS = 0, totP = 0, hist_cap = 0;
+    DevBuf<uint64_t> xk;
+    DevBuf<int64_t> xv, xcnt;
+    DevBuf<double> xatt;
+    DevBuf<int> wg_run;
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, hist_cap = 0;
     std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
     Pools pools() {
         Pools P{};
@@ -405,6 +549,11 @@ struct rimu_batch {
         P.pc = pc.p;
         P.hist = hist.p;
         P.hist_cap = hist_cap;
+        P.xk = xk.p;
+        P.xv = xv.p;
+        P.xcnt = xcnt.p;
+        P.xatt = xatt.p;
+        P.wg_run = wg_run.p;
         return P;
     }
 };
@@ -413,10 +562,20 @@ namespace {
 
 int launch_step(rimu_batch* b) {
     const Pools P = b->pools();
+    const int K = (int)b->totK;
     switch (b->block) {
-        case 256: batch_step_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
-        case 1024: batch_step_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
-        default: batch_step_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix); break;
+        case 256:
+            batch_spawn_kernel<256><<<K, 256, 0, b->stream>>>(b->runs.p, P);
+            batch_step_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            break;
+        case 1024:
+            batch_spawn_kernel<1024><<<K, 1024, 0, b->stream>>>(b->runs.p, P);
+            batch_step_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            break;
+        default:
+            batch_spawn_kernel<512><<<K, 512, 0, b->stream>>>(b->runs.p, P);
+            batch_step_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            break;
     }
     HIP_TRY(hipGetLastError());
     return RIMU_OK;
@@ -476,7 +635,8 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
     std::vector<RunState> st(nruns);
     std::vector<uint64_t> start(nruns);
-    int64_t totV = 0, totS = 0, totP = 0;
+    std::vector<int> wg_run;
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0;
     for (int r = 0; r < nruns; ++r) {
         const rimu_batch_run& q = runs[r];
         if (q.M < 2 || q.N < 1 || q.M + q.N - 1 > 64) return fail(RIMU_ERR_ARG, "run: need M >= 2, N >= 1, N + M - 1 <= 64");
@@ -496,13 +656,19 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         s.capV = q.max_length;
         s.capS = (int64_t)std::min(4.0 * q.target_walkers + 4096.0, 4.0e9);
         s.offV = totV; s.offS = totS; s.offP = totP; s.nproj = 0;
-        totV += s.capV; totS += s.capS; totP += 2 * q.M + 1;
+        // spawning of a run is split over K workgroups, one per WALKERS_PER_SLICE target walkers
+        s.K = std::min<int64_t>(MAX_SLICES, std::max<int64_t>(1, (int64_t)std::ceil(q.target_walkers / WALKERS_PER_SLICE)));
+        s.slice0 = (int64_t)wg_run.size();
+        s.offX = totX;
+        for (int64_t k = 0; k < s.K; ++k) wg_run.push_back(r);
+        totV += s.capV; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
         start[r] = q.start_address;
     }
     rimu_batch* b = new rimu_batch();
     b->nruns = nruns;
     b->host_init = st;
     b->totV = totV; b->totS = totS; b->totP = totP; b->hist_cap = history_steps;
+    b->totX = totX; b->totK = (int64_t)wg_run.size();
     auto bad = [&](const char* what, hipError_t e) {
         std::string m = std::string(what) + ": " + hipGetErrorString(e);
         rimu_batch_destroy(b);
@@ -522,6 +688,12 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->pc.alloc(totP)) != hipSuccess) return bad("projector coefficients", e);
     if ((e = b->start.alloc(nruns)) != hipSuccess) return bad("start addresses", e);
     if ((e = b->hist.alloc(std::max<int64_t>(1, NHIST * history_steps * nruns))) != hipSuccess) return bad("history", e);
+    if ((e = b->xk.alloc(std::max<int64_t>(1, totX))) != hipSuccess) return bad("slice spawn keys", e);
+    if ((e = b->xv.alloc(std::max<int64_t>(1, totX))) != hipSuccess) return bad("slice spawn values", e);
+    if ((e = b->xcnt.alloc(b->totK)) != hipSuccess) return bad("slice counts", e);
+    if ((e = b->xatt.alloc(b->totK)) != hipSuccess) return bad("slice attempts", e);
+    if ((e = b->wg_run.alloc(b->totK)) != hipSuccess) return bad("slice runs", e);
+    if ((e = hipMemcpy(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpy(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpy(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
@@ -543,6 +715,7 @@ int rimu_batch_destroy(rimu_batch* b) {
     b->runs.release();
     for (int k = 0; k < 2; ++k) { b->vk[k].release(); b->vv[k].release(); b->sk[k].release(); b->sv[k].release(); }
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
+    b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return RIMU_OK;
