@@ -29,9 +29,11 @@ namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
 constexpr int EPT = 4;           // entries per thread per spawn chunk
-constexpr int SPLITS = 1024;     // merge tile boundaries located per pass
-constexpr double WALKERS_PER_SLICE = 8192;  // target walkers per spawn workgroup of a run
-constexpr int64_t MAX_SLICES = 16;
+constexpr double WALKERS_PER_SLICE = 2048;  // target walkers per spawn workgroup of a run
+constexpr int64_t MAX_SLICES = 64;
+constexpr int SPAWN_NT = 256;                // spawn workgroup size (several per CU)
+constexpr int64_t MIN_TILE = 256 * MIPT;     // merge tile of the smallest workgroup size
+constexpr int64_t MAX_TILE = 1024 * MIPT;    // and of the largest
 constexpr int MAXPROJ = 160;     // projector entries per run (start address and neighbours)
 constexpr int NHIST = 6;         // shift, norm, len, vproj, hproj, attempts
 
@@ -44,6 +46,10 @@ struct RunState {
     int64_t len, cur, step, nrec;     // nrec: history rows written
     int64_t capV, capS, offV, offS, offP, nproj;
     int64_t K, slice0, offX;          // spawn slices, first slice's global index, slice pool offset
+    int64_t offT, offM;               // merge tile arrays, merge slot pool
+    // handed from the sort kernel to the merge and final kernels of the same step
+    int64_t S_cur, ntl, sbuf, flag;
+    double att;
 };
 
 struct Pools {
@@ -60,7 +66,13 @@ struct Pools {
     int64_t* xv;
     int64_t* xcnt;       // records of every slice
     double* xatt;        // spawn attempts of every slice
-    const int* wg_run;   // run of every spawn workgroup
+    const int* wg_run;   // run of every spawn / merge workgroup
+    int64_t* msplit;     // per merge tile: spawns before its start, then survivor offsets
+    uint64_t* mprev;     // per merge tile: merged key just before it
+    int64_t* mcnt;       // per merge tile: survivors
+    double* mnrm;        // per merge tile: norm of the survivors
+    uint64_t* mk;        // per merge tile: survivor slots (TILE each)
+    int64_t* mv;
 };
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
@@ -213,22 +225,16 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
 
 }
 
+// Gather of the slices' spawn records, their radix sort, and the boundaries of every merge
+// tile (one workgroup per run).  Hands S, the sorted buffer and the tile count to the next
+// kernels through the run state.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ runs, int nruns, Pools P,
-                                                        int radix_bits) {
+__global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ runs, Pools P, int radix_bits) {
     constexpr int NW = NT / WAVE;
     constexpr int TILE = NT * MIPT;
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_wcnt[NW][256];
     __shared__ int s_flag;
-    __shared__ struct {
-        struct {
-            uint64_t a[TILE], b[TILE];    // tile keys: spawns, vector entries
-            int64_t av[TILE], bv[TILE];   // their values (diagonal survivors for b)
-            int64_t split[SPLITS + 1];    // spawns before each tile boundary
-            uint64_t prev[SPLITS + 1];    // merged key just before each tile
-        } mg;
-    } sh;
 
     const int run = blockIdx.x;
     RunState& st = runs[run];
@@ -236,16 +242,10 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-
     const int B = st.B;
-    const double dtau = st.dtau, shift = st.shift;
-    const int64_t L = st.len, capV = st.capV, capS = st.capS;
+    const int64_t L = st.len, capS = st.capS;
     const int cur = (int)st.cur;
-    const int64_t step = st.step + 1;
     const uint64_t* vk = P.vk[cur] + st.offV;
-    uint64_t* nk = P.vk[1 - cur] + st.offV;
-    int64_t* nv = P.vv[1 - cur] + st.offV;
-    int64_t* dv = P.dv + st.offV;
     uint64_t* sk0 = P.sk[0] + st.offS;
     int64_t* sv0 = P.sv[0] + st.offS;
     uint64_t* sk1 = P.sk[1] + st.offS;
@@ -329,124 +329,200 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
         int64_t* tv = srcv; srcv = dstv; dstv = tv;
     }
 
-    // ---- merge the sorted spawns (A) with the diagonal survivors (B), fold, count ------
-    // merged order: by key, B before A on equal keys; each address folds diag + spawns.
-    // The tile boundaries of up to SPLITS tiles are located at once (one thread each).
+    // ---- merge tile boundaries: spawns before each tile start and the key before it -------
     const uint64_t* ak = srck;
-    const int64_t* av = srcv;
-    int64_t out = 0;
-    double nrm = 0;
     const int64_t ntl = (S + L + TILE - 1) / TILE;
-    for (int64_t tb = 0; tb < ntl; tb += SPLITS) {
-        const int64_t te = min(tb + SPLITS, ntl);
-        __syncthreads();
-        for (int64_t q = threadIdx.x; q <= te - tb; q += NT) {
-            const int64_t d = min((tb + q) * TILE, S + L);
-            int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (ak[mid] < vk[d - mid - 1]) lo = mid + 1; else hi = mid;
+    for (int64_t q = threadIdx.x; q <= ntl; q += NT) {
+        const int64_t d = min(q * TILE, S + L);
+        int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (ak[mid] < vk[d - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        P.msplit[st.offT + q] = lo;
+        const int64_t bi = d - lo;
+        const uint64_t pa = lo > 0 ? ak[lo - 1] : 0, pb = bi > 0 ? vk[bi - 1] : 0;
+        P.mprev[st.offT + q] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
+    }
+    if (threadIdx.x == 0) {
+        st.S_cur = S;
+        st.ntl = ntl;
+        st.sbuf = srck == sk0 ? 0 : 1;
+        st.flag = s_flag;
+        st.att = att_sum;
+    }
+}
+
+// Merge of the sorted spawns (A) with the diagonal survivors (B): workgroup w of a run takes
+// the tiles [ntl*w/K, ntl*(w+1)/K).  Merged order: by key, B before A on equal keys; each
+// address folds diag + spawns.  Survivors go to the tile's slots with their count and norm.
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ runs, Pools P) {
+    constexpr int TILE = NT * MIPT;
+    __shared__ uint64_t s_a[TILE], s_b[TILE];   // tile keys: spawns, vector entries
+    __shared__ int64_t s_av[TILE], s_bv[TILE];  // their values (diagonal survivors for b)
+
+    const int run = P.wg_run[blockIdx.x];
+    const RunState& st = runs[run];
+    if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const int64_t w = (int64_t)blockIdx.x - st.slice0;
+    const int64_t ntl = st.ntl, S = st.S_cur, L = st.len;
+    const int64_t t_lo = ntl * w / st.K, t_hi = ntl * (w + 1) / st.K;
+    const int cur = (int)st.cur;
+    const uint64_t* vk = P.vk[cur] + st.offV;
+    const int64_t* dv = P.dv + st.offV;
+    const uint64_t* ak = P.sk[st.sbuf] + st.offS;
+    const int64_t* av = P.sv[st.sbuf] + st.offS;
+    uint64_t* mk = P.mk + st.offM;
+    int64_t* mv = P.mv + st.offM;
+
+    for (int64_t tl = t_lo; tl < t_hi; ++tl) {
+        const int64_t d0 = tl * TILE;
+        const int64_t d1 = min(d0 + TILE, S + L);
+        const int64_t a0 = P.msplit[st.offT + tl], a1 = P.msplit[st.offT + tl + 1];
+        const uint64_t tile_prev = P.mprev[st.offT + tl];
+        const int64_t b0 = d0 - a0, b1 = d1 - a1;
+        const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+        {
+            uint64_t rk[MIPT];
+            int64_t rv[MIPT];
+#pragma unroll
+            for (int r = 0; r < MIPT; ++r) {  // all loads in flight before any is used
+                const int i = (int)threadIdx.x + r * NT;
+                if (i < na + nb) {
+                    const bool in_a = i < na;
+                    const uint64_t* kp = in_a ? ak + (a0 + i) : vk + (b0 + (i - na));
+                    const int64_t* vp = in_a ? av + (a0 + i) : dv + (b0 + (i - na));
+                    rk[r] = *kp;
+                    rv[r] = *vp;
+                }
             }
-            sh.mg.split[q] = lo;
-            const int64_t bi = d - lo;
-            const uint64_t pa = lo > 0 ? ak[lo - 1] : 0, pb = bi > 0 ? vk[bi - 1] : 0;
-            sh.mg.prev[q] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
+#pragma unroll
+            for (int r = 0; r < MIPT; ++r) {
+                const int i = (int)threadIdx.x + r * NT;
+                if (i < na) { s_a[i] = rk[r]; s_av[i] = rv[r]; }
+                else if (i < na + nb) { s_b[i - na] = rk[r]; s_bv[i - na] = rv[r]; }
+            }
         }
         __syncthreads();
-        for (int64_t tl = tb; tl < te; ++tl) {
-            const int64_t d0 = tl * TILE;
-            const int64_t d1 = min(d0 + TILE, S + L);
-            const int64_t a0 = sh.mg.split[tl - tb], a1 = sh.mg.split[tl - tb + 1];
-            const uint64_t tile_prev = sh.mg.prev[tl - tb];
-            const int64_t b0 = d0 - a0, b1 = d1 - a1;
-            const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-            {
-                uint64_t rk[MIPT];
-                int64_t rv[MIPT];
+        const int dd = threadIdx.x * MIPT;
+        const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
+        int ia;
+        {
+            int lo = max(0, dd - nb), hi = min(dd, na);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
+            }
+            ia = lo;
+        }
+        int ib = dd - ia;
+        bool have_prev = false;
+        uint64_t prev = 0;
+        if (dd > 0 && n_here > 0) {
+            have_prev = true;
+            const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0, pb = ib > 0 ? s_b[ib - 1] : 0;
+            prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
+        } else if (d0 > 0 && n_here > 0) {
+            have_prev = true;
+            prev = tile_prev;
+        }
+        uint64_t ok_k[MIPT];
+        int64_t ok_v[MIPT];
+        int cnt = 0;
+        double nrm = 0;
 #pragma unroll
-                for (int r = 0; r < MIPT; ++r) {  // all loads in flight before any is used
-                    const int i = (int)threadIdx.x + r * NT;
-                    if (i < na + nb) {
-                        const bool in_a = i < na;
-                        const uint64_t* kp = in_a ? ak + (a0 + i) : vk + (b0 + (i - na));
-                        const int64_t* vp = in_a ? av + (a0 + i) : dv + (b0 + (i - na));
-                        rk[r] = *kp;
-                        rv[r] = *vp;
+        for (int j = 0; j < MIPT; ++j) {
+            if (j < n_here) {
+                const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
+                const uint64_t key = take_b ? s_b[ib] : s_a[ia];
+                if (!have_prev || key != prev) {
+                    int64_t acc = 0;
+                    if (take_b) acc += s_bv[ib];
+                    int la = ia;
+                    while (la < na && s_a[la] == key) { acc += s_av[la]; ++la; }
+                    if (la == na) {  // the run may continue past the tile
+                        int64_t ga = a0 + la;
+                        while (ga < S && ak[ga] == key) { acc += av[ga]; ++ga; }
+                    }
+                    if (acc != 0) {
+                        ok_k[cnt] = key;
+                        ok_v[cnt] = acc;
+                        ++cnt;
+                        nrm += fabs((double)acc);
                     }
                 }
-#pragma unroll
-                for (int r = 0; r < MIPT; ++r) {
-                    const int i = (int)threadIdx.x + r * NT;
-                    if (i < na) { sh.mg.a[i] = rk[r]; sh.mg.av[i] = rv[r]; }
-                    else if (i < na + nb) { sh.mg.b[i - na] = rk[r]; sh.mg.bv[i - na] = rv[r]; }
-                }
-            }
-            __syncthreads();
-            const int dd = threadIdx.x * MIPT;
-            const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
-            int ia;
-            {
-                int lo = max(0, dd - nb), hi = min(dd, na);
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sh.mg.a[mid] < sh.mg.b[dd - mid - 1]) lo = mid + 1; else hi = mid;
-                }
-                ia = lo;
-            }
-            int ib = dd - ia;
-            bool have_prev = false;
-            uint64_t prev = 0;
-            if (dd > 0 && n_here > 0) {
+                prev = key;
                 have_prev = true;
-                const uint64_t pa = ia > 0 ? sh.mg.a[ia - 1] : 0, pb = ib > 0 ? sh.mg.b[ib - 1] : 0;
-                prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
-            } else if (d0 > 0 && n_here > 0) {
-                have_prev = true;
-                prev = tile_prev;
+                if (take_b) ++ib; else ++ia;
             }
-            uint64_t ok_k[MIPT];
-            int64_t ok_v[MIPT];
-            int cnt = 0;
-#pragma unroll
-            for (int j = 0; j < MIPT; ++j) {
-                if (j < n_here) {
-                    const bool take_b = ib < nb && (ia >= na || sh.mg.b[ib] <= sh.mg.a[ia]);
-                    const uint64_t key = take_b ? sh.mg.b[ib] : sh.mg.a[ia];
-                    if (!have_prev || key != prev) {
-                        int64_t acc = 0;
-                        if (take_b) acc += sh.mg.bv[ib];
-                        int la = ia;
-                        while (la < na && sh.mg.a[la] == key) { acc += sh.mg.av[la]; ++la; }
-                        if (la == na) {  // the run may continue past the tile
-                            int64_t ga = a0 + la;
-                            while (ga < S && ak[ga] == key) { acc += av[ga]; ++ga; }
-                        }
-                        if (acc != 0) {
-                            ok_k[cnt] = key;
-                            ok_v[cnt] = acc;
-                            ++cnt;
-                            nrm += fabs((double)acc);
-                        }
-                    }
-                    prev = key;
-                    have_prev = true;
-                    if (take_b) ++ib; else ++ia;
-                }
-            }
-            int tot;
-            const int ex = block_exclusive_scan<int, NT>(cnt, &tot);  // ends with a barrier
-            if (out + tot > capV) {
-                if (threadIdx.x == 0) s_flag = RUN_VEC_OVERFLOW;
-            } else {
-                for (int j = 0; j < cnt; ++j) {
-                    nk[out + ex + j] = ok_k[j];
-                    nv[out + ex + j] = ok_v[j];
-                }
-            }
-            out += tot;
+        }
+        int tot;
+        const int ex = block_exclusive_scan<int, NT>(cnt, &tot);  // ends with a barrier
+        for (int j = 0; j < cnt; ++j) {
+            mk[d0 + ex + j] = ok_k[j];
+            mv[d0 + ex + j] = ok_v[j];
+        }
+        const double tn = block_sum<NT>(nrm);
+        if (threadIdx.x == 0) {
+            P.mcnt[st.offT + tl] = tot;
+            P.mnrm[st.offT + tl] = tn;
         }
     }
+}
+
+// The new vector (the tiles' survivors in tile order), ProjectedEnergy, statistics, the
+// history row and the DoubleLogUpdate shift (one workgroup per run).  Tile t is copied only
+// if all of it fits max_length, so an overflowing step leaves a prefix of whole tiles.
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ runs, int nruns, Pools P) {
+    constexpr int TILE = NT * MIPT;
+    const int run = blockIdx.x;
+    RunState& st = runs[run];
+    if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const double dtau = st.dtau, shift = st.shift;
+    const int64_t capV = st.capV, ntl = st.ntl;
+    const int cur = (int)st.cur;
+    const int64_t step = st.step + 1;
+    uint64_t* nk = P.vk[1 - cur] + st.offV;
+    int64_t* nv = P.vv[1 - cur] + st.offV;
+    const uint64_t* mk = P.mk + st.offM;
+    const int64_t* mv = P.mv + st.offM;
+    int64_t* offs = P.msplit + st.offT;   // the splits are spent: offsets of the survivors
+
+    // exclusive scan of the tile counts, NT tiles at a time
+    int64_t carry = 0;
+    for (int64_t t0 = 0; t0 < ntl; t0 += NT) {
+        const int64_t t = t0 + threadIdx.x;
+        const int64_t c = t < ntl ? P.mcnt[st.offT + t] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan<int64_t, NT>(c, &tot);
+        if (t < ntl) offs[t] = carry + ex;
+        carry += tot;
+    }
+    const int64_t out = carry;
+    __syncthreads();
+    // one wavefront per tile; tiles that do not fit entirely are skipped (and so is every
+    // later one, since the offsets grow)
+    {
+        constexpr int NW = NT / WAVE;
+        const int lane = threadIdx.x & (WAVE - 1);
+        for (int64_t t = threadIdx.x / WAVE; t < ntl; t += NW) {
+            const int64_t o = offs[t], n = P.mcnt[st.offT + t];
+            if (o + n > capV) continue;
+            for (int64_t i = lane; i < n; i += WAVE) {
+                nk[o + i] = mk[t * TILE + i];
+                nv[o + i] = mv[t * TILE + i];
+            }
+        }
+    }
+    __syncthreads();
+    double nrm = 0;  // integer walkers: the sum of the tile norms is exact in any order
+    for (int64_t t = threadIdx.x; t < ntl; t += NT) nrm += P.mnrm[st.offT + t];
     const int64_t Lnew = min(out, capV);
+    int flag = (int)st.flag;
+    if (out > capV && flag == 0) flag = RUN_VEC_OVERFLOW;
+    const double att_sum = st.att;
 
     // ---- ProjectedEnergy, statistics and the shift update -----------------------------
     double vp = 0, hp = 0;
@@ -480,8 +556,8 @@ __global__ __launch_bounds__(NT) void batch_step_kernel(RunState* __restrict__ r
             P.hist[5 * plane + base] = att_sum;
             st.nrec = row + 1;
         }
-        if (s_flag != 0) {
-            st.status = s_flag;
+        if (flag != 0) {
+            st.status = flag;
         } else if (Lnew == 0) {
             st.status = RUN_DEAD;
         } else {
@@ -534,7 +610,10 @@ struct rimu_batch {
     DevBuf<int64_t> xv, xcnt;
     DevBuf<double> xatt;
     DevBuf<int> wg_run;
-    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, hist_cap = 0;
+    DevBuf<int64_t> msplit, mcnt, mv;
+    DevBuf<uint64_t> mprev, mk;
+    DevBuf<double> mnrm;
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, hist_cap = 0;
     std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
     Pools pools() {
         Pools P{};
@@ -554,6 +633,12 @@ struct rimu_batch {
         P.xcnt = xcnt.p;
         P.xatt = xatt.p;
         P.wg_run = wg_run.p;
+        P.msplit = msplit.p;
+        P.mprev = mprev.p;
+        P.mcnt = mcnt.p;
+        P.mnrm = mnrm.p;
+        P.mk = mk.p;
+        P.mv = mv.p;
         return P;
     }
 };
@@ -563,18 +648,22 @@ namespace {
 int launch_step(rimu_batch* b) {
     const Pools P = b->pools();
     const int K = (int)b->totK;
+    batch_spawn_kernel<SPAWN_NT><<<K, SPAWN_NT, 0, b->stream>>>(b->runs.p, P);
     switch (b->block) {
         case 256:
-            batch_spawn_kernel<256><<<K, 256, 0, b->stream>>>(b->runs.p, P);
-            batch_step_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            batch_sort_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, P, b->radix);
+            batch_merge_kernel<256><<<K, 256, 0, b->stream>>>(b->runs.p, P);
+            batch_final_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
         case 1024:
-            batch_spawn_kernel<1024><<<K, 1024, 0, b->stream>>>(b->runs.p, P);
-            batch_step_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            batch_sort_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, P, b->radix);
+            batch_merge_kernel<1024><<<K, 1024, 0, b->stream>>>(b->runs.p, P);
+            batch_final_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
         default:
-            batch_spawn_kernel<512><<<K, 512, 0, b->stream>>>(b->runs.p, P);
-            batch_step_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P, b->radix);
+            batch_sort_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, P, b->radix);
+            batch_merge_kernel<512><<<K, 512, 0, b->stream>>>(b->runs.p, P);
+            batch_final_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
     }
     HIP_TRY(hipGetLastError());
@@ -636,7 +725,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     std::vector<RunState> st(nruns);
     std::vector<uint64_t> start(nruns);
     std::vector<int> wg_run;
-    int64_t totV = 0, totS = 0, totP = 0, totX = 0;
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totT = 0, totM = 0;
     for (int r = 0; r < nruns; ++r) {
         const rimu_batch_run& q = runs[r];
         if (q.M < 2 || q.N < 1 || q.M + q.N - 1 > 64) return fail(RIMU_ERR_ARG, "run: need M >= 2, N >= 1, N + M - 1 <= 64");
@@ -661,6 +750,11 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         s.slice0 = (int64_t)wg_run.size();
         s.offX = totX;
         for (int64_t k = 0; k < s.K; ++k) wg_run.push_back(r);
+        // merge tiles: boundaries for the smallest tile, slots for the largest
+        s.offT = totT;
+        s.offM = totM;
+        totT += (s.capS + s.capV) / MIN_TILE + 3;
+        totM += s.capS + s.capV + MAX_TILE;
         totV += s.capV; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
         start[r] = q.start_address;
     }
@@ -668,7 +762,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     b->nruns = nruns;
     b->host_init = st;
     b->totV = totV; b->totS = totS; b->totP = totP; b->hist_cap = history_steps;
-    b->totX = totX; b->totK = (int64_t)wg_run.size();
+    b->totX = totX; b->totK = (int64_t)wg_run.size(); b->totT = totT; b->totM = totM;
     auto bad = [&](const char* what, hipError_t e) {
         std::string m = std::string(what) + ": " + hipGetErrorString(e);
         rimu_batch_destroy(b);
@@ -693,6 +787,12 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->xcnt.alloc(b->totK)) != hipSuccess) return bad("slice counts", e);
     if ((e = b->xatt.alloc(b->totK)) != hipSuccess) return bad("slice attempts", e);
     if ((e = b->wg_run.alloc(b->totK)) != hipSuccess) return bad("slice runs", e);
+    if ((e = b->msplit.alloc(totT)) != hipSuccess) return bad("merge splits", e);
+    if ((e = b->mprev.alloc(totT)) != hipSuccess) return bad("merge keys", e);
+    if ((e = b->mcnt.alloc(totT)) != hipSuccess) return bad("merge counts", e);
+    if ((e = b->mnrm.alloc(totT)) != hipSuccess) return bad("merge norms", e);
+    if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
+    if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
     if ((e = hipMemcpy(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpy(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpy(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
@@ -716,6 +816,7 @@ int rimu_batch_destroy(rimu_batch* b) {
     for (int k = 0; k < 2; ++k) { b->vk[k].release(); b->vv[k].release(); b->sk[k].release(); b->sv[k].release(); }
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
     b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
+    b->msplit.release(); b->mprev.release(); b->mcnt.release(); b->mnrm.release(); b->mk.release(); b->mv.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return RIMU_OK;
