@@ -29,6 +29,7 @@ namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
 constexpr int EPT = 4;           // entries per thread per spawn chunk
+constexpr int AIT = 4;           // attempts per thread per spawn group
 constexpr double WALKERS_PER_SLICE = 2048;  // target walkers per spawn workgroup of a run
 constexpr int64_t MAX_SLICES = 64;
 constexpr int SPAWN_NT = 256;                // spawn workgroup size (several per CU)
@@ -84,7 +85,8 @@ template <int NT>
 __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ runs, Pools P) {
     constexpr int NW = NT / WAVE;
     constexpr int CH = NT * EPT;          // entries per spawn chunk
-    __shared__ int s_wc[NW];
+    __shared__ int s_cnt[AIT * NW];
+    __shared__ int s_tot;
     __shared__ int64_t s_S;
     __shared__ struct {
         uint64_t k[CH];        // the chunk's keys
@@ -170,51 +172,66 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
         if (threadIdx.x == 0) sp.aoff[CH] = T;
         st_att += (double)sum;
         __syncthreads();
-        for (int64_t a0 = 0; a0 < T; a0 += NT) {
-            const int64_t att = a0 + threadIdx.x;
-            bool ok = false;
-            uint64_t nbk = 0;
-            int64_t nbv = 0;
-            if (att < T) {
-                int lo = 0, hi = CH;  // aoff[lo] <= att < aoff[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sp.aoff[mid] <= att) lo = mid; else hi = mid;
+        // attempts in groups of NT * AIT: one ballot per attempt, then a scan of the
+        // (attempt item, wavefront) counts gives every record its place in attempt order
+        for (int64_t a0 = 0; a0 < T; a0 += NT * AIT) {
+            bool ok[AIT];
+            uint64_t nbk[AIT];
+            int64_t nbv[AIT];
+            int rpos[AIT];
+#pragma unroll
+            for (int it = 0; it < AIT; ++it) {
+                const int64_t att = a0 + it * NT + threadIdx.x;
+                ok[it] = false;
+                nbk[it] = 0;
+                nbv[it] = 0;
+                if (att < T) {
+                    int lo = 0, hi = CH;  // aoff[lo] <= att < aoff[hi]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sp.aoff[mid] <= att) lo = mid; else hi = mid;
+                    }
+                    const uint64_t k = sp.k[lo];
+                    const int64_t v = sp.v[lo];
+                    const int64_t j = att - sp.aoff[lo];
+                    const int64_t n_od = 2 * (int64_t)num_occupied(k);
+                    const int64_t na = v < 0 ? -v : v;
+                    const double magnitude = (double)v / (double)na;
+                    const double prob = 1.0 / (double)n_od;
+                    const U4 r = draw(seed, k, (uint32_t)j, step, TAG_SPAWN);
+                    int64_t prod;
+                    nbk[it] = real1d_hop(k, pick1(r.x, n_od), M, B, &prod);
+                    const double mel = -(-t * sqrt((double)prod)) * dtau;
+                    const double x = mel * magnitude / prob;
+                    nbv[it] = isgn(x) * (int64_t)floor(fabs(x) + u53(r.y, r.z));
+                    st_spawn += fabs((double)nbv[it]);
+                    ok[it] = nbv[it] != 0;
                 }
-                const uint64_t k = sp.k[lo];
-                const int64_t v = sp.v[lo];
-                const int64_t j = att - sp.aoff[lo];
-                const int64_t n_od = 2 * (int64_t)num_occupied(k);
-                const int64_t na = v < 0 ? -v : v;
-                const double magnitude = (double)v / (double)na;
-                const double prob = 1.0 / (double)n_od;
-                const U4 r = draw(seed, k, (uint32_t)j, step, TAG_SPAWN);
-                int64_t prod;
-                nbk = real1d_hop(k, pick1(r.x, n_od), M, B, &prod);
-                const double mel = -(-t * sqrt((double)prod)) * dtau;
-                const double x = mel * magnitude / prob;
-                nbv = isgn(x) * (int64_t)floor(fabs(x) + u53(r.y, r.z));
-                st_spawn += fabs((double)nbv);
-                ok = nbv != 0;
-            }
-            const uint64_t bal = __ballot(ok);
-            if (lane == 0) s_wc[wid] = __popcll(bal);
-            __syncthreads();
-            int64_t before = s_S;
-            int tot = 0;
-            for (int w = 0; w < NW; ++w) {
-                if (w < wid) before += s_wc[w];
-                tot += s_wc[w];
-            }
-            if (ok) {
-                const int64_t pos = before + __popcll(bal & lt_mask);
-                if (pos < capS) { sk0[pos] = nbk; sv0[pos] = nbv; }
+                const uint64_t bal = __ballot(ok[it]);
+                rpos[it] = __popcll(bal & lt_mask);
+                if (lane == 0) s_cnt[it * NW + wid] = __popcll(bal);
             }
             __syncthreads();
-            if (threadIdx.x == 0) s_S += tot;
+            if (threadIdx.x == 0) {
+                int64_t run = s_S;
+                for (int q = 0; q < AIT * NW; ++q) { const int c = s_cnt[q]; s_cnt[q] = (int)(run - s_S); run += c; }
+                s_tot = (int)(run - s_S);
+            }
+            __syncthreads();
+            const int64_t base = s_S;
+#pragma unroll
+            for (int it = 0; it < AIT; ++it) {
+                if (ok[it]) {
+                    const int64_t pos = base + s_cnt[it * NW + wid] + rpos[it];
+                    if (pos < capS) { sk0[pos] = nbk[it]; sv0[pos] = nbv[it]; }
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_S += s_tot;
             __syncthreads();
         }
     }
+    __syncthreads();
     // record count (it may exceed capS: the gather then reports the overflow) and attempts
     const double att_sum = block_sum<NT>(st_att);
     if (threadIdx.x == 0) {
