@@ -73,6 +73,12 @@ def lib():
                               ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                               u64p, vp, ctypes.c_int64, u64p, vp, ctypes.c_int64, dp]
         L.or_step.restype = ctypes.c_int64
+        L.or_step_initiator.argtypes = [P(Model), P(Style), ctypes.c_uint64, ctypes.c_int64,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double, u64p, vp, ctypes.c_int64, u64p, vp,
+                                        ctypes.c_int64, dp]
+        L.or_step_initiator.restype = ctypes.c_int64
         L.or_step_threaded.argtypes = [P(Model), P(Style), ctypes.c_uint64, ctypes.c_int64,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int32, ctypes.c_int32, u64p, vp, ctypes.c_int64,
@@ -228,6 +234,32 @@ def step(model: Model, sty: Style, keys, vals, *, seed: int, step: int, shift: f
                               boost, order, vtype, _ptr(keys, ctypes.c_uint64), vals.ctypes.data,
                               len(keys), _ptr(ok, ctypes.c_uint64), ov.ctypes.data, cap,
                               _ptr(st, ctypes.c_double))
+        if n >= 0:
+            return ok[:n], ov[:n], dict(zip(STAT_NAMES, st[:10].tolist()))
+        cap = -n + 16
+
+
+INITIATOR_RULES = {"none": 0, "initiator": 1, "simple": 2, "coherent": 3}
+
+
+def step_initiator(model: Model, sty: Style, keys, vals, *, rule: str, threshold: float,
+                   seed: int, step: int, shift: float, dtau: float, boost: float = 1.0,
+                   order: int = 0):
+    """One step with an initiator rule (``rule`` in INITIATOR_RULES).  With NoCompression
+    the result may hold zero values (addresses that received only discarded deposits)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    vtype = 0 if sty.style == STYLE_INTEGER else 1
+    vals = np.ascontiguousarray(vals, dtype=np.int64 if vtype == 0 else np.float64)
+    cap = max(64, 4 * len(keys) + 1024)
+    while True:
+        ok = np.zeros(cap, dtype=np.uint64)
+        ov = np.zeros(cap, dtype=vals.dtype)
+        st = np.zeros(16, dtype=np.float64)
+        n = lib().or_step_initiator(ctypes.byref(model), ctypes.byref(sty), seed, step, shift,
+                                    dtau, boost, order, vtype, INITIATOR_RULES[rule],
+                                    float(threshold), _ptr(keys, ctypes.c_uint64),
+                                    vals.ctypes.data, len(keys), _ptr(ok, ctypes.c_uint64),
+                                    ov.ctypes.data, cap, _ptr(st, ctypes.c_double))
         if n >= 0:
             return ok[:n], ov[:n], dict(zip(STAT_NAMES, st[:10].tolist()))
         cap = -n + 16

@@ -541,9 +541,138 @@ int64_t step_impl(const Params& p, int order, const uint64_t* keys, const V* val
     return dump_sorted<V>(w.m, out_keys, out_vals, cap);
 }
 
+// ---------------------------------------------------------------------------------
+// Initiator approximation (DictVectors/initiators.jl).  The working memory holds an
+// InitiatorValue (safe, unsafe, initiator) per address (initiators.jl:22-45).  A deposit
+// from parent p_add => p_val lands in one component by to_initiator_value
+// (initiators.jl:143-157): on the parent's own address (the diagonal deposit) in
+// `initiator` when |p_val| > threshold, else in `safe`; on any other address in `safe`
+// when |p_val| > threshold, else in `unsafe`.  An entry whose three components are all
+// zero is deleted (deposit!, pdworkingmemory.jl:21-31; initiatordvec.jl deposit!).
+// move_and_compress! (pdworkingmemory.jl:262-273) maps every entry through the rule's
+// from_initiator_value and hands the values to compress!: ThresholdCompression keeps
+// nonzero results only (compression.jl:14-26, _set_value! on a Dict deletes zeros) and
+// counts every entry in len_before; NoCompression stores every entry, a zero result
+// included (stochasticstyles.jl:116-121, plain Dict setindex!).
+// ---------------------------------------------------------------------------------
+template <class V>
+struct IValue {
+    V safe = 0, unsafe = 0, init = 0;
+};
+
+// rule: 0 NonInitiator, 1 Initiator, 2 SimpleInitiator, 3 CoherentInitiator.
+template <class V>
+V from_initiator_value(int rule, double thr, const IValue<V>& x) {
+    switch (rule) {
+        case 1:  // initiators.jl:138-140: safe + initiator + !iszero(initiator) * unsafe
+            return (x.safe + x.init) + (x.init != 0 ? x.unsafe : V(0));
+        case 2:  // initiators.jl:180-182: safe + initiator
+            return x.safe + x.init;
+        case 3:  // initiators.jl:208-214
+            if (x.init != 0 || std::fabs((double)x.unsafe) > thr) return (x.init + x.safe) + x.unsafe;
+            return x.init + x.safe;
+        default:  // NonInitiator: a single value, kept in `safe`
+            return x.safe;
+    }
+}
+
+template <class V>
+struct InitiatorMemory {
+    std::unordered_map<uint64_t, IValue<V>> m;
+    int rule;
+    double thr;
+    // deposit!(w, add, val, p_add => p_val) with to_initiator_value
+    void deposit(uint64_t add, V val, uint64_t p_add, V p_val) {
+        IValue<V> d;
+        const bool big = std::fabs((double)p_val) > thr;
+        if (rule == 0) d.safe = val;
+        else if (p_add == add) { if (big) d.init = val; else d.safe = val; }
+        else { if (big) d.safe = val; else d.unsafe = val; }
+        auto it = m.find(add);
+        IValue<V> nv = it == m.end() ? IValue<V>{} : it->second;
+        nv.safe = nv.safe + d.safe;
+        nv.unsafe = nv.unsafe + d.unsafe;
+        nv.init = nv.init + d.init;
+        const bool zero = nv.safe == 0 && nv.unsafe == 0 && nv.init == 0;
+        if (zero) { if (it != m.end()) m.erase(it); }
+        else if (it == m.end()) m.emplace(add, nv);
+        else it->second = nv;
+    }
+};
+
+template <class V>
+int64_t step_initiator_impl(const Params& p, int order, int rule, double thr, const uint64_t* keys,
+                            const V* vals, int64_t n, uint64_t* out_keys, V* out_vals, int64_t cap,
+                            double* stats) {
+    InitiatorMemory<V> w;
+    w.rule = rule;
+    w.thr = thr;
+    Stats st;
+    if (order == 0) {
+        for (int64_t i = 0; i < n; ++i) {
+            const uint64_t self = keys[i];
+            const V pv = vals[i];
+            apply_column<V>(p, self, pv, st, [&](uint64_t k, V v) { w.deposit(k, v, self, pv); });
+        }
+    } else {
+        // list order: every diagonal deposit first (entry order), then every spawn
+        struct Dep { uint64_t k; V v; uint64_t pk; V pv; };
+        std::vector<Dep> diag, spawns;
+        for (int64_t i = 0; i < n; ++i) {
+            const uint64_t self = keys[i];
+            const V pv = vals[i];
+            apply_column<V>(p, self, pv, st, [&](uint64_t k, V v) {
+                if (k == self) diag.push_back({k, v, self, pv}); else spawns.push_back({k, v, self, pv});
+            });
+        }
+        for (auto& d : diag) w.deposit(d.k, d.v, d.pk, d.pv);
+        for (auto& d : spawns) w.deposit(d.k, d.v, d.pk, d.pv);
+    }
+    // move_and_compress!
+    const double len_before = (double)w.m.size();
+    std::unordered_map<uint64_t, V> out;
+    const bool threshold = p.s->compression == 1 && !std::is_integral<V>::value;
+    for (auto& kv : w.m) {
+        V val = from_initiator_value<V>(rule, thr, kv.second);
+        if (threshold) {
+            const double th = p.s->compression_threshold;
+            const double prob = std::fabs((double)val) / th;
+            if (val != 0 && prob < 1) {
+                P4 r = draw(p.seed, kv.first, 0, p.step, TAG_COMPRESS);
+                const double sg = (val > 0) - (val < 0);
+                val = (V)((prob > u53(r.v[0], r.v[1])) ? th * sg : 0.0);
+            }
+            if (val == 0) continue;
+        }
+        out.emplace(kv.first, val);
+    }
+    double norm = 0;
+    for (auto& kv : out) norm += std::fabs((double)kv.second);
+    stats[0] = st.attempts; stats[1] = st.exact; stats[2] = st.inexact; stats[3] = st.spawns;
+    stats[4] = st.deaths; stats[5] = st.clones; stats[6] = st.zombies; stats[7] = len_before;
+    stats[8] = (double)out.size(); stats[9] = norm;
+    return dump_sorted<V>(out, out_keys, out_vals, cap);
+}
+
 }  // namespace
 
 extern "C" {
+
+// One FCIQMC step of a vector with an initiator rule (rule 0 NonInitiator, 1 Initiator,
+// 2 SimpleInitiator, 3 CoherentInitiator; threshold as in the rule's constructor).
+// Arguments and result otherwise as or_step.  With NoCompression the output may hold
+// entries whose value is zero (an address that received only discarded deposits).
+int64_t or_step_initiator(const or_model* h, const or_style* s, uint64_t seed, int64_t step,
+                          double shift, double dtau, double boost, int32_t order, int32_t vtype,
+                          int32_t rule, double threshold, const uint64_t* keys, const void* vals,
+                          int64_t n, uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
+    Params p{h, s, seed, step, shift, dtau, boost};
+    if (vtype == 0)
+        return step_initiator_impl<int64_t>(p, order, rule, threshold, keys, (const int64_t*)vals, n,
+                                            out_keys, (int64_t*)out_vals, cap, stats);
+    return step_initiator_impl<double>(p, order, rule, threshold, keys, (const double*)vals, n,
+                                       out_keys, (double*)out_vals, cap, stats);
+}
 
 // One FCIQMC step (apply_operator! + compress!) of the source vector (keys, vals).
 // vtype 0: int64 values (IsStochasticInteger); 1: double values.
