@@ -136,6 +136,25 @@ int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int
 int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
               rimu_step_stats* stats);
 
+/* Initiator rules: NonInitiator, Initiator, SimpleInitiator, CoherentInitiator
+ * (DictVectors/initiators.jl:132-236). */
+enum {
+    RIMU_INITIATOR_NONE = 0,
+    RIMU_INITIATOR_DEFAULT = 1,
+    RIMU_INITIATOR_SIMPLE = 2,
+    RIMU_INITIATOR_COHERENT = 3
+};
+
+/* Select the initiator rule of the engine's vector: the `initiator` keyword of
+ * PDVec / InitiatorDVec (pdvec.jl:168-185, initiatordvec.jl:42-49); threshold is the
+ * rule's `threshold` (an entry with |v| > threshold is an initiator).  From the next step
+ * on, every deposit is classified by its parent (to_initiator_value, initiators.jl:143-157)
+ * and each address keeps the rule's from_initiator_value of its three components
+ * (move_and_compress!, pdworkingmemory.jl:262-273).  Under NoCompression an address that
+ * received only discarded deposits stays in the vector with value 0, as in the reference.
+ * Needs N + M - 1 <= 63.  RIMU_INITIATOR_NONE restores the plain step. */
+int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold);
+
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on
  * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled);

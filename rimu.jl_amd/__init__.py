@@ -17,7 +17,9 @@ from .hamiltonians import (HubbardMom1D, HubbardReal1D, diagonal_element, dimens
                            hubbard_dispersion)
 from .styles import (IsDeterministic, IsDynamicSemistochastic, IsStochasticInteger,
                      NoCompression, ThresholdCompression, WithReplacement, default_style)
-from .dictvectors import DVec, DeviceDVec, FrozenDVec, freeze
+from .initiators import (CoherentInitiator, Initiator, InitiatorRule, InitiatorValue,
+                         NonInitiator, SimpleInitiator)
+from .dictvectors import DVec, DeviceDVec, FrozenDVec, InitiatorDVec, PDVec, freeze
 from .strategies import (ConstantTimeStep, DoubleLogUpdate, DoubleLogUpdateAfterTargetWalkers,
                          DontUpdate, LogUpdate, LogUpdateAfterTargetWalkers, ProjectedEnergy)
 from .pmc import (PMCSimulation, ProjectorMonteCarloProblem, default_starting_vector, init,

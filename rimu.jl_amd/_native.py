@@ -17,12 +17,14 @@ LIB_PATH = HERE / "librimu_hip.so"
 RIMU_OK, RIMU_ERR_ARG, RIMU_ERR_HIP, RIMU_ERR_CAPACITY, RIMU_ERR_NODEVICE, RIMU_ERR_INTERNAL = range(6)
 MODEL_REAL1D, MODEL_MOM1D = 1, 2
 STYLE_INTEGER, STYLE_DETERMINISTIC, STYLE_DYNAMIC = 1, 2, 3
+INITIATOR_NONE, INITIATOR_DEFAULT, INITIATOR_SIMPLE, INITIATOR_COHERENT = 0, 1, 2, 3
 
 # Every symbol declared in include/rimu_hip.h (checked by the CPU test-suite).
 EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
-    "rimu_set_projector", "rimu_step", "rimu_engine_set_timing", "rimu_engine_kernel_times",
+    "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_engine_set_timing",
+    "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
     "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
@@ -91,6 +93,7 @@ def lib():
         L.rimu_vector_download.argtypes = [vp, vp, vp, i64, P(i64)]
         L.rimu_set_projector.argtypes = [vp, vp, vp, i64, vp, vp, i64]
         L.rimu_step.argtypes = [vp, i64, dbl, dbl, dbl, P(StepStats)]
+        L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
         L.rimu_engine_set_timing.argtypes = [vp, i32]
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
         L.rimu_engine_last_traffic.argtypes = [vp, P(dbl), P(dbl)]
@@ -241,6 +244,10 @@ class Engine:
                 continue
             check(rc)
             return st
+
+    def set_initiator(self, rule: int, threshold: float = 1.0):
+        """Initiator rule of the vector (INITIATOR_* codes) and its threshold."""
+        check(lib().rimu_engine_set_initiator(self._h, int(rule), float(threshold)))
 
     def set_timing(self, enabled: bool):
         check(lib().rimu_engine_set_timing(self._h, 1 if enabled else 0))

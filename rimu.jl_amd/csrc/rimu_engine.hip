@@ -81,6 +81,8 @@ struct rimu_engine {
     bool int_vals = false;
     size_t vsize = 8;
     int64_t max_entries = 0, max_spawns = 0;
+    int32_t init_rule = 0;          // RIMU_INITIATOR_*
+    double init_threshold = 0;
 
     // vector (double buffered)
     DevBuf<uint64_t> vkeys[2];
@@ -163,6 +165,8 @@ StepArgs make_args(rimu_engine* e, int64_t step, double shift, double dtau, doub
     a.proj_threshold = e->style.proj_threshold;
     a.rel_threshold = e->style.rel_threshold;
     a.abs_threshold = e->style.abs_threshold;
+    a.init_rule = e->init_rule;
+    a.init_threshold = e->init_threshold;
     return a;
 }
 
@@ -188,8 +192,9 @@ int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
 }
 
 // Stable LSD radix sort of n (key, value) records on key bits [0, nbits), skipping passes
-// whose digit is the same for every key.  Input in buffer `in`; returns the buffer index
-// holding the result.
+// whose digit is the same for every key.  Bits at and above nbits are carried, not sorted
+// (the initiator class bit).  Input in buffer `in`; returns the buffer index holding the
+// result.
 template <class V>
 int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, int* result) {
     int src = in;
@@ -200,9 +205,10 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
     HIP_TRY(e->rchunks.alloc((size_t)nchunks * RADIX));
     for (int shift = 0; shift < nbits; shift += RADIX_BITS) {
         if (((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
+        const uint32_t dmask = shift + RADIX_BITS > nbits ? (1u << (nbits - shift)) - 1u : (uint32_t)(RADIX - 1);
         hipEvent_t ev{};
         e->timer.begin("radix_count", e->stream, &ev);
-        radix_count_kernel<<<tiles, NT, 0, e->stream>>>(e->skeys[src].p, n, shift, e->rcounts.p, tiles);
+        radix_count_kernel<<<tiles, NT, 0, e->stream>>>(e->skeys[src].p, n, shift, dmask, e->rcounts.p, tiles);
         e->timer.end(e->stream);
         e->timer.begin("radix_scan", e->stream, &ev);
         radix_chunk_sums_kernel<<<nchunks, RADIX, 0, e->stream>>>(e->rcounts.p, tiles, nchunks, e->rchunks.p);
@@ -211,7 +217,7 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
         e->timer.end(e->stream);
         e->timer.begin("radix_scatter", e->stream, &ev);
         radix_scatter_kernel<V><<<tiles, NT, 0, e->stream>>>(
-            e->skeys[src].p, (const V*)e->svals[src].p, n, shift, e->rcounts.p, tiles,
+            e->skeys[src].p, (const V*)e->svals[src].p, n, shift, dmask, e->rcounts.p, tiles,
             e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
@@ -224,11 +230,14 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
 // Merge the sorted spawn records (ak, av; S) with the vector keys and diagonal values
 // (bk, bv; L) into (ok, ov) of capacity cap: tile splits, merge into per-tile slots, scan
 // of the tile counts, compaction.  The new length lands in dm->L_new; `tiles` receives the
-// number of merge tiles (rows of merge_p).
+// number of merge tiles (rows of merge_p).  With an initiator rule (a.init_rule != 0) the
+// spawn keys carry the class bit and bo holds the vector's values before the step.
 template <class V>
 int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
                const uint64_t* bk, const V* bv, int64_t L, uint64_t* ok, V* ov, int64_t cap,
-               rimu_engine::Misc* dm, int64_t* tiles) {
+               rimu_engine::Misc* dm, int64_t* tiles, const V* bo = nullptr) {
+    const bool init = a.init_rule != 0;
+    if (init && !bo) return fail(RIMU_ERR_INTERNAL, "initiator merge without the old values");
     constexpr int64_t TILE = NT * MERGE_IPT;
     const int64_t nt = std::max<int64_t>(ceil_div(S + L, TILE), 1);
     *tiles = nt;
@@ -239,12 +248,19 @@ int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* a
     HIP_TRY(e->slot_k.alloc((size_t)nt * TILE));
     HIP_TRY(e->slot_v.alloc((size_t)nt * TILE));
     HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nt, 0, sizeof(int64_t), e->stream));
-    merge_split_kernel<<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(ak, S, bk, L, TILE, nt, e->splits.p,
-                                                                    e->prevk.p);
+    merge_split_kernel<<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(ak, S, bk, L, TILE, nt,
+                                                                    init ? ~CLASS_BIT : ~0ull,
+                                                                    e->splits.p, e->prevk.p);
     hipEvent_t ev{};
     e->timer.begin("merge", e->stream, &ev);
-    merge_kernel<V><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p, (V*)e->slot_v.p,
-                                              e->tile_cnt.p, e->merge_p.p, e->splits.p, e->prevk.p);
+    if (init)
+        merge_kernel<V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p,
+                                                        (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
+                                                        e->splits.p, e->prevk.p, bo);
+    else
+        merge_kernel<V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p,
+                                                         (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
+                                                         e->splits.p, e->prevk.p, nullptr);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
@@ -364,7 +380,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     int64_t merge_tiles = 0;
     if (int rc = merge_into<V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
                                (const V*)e->dvals.p, L, e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
-                               e->max_entries, dm, &merge_tiles)) return rc;
+                               e->max_entries, dm, &merge_tiles, vv)) return rc;
 
     // 5. projections and statistics
     if (e->np + e->nh > 0) {
@@ -602,6 +618,18 @@ int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     if (e->len > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
     return e->int_vals ? step_impl<int64_t>(e, step, shift, dtau, boost, stats)
                        : step_impl<double>(e, step, shift, dtau, boost, stats);
+}
+
+int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (rule < RIMU_INITIATOR_NONE || rule > RIMU_INITIATOR_COHERENT)
+        return fail(RIMU_ERR_ARG, "unknown initiator rule");
+    if (std::isnan(threshold)) return fail(RIMU_ERR_ARG, "initiator threshold is NaN");
+    if (rule != RIMU_INITIATOR_NONE && e->hdev.B > 63)
+        return fail(RIMU_ERR_ARG, "initiator rules need N + M - 1 <= 63 (key bit 63 holds the class)");
+    e->init_rule = rule;
+    e->init_threshold = rule == RIMU_INITIATOR_NONE ? 0.0 : threshold;
+    return RIMU_OK;
 }
 
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled) {

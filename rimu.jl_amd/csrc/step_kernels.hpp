@@ -37,7 +37,20 @@ struct StepArgs {
     int style;        // 1 integer, 2 deterministic, 3 dynamic semistochastic
     int compression;  // 0 none, 1 threshold
     double comp_threshold, proj_threshold, rel_threshold, abs_threshold;
+    int init_rule;    // 0 NonInitiator, 1 Initiator, 2 SimpleInitiator, 3 CoherentInitiator
+    double init_threshold;
 };
+
+// Initiator class of a record, carried in key bit 63 (addresses then have B <= 63 bits).
+// The bit marks a deposit whose parent p_val has |p_val| <= threshold, i.e. is not an
+// initiator (to_initiator_value, initiators.jl:143-157): on a spawn record it selects the
+// `unsafe` component (else `safe`); on the diagonal deposit of an entry, staged in the
+// merge, it selects `safe` (else `initiator`).  Radix passes and merge comparisons ignore it.
+constexpr uint64_t CLASS_BIT = 1ull << 63;
+
+__device__ inline bool below_initiator_threshold(const StepArgs& a, double v) {
+    return !(fabs(v) > a.init_threshold);
+}
 
 template <class V> struct is_int_v { static constexpr bool value = false; };
 template <> struct is_int_v<int64_t> { static constexpr bool value = true; };
@@ -424,6 +437,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
             if (ok) { kor |= nb; kand &= nb; }
         }
         rk[it] = nb;
+        if (a.init_rule != 0 && att < A && below_initiator_threshold(a, (double)eval[it])) rk[it] |= CLASS_BIT;
         rv[it] = nv;
         rok[it] = ok;
         if (!DENSE) {
@@ -490,9 +504,11 @@ __global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restric
 
 // Merge-path split of every merge tile boundary d = t*TILE (clamped to S+L): number of A
 // elements among the first d merged elements (B before A on equal keys).
+// kmask clears the initiator class bit of the spawn keys (all ones without initiators).
 __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restrict__ ak, int64_t S,
                                                          const uint64_t* __restrict__ bk, int64_t L,
                                                          int64_t tile_size, int64_t ntiles,
+                                                         uint64_t kmask,
                                                          int64_t* __restrict__ splits,
                                                          uint64_t* __restrict__ prevk) {
     const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -501,12 +517,12 @@ __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restr
     int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if (ak[mid] < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
+        if ((ak[mid] & kmask) < bk[d - mid - 1]) lo = mid + 1; else hi = mid;
     }
     splits[t] = lo;
     // key of the merged element just before the tile (read only when the tile starts past 0)
     const int64_t bi = d - lo;
-    const uint64_t pa = lo > 0 ? ak[lo - 1] : 0;
+    const uint64_t pa = lo > 0 ? (ak[lo - 1] & kmask) : 0;
     const uint64_t pb = bi > 0 ? bk[bi - 1] : 0;
     prevk[t] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
 }
@@ -551,8 +567,11 @@ __global__ __launch_bounds__(1024) void key_bits_kernel(const uint64_t* __restri
 // ranks within the tile from wavefront ballots, staged in LDS in digit order, then
 // written to the global digit offsets).
 // ---------------------------------------------------------------------------------------
+// dmask selects the digit bits of the pass: RADIX - 1, or fewer bits for a top digit that
+// would reach the initiator class bit.
 __global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restrict__ keys, int64_t n,
-                                                         int shift, uint32_t* __restrict__ counts,
+                                                         int shift, uint32_t dmask,
+                                                         uint32_t* __restrict__ counts,
                                                          int64_t num_tiles) {
     __shared__ uint32_t hist[RADIX];
     hist[threadIdx.x] = 0;
@@ -567,7 +586,7 @@ __global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restr
 #pragma unroll
     for (int r = 0; r < SORT_IPT; ++r) {
         const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
-        if (idx < n) atomicAdd(&hist[(k[r] >> shift) & (RADIX - 1)], 1u);
+        if (idx < n) atomicAdd(&hist[(uint32_t)(k[r] >> shift) & dmask], 1u);
     }
     __syncthreads();
     counts[(int64_t)blockIdx.x * RADIX + threadIdx.x] = hist[threadIdx.x];  // tile-major row
@@ -616,7 +635,8 @@ __global__ __launch_bounds__(RADIX) void radix_chunk_offsets_kernel(uint32_t* __
 template <class V>
 __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __restrict__ keys_in,
                                                            const V* __restrict__ vals_in, int64_t n,
-                                                           int shift, const uint32_t* __restrict__ offs,
+                                                           int shift, uint32_t dmask,
+                                                           const uint32_t* __restrict__ offs,
                                                            int64_t num_tiles,
                                                            uint64_t* __restrict__ keys_out,
                                                            V* __restrict__ vals_out) {
@@ -655,7 +675,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     for (int r = 0; r < SORT_IPT; ++r) {
         const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         const bool valid = idx < n;
-        const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
+        const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < RADIX_BITS; ++b) {
@@ -689,7 +709,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     for (int r = 0; r < SORT_IPT; ++r) {
         const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         if (idx < n) {
-            const uint32_t digit = (uint32_t)((k[r] >> shift) & (RADIX - 1));
+            const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
             const uint32_t pos = s_start[digit] + s_wcnt[wid][digit] + rank[r];
             s_k[pos] = k[r];
             s_v[pos] = v[r];
@@ -699,7 +719,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     const int64_t cnt_tile = min((int64_t)SORT_TILE, n - base);
     for (int i = threadIdx.x; i < cnt_tile; i += NT) {
         const uint64_t kk = s_k[i];
-        const uint32_t digit = (uint32_t)((kk >> shift) & (RADIX - 1));
+        const uint32_t digit = (uint32_t)(kk >> shift) & dmask;
         const int64_t g = (int64_t)s_goff[digit] + (i - (int64_t)s_start[digit]);
         keys_out[g] = kk;
         vals_out[g] = s_v[i];
@@ -721,7 +741,22 @@ constexpr int MERGE_EXTRA = 64;  // spawn records staged past the tile for runs 
 static_assert(MERGE_IPT <= 32, "merge candidate masks are 32-bit");
 static_assert(MERGE_EXTRA <= NT, "one extra record per thread at most");
 
+// from_initiator_value of the rule for the components (safe, unsafe, initiator):
+// Initiator initiators.jl:138-140, SimpleInitiator :180-182, CoherentInitiator :208-214.
 template <class V>
+__device__ inline V initiator_value(const StepArgs& a, V s, V u, V i) {
+    if (a.init_rule == 2) return s + i;
+    if (a.init_rule == 3) return (i != (V)0 || fabs((double)u) > a.init_threshold) ? (i + s) + u : i + s;
+    return (s + i) + (i != (V)0 ? u : (V)0);
+}
+
+// INIT: the initiator approximation.  Spawn keys carry CLASS_BIT; the vector keys are
+// staged with CLASS_BIT set where the old value bo[i] is not above the threshold.  Each
+// address then folds three components (safe, unsafe, initiator), each a left fold of its
+// deposits in list order; an address exists when a component is nonzero (deposit! deletes
+// all-zero entries, pdworkingmemory.jl:21-31) and takes the rule's value, which
+// NoCompression keeps even when it is zero (stochasticstyles.jl:116-121).
+template <class V, bool INIT>
 __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
                                                    const V* __restrict__ av, int64_t S,
                                                    const uint64_t* __restrict__ bk,
@@ -730,8 +765,10 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                                                    int64_t* __restrict__ tile_cnt,
                                                    double* __restrict__ partials,
                                                    const int64_t* __restrict__ splits,
-                                                   const uint64_t* __restrict__ prevk) {
+                                                   const uint64_t* __restrict__ prevk,
+                                                   const V* __restrict__ bo) {
     constexpr int TILE = NT * MERGE_IPT;
+    constexpr uint64_t KM = INIT ? ~CLASS_BIT : ~0ull;
     // the tile's A range, its B range, then up to MERGE_EXTRA records of A past the tile
     __shared__ uint64_t s_key[TILE + MERGE_EXTRA];
     __shared__ V s_val[TILE + MERGE_EXTRA];
@@ -759,6 +796,9 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                 const V* vp = in_a ? av + (a0 + i) : bv + (b0 + (i - na));
                 rk[r] = *kp;
                 rv[r] = *vp;
+                if constexpr (INIT) {
+                    if (!in_a && below_initiator_threshold(a, (double)bo[b0 + (i - na)])) rk[r] |= CLASS_BIT;
+                }
             }
         }
         uint64_t ek = 0;
@@ -784,7 +824,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         int lo = max(0, dd - nb), hi = min(dd, na);
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (s_a[mid] < s_b[dd - mid - 1]) lo = mid + 1; else hi = mid;
+            if ((s_a[mid] & KM) < (s_b[dd - mid - 1] & KM)) lo = mid + 1; else hi = mid;
         }
         ia = lo;
     }
@@ -794,8 +834,8 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     uint64_t prev = 0;
     if (dd > 0) {
         have_prev = true;
-        const uint64_t pa = ia > 0 ? s_a[ia - 1] : 0;
-        const uint64_t pb = ib > 0 ? s_b[ib - 1] : 0;
+        const uint64_t pa = ia > 0 ? (s_a[ia - 1] & KM) : 0;
+        const uint64_t pb = ib > 0 ? (s_b[ib - 1] & KM) : 0;
         prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
     } else if (d0 > 0) {
         have_prev = true;
@@ -813,9 +853,9 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         ck[j] = 0;
         cv[j] = (V)0;
         if (j < n_here) {
-            const bool take_b = ib < nb && (ia >= na || s_b[ib] <= s_a[ia]);
-            const uint64_t key = take_b ? s_b[ib] : s_a[ia];
-            if (!have_prev || key != prev) {
+            const bool take_b = ib < nb && (ia >= na || (s_b[ib] & KM) <= (s_a[ia] & KM));
+            const uint64_t key = (take_b ? s_b[ib] : s_a[ia]) & KM;
+            if ((!have_prev || key != prev) && !INIT) {
                 // segment head: fold diag (if any) then the run of spawns in A
                 V acc = (V)0;
                 if (take_b) acc = acc + s_val[na + ib];
@@ -834,6 +874,39 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                     ck[j] = key;
                     cv[j] = acc;
                     cvalid |= 1u << j;
+                }
+            } else if ((!have_prev || key != prev) && INIT) {
+                // the same walk with three components
+                V cs = (V)0, cu = (V)0, ci = (V)0;
+                auto dep = [&](uint64_t rawk, V x) {
+                    if (rawk & CLASS_BIT) cu = cu + x; else cs = cs + x;
+                };
+                if (take_b) {
+                    const V d = s_val[na + ib];
+                    if (s_b[ib] & CLASS_BIT) cs = cs + d; else ci = ci + d;
+                }
+                int la = ia;
+                while (la < na && (s_a[la] & KM) == key) { dep(s_a[la], s_val[la]); ++la; }
+                if (la == na) {
+                    int x = 0;
+                    while (x < ne && (s_key[na + nb + x] & KM) == key) { dep(s_key[na + nb + x], s_val[na + nb + x]); ++x; }
+                    if (x == ne) {
+                        for (int64_t ga = a1 + ne; ga < S; ++ga) {
+                            const uint64_t rk = ak[ga];
+                            if ((rk & KM) != key) break;
+                            dep(rk, av[ga]);
+                        }
+                    }
+                }
+                if (cs != (V)0 || cu != (V)0 || ci != (V)0) {
+                    lb += 1;
+                    const V val = initiator_value<V>(a, cs, cu, ci);
+                    const bool keep_zero = is_int_v<V>::value || a.compression != 1;
+                    if (val != (V)0 || keep_zero) {
+                        ck[j] = key;
+                        cv[j] = val;
+                        cvalid |= 1u << j;
+                    }
                 }
             }
             prev = key;

@@ -4,18 +4,24 @@
 References: ``src/DictVectors/dvec.jl:44-136`` (DVec, zero deletes the entry),
 ``src/Interfaces/dictvectors.jl:40-51`` (``AbstractDVec``, ``deposit!``),
 ``src/DictVectors/abstractdvec.jl:258-270`` (``walkernumber``, ``walkernumber_and_length``),
-``src/DictVectors/projectors.jl:140-173`` (``FrozenDVec`` and its ``dot``).
+``src/DictVectors/projectors.jl:140-173`` (``FrozenDVec`` and its ``dot``),
+``src/DictVectors/pdvec.jl:156-200`` and ``src/DictVectors/initiatordvec.jl:42-110``
+(the ``initiator`` keyword of ``PDVec`` and ``InitiatorDVec``).
 """
 from __future__ import annotations
 
 import numpy as np
 
 from .addresses import BoseFS
+from .initiators import Initiator, InitiatorRule, NonInitiator, initiator_rule
 from .styles import IsStochasticInteger, StochasticStyle, default_style
 
 
 class DVec:
-    """Dictionary-backed sparse vector (dvec.jl:44).  Setting a value to zero deletes it."""
+    """Dictionary-backed sparse vector (dvec.jl:44).  Setting a value to zero deletes it.
+    ``initiator`` is the vector's initiator rule (NonInitiator for a DVec)."""
+
+    initiator: InitiatorRule = NonInitiator()
 
     def __init__(self, *pairs, style: StochasticStyle | None = None, capacity: int = 0):
         self.storage = {}
@@ -69,7 +75,9 @@ class DVec:
         return float(sum(abs(v) for v in self.storage.values()))
 
     def copy(self):
-        return DVec(dict(self.storage), style=self.style)
+        out = DVec(dict(self.storage), style=self.style)
+        out.initiator = self.initiator
+        return out
 
     def to_arrays(self):
         ks = sorted(self.storage)
@@ -77,7 +85,29 @@ class DVec:
                 np.array([self.storage[k] for k in ks]))
 
     def __repr__(self):
-        return f"DVec with {len(self)} entries, style = {self.style}"
+        return f"{type(self).__name__} with {len(self)} entries, style = {self.style}, initiator = {self.initiator}"
+
+
+class PDVec(DVec):
+    """``PDVec(pairs...; style, initiator, initiator_threshold)`` (pdvec.jl:156-200): the
+    host container of a start vector with an initiator rule; ``initiator`` is ``False``
+    (default), ``True`` (``Initiator(initiator_threshold)``, threshold 1 when 0) or a rule."""
+
+    def __init__(self, *pairs, style: StochasticStyle | None = None, initiator=None,
+                 initiator_threshold=0, capacity: int = 0):
+        super().__init__(*pairs, style=style, capacity=capacity)
+        self.initiator = initiator_rule(initiator, initiator_threshold)
+
+
+class InitiatorDVec(DVec):
+    """``InitiatorDVec(pairs...; style, initiator, initiator_threshold)``
+    (initiatordvec.jl:42-110): default rule ``Initiator(initiator_threshold)`` with
+    threshold 1."""
+
+    def __init__(self, *pairs, style: StochasticStyle | None = None, initiator=None,
+                 initiator_threshold=1, capacity: int = 0):
+        super().__init__(*pairs, style=style, capacity=capacity)
+        self.initiator = Initiator(initiator_threshold) if initiator is None else initiator
 
 
 class FrozenDVec:
@@ -106,10 +136,11 @@ class DeviceDVec:
     ``_native.Engine`` in device memory.  Reading it (``pairs``, ``getindex``) copies to
     the host."""
 
-    def __init__(self, engine, hamiltonian, style):
+    def __init__(self, engine, hamiltonian, style, initiator: InitiatorRule = NonInitiator()):
         self.engine = engine
         self.hamiltonian = hamiltonian
         self.style = style
+        self.initiator = initiator
         self._norm = None
 
     def __len__(self):
@@ -146,4 +177,4 @@ class DeviceDVec:
         return 0
 
     def __repr__(self):
-        return f"DeviceDVec with {len(self)} entries, style = {self.style}"
+        return f"DeviceDVec with {len(self)} entries, style = {self.style}, initiator = {self.initiator}"

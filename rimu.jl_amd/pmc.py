@@ -17,8 +17,9 @@ import numpy as np
 
 from . import _native
 from .addresses import BoseFS
-from .dictvectors import DVec, DeviceDVec, FrozenDVec
+from .dictvectors import DVec, DeviceDVec, FrozenDVec, PDVec
 from .hamiltonians import AbstractHamiltonian
+from .initiators import Initiator, InitiatorRule, NonInitiator
 from .strategies import (ConstantTimeStep, DefaultShiftParameters, DoubleLogUpdate,
                          ProjectedEnergy)
 from .styles import IsDynamicSemistochastic, IsStochasticInteger, StochasticStyle
@@ -35,8 +36,14 @@ class ProjectorMonteCarloProblem:
                  post_step_strategy=(), max_length: int | None = None, metadata=None,
                  display_name: str = "PMCSimulation", random_seed=True, n_replicas: int = 1,
                  reporting_interval: int = 1):
-        if initiator not in (False, None):
-            raise NotImplementedError("initiator rules are not part of this engine")
+        # projector_monte_carlo_problem.jl:221-223
+        if initiator is None or initiator is False:
+            initiator = NonInitiator()
+        elif initiator is True:
+            initiator = Initiator()
+        elif not isinstance(initiator, InitiatorRule):
+            raise ValueError(f"Invalid initiator {initiator!r}")
+        self.initiator = initiator
         if n_replicas != 1:
             raise NotImplementedError("one replica per problem")
         self.hamiltonian = hamiltonian
@@ -73,22 +80,25 @@ class ProjectorMonteCarloProblem:
                 f"shift_strategy={self.shift_strategy}, max_length={self.max_length})")
 
 
-def default_starting_vector(address: BoseFS, style=None, population=10) -> DVec:
-    """``default_starting_vector`` (qmc_states.jl:229-260): address => population."""
+def default_starting_vector(address: BoseFS, style=None, population=10,
+                            initiator: InitiatorRule | None = None) -> DVec:
+    """``default_starting_vector`` (qmc_states.jl:229-260): ``PDVec(address =>
+    population; style, initiator)``."""
     style = style or IsDynamicSemistochastic()
-    return DVec((address, population), style=style)
+    return PDVec((address, population), style=style, initiator=initiator or NonInitiator())
 
 
 def _start_vector(problem: ProjectorMonteCarloProblem) -> DVec:
     """``_set_up_starting_vectors`` (pmc_simulation.jl:33-85): an address or a frozen
-    vector gets the problem's style; a ``DVec`` is copied with its own style."""
+    vector gets the problem's style and initiator rule; a ``DVec`` is copied with its own
+    style and rule."""
     s = problem.start_at
     if isinstance(s, BoseFS):
-        return default_starting_vector(s, problem.style)
+        return default_starting_vector(s, problem.style, initiator=problem.initiator)
     if isinstance(s, FrozenDVec):
-        return DVec(list(s.pairs), style=problem.style)
+        return PDVec(list(s.pairs), style=problem.style, initiator=problem.initiator)
     if isinstance(s, DVec):
-        return DVec(dict(s.storage), style=s.style)
+        return s.copy()
     raise TypeError("start_at must be a BoseFS, a FrozenDVec or a DVec")
 
 
@@ -126,9 +136,11 @@ class PMCSimulation:
         self.style = v0.style
         self.engine = _native.Engine(h.descriptor(), self.style.descriptor(), seed,
                                      max_entries, max_spawns)
+        self.initiator = v0.initiator
+        self.engine.set_initiator(*self.initiator.descriptor())
         keys, vals = v0.to_arrays()
         self.engine.upload(keys, vals)
-        self.v = DeviceDVec(self.engine, h, self.style)
+        self.v = DeviceDVec(self.engine, h, self.style, self.initiator)
         # initial shift: Rayleigh quotient of the start vector unless given
         shift = problem.shift
         if shift is None:
@@ -259,12 +271,14 @@ def solve(problem: ProjectorMonteCarloProblem) -> PMCSimulation:
 
 def lomc(ham: AbstractHamiltonian, v=None, *, style=None, targetwalkers=1000, s_strat=None,
          laststep=100, dtau=0.01, shift=None, address=None, post_step=(), maxlength=None,
-         step=0, random_seed=None):
+         step=0, random_seed=None, initiator: InitiatorRule | None = None):
     """``lomc!`` (lomc.jl:95-195): default style ``IsStochasticInteger``, default shift the
-    diagonal element of the starting address, no re-seeding.  Returns (df, simulation)."""
+    diagonal element of the starting address, ``initiator`` the rule of the default start
+    vector (unused when ``v`` is given), no re-seeding.  Returns (df, simulation)."""
     address = ham.starting_address() if address is None else address
     if v is None:
-        v = default_starting_vector(address, style or IsStochasticInteger())
+        v = default_starting_vector(address, style or IsStochasticInteger(),
+                                    initiator=initiator or NonInitiator())
     style = v.style if style is None else style
     if s_strat is None:
         s_strat = DoubleLogUpdate(target_walkers=targetwalkers)
