@@ -22,6 +22,13 @@ Two more objects on the same line:
   differences on the plain engine.
 * "momentum": the momentum-space case, HubbardMom1D M = N = 16, u = 4, integer walkers,
   2e7 walkers, dtau = 5e-4: spawn attempts/s and per-kernel times at steady state.
+* "initiator": the initiator approximation.  (a) Rimu's benchmark case "(10, 20) Mom space
+  with projected energy and initiator" (benchmark/benchmarks.jl:58-66): HubbardMom1D with
+  10 bosons in mode 10 of 20, u = 1, PDVec(addr => 1.0; IsDynamicSemistochastic,
+  initiator = true), ProjectedEnergy, DoubleLogUpdate(target_walkers = 40_000),
+  dtau = 1e-4, 8000 steps through lomc, timed whole; steady-state step times of the engine
+  and of the oracle's initiator step (one core) from its final vector.  (b) The headline
+  state (1e8 walkers) stepped with Initiator(1): per-kernel times next to the plain step.
 """
 from __future__ import annotations
 
@@ -255,6 +262,104 @@ def run_momentum_leg(rimu, args):
             "shift": sp.shift, "kernels": kern}
 
 
+def run_initiator_case(rimu, args):
+    """benchmark/benchmarks.jl:58-66 through lomc, then steady-state step times."""
+    M, N = 20, 10
+    onr = [0] * M
+    onr[9] = N
+    addr = rimu.BoseFS(*onr)
+    H = rimu.HubbardMom1D(addr, u=1.0)
+    dv = rimu.PDVec((addr, 1.0), style=rimu.IsDynamicSemistochastic(), initiator=True)
+    post = rimu.ProjectedEnergy(H, dv)
+    s_strat = rimu.DoubleLogUpdate(target_walkers=40_000)
+    dtau, last = 1e-4, args.init_laststep
+    t0 = time.perf_counter()
+    df, sim = rimu.lomc(H, dv, s_strat=s_strat, post_step=post, dtau=dtau, laststep=last,
+                        random_seed=17)
+    run_s = time.perf_counter() - t0
+    shift = float(df["shift"].to_numpy()[-1])
+    half = len(df) // 2
+    e_shift = float(np.mean(df["shift"].to_numpy()[half:]))
+    e_proj = float(np.sum(df["hproj"].to_numpy()[half:]) / np.sum(df["vproj"].to_numpy()[half:]))
+    keys, vals = sim.engine.download()
+    eng = sim.engine
+    eng.set_timing(True)
+    n_gpu = args.init_gpu_steps
+    t1 = time.perf_counter()
+    for i in range(n_gpu):
+        st = eng.step(last + 1 + i, shift, dtau)
+    gpu_step = (time.perf_counter() - t1) / n_gpu
+    times = eng.kernel_times()
+    sim.close()
+    # the oracle's initiator step on one core from the same vector, bounded in time
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    model = O.mom1d(M, N, 1.0, 1.0)
+    sty = O.style("dynamic")
+    k, v = keys, vals
+    n_cpu = 0
+    t2 = time.perf_counter()
+    while n_cpu < 2000 and time.perf_counter() - t2 < args.init_cpu_seconds:
+        k, v, _ = O.step_initiator(model, sty, k, v, rule="initiator", threshold=1.0, seed=17,
+                                   step=last + 1 + n_cpu, shift=shift, dtau=dtau, order=0)
+        n_cpu += 1
+    cpu_step = (time.perf_counter() - t2) / max(n_cpu, 1)
+    return {"workload": "HubbardMom1D M=20 N=10 (10 bosons in mode 10) u=1, IsDynamicSemistochastic, "
+                        "Initiator(1), ProjectedEnergy, DoubleLogUpdate target 40000, dtau 1e-4",
+            "steps": last, "seconds": run_s, "steps_per_s": last / run_s,
+            "walkers": float(df["norm"].to_numpy()[-1]), "len": int(df["len"].to_numpy()[-1]),
+            "shift_mean_second_half": e_shift, "projected_energy_second_half": e_proj,
+            "gpu_ms_per_step_steady": 1e3 * gpu_step,
+            "gpu_kernels_ms_per_step": {n: ms / n_gpu for n, (ms, c) in times.items()},
+            "cpu_baseline": {"ms_per_step": 1e3 * cpu_step, "cores": 1, "kind": "port",
+                             "sample": f"{n_cpu} oracle initiator steps from the final vector "
+                                       f"({len(keys)} entries), order of the reference's columns"},
+            "cpu_over_gpu_steady": cpu_step / gpu_step}
+
+
+def run_initiator_headline(rimu, keys, vals, shift, step0, args):
+    """The equilibrated 1e8-walker HubbardReal1D 20x20 state stepped with Initiator(1)."""
+    M = N = 20
+    H = rimu.HubbardReal1D(rimu.near_uniform(N, M), u=6.0, t=1.0)
+    style = rimu.IsDynamicSemistochastic()
+    target = args.target
+    eng = rimu._native.Engine(H.descriptor(), style.descriptor(), 17, int(round(2 * target + 100)) + 1,
+                              int(min(4 * target + 4096, 2 ** 32 - 2)))
+    eng.set_initiator(rimu._native.INITIATOR_DEFAULT, 1.0)
+    eng.upload(keys, vals)
+    dtau = 1e-3
+    for i in range(3):
+        eng.step(step0 + i, shift, dtau)
+    eng.set_timing(True)
+    K = args.init_steps
+    S_tot = L_tot = Lnew_tot = 0.0
+    L_prev = len(eng)
+    t0 = time.perf_counter()
+    for i in range(K):
+        st = eng.step(step0 + 3 + i, shift, dtau)
+        b, records = eng.last_traffic()
+        S_tot += records - L_prev
+        L_tot += L_prev
+        Lnew_tot += st.len
+        L_prev = st.len
+    elapsed = time.perf_counter() - t0
+    times = eng.kernel_times()
+    eng.close()
+    S, L, Ln = S_tot / K, L_tot / K, Lnew_tot / K
+    kern = {}
+    for name, (ms, c) in times.items():
+        kern[name] = {"ms_avg": ms / max(c, 1), "launches_per_step": c / K}
+        bpl = kernel_alg_bytes(name, S, L, Ln)
+        if name == "merge":
+            bpl += L * 8   # the old values, read for the diagonal deposit's class
+        if bpl is not None:
+            kern[name]["alg_GBps"] = bpl / (kern[name]["ms_avg"] / 1e3) / 1e9
+    return {"workload": "HubbardReal1D M=N=20 u=6 equilibrated plain state (target 1e8), "
+                        "stepped with Initiator(1), dtau 1e-3, fixed shift",
+            "ms_per_step": 1e3 * elapsed / K, "steps": K, "len": st.len, "walkers": st.norm,
+            "spawn_records_per_step": S, "kernels": kern}
+
+
 def run_adjoint_leg(rimu, args):
     """The differentiable projector: HubbardReal1D M = N = 12 (dimension 1,352,078),
     u in {2,4,6,8} as four right-hand sides of one product, K = 2000 steps of dtau 1e-3,
@@ -329,6 +434,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-details", default=None, help="write the per-run energy check to this JSON file")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-initiator", action="store_true")
+    ap.add_argument("--init-steps", type=int, default=10)
+    ap.add_argument("--init-laststep", type=int, default=8000,
+                    help="steps of the initiator benchmark case (0 skips it)")
+    ap.add_argument("--init-gpu-steps", type=int, default=200)
+    ap.add_argument("--init-cpu-seconds", type=float, default=10.0)
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -446,7 +557,15 @@ def main():
         log(f"cpu baseline: {cpu['value']:.4g} attempts/s on {cpu['cores']} threads ({cpu['sample']})")
 
     final_norm, final_len = st.norm, st.len
-    eng.close()
+    init_headline = None
+    if not args.no_initiator:
+        keys, vals = eng.download()
+        eng.close()
+        init_headline = run_initiator_headline(rimu, keys, vals, sp.shift, step + 1, args)
+        del keys, vals
+        log(f"initiator headline: {init_headline['ms_per_step']:.2f} ms/step")
+    else:
+        eng.close()
     batched = None if args.no_batched else run_batched_leg(rimu, args)
     if batched:
         log(f"batched: {batched['value']:.3g} solves/s vs plain {batched['plain_value']:.3g} "
@@ -454,6 +573,14 @@ def main():
     momentum = None if args.no_momentum else run_momentum_leg(rimu, args)
     if momentum:
         log(f"momentum: {momentum['value']:.3g} attempts/s, {momentum['ms_per_step']:.2f} ms/step")
+    initiator = None
+    if not args.no_initiator and args.init_laststep > 0:
+        initiator = run_initiator_case(rimu, args)
+        log(f"initiator case: {initiator['seconds']:.1f} s for {initiator['steps']} steps, steady "
+            f"{initiator['gpu_ms_per_step_steady']:.3f} ms/step GPU vs "
+            f"{initiator['cpu_baseline']['ms_per_step']:.1f} ms/step CPU")
+    if init_headline is not None:
+        initiator = dict(initiator or {}, headline=init_headline)
     adjoint = None if args.no_adjoint else run_adjoint_leg(rimu, args)
     if adjoint:
         log(f"adjoint: {adjoint['value']:.3g} grads/s vs plain {adjoint['plain_value']:.3g} "
@@ -485,6 +612,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "momentum": momentum,
+        "initiator": initiator,
     }
     print(json.dumps(out), flush=True)
 
