@@ -29,6 +29,8 @@ Two more objects on the same line:
   dtau = 1e-4, 8000 steps through lomc, timed whole; steady-state step times of the engine
   and of the oracle's initiator step (one core) from its final vector.  (b) The headline
   state (1e8 walkers) stepped with Initiator(1): per-kernel times next to the plain step.
+* "checkpoint": save_state / load_state of the headline state as a Rimu Arrow state file
+  (download + zstd IPC write; read + upload), in GB/s of state.
 """
 from __future__ import annotations
 
@@ -262,6 +264,41 @@ def run_momentum_leg(rimu, args):
             "shift": sp.shift, "kernels": kern}
 
 
+def run_checkpoint_leg(rimu, eng, M, N, step, shift):
+    """save_state / load_state of the equilibrated headline state (RimuIO.jl:92-180): device
+    download, Arrow table with the BoseFS extension type, zstd IPC file; then read back,
+    compare and upload into the engine."""
+    import shutil
+    import tempfile
+    import pyarrow.ipc as ipc
+    from rimu_amd import rimuio
+    d = tempfile.mkdtemp(prefix="rimu_ckpt_")
+    path = os.path.join(d, "state.arrow")
+    try:
+        t0 = time.perf_counter()
+        keys, vals = eng.download()
+        t1 = time.perf_counter()
+        table = rimuio.state_table(M, N, keys, vals, step=step, shift=shift)
+        with ipc.new_file(path, table.schema,
+                          options=ipc.IpcWriteOptions(compression="zstd", use_threads=True)) as w:
+            w.write_table(table)
+        t2 = time.perf_counter()
+        _, _, k2, v2, meta = rimuio.read_state_arrays(path)
+        t3 = time.perf_counter()
+        eng.upload(k2, v2)
+        t4 = time.perf_counter()
+        size = os.path.getsize(path)
+        same = bool(np.array_equal(keys, k2) and np.array_equal(vals.view(np.uint64), v2.view(np.uint64)))
+        del table
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    nbytes = 16.0 * len(keys)
+    return {"entries": len(keys), "state_bytes": nbytes, "file_bytes": size,
+            "download_s": t1 - t0, "write_s": t2 - t1, "read_s": t3 - t2, "upload_s": t4 - t3,
+            "save_GBps": nbytes / (t2 - t0) / 1e9, "load_GBps": nbytes / (t4 - t2) / 1e9,
+            "round_trip_bit_exact": same, "meta_step": meta.get("step")}
+
+
 def run_initiator_case(rimu, args):
     """benchmark/benchmarks.jl:58-66 through lomc, then steady-state step times."""
     M, N = 20, 10
@@ -435,6 +472,7 @@ def main():
     ap.add_argument("--batched-details", default=None, help="write the per-run energy check to this JSON file")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-initiator", action="store_true")
+    ap.add_argument("--no-checkpoint", action="store_true")
     ap.add_argument("--init-steps", type=int, default=10)
     ap.add_argument("--init-laststep", type=int, default=8000,
                     help="steps of the initiator benchmark case (0 skips it)")
@@ -550,6 +588,12 @@ def main():
             if bpl is not None:
                 kernels[name]["alg_GBps"] = bpl / (kernels[name]["ms_avg"] / 1e3) / 1e9
 
+    checkpoint = None
+    if not args.no_checkpoint:
+        checkpoint = run_checkpoint_leg(rimu, eng, M, N, step, sp.shift)
+        log(f"checkpoint: save {checkpoint['save_GBps']:.2f} GB/s, load {checkpoint['load_GBps']:.2f} GB/s, "
+            f"file {checkpoint['file_bytes'] / 1e9:.3f} GB for {checkpoint['state_bytes'] / 1e9:.3f} GB")
+
     cpu = None
     if not args.no_cpu:
         keys, vals = eng.download()
@@ -613,6 +657,7 @@ def main():
         "adjoint": adjoint,
         "momentum": momentum,
         "initiator": initiator,
+        "checkpoint": checkpoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -25,6 +25,8 @@ from .strategies import (ConstantTimeStep, DoubleLogUpdate, DoubleLogUpdateAfter
 from .pmc import (PMCSimulation, ProjectorMonteCarloProblem, default_starting_vector, init,
                   lomc, solve)
 from .dense import DenseHamiltonians, basis_size
+from . import rimuio as RimuIO
+from .rimuio import load_df, load_state, save_df, save_state
 from .batched import BatchedRuns, RunSpec, u_sweep_specs
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -207,10 +207,11 @@ class Engine:
     def upload(self, keys, vals):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         vals = np.ascontiguousarray(vals, dtype=self.vdtype)
-        order = np.argsort(keys, kind="stable")
-        keys, vals = keys[order], vals[order]
-        if len(keys) > 1 and np.any(keys[1:] == keys[:-1]):
-            raise ValueError("duplicate keys in vector")
+        if len(keys) > 1 and not np.all(keys[1:] > keys[:-1]):
+            order = np.argsort(keys, kind="stable")
+            keys, vals = keys[order], vals[order]
+            if np.any(keys[1:] == keys[:-1]):
+                raise ValueError("duplicate keys in vector")
         check(lib().rimu_vector_upload(self._h, _p(keys), _p(vals), len(keys)))
 
     def __len__(self):
