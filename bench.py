@@ -31,6 +31,8 @@ Two more objects on the same line:
   state (1e8 walkers) stepped with Initiator(1): per-kernel times next to the plain step.
 * "checkpoint": save_state / load_state of the headline state as a Rimu Arrow state file
   (download + zstd IPC write; read + upload), in GB/s of state.
+* "overlaps": the AllOverlaps reductions of the headline state v against the state one step
+  later w: dot(v, w), dot(v, G2RealCorrelator(1), w), dot(v, H, w).
 """
 from __future__ import annotations
 
@@ -299,6 +301,49 @@ def run_checkpoint_leg(rimu, eng, M, N, step, shift):
             "round_trip_bit_exact": same, "meta_step": meta.get("step")}
 
 
+def run_overlap_leg(rimu, eng, H, step, shift, args):
+    """AllOverlaps' reductions at the headline scale: the equilibrated state v and the state
+    one step later w (a second engine), dot(v, w), dot(v, G2RealCorrelator(1), w) and
+    dot(v, H, w) on the device (rimu_overlaps), timed with HIP events."""
+    keys, vals = eng.download()
+    target = args.target
+    eng2 = rimu._native.Engine(H.descriptor(), rimu.IsDynamicSemistochastic().descriptor(), 17,
+                               int(round(2 * target + 100)) + 1, int(min(4 * target + 4096, 2 ** 32 - 2)))
+    eng2.upload(keys, vals)
+    eng2.step(step + 1, shift, 1e-3)
+    L1, L2 = len(eng), len(eng2)
+    wk, _ = eng2.download()
+    lookups = float(2 * np.bitwise_count(wk & ~(wk << np.uint64(1))).astype(np.int64).sum())
+    del keys, vals, wk
+    out = {"len_v": L1, "len_w": L2}
+    cases = [("dot", []), ("g2", [rimu.G2RealCorrelator(1)]), ("hamiltonian", [H])]
+    for name, ops in cases:
+        descs = [rimu.observables.operator_descriptor(o, H.M, H.N) for o in ops]
+        rimu._native.overlaps(eng, eng2, descs)           # warm-up
+        eng2.set_timing(True)
+        reps = args.overlap_reps if name != "hamiltonian" else 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            vals = rimu._native.overlaps(eng, eng2, descs)
+        wall = (time.perf_counter() - t0) / reps
+        times = eng2.kernel_times()
+        eng2.set_timing(False)
+        ms, launches = times.get("overlap", (float("nan"), 1))
+        kernel_ms = ms / max(launches, 1)
+        rec = {"values": [float(x) for x in vals], "kernel_ms": kernel_ms, "call_ms": 1e3 * wall}
+        alg = 16.0 * (L1 + L2)   # every entry of both vectors read once
+        if name != "hamiltonian":
+            rec["alg_bytes"] = alg
+            rec["alg_GBps"] = alg / (kernel_ms / 1e3) / 1e9
+            rec["hbm_frac"] = rec["alg_GBps"] * 1e9 / HBM_PEAK
+        else:
+            rec["neighbour_lookups"] = lookups
+            rec["lookups_per_s"] = lookups / (kernel_ms / 1e3)
+        out[name] = rec
+    eng2.close()
+    return out
+
+
 def run_initiator_case(rimu, args):
     """benchmark/benchmarks.jl:58-66 through lomc, then steady-state step times."""
     M, N = 20, 10
@@ -473,6 +518,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-initiator", action="store_true")
     ap.add_argument("--no-checkpoint", action="store_true")
+    ap.add_argument("--no-overlaps", action="store_true")
+    ap.add_argument("--overlap-reps", type=int, default=5)
     ap.add_argument("--init-steps", type=int, default=10)
     ap.add_argument("--init-laststep", type=int, default=8000,
                     help="steps of the initiator benchmark case (0 skips it)")
@@ -594,6 +641,12 @@ def main():
         log(f"checkpoint: save {checkpoint['save_GBps']:.2f} GB/s, load {checkpoint['load_GBps']:.2f} GB/s, "
             f"file {checkpoint['file_bytes'] / 1e9:.3f} GB for {checkpoint['state_bytes'] / 1e9:.3f} GB")
 
+    overlaps = None
+    if not args.no_overlaps:
+        overlaps = run_overlap_leg(rimu, eng, H, step, sp.shift, args)
+        log("overlaps: " + ", ".join(f"{k} {v['kernel_ms']:.3f} ms" for k, v in overlaps.items()
+                                     if isinstance(v, dict)))
+
     cpu = None
     if not args.no_cpu:
         keys, vals = eng.download()
@@ -658,6 +711,7 @@ def main():
         "momentum": momentum,
         "initiator": initiator,
         "checkpoint": checkpoint,
+        "overlaps": overlaps,
     }
     print(json.dumps(out), flush=True)
 

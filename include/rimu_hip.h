@@ -155,6 +155,25 @@ enum {
  * Needs N + M - 1 <= 63.  RIMU_INITIATOR_NONE restores the plain step. */
 int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold);
 
+/* Operators of the three-way dot: a Hamiltonian (model, off-diagonals included),
+ * G2RealCorrelator(d) = (1/M) sum_i n_i (n_{i+d} - delta_{0d}) with |d| < M
+ * (Hamiltonians/correlation_functions.jl:68-92), DensityMatrixDiagonal(mode) = n_mode with
+ * 1 <= mode <= M (Hamiltonians/DensityMatrixDiagonal.jl:19-33). */
+enum { RIMU_OP_HAMILTONIAN = 1, RIMU_OP_G2_REAL = 2, RIMU_OP_DENSITY_DIAGONAL = 3 };
+typedef struct rimu_operator {
+    int32_t kind;
+    int32_t param;      /* G2RealCorrelator: d; DensityMatrixDiagonal: mode */
+    rimu_model model;   /* RIMU_OP_HAMILTONIAN: same M and N as the vectors */
+} rimu_operator;
+
+/* Overlaps of two engines' vectors (they may be the same engine): out[0] = dot(left, right),
+ * out[1 + o] = dot(left, ops[o], right) for nops <= 8 operators -- LinearAlgebra.dot on
+ * DVecs and dot_from_right (DictVectors/abstractdvec.jl:313-324), the values AllOverlaps
+ * reports per replica pair (replicastrategy.jl:160-183).  Both vectors share M and N.
+ * Sums run in a fixed order on the device; integer dots are exact below 2^53. */
+int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
+                  double* out);
+
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on
  * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled);

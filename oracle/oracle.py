@@ -95,6 +95,14 @@ def lib():
         L.or_matrix_triplets.argtypes = [P(Model), u64p, ctypes.c_int64, P(ctypes.c_int64),
                                          P(ctypes.c_int64), dp, ctypes.c_int64]
         L.or_matrix_triplets.restype = ctypes.c_int64
+        L.or_g2_real.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32]
+        L.or_g2_real.restype = ctypes.c_double
+        L.or_density_diagonal.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32]
+        L.or_density_diagonal.restype = ctypes.c_double
+        L.or_overlaps.argtypes = [u64p, vp, ctypes.c_int32, ctypes.c_int64, u64p, vp, ctypes.c_int32,
+                                  ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i32p, i32p,
+                                  P(Model), dp]
+        L.or_overlaps.restype = ctypes.c_int32
         L.or_rank.argtypes = [ctypes.c_uint64]
         L.or_rank.restype = ctypes.c_int64
         _LIB = L
@@ -278,6 +286,45 @@ def annihilate(keys, vals):
     n = lib().or_annihilate(vtype, _ptr(keys, ctypes.c_uint64), vals.ctypes.data, len(keys),
                             _ptr(ok, ctypes.c_uint64), ov.ctypes.data, cap)
     return ok[:n], ov[:n]
+
+
+def g2_real(bits: int, M: int, d: int) -> float:
+    return float(lib().or_g2_real(ctypes.c_uint64(bits), M, d))
+
+
+def density_diagonal(bits: int, M: int, mode: int) -> float:
+    return float(lib().or_density_diagonal(ctypes.c_uint64(bits), M, mode))
+
+
+def overlaps(tkeys, tvals, ukeys, uvals, M, ops=()):
+    """[dot(t, u), dot(t, op, u) for op in ops]; an op is ("ham", Model), ("g2", d) or
+    ("density", mode)."""
+    tk = np.ascontiguousarray(tkeys, dtype=np.uint64)
+    uk = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    tv = np.ascontiguousarray(tvals)
+    uv = np.ascontiguousarray(uvals)
+    tvt = 0 if tv.dtype == np.int64 else 1
+    uvt = 0 if uv.dtype == np.int64 else 1
+    tv = tv if tvt == 0 else tv.astype(np.float64)
+    uv = uv if uvt == 0 else uv.astype(np.float64)
+    n = len(ops)
+    kinds = np.zeros(max(n, 1), dtype=np.int32)
+    params = np.zeros(max(n, 1), dtype=np.int32)
+    models = (Model * max(n, 1))()
+    for i, (kind, arg) in enumerate(ops):
+        if kind == "ham":
+            kinds[i] = 1
+            models[i] = arg
+        elif kind == "g2":
+            kinds[i], params[i] = 2, arg
+        else:
+            kinds[i], params[i] = 3, arg
+    out = np.zeros(n + 1)
+    lib().or_overlaps(_ptr(tk, ctypes.c_uint64), tv.ctypes.data, tvt, len(tk),
+                      _ptr(uk, ctypes.c_uint64), uv.ctypes.data, uvt, len(uk), M, n,
+                      _ptr(kinds, ctypes.c_int32), _ptr(params, ctypes.c_int32), models,
+                      _ptr(out, ctypes.c_double))
+    return out
 
 
 def double_log_update(shift, tnorm, pnorm, target, zeta=0.08, xi=None, dtau=0.01):

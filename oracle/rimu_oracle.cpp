@@ -705,6 +705,74 @@ int64_t or_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int
     return dump_sorted<double>(w.m, out_keys, (double*)out_vals, cap);
 }
 
+// ---------------------------------------------------------------------------------
+// Overlaps: dot(t, u) and the three-way dot(t, op, u) = dot_from_right(t, op, u),
+// DictVectors/abstractdvec.jl:313-324: a sum over the entries (k, v) of u of
+//   conj(t[k]) * diagonal_element(op, k) * v  +  sum over offdiagonals (k', h') of
+//   conj(t[k']) * h' * v.
+// Operators: kind 1 a Hamiltonian (model), kind 2 G2RealCorrelator(d)
+// (correlation_functions.jl:68-92: (1/M) sum_i n_i (n_{i+d} - delta_{0d}), periodic),
+// kind 3 DensityMatrixDiagonal(mode) (DensityMatrixDiagonal.jl:19-33: n_mode).
+// Sums are taken in the order of u's entries as given.
+// ---------------------------------------------------------------------------------
+double or_g2_real(uint64_t bs, int32_t M, int32_t d) {
+    int onr[64];
+    to_onr(bs, M, onr);
+    int64_t acc = 0;
+    const int dd = ((d % M) + M) % M;
+    if (dd == 0) {
+        for (int i = 0; i < M; ++i) acc += (int64_t)onr[i] * (onr[i] - 1);
+    } else {
+        for (int i = 0; i < M; ++i) acc += (int64_t)onr[i] * onr[(i + dd) % M];
+    }
+    return (double)acc / (double)M;
+}
+
+double or_density_diagonal(uint64_t bs, int32_t M, int32_t mode) {
+    int onr[64];
+    to_onr(bs, M, onr);
+    return (double)onr[mode - 1];
+}
+
+// out[0] = dot(t, u); out[1 + o] = dot(t, op_o, u) for the nops operators (kinds[o], params[o],
+// models[o]).  tv / uv are int64 (vtype 0) or double (vtype 1) per vector.  Returns 0.
+int32_t or_overlaps(const uint64_t* tk, const void* tv, int32_t t_vtype, int64_t nt,
+                    const uint64_t* uk, const void* uv, int32_t u_vtype, int64_t nu,
+                    int32_t M, int32_t nops, const int32_t* kinds, const int32_t* params,
+                    const or_model* models, double* out) {
+    std::unordered_map<uint64_t, double> t;
+    t.reserve((size_t)nt * 2 + 1);
+    for (int64_t i = 0; i < nt; ++i)
+        t[tk[i]] = t_vtype == 0 ? (double)((const int64_t*)tv)[i] : ((const double*)tv)[i];
+    auto T = [&](uint64_t k) { auto it = t.find(k); return it == t.end() ? 0.0 : it->second; };
+    for (int o = 0; o <= nops; ++o) out[o] = 0;
+    for (int64_t i = 0; i < nu; ++i) {
+        const uint64_t k = uk[i];
+        const double v = u_vtype == 0 ? (double)((const int64_t*)uv)[i] : ((const double*)uv)[i];
+        const double tk_ = T(k);
+        out[0] += tk_ * v;
+        for (int o = 0; o < nops; ++o) {
+            double res;
+            if (kinds[o] == 2) {
+                res = tk_ * or_g2_real(k, M, params[o]) * v;
+            } else if (kinds[o] == 3) {
+                res = tk_ * or_density_diagonal(k, M, params[o]) * v;
+            } else {
+                const or_model* h = &models[o];
+                res = tk_ * or_diagonal_element(h, k) * v;
+                const int64_t n = or_num_offdiagonals(h, k);
+                for (int64_t c = 1; c <= n; ++c) {
+                    double hv;
+                    const uint64_t nb = or_get_offdiagonal(h, k, c, &hv);
+                    res += T(nb) * hv * v;
+                }
+            }
+            out[1 + o] += res;
+        }
+    }
+    return 0;
+}
+
 // DoubleLogUpdate, shiftstrategy.jl:174-181.
 double or_double_log_update(double shift, double tnorm, double pnorm, double target,
                             double zeta, double xi, double dtau) {

@@ -24,6 +24,8 @@ from .strategies import (ConstantTimeStep, DoubleLogUpdate, DoubleLogUpdateAfter
                          DontUpdate, LogUpdate, LogUpdateAfterTargetWalkers, ProjectedEnergy)
 from .pmc import (PMCSimulation, ProjectorMonteCarloProblem, default_starting_vector, init,
                   lomc, solve)
+from .observables import DensityMatrixDiagonal, G2RealCorrelator, dot
+from .replicas import AllOverlaps, NoStats
 from .dense import DenseHamiltonians, basis_size
 from . import rimuio as RimuIO
 from .rimuio import load_df, load_state, save_df, save_state

@@ -18,12 +18,15 @@ RIMU_OK, RIMU_ERR_ARG, RIMU_ERR_HIP, RIMU_ERR_CAPACITY, RIMU_ERR_NODEVICE, RIMU_
 MODEL_REAL1D, MODEL_MOM1D = 1, 2
 STYLE_INTEGER, STYLE_DETERMINISTIC, STYLE_DYNAMIC = 1, 2, 3
 INITIATOR_NONE, INITIATOR_DEFAULT, INITIATOR_SIMPLE, INITIATOR_COHERENT = 0, 1, 2, 3
+OP_HAMILTONIAN, OP_G2_REAL, OP_DENSITY_DIAGONAL = 1, 2, 3
+MAX_OPERATORS = 8
 
 # Every symbol declared in include/rimu_hip.h (checked by the CPU test-suite).
 EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
-    "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_engine_set_timing",
+    "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
+    "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
@@ -54,6 +57,10 @@ class Style(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("compression", ctypes.c_int32),
                 ("compression_threshold", ctypes.c_double), ("proj_threshold", ctypes.c_double),
                 ("rel_threshold", ctypes.c_double), ("abs_threshold", ctypes.c_double)]
+
+
+class Operator(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("param", ctypes.c_int32), ("model", Model)]
 
 
 class BatchRun(ctypes.Structure):
@@ -94,6 +101,7 @@ def lib():
         L.rimu_set_projector.argtypes = [vp, vp, vp, i64, vp, vp, i64]
         L.rimu_step.argtypes = [vp, i64, dbl, dbl, dbl, P(StepStats)]
         L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
+        L.rimu_overlaps.argtypes = [vp, vp, P(Operator), i32, P(dbl)]
         L.rimu_engine_set_timing.argtypes = [vp, i32]
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
         L.rimu_engine_last_traffic.argtypes = [vp, P(dbl), P(dbl)]
@@ -155,6 +163,18 @@ def annihilate(keys, vals):
                                 ctypes.byref(nout)))
     m = nout.value
     return ok[:m].copy(), ov[:m].copy()
+
+
+def overlaps(left: "Engine", right: "Engine", ops=()) -> np.ndarray:
+    """[dot(left, right), dot(left, op, right) for op in ops] (``rimu_overlaps``); each op is
+    an ``Operator`` structure."""
+    n = len(ops)
+    if n > MAX_OPERATORS:
+        raise ValueError(f"at most {MAX_OPERATORS} operators per call")
+    arr = (Operator * max(n, 1))(*ops)
+    out = (ctypes.c_double * (n + 1))()
+    check(lib().rimu_overlaps(left._h, right._h, arr, n, out))
+    return np.array(out[:], dtype=np.float64)
 
 
 def hamiltonian_eval(model: Model, addrs, chosen):

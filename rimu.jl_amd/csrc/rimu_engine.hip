@@ -18,6 +18,7 @@
 
 #include "../../include/rimu_hip.h"
 #include "capi_util.hpp"
+#include "overlap_kernels.hpp"
 #include "step_kernels.hpp"
 
 using namespace rimu;
@@ -105,6 +106,10 @@ struct rimu_engine {
     // projector
     DevBuf<uint64_t> pkeys;
     DevBuf<double> pcoef;
+    // overlaps (this engine as the right-hand vector)
+    DevBuf<unsigned char> ov_ops;
+    DevBuf<double> ov_part, ov_out;
+    DevBuf<int64_t> ov_ranges;
     int64_t np = 0, nh = 0;
 
     // host staging
@@ -523,6 +528,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->rtmp.release(); e->plan_status.release(); e->tile_or.release(); e->tile_and.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
+    e->ov_ops.release(); e->ov_part.release(); e->ov_out.release(); e->ov_ranges.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);
     if (e->h_stats) (void)hipHostFree(e->h_stats);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -618,6 +624,76 @@ int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     if (e->len > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
     return e->int_vals ? step_impl<int64_t>(e, step, shift, dtau, boost, stats)
                        : step_impl<double>(e, step, shift, dtau, boost, stats);
+}
+
+int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
+                  double* out) {
+    if (!left || !right || !out || (nops > 0 && !ops)) return fail(RIMU_ERR_ARG, "null argument");
+    if (nops < 0 || nops > OV_MAX_OPS) return fail(RIMU_ERR_ARG, "between 0 and 8 operators");
+    if (left->hdev.M != right->hdev.M || left->hdev.N != right->hdev.N)
+        return fail(RIMU_ERR_ARG, "vectors of different address spaces (M, N)");
+    if (left->device != right->device) return fail(RIMU_ERR_ARG, "vectors on different devices");
+    const int M = right->hdev.M;
+    std::vector<OpDev> dops((size_t)std::max(nops, 1));
+    for (int o = 0; o < nops; ++o) {
+        OpDev d{};
+        d.kind = ops[o].kind;
+        d.param = ops[o].param;
+        if (d.kind == RIMU_OP_HAMILTONIAN) {
+            if (int rc = check_model(&ops[o].model)) return rc;
+            if (ops[o].model.M != M || ops[o].model.N != right->hdev.N)
+                return fail(RIMU_ERR_ARG, "operator Hamiltonian on another address space");
+            d.h = to_dev(ops[o].model);
+        } else if (d.kind == RIMU_OP_G2_REAL) {
+            if (d.param <= -M || d.param >= M) return fail(RIMU_ERR_ARG, "G2RealCorrelator needs |d| < M");
+        } else if (d.kind == RIMU_OP_DENSITY_DIAGONAL) {
+            if (d.param < 1 || d.param > M) return fail(RIMU_ERR_ARG, "DensityMatrixDiagonal mode outside 1..M");
+        } else {
+            return fail(RIMU_ERR_ARG, "unknown operator kind");
+        }
+        dops[o] = d;
+    }
+    for (int o = 0; o <= nops; ++o) out[o] = 0.0;
+    const int64_t Lt = left->len, Lu = right->len;
+    if (Lt == 0 || Lu == 0) return RIMU_OK;
+    rimu_engine* e = right;
+    const int64_t nb = ceil_div(Lu, OV_TILE);
+    HIP_TRY(e->ov_ops.alloc(sizeof(OpDev) * dops.size()));
+    HIP_TRY(e->ov_part.alloc((size_t)nb * (nops + 1)));
+    HIP_TRY(e->ov_out.alloc(OV_MAX_OPS + 1));
+    HIP_TRY(e->ov_ranges.alloc(nb + 1));
+    HIP_TRY(hipMemcpyAsync(e->ov_ops.p, dops.data(), sizeof(OpDev) * dops.size(), hipMemcpyHostToDevice,
+                           e->stream));
+    const uint64_t* tk = left->vkeys[left->cur].p;
+    const void* tv = left->vvals[left->cur].p;
+    const uint64_t* uk = right->vkeys[right->cur].p;
+    const void* uv = right->vvals[right->cur].p;
+    const OpDev* dp = (const OpDev*)e->ov_ops.p;
+    int need_split = 0;
+    for (int o = 0; o < nops; ++o) need_split |= dops[o].kind == RIMU_OP_HAMILTONIAN;
+    hipEvent_t ev{};
+    e->timer.begin("overlap", e->stream, &ev);
+    overlap_ranges_kernel<<<ceil_div(nb + 1, NT), NT, 0, e->stream>>>(tk, Lt, uk, nb, e->ov_ranges.p);
+    if (left->int_vals && right->int_vals)
+        overlap_kernel<int64_t, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
+            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+    else if (left->int_vals)
+        overlap_kernel<int64_t, double><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
+            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+    else if (right->int_vals)
+        overlap_kernel<double, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
+            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+    else
+        overlap_kernel<double, double><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
+            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    overlap_final_kernel<<<nops + 1, NT, 0, e->stream>>>(e->ov_part.p, nb, e->ov_out.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, e->ov_out.p, sizeof(double) * (nops + 1), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->timer.collect();
+    return RIMU_OK;
 }
 
 int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold) {

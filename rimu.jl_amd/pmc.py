@@ -20,6 +20,7 @@ from .addresses import BoseFS
 from .dictvectors import DVec, DeviceDVec, FrozenDVec, PDVec
 from .hamiltonians import AbstractHamiltonian
 from .initiators import Initiator, InitiatorRule, NonInitiator
+from .replicas import NoStats, ReplicaStrategy
 from .strategies import (ConstantTimeStep, DefaultShiftParameters, DoubleLogUpdate,
                          ProjectedEnergy)
 from .styles import IsDynamicSemistochastic, IsStochasticInteger, StochasticStyle
@@ -35,7 +36,7 @@ class ProjectorMonteCarloProblem:
                  xi: float | None = None, shift_strategy=None, time_step_strategy=None,
                  post_step_strategy=(), max_length: int | None = None, metadata=None,
                  display_name: str = "PMCSimulation", random_seed=True, n_replicas: int = 1,
-                 reporting_interval: int = 1):
+                 replica_strategy: ReplicaStrategy | None = None, reporting_interval: int = 1):
         # projector_monte_carlo_problem.jl:221-223
         if initiator is None or initiator is False:
             initiator = NonInitiator()
@@ -44,8 +45,13 @@ class ProjectorMonteCarloProblem:
         elif not isinstance(initiator, InitiatorRule):
             raise ValueError(f"Invalid initiator {initiator!r}")
         self.initiator = initiator
-        if n_replicas != 1:
-            raise NotImplementedError("one replica per problem")
+        # projector_monte_carlo_problem.jl:164, :205: the strategy fixes the replica count
+        if replica_strategy is None:
+            replica_strategy = NoStats(n_replicas)
+        self.replica_strategy = replica_strategy
+        self.n_replicas = replica_strategy.n_replicas
+        if self.n_replicas < 1:
+            raise ValueError("at least one replica")
         self.hamiltonian = hamiltonian
         self.start_at = hamiltonian.starting_address() if start_at is None else start_at
         self.style = IsDynamicSemistochastic() if style is None else style
@@ -117,6 +123,28 @@ def rayleigh_quotient_dvec(h: AbstractHamiltonian, v: DVec) -> float:
     return num / float(np.dot(vals, vals))
 
 
+class _Replica:
+    """One replica: its engine, device vector and shift parameters (SingleState,
+    pmc_simulation.jl:126-146).  ``id`` is the report column suffix: "" for a single replica,
+    "_i" otherwise."""
+
+    def __init__(self, problem, h, v0, seed, index, n):
+        tw = abs(getattr(problem.shift_strategy, "target_walkers", 1000))
+        max_entries = problem.max_length + 1
+        max_spawns = int(min(max(4 * tw + 4096, 4 * len(v0) + 4096), 2 ** 32 - 2))
+        self.style = v0.style
+        self.engine = _native.Engine(h.descriptor(), self.style.descriptor(), seed,
+                                     max_entries, max_spawns)
+        self.initiator = v0.initiator
+        self.engine.set_initiator(*self.initiator.descriptor())
+        keys, vals = v0.to_arrays()
+        self.engine.upload(keys, vals)
+        self.v = DeviceDVec(self.engine, h, self.style, self.initiator)
+        self.id = "" if n == 1 else f"_{index + 1}"
+        self.seed = seed
+        self.last_stats = None
+
+
 class PMCSimulation:
     """State, report and status of a run (pmc_simulation.jl:22-31)."""
 
@@ -130,31 +158,28 @@ class PMCSimulation:
             _seed_counter[0] += 0x9E3779B97F4A7C15
             seed = _seed_counter[0] & (2 ** 64 - 1)
         self.seed = seed
-        tw = abs(getattr(problem.shift_strategy, "target_walkers", 1000))
-        max_entries = problem.max_length + 1
-        max_spawns = int(min(max(4 * tw + 4096, 4 * len(v0) + 4096), 2 ** 32 - 2))
+        n = problem.n_replicas
+        # replica i draws from its own Philox key; replica 1 keeps the problem's seed
+        self.replicas = [_Replica(problem, h, v0, (seed + i * 0xD1B54A32D192ED03) & (2 ** 64 - 1), i, n)
+                         for i in range(n)]
         self.style = v0.style
-        self.engine = _native.Engine(h.descriptor(), self.style.descriptor(), seed,
-                                     max_entries, max_spawns)
         self.initiator = v0.initiator
-        self.engine.set_initiator(*self.initiator.descriptor())
-        keys, vals = v0.to_arrays()
-        self.engine.upload(keys, vals)
-        self.v = DeviceDVec(self.engine, h, self.style, self.initiator)
         # initial shift: Rayleigh quotient of the start vector unless given
         shift = problem.shift
         if shift is None:
             shift = rayleigh_quotient_dvec(h, v0)
-        self.shift_parameters = DefaultShiftParameters(float(shift), v0.walkernumber(),
-                                                       problem.time_step)
+        for r in self.replicas:
+            r.shift_parameters = DefaultShiftParameters(float(shift), v0.walkernumber(),
+                                                        problem.time_step)
         # projections (only ProjectedEnergy is evaluated on the device)
         self.projected = [p for p in problem.post_step_strategy if isinstance(p, ProjectedEnergy)]
         if len(self.projected) > 1:
             raise NotImplementedError("one ProjectedEnergy per problem")
         if self.projected:
             p = self.projected[0]
-            self.engine.set_projector(p.vproj.keys_bits(), p.vproj.coefs(),
-                                      p.hproj.keys_bits(), p.hproj.coefs())
+            for r in self.replicas:
+                r.engine.set_projector(p.vproj.keys_bits(), p.vproj.coefs(),
+                                       p.hproj.keys_bits(), p.hproj.coefs())
         self.step = problem.starting_step
         self.last_step = problem.last_step
         self.wall_time = problem.wall_time
@@ -164,6 +189,27 @@ class PMCSimulation:
         self.success = False
         self.message = ""
         self.elapsed_time = 0.0
+
+    # the first replica's state (the only one for n_replicas = 1)
+    @property
+    def engine(self):
+        return self.replicas[0].engine
+
+    @property
+    def v(self) -> DeviceDVec:
+        return self.replicas[0].v
+
+    @property
+    def shift_parameters(self):
+        return self.replicas[0].shift_parameters
+
+    @property
+    def last_stats(self):
+        return self.replicas[0].last_stats
+
+    @property
+    def state_vectors(self):
+        return [r.v for r in self.replicas]
 
     # report ---------------------------------------------------------------------------
     def _report(self, name, value):
@@ -184,29 +230,31 @@ class PMCSimulation:
         return self.df
 
     # stepping -------------------------------------------------------------------------
-    def advance(self) -> bool:
-        """``advance!`` (fciqmc.jl:126-181)."""
+    def advance(self, r: _Replica | None = None) -> bool:
+        """``advance!`` (fciqmc.jl:126-181) of one replica."""
+        r = self.replicas[0] if r is None else r
         p = self.problem
-        sp = self.shift_parameters
+        sp = r.shift_parameters
         st = None
         try:
-            st = self.engine.step(self.step, sp.shift, sp.time_step)
+            st = r.engine.step(self.step, sp.shift, sp.time_step)
         except _native.CapacityError:
             self.message_detail = "`max_length` reached. Aborting."
             return False
         tnorm, length = st.norm, int(st.len)
-        self.v._norm = tnorm
+        r.v._norm = tnorm
         shift_stats, proceed = p.shift_strategy.update(sp, tnorm)
         if self.step % p.reporting_interval == 0:
-            self._report("len", length)
+            i = r.id
+            self._report("len" + i, length)
             for k, val in shift_stats.items():
-                self._report(k, val)
+                self._report(k + i, val)
             for name, val in zip(self.style.step_stat_names(), self.style.stat_values(st)):
-                self._report(name, val)
+                self._report(name + i, val)
             for pe in self.projected:
-                self._report(pe.vproj_name, st.vproj)
-                self._report(pe.hproj_name, st.hproj)
-        self.last_stats = st
+                self._report(pe.vproj_name + i, st.vproj)
+                self._report(pe.hproj_name + i, st.hproj)
+        r.last_stats = st
         if length == 0:
             self.message_detail = "Population is dead. Aborting."
             return False
@@ -220,10 +268,17 @@ class PMCSimulation:
         if self.aborted or self.success or self.step >= self.last_step:
             return self
         self.step += 1
-        if self.step % self.problem.reporting_interval == 0:
+        reporting = self.step % self.problem.reporting_interval == 0
+        if reporting:
             self._report("step", self.step)
-        proceed = self.advance()
+        proceed = True
+        for r in self.replicas:     # every replica advances, as proceed &= advance!(...)
+            proceed = self.advance(r) and proceed
         self.modified = True
+        if reporting:               # pmc_simulation.jl:293-297
+            names, values = self.problem.replica_strategy.replica_stats(self.state_vectors)
+            for nm, val in zip(names, values):
+                self._report(nm, val)
         if not proceed:
             self.aborted = True
             self.message = f"Aborted in step {self.step}."
@@ -258,7 +313,8 @@ class PMCSimulation:
         return self.v
 
     def close(self):
-        self.engine.close()
+        for r in self.replicas:
+            r.engine.close()
 
 
 def init(problem: ProjectorMonteCarloProblem) -> PMCSimulation:
@@ -271,7 +327,8 @@ def solve(problem: ProjectorMonteCarloProblem) -> PMCSimulation:
 
 def lomc(ham: AbstractHamiltonian, v=None, *, style=None, targetwalkers=1000, s_strat=None,
          laststep=100, dtau=0.01, shift=None, address=None, post_step=(), maxlength=None,
-         step=0, random_seed=None, initiator: InitiatorRule | None = None):
+         step=0, random_seed=None, initiator: InitiatorRule | None = None,
+         replica_strategy: ReplicaStrategy | None = None):
     """``lomc!`` (lomc.jl:95-195): default style ``IsStochasticInteger``, default shift the
     diagonal element of the starting address, ``initiator`` the rule of the default start
     vector (unused when ``v`` is given), no re-seeding.  Returns (df, simulation)."""
@@ -290,6 +347,6 @@ def lomc(ham: AbstractHamiltonian, v=None, *, style=None, targetwalkers=1000, s_
                                    starting_step=step, last_step=laststep, shift_strategy=s_strat,
                                    post_step_strategy=post_step, max_length=maxlength,
                                    random_seed=False if random_seed is None else random_seed,
-                                   display_name="lomc!")
+                                   replica_strategy=replica_strategy, display_name="lomc!")
     sim = init(p).solve_()
     return sim.df, sim

@@ -128,3 +128,18 @@ def shift_estimator(df, skip=0, **kw) -> BlockingResult:
 
 def projected_energy(df, skip=0, hproj="hproj", vproj="vproj", **kw) -> RatioResult:
     return ratio_of_means(np.asarray(df[hproj]), np.asarray(df[vproj]), skip=skip, **kw)
+
+
+def rayleigh_replica_estimator(df, op_name="Op1", vec_name="dot", skip=0, Anorm=1.0,
+                               **kw) -> RatioResult:
+    """``rayleigh_replica_estimator(df; op_name, vec_name, skip, Anorm)`` without reweighting
+    (h = 0; reweighting.jl:526-583): the ratio of means of the operator overlaps summed over
+    replica pairs, ``sum_{a<b} c{a}_{op}_c{b}``, to the vector overlaps ``sum c{a}_{vec}_c{b}``."""
+    n = sum(1 for c in df.columns if c.startswith("norm_"))
+    num = 0.0
+    den = 0.0
+    for a in range(1, n + 1):
+        for b in range(a + 1, n + 1):
+            num = num + np.asarray(df[f"c{a}_{op_name}_c{b}"], dtype=float) * Anorm
+            den = den + np.asarray(df[f"c{a}_{vec_name}_c{b}"], dtype=float)
+    return ratio_of_means(np.asarray(num)[skip:], np.asarray(den)[skip:], **kw)
