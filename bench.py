@@ -33,6 +33,12 @@ Two more objects on the same line:
   (download + zstd IPC write; read + upload), in GB/s of state.
 * "overlaps": the AllOverlaps reductions of the headline state v against the state one step
   later w: dot(v, w), dot(v, G2RealCorrelator(1), w), dot(v, H, w).
+* "wide": two-chunk addresses (99 bits).  (a) Rimu's benchmark case "(50, 50) Real space"
+  (benchmark/benchmarks.jl:66-73): near_uniform(BoseFS{50,50}), HubbardReal1D u = 6,
+  PDVec(addr => 1.0; IsDynamicSemistochastic), DoubleLogUpdate(target_walkers = 50_000),
+  dtau = 1e-4, 1000 steps through lomc, timed whole; steady-state step times of the engine
+  and of the oracle (one core) from its final vector.  (b) The same model grown to 1e8
+  walkers with the headline's settings: spawn attempts/s and per-kernel times.
 """
 from __future__ import annotations
 
@@ -62,16 +68,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_alg_bytes(name, S, L, Lnew, vsize=8):
+def kernel_alg_bytes(name, S, L, Lnew, vsize=8, ksize=8):
     """Algorithmic bytes of one launch: every record the kernel must touch, read or
-    written once.  Records are (u64 key, 8-byte value)."""
-    rec = 8 + vsize
+    written once.  Records are (key, 8-byte value); keys are 8 bytes (one-chunk addresses)
+    or 16 bytes (two chunks)."""
+    rec = ksize + vsize
     if name == "radix_scatter":
         return 2 * S * rec
     if name == "radix_count":
-        return S * 8
+        return S * ksize
     if name == "merge":
-        return S * rec + L * (8 + vsize) + Lnew * rec
+        return S * rec + L * (ksize + vsize) + Lnew * rec
     if name == "compact":
         return 2 * Lnew * rec
     if name == "spawn":
@@ -442,6 +449,112 @@ def run_initiator_headline(rimu, keys, vals, shift, step0, args):
             "spawn_records_per_step": S, "kernels": kern}
 
 
+def run_wide_case(rimu, args):
+    """benchmark/benchmarks.jl:66-73, "(50, 50) Real space" (99-bit addresses, two chunks)
+    through lomc, then steady-state step times of the engine and of the oracle."""
+    M = N = 50
+    addr = rimu.near_uniform(N, M)
+    H = rimu.HubbardReal1D(addr, u=6.0)
+    dv = rimu.PDVec((addr, 1.0), style=rimu.IsDynamicSemistochastic())
+    s_strat = rimu.DoubleLogUpdate(target_walkers=50_000)
+    dtau, last = 1e-4, args.wide_laststep
+    t0 = time.perf_counter()
+    df, sim = rimu.lomc(H, dv, s_strat=s_strat, dtau=dtau, laststep=last, random_seed=17)
+    run_s = time.perf_counter() - t0
+    shift = float(df["shift"].to_numpy()[-1])
+    keys, vals = sim.engine.download()
+    eng = sim.engine
+    eng.set_timing(True)
+    n_gpu = args.wide_gpu_steps
+    t1 = time.perf_counter()
+    for i in range(n_gpu):
+        eng.step(last + 1 + i, shift, dtau)
+    gpu_step = (time.perf_counter() - t1) / n_gpu
+    times = eng.kernel_times()
+    sim.close()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    model = O.real1d(M, N, 6.0, 1.0)
+    sty = O.style("dynamic")
+    k, v = keys, vals
+    n_cpu = 0
+    t2 = time.perf_counter()
+    while n_cpu < 2000 and time.perf_counter() - t2 < args.wide_cpu_seconds:
+        k, v, _ = O.step(model, sty, k, v, seed=17, step=last + 1 + n_cpu, shift=shift,
+                         dtau=dtau, order=0)
+        n_cpu += 1
+    cpu_step = (time.perf_counter() - t2) / max(n_cpu, 1)
+    return {"workload": "HubbardReal1D M=N=50 near_uniform u=6, PDVec(addr => 1.0) "
+                        "IsDynamicSemistochastic, DoubleLogUpdate target 50000, dtau 1e-4",
+            "steps": last, "seconds": run_s, "steps_per_s": last / run_s,
+            "walkers": float(df["norm"].to_numpy()[-1]), "len": int(df["len"].to_numpy()[-1]),
+            "shift_last": shift,
+            "gpu_ms_per_step_steady": 1e3 * gpu_step,
+            "gpu_kernels_ms_per_step": {n: ms / n_gpu for n, (ms, c) in times.items()},
+            "cpu_baseline": {"ms_per_step": 1e3 * cpu_step, "cores": 1, "kind": "port",
+                             "sample": f"{n_cpu} oracle steps from the final vector "
+                                       f"({len(keys)} entries), order of the reference's columns"},
+            "cpu_over_gpu_steady": cpu_step / gpu_step}
+
+
+def run_wide_headline(rimu, args):
+    """HubbardReal1D M = N = 50 (two-chunk addresses) grown to args.wide_target walkers with
+    the headline's settings: spawn attempts/s and per-kernel times with 16-byte keys."""
+    M = N = 50
+    H = rimu.HubbardReal1D(rimu.near_uniform(N, M), u=6.0, t=1.0)
+    style = rimu.IsDynamicSemistochastic()
+    target = args.wide_target
+    eng = rimu._native.Engine(H.descriptor(), style.descriptor(), 17, int(round(2 * target + 100)) + 1,
+                              int(min(4 * target + 4096, 2 ** 32 - 2)))
+    start = H.starting_address()
+    eng.upload(rimu.addresses.pack_keys([start.bits], 2), np.array([10.0]))
+    dtau = 1e-3
+    strat = rimu.DoubleLogUpdate(target_walkers=target)
+    sp = rimu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, dtau)
+    step = 0
+    t_eq = time.perf_counter()
+    last_log = t_eq
+    while step < args.wide_equilibration:
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        if time.perf_counter() - last_log > 10:
+            log(f"wide equilibration step {step}: len {st.len:.3g} norm {st.norm:.3g} shift {sp.shift:.4f}")
+            last_log = time.perf_counter()
+    t_eq = time.perf_counter() - t_eq
+    eng.set_timing(True)
+    K = args.wide_steps
+    attempts = S_tot = L_tot = Lnew_tot = 0.0
+    L_prev = len(eng)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        attempts += st.spawn_attempts
+        b, records = eng.last_traffic()
+        S_tot += records - L_prev
+        L_tot += L_prev
+        Lnew_tot += st.len
+        L_prev = st.len
+    elapsed = time.perf_counter() - t0
+    times = eng.kernel_times()
+    eng.close()
+    S, L, Ln = S_tot / K, L_tot / K, Lnew_tot / K
+    kern = {}
+    for name, (ms, c) in times.items():
+        kern[name] = {"ms_avg": ms / max(c, 1), "launches_per_step": c / K}
+        bpl = kernel_alg_bytes(name, S, L, Ln, ksize=16)
+        if bpl is not None:
+            kern[name]["alg_GBps"] = bpl / (kern[name]["ms_avg"] / 1e3) / 1e9
+    return {"workload": f"HubbardReal1D M=N=50 u=6 t=1 (99-bit addresses), IsDynamicSemistochastic "
+                        f"FP64, DoubleLogUpdate target {target:.3g}, dtau 1e-3, seed 17",
+            "equilibration_steps": args.wide_equilibration, "equilibration_s": t_eq,
+            "value": attempts / elapsed, "unit": "spawn_attempts/s",
+            "ms_per_step": 1e3 * elapsed / K, "steps": K, "len": st.len, "walkers": st.norm,
+            "spawn_records_per_step": S, "kernels": kern}
+
+
 def run_adjoint_leg(rimu, args):
     """The differentiable projector: HubbardReal1D M = N = 12 (dimension 1,352,078),
     u in {2,4,6,8} as four right-hand sides of one product, K = 2000 steps of dtau 1e-3,
@@ -525,6 +638,14 @@ def main():
                     help="steps of the initiator benchmark case (0 skips it)")
     ap.add_argument("--init-gpu-steps", type=int, default=200)
     ap.add_argument("--init-cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--wide-laststep", type=int, default=1000,
+                    help="steps of the (50, 50) real-space case (0 skips it)")
+    ap.add_argument("--wide-gpu-steps", type=int, default=200)
+    ap.add_argument("--wide-cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--wide-target", type=float, default=1e8)
+    ap.add_argument("--wide-equilibration", type=int, default=800)
+    ap.add_argument("--wide-steps", type=int, default=10)
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -678,6 +799,18 @@ def main():
             f"{initiator['cpu_baseline']['ms_per_step']:.1f} ms/step CPU")
     if init_headline is not None:
         initiator = dict(initiator or {}, headline=init_headline)
+    wide = None
+    if not args.no_wide:
+        wide = {}
+        if args.wide_laststep > 0:
+            wide["case"] = run_wide_case(rimu, args)
+            log(f"wide case: {wide['case']['seconds']:.1f} s for {wide['case']['steps']} steps, steady "
+                f"{wide['case']['gpu_ms_per_step_steady']:.3f} ms/step GPU vs "
+                f"{wide['case']['cpu_baseline']['ms_per_step']:.1f} ms/step CPU")
+        if args.wide_steps > 0:
+            wide["headline"] = run_wide_headline(rimu, args)
+            log(f"wide headline: {wide['headline']['value']:.3g} attempts/s, "
+                f"{wide['headline']['ms_per_step']:.2f} ms/step")
     adjoint = None if args.no_adjoint else run_adjoint_leg(rimu, args)
     if adjoint:
         log(f"adjoint: {adjoint['value']:.3g} grads/s vs plain {adjoint['plain_value']:.3g} "
@@ -712,6 +845,7 @@ def main():
         "initiator": initiator,
         "checkpoint": checkpoint,
         "overlaps": overlaps,
+        "wide": wide,
     }
     print(json.dumps(out), flush=True)
 

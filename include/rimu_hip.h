@@ -12,9 +12,14 @@
  * the call and copied.  No function throws or unwinds across the ABI.  An engine is driven
  * by one host thread at a time (its work is ordered on one HIP stream).
  *
- * Addresses are BoseFS bit strings on one 64-bit chunk (B = N + M - 1 <= 64 bits), in
- * the layout of BitStringAddresses/bitstring.jl:464-471: mode 1 in the least significant
- * bits, mode i written as n_i one-bits followed by a zero separator.
+ * Addresses are BoseFS bit strings of B = N + M - 1 <= 128 bits, in the layout of
+ * BitStringAddresses/bitstring.jl:464-508: mode 1 in the least significant bits, mode i
+ * written as n_i one-bits followed by a zero separator.  Through this ABI an address is
+ * kw uint64 words: kw = 1 for B <= 64, kw = 2 (low word first, then high word) for
+ * 64 < B <= 128.  The order of addresses is the numeric order of the whole bit string,
+ * Rimu's isless on BitStrings (bitstring.jl:317-324, chunk 1 most significant).  Arrays
+ * of "keys" below hold kw words per address, where kw is that of the engine's or model's
+ * M and N.  The batched and dense entry points take one-word addresses (B <= 64).
  */
 #ifndef RIMU_HIP_H
 #define RIMU_HIP_H
@@ -114,18 +119,19 @@ int rimu_engine_destroy(rimu_engine* e);
  * of (seed, address, attempt, step), gives the same result when repeated. */
 int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t max_spawns);
 
-/* Replace the vector.  keys must be strictly ascending; vals are int64 for
+/* Replace the vector.  keys (n addresses of kw words) must be strictly ascending; vals are int64 for
  * RIMU_STYLE_STOCHASTIC_INTEGER and double otherwise; zero values are dropped.
  * (copyto!/setindex! on the DVec, dvec.jl:129-136.) */
 int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n);
 /* Current length (length(v)). */
 int rimu_vector_length(rimu_engine* e, int64_t* n);
-/* Copy the vector out, ascending keys (pairs(v)).  cap is the capacity of keys/vals. */
+/* Copy the vector out, ascending keys of kw words each (pairs(v)).  cap is the capacity of
+ * keys/vals in addresses. */
 int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap, int64_t* n);
 
 /* Projector of the ProjectedEnergy post-step (poststepstrategy.jl:82-121): vproj is
  * sum_i pc_i * v[pk_i], hproj is sum_i hc_i * v[hk_i] (hproj = freeze(H' * projector)).
- * keys need not be sorted.  n = nh = 0 disables the projection. */
+ * keys (kw words each) need not be sorted.  n = nh = 0 disables the projection. */
 int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int64_t n,
                        const uint64_t* hk, const double* hc, int64_t nh);
 
@@ -152,7 +158,8 @@ enum {
  * and each address keeps the rule's from_initiator_value of its three components
  * (move_and_compress!, pdworkingmemory.jl:262-273).  Under NoCompression an address that
  * received only discarded deposits stays in the vector with value 0, as in the reference.
- * Needs N + M - 1 <= 63.  RIMU_INITIATOR_NONE restores the plain step. */
+ * Needs N + M - 1 <= 64 * kw - 1 (one free bit for the deposit's class).
+ * RIMU_INITIATOR_NONE restores the plain step. */
 int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold);
 
 /* Operators of the three-way dot: a Hamiltonian (model, off-diagonals included),
@@ -186,13 +193,15 @@ int rimu_engine_last_traffic(rimu_engine* e, double* alg_bytes, double* records)
 
 /* Annihilation of a spawn list on its own (deposit! folded over the list,
  * Interfaces/dictvectors.jl:49-51 with dvec.jl:129-136): equal keys are summed in list
- * order, exact zeros dropped; output ascending.  vtype 0: int64 values, 1: double. */
+ * order, exact zeros dropped; output ascending.  vtype 0: int64 values, 1: double.  Keys
+ * are single uint64 words. */
 int rimu_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
                     uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out);
 
 /* Hamiltonian interface on the device for a batch of addresses (for tests and
  * ProjectedEnergy set-up): diagonal_element and num_offdiagonals
- * (Interfaces/hamiltonians.jl), and get_offdiagonal for one chosen index per address. */
+ * (Interfaces/hamiltonians.jl), and get_offdiagonal for one chosen index per address.
+ * addrs and new_addr hold kw words per address (kw of the model's M and N). */
 int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const int64_t* chosen,
                           int64_t n, double* diag, int64_t* num_offdiag, uint64_t* new_addr,
                           double* offdiag_value);

@@ -100,8 +100,16 @@ def lib():
         L.or_density_diagonal.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32]
         L.or_density_diagonal.restype = ctypes.c_double
         L.or_overlaps.argtypes = [u64p, vp, ctypes.c_int32, ctypes.c_int64, u64p, vp, ctypes.c_int32,
-                                  ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, i32p, i32p,
-                                  P(Model), dp]
+                                  ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  i32p, i32p, P(Model), dp]
+        u64 = ctypes.c_uint64
+        L.or_from_onr2.argtypes = [i32p, ctypes.c_int32, u64p]
+        L.or_to_onr2.argtypes = [u64, u64, ctypes.c_int32, i32p]
+        L.or_num_offdiagonals2.argtypes = [P(Model), u64, u64]
+        L.or_num_offdiagonals2.restype = ctypes.c_int64
+        L.or_diagonal_element2.argtypes = [P(Model), u64, u64]
+        L.or_diagonal_element2.restype = ctypes.c_double
+        L.or_get_offdiagonal2.argtypes = [P(Model), u64, u64, ctypes.c_int64, dp, u64p]
         L.or_overlaps.restype = ctypes.c_int32
         L.or_rank.argtypes = [ctypes.c_uint64]
         L.or_rank.restype = ctypes.c_int64
@@ -165,14 +173,44 @@ def style(kind: str, compression_threshold: float = 1.0, proj_threshold: float =
 # ----------------------------------------------------------------------------------------
 # Addresses and matrix elements
 # ----------------------------------------------------------------------------------------
+MASK64 = (1 << 64) - 1
+
+
+def key_words(model) -> int:
+    """64-bit words per address: 1, or 2 when N + M - 1 > 64."""
+    return 2 if model.M + model.N - 1 > 64 else 1
+
+
+def pack_keys(ints, kw: int) -> np.ndarray:
+    """Python-int addresses as uint64 words: shape (n,) for kw = 1, (n, 2) [lo, hi] for kw = 2."""
+    ints = [int(x) for x in ints]
+    if kw == 1:
+        return np.array(ints, dtype=np.uint64)
+    return np.array([[x & MASK64, x >> 64] for x in ints], dtype=np.uint64).reshape(len(ints), 2)
+
+
+def unpack_keys(arr) -> list:
+    arr = np.asarray(arr, dtype=np.uint64)
+    if arr.ndim == 1:
+        return [int(x) for x in arr.tolist()]
+    return [int(lo) | (int(hi) << 64) for lo, hi in arr.tolist()]
+
+
 def from_onr(onr) -> int:
     a = np.ascontiguousarray(onr, dtype=np.int32)
+    if len(a) + int(a.sum()) - 1 > 64:
+        out = np.zeros(2, dtype=np.uint64)
+        lib().or_from_onr2(_ptr(a, ctypes.c_int32), len(a), _ptr(out, ctypes.c_uint64))
+        return int(out[0]) | (int(out[1]) << 64)
     return int(lib().or_from_onr(_ptr(a, ctypes.c_int32), len(a)))
 
 
 def to_onr(bits: int, M: int):
     out = np.zeros(M, dtype=np.int32)
-    lib().or_to_onr(ctypes.c_uint64(bits), M, _ptr(out, ctypes.c_int32))
+    if bits >> 64:
+        lib().or_to_onr2(bits & MASK64, bits >> 64, M, _ptr(out, ctypes.c_int32))
+    else:
+        lib().or_to_onr(ctypes.c_uint64(bits), M, _ptr(out, ctypes.c_int32))
     return tuple(int(x) for x in out)
 
 
@@ -188,15 +226,24 @@ def bose_hubbard_interaction(bits: int) -> int:
 
 
 def num_offdiagonals(model: Model, bits: int) -> int:
+    if key_words(model) == 2:
+        return int(lib().or_num_offdiagonals2(ctypes.byref(model), bits & MASK64, bits >> 64))
     return int(lib().or_num_offdiagonals(ctypes.byref(model), ctypes.c_uint64(bits)))
 
 
 def diagonal_element(model: Model, bits: int) -> float:
+    if key_words(model) == 2:
+        return float(lib().or_diagonal_element2(ctypes.byref(model), bits & MASK64, bits >> 64))
     return float(lib().or_diagonal_element(ctypes.byref(model), ctypes.c_uint64(bits)))
 
 
 def get_offdiagonal(model: Model, bits: int, chosen: int):
     v = ctypes.c_double()
+    if key_words(model) == 2:
+        out = np.zeros(2, dtype=np.uint64)
+        lib().or_get_offdiagonal2(ctypes.byref(model), bits & MASK64, bits >> 64, chosen,
+                                  ctypes.byref(v), _ptr(out, ctypes.c_uint64))
+        return int(out[0]) | (int(out[1]) << 64), float(v.value)
     nb = lib().or_get_offdiagonal(ctypes.byref(model), ctypes.c_uint64(bits), chosen, ctypes.byref(v))
     return int(nb), float(v.value)
 
@@ -223,13 +270,17 @@ STAT_NAMES = ("attempts", "exact_steps", "inexact_steps", "spawns", "deaths", "c
 def step(model: Model, sty: Style, keys, vals, *, seed: int, step: int, shift: float,
          dtau: float, boost: float = 1.0, order: int = 0, threads: int = 0):
     """One ``apply_operator!`` + ``compress!`` of (keys, vals).  ``threads > 0`` uses the
-    threaded PDVec-style restatement (output unsorted)."""
+    threaded PDVec-style restatement (output unsorted, one-chunk addresses only).  Keys are
+    (n,) uint64, or (n, 2) [lo, hi] words for two-chunk addresses."""
+    kw = key_words(model)
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    if kw == 2:
+        keys = keys.reshape(-1, 2)
     vtype = 0 if sty.style == STYLE_INTEGER else 1
     vals = np.ascontiguousarray(vals, dtype=np.int64 if vtype == 0 else np.float64)
     cap = max(64, 4 * len(keys) + 1024)
     while True:
-        ok = np.zeros(cap, dtype=np.uint64)
+        ok = np.zeros((cap, 2) if kw == 2 else cap, dtype=np.uint64)
         ov = np.zeros(cap, dtype=vals.dtype)
         st = np.zeros(16, dtype=np.float64)
         if threads:
@@ -255,12 +306,15 @@ def step_initiator(model: Model, sty: Style, keys, vals, *, rule: str, threshold
                    order: int = 0):
     """One step with an initiator rule (``rule`` in INITIATOR_RULES).  With NoCompression
     the result may hold zero values (addresses that received only discarded deposits)."""
+    kw = key_words(model)
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    if kw == 2:
+        keys = keys.reshape(-1, 2)
     vtype = 0 if sty.style == STYLE_INTEGER else 1
     vals = np.ascontiguousarray(vals, dtype=np.int64 if vtype == 0 else np.float64)
     cap = max(64, 4 * len(keys) + 1024)
     while True:
-        ok = np.zeros(cap, dtype=np.uint64)
+        ok = np.zeros((cap, 2) if kw == 2 else cap, dtype=np.uint64)
         ov = np.zeros(cap, dtype=vals.dtype)
         st = np.zeros(16, dtype=np.float64)
         n = lib().or_step_initiator(ctypes.byref(model), ctypes.byref(sty), seed, step, shift,
@@ -301,6 +355,7 @@ def overlaps(tkeys, tvals, ukeys, uvals, M, ops=()):
     ("density", mode)."""
     tk = np.ascontiguousarray(tkeys, dtype=np.uint64)
     uk = np.ascontiguousarray(ukeys, dtype=np.uint64)
+    kw = 2 if tk.ndim == 2 or uk.ndim == 2 else 1
     tv = np.ascontiguousarray(tvals)
     uv = np.ascontiguousarray(uvals)
     tvt = 0 if tv.dtype == np.int64 else 1
@@ -321,7 +376,7 @@ def overlaps(tkeys, tvals, ukeys, uvals, M, ops=()):
             kinds[i], params[i] = 3, arg
     out = np.zeros(n + 1)
     lib().or_overlaps(_ptr(tk, ctypes.c_uint64), tv.ctypes.data, tvt, len(tk),
-                      _ptr(uk, ctypes.c_uint64), uv.ctypes.data, uvt, len(uk), M, n,
+                      _ptr(uk, ctypes.c_uint64), uv.ctypes.data, uvt, len(uk), M, kw, n,
                       _ptr(kinds, ctypes.c_int32), _ptr(params, ctypes.c_int32), models,
                       _ptr(out, ctypes.c_double))
     return out

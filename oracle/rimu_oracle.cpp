@@ -11,9 +11,11 @@
 // run seed and addressed by (source address, attempt, step, purpose).  The GPU engine
 // uses the same streams, so a single step can be compared record for record.
 //
-// Addresses are BoseFS on one BitString chunk (B = N + M - 1 <= 64 bits), layout of
-// bitstring.jl:464-471: mode 1 occupies the least significant bits, each mode is
-// written as n_i one-bits followed by a zero separator.
+// Addresses are BoseFS on one BitString chunk (B = N + M - 1 <= 64 bits) or two (B <= 128,
+// held as one 128-bit integer whose low 64 bits are the last chunk), layout of
+// bitstring.jl:464-508: mode 1 occupies the least significant bits, each mode is written as
+// n_i one-bits followed by a zero separator.  Across the C interface a two-chunk address is
+// two uint64 words, the low word first.
 
 #include <algorithm>
 #include <cmath>
@@ -66,6 +68,16 @@ inline P4 draw(uint64_t seed, uint64_t addr, uint32_t attempt, int64_t step, uin
                          (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
+typedef unsigned __int128 u128;
+
+// Two-chunk addresses: the high chunk selects the key (seed ^ hi * 0x9E3779B97F4A7C15, a
+// bijection of hi for a given seed), the low chunk the counter; hi = 0 gives the one-chunk
+// streams.
+inline P4 draw(uint64_t seed, u128 addr, uint32_t attempt, int64_t step, uint32_t tag) {
+    const uint64_t hi = (uint64_t)(addr >> 64);
+    return draw(seed ^ (hi * 0x9E3779B97F4A7C15ull), (uint64_t)addr, attempt, step, tag);
+}
+
 // 53-bit uniform double in [0, 1) from two 32-bit words.
 inline double u53(uint32_t hi, uint32_t lo) {
     const uint64_t x = ((((uint64_t)hi) << 32) | lo) >> 11;
@@ -82,21 +94,30 @@ inline int64_t pick1(uint32_t r, int64_t n) {
 // ---------------------------------------------------------------------------------
 struct BoseIndex { int occnum, mode, offset; };  // BoseFSIndex, bosefs.jl / fockaddress.jl
 
-inline int tz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
-inline int to64(uint64_t x) { return tz64(~x); }
+template <class A> constexpr int WIDTH = 8 * (int)sizeof(A);
+inline int tzA(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
+inline int tzA(u128 x) {
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    return lo ? __builtin_ctzll(lo) : (hi ? 64 + __builtin_ctzll(hi) : 128);
+}
+template <class A> inline int toA(A x) { return tzA((A)~x); }
+inline int tz64(uint64_t x) { return tzA(x); }
+inline int to64(uint64_t x) { return toA(x); }
 
-// occupied_modes iterator, bitstring.jl:636-655 (single chunk).
-inline int occupied_modes(uint64_t chunk, BoseIndex* out) {
+// occupied_modes iterator, bitstring.jl:636-655.
+template <class A>
+inline int occupied_modes(A chunk, BoseIndex* out) {
+    constexpr int W = WIDTH<A>;
     int n = 0;
-    int empty = tz64(chunk);
-    if (empty >= 64) return 0;
+    int empty = tzA(chunk);
+    if (empty >= W) return 0;
     chunk >>= empty;
     int bit = empty, mode = 1 + empty;
     while (chunk) {
-        int bosons = to64(chunk);
-        chunk = bosons >= 64 ? 0 : (chunk >> bosons);
-        int e = chunk ? tz64(chunk) : 0;
-        chunk = e >= 64 ? 0 : (chunk >> e);
+        int bosons = toA(chunk);
+        chunk = bosons >= W ? (A)0 : (A)(chunk >> bosons);
+        int e = chunk ? tzA(chunk) : 0;
+        chunk = e >= W ? (A)0 : (A)(chunk >> e);
         out[n++] = BoseIndex{bosons, mode, bit};
         bit += bosons + e;
         mode += e;
@@ -104,24 +125,29 @@ inline int occupied_modes(uint64_t chunk, BoseIndex* out) {
     return n;
 }
 
-// from_bose_onr, bitstring.jl:464-471.
-inline uint64_t from_onr(const int* onr, int M) {
-    uint64_t r = 0;
+// from_bose_onr, bitstring.jl:464-508 (the chunks read as one integer).
+template <class A>
+inline A from_onr_t(const int* onr, int M) {
+    constexpr int W = WIDTH<A>;
+    A r = 0;
     for (int i = M - 1; i >= 0; --i) {
         const int n = onr[i];
-        r = (n + 1 >= 64) ? 0 : (r << (n + 1));
-        r |= (n >= 64 ? ~0ull : ((1ull << n) - 1));
+        r = (n + 1 >= W) ? (A)0 : (A)(r << (n + 1));
+        r |= (n >= W ? (A)~(A)0 : (A)(((A)1 << n) - 1));
     }
     return r;
 }
+inline uint64_t from_onr(const int* onr, int M) { return from_onr_t<uint64_t>(onr, M); }
 
 // to_bose_onr, bitstring.jl:511-520.
-inline void to_onr(uint64_t bs, int M, int* onr) {
+template <class A>
+inline void to_onr(A bs, int M, int* onr) {
+    constexpr int W = WIDTH<A>;
     for (int i = 0; i < M; ++i) onr[i] = 0;
     for (int mode = 0; mode < M; ++mode) {
-        int b = to64(bs);
+        int b = toA(bs);
         onr[mode] = b;
-        bs = (b + 1 >= 64) ? 0 : (bs >> (b + 1));
+        bs = (b + 1 >= W) ? (A)0 : (A)(bs >> (b + 1));
         if (!bs) break;
     }
 }
@@ -161,18 +187,18 @@ inline int64_t excitation_value(const BoseIndex* cre, int nc, const BoseIndex* d
 // excitation(b, creations, destructions), bosefs.jl:252-255 + bitstring.jl:581-595.
 // The particle moves are restated on the occupation-number representation: the resulting
 // address is the one with the destroyed modes decremented and created modes incremented.
-inline uint64_t excitation(uint64_t bs, int M, const BoseIndex* cre, const BoseIndex* des,
-                           int n, double* val) {
+template <class A>
+inline A excitation(A bs, int M, const BoseIndex* cre, const BoseIndex* des, int n, double* val) {
     BoseIndex cre_rev[4], des_rev[4];
     for (int i = 0; i < n; ++i) { cre_rev[i] = cre[n - 1 - i]; des_rev[i] = des[n - 1 - i]; }
     const int64_t v = excitation_value(cre_rev, n, des_rev, n);
     if (v == 0) { *val = 0.0; return bs; }
-    int onr[64];
+    int onr[128];
     to_onr(bs, M, onr);
     for (int i = 0; i < n; ++i) onr[des[i].mode - 1] -= 1;
     for (int i = 0; i < n; ++i) onr[cre[i].mode - 1] += 1;
     *val = std::sqrt((double)v);
-    return from_onr(onr, M);
+    return from_onr_t<A>(onr, M);
 }
 
 inline int mod1(int x, int m) { int r = ((x - 1) % m + m) % m; return r + 1; }
@@ -239,9 +265,14 @@ int64_t or_bose_hubbard_interaction(uint64_t bs) {
     return bose_hubbard_interaction(occ, n);
 }
 
+}  // extern "C"
+
+namespace {
+
 // num_offdiagonals: HubbardReal1D.jl:51-53; HubbardMom1D.jl:131-140.
-int64_t or_num_offdiagonals(const or_model* h, uint64_t bs) {
-    BoseIndex occ[64];
+template <class A>
+int64_t num_offdiagonals_t(const or_model* h, A bs) {
+    BoseIndex occ[128];
     const int s = occupied_modes(bs, occ);
     if (h->model == 1) return 2 * (int64_t)s;
     int d = 0;
@@ -252,8 +283,9 @@ int64_t or_num_offdiagonals(const or_model* h, uint64_t bs) {
 
 // diagonal_element: HubbardReal1D.jl:55-57; HubbardMom1D.jl:174-177 with
 // momentum_transfer_diagonal excitations.jl:126-137 and dot(kes, map) fockaddress.jl:316-321.
-double or_diagonal_element(const or_model* h, uint64_t bs) {
-    BoseIndex occ[64];
+template <class A>
+double diagonal_element_t(const or_model* h, A bs) {
+    BoseIndex occ[128];
     const int s = occupied_modes(bs, occ);
     if (h->model == 1) {
         return h->u * (double)bose_hubbard_interaction(occ, s) / 2;
@@ -270,21 +302,23 @@ double or_diagonal_element(const or_model* h, uint64_t bs) {
 }
 
 // hopnextneighbour generic form, bosefs.jl:347-353; value -t*sqrt(...), HubbardReal1D.jl:59-62.
-static uint64_t real1d_offdiagonal(const or_model* h, uint64_t bs, const BoseIndex* occ,
-                                   int nocc, int64_t chosen, double* value) {
+template <class A>
+A real1d_offdiagonal(const or_model* h, A bs, const BoseIndex* occ, int nocc, int64_t chosen,
+                     double* value) {
     const BoseIndex src = occ[(chosen + 1) / 2 - 1];       // find_occupied_mode
     const int dir = (chosen & 1) ? 1 : -1;
     const BoseIndex dst = find_mode(occ, nocc, mod1(src.mode + dir, h->M));
     double v;
-    const uint64_t nb = excitation(bs, h->M, &dst, &src, 1, &v);
+    const A nb = excitation(bs, h->M, &dst, &src, 1, &v);
     *value = -h->t * v;
     return nb;
 }
 
 // momentum_transfer_excitation (fold=true), excitations.jl:26-80; value u/(2M)*onproduct,
 // HubbardMom1D.jl:189-194.
-static uint64_t mom1d_offdiagonal(const or_model* h, uint64_t bs, const BoseIndex* occ,
-                                  int nocc, int64_t chosen, double* value) {
+template <class A>
+A mom1d_offdiagonal(const or_model* h, A bs, const BoseIndex* occ, int nocc, int64_t chosen,
+                    double* value) {
     const int M = h->M;
     const int64_t singlies = nocc;
     int64_t dbl = chosen - singlies * (singlies - 1) * (M - 2);
@@ -313,16 +347,55 @@ static uint64_t mom1d_offdiagonal(const or_model* h, uint64_t bs, const BoseInde
     const int d1 = mod1((int)(src[1].mode - mom_change), M);
     BoseIndex dst[2] = {find_mode(occ, nocc, d0), find_mode(occ, nocc, d1)};
     double v;
-    const uint64_t nb = excitation(bs, M, dst, src, 2, &v);
+    const A nb = excitation(bs, M, dst, src, 2, &v);
     *value = h->u / (2 * M) * v;
     return nb;
 }
 
-uint64_t or_get_offdiagonal(const or_model* h, uint64_t bs, int64_t chosen, double* value) {
-    BoseIndex occ[64];
+template <class A>
+A get_offdiagonal_t(const or_model* h, A bs, int64_t chosen, double* value) {
+    BoseIndex occ[128];
     const int nocc = occupied_modes(bs, occ);
     if (h->model == 1) return real1d_offdiagonal(h, bs, occ, nocc, chosen, value);
     return mom1d_offdiagonal(h, bs, occ, nocc, chosen, value);
+}
+
+inline u128 join(uint64_t lo, uint64_t hi) { return ((u128)hi << 64) | lo; }
+
+}  // namespace
+
+extern "C" {
+
+int64_t or_num_offdiagonals(const or_model* h, uint64_t bs) { return num_offdiagonals_t(h, bs); }
+double or_diagonal_element(const or_model* h, uint64_t bs) { return diagonal_element_t(h, bs); }
+uint64_t or_get_offdiagonal(const or_model* h, uint64_t bs, int64_t chosen, double* value) {
+    return get_offdiagonal_t(h, bs, chosen, value);
+}
+
+// Two-chunk addresses (B = N + M - 1 > 64) as (lo, hi) words.
+void or_from_onr2(const int32_t* onr, int32_t M, uint64_t* out) {
+    int tmp[128];
+    for (int i = 0; i < M; ++i) tmp[i] = onr[i];
+    const u128 r = from_onr_t<u128>(tmp, M);
+    out[0] = (uint64_t)r;
+    out[1] = (uint64_t)(r >> 64);
+}
+void or_to_onr2(uint64_t lo, uint64_t hi, int32_t M, int32_t* onr) {
+    int tmp[128];
+    to_onr(join(lo, hi), M, tmp);
+    for (int i = 0; i < M; ++i) onr[i] = tmp[i];
+}
+int64_t or_num_offdiagonals2(const or_model* h, uint64_t lo, uint64_t hi) {
+    return num_offdiagonals_t(h, join(lo, hi));
+}
+double or_diagonal_element2(const or_model* h, uint64_t lo, uint64_t hi) {
+    return diagonal_element_t(h, join(lo, hi));
+}
+void or_get_offdiagonal2(const or_model* h, uint64_t lo, uint64_t hi, int64_t chosen, double* value,
+                         uint64_t* out) {
+    const u128 r = get_offdiagonal_t(h, join(lo, hi), chosen, value);
+    out[0] = (uint64_t)r;
+    out[1] = (uint64_t)(r >> 64);
 }
 
 }  // extern "C"
@@ -376,16 +449,16 @@ inline void cdz(V res, V val, Stats& st) {
 
 // apply_column! of one entry.  emit(addr, value) receives every deposit in the order the
 // reference performs them: the diagonal deposit first, then the spawns.
-template <class V, class Emit>
-void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit) {
-    BoseIndex occ[64];
+template <class V, class A, class Emit>
+void apply_column(const Params& p, A addr, V val, Stats& st, Emit&& emit) {
+    BoseIndex occ[128];
     const int nocc = occupied_modes(addr, occ);
     const or_model* h = p.h;
     const int style = p.s->style;
 
     // diagonal_step!, spawning.jl:73-77; diagonal_element(T, add) fciqmc.jl:93-96.
     {
-        const double hd = or_diagonal_element(h, addr);
+        const double hd = diagonal_element_t(h, addr);
         const double d = 1 - p.dtau * (hd - p.shift);
         const double x = d * (double)val;
         if constexpr (std::is_integral<V>::value) {
@@ -405,15 +478,15 @@ void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit)
     }
 
     // offdiagonals(T, add), fciqmc.jl:104-112: matrix elements -dt * H_ba.
-    const int64_t n_od = or_num_offdiagonals(h, addr);
+    const int64_t n_od = num_offdiagonals_t(h, addr);
     auto offdiag = [&](int64_t chosen, double* mel) {
         double hv;
-        const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, chosen, &hv)
-                                            : mom1d_offdiagonal(h, addr, occ, nocc, chosen, &hv);
+        const A nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, chosen, &hv)
+                                     : mom1d_offdiagonal(h, addr, occ, nocc, chosen, &hv);
         *mel = -hv * p.dtau;
         return nb;
     };
-    auto deposit = [&](uint64_t nb, double x, double U) {
+    auto deposit = [&](A nb, double x, double U) {
         if constexpr (std::is_integral<V>::value) {
             const V nv = (V)project_int(x, U);
             if (nv != 0) emit(nb, nv);
@@ -428,7 +501,7 @@ void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit)
     auto exact = [&]() {
         for (int64_t j = 0; j < n_od; ++j) {
             double mel;
-            const uint64_t nb = offdiag(j + 1, &mel);
+            const A nb = offdiag(j + 1, &mel);
             double U = 0;
             if (p.s->proj_threshold > 0) { P4 r = draw(p.seed, addr, (uint32_t)j, p.step, TAG_SPAWN); U = u53(r.v[1], r.v[2]); }
             deposit(nb, (double)val * mel, U);
@@ -443,7 +516,7 @@ void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit)
         for (int64_t j = 0; j < n_att; ++j) {
             P4 r = draw(p.seed, addr, (uint32_t)j, p.step, TAG_SPAWN);
             double mel;
-            const uint64_t nb = offdiag(pick1(r.v[0], n_od), &mel);
+            const A nb = offdiag(pick1(r.v[0], n_od), &mel);
             deposit(nb, mel * magnitude / prob, u53(r.v[1], r.v[2]));
         }
         st.attempts += (double)n_att;
@@ -464,11 +537,30 @@ void apply_column(const Params& p, uint64_t addr, V val, Stats& st, Emit&& emit)
     }
 }
 
+// Hashing and the (lo, hi) word layout of addresses at the C interface.
+struct KeyHash {
+    size_t operator()(uint64_t x) const { return std::hash<uint64_t>()(x); }
+    size_t operator()(u128 x) const {
+        return std::hash<uint64_t>()((uint64_t)x ^ ((uint64_t)(x >> 64) * 0x9E3779B97F4A7C15ull));
+    }
+};
+template <class A> inline A load_key(const uint64_t* keys, int64_t i);
+template <> inline uint64_t load_key<uint64_t>(const uint64_t* keys, int64_t i) { return keys[i]; }
+template <> inline u128 load_key<u128>(const uint64_t* keys, int64_t i) {
+    return ((u128)keys[2 * i + 1] << 64) | keys[2 * i];
+}
+inline void store_key(uint64_t* out, int64_t i, uint64_t k) { out[i] = k; }
+inline void store_key(uint64_t* out, int64_t i, u128 k) {
+    out[2 * i] = (uint64_t)k;
+    out[2 * i + 1] = (uint64_t)(k >> 64);
+}
+template <class A, class V> using KeyMap = std::unordered_map<A, V, KeyHash>;
+
 // DVec storage: Dict semantics of dvec.jl:126-136 (a value that becomes zero is deleted).
-template <class V>
+template <class V, class A = uint64_t>
 struct DictVec {
-    std::unordered_map<uint64_t, V> m;
-    void deposit(uint64_t k, V v) {  // deposit!, Interfaces/dictvectors.jl:49-51
+    KeyMap<A, V> m;
+    void deposit(A k, V v) {  // deposit!, Interfaces/dictvectors.jl:49-51
         auto it = m.find(k);
         if (it == m.end()) {
             if (v != 0) m.emplace(k, V(0) + v);
@@ -479,19 +571,20 @@ struct DictVec {
     }
 };
 
-template <class V>
-int64_t dump_sorted(const std::unordered_map<uint64_t, V>& m, uint64_t* keys, V* vals, int64_t cap) {
-    std::vector<std::pair<uint64_t, V>> items(m.begin(), m.end());
+template <class V, class Map>
+int64_t dump_sorted(const Map& m, uint64_t* keys, V* vals, int64_t cap) {
+    using A = typename Map::key_type;
+    std::vector<std::pair<A, V>> items(m.begin(), m.end());
     std::sort(items.begin(), items.end(), [](auto& a, auto& b) { return a.first < b.first; });
     const int64_t n = (int64_t)items.size();
     if (n > cap) return -n;
-    for (int64_t i = 0; i < n; ++i) { keys[i] = items[i].first; vals[i] = items[i].second; }
+    for (int64_t i = 0; i < n; ++i) { store_key(keys, i, items[i].first); vals[i] = items[i].second; }
     return n;
 }
 
 // ThresholdCompression, compression.jl:18-33.
-template <class V>
-void compress(const Params& p, std::unordered_map<uint64_t, V>& m) {
+template <class V, class Map>
+void compress(const Params& p, Map& m) {
     if (p.s->compression != 1) return;
     if constexpr (!std::is_integral<V>::value) {
         const double th = p.s->compression_threshold;
@@ -507,23 +600,23 @@ void compress(const Params& p, std::unordered_map<uint64_t, V>& m) {
     }
 }
 
-template <class V>
+template <class A, class V>
 int64_t step_impl(const Params& p, int order, const uint64_t* keys, const V* vals, int64_t n,
                   uint64_t* out_keys, V* out_vals, int64_t cap, double* stats) {
-    DictVec<V> w;
+    DictVec<V, A> w;
     Stats st;
     if (order == 0) {
         // apply_operator!, Interfaces/dictvectors.jl:112-123: one column at a time.
         for (int64_t i = 0; i < n; ++i)
-            apply_column<V>(p, keys[i], vals[i], st, [&](uint64_t k, V v) { w.deposit(k, v); });
+            apply_column<V>(p, load_key<A>(keys, i), vals[i], st, [&](A k, V v) { w.deposit(k, v); });
     } else {
         // Same deposits, reordered into a list: every diagonal deposit first (entry order),
         // then every spawn (entry, attempt order).  This is the list the GPU engine merges.
-        std::vector<std::pair<uint64_t, V>> diag, spawns;
+        std::vector<std::pair<A, V>> diag, spawns;
         for (int64_t i = 0; i < n; ++i) {
             bool first = true;
-            const uint64_t self = keys[i];
-            apply_column<V>(p, keys[i], vals[i], st, [&](uint64_t k, V v) {
+            const A self = load_key<A>(keys, i);
+            apply_column<V>(p, self, vals[i], st, [&](A k, V v) {
                 if (first && k == self) diag.emplace_back(k, v); else spawns.emplace_back(k, v);
                 first = false;
             });
@@ -576,13 +669,13 @@ V from_initiator_value(int rule, double thr, const IValue<V>& x) {
     }
 }
 
-template <class V>
+template <class V, class A>
 struct InitiatorMemory {
-    std::unordered_map<uint64_t, IValue<V>> m;
+    KeyMap<A, IValue<V>> m;
     int rule;
     double thr;
     // deposit!(w, add, val, p_add => p_val) with to_initiator_value
-    void deposit(uint64_t add, V val, uint64_t p_add, V p_val) {
+    void deposit(A add, V val, A p_add, V p_val) {
         IValue<V> d;
         const bool big = std::fabs((double)p_val) > thr;
         if (rule == 0) d.safe = val;
@@ -600,28 +693,28 @@ struct InitiatorMemory {
     }
 };
 
-template <class V>
+template <class A, class V>
 int64_t step_initiator_impl(const Params& p, int order, int rule, double thr, const uint64_t* keys,
                             const V* vals, int64_t n, uint64_t* out_keys, V* out_vals, int64_t cap,
                             double* stats) {
-    InitiatorMemory<V> w;
+    InitiatorMemory<V, A> w;
     w.rule = rule;
     w.thr = thr;
     Stats st;
     if (order == 0) {
         for (int64_t i = 0; i < n; ++i) {
-            const uint64_t self = keys[i];
+            const A self = load_key<A>(keys, i);
             const V pv = vals[i];
-            apply_column<V>(p, self, pv, st, [&](uint64_t k, V v) { w.deposit(k, v, self, pv); });
+            apply_column<V>(p, self, pv, st, [&](A k, V v) { w.deposit(k, v, self, pv); });
         }
     } else {
         // list order: every diagonal deposit first (entry order), then every spawn
-        struct Dep { uint64_t k; V v; uint64_t pk; V pv; };
+        struct Dep { A k; V v; A pk; V pv; };
         std::vector<Dep> diag, spawns;
         for (int64_t i = 0; i < n; ++i) {
-            const uint64_t self = keys[i];
+            const A self = load_key<A>(keys, i);
             const V pv = vals[i];
-            apply_column<V>(p, self, pv, st, [&](uint64_t k, V v) {
+            apply_column<V>(p, self, pv, st, [&](A k, V v) {
                 if (k == self) diag.push_back({k, v, self, pv}); else spawns.push_back({k, v, self, pv});
             });
         }
@@ -630,7 +723,7 @@ int64_t step_initiator_impl(const Params& p, int order, int rule, double thr, co
     }
     // move_and_compress!
     const double len_before = (double)w.m.size();
-    std::unordered_map<uint64_t, V> out;
+    KeyMap<A, V> out;
     const bool threshold = p.s->compression == 1 && !std::is_integral<V>::value;
     for (auto& kv : w.m) {
         V val = from_initiator_value<V>(rule, thr, kv.second);
@@ -667,15 +760,23 @@ int64_t or_step_initiator(const or_model* h, const or_style* s, uint64_t seed, i
                           int32_t rule, double threshold, const uint64_t* keys, const void* vals,
                           int64_t n, uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
     Params p{h, s, seed, step, shift, dtau, boost};
+    if (h->M + h->N - 1 > 64) {
+        if (vtype == 0)
+            return step_initiator_impl<u128, int64_t>(p, order, rule, threshold, keys, (const int64_t*)vals,
+                                                      n, out_keys, (int64_t*)out_vals, cap, stats);
+        return step_initiator_impl<u128, double>(p, order, rule, threshold, keys, (const double*)vals, n,
+                                                 out_keys, (double*)out_vals, cap, stats);
+    }
     if (vtype == 0)
-        return step_initiator_impl<int64_t>(p, order, rule, threshold, keys, (const int64_t*)vals, n,
-                                            out_keys, (int64_t*)out_vals, cap, stats);
-    return step_initiator_impl<double>(p, order, rule, threshold, keys, (const double*)vals, n,
-                                       out_keys, (double*)out_vals, cap, stats);
+        return step_initiator_impl<uint64_t, int64_t>(p, order, rule, threshold, keys, (const int64_t*)vals, n,
+                                                      out_keys, (int64_t*)out_vals, cap, stats);
+    return step_initiator_impl<uint64_t, double>(p, order, rule, threshold, keys, (const double*)vals, n,
+                                                 out_keys, (double*)out_vals, cap, stats);
 }
 
 // One FCIQMC step (apply_operator! + compress!) of the source vector (keys, vals).
-// vtype 0: int64 values (IsStochasticInteger); 1: double values.
+// vtype 0: int64 values (IsStochasticInteger); 1: double values.  Keys: one uint64 word per
+// address, or two (lo, hi) when N + M - 1 > 64 (also for out_keys).
 // order 0: deposits in the reference's column order; 1: list order (diagonal first).
 // stats[0..9] = attempts, exact_steps, inexact_steps, spawns, deaths, clones, zombies,
 //               len_before, len, norm.  Returns the output length, or -(needed) if cap
@@ -685,9 +786,18 @@ int64_t or_step(const or_model* h, const or_style* s, uint64_t seed, int64_t ste
                 const uint64_t* keys, const void* vals, int64_t n,
                 uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
     Params p{h, s, seed, step, shift, dtau, boost};
+    if (h->M + h->N - 1 > 64) {   // two-chunk addresses: keys hold (lo, hi) words
+        if (vtype == 0)
+            return step_impl<u128, int64_t>(p, order, keys, (const int64_t*)vals, n, out_keys,
+                                            (int64_t*)out_vals, cap, stats);
+        return step_impl<u128, double>(p, order, keys, (const double*)vals, n, out_keys,
+                                       (double*)out_vals, cap, stats);
+    }
     if (vtype == 0)
-        return step_impl<int64_t>(p, order, keys, (const int64_t*)vals, n, out_keys, (int64_t*)out_vals, cap, stats);
-    return step_impl<double>(p, order, keys, (const double*)vals, n, out_keys, (double*)out_vals, cap, stats);
+        return step_impl<uint64_t, int64_t>(p, order, keys, (const int64_t*)vals, n, out_keys,
+                                            (int64_t*)out_vals, cap, stats);
+    return step_impl<uint64_t, double>(p, order, keys, (const double*)vals, n, out_keys,
+                                       (double*)out_vals, cap, stats);
 }
 
 // Annihilation of a spawn list: a left fold of deposit! over the list, dvec.jl:126-136.
@@ -734,42 +844,63 @@ double or_density_diagonal(uint64_t bs, int32_t M, int32_t mode) {
     return (double)onr[mode - 1];
 }
 
-// out[0] = dot(t, u); out[1 + o] = dot(t, op_o, u) for the nops operators (kinds[o], params[o],
-// models[o]).  tv / uv are int64 (vtype 0) or double (vtype 1) per vector.  Returns 0.
-int32_t or_overlaps(const uint64_t* tk, const void* tv, int32_t t_vtype, int64_t nt,
-                    const uint64_t* uk, const void* uv, int32_t u_vtype, int64_t nu,
-                    int32_t M, int32_t nops, const int32_t* kinds, const int32_t* params,
-                    const or_model* models, double* out) {
-    std::unordered_map<uint64_t, double> t;
+}  // extern "C"
+
+namespace {
+template <class A>
+void overlaps_impl(const uint64_t* tk, const void* tv, int32_t t_vtype, int64_t nt, const uint64_t* uk,
+                   const void* uv, int32_t u_vtype, int64_t nu, int32_t M, int32_t nops,
+                   const int32_t* kinds, const int32_t* params, const or_model* models, double* out) {
+    KeyMap<A, double> t;
     t.reserve((size_t)nt * 2 + 1);
     for (int64_t i = 0; i < nt; ++i)
-        t[tk[i]] = t_vtype == 0 ? (double)((const int64_t*)tv)[i] : ((const double*)tv)[i];
-    auto T = [&](uint64_t k) { auto it = t.find(k); return it == t.end() ? 0.0 : it->second; };
+        t[load_key<A>(tk, i)] = t_vtype == 0 ? (double)((const int64_t*)tv)[i] : ((const double*)tv)[i];
+    auto T = [&](A k) { auto it = t.find(k); return it == t.end() ? 0.0 : it->second; };
     for (int o = 0; o <= nops; ++o) out[o] = 0;
     for (int64_t i = 0; i < nu; ++i) {
-        const uint64_t k = uk[i];
+        const A k = load_key<A>(uk, i);
         const double v = u_vtype == 0 ? (double)((const int64_t*)uv)[i] : ((const double*)uv)[i];
         const double tk_ = T(k);
         out[0] += tk_ * v;
+        int onr[128];
+        to_onr(k, M, onr);
         for (int o = 0; o < nops; ++o) {
             double res;
             if (kinds[o] == 2) {
-                res = tk_ * or_g2_real(k, M, params[o]) * v;
+                const int dd = ((params[o] % M) + M) % M;
+                int64_t acc = 0;
+                for (int j = 0; j < M; ++j)
+                    acc += (int64_t)onr[j] * (dd == 0 ? onr[j] - 1 : onr[(j + dd) % M]);
+                res = tk_ * ((double)acc / (double)M) * v;
             } else if (kinds[o] == 3) {
-                res = tk_ * or_density_diagonal(k, M, params[o]) * v;
+                res = tk_ * (double)onr[params[o] - 1] * v;
             } else {
                 const or_model* h = &models[o];
-                res = tk_ * or_diagonal_element(h, k) * v;
-                const int64_t n = or_num_offdiagonals(h, k);
+                res = tk_ * diagonal_element_t(h, k) * v;
+                const int64_t n = num_offdiagonals_t(h, k);
                 for (int64_t c = 1; c <= n; ++c) {
                     double hv;
-                    const uint64_t nb = or_get_offdiagonal(h, k, c, &hv);
+                    const A nb = get_offdiagonal_t(h, k, c, &hv);
                     res += T(nb) * hv * v;
                 }
             }
             out[1 + o] += res;
         }
     }
+}
+}  // namespace
+
+extern "C" {
+
+// out[0] = dot(t, u); out[1 + o] = dot(t, op_o, u) for the nops operators (kinds[o], params[o],
+// models[o]).  tv / uv are int64 (vtype 0) or double (vtype 1) per vector; kw = words per
+// address (1, or 2 for N + M - 1 > 64).  Returns 0.
+int32_t or_overlaps(const uint64_t* tk, const void* tv, int32_t t_vtype, int64_t nt,
+                    const uint64_t* uk, const void* uv, int32_t u_vtype, int64_t nu,
+                    int32_t M, int32_t kw, int32_t nops, const int32_t* kinds, const int32_t* params,
+                    const or_model* models, double* out) {
+    if (kw == 2) overlaps_impl<u128>(tk, tv, t_vtype, nt, uk, uv, u_vtype, nu, M, nops, kinds, params, models, out);
+    else overlaps_impl<uint64_t>(tk, tv, t_vtype, nt, uk, uv, u_vtype, nu, M, nops, kinds, params, models, out);
     return 0;
 }
 

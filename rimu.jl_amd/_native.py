@@ -177,13 +177,22 @@ def overlaps(left: "Engine", right: "Engine", ops=()) -> np.ndarray:
     return np.array(out[:], dtype=np.float64)
 
 
+def model_key_words(model: Model) -> int:
+    return 2 if model.M + model.N - 1 > 64 else 1
+
+
 def hamiltonian_eval(model: Model, addrs, chosen):
+    """Diagonal, number of off-diagonals and one chosen off-diagonal per address; addresses
+    are word arrays ((n,) or (n, 2))."""
+    kw = model_key_words(model)
     addrs = np.ascontiguousarray(addrs, dtype=np.uint64)
+    if kw == 2:
+        addrs = addrs.reshape(-1, 2)
     chosen = np.ascontiguousarray(chosen, dtype=np.int64)
     n = len(addrs)
     diag = np.zeros(max(n, 1))
     nod = np.zeros(max(n, 1), dtype=np.int64)
-    na = np.zeros(max(n, 1), dtype=np.uint64)
+    na = np.zeros((max(n, 1), 2) if kw == 2 else max(n, 1), dtype=np.uint64)
     ov = np.zeros(max(n, 1))
     check(lib().rimu_hamiltonian_eval(ctypes.byref(model), _p(addrs), _p(chosen), n, _p(diag),
                                       _p(nod), _p(na), _p(ov)))
@@ -198,6 +207,7 @@ class Engine:
         self.model = model
         self.style = style
         self.int_vals = style.kind == STYLE_INTEGER
+        self.kw = model_key_words(model)
         self.max_entries = int(max_entries)
         self.max_spawns = int(max_spawns)
         check(lib().rimu_engine_create(ctypes.byref(model), ctypes.byref(style),
@@ -226,11 +236,18 @@ class Engine:
 
     def upload(self, keys, vals):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        if self.kw == 2:
+            keys = keys.reshape(-1, 2)
+            ascending = len(keys) < 2 or bool(np.all(
+                (keys[1:, 1] > keys[:-1, 1]) | ((keys[1:, 1] == keys[:-1, 1]) & (keys[1:, 0] > keys[:-1, 0]))))
+        else:
+            ascending = len(keys) < 2 or bool(np.all(keys[1:] > keys[:-1]))
         vals = np.ascontiguousarray(vals, dtype=self.vdtype)
-        if len(keys) > 1 and not np.all(keys[1:] > keys[:-1]):
-            order = np.argsort(keys, kind="stable")
-            keys, vals = keys[order], vals[order]
-            if np.any(keys[1:] == keys[:-1]):
+        if not ascending:
+            order = np.argsort(keys, kind="stable") if self.kw == 1 else np.lexsort((keys[:, 0], keys[:, 1]))
+            keys, vals = np.ascontiguousarray(keys[order]), vals[order]
+            same = (keys[1:] == keys[:-1]) if self.kw == 1 else np.all(keys[1:] == keys[:-1], axis=1)
+            if np.any(same):
                 raise ValueError("duplicate keys in vector")
         check(lib().rimu_vector_upload(self._h, _p(keys), _p(vals), len(keys)))
 
@@ -241,7 +258,7 @@ class Engine:
 
     def download(self):
         n = len(self)
-        keys = np.zeros(max(n, 1), dtype=np.uint64)
+        keys = np.zeros((max(n, 1), 2) if self.kw == 2 else max(n, 1), dtype=np.uint64)
         vals = np.zeros(max(n, 1), dtype=self.vdtype)
         out = ctypes.c_int64(0)
         check(lib().rimu_vector_download(self._h, _p(keys), _p(vals), max(n, 1), ctypes.byref(out)))

@@ -1,12 +1,46 @@
-"""``BoseFS`` addresses on one 64-bit chunk.
+"""``BoseFS`` addresses on one or two 64-bit chunks.
 
-Layout of ``BitStringAddresses/bitstring.jl:464-471`` (``from_bose_onr``): mode 1 in the
+Layout of ``BitStringAddresses/bitstring.jl:464-508`` (``from_bose_onr``): mode 1 in the
 least significant bits, mode i written as n_i one-bits followed by a zero separator, B =
-N + M - 1 bits in total.  Constructors follow ``BitStringAddresses/bosefs.jl:54-112``.
+N + M - 1 bits in total (at most 128 here).  Constructors follow
+``BitStringAddresses/bosefs.jl:54-112``.  Arrays of addresses for the device are uint64
+words: shape (n,) for B <= 64, (n, 2) with the low word first for 64 < B <= 128.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+def key_words(M: int, N: int) -> int:
+    """64-bit words per address of N bosons in M modes: 1 or 2."""
+    return 2 if M + N - 1 > 64 else 1
+
+
+def pack_keys(ints, kw: int) -> np.ndarray:
+    """Python-int bit strings as a word array ((n,) or (n, 2) [lo, hi])."""
+    ints = [int(x) for x in ints]
+    if kw == 1:
+        return np.array(ints, dtype=np.uint64)
+    return np.array([[x & MASK64, x >> 64] for x in ints], dtype=np.uint64).reshape(len(ints), 2)
+
+
+def unpack_keys(arr) -> list:
+    """The inverse of pack_keys: a list of Python ints."""
+    arr = np.asarray(arr, dtype=np.uint64)
+    if arr.ndim == 1:
+        return [int(x) for x in arr.tolist()]
+    return [int(lo) | (int(hi) << 64) for lo, hi in arr.tolist()]
+
+
+def sort_keys(keys: np.ndarray) -> np.ndarray:
+    """Stable ascending order of a word array (numeric order of the bit strings)."""
+    if keys.ndim == 1:
+        return np.argsort(keys, kind="stable")
+    return np.lexsort((keys[:, 0], keys[:, 1]))
 
 
 def onr_to_bits(onr) -> int:
@@ -56,8 +90,8 @@ class BoseFS:
             raise ValueError(f"invalid ONR: {M} modes expected, {len(onr)} given")
         if N is not None and sum(onr) != N:
             raise ValueError(f"invalid ONR: {N} particles expected, {sum(onr)} given")
-        if len(onr) + sum(onr) - 1 > 64:
-            raise ValueError("address does not fit one 64-bit chunk (N + M - 1 > 64)")
+        if len(onr) + sum(onr) - 1 > 128:
+            raise ValueError("address does not fit two 64-bit chunks (N + M - 1 > 128)")
         object.__setattr__(self, "N", sum(onr))
         object.__setattr__(self, "M", len(onr))
         object.__setattr__(self, "bits", onr_to_bits(onr))
@@ -74,6 +108,10 @@ class BoseFS:
     @property
     def onr(self):
         return bits_to_onr(self.bits, self.M)
+
+    @property
+    def key_words(self) -> int:
+        return key_words(self.M, self.N)
 
     def num_occupied_modes(self) -> int:
         return sum(1 for n in self.onr if n > 0)

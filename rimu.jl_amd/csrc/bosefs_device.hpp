@@ -1,9 +1,11 @@
 // bosefs_device.hpp -- BoseFS addresses, Hubbard matrix elements and Philox streams,
 // written as register-only bit arithmetic for one wavefront lane.
 //
-// Address layout (BitStringAddresses/bitstring.jl:464-471): one 64-bit chunk holds
-// B = N + M - 1 bits; mode 1 sits in the least significant bits and every mode is n_i
-// one-bits followed by a zero separator.  A "run" is the block of ones of one mode.
+// Address layout (BitStringAddresses/bitstring.jl:464-508): B = N + M - 1 bits in one
+// 64-bit chunk, or in two (B <= 128) held as one 128-bit integer with the last BitString
+// chunk (the least significant bits) low; mode 1 sits in the least significant bits and
+// every mode is n_i one-bits followed by a zero separator.  A "run" is the block of ones of
+// one mode.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,6 +46,15 @@ __host__ __device__ inline U4 draw(uint64_t seed, uint64_t addr, uint32_t attemp
                          (uint32_t)(seed >> 32));
 }
 
+// Two-chunk addresses: the high chunk selects the Philox key (seed ^ hi * golden ratio, a
+// bijection of hi for a given seed), the low chunk the counter as above; hi = 0 gives the
+// one-chunk streams.
+__host__ __device__ inline U4 draw(uint64_t seed, unsigned __int128 addr, uint32_t attempt,
+                                   int64_t step, uint32_t tag) {
+    const uint64_t hi = (uint64_t)(addr >> 64);
+    return draw(seed ^ (hi * 0x9E3779B97F4A7C15ull), (uint64_t)addr, attempt, step, tag);
+}
+
 // 53-bit uniform double in [0, 1).
 __host__ __device__ inline double u53(uint32_t hi, uint32_t lo) {
     const uint64_t x = ((((uint64_t)hi) << 32) | lo) >> 11;
@@ -56,14 +67,38 @@ __host__ __device__ inline int64_t pick1(uint32_t r, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Bit helpers
+// Bit helpers.  An address is one unsigned integer A: uint64_t for B <= 64 bits, u128 (two
+// BitString chunks, the low chunk in the low 64 bits) for 64 < B <= 128.  Both layouts are
+// the same bit string: mode 1 in the least significant bits.
 // ---------------------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+
+template <class A> struct addr_bits { static constexpr int value = 8 * (int)sizeof(A); };
+
 __device__ inline int popc64(uint64_t x) { return __popcll(x); }
 __device__ inline int ctz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
 __device__ inline int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
 __device__ inline uint64_t low_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
 __device__ inline uint64_t shr(uint64_t x, int n) { return n >= 64 ? 0ull : (x >> n); }
 __device__ inline uint64_t shl(uint64_t x, int n) { return n >= 64 ? 0ull : (x << n); }
+
+__device__ inline int popcA(uint64_t x) { return __popcll(x); }
+__device__ inline int popcA(u128 x) { return __popcll((uint64_t)x) + __popcll((uint64_t)(x >> 64)); }
+__device__ inline int ctzA(uint64_t x) { return ctz64(x); }
+__device__ inline int ctzA(u128 x) {
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    return lo ? __builtin_ctzll(lo) : (hi ? 64 + __builtin_ctzll(hi) : 128);
+}
+__device__ inline int clzA(uint64_t x) { return clz64(x); }
+__device__ inline int clzA(u128 x) {
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    return hi ? __builtin_clzll(hi) : (lo ? 64 + __builtin_clzll(lo) : 128);
+}
+template <class A> __device__ inline A shrA(A x, int n) { return n >= addr_bits<A>::value ? (A)0 : (A)(x >> n); }
+template <class A> __device__ inline A shlA(A x, int n) { return n >= addr_bits<A>::value ? (A)0 : (A)(x << n); }
+template <class A> __device__ inline A low_maskA(int n) {
+    return n >= addr_bits<A>::value ? (A)~(A)0 : (A)(((A)1 << n) - (A)1);
+}
 
 // Position of the k-th (0-based) set bit of x; x must have more than k set bits.
 __device__ inline int select_bit(uint64_t x, int k) {
@@ -83,26 +118,31 @@ __device__ inline int select_bit(uint64_t x, int k) {
     if (k >= c) { pos += 1; }
     return pos;
 }
+__device__ inline int select_bit(u128 x, int k) {
+    const uint64_t lo = (uint64_t)x;
+    const int c = __popcll(lo);
+    return k < c ? select_bit(lo, k) : 64 + select_bit((uint64_t)(x >> 64), k - c);
+}
 
 // Number of consecutive ones starting at bit p upwards.
-__device__ inline int ones_from(uint64_t bits, int p) { return ctz64(~shr(bits, p)); }
+template <class A> __device__ inline int ones_from(A bits, int p) { return ctzA((A)~shrA(bits, p)); }
 
 // First bit of the run of mode m (1-based): modes are separated by zeros, so the run of
 // mode m starts right after the (m-1)-th zero.
-__device__ inline int mode_start(uint64_t bits, int m) {
-    return m == 1 ? 0 : select_bit(~bits, m - 2) + 1;
+template <class A> __device__ inline int mode_start(A bits, int m) {
+    return m == 1 ? 0 : select_bit((A)~bits, m - 2) + 1;
 }
 
 // Run starts: a one whose lower neighbour is zero (or bit 0).
-__device__ inline uint64_t run_starts(uint64_t bits) { return bits & ~(bits << 1); }
+template <class A> __device__ inline A run_starts(A bits) { return bits & (A)~(bits << 1); }
 
 // Delete the bit at position p (higher bits move down).
-__device__ inline uint64_t delete_bit(uint64_t x, int p) {
-    return (x & low_mask(p)) | shl(shr(x, p + 1), p);
+template <class A> __device__ inline A delete_bit(A x, int p) {
+    return (x & low_maskA<A>(p)) | shlA(shrA(x, p + 1), p);
 }
 // Insert a one at position p (bits at p and above move up).
-__device__ inline uint64_t insert_one(uint64_t x, int p) {
-    return (x & low_mask(p)) | (1ull << p) | shl(shr(x, p), p + 1);
+template <class A> __device__ inline A insert_one(A x, int p) {
+    return (x & low_maskA<A>(p)) | ((A)1 << p) | shlA(shrA(x, p), p + 1);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -116,46 +156,46 @@ struct ModelDev {
 };
 
 // Sum over occupied modes of n(n-1) (bose_hubbard_interaction, bosefs.jl:415-428).
-__device__ inline int64_t bose_hubbard_interaction(uint64_t bits) {
+template <class A> __device__ inline int64_t bose_hubbard_interaction(A bits) {
     int64_t acc = 0;
-    uint64_t x = bits;
+    A x = bits;
     while (x) {
-        x = shr(x, ctz64(x));
-        const int n = ctz64(~x);
+        x = shrA(x, ctzA(x));
+        const int n = ctzA((A)~x);
         acc += (int64_t)n * (n - 1);
-        x = shr(x, n);
+        x = shrA(x, n);
     }
     return acc;
 }
 
-__device__ inline int num_occupied(uint64_t bits) { return popc64(run_starts(bits)); }
+template <class A> __device__ inline int num_occupied(A bits) { return popcA(run_starts(bits)); }
 
 // num_offdiagonals: HubbardReal1D.jl:51-53 (2 * occupied modes);
 // HubbardMom1D.jl:131-140 (s(s-1)(M-2) + d(M-1)).
-__device__ inline int64_t num_offdiagonals(const ModelDev& h, uint64_t bits) {
-    const uint64_t st = run_starts(bits);
-    const int64_t s = popc64(st);
+template <class A> __device__ inline int64_t num_offdiagonals(const ModelDev& h, A bits) {
+    const A st = run_starts(bits);
+    const int64_t s = popcA(st);
     if (h.kind == 1) return 2 * s;
-    const int64_t d = popc64(st & (bits >> 1));  // runs of length >= 2
+    const int64_t d = popcA((A)(st & (bits >> 1)));  // runs of length >= 2
     return s * (s - 1) * (h.M - 2) + d * (h.M - 1);
 }
 
 // diagonal_element: HubbardReal1D.jl:55-57; HubbardMom1D.jl:174-177 (kinetic sum over
 // occupied modes in ascending order, then u/(2M) * onproduct, excitations.jl:126-137).
-__device__ inline double diagonal_element(const ModelDev& h, uint64_t bits) {
+template <class A> __device__ inline double diagonal_element(const ModelDev& h, A bits) {
     if (h.kind == 1) return h.u * (double)bose_hubbard_interaction(bits) / 2;
     double kin = 0.0;
     int64_t sumsq = 0;
-    uint64_t x = bits;
+    A x = bits;
     int mode = 1;
     while (x) {
-        const int z = ctz64(x);
+        const int z = ctzA(x);
         mode += z;
-        x = shr(x, z);
-        const int n = ctz64(~x);
+        x = shrA(x, z);
+        const int n = ctzA((A)~x);
         kin += h.kes[mode - 1] * n;
         sumsq += (int64_t)n * n;
-        x = shr(x, n);
+        x = shrA(x, n);
         // the separator after this run is consumed by the next ctz (counts as one mode step)
     }
     const int64_t N = h.N;
@@ -166,36 +206,38 @@ __device__ inline double diagonal_element(const ModelDev& h, uint64_t bits) {
 // hopnextneighbour (bosefs.jl:297-345): the neighbour address and the occupation product
 // under the square root.  chosen in 1..2*s: site (chosen+1)/2 is the site-th occupied mode;
 // odd hops right.  M modes, B = N + M - 1 bits.
-__device__ inline uint64_t real1d_hop(uint64_t bits, int64_t chosen, int M, int B, int64_t* prod_out) {
+template <class A>
+__device__ inline A real1d_hop(A bits, int64_t chosen, int M, int B, int64_t* prod_out) {
+    constexpr int W = addr_bits<A>::value;
     const int site = (int)((chosen + 1) >> 1);
-    const uint64_t st = run_starts(bits);
+    const A st = run_starts(bits);
     const int p = select_bit(st, site - 1);
     const int n = ones_from(bits, p);
-    const int m = p - popc64(bits & low_mask(p)) + 1;  // mode = zeros below p, plus one
-    const uint64_t maskB = low_mask(B);
-    uint64_t nb;
+    const int m = p - popcA((A)(bits & low_maskA<A>(p))) + 1;  // mode = zeros below p, plus one
+    const A maskB = low_maskA<A>(B);
+    A nb;
     int64_t prod;
     if (chosen & 1) {  // right
         if (m == M) {
-            nb = ((bits << 1) | 1ull) & maskB;
+            nb = ((bits << 1) | (A)1) & maskB;
             prod = (int64_t)n * (ones_from(bits, 0) + 1);
         } else {
             const int q = p + n - 1;
-            nb = bits ^ (3ull << q);
+            nb = bits ^ ((A)3 << q);
             prod = (int64_t)n * (ones_from(bits, q + 2) + 1);
         }
     } else {  // left
         if (m == 1) {
-            nb = (bits >> 1) | (1ull << (B - 1));
-            const int hz = 63 - clz64(~bits & maskB);  // highest zero inside the B bits
+            nb = (bits >> 1) | ((A)1 << (B - 1));
+            const int hz = W - 1 - clzA((A)(~bits & maskB));  // highest zero inside the B bits
             prod = (int64_t)n * ((B - 1 - hz) + 1);
         } else {
-            nb = bits ^ (3ull << (p - 1));
+            nb = bits ^ ((A)3 << (p - 1));
             int nprev = 0;
             if (p >= 2) {
-                const uint64_t below = bits & low_mask(p - 1);  // bits 0 .. p-2
-                const uint64_t z = ~below & low_mask(p - 1);
-                nprev = z ? (p - 2 - (63 - clz64(z))) : (p - 1);
+                const A below = bits & low_maskA<A>(p - 1);  // bits 0 .. p-2
+                const A z = (A)~below & low_maskA<A>(p - 1);
+                nprev = z ? (p - 2 - (W - 1 - clzA(z))) : (p - 1);
             }
             prod = (int64_t)n * (nprev + 1);
         }
@@ -205,10 +247,10 @@ __device__ inline uint64_t real1d_hop(uint64_t bits, int64_t chosen, int M, int 
 }
 
 // get_offdiagonal of HubbardReal1D: value -t*sqrt(prod) (HubbardReal1D.jl:59-62).
-__device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
-                                              double* value) {
+template <class A>
+__device__ inline A real1d_offdiagonal(const ModelDev& h, A bits, int64_t chosen, double* value) {
     int64_t prod;
-    const uint64_t nb = real1d_hop(bits, chosen, h.M, h.B, &prod);
+    const A nb = real1d_hop(bits, chosen, h.M, h.B, &prod);
     *value = -h.t * sqrt((double)prod);
     return nb;
 }
@@ -216,17 +258,17 @@ __device__ inline uint64_t real1d_offdiagonal(const ModelDev& h, uint64_t bits, 
 __device__ inline int imod1(int x, int m) { int r = ((x - 1) % m + m) % m; return r + 1; }
 
 // Occupation number of mode m.
-__device__ inline int occupation(uint64_t bits, int m) { return ones_from(bits, mode_start(bits, m)); }
+template <class A> __device__ inline int occupation(A bits, int m) { return ones_from(bits, mode_start(bits, m)); }
 
 // momentum_transfer_excitation (fold=true), excitations.jl:26-80, value u/(2M)*sqrt(...)
 // (HubbardMom1D.jl:189-194).  The map of occupied modes is read from the run starts.
-__device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
-                                             double* value) {
+template <class A>
+__device__ inline A mom1d_offdiagonal(const ModelDev& h, A bits, int64_t chosen, double* value) {
     // chosen <= s(s-1)(M-2) + d(M-1) < 2^31 for M <= 64: the index arithmetic runs in 32-bit
     // unsigned integers (64-bit division is a long software sequence on the GPU)
     const int M = h.M;
-    const uint64_t st = run_starts(bits);
-    const int s = popc64(st);
+    const A st = run_starts(bits);
+    const int s = popcA(st);
     const uint32_t c = (uint32_t)chosen;
     const uint32_t single = (uint32_t)(s * (s - 1) * (M - 2));
     int p_mode, q_mode;  // source modes (p <= q)
@@ -236,9 +278,9 @@ __device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, i
         const uint32_t m1 = (uint32_t)(M - 1);
         mom = (int)((dbl - 1) % m1) + 1;
         const uint32_t d = (dbl - 1) / m1 + 1;
-        const uint64_t dstarts = st & (bits >> 1);
+        const A dstarts = st & (bits >> 1);
         const int pos = select_bit(dstarts, (int)d - 1);
-        p_mode = q_mode = pos - popc64(bits & low_mask(pos)) + 1;
+        p_mode = q_mode = pos - popcA((A)(bits & low_maskA<A>(pos))) + 1;
     } else {
         const uint32_t m2 = (uint32_t)(M - 2), s1 = (uint32_t)(s - 1);
         mom = (int)((c - 1) % m2) + 1;
@@ -250,8 +292,8 @@ __device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, i
         else { f_hole = fst; s_hole = snd + 1; }
         const int pf = select_bit(st, (int)f_hole - 1);
         const int ps = select_bit(st, (int)s_hole - 1);
-        p_mode = pf - popc64(bits & low_mask(pf)) + 1;
-        q_mode = ps - popc64(bits & low_mask(ps)) + 1;
+        p_mode = pf - popcA((A)(bits & low_maskA<A>(pf))) + 1;
+        q_mode = ps - popcA((A)(bits & low_maskA<A>(ps))) + 1;
         if (mom >= q_mode - p_mode) mom += 1;
     }
     const int r_mode = imod1((int)(p_mode + mom), M);
@@ -267,7 +309,7 @@ __device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, i
     v *= (ns - (s_mode == q_mode) - (s_mode == p_mode) + 1);
     v *= (nr - (r_mode == q_mode) - (r_mode == p_mode) + (r_mode == s_mode) + 1);
     // new address: remove one boson from p and q, add one to r and s
-    uint64_t nb = bits;
+    A nb = bits;
     nb = delete_bit(nb, mode_start(nb, q_mode));
     nb = delete_bit(nb, mode_start(nb, p_mode));
     nb = insert_one(nb, mode_start(nb, s_mode));
@@ -276,8 +318,8 @@ __device__ inline uint64_t mom1d_offdiagonal(const ModelDev& h, uint64_t bits, i
     return nb;
 }
 
-__device__ inline uint64_t get_offdiagonal(const ModelDev& h, uint64_t bits, int64_t chosen,
-                                           double* value) {
+template <class A>
+__device__ inline A get_offdiagonal(const ModelDev& h, A bits, int64_t chosen, double* value) {
     return h.kind == 1 ? real1d_offdiagonal(h, bits, chosen, value)
                        : mom1d_offdiagonal(h, bits, chosen, value);
 }

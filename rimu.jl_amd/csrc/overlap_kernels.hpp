@@ -21,6 +21,7 @@ constexpr int OV_TILE = NT * OV_IPT;
 constexpr int OV_SPLIT = 1024;
 constexpr int OV_MAX_OPS = 8;
 constexpr int OV_LDS = 3072;   // t's keys of a tile's range staged in LDS when they fit
+template <class K> struct ov_lds { static constexpr int value = sizeof(K) == 8 ? OV_LDS : 2048; };
 
 enum { OP_HAMILTONIAN = 1, OP_G2_REAL = 2, OP_DENSITY = 3 };
 
@@ -34,16 +35,17 @@ static_assert(sizeof(OpDev) % 8 == 0, "operators are staged in LDS as 8-byte wor
 
 // G2RealCorrelator(d): (1/M) sum_i n_i (n_{i+d} - delta_{0d}), periodic
 // (correlation_functions.jl:82-92); the integer sum is exact, then one division.
-__device__ inline double g2_real(uint64_t bits, int M, int d) {
+template <class A>
+__device__ inline double g2_real(A bits, int M, int d) {
     const int dd = ((d % M) + M) % M;
     int64_t acc = 0;
-    uint64_t x = bits;
+    A x = bits;
     int mode = 1;
     while (x) {
-        const int z = ctz64(x);
+        const int z = ctzA(x);
         mode += z;
-        x = shr(x, z);
-        const int n = ctz64(~x);
+        x = shrA(x, z);
+        const int n = ctzA((A)~x);
         if (dd == 0) {
             acc += (int64_t)n * (n - 1);
         } else {
@@ -51,13 +53,14 @@ __device__ inline double g2_real(uint64_t bits, int M, int d) {
             if (j > M) j -= M;
             acc += (int64_t)n * occupation(bits, j);
         }
-        x = shr(x, n);
+        x = shrA(x, n);
     }
     return (double)acc / (double)M;
 }
 
 // Position of key in keys[a, b), or -1.
-__device__ inline int64_t find_in(const uint64_t* __restrict__ keys, int64_t a, int64_t b, uint64_t key) {
+template <class K>
+__device__ inline int64_t find_in(const K* __restrict__ keys, int64_t a, int64_t b, K key) {
     const int64_t end = b;
     while (a < b) {
         const int64_t mid = (a + b) >> 1;
@@ -67,8 +70,8 @@ __device__ inline int64_t find_in(const uint64_t* __restrict__ keys, int64_t a, 
 }
 
 // Position of key in keys[0, L), or -1; s_split[j] = keys[j * L / OV_SPLIT].
-__device__ inline int64_t find_split(const uint64_t* __restrict__ keys, int64_t L,
-                                     const uint64_t* s_split, uint64_t key) {
+template <class K>
+__device__ inline int64_t find_split(const K* __restrict__ keys, int64_t L, const K* s_split, K key) {
     int lo = 0, hi = OV_SPLIT;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -81,16 +84,17 @@ __device__ inline int64_t find_split(const uint64_t* __restrict__ keys, int64_t 
 }
 
 // One entry (key, v) of u against operator op: t[key] * diag * v plus the off-diagonal terms.
-__device__ __noinline__ double op_term(const OpDev& op, int M, uint64_t key, double tval, double v,
-                                       const uint64_t* __restrict__ tk, const double* tvd,
-                                       const int64_t* tvi, int64_t Lt, const uint64_t* s_split) {
+template <class K>
+__device__ __noinline__ double op_term(const OpDev& op, int M, K key, double tval, double v,
+                                       const K* __restrict__ tk, const double* tvd,
+                                       const int64_t* tvi, int64_t Lt, const K* s_split) {
     if (op.kind == OP_G2_REAL) return tval * g2_real(key, M, op.param) * v;
     if (op.kind == OP_DENSITY) return tval * (double)occupation(key, op.param) * v;
     double res = tval * diagonal_element(op.h, key) * v;
     const int64_t n = num_offdiagonals(op.h, key);
     for (int64_t c = 1; c <= n; ++c) {
         double hv;
-        const uint64_t nb = get_offdiagonal(op.h, key, c, &hv);
+        const K nb = get_offdiagonal(op.h, key, c, &hv);
         if (hv == 0.0) continue;
         const int64_t q = find_split(tk, Lt, s_split, nb);
         if (q >= 0) res += (tvd ? tvd[q] : (double)tvi[q]) * hv * v;
@@ -100,13 +104,14 @@ __device__ __noinline__ double op_term(const OpDev& op, int M, uint64_t key, dou
 
 // ranges[t] = lower bound in t of u's key at t * OV_TILE (t < ntiles), ranges[ntiles] = Lt.
 // u's keys are unique, so tile t's keys lie in t[ranges[t], ranges[t + 1]).
-__global__ __launch_bounds__(NT) void overlap_ranges_kernel(const uint64_t* __restrict__ tk, int64_t Lt,
-                                                            const uint64_t* __restrict__ uk,
+template <class K>
+__global__ __launch_bounds__(NT) void overlap_ranges_kernel(const K* __restrict__ tk, int64_t Lt,
+                                                            const K* __restrict__ uk,
                                                             int64_t ntiles, int64_t* __restrict__ ranges) {
     const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (t > ntiles) return;
     if (t == ntiles) { ranges[t] = Lt; return; }
-    const uint64_t key = uk[t * OV_TILE];
+    const K key = uk[t * OV_TILE];
     int64_t a = 0, b = Lt;
     while (a < b) {
         const int64_t mid = (a + b) >> 1;
@@ -117,31 +122,32 @@ __global__ __launch_bounds__(NT) void overlap_ranges_kernel(const uint64_t* __re
 
 // partials[o * gridDim.x + block], o = 0 (dot) .. nops (operators).  Lt >= 1, Lu >= 1.
 // need_split: an operator has off-diagonals (the splitters are only read then).
-template <class VT, class VU>
-__global__ __launch_bounds__(NT) void overlap_kernel(const uint64_t* __restrict__ tk,
+template <class K, class VT, class VU>
+__global__ __launch_bounds__(NT) void overlap_kernel(const K* __restrict__ tk,
                                                      const VT* __restrict__ tv, int64_t Lt,
-                                                     const uint64_t* __restrict__ uk,
+                                                     const K* __restrict__ uk,
                                                      const VU* __restrict__ uv, int64_t Lu,
                                                      const OpDev* __restrict__ ops, int nops, int M,
                                                      const int64_t* __restrict__ ranges, int need_split,
                                                      double* __restrict__ partials) {
-    __shared__ uint64_t s_split[OV_SPLIT];
+    constexpr int LDSN = ov_lds<K>::value;
+    __shared__ K s_split[OV_SPLIT];
     __shared__ OpDev s_ops[OV_MAX_OPS];
-    __shared__ uint64_t s_lk[OV_LDS];
+    __shared__ K s_lk[LDSN];
     const int64_t t0 = (int64_t)blockIdx.x * OV_TILE;
     if (need_split)
         for (int j = threadIdx.x; j < OV_SPLIT; j += NT) s_split[j] = tk[(int64_t)j * Lt / OV_SPLIT];
     for (int j = threadIdx.x; j < nops * (int)(sizeof(OpDev) / 8); j += NT)
         ((uint64_t*)s_ops)[j] = ((const uint64_t*)ops)[j];
     const int64_t lo = ranges[blockIdx.x], hi = ranges[blockIdx.x + 1];
-    const bool staged = hi - lo <= OV_LDS;
+    const bool staged = hi - lo <= LDSN;
     if (staged) {
         const int n = (int)(hi - lo);
         for (int j = threadIdx.x; j < n; j += NT) s_lk[j] = tk[lo + j];
     }
     __syncthreads();
 
-    uint64_t k[OV_IPT];
+    K k[OV_IPT];
     double v[OV_IPT];
 #pragma unroll
     for (int j = 0; j < OV_IPT; ++j) {
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(NT) void overlap_kernel(const uint64_t* __restrict_
         acc[0] += tval * v[j];
 #pragma unroll
         for (int o = 0; o < OV_MAX_OPS; ++o)
-            if (o < nops) acc[1 + o] += op_term(s_ops[o], M, k[j], tval, v[j], tk, tvd, tvi, Lt, s_split);
+            if (o < nops) acc[1 + o] += op_term<K>(s_ops[o], M, k[j], tval, v[j], tk, tvd, tvi, Lt, s_split);
     }
 #pragma unroll
     for (int o = 0; o <= OV_MAX_OPS; ++o) {

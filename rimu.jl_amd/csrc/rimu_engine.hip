@@ -81,11 +81,12 @@ struct rimu_engine {
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
+    int kw = 1;                     // 64-bit words per address: 1 (B <= 64) or 2 (B <= 128)
     int64_t max_entries = 0, max_spawns = 0;
     int32_t init_rule = 0;          // RIMU_INITIATOR_*
     double init_threshold = 0;
 
-    // vector (double buffered)
+    // vector (double buffered); key buffers hold kw words per address
     DevBuf<uint64_t> vkeys[2];
     DevBuf<unsigned char> vvals[2];
     int cur = 0;
@@ -119,8 +120,8 @@ struct rimu_engine {
         int64_t W;
         int64_t S;
         int64_t L_new;
-        unsigned long long key_or;
-        unsigned long long key_and;
+        u128 key_or;     // as wide as the widest key; a one-chunk key fills the low word
+        u128 key_and;
     };
     Misc* h_misc = nullptr;    // pinned
     double* h_stats = nullptr; // pinned, 12 doubles
@@ -147,10 +148,10 @@ int check_model(const rimu_model* m) {
     if (!m) return fail(RIMU_ERR_ARG, "model is null");
     if (m->kind != RIMU_MODEL_HUBBARD_REAL_1D && m->kind != RIMU_MODEL_HUBBARD_MOM_1D)
         return fail(RIMU_ERR_ARG, "unknown model kind");
-    if (m->M < 1 || m->N < 1 || m->M + m->N - 1 > 64)
-        return fail(RIMU_ERR_ARG, "address must fit one 64-bit chunk: need N + M - 1 <= 64");
-    if (m->kind == RIMU_MODEL_HUBBARD_MOM_1D && m->M < 3)
-        return fail(RIMU_ERR_ARG, "HubbardMom1D needs M >= 3");
+    if (m->M < 1 || m->N < 1 || m->M + m->N - 1 > 128)
+        return fail(RIMU_ERR_ARG, "address must fit two 64-bit chunks: need N + M - 1 <= 128");
+    if (m->kind == RIMU_MODEL_HUBBARD_MOM_1D && (m->M < 3 || m->M > 64))
+        return fail(RIMU_ERR_ARG, "HubbardMom1D needs 3 <= M <= 64");
     if (m->kind == RIMU_MODEL_HUBBARD_REAL_1D && m->M < 2)
         return fail(RIMU_ERR_ARG, "HubbardReal1D needs M >= 2");
     return RIMU_OK;
@@ -200,20 +201,21 @@ int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
 // whose digit is the same for every key.  Bits at and above nbits are carried, not sorted
 // (the initiator class bit).  Input in buffer `in`; returns the buffer index holding the
 // result.
-template <class V>
-int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, int* result) {
+template <class K, class V>
+int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result) {
     int src = in;
     if (n <= 1) { *result = src; return RIMU_OK; }
-    const int64_t tiles = ceil_div(n, SORT_TILE);
+    const int64_t tiles = ceil_div(n, sort_tile<K>::value);
     const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
     HIP_TRY(e->rchunks.alloc((size_t)nchunks * RADIX));
     for (int shift = 0; shift < nbits; shift += RADIX_BITS) {
-        if (((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
+        if ((uint32_t)((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
         const uint32_t dmask = shift + RADIX_BITS > nbits ? (1u << (nbits - shift)) - 1u : (uint32_t)(RADIX - 1);
         hipEvent_t ev{};
         e->timer.begin("radix_count", e->stream, &ev);
-        radix_count_kernel<<<tiles, NT, 0, e->stream>>>(e->skeys[src].p, n, shift, dmask, e->rcounts.p, tiles);
+        radix_count_kernel<K><<<tiles, NT, 0, e->stream>>>((const K*)e->skeys[src].p, n, shift, dmask,
+                                                           e->rcounts.p, tiles);
         e->timer.end(e->stream);
         e->timer.begin("radix_scan", e->stream, &ev);
         radix_chunk_sums_kernel<<<nchunks, RADIX, 0, e->stream>>>(e->rcounts.p, tiles, nchunks, e->rchunks.p);
@@ -221,9 +223,9 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
         radix_chunk_offsets_kernel<<<nchunks, RADIX, 0, e->stream>>>(e->rcounts.p, tiles, nchunks, e->rchunks.p);
         e->timer.end(e->stream);
         e->timer.begin("radix_scatter", e->stream, &ev);
-        radix_scatter_kernel<V><<<tiles, NT, 0, e->stream>>>(
-            e->skeys[src].p, (const V*)e->svals[src].p, n, shift, dmask, e->rcounts.p, tiles,
-            e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
+        radix_scatter_kernel<K, V><<<tiles, NT, 0, e->stream>>>(
+            (const K*)e->skeys[src].p, (const V*)e->svals[src].p, n, shift, dmask, e->rcounts.p, tiles,
+            (K*)e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
         src = 1 - src;
@@ -237,10 +239,11 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, uint64_t varying, int in, i
 // of the tile counts, compaction.  The new length lands in dm->L_new; `tiles` receives the
 // number of merge tiles (rows of merge_p).  With an initiator rule (a.init_rule != 0) the
 // spawn keys carry the class bit and bo holds the vector's values before the step.
-template <class V>
-int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
-               const uint64_t* bk, const V* bv, int64_t L, uint64_t* ok, V* ov, int64_t cap,
+template <class K, class V>
+int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int64_t S,
+               const K* bk, const V* bv, int64_t L, K* ok, V* ov, int64_t cap,
                rimu_engine::Misc* dm, int64_t* tiles, const V* bo = nullptr) {
+    constexpr int KW = sizeof(K) / 8;
     const bool init = a.init_rule != 0;
     if (init && !bo) return fail(RIMU_ERR_INTERNAL, "initiator merge without the old values");
     constexpr int64_t TILE = NT * MERGE_IPT;
@@ -248,42 +251,42 @@ int merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* a
     *tiles = nt;
     HIP_TRY(e->merge_p.alloc((size_t)nt * 3));
     HIP_TRY(e->splits.alloc(nt + 1));
-    HIP_TRY(e->prevk.alloc(nt + 1));
+    HIP_TRY(e->prevk.alloc((size_t)(nt + 1) * KW));
     HIP_TRY(e->tile_cnt.alloc(nt + 1));
-    HIP_TRY(e->slot_k.alloc((size_t)nt * TILE));
+    HIP_TRY(e->slot_k.alloc((size_t)nt * TILE * KW));
     HIP_TRY(e->slot_v.alloc((size_t)nt * TILE));
     HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nt, 0, sizeof(int64_t), e->stream));
-    merge_split_kernel<<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(ak, S, bk, L, TILE, nt,
-                                                                    init ? ~CLASS_BIT : ~0ull,
-                                                                    e->splits.p, e->prevk.p);
+    merge_split_kernel<K><<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(
+        ak, S, bk, L, TILE, nt, init ? (K)~class_bit<K>() : (K)~(K)0, e->splits.p, (K*)e->prevk.p);
     hipEvent_t ev{};
     e->timer.begin("merge", e->stream, &ev);
     if (init)
-        merge_kernel<V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p,
-                                                        (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
-                                                        e->splits.p, e->prevk.p, bo);
+        merge_kernel<K, V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
+                                                           (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
+                                                           e->splits.p, (const K*)e->prevk.p, bo);
     else
-        merge_kernel<V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, e->slot_k.p,
-                                                         (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
-                                                         e->splits.p, e->prevk.p, nullptr);
+        merge_kernel<K, V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
+                                                            (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
+                                                            e->splits.p, (const K*)e->prevk.p, nullptr);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
     if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nt + 1)) return rc;
-    compact_kernel<V, MERGE_IPT, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
-        e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok, ov, cap, &dm->L_new, &dm->flags);
+    compact_kernel<K, V, MERGE_IPT, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
+        (const K*)e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok, ov, cap, &dm->L_new, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     return RIMU_OK;
 }
 
-template <class V>
+template <class K, class V>
 int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
               rimu_step_stats* out) {
+    constexpr int KW = sizeof(K) / 8;
     std::memset(out, 0, sizeof(*out));
     const StepArgs a = make_args(e, step, shift, dtau, boost);
     const int64_t L = e->len;
-    const uint64_t* vk = e->vkeys[e->cur].p;
+    const K* vk = (const K*)e->vkeys[e->cur].p;
     const V* vv = (const V*)e->vvals[e->cur].p;
     rimu_engine::Misc* dm = (rimu_engine::Misc*)e->misc.p;
     hipEvent_t ev{};
@@ -298,7 +301,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(hipMemsetAsync(e->plan_status.p, 0, sizeof(uint64_t) * plan_tiles, e->stream));
     const int64_t tf_cap = (int64_t)e->tile_first.n;
     e->timer.begin("plan", e->stream, &ev);
-    plan_kernel<V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
+    plan_kernel<K, V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
                                                      e->tile_first.p, tf_cap, e->plan_status.p,
                                                      &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
     e->timer.end(e->stream);
@@ -313,7 +316,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     // ranges and a scan of the tile counts plus a compaction place the records.
     int64_t S = 0;
     int64_t spawn_tiles = 0;
-    uint64_t kor = 0, kand = ~0ull;
+    u128 kor = 0, kand = ~(u128)0;
     if (W > 0) {
         spawn_tiles = ceil_div(W, SPAWN_TILE);
         // every attempt yields a record when values are floating and deposits unprojected
@@ -323,40 +326,40 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
             return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
         }
         HIP_TRY(e->spawn_p.alloc(spawn_tiles));
-        HIP_TRY(e->tile_or.alloc(spawn_tiles));
-        HIP_TRY(e->tile_and.alloc(spawn_tiles));
+        HIP_TRY(e->tile_or.alloc((size_t)spawn_tiles * KW));
+        HIP_TRY(e->tile_and.alloc((size_t)spawn_tiles * KW));
         if (spawn_tiles > tf_cap) {  // plan filled only the first tf_cap tile heads
             HIP_TRY(e->tile_first.alloc(spawn_tiles));
             tile_first_kernel<<<ceil_div(L, NT), NT, 0, e->stream>>>(e->woff.p, L, e->tile_first.p);
         }
-        uint64_t* sk = e->skeys[0].p;
+        K* sk = (K*)e->skeys[0].p;
         V* sv = (V*)e->svals[0].p;
         if (!dense) {
-            HIP_TRY(e->slot_k.alloc((size_t)spawn_tiles * SPAWN_TILE));
+            HIP_TRY(e->slot_k.alloc((size_t)spawn_tiles * SPAWN_TILE * KW));
             HIP_TRY(e->slot_v.alloc((size_t)spawn_tiles * SPAWN_TILE));
             HIP_TRY(e->tile_cnt.alloc(spawn_tiles + 1));
             HIP_TRY(hipMemsetAsync(e->tile_cnt.p + spawn_tiles, 0, sizeof(int64_t), e->stream));
-            sk = e->slot_k.p;
+            sk = (K*)e->slot_k.p;
             sv = (V*)e->slot_v.p;
         }
         e->timer.begin("spawn", e->stream, &ev);
         if (dense)
-            spawn_kernel<V, true><<<spawn_tiles, NT, 0, e->stream>>>(
+            spawn_kernel<K, V, true><<<spawn_tiles, NT, 0, e->stream>>>(
                 a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                e->tile_or.p, e->tile_and.p);
+                (K*)e->tile_or.p, (K*)e->tile_and.p);
         else
-            spawn_kernel<V, false><<<spawn_tiles, NT, 0, e->stream>>>(
+            spawn_kernel<K, V, false><<<spawn_tiles, NT, 0, e->stream>>>(
                 a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                e->tile_or.p, e->tile_and.p);
+                (K*)e->tile_or.p, (K*)e->tile_and.p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
-        key_bits_kernel<<<1, 1024, 0, e->stream>>>(e->tile_or.p, e->tile_and.p, spawn_tiles, &dm->key_or,
-                                                   &dm->key_and);
+        key_bits_kernel<K><<<1, 1024, 0, e->stream>>>((const K*)e->tile_or.p, (const K*)e->tile_and.p,
+                                                      spawn_tiles, (K*)&dm->key_or, (K*)&dm->key_and);
         if (!dense) {
             e->timer.begin("spawn_compact", e->stream, &ev);
             if (int rc = scan_inplace_i64(e, e->tile_cnt.p, spawn_tiles + 1)) return rc;
-            compact_kernel<V, SPAWN_ITEMS, FLAG_SPAWN_OVERFLOW><<<spawn_tiles, NT, 0, e->stream>>>(
-                e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, spawn_tiles, e->skeys[0].p,
+            compact_kernel<K, V, SPAWN_ITEMS, FLAG_SPAWN_OVERFLOW><<<spawn_tiles, NT, 0, e->stream>>>(
+                (const K*)e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, spawn_tiles, (K*)e->skeys[0].p,
                 (V*)e->svals[0].p, e->max_spawns, &dm->S, &dm->flags);
             e->timer.end(e->stream);
         }
@@ -371,26 +374,27 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         S = dense ? W : e->h_misc->S;
         kor = e->h_misc->key_or;
         kand = e->h_misc->key_and;
+        if (KW == 1) { kor = (uint64_t)kor; kand = (uint64_t)kand; }
     }
 
     // 3. sort the spawn records
     int sorted = 0;
     if (S > 1) {
         const int nbits = e->hdev.B;
-        if (int rc = radix_sort<V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
+        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
     }
 
     // 4. merge with the diagonal survivors, fold, compress, count
     const int nxt = 1 - e->cur;
     int64_t merge_tiles = 0;
-    if (int rc = merge_into<V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
-                               (const V*)e->dvals.p, L, e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
-                               e->max_entries, dm, &merge_tiles, vv)) return rc;
+    if (int rc = merge_into<K, V>(e, a, (const K*)e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
+                                  (const V*)e->dvals.p, L, (K*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
+                                  e->max_entries, dm, &merge_tiles, vv)) return rc;
 
     // 5. projections and statistics
     if (e->np + e->nh > 0) {
-        project_kernel<V><<<ceil_div(e->np + e->nh, NT), NT, 0, e->stream>>>(
-            e->vkeys[nxt].p, (const V*)e->vvals[nxt].p, &dm->L_new, e->pkeys.p, e->pcoef.p,
+        project_kernel<K, V><<<ceil_div(e->np + e->nh, NT), NT, 0, e->stream>>>(
+            (const K*)e->vkeys[nxt].p, (const V*)e->vvals[nxt].p, &dm->L_new, (const K*)e->pkeys.p, e->pcoef.p,
             e->np + e->nh, e->terms.p);
         HIP_TRY(hipGetLastError());
     }
@@ -441,12 +445,16 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     e->cur = nxt;
     e->len = L_new;
     // algorithmic bytes: read v, write S records, read S records into the merge, write v'
-    const double rec = 8.0 + (double)e->vsize;
+    const double rec = 8.0 * KW + (double)e->vsize;
     const double records = (double)(S + L);  // diagonal survivors included
     e->last_records = records;
     e->last_alg_bytes = (double)L * rec + 2.0 * records * rec + (double)L_new * rec;
     return RIMU_OK;
 }
+
+template <class K>
+int overlaps_impl(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
+                  double* out);
 
 }  // namespace
 
@@ -481,6 +489,7 @@ int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_
     e->hdev = to_dev(*model);
     e->seed = seed;
     e->int_vals = style->kind == RIMU_STYLE_STOCHASTIC_INTEGER;
+    e->kw = e->hdev.B > 64 ? 2 : 1;
     e->vsize = 8;
     e->max_entries = max_entries;
     e->max_spawns = max_spawns;
@@ -493,9 +502,9 @@ int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_
     if ((err = hipGetDevice(&e->device)) != hipSuccess) return bad("hipGetDevice", err);
     if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bad("stream", err);
     for (int b = 0; b < 2; ++b) {
-        if ((err = e->vkeys[b].alloc(max_entries)) != hipSuccess) return bad("vector keys", err);
+        if ((err = e->vkeys[b].alloc(max_entries * e->kw)) != hipSuccess) return bad("vector keys", err);
         if ((err = e->vvals[b].alloc(max_entries * 8)) != hipSuccess) return bad("vector values", err);
-        if ((err = e->skeys[b].alloc(max_spawns)) != hipSuccess) return bad("spawn keys", err);
+        if ((err = e->skeys[b].alloc(max_spawns * e->kw)) != hipSuccess) return bad("spawn keys", err);
         if ((err = e->svals[b].alloc(max_spawns * 8)) != hipSuccess) return bad("spawn values", err);
     }
     if ((err = e->dvals.alloc(max_entries * 8)) != hipSuccess) return bad("diagonal values", err);
@@ -539,31 +548,38 @@ int rimu_engine_destroy(rimu_engine* e) {
 int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
     if (n < 0 || (n > 0 && (!keys || !vals))) return fail(RIMU_ERR_ARG, "bad vector arguments");
-    const uint64_t maskB = (e->hdev.B >= 64) ? ~0ull : ((1ull << e->hdev.B) - 1);
+    const int B = e->hdev.B, kw = e->kw;
+    const u128 maskB = B >= 128 ? ~(u128)0 : (((u128)1 << B) - 1);
+    auto key_at = [&](int64_t i) -> u128 {
+        return kw == 1 ? (u128)keys[i] : (((u128)keys[2 * i + 1] << 64) | keys[2 * i]);
+    };
     std::vector<uint64_t> k;
     std::vector<int64_t> vi;
     std::vector<double> vd;
-    k.reserve(n);
+    k.reserve((size_t)n * kw);
+    u128 prev = 0;
     for (int64_t i = 0; i < n; ++i) {
-        if (i > 0 && keys[i] <= keys[i - 1]) return fail(RIMU_ERR_ARG, "keys must be strictly ascending");
-        if ((keys[i] & ~maskB) != 0 || __builtin_popcountll(keys[i]) != e->hdev.N)
+        const u128 key = key_at(i);
+        if (i > 0 && key <= prev) return fail(RIMU_ERR_ARG, "keys must be strictly ascending");
+        prev = key;
+        const int ones = __builtin_popcountll((uint64_t)key) + __builtin_popcountll((uint64_t)(key >> 64));
+        if ((key & ~maskB) != 0 || ones != e->hdev.N)
             return fail(RIMU_ERR_ARG, "key is not an N-boson M-mode BoseFS address");
         if (e->int_vals) {
             const int64_t v = ((const int64_t*)vals)[i];
             if (v == 0) continue;
-            k.push_back(keys[i]);
             vi.push_back(v);
         } else {
             const double v = ((const double*)vals)[i];
             if (v == 0) continue;
-            k.push_back(keys[i]);
             vd.push_back(v);
         }
+        for (int w = 0; w < kw; ++w) k.push_back(keys[(size_t)kw * i + w]);
     }
-    const int64_t m = (int64_t)k.size();
+    const int64_t m = (int64_t)(k.size() / kw);
     if (m > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
     if (m > 0) {
-        HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, k.data(), m * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, k.data(), m * 8 * kw, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->vvals[e->cur].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
                                m * 8, hipMemcpyHostToDevice, e->stream));
     }
@@ -583,7 +599,7 @@ int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap
     *n = e->len;
     if (e->len > cap) return fail(RIMU_ERR_CAPACITY, "download capacity too small");
     if (e->len > 0) {
-        HIP_TRY(hipMemcpyAsync(keys, e->vkeys[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(keys, e->vkeys[e->cur].p, e->len * 8 * e->kw, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipMemcpyAsync(vals, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -595,15 +611,15 @@ int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
     if (n < 0 || nh < 0) return fail(RIMU_ERR_ARG, "negative projector length");
     const int64_t tot = n + nh;
-    HIP_TRY(e->pkeys.alloc(std::max<int64_t>(tot, 1)));
+    HIP_TRY(e->pkeys.alloc(std::max<int64_t>(tot, 1) * e->kw));
     HIP_TRY(e->pcoef.alloc(std::max<int64_t>(tot, 1)));
     HIP_TRY(e->terms.alloc(std::max<int64_t>(tot, 16)));
     if (n > 0) {
-        HIP_TRY(hipMemcpyAsync(e->pkeys.p, pk, n * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->pkeys.p, pk, n * 8 * e->kw, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->pcoef.p, pc, n * 8, hipMemcpyHostToDevice, e->stream));
     }
     if (nh > 0) {
-        HIP_TRY(hipMemcpyAsync(e->pkeys.p + n, hk, nh * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->pkeys.p + n * e->kw, hk, nh * 8 * e->kw, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->pcoef.p + n, hc, nh * 8, hipMemcpyHostToDevice, e->stream));
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -622,8 +638,11 @@ int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         return RIMU_OK;
     }
     if (e->len > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
-    return e->int_vals ? step_impl<int64_t>(e, step, shift, dtau, boost, stats)
-                       : step_impl<double>(e, step, shift, dtau, boost, stats);
+    if (e->kw == 2)
+        return e->int_vals ? step_impl<u128, int64_t>(e, step, shift, dtau, boost, stats)
+                           : step_impl<u128, double>(e, step, shift, dtau, boost, stats);
+    return e->int_vals ? step_impl<uint64_t, int64_t>(e, step, shift, dtau, boost, stats)
+                       : step_impl<uint64_t, double>(e, step, shift, dtau, boost, stats);
 }
 
 int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
@@ -632,6 +651,16 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
     if (nops < 0 || nops > OV_MAX_OPS) return fail(RIMU_ERR_ARG, "between 0 and 8 operators");
     if (left->hdev.M != right->hdev.M || left->hdev.N != right->hdev.N)
         return fail(RIMU_ERR_ARG, "vectors of different address spaces (M, N)");
+    if (right->kw == 2) return overlaps_impl<u128>(left, right, ops, nops, out);
+    return overlaps_impl<uint64_t>(left, right, ops, nops, out);
+}
+
+}  // extern "C"
+
+namespace {
+template <class K>
+int overlaps_impl(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
+                  double* out) {
     if (left->device != right->device) return fail(RIMU_ERR_ARG, "vectors on different devices");
     const int M = right->hdev.M;
     std::vector<OpDev> dops((size_t)std::max(nops, 1));
@@ -664,27 +693,27 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
     HIP_TRY(e->ov_ranges.alloc(nb + 1));
     HIP_TRY(hipMemcpyAsync(e->ov_ops.p, dops.data(), sizeof(OpDev) * dops.size(), hipMemcpyHostToDevice,
                            e->stream));
-    const uint64_t* tk = left->vkeys[left->cur].p;
+    const K* tk = (const K*)left->vkeys[left->cur].p;
     const void* tv = left->vvals[left->cur].p;
-    const uint64_t* uk = right->vkeys[right->cur].p;
+    const K* uk = (const K*)right->vkeys[right->cur].p;
     const void* uv = right->vvals[right->cur].p;
     const OpDev* dp = (const OpDev*)e->ov_ops.p;
     int need_split = 0;
     for (int o = 0; o < nops; ++o) need_split |= dops[o].kind == RIMU_OP_HAMILTONIAN;
     hipEvent_t ev{};
     e->timer.begin("overlap", e->stream, &ev);
-    overlap_ranges_kernel<<<ceil_div(nb + 1, NT), NT, 0, e->stream>>>(tk, Lt, uk, nb, e->ov_ranges.p);
+    overlap_ranges_kernel<K><<<ceil_div(nb + 1, NT), NT, 0, e->stream>>>(tk, Lt, uk, nb, e->ov_ranges.p);
     if (left->int_vals && right->int_vals)
-        overlap_kernel<int64_t, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
+        overlap_kernel<K, int64_t, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
             (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
     else if (left->int_vals)
-        overlap_kernel<int64_t, double><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
+        overlap_kernel<K, int64_t, double><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
             (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
     else if (right->int_vals)
-        overlap_kernel<double, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
+        overlap_kernel<K, double, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
             (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
     else
-        overlap_kernel<double, double><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
+        overlap_kernel<K, double, double><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
             (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
@@ -695,14 +724,17 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
     e->timer.collect();
     return RIMU_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
     if (rule < RIMU_INITIATOR_NONE || rule > RIMU_INITIATOR_COHERENT)
         return fail(RIMU_ERR_ARG, "unknown initiator rule");
     if (std::isnan(threshold)) return fail(RIMU_ERR_ARG, "initiator threshold is NaN");
-    if (rule != RIMU_INITIATOR_NONE && e->hdev.B > 63)
-        return fail(RIMU_ERR_ARG, "initiator rules need N + M - 1 <= 63 (key bit 63 holds the class)");
+    if (rule != RIMU_INITIATOR_NONE && e->hdev.B > 64 * e->kw - 1)
+        return fail(RIMU_ERR_ARG, "initiator rules need N + M - 1 <= 63 or 65..127 (the top key bit holds the class)");
     e->init_rule = rule;
     e->init_threshold = rule == RIMU_INITIATOR_NONE ? 0.0 : threshold;
     return RIMU_OK;
@@ -747,14 +779,15 @@ int rimu_engine_last_traffic(rimu_engine* e, double* alg_bytes, double* records)
 // ---------------------------------------------------------------------------------------
 namespace {
 
-__global__ __launch_bounds__(NT) void ham_eval_kernel(ModelDev h, const uint64_t* __restrict__ addrs,
+template <class K>
+__global__ __launch_bounds__(NT) void ham_eval_kernel(ModelDev h, const K* __restrict__ addrs,
                                                       const int64_t* __restrict__ chosen, int64_t n,
                                                       double* __restrict__ diag, int64_t* __restrict__ nod,
-                                                      uint64_t* __restrict__ naddr,
+                                                      K* __restrict__ naddr,
                                                       double* __restrict__ oval) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i >= n) return;
-    const uint64_t k = addrs[i];
+    const K k = addrs[i];
     diag[i] = diagonal_element(h, k);
     const int64_t c = num_offdiagonals(h, k);
     nod[i] = c;
@@ -776,12 +809,12 @@ int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int6
     HIP_TRY(hipMemcpyAsync(e->skeys[0].p, keys, n * 8, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->svals[0].p, vals, n * 8, hipMemcpyHostToDevice, e->stream));
     int sorted = 0;
-    if (int rc = radix_sort<V>(e, n, nbits, varying, 0, &sorted)) return rc;
+    if (int rc = radix_sort<uint64_t, V>(e, n, nbits, varying, 0, &sorted)) return rc;
     StepArgs a = make_args(e, 0, 0.0, 1.0, 1.0);
     a.compression = 0;
     HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
     int64_t tiles = 0;
-    if (int rc = merge_into<V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n, e->vkeys[0].p,
+    if (int rc = merge_into<uint64_t, V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n, e->vkeys[0].p,
                                (const V*)e->dvals.p, 0, e->vkeys[1].p, (V*)e->vvals[1].p, e->max_entries,
                                dm, &tiles)) return rc;
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
@@ -841,27 +874,35 @@ int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
     const ModelDev h = to_dev(*model);
-    const uint64_t maskB = (h.B >= 64) ? ~0ull : ((1ull << h.B) - 1);
-    for (int64_t i = 0; i < n; ++i)
-        if ((addrs[i] & ~maskB) != 0 || __builtin_popcountll(addrs[i]) != h.N)
+    const int kw = h.B > 64 ? 2 : 1;
+    const u128 maskB = h.B >= 128 ? ~(u128)0 : (((u128)1 << h.B) - 1);
+    for (int64_t i = 0; i < n; ++i) {
+        const u128 key = kw == 1 ? (u128)addrs[i] : (((u128)addrs[2 * i + 1] << 64) | addrs[2 * i]);
+        const int ones = __builtin_popcountll((uint64_t)key) + __builtin_popcountll((uint64_t)(key >> 64));
+        if ((key & ~maskB) != 0 || ones != h.N)
             return fail(RIMU_ERR_ARG, "address is not an N-boson M-mode BoseFS address");
+    }
     DevBuf<uint64_t> a, na;
     DevBuf<int64_t> c, nod;
     DevBuf<double> d, ov;
     auto cleanup = [&]() { a.release(); na.release(); c.release(); nod.release(); d.release(); ov.release(); };
-    if (a.alloc(n) || na.alloc(n) || c.alloc(n) || nod.alloc(n) || d.alloc(n) || ov.alloc(n)) {
+    if (a.alloc(n * kw) || na.alloc(n * kw) || c.alloc(n) || nod.alloc(n) || d.alloc(n) || ov.alloc(n)) {
         cleanup();
         return fail(RIMU_ERR_HIP, "allocation failed");
     }
-    hipError_t err = hipMemcpy(a.p, addrs, n * 8, hipMemcpyHostToDevice);
+    hipError_t err = hipMemcpy(a.p, addrs, n * 8 * kw, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(c.p, chosen, n * 8, hipMemcpyHostToDevice);
     if (err == hipSuccess) {
-        ham_eval_kernel<<<ceil_div(n, NT), NT>>>(h, a.p, c.p, n, d.p, nod.p, na.p, ov.p);
+        if (kw == 2)
+            ham_eval_kernel<u128><<<ceil_div(n, NT), NT>>>(h, (const u128*)a.p, c.p, n, d.p, nod.p,
+                                                          (u128*)na.p, ov.p);
+        else
+            ham_eval_kernel<uint64_t><<<ceil_div(n, NT), NT>>>(h, a.p, c.p, n, d.p, nod.p, na.p, ov.p);
         err = hipGetLastError();
     }
     if (err == hipSuccess) err = hipMemcpy(diag, d.p, n * 8, hipMemcpyDeviceToHost);
     if (err == hipSuccess) err = hipMemcpy(num_offdiag, nod.p, n * 8, hipMemcpyDeviceToHost);
-    if (err == hipSuccess) err = hipMemcpy(new_addr, na.p, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(new_addr, na.p, n * 8 * kw, hipMemcpyDeviceToHost);
     if (err == hipSuccess) err = hipMemcpy(offdiag_value, ov.p, n * 8, hipMemcpyDeviceToHost);
     cleanup();
     if (err != hipSuccess) return fail(RIMU_ERR_HIP, std::string("hamiltonian_eval: ") + hipGetErrorString(err));
@@ -876,7 +917,7 @@ extern "C" int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t 
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (max_spawns > e->max_spawns) {
         for (int b = 0; b < 2; ++b) {
-            HIP_TRY(e->skeys[b].alloc(max_spawns));
+            HIP_TRY(e->skeys[b].alloc(max_spawns * e->kw));
             HIP_TRY(e->svals[b].alloc(max_spawns * 8));
         }
         e->max_spawns = max_spawns;
@@ -885,10 +926,10 @@ extern "C" int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t 
         for (int b = 0; b < 2; ++b) {
             DevBuf<uint64_t> nk;
             DevBuf<unsigned char> nv;
-            HIP_TRY(nk.alloc(max_entries));
+            HIP_TRY(nk.alloc(max_entries * e->kw));
             HIP_TRY(nv.alloc(max_entries * 8));
             if (b == e->cur && e->len > 0) {
-                HIP_TRY(hipMemcpyAsync(nk.p, e->vkeys[b].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipMemcpyAsync(nk.p, e->vkeys[b].p, e->len * 8 * e->kw, hipMemcpyDeviceToDevice, e->stream));
                 HIP_TRY(hipMemcpyAsync(nv.p, e->vvals[b].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
                 HIP_TRY(hipStreamSynchronize(e->stream));
             }

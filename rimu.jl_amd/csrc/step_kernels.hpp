@@ -23,8 +23,12 @@ constexpr int MERGE_IPT = 8;     // merged positions per thread
 constexpr int SPAWN_TILE_SIZE = 1024;  // attempts per spawn tile (NT * SPAWN_ITEMS)
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
-constexpr int SORT_IPT = 16;     // keys per thread per radix tile
+constexpr int SORT_IPT = 16;     // keys per thread per radix tile (one-chunk keys)
 constexpr int SORT_TILE = NT * SORT_IPT;
+// Keys per thread of a radix tile for key type K: two-chunk keys take half as many, so a
+// tile's keys and values staged in LDS stay within 48 KiB.
+template <class K> struct sort_ipt { static constexpr int value = sizeof(K) == 8 ? SORT_IPT : SORT_IPT / 2; };
+template <class K> struct sort_tile { static constexpr int value = NT * sort_ipt<K>::value; };
 static_assert(RADIX == NT, "radix kernels use one thread per digit");
 
 enum { FLAG_SPAWN_OVERFLOW = 1, FLAG_LOOKBACK = 2, FLAG_VECTOR_OVERFLOW = 4 };
@@ -47,6 +51,15 @@ struct StepArgs {
 // `unsafe` component (else `safe`); on the diagonal deposit of an entry, staged in the
 // merge, it selects `safe` (else `initiator`).  Radix passes and merge comparisons ignore it.
 constexpr uint64_t CLASS_BIT = 1ull << 63;
+template <class K> __host__ __device__ constexpr K class_bit() { return (K)1 << (8 * sizeof(K) - 1); }
+
+// Wavefront shuffles of keys (two 64-bit shuffles for two-chunk keys).
+__device__ inline uint64_t shfl_xor_key(uint64_t x, int off) { return __shfl_xor(x, off, WAVE); }
+__device__ inline u128 shfl_xor_key(u128 x, int off) {
+    const uint64_t lo = __shfl_xor((uint64_t)x, off, WAVE);
+    const uint64_t hi = __shfl_xor((uint64_t)(x >> 64), off, WAVE);
+    return ((u128)hi << 64) | lo;
+}
 
 __device__ inline bool below_initiator_threshold(const StepArgs& a, double v) {
     return !(fabs(v) > a.init_threshold);
@@ -77,8 +90,8 @@ struct ColumnPlan {
     bool exact;
 };
 
-template <class V>
-__device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
+template <class K, class V>
+__device__ inline ColumnPlan plan_column(const StepArgs& a, K k, V v) {
     ColumnPlan c;
     c.n_od = num_offdiagonals(a.h, k);
     const double av = fabs((double)v);
@@ -108,8 +121,8 @@ __device__ inline ColumnPlan plan_column(const StepArgs& a, uint64_t k, V v) {
 constexpr int PLAN_IPT = 8;
 constexpr int PLAN_TILE = NT * PLAN_IPT;
 
-template <class V>
-__global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __restrict__ keys,
+template <class K, class V>
+__global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restrict__ keys,
                                                   const V* __restrict__ vals, int64_t L,
                                                   V* __restrict__ dvals, int64_t* __restrict__ aoff,
                                                   int64_t* __restrict__ tile_first, int64_t tf_cap,
@@ -121,7 +134,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __
     const int64_t t0 = (int64_t)tile * PLAN_TILE;
     double st[6] = {0, 0, 0, 0, 0, 0};
     // striped: entry t0 + j*NT + x (coalesced loads and stores)
-    uint64_t kk[PLAN_IPT];
+    K kk[PLAN_IPT];
     V vv[PLAN_IPT];
 #pragma unroll
     for (int j = 0; j < PLAN_IPT; ++j) {  // all loads in flight before any is used
@@ -134,7 +147,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __
         const int64_t i = t0 + j * NT + threadIdx.x;
         int64_t n_items = 0;
         if (i < L) {
-            const uint64_t k = kk[j];
+            const K k = kk[j];
             const V v = vv[j];
             // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
             const double hd = diagonal_element(a.h, k);
@@ -152,7 +165,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const uint64_t* __
             if (res > val) st[1] += fabs(res - val);
             else if (sgn(res) != sgn(val)) { st[0] += fabs(val); st[2] += fabs(res); }
             else st[0] += fabs(res - val);
-            const ColumnPlan c = plan_column<V>(a, k, v);
+            const ColumnPlan c = plan_column<K, V>(a, k, v);
             if (c.n_od > 0) {
                 if (a.style != 1) { if (c.exact) st[3] += 1; else st[4] += 1; }
                 st[5] += (double)c.n_items;
@@ -281,17 +294,17 @@ constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
 static_assert(SPAWN_TILE % NT == 0, "whole slots per thread in the entry scan");
 static_assert(SPAWN_TILE == SPAWN_TILE_SIZE, "plan and spawn agree on the spawn tile");
 
-template <class V, bool DENSE>
-__global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* __restrict__ keys,
+template <class K, class V, bool DENSE>
+__global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restrict__ keys,
                                                    const V* __restrict__ vals, int64_t L,
                                                    const int64_t* __restrict__ aoff, int64_t A,
-                                                   uint64_t* __restrict__ out_keys,
+                                                   K* __restrict__ out_keys,
                                                    V* __restrict__ out_vals,
                                                    int64_t* __restrict__ tile_cnt,
                                                    double* __restrict__ partials,
                                                    const int64_t* __restrict__ tile_first,
-                                                   uint64_t* __restrict__ tile_or,
-                                                   uint64_t* __restrict__ tile_and) {
+                                                   K* __restrict__ tile_or,
+                                                   K* __restrict__ tile_and) {
     constexpr int NW = NT / WAVE;
     constexpr int WIN = SPAWN_TILE + 1;
     __shared__ int64_t s_aoff[WIN];
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     __shared__ int s_wmax[NW];
     __shared__ int s_cnt[SPAWN_ITEMS * NW];
     __shared__ int s_total;
-    __shared__ uint64_t s_kor[NW], s_kand[NW];
+    __shared__ K s_kor[NW], s_kand[NW];
 
     const uint32_t tile = blockIdx.x;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -365,12 +378,12 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     }
     __syncthreads();
 
-    uint64_t rk[SPAWN_ITEMS];
+    K rk[SPAWN_ITEMS];
     V rv[SPAWN_ITEMS];
     int rpos[SPAWN_ITEMS];
     bool rok[SPAWN_ITEMS];
     double spawned = 0;
-    uint64_t kor = 0, kand = ~0ull;
+    K kor = 0, kand = (K)~(K)0;
     // entry of each attempt: last i with aoff[i] <= att (every entry has at least one
     // attempt unless it has no off-diagonals; outside the window: global search)
     int64_t ent[SPAWN_ITEMS], aidx[SPAWN_ITEMS];
@@ -397,7 +410,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
         aidx[it] = ai;
     }
     // the entries' keys and values, all loads in flight at once (e0 < L whenever A > 0)
-    uint64_t ekey[SPAWN_ITEMS];
+    K ekey[SPAWN_ITEMS];
     V eval[SPAWN_ITEMS];
 #pragma unroll
     for (int it = 0; it < SPAWN_ITEMS; ++it) {
@@ -409,13 +422,13 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     for (int it = 0; it < SPAWN_ITEMS; ++it) {
         const int64_t att = a0 + (int64_t)it * NT + threadIdx.x;
         bool ok = false;
-        uint64_t nb = 0;
+        K nb = 0;
         V nv = (V)0;
         if (att < A) {
             const int64_t j = att - aidx[it];
-            const uint64_t k = ekey[it];
+            const K k = ekey[it];
             const V v = eval[it];
-            const ColumnPlan c = plan_column<V>(a, k, v);
+            const ColumnPlan c = plan_column<K, V>(a, k, v);
             const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
             U4 r = {0, 0, 0, 0};
             if (!c.exact || need_u) r = draw(a.seed, k, (uint32_t)j, a.step, TAG_SPAWN);
@@ -437,7 +450,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
             if (ok) { kor |= nb; kand &= nb; }
         }
         rk[it] = nb;
-        if (a.init_rule != 0 && att < A && below_initiator_threshold(a, (double)eval[it])) rk[it] |= CLASS_BIT;
+        if (a.init_rule != 0 && att < A && below_initiator_threshold(a, (double)eval[it])) rk[it] |= class_bit<K>();
         rv[it] = nv;
         rok[it] = ok;
         if (!DENSE) {
@@ -474,8 +487,8 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     // key OR/AND of the tile (which radix digits vary; reduced over the tiles afterwards)
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
-        kor |= __shfl_xor(kor, off, WAVE);
-        kand &= __shfl_xor(kand, off, WAVE);
+        kor |= shfl_xor_key(kor, off);
+        kand &= shfl_xor_key(kand, off);
     }
     if (lane == 0) {
         s_kor[wid] = kor;
@@ -483,7 +496,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const uint64_t* _
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t bo = 0, ba = ~0ull;
+        K bo = 0, ba = (K)~(K)0;
         for (int q = 0; q < NW; ++q) { bo |= s_kor[q]; ba &= s_kand[q]; }
         partials[tile] = s;
         tile_or[tile] = bo;
@@ -505,12 +518,13 @@ __global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restric
 // Merge-path split of every merge tile boundary d = t*TILE (clamped to S+L): number of A
 // elements among the first d merged elements (B before A on equal keys).
 // kmask clears the initiator class bit of the spawn keys (all ones without initiators).
-__global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restrict__ ak, int64_t S,
-                                                         const uint64_t* __restrict__ bk, int64_t L,
+template <class K>
+__global__ __launch_bounds__(NT) void merge_split_kernel(const K* __restrict__ ak, int64_t S,
+                                                         const K* __restrict__ bk, int64_t L,
                                                          int64_t tile_size, int64_t ntiles,
-                                                         uint64_t kmask,
+                                                         K kmask,
                                                          int64_t* __restrict__ splits,
-                                                         uint64_t* __restrict__ prevk) {
+                                                         K* __restrict__ prevk) {
     const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (t > ntiles) return;
     const int64_t d = min(t * tile_size, S + L);
@@ -522,33 +536,33 @@ __global__ __launch_bounds__(NT) void merge_split_kernel(const uint64_t* __restr
     splits[t] = lo;
     // key of the merged element just before the tile (read only when the tile starts past 0)
     const int64_t bi = d - lo;
-    const uint64_t pa = lo > 0 ? (ak[lo - 1] & kmask) : 0;
-    const uint64_t pb = bi > 0 ? bk[bi - 1] : 0;
+    const K pa = lo > 0 ? (K)(ak[lo - 1] & kmask) : (K)0;
+    const K pb = bi > 0 ? bk[bi - 1] : (K)0;
     prevk[t] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
 }
 
 // OR and AND of the per-tile key words (one workgroup).
-__global__ __launch_bounds__(1024) void key_bits_kernel(const uint64_t* __restrict__ tile_or,
-                                                        const uint64_t* __restrict__ tile_and,
-                                                        int64_t ntiles, unsigned long long* key_or,
-                                                        unsigned long long* key_and) {
-    __shared__ uint64_t s_o[1024 / WAVE], s_a[1024 / WAVE];
-    uint64_t o = 0, an = ~0ull;
+template <class K>
+__global__ __launch_bounds__(1024) void key_bits_kernel(const K* __restrict__ tile_or,
+                                                        const K* __restrict__ tile_and,
+                                                        int64_t ntiles, K* key_or, K* key_and) {
+    __shared__ K s_o[1024 / WAVE], s_a[1024 / WAVE];
+    K o = 0, an = (K)~(K)0;
     for (int64_t base = 0; base < ntiles; base += 4 * 1024) {
-        uint64_t xo[4], xa[4];
+        K xo[4], xa[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t t = base + r * 1024 + threadIdx.x;
-            xo[r] = t < ntiles ? tile_or[t] : 0ull;
-            xa[r] = t < ntiles ? tile_and[t] : ~0ull;
+            xo[r] = t < ntiles ? tile_or[t] : (K)0;
+            xa[r] = t < ntiles ? tile_and[t] : (K)~(K)0;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) { o |= xo[r]; an &= xa[r]; }
     }
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
-        o |= __shfl_xor(o, off, WAVE);
-        an &= __shfl_xor(an, off, WAVE);
+        o |= shfl_xor_key(o, off);
+        an &= shfl_xor_key(an, off);
     }
     if ((threadIdx.x & (WAVE - 1)) == 0) {
         s_o[threadIdx.x / WAVE] = o;
@@ -569,22 +583,24 @@ __global__ __launch_bounds__(1024) void key_bits_kernel(const uint64_t* __restri
 // ---------------------------------------------------------------------------------------
 // dmask selects the digit bits of the pass: RADIX - 1, or fewer bits for a top digit that
 // would reach the initiator class bit.
-__global__ __launch_bounds__(NT) void radix_count_kernel(const uint64_t* __restrict__ keys, int64_t n,
+template <class K>
+__global__ __launch_bounds__(NT) void radix_count_kernel(const K* __restrict__ keys, int64_t n,
                                                          int shift, uint32_t dmask,
                                                          uint32_t* __restrict__ counts,
                                                          int64_t num_tiles) {
+    constexpr int IPT = sort_ipt<K>::value;
     __shared__ uint32_t hist[RADIX];
     hist[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    uint64_t k[SORT_IPT];
+    const int64_t base = (int64_t)blockIdx.x * sort_tile<K>::value;
+    K k[IPT];
 #pragma unroll
-    for (int r = 0; r < SORT_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
         k[r] = keys[min(idx, n - 1)];  // all loads issued before the first atomic
     }
 #pragma unroll
-    for (int r = 0; r < SORT_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
         if (idx < n) atomicAdd(&hist[(uint32_t)(k[r] >> shift) & dmask], 1u);
     }
@@ -632,20 +648,22 @@ __global__ __launch_bounds__(RADIX) void radix_chunk_offsets_kernel(uint32_t* __
     }
 }
 
-template <class V>
-__global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __restrict__ keys_in,
+template <class K, class V>
+__global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__ keys_in,
                                                            const V* __restrict__ vals_in, int64_t n,
                                                            int shift, uint32_t dmask,
                                                            const uint32_t* __restrict__ offs,
                                                            int64_t num_tiles,
-                                                           uint64_t* __restrict__ keys_out,
+                                                           K* __restrict__ keys_out,
                                                            V* __restrict__ vals_out) {
-    // Wavefront w ranks the contiguous sub-tile [w*SORT_IPT*64, (w+1)*SORT_IPT*64) of the
-    // tile with private digit counters: round r handles keys w*SORT_IPT*64 + r*64 + lane,
+    // Wavefront w ranks the contiguous sub-tile [w*IPT*64, (w+1)*IPT*64) of the
+    // tile with private digit counters: round r handles keys w*IPT*64 + r*64 + lane,
     // so (round, lane) is the element order and ranks are stable without barriers.
     constexpr int NW = NT / WAVE;
-    __shared__ uint64_t s_k[SORT_TILE];
-    __shared__ V s_v[SORT_TILE];
+    constexpr int IPT = sort_ipt<K>::value;
+    constexpr int TILE = sort_tile<K>::value;
+    __shared__ K s_k[TILE];
+    __shared__ V s_v[TILE];
     __shared__ uint32_t s_wcnt[NW][RADIX];  // per-wavefront digit counts, then prefixes
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_goff[RADIX];
@@ -653,26 +671,26 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    const int64_t wbase = base + (int64_t)wid * SORT_IPT * WAVE;
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    const int64_t wbase = base + (int64_t)wid * IPT * WAVE;
 
     for (int j = threadIdx.x; j < NW * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
     s_goff[threadIdx.x] = offs[(int64_t)blockIdx.x * RADIX + threadIdx.x];
     __syncthreads();
 
-    uint64_t k[SORT_IPT];
-    V v[SORT_IPT];
-    uint32_t rank[SORT_IPT];
+    K k[IPT];
+    V v[IPT];
+    uint32_t rank[IPT];
     uint32_t* cnt = s_wcnt[wid];
 #pragma unroll
-    for (int r = 0; r < SORT_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         const bool valid = idx < n;
-        k[r] = valid ? keys_in[idx] : 0ull;
+        k[r] = valid ? keys_in[idx] : (K)0;
         v[r] = valid ? vals_in[idx] : (V)0;
     }
 #pragma unroll
-    for (int r = 0; r < SORT_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         const bool valid = idx < n;
         const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
@@ -706,7 +724,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < SORT_IPT; ++r) {
+    for (int r = 0; r < IPT; ++r) {
         const int64_t idx = wbase + (int64_t)r * WAVE + lane;
         if (idx < n) {
             const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
@@ -716,9 +734,9 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const uint64_t* __res
         }
     }
     __syncthreads();
-    const int64_t cnt_tile = min((int64_t)SORT_TILE, n - base);
+    const int64_t cnt_tile = min((int64_t)TILE, n - base);
     for (int i = threadIdx.x; i < cnt_tile; i += NT) {
-        const uint64_t kk = s_k[i];
+        const K kk = s_k[i];
         const uint32_t digit = (uint32_t)(kk >> shift) & dmask;
         const int64_t g = (int64_t)s_goff[digit] + (i - (int64_t)s_start[digit]);
         keys_out[g] = kk;
@@ -756,21 +774,21 @@ __device__ inline V initiator_value(const StepArgs& a, V s, V u, V i) {
 // deposits in list order; an address exists when a component is nonzero (deposit! deletes
 // all-zero entries, pdworkingmemory.jl:21-31) and takes the rule's value, which
 // NoCompression keeps even when it is zero (stochasticstyles.jl:116-121).
-template <class V, bool INIT>
-__global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
+template <class K, class V, bool INIT>
+__global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restrict__ ak,
                                                    const V* __restrict__ av, int64_t S,
-                                                   const uint64_t* __restrict__ bk,
+                                                   const K* __restrict__ bk,
                                                    const V* __restrict__ bv, int64_t L,
-                                                   uint64_t* __restrict__ slot_k, V* __restrict__ slot_v,
+                                                   K* __restrict__ slot_k, V* __restrict__ slot_v,
                                                    int64_t* __restrict__ tile_cnt,
                                                    double* __restrict__ partials,
                                                    const int64_t* __restrict__ splits,
-                                                   const uint64_t* __restrict__ prevk,
+                                                   const K* __restrict__ prevk,
                                                    const V* __restrict__ bo) {
     constexpr int TILE = NT * MERGE_IPT;
-    constexpr uint64_t KM = INIT ? ~CLASS_BIT : ~0ull;
+    constexpr K KM = INIT ? (K)~class_bit<K>() : (K)~(K)0;
     // the tile's A range, its B range, then up to MERGE_EXTRA records of A past the tile
-    __shared__ uint64_t s_key[TILE + MERGE_EXTRA];
+    __shared__ K s_key[TILE + MERGE_EXTRA];
     __shared__ V s_val[TILE + MERGE_EXTRA];
 
     const int64_t tile = blockIdx.x;
@@ -781,27 +799,27 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
     const int ne = (int)min((int64_t)MERGE_EXTRA, S - a1);
-    const uint64_t tile_prev = prevk[tile];
+    const K tile_prev = prevk[tile];
     // stage A, B and the extra records: all loads of a thread are issued before the first
     // LDS store, so a wave waits for one memory round trip instead of one per element
     {
-        uint64_t rk[MERGE_IPT];
+        K rk[MERGE_IPT];
         V rv[MERGE_IPT];
 #pragma unroll
         for (int r = 0; r < MERGE_IPT; ++r) {
             const int i = (int)threadIdx.x + r * NT;
             if (i < na + nb) {
                 const bool in_a = i < na;
-                const uint64_t* kp = in_a ? ak + (a0 + i) : bk + (b0 + (i - na));
+                const K* kp = in_a ? ak + (a0 + i) : bk + (b0 + (i - na));
                 const V* vp = in_a ? av + (a0 + i) : bv + (b0 + (i - na));
                 rk[r] = *kp;
                 rv[r] = *vp;
                 if constexpr (INIT) {
-                    if (!in_a && below_initiator_threshold(a, (double)bo[b0 + (i - na)])) rk[r] |= CLASS_BIT;
+                    if (!in_a && below_initiator_threshold(a, (double)bo[b0 + (i - na)])) rk[r] |= class_bit<K>();
                 }
             }
         }
-        uint64_t ek = 0;
+        K ek = 0;
         V ev = (V)0;
         const int x = (int)threadIdx.x;
         if (x < ne) { ek = ak[a1 + x]; ev = av[a1 + x]; }
@@ -813,8 +831,8 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         if (x < ne) { s_key[na + nb + x] = ek; s_val[na + nb + x] = ev; }
     }
     __syncthreads();
-    const uint64_t* s_a = s_key;
-    const uint64_t* s_b = s_key + na;
+    const K* s_a = s_key;
+    const K* s_b = s_key + na;
 
     // this thread's positions within the tile
     const int dd = (int)threadIdx.x * MERGE_IPT;
@@ -831,11 +849,11 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
     int ib = dd - ia;
     // key of the merged element just before this thread's first position
     bool have_prev = false;
-    uint64_t prev = 0;
+    K prev = 0;
     if (dd > 0) {
         have_prev = true;
-        const uint64_t pa = ia > 0 ? (s_a[ia - 1] & KM) : 0;
-        const uint64_t pb = ib > 0 ? (s_b[ib - 1] & KM) : 0;
+        const K pa = ia > 0 ? (K)(s_a[ia - 1] & KM) : (K)0;
+        const K pb = ib > 0 ? (K)(s_b[ib - 1] & KM) : (K)0;
         prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
     } else if (d0 > 0) {
         have_prev = true;
@@ -844,7 +862,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
 
     // candidates stay in the slot of their position j (static register indices); bit j of
     // cvalid marks a segment head with a nonzero folded value
-    uint64_t ck[MERGE_IPT];
+    K ck[MERGE_IPT];
     V cv[MERGE_IPT];
     uint32_t cvalid = 0;
     double lb = 0, ln = 0, nrm = 0;
@@ -854,7 +872,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
         cv[j] = (V)0;
         if (j < n_here) {
             const bool take_b = ib < nb && (ia >= na || (s_b[ib] & KM) <= (s_a[ia] & KM));
-            const uint64_t key = (take_b ? s_b[ib] : s_a[ia]) & KM;
+            const K key = (take_b ? s_b[ib] : s_a[ia]) & KM;
             if ((!have_prev || key != prev) && !INIT) {
                 // segment head: fold diag (if any) then the run of spawns in A
                 V acc = (V)0;
@@ -878,12 +896,12 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
             } else if ((!have_prev || key != prev) && INIT) {
                 // the same walk with three components
                 V cs = (V)0, cu = (V)0, ci = (V)0;
-                auto dep = [&](uint64_t rawk, V x) {
-                    if (rawk & CLASS_BIT) cu = cu + x; else cs = cs + x;
+                auto dep = [&](K rawk, V x) {
+                    if (rawk & class_bit<K>()) cu = cu + x; else cs = cs + x;
                 };
                 if (take_b) {
                     const V d = s_val[na + ib];
-                    if (s_b[ib] & CLASS_BIT) cs = cs + d; else ci = ci + d;
+                    if (s_b[ib] & class_bit<K>()) cs = cs + d; else ci = ci + d;
                 }
                 int la = ia;
                 while (la < na && (s_a[la] & KM) == key) { dep(s_a[la], s_val[la]); ++la; }
@@ -892,7 +910,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
                     while (x < ne && (s_key[na + nb + x] & KM) == key) { dep(s_key[na + nb + x], s_val[na + nb + x]); ++x; }
                     if (x == ne) {
                         for (int64_t ga = a1 + ne; ga < S; ++ga) {
-                            const uint64_t rk = ak[ga];
+                            const K rk = ak[ga];
                             if ((rk & KM) != key) break;
                             dep(rk, av[ga]);
                         }
@@ -929,7 +947,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
             const int incl = wave_inclusive_scan(nneed);
             const int wtot = __shfl(incl, WAVE - 1, WAVE);
             const int mybase = incl - nneed;
-            uint64_t* wk = s_key + wid * (WAVE * MERGE_IPT);
+            K* wk = s_key + wid * (WAVE * MERGE_IPT);
             V* wv = s_val + wid * (WAVE * MERGE_IPT);
             __syncthreads();  // the tile's inputs are no longer read
 #pragma unroll
@@ -942,7 +960,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
             }
             __syncthreads();
             for (int q = lane; q < wtot; q += WAVE) {
-                const uint64_t key = wk[q];
+                const K key = wk[q];
                 const double x = (double)wv[q];
                 const U4 r = draw(a.seed, key, 0, a.step, TAG_COMPRESS);
                 wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
@@ -995,11 +1013,11 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const uint64_t* _
 // its slot range to [offs[t], offs[t+1]) (offs is the exclusive scan of the tile counts,
 // offs[ntiles] the new length).  If the new length exceeds cap nothing is written and
 // FLAG is raised.
-template <class V, int IPT, int FLAG>
-__global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict__ slot_k,
+template <class K, class V, int IPT, int FLAG>
+__global__ __launch_bounds__(NT) void compact_kernel(const K* __restrict__ slot_k,
                                                      const V* __restrict__ slot_v,
                                                      const int64_t* __restrict__ offs, int64_t ntiles,
-                                                     uint64_t* __restrict__ ok, V* __restrict__ ov,
+                                                     K* __restrict__ ok, V* __restrict__ ov,
                                                      int64_t cap, int64_t* total_out, int* flags) {
     constexpr int TILE = NT * IPT;
     const int64_t tile = blockIdx.x;
@@ -1015,7 +1033,7 @@ __global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict_
     const int64_t off = offs[tile];
     const int n = (int)(offs[tile + 1] - off);
     const int64_t src = tile * TILE;
-    uint64_t k[IPT];
+    K k[IPT];
     V v[IPT];
 #pragma unroll
     for (int r = 0; r < IPT; ++r) {
@@ -1032,16 +1050,16 @@ __global__ __launch_bounds__(NT) void compact_kernel(const uint64_t* __restrict_
 // ---------------------------------------------------------------------------------------
 // project: terms[i] = coef_i * v[key_i] (0 if absent) by binary search in the new vector.
 // ---------------------------------------------------------------------------------------
-template <class V>
-__global__ __launch_bounds__(NT) void project_kernel(const uint64_t* __restrict__ keys,
+template <class K, class V>
+__global__ __launch_bounds__(NT) void project_kernel(const K* __restrict__ keys,
                                                      const V* __restrict__ vals, const int64_t* len_p,
-                                                     const uint64_t* __restrict__ pk,
+                                                     const K* __restrict__ pk,
                                                      const double* __restrict__ pc, int64_t np,
                                                      double* __restrict__ terms) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i >= np) return;
     const int64_t L = *len_p;
-    const uint64_t key = pk[i];
+    const K key = pk[i];
     int64_t lo = 0, hi = L;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;

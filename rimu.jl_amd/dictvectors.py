@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .addresses import BoseFS
+from .addresses import BoseFS, key_words, pack_keys, unpack_keys
 from .initiators import Initiator, InitiatorRule, NonInitiator, initiator_rule
 from .styles import IsStochasticInteger, StochasticStyle, default_style
 
@@ -80,9 +80,10 @@ class DVec:
         return out
 
     def to_arrays(self):
+        """Sorted word array of the keys ((n,) or (n, 2)) and the values."""
         ks = sorted(self.storage)
-        return (np.array([k.bits for k in ks], dtype=np.uint64),
-                np.array([self.storage[k] for k in ks]))
+        kw = key_words(ks[0].M, ks[0].N) if ks else 1
+        return pack_keys([k.bits for k in ks], kw), np.array([self.storage[k] for k in ks])
 
     def __repr__(self):
         return f"{type(self).__name__} with {len(self)} entries, style = {self.style}, initiator = {self.initiator}"
@@ -117,7 +118,8 @@ class FrozenDVec:
         self.pairs = list(pairs)
 
     def keys_bits(self):
-        return np.array([k.bits for k, _ in self.pairs], dtype=np.uint64)
+        kw = key_words(self.pairs[0][0].M, self.pairs[0][0].N) if self.pairs else 1
+        return pack_keys([k.bits for k, _ in self.pairs], kw)
 
     def coefs(self):
         return np.array([float(v) for _, v in self.pairs], dtype=np.float64)
@@ -162,8 +164,8 @@ class DeviceDVec:
         keys, vals = self.engine.download()
         M, N = self.hamiltonian.M, self.hamiltonian.N
         out = DVec(style=self.style)
-        for k, v in zip(keys.tolist(), vals.tolist()):
-            out.storage[BoseFS(M=M, N=N, bits=int(k))] = v
+        for k, v in zip(unpack_keys(keys), vals.tolist()):
+            out.storage[BoseFS(M=M, N=N, bits=k)] = v
         return out
 
     def pairs(self):
@@ -171,10 +173,9 @@ class DeviceDVec:
 
     def __getitem__(self, addr: BoseFS):
         keys, vals = self.engine.download()
-        i = np.searchsorted(keys, np.uint64(addr.bits))
-        if i < len(keys) and keys[i] == addr.bits:
-            return vals[i].item()
-        return 0
+        index = {k: i for i, k in enumerate(unpack_keys(keys))}
+        i = index.get(addr.bits)
+        return vals[i].item() if i is not None else 0
 
     def __repr__(self):
         return f"DeviceDVec with {len(self)} entries, style = {self.style}, initiator = {self.initiator}"

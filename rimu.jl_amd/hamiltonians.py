@@ -13,7 +13,7 @@ from fractions import Fraction
 import numpy as np
 
 from . import _native
-from .addresses import BoseFS
+from .addresses import BoseFS, key_words, pack_keys, unpack_keys
 
 
 class AbstractHamiltonian:
@@ -43,43 +43,53 @@ class AbstractHamiltonian:
         if addr.M != self.M or addr.N != self.N:
             raise ValueError("address does not match the Hamiltonian's N and M")
 
+    @property
+    def key_words(self) -> int:
+        return key_words(self.M, self.N)
+
+    def _keys(self, bits_list):
+        return pack_keys(bits_list, self.key_words)
+
     # Interface -------------------------------------------------------------------------
     def diagonal_element(self, addr: BoseFS) -> float:
         self._check(addr)
-        d, _, _, _ = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [0])
+        d, _, _, _ = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits]), [0])
         return float(d[0])
 
     def num_offdiagonals(self, addr: BoseFS) -> int:
         self._check(addr)
-        _, n, _, _ = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [0])
+        _, n, _, _ = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits]), [0])
         return int(n[0])
 
     def get_offdiagonal(self, addr: BoseFS, chosen: int):
         self._check(addr)
-        _, n, na, ov = _native.hamiltonian_eval(self.descriptor(), [addr.bits], [chosen])
+        _, n, na, ov = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits]), [chosen])
         if not 1 <= chosen <= n[0]:
             raise IndexError(f"chosen={chosen} outside 1:{n[0]}")
-        return BoseFS(M=self.M, N=self.N, bits=int(na[0])), float(ov[0])
+        return BoseFS(M=self.M, N=self.N, bits=unpack_keys(na)[0]), float(ov[0])
 
     def offdiagonals(self, addr: BoseFS):
         self._check(addr)
         n = self.num_offdiagonals(addr)
         if n == 0:
             return []
-        _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), [addr.bits] * n,
+        _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits] * n),
                                                 list(range(1, n + 1)))
-        return [(BoseFS(M=self.M, N=self.N, bits=int(a)), float(v)) for a, v in zip(na, ov)]
+        return [(BoseFS(M=self.M, N=self.N, bits=a), float(v)) for a, v in zip(unpack_keys(na), ov)]
 
     def column(self, addrs_bits):
         """Diagonal elements and all off-diagonal (row, value) pairs of the columns of a batch
-        of addresses (given as bit patterns): returns (diag, rows, cols, vals)."""
+        of addresses (a word array, (n,) or (n, 2)): returns (diag, rows, cols, vals) with rows
+        a word array of the same kind."""
         addrs_bits = np.asarray(addrs_bits, dtype=np.uint64)
+        if self.key_words == 2:
+            addrs_bits = addrs_bits.reshape(-1, 2)
         d, nod, _, _ = _native.hamiltonian_eval(self.descriptor(), addrs_bits,
                                                 np.zeros(len(addrs_bits), dtype=np.int64))
         cols = np.repeat(np.arange(len(addrs_bits)), nod)
         chosen = np.concatenate([np.arange(1, n + 1) for n in nod]) if len(nod) else np.zeros(0, np.int64)
         if len(cols) == 0:
-            return d, np.zeros(0, np.uint64), cols, np.zeros(0)
+            return d, np.zeros((0, 2) if self.key_words == 2 else 0, np.uint64), cols, np.zeros(0)
         _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), addrs_bits[cols], chosen)
         return d, na, cols, ov
 

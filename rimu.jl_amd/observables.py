@@ -29,7 +29,7 @@ class G2RealCorrelator(AbstractObservable):
         self.d = int(d)
 
     def diagonal_element(self, addr: BoseFS) -> float:
-        onr = addr.onr()
+        onr = addr.onr
         M = len(onr)
         dd = self.d % M
         acc = sum(onr[i] * (onr[(i + dd) % M] - (dd == 0)) for i in range(M))
@@ -62,7 +62,7 @@ class DensityMatrixDiagonal(AbstractObservable):
         self.component = component
 
     def diagonal_element(self, addr: BoseFS) -> float:
-        return float(addr.onr()[self.mode - 1])
+        return float(addr.onr[self.mode - 1])
 
     def operator_descriptor(self, M, N):
         if not 1 <= self.mode <= M:

@@ -16,7 +16,7 @@ import time
 import numpy as np
 
 from . import _native
-from .addresses import BoseFS
+from .addresses import BoseFS, key_words, pack_keys, unpack_keys
 from .dictvectors import DVec, DeviceDVec, FrozenDVec, PDVec
 from .hamiltonians import AbstractHamiltonian
 from .initiators import Initiator, InitiatorRule, NonInitiator
@@ -111,13 +111,14 @@ def _start_vector(problem: ProjectorMonteCarloProblem) -> DVec:
 def rayleigh_quotient_dvec(h: AbstractHamiltonian, v: DVec) -> float:
     """dot(v, H, v) / dot(v, v) (fciqmc.jl:51-61, abstractdvec.jl:313-324)."""
     items = list(v.pairs())
-    keys = np.array([k.bits for k, _ in items], dtype=np.uint64)
+    bits = [k.bits for k, _ in items]
+    keys = pack_keys(bits, key_words(h.M, h.N))
     vals = np.array([float(x) for _, x in items])
     diag, rows, cols, offv = h.column(keys)
-    index = {int(k): i for i, k in enumerate(keys.tolist())}
+    index = {k: i for i, k in enumerate(bits)}
     num = float(np.dot(vals * diag, vals))
-    for r, c, x in zip(rows.tolist(), cols.tolist(), offv.tolist()):
-        i = index.get(int(r))
+    for r, c, x in zip(unpack_keys(rows), cols.tolist(), offv.tolist()):
+        i = index.get(r)
         if i is not None:
             num += vals[i] * x * vals[c]
     return num / float(np.dot(vals, vals))

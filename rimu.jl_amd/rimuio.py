@@ -9,9 +9,10 @@ metadata)`` produces it (``RimuIO.jl:113-118``, the table of ``tables.jl``):
   ``RimuIO.jl:149-152``);
 * ``key``: the Arrow extension type of BoseFS on a BitString (``arrowtypes.jl:58-83``):
   extension name ``Rimu.BoseFS.BitString``, extension metadata ``"N.M.B"`` (particles,
-  modes, bits), storage ``NTuple{1,T}`` = FixedSizeList of one non-nullable chunk, where the
-  chunk type T is UInt8/16/32/64 for B <= 8/16/32/64 (``num_chunks``,
-  ``bitstring.jl:6-18``) and the chunk is the bit string itself (mode 1 in the lowest bits);
+  modes, bits), storage ``NTuple{K,T}`` = FixedSizeList of K non-nullable chunks, where the
+  chunk type T is UInt8/16/32/64 for B <= 8/16/32/64 and K = 2 UInt64 chunks for
+  64 < B <= 128 (``num_chunks``, ``bitstring.jl:6-18``); chunk 1 holds the most significant
+  bits (``bitstring.jl:64-75``, mode 1 in the lowest bits of the last chunk);
 * ``value``: Int64 (integer walkers) or Float64;
 * schema metadata: ``RIMU_PACKAGE_VERSION`` followed by the keyword arguments of
   ``save_state``, each as a string; ``load_state`` parses each back as Int, Float64,
@@ -28,7 +29,7 @@ import re
 
 import numpy as np
 
-from .addresses import BoseFS
+from .addresses import BoseFS, key_words, pack_keys, unpack_keys
 from .dictvectors import DVec
 from .styles import IsDynamicSemistochastic, IsStochasticInteger
 
@@ -45,11 +46,13 @@ def _pa():
 
 
 def chunk_dtype(B: int):
-    """Chunk type of a one-chunk BitString of B bits (num_chunks, bitstring.jl:6-18)."""
+    """Chunk type of a BitString of B bits (num_chunks, bitstring.jl:6-18)."""
     for bits, dt in _CHUNK_TYPES:
         if B <= bits:
             return dt
-    raise ValueError("addresses above 64 bits need more than one chunk")
+    if B <= 128:
+        return np.uint64
+    raise ValueError("addresses above 128 bits are not supported")
 
 
 def _julia_string(v) -> str:
@@ -107,7 +110,7 @@ def _vector_arrays(vector):
             M, N = items[0][0].M, items[0][0].N
         else:
             M = N = None
-        keys = np.array([k.bits for k, _ in items], dtype=np.uint64)
+        keys = pack_keys([k.bits for k, _ in items], key_words(M, N) if items else 1)
         integer = isinstance(vector.style, IsStochasticInteger)
         vals = np.array([v for _, v in items], dtype=np.int64 if integer else np.float64)
         return M, N, keys, vals, vector.style
@@ -115,16 +118,25 @@ def _vector_arrays(vector):
 
 
 def state_table(M: int, N: int, keys: np.ndarray, vals: np.ndarray, **metadata):
-    """The Arrow table of a state (keys: u64 bit strings, vals: int64 or float64)."""
+    """The Arrow table of a state (keys: a word array, (n,) or (n, 2) [lo, hi]; vals: int64
+    or float64)."""
     pa, _ = _pa()
     B = M + N - 1
     dt = chunk_dtype(B)
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
-    if B < 64 and len(keys) and int(keys.max()) >> B:
-        raise ValueError("key outside the B-bit address space")
+    nchunks = key_words(M, N)
+    if nchunks == 1:
+        if B < 64 and len(keys) and int(keys.max()) >> B:
+            raise ValueError("key outside the B-bit address space")
+        flat = keys.astype(dt, copy=False)
+    else:
+        keys = keys.reshape(-1, 2)
+        if len(keys) and int(keys[:, 1].max()) >> (B - 64):
+            raise ValueError("key outside the B-bit address space")
+        flat = np.ascontiguousarray(keys[:, ::-1]).reshape(-1)   # chunk 1 = high word first
     chunk_type = pa.from_numpy_dtype(dt)
-    key_type = pa.list_(pa.field("item", chunk_type, nullable=False), 1)
-    key_arr = pa.FixedSizeListArray.from_arrays(pa.array(keys.astype(dt, copy=False)), type=key_type)
+    key_type = pa.list_(pa.field("item", chunk_type, nullable=False), nchunks)
+    key_arr = pa.FixedSizeListArray.from_arrays(pa.array(flat), type=key_type)
     vals = np.ascontiguousarray(vals)
     if vals.dtype not in (np.int64, np.float64):
         raise TypeError("values must be int64 or float64")
@@ -163,13 +175,21 @@ def read_state_arrays(filename):
     if name != BOSEFS_BS:
         raise NotImplementedError(f"key type {name or kf.type} is not a one-chunk BoseFS")
     N, M, B = (int(x) for x in fm[b"ARROW:extension:metadata"].decode().split("."))
-    if B != N + M - 1 or B > 64 or kf.type.list_size != 1:
-        raise NotImplementedError("only BoseFS on one BitString chunk is supported")
+    nchunks = key_words(M, N)
+    if B != N + M - 1 or B > 128 or kf.type.list_size != nchunks:
+        raise NotImplementedError("only BoseFS on one or two BitString chunks is supported")
     col = table.column("key").combine_chunks()
-    keys = col.flatten().to_numpy(zero_copy_only=False).astype(np.uint64)
+    flat = col.flatten().to_numpy(zero_copy_only=False).astype(np.uint64)
+    if nchunks == 1:
+        keys = flat
+        ascending = len(keys) < 2 or bool(np.all(keys[1:] > keys[:-1]))
+    else:
+        keys = np.ascontiguousarray(flat.reshape(-1, 2)[:, ::-1])   # back to [lo, hi]
+        ascending = len(keys) < 2 or bool(np.all(
+            (keys[1:, 1] > keys[:-1, 1]) | ((keys[1:, 1] == keys[:-1, 1]) & (keys[1:, 0] > keys[:-1, 0]))))
     vals = table.column("value").combine_chunks().to_numpy(zero_copy_only=False)
-    if len(keys) > 1 and not np.all(keys[1:] > keys[:-1]):   # files from PDVecs are unsorted
-        order = np.argsort(keys, kind="stable")
+    if not ascending:   # files from PDVecs are unsorted
+        order = np.argsort(keys, kind="stable") if nchunks == 1 else np.lexsort((keys[:, 0], keys[:, 1]))
         keys, vals = keys[order], vals[order]
     raw = table.schema.metadata or {}
     meta = {k.decode(): _parse_meta(k.decode(), v.decode()) for k, v in raw.items()}
@@ -184,8 +204,8 @@ def load_state(filename, style=None):
     if style is None:
         style = IsStochasticInteger() if vals.dtype.kind in "iu" else IsDynamicSemistochastic()
     out = DVec(style=style)
-    for k, v in zip(keys.tolist(), vals.tolist()):
-        out[BoseFS(M=M, N=N, bits=int(k))] = v
+    for k, v in zip(unpack_keys(keys), vals.tolist()):
+        out[BoseFS(M=M, N=N, bits=k)] = v
     return out, meta
 
 

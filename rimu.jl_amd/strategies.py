@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .dictvectors import DVec, FrozenDVec, freeze
-from .addresses import BoseFS
+from .addresses import BoseFS, unpack_keys
 
 
 @dataclass
@@ -120,9 +120,9 @@ class ProjectedEnergy:
         coefs = fp.coefs()
         diag, rows, cols, vals = h.column(keys)
         acc = {}
-        for k, d, c in zip(keys.tolist(), diag.tolist(), coefs.tolist()):
+        for k, d, c in zip(unpack_keys(keys), diag.tolist(), coefs.tolist()):
             acc[k] = acc.get(k, 0.0) + d * c
-        for r, j, v in zip(rows.tolist(), cols.tolist(), vals.tolist()):
+        for r, j, v in zip(unpack_keys(rows), cols.tolist(), vals.tolist()):
             acc[r] = acc.get(r, 0.0) + v * coefs[j]
         M, N = h.M, h.N
         return FrozenDVec([(BoseFS(M=M, N=N, bits=k), v) for k, v in acc.items() if v != 0])

@@ -44,12 +44,16 @@ def test_errors_without_device(rimu):
 
 def test_argument_errors(rimu):
     m = rimu._native.Model()
-    m.kind, m.M, m.N = 1, 40, 30  # 69 bits: does not fit one chunk
+    m.kind, m.M, m.N = 1, 70, 60  # 129 bits: does not fit two chunks
     s = rimu.IsStochasticInteger().descriptor()
     h = ctypes.c_void_p()
     rc = rimu._native.lib().rimu_engine_create(ctypes.byref(m), ctypes.byref(s), 0, 10, 10, ctypes.byref(h))
     assert rc == rimu._native.RIMU_ERR_ARG
-    assert b"64-bit" in rimu._native.lib().rimu_last_error()
+    assert b"two 64-bit chunks" in rimu._native.lib().rimu_last_error()
+    m.kind, m.M, m.N = 2, 65, 10  # HubbardMom1D's dispersion table holds 64 modes
+    rc = rimu._native.lib().rimu_engine_create(ctypes.byref(m), ctypes.byref(s), 0, 10, 10, ctypes.byref(h))
+    assert rc == rimu._native.RIMU_ERR_ARG
+    m.kind = 1
     s2 = rimu.IsStochasticInteger().descriptor()
     s2.proj_threshold = 1.0
     m.M, m.N = 5, 5

@@ -24,7 +24,8 @@ def R(rimu):
 
 
 @pytest.mark.parametrize("M,N,chunk", [(4, 5, "uint8"), (9, 8, "uint16"), (17, 16, "uint32"),
-                                       (10, 10, "uint32"), (20, 20, "uint64"), (33, 32, "uint64")])
+                                       (10, 10, "uint32"), (20, 20, "uint64"), (33, 32, "uint64"),
+                                       (50, 50, "uint64"), (64, 64, "uint64")])
 @pytest.mark.parametrize("vtype", ["int", "float"])
 def test_state_round_trip_and_layout(R, tmp_path, M, N, chunk, vtype):
     rng = np.random.default_rng(M * 100 + N)
@@ -35,13 +36,15 @@ def test_state_round_trip_and_layout(R, tmp_path, M, N, chunk, vtype):
         cuts = np.sort(rng.choice(B, M - 1, replace=False))
         parts = np.diff(np.concatenate([[-1], cuts, [B]])) - 1
         keys.add(O.from_onr(parts))
-    keys = np.array(sorted(keys), dtype=np.uint64)
+    ints = sorted(keys)
+    kw = 2 if B > 64 else 1
+    keys = O.pack_keys(ints, kw)
     vals = (rng.integers(-50, 51, size=len(keys)) if vtype == "int"
             else rng.normal(size=len(keys)) * 10.0 ** rng.integers(-5, 5, size=len(keys)))
     vals[vals == 0] = 1
     style = R.IsStochasticInteger() if vtype == "int" else R.IsDeterministic()
     dv = R.DVec(style=style)
-    for k, v in zip(keys.tolist(), vals.tolist()):
+    for k, v in zip(ints, vals.tolist()):
         dv[R.BoseFS(M=M, N=N, bits=k)] = v
     f = tmp_path / "state.arrow"
     R.save_state(f, dv, step=12, shift=-1.25)
@@ -51,7 +54,7 @@ def test_state_round_trip_and_layout(R, tmp_path, M, N, chunk, vtype):
     assert t.schema.names == ["key", "value"]
     kf = t.schema.field("key")
     assert not kf.nullable and not t.schema.field("value").nullable
-    assert pa.types.is_fixed_size_list(kf.type) and kf.type.list_size == 1
+    assert pa.types.is_fixed_size_list(kf.type) and kf.type.list_size == kw
     assert kf.type.value_type == getattr(pa, chunk)() and not kf.type.value_field.nullable
     assert kf.metadata[b"ARROW:extension:name"] == b"Rimu.BoseFS.BitString"
     assert kf.metadata[b"ARROW:extension:metadata"] == f"{N}.{M}.{B}".encode()
@@ -59,12 +62,16 @@ def test_state_round_trip_and_layout(R, tmp_path, M, N, chunk, vtype):
     meta = t.schema.metadata
     assert meta[b"RIMU_PACKAGE_VERSION"] == b"0.14.0" and meta[b"step"] == b"12"
     assert meta[b"shift"] == b"-1.25"
-    chunks = t.column("key").combine_chunks().flatten().to_numpy()
-    assert set(chunks.astype(np.uint64).tolist()) == set(keys.tolist())
+    chunks = t.column("key").combine_chunks().flatten().to_numpy().astype(np.uint64)
+    if kw == 1:
+        assert set(chunks.tolist()) == set(ints)
+    else:   # chunk 1 holds the most significant bits (bitstring.jl:64-75)
+        pairs = chunks.reshape(-1, 2).tolist()
+        assert set((hi << 64) | lo for hi, lo in pairs) == set(ints)
     # and back
     out, md = R.load_state(f)
     assert md["step"] == 12 and md["shift"] == -1.25
-    assert dict((k.bits, v) for k, v in out.pairs()) == dict(zip(keys.tolist(), vals.tolist()))
+    assert dict((k.bits, v) for k, v in out.pairs()) == dict(zip(ints, vals.tolist()))
     assert isinstance(out.style, R.IsStochasticInteger if vtype == "int" else R.IsDynamicSemistochastic)
 
 
@@ -109,10 +116,11 @@ def test_empty_state(R, tmp_path):
 def test_chunk_type_rule():
     # num_chunks, bitstring.jl:6-18
     from rimu_amd.rimuio import chunk_dtype
-    assert [chunk_dtype(b) for b in (1, 8, 9, 16, 17, 32, 33, 64)] == \
-        [np.uint8, np.uint8, np.uint16, np.uint16, np.uint32, np.uint32, np.uint64, np.uint64]
+    assert [chunk_dtype(b) for b in (1, 8, 9, 16, 17, 32, 33, 64, 65, 128)] == \
+        [np.uint8, np.uint8, np.uint16, np.uint16, np.uint32, np.uint32, np.uint64, np.uint64,
+         np.uint64, np.uint64]
     with pytest.raises(ValueError):
-        chunk_dtype(65)
+        chunk_dtype(129)
 
 
 def test_report_round_trip_and_compression(R, tmp_path):
