@@ -155,17 +155,20 @@ struct ModelDev {
     double kes[64];
 };
 
-// Sum over occupied modes of n(n-1) (bose_hubbard_interaction, bosefs.jl:415-428).
+// Sum over occupied modes of n(n-1) (bose_hubbard_interaction, bosefs.jl:415-428), bit
+// parallel: y_d = bits & (bits >> 1) & ... & (bits >> d) holds n - d ones for every run of
+// n ones (runs are separated by zeros), so sum_{d >= 1} popcount(y_d) = sum n(n-1)/2.  The
+// loop runs (largest occupation - 1) times instead of once per occupied mode.
 template <class A> __device__ inline int64_t bose_hubbard_interaction(A bits) {
     int64_t acc = 0;
-    A x = bits;
-    while (x) {
-        x = shrA(x, ctzA(x));
-        const int n = ctzA((A)~x);
-        acc += (int64_t)n * (n - 1);
-        x = shrA(x, n);
+    A s = bits >> 1;
+    A y = bits & s;
+    while (y) {
+        acc += popcA(y);
+        s >>= 1;
+        y &= s;
     }
-    return acc;
+    return 2 * acc;
 }
 
 template <class A> __device__ inline int num_occupied(A bits) { return popcA(run_starts(bits)); }
