@@ -39,9 +39,11 @@ enum {
     RIMU_ERR_INTERNAL = 5
 };
 
-/* Hamiltonian kinds: HubbardReal1D (Hamiltonians/HubbardReal1D.jl:22-62) and
- * HubbardMom1D (Hamiltonians/HubbardMom1D.jl:43-239, BoseFS branch). */
-enum { RIMU_MODEL_HUBBARD_REAL_1D = 1, RIMU_MODEL_HUBBARD_MOM_1D = 2 };
+/* Hamiltonian kinds: HubbardReal1D (Hamiltonians/HubbardReal1D.jl:22-62),
+ * HubbardMom1D (Hamiltonians/HubbardMom1D.jl:43-239, BoseFS branch) and HubbardRealSpace
+ * (Hamiltonians/HubbardRealSpace.jl:130-391 with the CubicGrid geometries of
+ * Hamiltonians/geometry.jl) on a BoseFS, a FermiFS or a two-component CompositeFS of them. */
+enum { RIMU_MODEL_HUBBARD_REAL_1D = 1, RIMU_MODEL_HUBBARD_MOM_1D = 2, RIMU_MODEL_HUBBARD_REAL_SPACE = 3 };
 
 /* Stochastic styles: IsStochasticInteger (StochasticStyles/styles.jl:11-25),
  * IsDeterministic (:76-105), IsDynamicSemistochastic (:175-214). */
@@ -52,16 +54,37 @@ enum {
 };
 
 /* Plain data of a Hamiltonian: the fields of HubbardReal1D{TT,A,U,T} (type parameters
- * u, t; HubbardReal1D.jl:46-49) or HubbardMom1D (u, t and the dispersion table kes,
- * HubbardMom1D.jl:43-47, :80-85).  kes[j-1] is epsilon of mode j. */
+ * u, t; HubbardReal1D.jl:46-49), HubbardMom1D (u, t and the dispersion table kes,
+ * HubbardMom1D.jl:43-47, :80-85; kes[j-1] is epsilon of mode j) or HubbardRealSpace
+ * (HubbardRealSpace.jl:163-172: t, u, the precomputed trap potential and the geometry,
+ * given here as its neighbour table).
+ *
+ * HubbardRealSpace keys: a single component is its own bit string (BoseFS: N + M - 1
+ * bits as for the 1D models; FermiFS: M bits, site j at bit j-1, fermifs.jl and
+ * bitstring.jl:713-733); two components are component 1's bits above component 2's, so
+ * numeric order is CompositeFS's isless (multicomponent.jl:79).  At most 128 bits. */
 typedef struct rimu_model {
     int32_t kind;
-    int32_t M;
-    int32_t N;
+    int32_t M;          /* modes (sites) */
+    int32_t N;          /* particles (HubbardRealSpace: of component 1) */
     int32_t reserved;
-    double u;
-    double t;
-    double kes[64];
+    double u;           /* HubbardRealSpace: u[1,1] */
+    double t;           /* HubbardRealSpace: t[1] */
+    double kes[64];     /* HubbardRealSpace: trap potential of component 1 at site j+1 */
+    /* HubbardRealSpace only (zero for the 1D models): */
+    int32_t ncomp;      /* 1, or 2 for a CompositeFS */
+    int32_t N2;         /* particles of component 2 */
+    int32_t fermi;      /* bit c set: component c+1 is a FermiFS, else a BoseFS */
+    int32_t ndir;       /* 2D neighbour directions per site, D <= 3 */
+    int32_t has_u;      /* 0 when u is the zero matrix (HubbardRealSpace.jl:211) */
+    int32_t has_pot;    /* 1 when the trap strengths v are nonzero (:214-224) */
+    double t2;          /* t[2] */
+    double u12;         /* u[1,2] = u[2,1] */
+    double u22;         /* u[2,2] */
+    double pot2[64];    /* trap potential of component 2 at site j+1 */
+    uint8_t nbr[64][6]; /* neighbor_site(geometry, s+1, d+1) - 1 (geometry.jl:236-240), or 255
+                           where the move leaves a hard wall; d in Directions order
+                           (+e1, .., +eD, -e1, .., -eD, geometry.jl:127-160) */
 } rimu_model;
 
 /* Plain data of a StochasticStyle and its CompressionStrategy.

@@ -29,6 +29,7 @@ _LIB = None
 
 MODEL_REAL1D = 1
 MODEL_MOM1D = 2
+MODEL_REALSPACE = 3
 STYLE_INTEGER = 1
 STYLE_DETERMINISTIC = 2
 STYLE_DYNAMIC = 3
@@ -36,7 +37,11 @@ STYLE_DYNAMIC = 3
 
 class Model(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("M", ctypes.c_int32), ("N", ctypes.c_int32),
-                ("u", ctypes.c_double), ("t", ctypes.c_double), ("kes", ctypes.c_double * 64)]
+                ("u", ctypes.c_double), ("t", ctypes.c_double), ("kes", ctypes.c_double * 64),
+                ("ncomp", ctypes.c_int32), ("N2", ctypes.c_int32), ("fermi", ctypes.c_int32),
+                ("ndir", ctypes.c_int32), ("has_u", ctypes.c_int32), ("has_pot", ctypes.c_int32),
+                ("t2", ctypes.c_double), ("u12", ctypes.c_double), ("u22", ctypes.c_double),
+                ("pot2", ctypes.c_double * 64), ("nbr", (ctypes.c_uint8 * 6) * 64)]
 
 
 class Style(ctypes.Structure):
@@ -155,6 +160,118 @@ def mom1d(M: int, N: int, u: float = 1.0, t: float = 1.0) -> Model:
     return m
 
 
+def cubic_neighbours(dims, fold):
+    """neighbor_site(geometry, s, d) - 1 for every site s and direction d, 255 where the move
+    leaves a hard wall.  Sites are linear indices with the first dimension fastest
+    (CartesianIndices / LinearIndices, geometry.jl:116-121); directions are +e_1..+e_D then
+    -e_1..-e_D (Directions, geometry.jl:127-160); a periodic dimension folds with mod1
+    (fold_vec, geometry.jl:100-112)."""
+    D = len(dims)
+    table = []
+    for s in range(math.prod(dims)):
+        x, r = [], s
+        for n in dims:
+            x.append(r % n + 1)
+            r //= n
+        row = []
+        for d in range(2 * D):
+            y = list(x)
+            y[d % D] += 1 if d < D else -1
+            ok = True
+            for i in range(D):
+                if fold[i]:
+                    y[i] = (y[i] - 1) % dims[i] + 1
+                elif not 1 <= y[i] <= dims[i]:
+                    ok = False
+            if not ok:
+                row.append(255)
+                continue
+            lin, mul = 0, 1
+            for i in range(D):
+                lin += (y[i] - 1) * mul
+                mul *= dims[i]
+            row.append(lin)
+        table.append(row)
+    return table
+
+
+def trap_potential(dims, v_c):
+    """Potential per site of one component: sum_d v_c[d] * x_d^2 with x_d running over
+    range(-fld(n_d, 2); length = n_d) (HubbardRealSpace.jl:216-223)."""
+    out = []
+    for s in range(math.prod(dims)):
+        r, acc = s, None
+        for d, n in enumerate(dims):
+            x = (r % n) - n // 2
+            r //= n
+            term = v_c[d] * (x * x)
+            acc = term if acc is None else acc + term
+        out.append(acc)
+    return out
+
+
+def real_space(dims, comps, t=None, u=None, v=None, fold=None) -> Model:
+    """HubbardRealSpace on a CubicGrid (PeriodicBoundaries by default).  comps: one or two
+    ("bose" | "fermi", N) pairs; t: per component (default ones); u: C x C (default ones);
+    v: C x D trap strengths (default zeros)."""
+    C = len(comps)
+    D = len(dims)
+    fold = tuple(fold) if fold is not None else (True,) * D
+    t = list(t) if t is not None else [1.0] * C
+    u = np.array(u if u is not None else np.ones((C, C)), dtype=float).reshape(C, C)
+    v = np.array(v if v is not None else np.zeros((C, D)), dtype=float).reshape(C, D)
+    m = Model()
+    m.model, m.M, m.N = MODEL_REALSPACE, math.prod(dims), comps[0][1]
+    m.ncomp, m.N2 = C, comps[1][1] if C == 2 else 0
+    m.fermi = sum(1 << c for c, (kind, _) in enumerate(comps) if kind == "fermi")
+    m.ndir = 2 * D
+    m.t, m.u = float(t[0]), float(u[0, 0])
+    if C == 2:
+        m.t2, m.u12, m.u22 = float(t[1]), float(u[0, 1]), float(u[1, 1])
+    m.has_u = int(bool(np.any(u != 0)))
+    m.has_pot = int(bool(np.any(v != 0)))
+    if m.has_pot:
+        for j, x in enumerate(trap_potential(dims, v[0])):
+            m.kes[j] = x
+        if C == 2:
+            for j, x in enumerate(trap_potential(dims, v[1])):
+                m.pot2[j] = x
+    for s, row in enumerate(cubic_neighbours(dims, fold)):
+        for d, x in enumerate(row):
+            m.nbr[s][d] = x
+    return m
+
+
+def rs_key(model, onrs) -> int:
+    """Key of a HubbardRealSpace address from the occupation numbers of each component."""
+    parts = []
+    for c, onr in enumerate(onrs):
+        if (model.fermi >> c) & 1:
+            parts.append((sum(1 << i for i, n in enumerate(onr) if n), model.M))
+        else:
+            parts.append((from_onr(onr) if sum(onr) else 0, sum(onr) + model.M - 1))
+    key = 0
+    for bits, w in parts:
+        key = (key << w) | bits
+    return key
+
+
+def rs_onrs(model, key: int):
+    """Occupation numbers of each component of a HubbardRealSpace key."""
+    widths = [model.M if (model.fermi >> c) & 1 else n + model.M - 1
+              for c, n in enumerate([model.N, model.N2][:model.ncomp])]
+    out = []
+    for c in reversed(range(model.ncomp)):
+        w = widths[c]
+        bits = key & ((1 << w) - 1)
+        key >>= w
+        if (model.fermi >> c) & 1:
+            out.append(tuple((bits >> i) & 1 for i in range(model.M)))
+        else:
+            out.append(to_onr(bits, model.M))
+    return tuple(reversed(out))
+
+
 def style(kind: str, compression_threshold: float = 1.0, proj_threshold: float = 0.0,
           rel_threshold: float = 1.0, abs_threshold: float = math.inf, compression=None) -> Style:
     s = Style()
@@ -176,9 +293,18 @@ def style(kind: str, compression_threshold: float = 1.0, proj_threshold: float =
 MASK64 = (1 << 64) - 1
 
 
+def model_bits(model) -> int:
+    """Bits of an address: N + M - 1 for the BoseFS models; for HubbardRealSpace the sum
+    over components (M for a FermiFS, N_c + M - 1 for a BoseFS)."""
+    if model.model != 3:
+        return model.M + model.N - 1
+    w = lambda c, n: model.M if (model.fermi >> c) & 1 else n + model.M - 1
+    return w(0, model.N) + (w(1, model.N2) if model.ncomp == 2 else 0)
+
+
 def key_words(model) -> int:
-    """64-bit words per address: 1, or 2 when N + M - 1 > 64."""
-    return 2 if model.M + model.N - 1 > 64 else 1
+    """64-bit words per address: 1, or 2 when the address has more than 64 bits."""
+    return 2 if model_bits(model) > 64 else 1
 
 
 def pack_keys(ints, kw: int) -> np.ndarray:

@@ -14,8 +14,9 @@
 // Addresses are BoseFS on one BitString chunk (B = N + M - 1 <= 64 bits) or two (B <= 128,
 // held as one 128-bit integer whose low 64 bits are the last chunk), layout of
 // bitstring.jl:464-508: mode 1 occupies the least significant bits, each mode is written as
-// n_i one-bits followed by a zero separator.  Across the C interface a two-chunk address is
-// two uint64 words, the low word first.
+// n_i one-bits followed by a zero separator.  HubbardRealSpace also takes FermiFS (site j
+// at bit j-1) and two-component keys (component 1 above component 2).  Across the C
+// interface a two-chunk address is two uint64 words, the low word first.
 
 #include <algorithm>
 #include <cmath>
@@ -220,10 +221,21 @@ inline int64_t bose_hubbard_interaction(const BoseIndex* occ, int nocc) {
 extern "C" {
 
 typedef struct {
-    int32_t model;  // 1 = HubbardReal1D, 2 = HubbardMom1D
-    int32_t M, N;
-    double u, t;
-    double kes[64];  // HubbardMom1D dispersion table, mode j at index j-1
+    int32_t model;  // 1 = HubbardReal1D, 2 = HubbardMom1D, 3 = HubbardRealSpace
+    int32_t M, N;   // modes (sites); particles (HubbardRealSpace: of component 1)
+    double u, t;    // HubbardRealSpace: u[1,1], t[1]
+    double kes[64];  // HubbardMom1D dispersion table, mode j at index j-1; HubbardRealSpace:
+                     // trap potential of component 1 per site
+    // HubbardRealSpace only
+    int32_t ncomp;   // 1 or 2 components
+    int32_t N2;      // particles of component 2
+    int32_t fermi;   // bit c: component c+1 is a FermiFS
+    int32_t ndir;    // 2D neighbour directions
+    int32_t has_u;   // u is not the zero matrix
+    int32_t has_pot; // trap strengths are nonzero
+    double t2, u12, u22;
+    double pot2[64];
+    uint8_t nbr[64][6];  // neighbor_site(geometry, s+1, d+1) - 1, 255 = none
 } or_model;
 
 typedef struct {
@@ -269,9 +281,169 @@ int64_t or_bose_hubbard_interaction(uint64_t bs) {
 
 namespace {
 
+// ---------------------------------------------------------------------------------
+// HubbardRealSpace (Hamiltonians/HubbardRealSpace.jl) on a BoseFS, a FermiFS or a
+// CompositeFS of two of them.  A two-component key holds component 1 above component 2
+// (CompositeFS isless compares components in order, multicomponent.jl:79); the geometry
+// arrives as the neighbour table nbr (neighbor_site, geometry.jl:236-240).
+// ---------------------------------------------------------------------------------
+inline bool rs_fermi(const or_model* h, int c) { return (h->fermi >> c) & 1; }
+inline int rs_width(const or_model* h, int c) {
+    return rs_fermi(h, c) ? h->M : (c == 0 ? h->N : h->N2) + h->M - 1;
+}
+inline int model_bits(const or_model* h) {
+    if (h->model != 3) return h->M + h->N - 1;
+    return rs_width(h, 0) + (h->ncomp == 2 ? rs_width(h, 1) : 0);
+}
+template <class A> inline A low_bits(int n) { return n >= WIDTH<A> ? (A)~(A)0 : (A)(((A)1 << n) - 1); }
+template <class A> inline A rs_component(const or_model* h, A key, int c) {
+    if (h->ncomp == 1) return key;
+    const int w2 = rs_width(h, 1);
+    return c == 0 ? (A)(key >> w2) : (A)(key & low_bits<A>(w2));
+}
+// update_component, multicomponent.jl:70-77
+template <class A> inline A rs_update_component(const or_model* h, A key, int c, A v) {
+    if (h->ncomp == 1) return v;
+    const int w2 = rs_width(h, 1);
+    return c == 0 ? (A)((v << w2) | (key & low_bits<A>(w2))) : (A)((key & ~low_bits<A>(w2)) | v);
+}
+
+// Occupied modes of one component as BoseFSIndex-like records (occnum, mode); a FermiFS
+// lists its set bits in ascending order with occnum 1 (bitstring.jl:794-831).
+template <class A> inline int rs_occupied(const or_model* h, A b, int c, BoseIndex* out) {
+    if (!rs_fermi(h, c)) return occupied_modes(b, out);
+    int n = 0;
+    for (int i = 0; i < WIDTH<A>; ++i)
+        if ((b >> i) & 1) out[n++] = BoseIndex{1, i + 1, i};
+    return n;
+}
+
+// dot(occupied_modes(a), occupied_modes(b)), fockaddress.jl:692-719; for two FermiFS
+// bitstrings count_ones(a & b), bitstring.jl:833-838.
+template <class A> inline int64_t rs_dot(const or_model* h, A a, A b) {
+    if (rs_fermi(h, 0) && rs_fermi(h, 1)) {
+        int64_t n = 0;
+        for (int i = 0; i < WIDTH<A>; ++i) n += ((a & b) >> i) & 1;
+        return n;
+    }
+    BoseIndex oa[128], ob[128];
+    const int na = rs_occupied(h, a, 0, oa), nb = rs_occupied(h, b, 1, ob);
+    int64_t acc = 0;
+    int i = 0, j = 0;
+    while (i < na && j < nb) {
+        if (oa[i].mode > ob[j].mode) ++j;
+        else if (oa[i].mode < ob[j].mode) ++i;
+        else { acc += (int64_t)oa[i].occnum * ob[j].occnum; ++i; ++j; }
+    }
+    return acc;
+}
+
+// local_interaction(b, u) = u * bose_hubbard_interaction(b) / 2 for a BoseFS, 0 for a
+// FermiFS (HubbardRealSpace.jl:18-19).
+template <class A> inline double rs_self(const or_model* h, A b, int c, double u) {
+    if (rs_fermi(h, c)) return 0.0;
+    BoseIndex occ[128];
+    const int n = occupied_modes(b, occ);
+    return u * (double)bose_hubbard_interaction(occ, n) / 2;
+}
+
+// external_potential, HubbardRealSpace.jl:98-106: sum over occupied modes of n * pot[i].
+template <class A> inline double rs_pot(const or_model* h, A b, int c, const double* pot) {
+    BoseIndex occ[128];
+    const int n = rs_occupied(h, b, c, occ);
+    double pe = 0.0;
+    for (int i = 0; i < n; ++i) pe += occ[i].occnum * pot[occ[i].mode - 1];
+    return pe;
+}
+
+// diagonal_element, HubbardRealSpace.jl:264-279, with _interactions (:57-73) unrolled for
+// two components: self(1) + cross(1,2), then + self(2).
+template <class A> inline double rs_diagonal(const or_model* h, A key) {
+    const A a = rs_component(h, key, 0);
+    double intr = 0.0;
+    if (h->has_u) {
+        if (h->ncomp == 1) {
+            intr = rs_self(h, a, 0, h->u);
+        } else {
+            const A b = rs_component(h, key, 1);
+            const double self_a = rs_self(h, a, 0, h->u);
+            const double row = h->u12 * (double)rs_dot(h, a, b);
+            intr = self_a + row;
+            intr = intr + rs_self(h, b, 1, h->u22);
+        }
+    }
+    double pot = 0.0;
+    if (h->has_pot) {
+        if (h->ncomp == 1) {
+            pot = rs_pot(h, a, 0, h->kes);
+        } else {
+            pot += rs_pot(h, a, 0, h->kes);
+            pot += rs_pot(h, rs_component(h, key, 1), 1, h->pot2);
+        }
+    }
+    return intr + pot;
+}
+
+template <class A> inline int64_t rs_num_offdiagonals(const or_model* h, A key) {
+    BoseIndex occ[128];
+    int64_t n = rs_occupied(h, rs_component(h, key, 0), 0, occ);
+    if (h->ncomp == 2) n += rs_occupied(h, rs_component(h, key, 1), 1, occ);
+    return n * h->ndir;
+}
+
+// fermi_excitation for one hop, bitstring.jl:741-791: the destruction, then the creation,
+// each counting the ones below its bit; (unchanged, 0.0) when a flip finds the wrong bit.
+template <class A> inline A fermi_hop(A bs, int dst_mode, int src_mode, double* val) {
+    const A orig = bs;
+    int count = 0;
+    const int d = src_mode - 1, c = dst_mode - 1;
+    for (int i = 0; i < d; ++i) count += (bs >> i) & 1;
+    bs ^= (A)1 << d;
+    if ((bs >> d) & 1) { *val = 0.0; return orig; }
+    for (int i = 0; i < c; ++i) count += (bs >> i) & 1;
+    bs ^= (A)1 << c;
+    if (!((bs >> c) & 1)) { *val = 0.0; return orig; }
+    *val = (count % 2 == 0) ? 1.0 : -1.0;
+    return bs;
+}
+
+// HubbardRealSpaceCompOffdiagonals getindex (HubbardRealSpace.jl:303-316) inside the
+// component walk of HubbardRealSpaceOffdiagonals (:380-391).
+template <class A> inline A rs_offdiagonal(const or_model* h, A key, int64_t chosen, double* value) {
+    BoseIndex occ[128];
+    int c = 0;
+    A b = rs_component(h, key, 0);
+    int nocc = rs_occupied(h, b, 0, occ);
+    if (h->ncomp == 2 && chosen > (int64_t)nocc * h->ndir) {
+        chosen -= (int64_t)nocc * h->ndir;
+        c = 1;
+        b = rs_component(h, key, 1);
+        nocc = rs_occupied(h, b, 1, occ);
+    }
+    int64_t particle, neigh;
+    fldmod1(chosen, h->ndir, &particle, &neigh);
+    const BoseIndex src = occ[particle - 1];           // find_occupied_mode
+    const int nsite = h->nbr[src.mode - 1][neigh - 1];
+    if (nsite == 255) { *value = 0.0; return key; }     // neighbor_site == 0
+    const double t = c == 0 ? h->t : h->t2;
+    double v;
+    A nb;
+    if (rs_fermi(h, c)) {
+        nb = fermi_hop(b, nsite + 1, src.mode, &v);
+    } else {
+        BoseIndex all[128];
+        const int nall = occupied_modes(b, all);
+        const BoseIndex dst = find_mode(all, nall, nsite + 1);
+        nb = excitation(b, h->M, &dst, &src, 1, &v);
+    }
+    *value = -t * v;
+    return rs_update_component(h, key, c, nb);
+}
+
 // num_offdiagonals: HubbardReal1D.jl:51-53; HubbardMom1D.jl:131-140.
 template <class A>
 int64_t num_offdiagonals_t(const or_model* h, A bs) {
+    if (h->model == 3) return rs_num_offdiagonals(h, bs);
     BoseIndex occ[128];
     const int s = occupied_modes(bs, occ);
     if (h->model == 1) return 2 * (int64_t)s;
@@ -285,6 +457,7 @@ int64_t num_offdiagonals_t(const or_model* h, A bs) {
 // momentum_transfer_diagonal excitations.jl:126-137 and dot(kes, map) fockaddress.jl:316-321.
 template <class A>
 double diagonal_element_t(const or_model* h, A bs) {
+    if (h->model == 3) return rs_diagonal(h, bs);
     BoseIndex occ[128];
     const int s = occupied_modes(bs, occ);
     if (h->model == 1) {
@@ -354,6 +527,7 @@ A mom1d_offdiagonal(const or_model* h, A bs, const BoseIndex* occ, int nocc, int
 
 template <class A>
 A get_offdiagonal_t(const or_model* h, A bs, int64_t chosen, double* value) {
+    if (h->model == 3) return rs_offdiagonal(h, bs, chosen, value);
     BoseIndex occ[128];
     const int nocc = occupied_modes(bs, occ);
     if (h->model == 1) return real1d_offdiagonal(h, bs, occ, nocc, chosen, value);
@@ -481,8 +655,9 @@ void apply_column(const Params& p, A addr, V val, Stats& st, Emit&& emit) {
     const int64_t n_od = num_offdiagonals_t(h, addr);
     auto offdiag = [&](int64_t chosen, double* mel) {
         double hv;
-        const A nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, chosen, &hv)
-                                     : mom1d_offdiagonal(h, addr, occ, nocc, chosen, &hv);
+        const A nb = (h->model == 3) ? rs_offdiagonal(h, addr, chosen, &hv)
+                     : (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, chosen, &hv)
+                                       : mom1d_offdiagonal(h, addr, occ, nocc, chosen, &hv);
         *mel = -hv * p.dtau;
         return nb;
     };
@@ -760,7 +935,7 @@ int64_t or_step_initiator(const or_model* h, const or_style* s, uint64_t seed, i
                           int32_t rule, double threshold, const uint64_t* keys, const void* vals,
                           int64_t n, uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
     Params p{h, s, seed, step, shift, dtau, boost};
-    if (h->M + h->N - 1 > 64) {
+    if (model_bits(h) > 64) {
         if (vtype == 0)
             return step_initiator_impl<u128, int64_t>(p, order, rule, threshold, keys, (const int64_t*)vals,
                                                       n, out_keys, (int64_t*)out_vals, cap, stats);
@@ -786,7 +961,7 @@ int64_t or_step(const or_model* h, const or_style* s, uint64_t seed, int64_t ste
                 const uint64_t* keys, const void* vals, int64_t n,
                 uint64_t* out_keys, void* out_vals, int64_t cap, double* stats) {
     Params p{h, s, seed, step, shift, dtau, boost};
-    if (h->M + h->N - 1 > 64) {   // two-chunk addresses: keys hold (lo, hi) words
+    if (model_bits(h) > 64) {   // two-chunk addresses: keys hold (lo, hi) words
         if (vtype == 0)
             return step_impl<u128, int64_t>(p, order, keys, (const int64_t*)vals, n, out_keys,
                                             (int64_t*)out_vals, cap, stats);
@@ -954,7 +1129,8 @@ int32_t or_apply_affine(const or_model* hs, int32_t nrhs, const uint64_t* basis,
             const int64_t n_od = or_num_offdiagonals(h, addr);
             for (int64_t c = 1; c <= n_od; ++c) {
                 double hv;
-                const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
+                const uint64_t nb = (h->model == 3) ? rs_offdiagonal(h, addr, c, &hv)
+                                  : (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
                                                     : mom1d_offdiagonal(h, addr, occ, nocc, c, &hv);
                 const int64_t k = std::lower_bound(basis, basis + D, nb) - basis;
                 if (k >= D || basis[k] != nb) return -1;
@@ -982,7 +1158,8 @@ int64_t or_matrix_triplets(const or_model* h, const uint64_t* basis, int64_t D, 
         const int64_t n_od = or_num_offdiagonals(h, addr);
         for (int64_t c = 1; c <= n_od; ++c) {
             double hv;
-            const uint64_t nb = (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
+            const uint64_t nb = (h->model == 3) ? rs_offdiagonal(h, addr, c, &hv)
+                              : (h->model == 1) ? real1d_offdiagonal(h, addr, occ, nocc, c, &hv)
                                                 : mom1d_offdiagonal(h, addr, occ, nocc, c, &hv);
             const int64_t k = std::lower_bound(basis, basis + D, nb) - basis;
             if (k >= D || basis[k] != nb) return -1 - cap;

@@ -3,7 +3,8 @@
 The package mirrors the part of Rimu.jl's API that drives the FCIQMC step:
 ``ProjectorMonteCarloProblem`` / ``init`` / ``solve`` / ``lomc`` (``lomc!``), the
 ``AbstractDVec`` and ``StochasticStyle`` contracts, and the Hamiltonian interface of
-``HubbardReal1D`` and ``HubbardMom1D``.  All compute goes through ``librimu_hip.so``
+``HubbardReal1D``, ``HubbardMom1D`` and ``HubbardRealSpace`` (BoseFS, FermiFS and
+two-component CompositeFS addresses on CubicGrid geometries).  All compute goes through ``librimu_hip.so``
 (``include/rimu_hip.h``); the package has no CPU fallback.
 
 Julia names with ``!`` map to: ``solve!`` -> ``PMCSimulation.solve_``, ``step!`` ->
@@ -11,10 +12,11 @@ Julia names with ``!`` map to: ``solve!`` -> ``PMCSimulation.solve_``, ``step!``
 """
 from . import _native, statstools
 from ._native import RimuError, CapacityError, annihilate, device_count
-from .addresses import BoseFS, near_uniform
-from .hamiltonians import (HubbardMom1D, HubbardReal1D, diagonal_element, dimension,
-                           get_offdiagonal, num_offdiagonals, offdiagonals, starting_address,
-                           hubbard_dispersion)
+from .addresses import BoseFS, CompositeFS, FermiFS, FermiFS2C, near_uniform
+from .geometry import CubicGrid, HardwallBoundaries, LadderBoundaries, PeriodicBoundaries
+from .hamiltonians import (HubbardMom1D, HubbardReal1D, HubbardRealSpace, diagonal_element,
+                           dimension, get_offdiagonal, num_offdiagonals, offdiagonals,
+                           starting_address, hubbard_dispersion)
 from .styles import (IsDeterministic, IsDynamicSemistochastic, IsStochasticInteger,
                      NoCompression, ThresholdCompression, WithReplacement, default_style)
 from .initiators import (CoherentInitiator, Initiator, InitiatorRule, InitiatorValue,

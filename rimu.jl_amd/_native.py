@@ -15,7 +15,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "librimu_hip.so"
 
 RIMU_OK, RIMU_ERR_ARG, RIMU_ERR_HIP, RIMU_ERR_CAPACITY, RIMU_ERR_NODEVICE, RIMU_ERR_INTERNAL = range(6)
-MODEL_REAL1D, MODEL_MOM1D = 1, 2
+MODEL_REAL1D, MODEL_MOM1D, MODEL_REALSPACE = 1, 2, 3
 STYLE_INTEGER, STYLE_DETERMINISTIC, STYLE_DYNAMIC = 1, 2, 3
 INITIATOR_NONE, INITIATOR_DEFAULT, INITIATOR_SIMPLE, INITIATOR_COHERENT = 0, 1, 2, 3
 OP_HAMILTONIAN, OP_G2_REAL, OP_DENSITY_DIAGONAL = 1, 2, 3
@@ -50,7 +50,11 @@ class CapacityError(RimuError):
 class Model(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("M", ctypes.c_int32), ("N", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("u", ctypes.c_double), ("t", ctypes.c_double),
-                ("kes", ctypes.c_double * 64)]
+                ("kes", ctypes.c_double * 64),
+                ("ncomp", ctypes.c_int32), ("N2", ctypes.c_int32), ("fermi", ctypes.c_int32),
+                ("ndir", ctypes.c_int32), ("has_u", ctypes.c_int32), ("has_pot", ctypes.c_int32),
+                ("t2", ctypes.c_double), ("u12", ctypes.c_double), ("u22", ctypes.c_double),
+                ("pot2", ctypes.c_double * 64), ("nbr", (ctypes.c_uint8 * 6) * 64)]
 
 
 class Style(ctypes.Structure):
@@ -177,8 +181,17 @@ def overlaps(left: "Engine", right: "Engine", ops=()) -> np.ndarray:
     return np.array(out[:], dtype=np.float64)
 
 
+def model_key_bits(model: Model) -> int:
+    """Bits of an address of the model (N + M - 1 for BoseFS; for HubbardRealSpace the sum
+    over components of M (FermiFS) or N_c + M - 1 (BoseFS))."""
+    if model.kind != MODEL_REALSPACE:
+        return model.M + model.N - 1
+    width = lambda c, n: model.M if (model.fermi >> c) & 1 else n + model.M - 1
+    return width(0, model.N) + (width(1, model.N2) if model.ncomp == 2 else 0)
+
+
 def model_key_words(model: Model) -> int:
-    return 2 if model.M + model.N - 1 > 64 else 1
+    return 2 if model_key_bits(model) > 64 else 1
 
 
 def hamiltonian_eval(model: Model, addrs, chosen):

@@ -149,10 +149,16 @@ template <class A> __device__ inline A insert_one(A x, int p) {
 // Model data passed to kernels by value.
 // ---------------------------------------------------------------------------------------
 struct ModelDev {
-    int kind;   // 1 HubbardReal1D, 2 HubbardMom1D
-    int M, N, B;
+    int kind;   // 1 HubbardReal1D, 2 HubbardMom1D, 3 HubbardRealSpace
+    int M, N, B;  // B: bits of the whole key
     double u, t;
-    double kes[64];
+    double kes[64];  // HubbardMom1D: dispersion; HubbardRealSpace: trap potential of component 1
+    // HubbardRealSpace
+    int ncomp, N2, fermi, ndir, has_u, has_pot;
+    int cbits2;  // bits of component 2 (component 1 sits above them); 0 for one component
+    double t2, u12, u22;
+    double pot2[64];
+    uint8_t nbr[64][6];
 };
 
 // Sum over occupied modes of n(n-1) (bose_hubbard_interaction, bosefs.jl:415-428), bit
@@ -325,6 +331,198 @@ template <class A>
 __device__ inline A get_offdiagonal(const ModelDev& h, A bits, int64_t chosen, double* value) {
     return h.kind == 1 ? real1d_offdiagonal(h, bits, chosen, value)
                        : mom1d_offdiagonal(h, bits, chosen, value);
+}
+
+// ---------------------------------------------------------------------------------------
+// HubbardRealSpace (HubbardRealSpace.jl) on one or two components.  Component 1 of a
+// two-component key is the high part (key >> cbits2), component 2 the low cbits2 bits.
+// A BoseFS component is a run-length bit string as above; a FermiFS component holds site j
+// at bit j-1 (bitstring.jl:713-733).
+// ---------------------------------------------------------------------------------------
+template <class A> __device__ inline A rs_comp(const ModelDev& h, A key, int c) {
+    if (h.ncomp == 1) return key;
+    return c == 0 ? shrA(key, h.cbits2) : (A)(key & low_maskA<A>(h.cbits2));
+}
+template <class A> __device__ inline A rs_with_comp(const ModelDev& h, A key, int c, A v) {
+    if (h.ncomp == 1) return v;
+    const A low = low_maskA<A>(h.cbits2);
+    return c == 0 ? (A)(shlA(v, h.cbits2) | (key & low)) : (A)((key & ~low) | v);
+}
+__device__ inline bool rs_fermi(const ModelDev& h, int c) { return (h.fermi >> c) & 1; }
+
+// num_occupied_modes of a component: N for a FermiFS (fermifs.jl:129), runs for a BoseFS.
+template <class A> __device__ inline int rs_occupied(const ModelDev& h, A b, int c) {
+    return rs_fermi(h, c) ? popcA(b) : popcA(run_starts(b));
+}
+
+// sum over sites of n_i * pot[i] in ascending site order (external_potential,
+// HubbardRealSpace.jl:98-106)
+template <class A> __device__ inline double rs_potential(const ModelDev& h, A b, int c, const double* pot) {
+    double pe = 0.0;
+    if (rs_fermi(h, c)) {
+        A x = b;
+        while (x) {
+            const int p = ctzA(x);
+            pe += 1 * pot[p];
+            x &= x - 1;
+        }
+    } else {
+        A x = b;
+        int mode = 1;
+        while (x) {
+            const int z = ctzA(x);
+            mode += z;
+            x = shrA(x, z);
+            const int n = ctzA((A)~x);
+            pe += n * pot[mode - 1];
+            x = shrA(x, n);
+        }
+    }
+    return pe;
+}
+
+// dot(occupied_modes(a), occupied_modes(b)) = sum_i n_ai n_bi (fockaddress.jl:692-719,
+// bitstring.jl:833-838), an integer
+template <class A> __device__ inline int64_t rs_overlap(const ModelDev& h, A a, A b) {
+    const bool fa = rs_fermi(h, 0), fb = rs_fermi(h, 1);
+    if (fa && fb) return popcA((A)(a & b));
+    int64_t acc = 0;
+    if (fa) {  // FermiFS a: n_ai = 1 on its set bits
+        A x = a;
+        while (x) {
+            const int p = ctzA(x);
+            acc += occupation(b, p + 1);
+            x &= x - 1;
+        }
+        return acc;
+    }
+    A x = a;
+    int mode = 1;
+    while (x) {
+        const int z = ctzA(x);
+        mode += z;
+        x = shrA(x, z);
+        const int n = ctzA((A)~x);
+        const int m = fb ? (int)((b >> (mode - 1)) & 1) : occupation(b, mode);
+        acc += (int64_t)n * m;
+        x = shrA(x, n);
+    }
+    return acc;
+}
+
+// num_offdiagonals: num_occupied_modes * 2D per component, summed (HubbardRealSpace.jl:284-290,
+// :346-349).
+template <class A> __device__ inline int64_t rs_num_offdiagonals(const ModelDev& h, A key) {
+    int64_t n = rs_occupied(h, rs_comp(h, key, 0), 0);
+    if (h.ncomp == 2) n += rs_occupied(h, rs_comp(h, key, 1), 1);
+    return n * h.ndir;
+}
+
+// diagonal_element (HubbardRealSpace.jl:264-279): local interactions (:18-25 and the
+// recursion _interactions :57-73: u11 self term, u12 cross term, u22 self term, in that
+// order of addition), then the trap potential.  A FermiFS has no self interaction.
+template <class A> __device__ inline double rs_diagonal_element(const ModelDev& h, A key) {
+    const A a = rs_comp(h, key, 0);
+    double intr = 0.0;
+    if (h.has_u) {
+        const double self_a = rs_fermi(h, 0) ? 0.0 : h.u * (double)bose_hubbard_interaction(a) / 2;
+        if (h.ncomp == 1) {
+            intr = self_a;
+        } else {
+            const A b = rs_comp(h, key, 1);
+            const double row = h.u12 * (double)rs_overlap(h, a, b);
+            const double self_b = rs_fermi(h, 1) ? 0.0 : h.u22 * (double)bose_hubbard_interaction(b) / 2;
+            intr = (self_a + row) + self_b;
+        }
+    }
+    double pot = 0.0;
+    if (h.has_pot) {
+        if (h.ncomp == 1) {
+            pot = rs_potential(h, a, 0, h.kes);
+        } else {
+            pot += rs_potential(h, a, 0, h.kes);
+            pot += rs_potential(h, rs_comp(h, key, 1), 1, h.pot2);
+        }
+    }
+    return intr + pot;
+}
+
+// get_offdiagonal (HubbardRealSpace.jl:303-316, :369-391): channels of component 1 first;
+// within a component chosen = (particle - 1) * 2D + direction.  The particle-th occupied
+// site hops to its neighbour in that direction: (key, 0.0) past a hard wall; a BoseFS
+// component gives -t sqrt(n_src (n_dst + 1)) (bose_excitation, bitstring.jl:581-595,
+// fockaddress.jl:559-567); a FermiFS component gives -t * (+-1) with the sign of the ones
+// passed (fermi_excitation, bitstring.jl:773-791), or -t * 0.0 and the unchanged key when
+// the target site is occupied.
+template <class A>
+__device__ inline A rs_offdiagonal(const ModelDev& h, A key, int64_t chosen, double* value) {
+    int c = 0;
+    A b = rs_comp(h, key, 0);
+    int64_t ch = chosen;
+    if (h.ncomp == 2) {
+        const int64_t len0 = (int64_t)rs_occupied(h, b, 0) * h.ndir;
+        if (ch > len0) {
+            c = 1;
+            ch -= len0;
+            b = rs_comp(h, key, 1);
+        }
+    }
+    const int particle = (int)((ch - 1) / h.ndir) + 1;
+    const int dir = (int)((ch - 1) % h.ndir);
+    const double t = c == 0 ? h.t : h.t2;
+    int src_pos, src_mode, n_src = 1;
+    if (rs_fermi(h, c)) {
+        src_pos = select_bit(b, particle - 1);
+        src_mode = src_pos + 1;
+    } else {
+        const A st = run_starts(b);
+        src_pos = select_bit(st, particle - 1);
+        n_src = ones_from(b, src_pos);
+        src_mode = src_pos - popcA((A)(b & low_maskA<A>(src_pos))) + 1;  // zeros below, plus one
+    }
+    const int dst = h.nbr[src_mode - 1][dir];
+    if (dst == 255) {
+        *value = 0.0;
+        return key;
+    }
+    A nb;
+    double v;
+    if (rs_fermi(h, c)) {
+        const int before_src = popcA((A)(b & low_maskA<A>(src_pos)));
+        const A removed = b ^ ((A)1 << src_pos);
+        if ((removed >> dst) & 1) {
+            *value = -t * 0.0;
+            return key;
+        }
+        const int before_dst = popcA((A)(removed & low_maskA<A>(dst)));
+        nb = removed | ((A)1 << dst);
+        v = ((before_src + before_dst) & 1) ? -1.0 : 1.0;
+    } else {
+        const int n_dst = occupation(b, dst + 1);
+        const A removed = delete_bit(b, src_pos);
+        nb = insert_one(removed, mode_start(removed, dst + 1));
+        v = sqrt((double)((int64_t)n_src * (n_dst + 1)));
+    }
+    *value = -t * v;
+    return rs_with_comp(h, key, c, nb);
+}
+
+// Model dispatch with the family fixed at compile time: RS selects HubbardRealSpace
+// (kind 3), otherwise the 1D models (kinds 1 and 2, told apart at run time).  The step
+// kernels are instantiated per family, so the 1D models' register allocation does not
+// carry the real-space code.
+template <bool RS, class A> __device__ inline int64_t model_num_offdiagonals(const ModelDev& h, A bits) {
+    if constexpr (RS) return rs_num_offdiagonals(h, bits);
+    else return num_offdiagonals(h, bits);
+}
+template <bool RS, class A> __device__ inline double model_diagonal_element(const ModelDev& h, A bits) {
+    if constexpr (RS) return rs_diagonal_element(h, bits);
+    else return diagonal_element(h, bits);
+}
+template <bool RS, class A>
+__device__ inline A model_offdiagonal(const ModelDev& h, A bits, int64_t chosen, double* value) {
+    if constexpr (RS) return rs_offdiagonal(h, bits, chosen, value);
+    else return get_offdiagonal(h, bits, chosen, value);
 }
 
 }  // namespace rimu

@@ -83,6 +83,23 @@ __device__ inline int64_t find_split(const K* __restrict__ keys, int64_t L, cons
     return find_in(keys, a, b, key);
 }
 
+// t[key] * diag * v plus the off-diagonal terms of a Hamiltonian (RS: HubbardRealSpace).
+template <bool RS, class K>
+__device__ inline double ham_term(const ModelDev& h, K key, double tval, double v,
+                                  const K* __restrict__ tk, const double* tvd,
+                                  const int64_t* tvi, int64_t Lt, const K* s_split) {
+    double res = tval * model_diagonal_element<RS>(h, key) * v;
+    const int64_t n = model_num_offdiagonals<RS>(h, key);
+    for (int64_t c = 1; c <= n; ++c) {
+        double hv;
+        const K nb = model_offdiagonal<RS>(h, key, c, &hv);
+        if (hv == 0.0) continue;
+        const int64_t q = find_split(tk, Lt, s_split, nb);
+        if (q >= 0) res += (tvd ? tvd[q] : (double)tvi[q]) * hv * v;
+    }
+    return res;
+}
+
 // One entry (key, v) of u against operator op: t[key] * diag * v plus the off-diagonal terms.
 template <class K>
 __device__ __noinline__ double op_term(const OpDev& op, int M, K key, double tval, double v,
@@ -90,16 +107,8 @@ __device__ __noinline__ double op_term(const OpDev& op, int M, K key, double tva
                                        const int64_t* tvi, int64_t Lt, const K* s_split) {
     if (op.kind == OP_G2_REAL) return tval * g2_real(key, M, op.param) * v;
     if (op.kind == OP_DENSITY) return tval * (double)occupation(key, op.param) * v;
-    double res = tval * diagonal_element(op.h, key) * v;
-    const int64_t n = num_offdiagonals(op.h, key);
-    for (int64_t c = 1; c <= n; ++c) {
-        double hv;
-        const K nb = get_offdiagonal(op.h, key, c, &hv);
-        if (hv == 0.0) continue;
-        const int64_t q = find_split(tk, Lt, s_split, nb);
-        if (q >= 0) res += (tvd ? tvd[q] : (double)tvi[q]) * hv * v;
-    }
-    return res;
+    if (op.h.kind == 3) return ham_term<true>(op.h, key, tval, v, tk, tvd, tvi, Lt, s_split);
+    return ham_term<false>(op.h, key, tval, v, tk, tvd, tvi, Lt, s_split);
 }
 
 // ranges[t] = lower bound in t of u's key at t * OV_TILE (t < ntiles), ranges[ntiles] = Lt.

@@ -132,6 +132,12 @@ struct rimu_engine {
 
 namespace {
 
+// Bits of one HubbardRealSpace component: M for a FermiFS, N + M - 1 for a BoseFS.
+int rs_component_bits(const rimu_model& m, int c) {
+    const int n = c == 0 ? m.N : m.N2;
+    return ((m.fermi >> c) & 1) ? m.M : n + m.M - 1;
+}
+
 ModelDev to_dev(const rimu_model& m) {
     ModelDev h{};
     h.kind = m.kind;
@@ -141,11 +147,66 @@ ModelDev to_dev(const rimu_model& m) {
     h.u = m.u;
     h.t = m.t;
     for (int j = 0; j < 64; ++j) h.kes[j] = m.kes[j];
+    if (m.kind == RIMU_MODEL_HUBBARD_REAL_SPACE) {
+        h.ncomp = m.ncomp;
+        h.N2 = m.N2;
+        h.fermi = m.fermi;
+        h.ndir = m.ndir;
+        h.has_u = m.has_u;
+        h.has_pot = m.has_pot;
+        h.cbits2 = m.ncomp == 2 ? rs_component_bits(m, 1) : 0;
+        h.B = rs_component_bits(m, 0) + h.cbits2;
+        h.t2 = m.t2;
+        h.u12 = m.u12;
+        h.u22 = m.u22;
+        for (int j = 0; j < 64; ++j) h.pot2[j] = m.pot2[j];
+        std::memcpy(h.nbr, m.nbr, sizeof(h.nbr));
+    }
     return h;
+}
+
+// Whether two models act on the same addresses (sites, particles per component, component
+// types).
+bool same_space(const ModelDev& a, const ModelDev& b) {
+    const bool ra = a.kind == RIMU_MODEL_HUBBARD_REAL_SPACE, rb = b.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
+    if (a.M != b.M || a.N != b.N || a.B != b.B || ra != rb) return false;
+    return !ra || (a.ncomp == b.ncomp && a.N2 == b.N2 && a.fermi == b.fermi);
+}
+
+// Whether key is an address of the model: inside B bits, with every component's particle
+// number (a BoseFS component of N particles has N ones among its N + M - 1 bits).
+bool address_ok(const ModelDev& h, u128 key) {
+    const u128 maskB = h.B >= 128 ? ~(u128)0 : (((u128)1 << h.B) - 1);
+    if ((key & ~maskB) != 0) return false;
+    auto ones = [](u128 x) { return __builtin_popcountll((uint64_t)x) + __builtin_popcountll((uint64_t)(x >> 64)); };
+    if (h.kind != RIMU_MODEL_HUBBARD_REAL_SPACE || h.ncomp == 1) return ones(key) == h.N;
+    const u128 low = ((u128)1 << h.cbits2) - 1;
+    return ones(key >> h.cbits2) == h.N && ones(key & low) == h.N2;
+}
+
+int check_real_space(const rimu_model* m) {
+    if (m->ncomp != 1 && m->ncomp != 2) return fail(RIMU_ERR_ARG, "HubbardRealSpace needs 1 or 2 components");
+    if (m->M < 2 || m->M > 64) return fail(RIMU_ERR_ARG, "HubbardRealSpace needs 2 <= M <= 64 sites");
+    if (m->ndir != 2 && m->ndir != 4 && m->ndir != 6) return fail(RIMU_ERR_ARG, "HubbardRealSpace geometry needs 1 to 3 dimensions");
+    if ((m->fermi & ~((1 << m->ncomp) - 1)) != 0) return fail(RIMU_ERR_ARG, "fermi flags a missing component");
+    const int n2 = m->ncomp == 2 ? m->N2 : 0;
+    if (m->N < 0 || n2 < 0 || m->N + n2 < 1) return fail(RIMU_ERR_ARG, "HubbardRealSpace needs particles");
+    for (int c = 0; c < m->ncomp; ++c)
+        if (((m->fermi >> c) & 1) && (c == 0 ? m->N : m->N2) > m->M)
+            return fail(RIMU_ERR_ARG, "a FermiFS holds at most M particles");
+    int bits = 0;
+    for (int c = 0; c < m->ncomp; ++c) bits += rs_component_bits(*m, c);
+    if (bits > 128) return fail(RIMU_ERR_ARG, "address must fit two 64-bit chunks (at most 128 bits)");
+    for (int s = 0; s < m->M; ++s)
+        for (int d = 0; d < m->ndir; ++d)
+            if (m->nbr[s][d] != 255 && m->nbr[s][d] >= m->M)
+                return fail(RIMU_ERR_ARG, "neighbour table names a site outside 1..M");
+    return RIMU_OK;
 }
 
 int check_model(const rimu_model* m) {
     if (!m) return fail(RIMU_ERR_ARG, "model is null");
+    if (m->kind == RIMU_MODEL_HUBBARD_REAL_SPACE) return check_real_space(m);
     if (m->kind != RIMU_MODEL_HUBBARD_REAL_1D && m->kind != RIMU_MODEL_HUBBARD_MOM_1D)
         return fail(RIMU_ERR_ARG, "unknown model kind");
     if (m->M < 1 || m->N < 1 || m->M + m->N - 1 > 128)
@@ -300,10 +361,16 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(e->plan_status.alloc(plan_tiles));
     HIP_TRY(hipMemsetAsync(e->plan_status.p, 0, sizeof(uint64_t) * plan_tiles, e->stream));
     const int64_t tf_cap = (int64_t)e->tile_first.n;
+    const bool rs = e->hdev.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
     e->timer.begin("plan", e->stream, &ev);
-    plan_kernel<K, V><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
-                                                     e->tile_first.p, tf_cap, e->plan_status.p,
-                                                     &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
+    if (rs)
+        plan_kernel<K, V, true><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
+                                                               e->tile_first.p, tf_cap, e->plan_status.p,
+                                                               &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
+    else
+        plan_kernel<K, V, false><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
+                                                                e->tile_first.p, tf_cap, e->plan_status.p,
+                                                                &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
@@ -319,8 +386,10 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     u128 kor = 0, kand = ~(u128)0;
     if (W > 0) {
         spawn_tiles = ceil_div(W, SPAWN_TILE);
-        // every attempt yields a record when values are floating and deposits unprojected
-        const bool dense = !e->int_vals && e->style.proj_threshold == 0;
+        // every attempt yields a record when values are floating and deposits unprojected,
+        // unless the model has zero-valued channels (hard walls, Pauli blocking: the
+        // reference never deposits their spawns)
+        const bool dense = !e->int_vals && e->style.proj_threshold == 0 && !rs;
         if (dense && W > e->max_spawns) {
             out->overflow = 1;
             return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
@@ -344,11 +413,15 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         }
         e->timer.begin("spawn", e->stream, &ev);
         if (dense)
-            spawn_kernel<K, V, true><<<spawn_tiles, NT, 0, e->stream>>>(
+            spawn_kernel<K, V, true, false><<<spawn_tiles, NT, 0, e->stream>>>(
+                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
+                (K*)e->tile_or.p, (K*)e->tile_and.p);
+        else if (rs)
+            spawn_kernel<K, V, false, true><<<spawn_tiles, NT, 0, e->stream>>>(
                 a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
                 (K*)e->tile_or.p, (K*)e->tile_and.p);
         else
-            spawn_kernel<K, V, false><<<spawn_tiles, NT, 0, e->stream>>>(
+            spawn_kernel<K, V, false, false><<<spawn_tiles, NT, 0, e->stream>>>(
                 a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
                 (K*)e->tile_or.p, (K*)e->tile_and.p);
         e->timer.end(e->stream);
@@ -548,8 +621,7 @@ int rimu_engine_destroy(rimu_engine* e) {
 int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, int64_t n) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
     if (n < 0 || (n > 0 && (!keys || !vals))) return fail(RIMU_ERR_ARG, "bad vector arguments");
-    const int B = e->hdev.B, kw = e->kw;
-    const u128 maskB = B >= 128 ? ~(u128)0 : (((u128)1 << B) - 1);
+    const int kw = e->kw;
     auto key_at = [&](int64_t i) -> u128 {
         return kw == 1 ? (u128)keys[i] : (((u128)keys[2 * i + 1] << 64) | keys[2 * i]);
     };
@@ -562,9 +634,8 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
         const u128 key = key_at(i);
         if (i > 0 && key <= prev) return fail(RIMU_ERR_ARG, "keys must be strictly ascending");
         prev = key;
-        const int ones = __builtin_popcountll((uint64_t)key) + __builtin_popcountll((uint64_t)(key >> 64));
-        if ((key & ~maskB) != 0 || ones != e->hdev.N)
-            return fail(RIMU_ERR_ARG, "key is not an N-boson M-mode BoseFS address");
+        if (!address_ok(e->hdev, key))
+            return fail(RIMU_ERR_ARG, "key is not an address of the model (bits or particle numbers)");
         if (e->int_vals) {
             const int64_t v = ((const int64_t*)vals)[i];
             if (v == 0) continue;
@@ -649,8 +720,8 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
                   double* out) {
     if (!left || !right || !out || (nops > 0 && !ops)) return fail(RIMU_ERR_ARG, "null argument");
     if (nops < 0 || nops > OV_MAX_OPS) return fail(RIMU_ERR_ARG, "between 0 and 8 operators");
-    if (left->hdev.M != right->hdev.M || left->hdev.N != right->hdev.N)
-        return fail(RIMU_ERR_ARG, "vectors of different address spaces (M, N)");
+    if (!same_space(left->hdev, right->hdev))
+        return fail(RIMU_ERR_ARG, "vectors of different address spaces");
     if (right->kw == 2) return overlaps_impl<u128>(left, right, ops, nops, out);
     return overlaps_impl<uint64_t>(left, right, ops, nops, out);
 }
@@ -670,9 +741,11 @@ int overlaps_impl(rimu_engine* left, rimu_engine* right, const rimu_operator* op
         d.param = ops[o].param;
         if (d.kind == RIMU_OP_HAMILTONIAN) {
             if (int rc = check_model(&ops[o].model)) return rc;
-            if (ops[o].model.M != M || ops[o].model.N != right->hdev.N)
-                return fail(RIMU_ERR_ARG, "operator Hamiltonian on another address space");
             d.h = to_dev(ops[o].model);
+            if (!same_space(d.h, right->hdev))
+                return fail(RIMU_ERR_ARG, "operator Hamiltonian on another address space");
+        } else if (right->hdev.kind == RIMU_MODEL_HUBBARD_REAL_SPACE) {
+            return fail(RIMU_ERR_ARG, "G2RealCorrelator and DensityMatrixDiagonal act on one-component 1D BoseFS vectors");
         } else if (d.kind == RIMU_OP_G2_REAL) {
             if (d.param <= -M || d.param >= M) return fail(RIMU_ERR_ARG, "G2RealCorrelator needs |d| < M");
         } else if (d.kind == RIMU_OP_DENSITY_DIAGONAL) {
@@ -788,13 +861,14 @@ __global__ __launch_bounds__(NT) void ham_eval_kernel(ModelDev h, const K* __res
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i >= n) return;
     const K k = addrs[i];
-    diag[i] = diagonal_element(h, k);
-    const int64_t c = num_offdiagonals(h, k);
+    const bool rs = h.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
+    diag[i] = rs ? model_diagonal_element<true>(h, k) : model_diagonal_element<false>(h, k);
+    const int64_t c = rs ? model_num_offdiagonals<true>(h, k) : model_num_offdiagonals<false>(h, k);
     nod[i] = c;
     const int64_t ch = chosen[i];
     if (ch >= 1 && ch <= c) {
         double v;
-        naddr[i] = get_offdiagonal(h, k, ch, &v);
+        naddr[i] = rs ? model_offdiagonal<true>(h, k, ch, &v) : model_offdiagonal<false>(h, k, ch, &v);
         oval[i] = v;
     } else {
         naddr[i] = k;
@@ -875,12 +949,10 @@ int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
     const ModelDev h = to_dev(*model);
     const int kw = h.B > 64 ? 2 : 1;
-    const u128 maskB = h.B >= 128 ? ~(u128)0 : (((u128)1 << h.B) - 1);
     for (int64_t i = 0; i < n; ++i) {
         const u128 key = kw == 1 ? (u128)addrs[i] : (((u128)addrs[2 * i + 1] << 64) | addrs[2 * i]);
-        const int ones = __builtin_popcountll((uint64_t)key) + __builtin_popcountll((uint64_t)(key >> 64));
-        if ((key & ~maskB) != 0 || ones != h.N)
-            return fail(RIMU_ERR_ARG, "address is not an N-boson M-mode BoseFS address");
+        if (!address_ok(h, key))
+            return fail(RIMU_ERR_ARG, "address is not an address of the model (bits or particle numbers)");
     }
     DevBuf<uint64_t> a, na;
     DevBuf<int64_t> c, nod;

@@ -90,10 +90,10 @@ struct ColumnPlan {
     bool exact;
 };
 
-template <class K, class V>
+template <class K, class V, bool RS>
 __device__ inline ColumnPlan plan_column(const StepArgs& a, K k, V v) {
     ColumnPlan c;
-    c.n_od = num_offdiagonals(a.h, k);
+    c.n_od = model_num_offdiagonals<RS>(a.h, k);
     const double av = fabs((double)v);
     if (a.style == 2) {
         c.exact = true;
@@ -121,7 +121,7 @@ __device__ inline ColumnPlan plan_column(const StepArgs& a, K k, V v) {
 constexpr int PLAN_IPT = 8;
 constexpr int PLAN_TILE = NT * PLAN_IPT;
 
-template <class K, class V>
+template <class K, class V, bool RS>
 __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restrict__ keys,
                                                   const V* __restrict__ vals, int64_t L,
                                                   V* __restrict__ dvals, int64_t* __restrict__ aoff,
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
             const K k = kk[j];
             const V v = vv[j];
             // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
-            const double hd = diagonal_element(a.h, k);
+            const double hd = model_diagonal_element<RS>(a.h, k);
             const double d = 1 - a.dtau * (hd - a.shift);
             const double x = d * (double)v;
             double U = 0;
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
             if (res > val) st[1] += fabs(res - val);
             else if (sgn(res) != sgn(val)) { st[0] += fabs(val); st[2] += fabs(res); }
             else st[0] += fabs(res - val);
-            const ColumnPlan c = plan_column<K, V>(a, k, v);
+            const ColumnPlan c = plan_column<K, V, RS>(a, k, v);
             if (c.n_od > 0) {
                 if (a.style != 1) { if (c.exact) st[3] += 1; else st[4] += 1; }
                 st[5] += (double)c.n_items;
@@ -294,7 +294,7 @@ constexpr int SPAWN_TILE = NT * SPAWN_ITEMS;
 static_assert(SPAWN_TILE % NT == 0, "whole slots per thread in the entry scan");
 static_assert(SPAWN_TILE == SPAWN_TILE_SIZE, "plan and spawn agree on the spawn tile");
 
-template <class K, class V, bool DENSE>
+template <class K, class V, bool DENSE, bool RS>
 __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restrict__ keys,
                                                    const V* __restrict__ vals, int64_t L,
                                                    const int64_t* __restrict__ aoff, int64_t A,
@@ -428,13 +428,13 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restri
             const int64_t j = att - aidx[it];
             const K k = ekey[it];
             const V v = eval[it];
-            const ColumnPlan c = plan_column<K, V>(a, k, v);
+            const ColumnPlan c = plan_column<K, V, RS>(a, k, v);
             const bool need_u = is_int_v<V>::value || a.proj_threshold > 0;
             U4 r = {0, 0, 0, 0};
             if (!c.exact || need_u) r = draw(a.seed, k, (uint32_t)j, a.step, TAG_SPAWN);
             const int64_t chosen = c.exact ? j + 1 : pick1(r.x, c.n_od);
             double hv;
-            nb = get_offdiagonal(a.h, k, chosen, &hv);
+            nb = model_offdiagonal<RS>(a.h, k, chosen, &hv);
             const double mel = -hv * a.dtau;  // FirstOrderOffdiagonals, fciqmc.jl:109-112
             double x;
             if (c.exact) {

@@ -12,7 +12,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from .addresses import BoseFS, key_words, pack_keys, unpack_keys
+from .addresses import BoseFS, CompositeFS, FermiFS, key_words, pack_keys, unpack_keys
+
+ADDRESS_TYPES = (BoseFS, FermiFS, CompositeFS)
 from .initiators import Initiator, InitiatorRule, NonInitiator, initiator_rule
 from .styles import IsStochasticInteger, StochasticStyle, default_style
 
@@ -82,7 +84,7 @@ class DVec:
     def to_arrays(self):
         """Sorted word array of the keys ((n,) or (n, 2)) and the values."""
         ks = sorted(self.storage)
-        kw = key_words(ks[0].M, ks[0].N) if ks else 1
+        kw = ks[0].key_words if ks else 1
         return pack_keys([k.bits for k in ks], kw), np.array([self.storage[k] for k in ks])
 
     def __repr__(self):
@@ -118,7 +120,7 @@ class FrozenDVec:
         self.pairs = list(pairs)
 
     def keys_bits(self):
-        kw = key_words(self.pairs[0][0].M, self.pairs[0][0].N) if self.pairs else 1
+        kw = self.pairs[0][0].key_words if self.pairs else 1
         return pack_keys([k.bits for k, _ in self.pairs], kw)
 
     def coefs(self):
@@ -128,7 +130,7 @@ class FrozenDVec:
 def freeze(v) -> FrozenDVec:
     if isinstance(v, FrozenDVec):
         return v
-    if isinstance(v, BoseFS):
+    if isinstance(v, ADDRESS_TYPES):
         return FrozenDVec([(v, 1.0)])
     return FrozenDVec(list(v.pairs()))
 
@@ -162,10 +164,10 @@ class DeviceDVec:
 
     def to_dvec(self) -> DVec:
         keys, vals = self.engine.download()
-        M, N = self.hamiltonian.M, self.hamiltonian.N
+        template = self.hamiltonian.address
         out = DVec(style=self.style)
         for k, v in zip(unpack_keys(keys), vals.tolist()):
-            out.storage[BoseFS(M=M, N=N, bits=k)] = v
+            out.storage[template.with_bits(k)] = v
         return out
 
     def pairs(self):

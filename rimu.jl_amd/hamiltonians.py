@@ -1,9 +1,11 @@
-"""``HubbardReal1D`` and ``HubbardMom1D`` host descriptors with the Hamiltonian interface
-(``diagonal_element``, ``num_offdiagonals``, ``get_offdiagonal``, ``offdiagonals``,
-``starting_address``, ``dimension``) evaluated on the device by ``rimu_hamiltonian_eval``.
+"""``HubbardReal1D``, ``HubbardMom1D`` and ``HubbardRealSpace`` host descriptors with the
+Hamiltonian interface (``diagonal_element``, ``num_offdiagonals``, ``get_offdiagonal``,
+``offdiagonals``, ``starting_address``, ``dimension``) evaluated on the device by
+``rimu_hamiltonian_eval``.
 
 References: ``src/Hamiltonians/HubbardReal1D.jl:22-62``, ``src/Hamiltonians/HubbardMom1D.jl:1-65``
-(momentum grid and dispersion), ``src/Interfaces/hamiltonians.jl`` (interface).
+(momentum grid and dispersion), ``src/Hamiltonians/HubbardRealSpace.jl:130-391``,
+``src/Interfaces/hamiltonians.jl`` (interface).
 """
 from __future__ import annotations
 
@@ -13,7 +15,8 @@ from fractions import Fraction
 import numpy as np
 
 from . import _native
-from .addresses import BoseFS, key_words, pack_keys, unpack_keys
+from .addresses import BoseFS, CompositeFS, FermiFS, key_words, pack_keys, unpack_keys
+from .geometry import CubicGrid, PeriodicBoundaries
 
 
 class AbstractHamiltonian:
@@ -39,13 +42,13 @@ class AbstractHamiltonian:
     def descriptor(self) -> _native.Model:
         raise NotImplementedError
 
-    def _check(self, addr: BoseFS):
-        if addr.M != self.M or addr.N != self.N:
-            raise ValueError("address does not match the Hamiltonian's N and M")
+    def _check(self, addr):
+        if getattr(addr, "signature", None) != self.address.signature:
+            raise ValueError("address does not match the Hamiltonian's address type, N and M")
 
     @property
     def key_words(self) -> int:
-        return key_words(self.M, self.N)
+        return self.address.key_words
 
     def _keys(self, bits_list):
         return pack_keys(bits_list, self.key_words)
@@ -66,7 +69,7 @@ class AbstractHamiltonian:
         _, n, na, ov = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits]), [chosen])
         if not 1 <= chosen <= n[0]:
             raise IndexError(f"chosen={chosen} outside 1:{n[0]}")
-        return BoseFS(M=self.M, N=self.N, bits=unpack_keys(na)[0]), float(ov[0])
+        return self.address.with_bits(unpack_keys(na)[0]), float(ov[0])
 
     def offdiagonals(self, addr: BoseFS):
         self._check(addr)
@@ -75,7 +78,7 @@ class AbstractHamiltonian:
             return []
         _, _, na, ov = _native.hamiltonian_eval(self.descriptor(), self._keys([addr.bits] * n),
                                                 list(range(1, n + 1)))
-        return [(BoseFS(M=self.M, N=self.N, bits=a), float(v)) for a, v in zip(unpack_keys(na), ov)]
+        return [(self.address.with_bits(a), float(v)) for a, v in zip(unpack_keys(na), ov)]
 
     def column(self, addrs_bits):
         """Diagonal elements and all off-diagonal (row, value) pairs of the columns of a batch
@@ -146,6 +149,97 @@ class HubbardMom1D(AbstractHamiltonian):
 
     def __repr__(self):
         return f"HubbardMom1D({self.address!r}; u={self.u}, t={self.t})"
+
+
+def _component_dimension(c) -> int:
+    if isinstance(c, FermiFS):
+        return math.comb(c.M, c.N)
+    return math.comb(c.N + c.M - 1, c.N)
+
+
+class HubbardRealSpace(AbstractHamiltonian):
+    """``HubbardRealSpace(address; geometry, t, u, v)`` (HubbardRealSpace.jl:130-230) on a
+    BoseFS, a FermiFS or a two-component CompositeFS.  t: one hopping per component; u: the
+    symmetric C x C interaction matrix (a FermiFS component has no self-interaction); v:
+    C x D trap strengths.  The default geometry is PeriodicBoundaries(M)."""
+
+    def __init__(self, address, geometry: CubicGrid | None = None, t=None, u=None, v=None):
+        comps = address.components if isinstance(address, CompositeFS) else (address,)
+        C = len(comps)
+        geometry = geometry if geometry is not None else PeriodicBoundaries(address.M)
+        D = geometry.num_dimensions()
+        t = [1.0] * C if t is None else [float(x) for x in np.ravel(t)]
+        u = np.ones((C, C)) if u is None else np.array(u, dtype=float)
+        v = np.zeros((C, D)) if v is None else np.array(v, dtype=float)
+        if len(geometry) != address.M:
+            raise ValueError("`geometry` does not have the correct number of sites")
+        if u.size != C * C:
+            raise ValueError(f"`u` must be a {C} × {C} matrix")
+        u = u.reshape(C, C)
+        if u.size != 1 and not np.array_equal(u, u.T):
+            raise ValueError("`u` must be symmetric")
+        if len(t) != C:
+            raise ValueError(f"`t` must be a vector of length {C}")
+        if v.size != C * D:
+            raise ValueError(f"`v` must be a {C} × {D} matrix")
+        if address.M > 64 or D > 3:
+            raise ValueError("the device takes at most 64 sites in at most 3 dimensions")
+        for c, comp in enumerate(comps):
+            if isinstance(comp, FermiFS) and not np.all(u == 1) and u[c, c] != 0:
+                import warnings
+                warnings.warn(f"component {c + 1} is fermionic, but was given a self-interaction "
+                              f"strength of {u[c, c]}")
+        self.address = address
+        self.geometry = geometry
+        self.t = t
+        self.u = u
+        self.v = v.reshape(C, D)
+
+    def dimension(self) -> int:
+        """``number_conserving_dimension`` (Hamiltonians/abstract.jl:100-118)."""
+        comps = self.address.components if isinstance(self.address, CompositeFS) else (self.address,)
+        return math.prod(_component_dimension(c) for c in comps)
+
+    def potential(self, c: int):
+        """Trap potential of component c per site: sum_d v[c, d] x_d^2 with x_d in
+        range(-fld(n_d, 2); length = n_d) (HubbardRealSpace.jl:214-224)."""
+        out = []
+        for s in range(1, self.M + 1):
+            x = [xi - 1 - n // 2 for xi, n in zip(self.geometry.coords(s), self.geometry.dims)]
+            acc = None
+            for d, xd in enumerate(x):
+                term = float(self.v[c, d]) * (xd * xd)
+                acc = term if acc is None else acc + term
+            out.append(acc)
+        return out
+
+    def descriptor(self) -> _native.Model:
+        comps = self.address.components if isinstance(self.address, CompositeFS) else (self.address,)
+        m = _native.Model()
+        m.kind, m.M, m.N = _native.MODEL_REALSPACE, self.M, comps[0].N
+        m.ncomp = len(comps)
+        m.N2 = comps[1].N if len(comps) == 2 else 0
+        m.fermi = sum(1 << c for c, x in enumerate(comps) if isinstance(x, FermiFS))
+        m.ndir = 2 * self.geometry.num_dimensions()
+        m.t, m.u = self.t[0], float(self.u[0, 0])
+        if len(comps) == 2:
+            m.t2, m.u12, m.u22 = self.t[1], float(self.u[0, 1]), float(self.u[1, 1])
+        m.has_u = int(bool(np.any(self.u != 0)))
+        m.has_pot = int(bool(np.any(self.v != 0)))
+        if m.has_pot:
+            for j, x in enumerate(self.potential(0)):
+                m.kes[j] = x
+            if len(comps) == 2:
+                for j, x in enumerate(self.potential(1)):
+                    m.pot2[j] = x
+        for s, row in enumerate(self.geometry.neighbour_table()):
+            for d, x in enumerate(row):
+                m.nbr[s][d] = x
+        return m
+
+    def __repr__(self):
+        return (f"HubbardRealSpace({self.address!r}; geometry={self.geometry}, t={self.t}, "
+                f"u={self.u.tolist()}" + (f", v={self.v.tolist()}" if np.any(self.v != 0) else "") + ")")
 
 
 def diagonal_element(h: AbstractHamiltonian, addr: BoseFS) -> float:

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _native
 from .addresses import BoseFS, key_words, pack_keys, unpack_keys
-from .dictvectors import DVec, DeviceDVec, FrozenDVec, PDVec
+from .dictvectors import ADDRESS_TYPES, DVec, DeviceDVec, FrozenDVec, PDVec
 from .hamiltonians import AbstractHamiltonian
 from .initiators import Initiator, InitiatorRule, NonInitiator
 from .replicas import NoStats, ReplicaStrategy
@@ -99,20 +99,20 @@ def _start_vector(problem: ProjectorMonteCarloProblem) -> DVec:
     vector gets the problem's style and initiator rule; a ``DVec`` is copied with its own
     style and rule."""
     s = problem.start_at
-    if isinstance(s, BoseFS):
+    if isinstance(s, ADDRESS_TYPES):
         return default_starting_vector(s, problem.style, initiator=problem.initiator)
     if isinstance(s, FrozenDVec):
         return PDVec(list(s.pairs), style=problem.style, initiator=problem.initiator)
     if isinstance(s, DVec):
         return s.copy()
-    raise TypeError("start_at must be a BoseFS, a FrozenDVec or a DVec")
+    raise TypeError("start_at must be an address, a FrozenDVec or a DVec")
 
 
 def rayleigh_quotient_dvec(h: AbstractHamiltonian, v: DVec) -> float:
     """dot(v, H, v) / dot(v, v) (fciqmc.jl:51-61, abstractdvec.jl:313-324)."""
     items = list(v.pairs())
     bits = [k.bits for k, _ in items]
-    keys = pack_keys(bits, key_words(h.M, h.N))
+    keys = pack_keys(bits, h.key_words)
     vals = np.array([float(x) for _, x in items])
     diag, rows, cols, offv = h.column(keys)
     index = {k: i for i, k in enumerate(bits)}

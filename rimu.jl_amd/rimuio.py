@@ -102,10 +102,14 @@ def _vector_arrays(vector):
     if hasattr(vector, "state_vector"):
         vector = vector.state_vector
     if hasattr(vector, "engine"):
+        if not isinstance(vector.hamiltonian.address, BoseFS):
+            raise NotImplementedError("state files hold BoseFS addresses only")
         keys, vals = vector.engine.download()
         return vector.hamiltonian.M, vector.hamiltonian.N, keys, vals, vector.style
     if isinstance(vector, DVec):
         items = sorted(vector.storage.items(), key=lambda kv: kv[0].bits)
+        if items and not isinstance(items[0][0], BoseFS):
+            raise NotImplementedError("state files hold BoseFS addresses only")
         if items:
             M, N = items[0][0].M, items[0][0].N
         else:

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .dictvectors import DVec, FrozenDVec, freeze
+from .dictvectors import ADDRESS_TYPES, DVec, FrozenDVec, freeze
 from .addresses import BoseFS, unpack_keys
 
 
@@ -109,7 +109,7 @@ class ProjectedEnergy:
         self.vproj_name = vproj
         self.hproj_name = hproj
         self.hamiltonian = hamiltonian
-        if isinstance(projector, BoseFS):
+        if isinstance(projector, ADDRESS_TYPES):
             projector = DVec((projector, 1.0))
         self.vproj = freeze(projector)
         self.hproj = self._compute_hproj(hamiltonian, self.vproj)
@@ -124,8 +124,7 @@ class ProjectedEnergy:
             acc[k] = acc.get(k, 0.0) + d * c
         for r, j, v in zip(unpack_keys(rows), cols.tolist(), vals.tolist()):
             acc[r] = acc.get(r, 0.0) + v * coefs[j]
-        M, N = h.M, h.N
-        return FrozenDVec([(BoseFS(M=M, N=N, bits=k), v) for k, v in acc.items() if v != 0])
+        return FrozenDVec([(h.address.with_bits(k), v) for k, v in acc.items() if v != 0])
 
     def names(self):
         return (self.vproj_name, self.hproj_name)
