@@ -39,6 +39,11 @@ Two more objects on the same line:
   dtau = 1e-4, 1000 steps through lomc, timed whole; steady-state step times of the engine
   and of the oracle (one core) from its final vector.  (b) The same model grown to 1e8
   walkers with the headline's settings: spawn attempts/s and per-kernel times.
+* "realspace": HubbardRealSpace.  (a) 2D Bose-Hubbard, 64 bosons on PeriodicBoundaries(8, 8),
+  u = 6 (127-bit addresses), grown to 1e8 walkers: spawn attempts/s, per-kernel times and
+  the oracle's step on a sample.  (b) The reference's "2D Hubbard" system
+  (benchmark/benchmarks.jl:8-14, FermiFS2C on PeriodicBoundaries(4, 4)) projected to its
+  ground state with IsDeterministic (2000 steps), energy and time.
 """
 from __future__ import annotations
 
@@ -555,6 +560,106 @@ def run_wide_headline(rimu, args):
             "spawn_records_per_step": S, "kernels": kern}
 
 
+def run_realspace_leg(rimu, args):
+    """HubbardRealSpace: (a) 2D Bose-Hubbard on PeriodicBoundaries(8, 8) with 64 bosons,
+    u = 6 (127-bit addresses) grown to args.rs_target walkers, spawn attempts/s and
+    per-kernel times, the oracle's step on a bounded sample beside it; (b) the reference's
+    "2D Hubbard" system (benchmark/benchmarks.jl:8-14: FermiFS2C(16, 1 => 1, 2 => 1,
+    1 => -1, 2 => -1) on PeriodicBoundaries(4, 4)) projected to its ground state with
+    IsDeterministic."""
+    out = {}
+    # (a)
+    H = rimu.HubbardRealSpace(rimu.near_uniform(64, 64), geometry=rimu.PeriodicBoundaries(8, 8), u=[[6.0]])
+    style = rimu.IsDynamicSemistochastic()
+    target = args.rs_target
+    eng = rimu._native.Engine(H.descriptor(), style.descriptor(), 17, int(round(2 * target + 100)) + 1,
+                              int(min(4 * target + 4096, 2 ** 32 - 2)))
+    start = H.starting_address()
+    eng.upload(rimu.addresses.pack_keys([start.bits], H.key_words), np.array([10.0]))
+    dtau = 1e-3
+    strat = rimu.DoubleLogUpdate(target_walkers=target)
+    sp = rimu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, dtau)
+    step = 0
+    t_eq = time.perf_counter()
+    last_log = t_eq
+    while step < args.rs_equilibration:
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        if time.perf_counter() - last_log > 10:
+            log(f"realspace equilibration step {step}: len {st.len:.3g} norm {st.norm:.3g} shift {sp.shift:.4f}")
+            last_log = time.perf_counter()
+    t_eq = time.perf_counter() - t_eq
+    eng.set_timing(True)
+    K = args.rs_steps
+    attempts = S_tot = L_tot = Lnew_tot = 0.0
+    L_prev = len(eng)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step += 1
+        st = eng.step(step, sp.shift, dtau)
+        strat.update(sp, st.norm)
+        attempts += st.spawn_attempts
+        b, records = eng.last_traffic()
+        S_tot += records - L_prev
+        L_tot += L_prev
+        Lnew_tot += st.len
+        L_prev = st.len
+    elapsed = time.perf_counter() - t0
+    times = eng.kernel_times()
+    keys, vals = eng.download()
+    eng.close()
+    S, L, Ln = S_tot / K, L_tot / K, Lnew_tot / K
+    kern = {}
+    for name, (ms, c) in times.items():
+        kern[name] = {"ms_avg": ms / max(c, 1), "launches_per_step": c / K}
+        bpl = kernel_alg_bytes(name, S, L, Ln, ksize=16)
+        if bpl is not None:
+            kern[name]["alg_GBps"] = bpl / (kern[name]["ms_avg"] / 1e3) / 1e9
+    gpu_step = elapsed / K
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    om = O.real_space((8, 8), [("bose", 64)], u=[[6.0]])
+    stride = max(1, len(keys) // 200_000)
+    k, v = np.ascontiguousarray(keys[::stride]), np.ascontiguousarray(vals[::stride])
+    n_cpu, att_cpu = 0, 0.0
+    t2 = time.perf_counter()
+    while n_cpu < 20 and time.perf_counter() - t2 < args.rs_cpu_seconds:
+        k, v, ost = O.step(om, O.style("dynamic"), k, v, seed=17, step=step + 1 + n_cpu,
+                           shift=sp.shift, dtau=dtau, order=0)
+        att_cpu += ost["attempts"]
+        n_cpu += 1
+    cpu_s = time.perf_counter() - t2
+    out["bose_8x8"] = {
+        "workload": f"HubbardRealSpace BoseFS 64 bosons on PeriodicBoundaries(8, 8), u=6, t=1 "
+                    f"(127-bit addresses), IsDynamicSemistochastic FP64, DoubleLogUpdate target {target:.3g}, "
+                    f"dtau 1e-3, seed 17",
+        "equilibration_steps": args.rs_equilibration, "equilibration_s": t_eq,
+        "value": attempts / elapsed, "unit": "spawn_attempts/s", "ms_per_step": 1e3 * gpu_step,
+        "steps": K, "len": st.len, "walkers": st.norm, "shift": sp.shift,
+        "spawn_records_per_step": S, "kernels": kern,
+        "cpu_baseline": {"value": att_cpu / cpu_s, "unit": "spawn_attempts/s", "cores": 1, "kind": "port",
+                         "sample": f"{n_cpu} oracle steps from every {stride}-th entry of the final vector "
+                                   f"({len(keys[::stride])} entries), {cpu_s:.1f} s"}}
+    # (b)
+    a = rimu.FermiFS2C(16, (1, 1), (2, 1), (1, -1), (2, -1))
+    H2 = rimu.HubbardRealSpace(a, geometry=rimu.PeriodicBoundaries(4, 4))
+    dv = rimu.DVec((a, 1.0), style=rimu.IsDeterministic())
+    steps = args.rs_hubbard_steps
+    t3 = time.perf_counter()
+    df, sim = rimu.lomc(H2, dv, laststep=steps, dtau=0.02, s_strat=rimu.DoubleLogUpdate(target_walkers=1e6))
+    v = sim.state_vector
+    e = rimu.dot(v, H2, v) / rimu.dot(v, v)
+    run_s = time.perf_counter() - t3
+    out["hubbard_2d_4x4"] = {
+        "workload": "HubbardRealSpace(FermiFS2C(16, 1 => 1, 2 => 1, 1 => -1, 2 => -1); "
+                    "geometry = PeriodicBoundaries(4, 4)), u = t = 1, IsDeterministic through lomc, dtau 0.02",
+        "steps": steps, "seconds": run_s, "steps_per_s": steps / run_s, "len": int(df["len"].to_numpy()[-1]),
+        "energy_rayleigh": e, "shift_last": float(df["shift"].to_numpy()[-1])}
+    sim.close()
+    return out
+
+
 def run_adjoint_leg(rimu, args):
     """The differentiable projector: HubbardReal1D M = N = 12 (dimension 1,352,078),
     u in {2,4,6,8} as four right-hand sides of one product, K = 2000 steps of dtau 1e-3,
@@ -646,6 +751,12 @@ def main():
     ap.add_argument("--wide-target", type=float, default=1e8)
     ap.add_argument("--wide-equilibration", type=int, default=800)
     ap.add_argument("--wide-steps", type=int, default=10)
+    ap.add_argument("--no-realspace", action="store_true")
+    ap.add_argument("--rs-target", type=float, default=1e8)
+    ap.add_argument("--rs-equilibration", type=int, default=700)
+    ap.add_argument("--rs-steps", type=int, default=10)
+    ap.add_argument("--rs-cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--rs-hubbard-steps", type=int, default=2000)
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -811,6 +922,12 @@ def main():
             wide["headline"] = run_wide_headline(rimu, args)
             log(f"wide headline: {wide['headline']['value']:.3g} attempts/s, "
                 f"{wide['headline']['ms_per_step']:.2f} ms/step")
+    realspace = None
+    if not args.no_realspace:
+        realspace = run_realspace_leg(rimu, args)
+        log(f"realspace: bose 8x8 {realspace['bose_8x8']['value']:.3g} attempts/s, "
+            f"{realspace['bose_8x8']['ms_per_step']:.2f} ms/step; 2D Hubbard 4x4 E = "
+            f"{realspace['hubbard_2d_4x4']['energy_rayleigh']:.10f} in {realspace['hubbard_2d_4x4']['seconds']:.2f} s")
     adjoint = None if args.no_adjoint else run_adjoint_leg(rimu, args)
     if adjoint:
         log(f"adjoint: {adjoint['value']:.3g} grads/s vs plain {adjoint['plain_value']:.3g} "
@@ -846,6 +963,7 @@ def main():
         "checkpoint": checkpoint,
         "overlaps": overlaps,
         "wide": wide,
+        "realspace": realspace,
     }
     print(json.dumps(out), flush=True)
 
