@@ -775,6 +775,38 @@ void compress(const Params& p, Map& m) {
     }
 }
 
+// The GPU engine's list order.  Above 40 address bits the engine stores each address x as
+// y = mix(x), a bijection of [0, 2^B) (multiply by an odd constant mod 2^B, then fold the
+// high half down with y ^= y >> ceil(B/2), twice), keeps its vector in ascending y and emits
+// deposits entry by entry in that order.  order = 1 replays this list; at or below 40 bits
+// y = x (ascending addresses).
+template <class A> A engine_order_key(int B, A x) {
+    if (B <= 40) return x;
+    const u128 c1 = ((u128)0x9E3779B97F4A7C15ull << 64) | 0xF39CC0605CEDC835ull;
+    const u128 c2 = ((u128)0xD1B54A32D192ED03ull << 64) | 0xABC98388FB8FAC03ull;
+    const int s = (B + 1) / 2;
+    const A mask = B >= WIDTH<A> ? (A)~(A)0 : (A)(((A)1 << B) - 1);
+    A y = x;
+    y = (A)(y * (A)c1) & mask;
+    y ^= y >> s;
+    y = (A)(y * (A)c2) & mask;
+    y ^= y >> s;
+    return y;
+}
+
+// Entry indices in the engine's list order.
+template <class A> std::vector<int64_t> engine_entry_order(const Params& p, const uint64_t* keys, int64_t n) {
+    std::vector<int64_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    const int B = model_bits(p.h);
+    if (B > 40) {
+        std::vector<A> y((size_t)n);
+        for (int64_t i = 0; i < n; ++i) y[i] = engine_order_key<A>(B, load_key<A>(keys, i));
+        std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return y[a] < y[b]; });
+    }
+    return idx;
+}
+
 template <class A, class V>
 int64_t step_impl(const Params& p, int order, const uint64_t* keys, const V* vals, int64_t n,
                   uint64_t* out_keys, V* out_vals, int64_t cap, double* stats) {
@@ -786,9 +818,10 @@ int64_t step_impl(const Params& p, int order, const uint64_t* keys, const V* val
             apply_column<V>(p, load_key<A>(keys, i), vals[i], st, [&](A k, V v) { w.deposit(k, v); });
     } else {
         // Same deposits, reordered into a list: every diagonal deposit first (entry order),
-        // then every spawn (entry, attempt order).  This is the list the GPU engine merges.
+        // then every spawn (entry, attempt order), entries in the engine's list order.  This
+        // is the list the GPU engine merges.
         std::vector<std::pair<A, V>> diag, spawns;
-        for (int64_t i = 0; i < n; ++i) {
+        for (int64_t i : engine_entry_order<A>(p, keys, n)) {
             bool first = true;
             const A self = load_key<A>(keys, i);
             apply_column<V>(p, self, vals[i], st, [&](A k, V v) {
@@ -886,7 +919,7 @@ int64_t step_initiator_impl(const Params& p, int order, int rule, double thr, co
         // list order: every diagonal deposit first (entry order), then every spawn
         struct Dep { A k; V v; A pk; V pv; };
         std::vector<Dep> diag, spawns;
-        for (int64_t i = 0; i < n; ++i) {
+        for (int64_t i : engine_entry_order<A>(p, keys, n)) {
             const A self = load_key<A>(keys, i);
             const V pv = vals[i];
             apply_column<V>(p, self, pv, st, [&](A k, V v) {

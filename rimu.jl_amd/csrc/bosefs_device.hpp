@@ -146,6 +146,63 @@ template <class A> __device__ inline A insert_one(A x, int p) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Storage order of keys.  Above MIX_MIN_BITS address bits the engine stores every address
+// x as y = mix(x), a bijection of [0, 2^B) (multiply by an odd constant mod 2^B, fold the
+// high half down, twice).  The top bits of y then depend on every bit of x, so sorting the
+// spawn records on the top MIX_SORT_BITS bits of y leaves only rare groups to order on the
+// rest.  The vector is kept in y order; the step's results per address are unchanged (every
+// draw and every matrix element is a function of x).  At or below MIX_MIN_BITS, y = x.
+// ---------------------------------------------------------------------------------------
+constexpr int MIX_MIN_BITS = 40;
+constexpr int MIX_SORT_BITS = 40;
+
+struct KeyMix {
+    int on, B, s1, s2;
+    u128 c1, c2, c1i, c2i;  // odd multipliers mod 2^B and their inverses
+};
+
+template <class T> __host__ __device__ inline T mix_mask(int B) {
+    return B >= 8 * (int)sizeof(T) ? (T)~(T)0 : (T)(((T)1 << B) - 1);
+}
+// T: uint64_t when B <= 64, u128 otherwise
+template <class T> __host__ __device__ inline T mix_bits(const KeyMix& m, T y) {
+    const T mask = mix_mask<T>(m.B);
+    y = (T)(y * (T)m.c1) & mask;
+    y ^= y >> m.s1;
+    y = (T)(y * (T)m.c2) & mask;
+    y ^= y >> m.s2;
+    return y;
+}
+template <class T> __host__ __device__ inline T unmix_bits(const KeyMix& m, T y) {
+    const T mask = mix_mask<T>(m.B);
+    y ^= y >> m.s2;  // s >= B/2: the fold is its own inverse
+    y = (T)(y * (T)m.c2i) & mask;
+    y ^= y >> m.s1;
+    y = (T)(y * (T)m.c1i) & mask;
+    return y;
+}
+template <class K> __host__ __device__ inline K mix_key(const KeyMix& m, K x) { return m.on ? mix_bits<K>(m, x) : x; }
+template <class K> __host__ __device__ inline K unmix_key(const KeyMix& m, K y) { return m.on ? unmix_bits<K>(m, y) : y; }
+
+inline u128 inverse_mod_2_128(u128 c) {  // Newton: each round doubles the correct low bits
+    u128 x = c;
+    for (int i = 0; i < 7; ++i) x *= (u128)2 - c * x;
+    return x;
+}
+inline KeyMix make_key_mix(int B) {
+    KeyMix m{};
+    m.B = B;
+    m.on = B > MIX_MIN_BITS;
+    m.s1 = (B + 1) / 2;
+    m.s2 = (B + 1) / 2;
+    m.c1 = ((u128)0x9E3779B97F4A7C15ull << 64) | 0xF39CC0605CEDC835ull;
+    m.c2 = ((u128)0xD1B54A32D192ED03ull << 64) | 0xABC98388FB8FAC03ull;
+    m.c1i = inverse_mod_2_128(m.c1);
+    m.c2i = inverse_mod_2_128(m.c2);
+    return m;
+}
+
+// ---------------------------------------------------------------------------------------
 // Model data passed to kernels by value.
 // ---------------------------------------------------------------------------------------
 struct ModelDev {

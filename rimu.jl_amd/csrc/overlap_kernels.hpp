@@ -29,6 +29,7 @@ struct OpDev {
     int kind;
     int param;  // G2RealCorrelator: d; DensityMatrixDiagonal: mode (1-based)
     ModelDev h; // OP_HAMILTONIAN
+    KeyMix mix;  // stored keys are mix(address) (the same for both vectors)
 };
 
 static_assert(sizeof(OpDev) % 8 == 0, "operators are staged in LDS as 8-byte words");
@@ -85,14 +86,14 @@ __device__ inline int64_t find_split(const K* __restrict__ keys, int64_t L, cons
 
 // t[key] * diag * v plus the off-diagonal terms of a Hamiltonian (RS: HubbardRealSpace).
 template <bool RS, class K>
-__device__ inline double ham_term(const ModelDev& h, K key, double tval, double v,
+__device__ inline double ham_term(const ModelDev& h, const KeyMix& mix, K key, double tval, double v,
                                   const K* __restrict__ tk, const double* tvd,
                                   const int64_t* tvi, int64_t Lt, const K* s_split) {
     double res = tval * model_diagonal_element<RS>(h, key) * v;
     const int64_t n = model_num_offdiagonals<RS>(h, key);
     for (int64_t c = 1; c <= n; ++c) {
         double hv;
-        const K nb = model_offdiagonal<RS>(h, key, c, &hv);
+        const K nb = mix_key(mix, model_offdiagonal<RS>(h, key, c, &hv));
         if (hv == 0.0) continue;
         const int64_t q = find_split(tk, Lt, s_split, nb);
         if (q >= 0) res += (tvd ? tvd[q] : (double)tvi[q]) * hv * v;
@@ -100,15 +101,17 @@ __device__ inline double ham_term(const ModelDev& h, K key, double tval, double 
     return res;
 }
 
-// One entry (key, v) of u against operator op: t[key] * diag * v plus the off-diagonal terms.
+// One entry (stored key, v) of u against operator op: t[key] * diag * v plus the
+// off-diagonal terms.
 template <class K>
-__device__ __noinline__ double op_term(const OpDev& op, int M, K key, double tval, double v,
+__device__ __noinline__ double op_term(const OpDev& op, int M, K skey, double tval, double v,
                                        const K* __restrict__ tk, const double* tvd,
                                        const int64_t* tvi, int64_t Lt, const K* s_split) {
+    const K key = unmix_key(op.mix, skey);  // the address
     if (op.kind == OP_G2_REAL) return tval * g2_real(key, M, op.param) * v;
     if (op.kind == OP_DENSITY) return tval * (double)occupation(key, op.param) * v;
-    if (op.h.kind == 3) return ham_term<true>(op.h, key, tval, v, tk, tvd, tvi, Lt, s_split);
-    return ham_term<false>(op.h, key, tval, v, tk, tvd, tvi, Lt, s_split);
+    if (op.h.kind == 3) return ham_term<true>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split);
+    return ham_term<false>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split);
 }
 
 // ranges[t] = lower bound in t of u's key at t * OV_TILE (t < ntiles), ranges[ntiles] = Lt.

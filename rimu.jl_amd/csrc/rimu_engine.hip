@@ -78,6 +78,7 @@ struct rimu_engine {
     rimu_model model{};
     rimu_style style{};
     ModelDev hdev{};
+    KeyMix mix{};                   // stored keys are mix(address) above MIX_MIN_BITS bits
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
@@ -173,6 +174,10 @@ bool same_space(const ModelDev& a, const ModelDev& b) {
     return !ra || (a.ncomp == b.ncomp && a.N2 == b.N2 && a.fermi == b.fermi);
 }
 
+// Downloaded vectors: stored keys back to addresses, then ascending address order (the
+// order the C ABI promises).
+void unmix_and_sort(rimu_engine* e, uint64_t* keys, void* vals, int64_t n);
+
 // Whether key is an address of the model: inside B bits, with every component's particle
 // number (a BoseFS component of N particles has N ones among its N + M - 1 bits).
 bool address_ok(const ModelDev& h, u128 key) {
@@ -182,6 +187,24 @@ bool address_ok(const ModelDev& h, u128 key) {
     if (h.kind != RIMU_MODEL_HUBBARD_REAL_SPACE || h.ncomp == 1) return ones(key) == h.N;
     const u128 low = ((u128)1 << h.cbits2) - 1;
     return ones(key >> h.cbits2) == h.N && ones(key & low) == h.N2;
+}
+
+void unmix_and_sort(rimu_engine* e, uint64_t* keys, void* vals, int64_t n) {
+    const int kw = e->kw;
+    std::vector<u128> addr((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        addr[i] = kw == 1 ? (u128)unmix_bits<uint64_t>(e->mix, keys[i])
+                          : unmix_bits<u128>(e->mix, ((u128)keys[2 * i + 1] << 64) | keys[2 * i]);
+    std::vector<int64_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return addr[a] < addr[b]; });
+    std::vector<uint64_t> v((const uint64_t*)vals, (const uint64_t*)vals + n);  // 8-byte values
+    for (int64_t j = 0; j < n; ++j) {
+        const u128 x = addr[ord[j]];
+        keys[(size_t)kw * j] = (uint64_t)x;
+        if (kw == 2) keys[(size_t)kw * j + 1] = (uint64_t)(x >> 64);
+        ((uint64_t*)vals)[j] = v[ord[j]];
+    }
 }
 
 int check_real_space(const rimu_model* m) {
@@ -234,6 +257,7 @@ StepArgs make_args(rimu_engine* e, int64_t step, double shift, double dtau, doub
     a.abs_threshold = e->style.abs_threshold;
     a.init_rule = e->init_rule;
     a.init_threshold = e->init_threshold;
+    a.mix = e->mix;
     return a;
 }
 
@@ -560,6 +584,7 @@ int rimu_engine_create(const rimu_model* model, const rimu_style* style, uint64_
     e->model = *model;
     e->style = *style;
     e->hdev = to_dev(*model);
+    e->mix = make_key_mix(e->hdev.B);
     e->seed = seed;
     e->int_vals = style->kind == RIMU_STYLE_STOCHASTIC_INTEGER;
     e->kw = e->hdev.B > 64 ? 2 : 1;
@@ -628,6 +653,7 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
     std::vector<uint64_t> k;
     std::vector<int64_t> vi;
     std::vector<double> vd;
+    std::vector<u128> stored;  // mix(address) of the kept entries
     k.reserve((size_t)n * kw);
     u128 prev = 0;
     for (int64_t i = 0; i < n; ++i) {
@@ -645,7 +671,25 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
             if (v == 0) continue;
             vd.push_back(v);
         }
-        for (int w = 0; w < kw; ++w) k.push_back(keys[(size_t)kw * i + w]);
+        if (e->mix.on) {
+            stored.push_back(kw == 1 ? (u128)mix_bits<uint64_t>(e->mix, (uint64_t)key) : mix_bits<u128>(e->mix, key));
+        } else {
+            for (int w = 0; w < kw; ++w) k.push_back(keys[(size_t)kw * i + w]);
+        }
+    }
+    if (e->mix.on) {  // the device vector is kept in the order of the stored keys
+        std::vector<int64_t> ord(stored.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int64_t)i;
+        std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return stored[a] < stored[b]; });
+        std::vector<int64_t> vi2;
+        std::vector<double> vd2;
+        for (int64_t i : ord) {
+            k.push_back((uint64_t)stored[i]);
+            if (kw == 2) k.push_back((uint64_t)(stored[i] >> 64));
+            if (e->int_vals) vi2.push_back(vi[i]); else vd2.push_back(vd[i]);
+        }
+        vi.swap(vi2);
+        vd.swap(vd2);
     }
     const int64_t m = (int64_t)(k.size() / kw);
     if (m > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
@@ -674,6 +718,7 @@ int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap
         HIP_TRY(hipMemcpyAsync(vals, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->mix.on && e->len > 0) unmix_and_sort(e, keys, vals, e->len);
     return RIMU_OK;
 }
 
@@ -685,6 +730,22 @@ int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int
     HIP_TRY(e->pkeys.alloc(std::max<int64_t>(tot, 1) * e->kw));
     HIP_TRY(e->pcoef.alloc(std::max<int64_t>(tot, 1)));
     HIP_TRY(e->terms.alloc(std::max<int64_t>(tot, 16)));
+    std::vector<uint64_t> mk;  // projector keys in the stored form
+    if (e->mix.on) {
+        mk.resize((size_t)tot * e->kw);
+        for (int64_t i = 0; i < tot; ++i) {
+            const uint64_t* src = i < n ? pk + i * e->kw : hk + (i - n) * e->kw;
+            if (e->kw == 1) {
+                mk[i] = mix_bits<uint64_t>(e->mix, src[0]);
+            } else {
+                const u128 y = mix_bits<u128>(e->mix, ((u128)src[1] << 64) | src[0]);
+                mk[2 * i] = (uint64_t)y;
+                mk[2 * i + 1] = (uint64_t)(y >> 64);
+            }
+        }
+        pk = mk.data();
+        hk = mk.data() + n * e->kw;
+    }
     if (n > 0) {
         HIP_TRY(hipMemcpyAsync(e->pkeys.p, pk, n * 8 * e->kw, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->pcoef.p, pc, n * 8, hipMemcpyHostToDevice, e->stream));
@@ -739,6 +800,7 @@ int overlaps_impl(rimu_engine* left, rimu_engine* right, const rimu_operator* op
         OpDev d{};
         d.kind = ops[o].kind;
         d.param = ops[o].param;
+        d.mix = right->mix;
         if (d.kind == RIMU_OP_HAMILTONIAN) {
             if (int rc = check_model(&ops[o].model)) return rc;
             d.h = to_dev(ops[o].model);

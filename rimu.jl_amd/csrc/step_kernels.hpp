@@ -43,6 +43,7 @@ struct StepArgs {
     double comp_threshold, proj_threshold, rel_threshold, abs_threshold;
     int init_rule;    // 0 NonInitiator, 1 Initiator, 2 SimpleInitiator, 3 CoherentInitiator
     double init_threshold;
+    KeyMix mix;       // stored keys are mix(address)
 };
 
 // Initiator class of a record, carried in key bit 63 (addresses then have B <= 63 bits).
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
         const int64_t i = t0 + j * NT + threadIdx.x;
         int64_t n_items = 0;
         if (i < L) {
-            const K k = kk[j];
+            const K k = unmix_key(a.mix, kk[j]);  // the address
             const V v = vv[j];
             // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
             const double hd = model_diagonal_element<RS>(a.h, k);
@@ -419,6 +420,8 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restri
         eval[it] = vals[i];
     }
 #pragma unroll
+    for (int it = 0; it < SPAWN_ITEMS; ++it) ekey[it] = unmix_key(a.mix, ekey[it]);  // addresses
+#pragma unroll
     for (int it = 0; it < SPAWN_ITEMS; ++it) {
         const int64_t att = a0 + (int64_t)it * NT + threadIdx.x;
         bool ok = false;
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restri
             if (!c.exact || need_u) r = draw(a.seed, k, (uint32_t)j, a.step, TAG_SPAWN);
             const int64_t chosen = c.exact ? j + 1 : pick1(r.x, c.n_od);
             double hv;
-            nb = model_offdiagonal<RS>(a.h, k, chosen, &hv);
+            nb = mix_key(a.mix, model_offdiagonal<RS>(a.h, k, chosen, &hv));
             const double mel = -hv * a.dtau;  // FirstOrderOffdiagonals, fciqmc.jl:109-112
             double x;
             if (c.exact) {
@@ -962,7 +965,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
             for (int q = lane; q < wtot; q += WAVE) {
                 const K key = wk[q];
                 const double x = (double)wv[q];
-                const U4 r = draw(a.seed, key, 0, a.step, TAG_COMPRESS);
+                const U4 r = draw(a.seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
                 wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
             }
             __syncthreads();
