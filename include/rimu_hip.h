@@ -204,6 +204,13 @@ typedef struct rimu_operator {
 int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
                   double* out);
 
+/* (No counterpart in the reference; for tests and tuning.)  Addresses of more than 40 bits
+ * are stored as a bijective mix of their bits, and a step's spawn records are radix-sorted
+ * on the top `bits` bits of the mixed keys only (default 40); runs that agree there are then
+ * ordered one by one, or, when such runs are long or many, by the remaining passes.  Any
+ * setting gives the same step results; fewer bits exercise the run ordering. */
+int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits);
+
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on
  * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled);

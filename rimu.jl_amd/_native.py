@@ -26,7 +26,7 @@ EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
     "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
-    "rimu_engine_set_timing",
+    "rimu_engine_set_sort_bits", "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
@@ -105,6 +105,7 @@ def lib():
         L.rimu_set_projector.argtypes = [vp, vp, vp, i64, vp, vp, i64]
         L.rimu_step.argtypes = [vp, i64, dbl, dbl, dbl, P(StepStats)]
         L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
+        L.rimu_engine_set_sort_bits.argtypes = [vp, i32]
         L.rimu_overlaps.argtypes = [vp, vp, P(Operator), i32, P(dbl)]
         L.rimu_engine_set_timing.argtypes = [vp, i32]
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
@@ -295,6 +296,10 @@ class Engine:
                 continue
             check(rc)
             return st
+
+    def set_sort_bits(self, bits: int):
+        """Top bits of mixed keys that the radix passes cover (tests and tuning)."""
+        check(lib().rimu_engine_set_sort_bits(self._h, int(bits)))
 
     def set_initiator(self, rule: int, threshold: float = 1.0):
         """Initiator rule of the vector (INITIATOR_* codes) and its threshold."""

@@ -79,6 +79,7 @@ struct rimu_engine {
     rimu_style style{};
     ModelDev hdev{};
     KeyMix mix{};                   // stored keys are mix(address) above MIX_MIN_BITS bits
+    int sort_bits = MIX_SORT_BITS;  // top bits of mixed keys the radix passes cover
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
@@ -112,6 +113,7 @@ struct rimu_engine {
     DevBuf<unsigned char> ov_ops;
     DevBuf<double> ov_part, ov_out;
     DevBuf<int64_t> ov_ranges;
+    DevBuf<int64_t> fix_runs;       // runs to reorder after a sort on the top key bits
     int64_t np = 0, nh = 0;
 
     // host staging
@@ -123,6 +125,7 @@ struct rimu_engine {
         int64_t L_new;
         u128 key_or;     // as wide as the widest key; a one-chunk key fills the low word
         u128 key_and;
+        uint32_t fix_count;  // runs listed by fixup_find_kernel
     };
     Misc* h_misc = nullptr;    // pinned
     double* h_stats = nullptr; // pinned, 12 doubles
@@ -287,14 +290,14 @@ int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
 // (the initiator class bit).  Input in buffer `in`; returns the buffer index holding the
 // result.
 template <class K, class V>
-int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result) {
+int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo = 0) {
     int src = in;
     if (n <= 1) { *result = src; return RIMU_OK; }
     const int64_t tiles = ceil_div(n, sort_tile<K>::value);
     const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
     HIP_TRY(e->rchunks.alloc((size_t)nchunks * RADIX));
-    for (int shift = 0; shift < nbits; shift += RADIX_BITS) {
+    for (int shift = lo; shift < nbits; shift += RADIX_BITS) {
         if ((uint32_t)((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
         const uint32_t dmask = shift + RADIX_BITS > nbits ? (1u << (nbits - shift)) - 1u : (uint32_t)(RADIX - 1);
         hipEvent_t ev{};
@@ -316,6 +319,36 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* 
         src = 1 - src;
     }
     *result = src;
+    return RIMU_OK;
+}
+
+// Sort of mixed keys: radix passes on the top MIX_SORT_BITS bits, then the runs that agree
+// there but not below are put in order (fixup kernels).  If a run is too long or there are
+// too many, the passes below the top bits and the top passes again make the order complete
+// (stable LSD passes compose, so equal keys keep their list order either way).
+template <class K, class V>
+int sort_mixed(rimu_engine* e, int64_t n, int nbits, u128 varying, bool init, rimu_engine::Misc* dm, int* result) {
+    const int lo = nbits - e->sort_bits;
+    int sorted = 0;
+    if (int rc = radix_sort<K, V>(e, n, nbits, varying, 0, &sorted, lo)) return rc;
+    const K kmask = init ? (K)~class_bit<K>() : (K)~(K)0;
+    HIP_TRY(e->fix_runs.alloc(FIXUP_LIST));
+    hipEvent_t ev{};
+    e->timer.begin("sort_fixup", e->stream, &ev);
+    fixup_find_kernel<K><<<ceil_div(n, NT), NT, 0, e->stream>>>((const K*)e->skeys[sorted].p, n, lo, kmask,
+                                                                e->fix_runs.p, &dm->fix_count, &dm->flags);
+    fixup_sort_kernel<K, V><<<64, NT, 0, e->stream>>>((K*)e->skeys[sorted].p, (V*)e->svals[sorted].p, n, lo,
+                                                      kmask, e->fix_runs.p, &dm->fix_count, &dm->flags);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->h_misc->flags & FLAG_FIXUP_FALLBACK) {
+        int low = 0;
+        if (int rc = radix_sort<K, V>(e, n, lo, varying, sorted, &low)) return rc;
+        if (int rc = radix_sort<K, V>(e, n, nbits, varying, low, &sorted, lo)) return rc;
+    }
+    *result = sorted;
     return RIMU_OK;
 }
 
@@ -478,7 +511,11 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     int sorted = 0;
     if (S > 1) {
         const int nbits = e->hdev.B;
-        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
+        if (!e->mix.on || nbits <= e->sort_bits) {
+            if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
+        } else if (int rc = sort_mixed<K, V>(e, S, nbits, kor ^ kand, a.init_rule != 0, dm, &sorted)) {
+            return rc;
+        }
     }
 
     // 4. merge with the diagonal survivors, fold, compress, count
@@ -872,6 +909,13 @@ int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold) {
         return fail(RIMU_ERR_ARG, "initiator rules need N + M - 1 <= 63 or 65..127 (the top key bit holds the class)");
     e->init_rule = rule;
     e->init_threshold = rule == RIMU_INITIATOR_NONE ? 0.0 : threshold;
+    return RIMU_OK;
+}
+
+int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (bits < 8 || bits > 128) return fail(RIMU_ERR_ARG, "sort bits must be in 8..128");
+    e->sort_bits = bits;
     return RIMU_OK;
 }
 

@@ -250,3 +250,40 @@ def test_two_chunk_resume_from_state_file_is_bit_exact(gpu, tmp_path):
     # uninterrupted run's is the device's sum over merge tiles: the shifts agree to rounding
     np.testing.assert_allclose(fdf["shift"].to_numpy()[60:], rdf["shift"].to_numpy(), rtol=1e-12)
     assert np.array_equal(fdf["len"].to_numpy()[60:], rdf["len"].to_numpy())
+
+
+@pytest.mark.parametrize("bits", [8, 12, 16, 24, 40, 128])
+@pytest.mark.parametrize("case", [("real", 50, 50, "dynamic", 3000, 20.0), ("mom", 40, 30, "dynamic", 1500, 10.0),
+                                  ("real", 32, 32, "integer", 3000, 8)],
+                         ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[3]}")
+def test_top_bit_sort_matches_oracle(gpu, case, bits):
+    """Keys above 40 bits are sorted on their top `bits` mixed bits, then runs that agree
+    there are ordered (or, past the run limits, the remaining passes run).  Every setting
+    gives the oracle's step bit for bit; 8 bits on the momentum case takes the fallback."""
+    kind, M, N, style, n, scale = case
+    rng = np.random.default_rng(zlib.crc32(repr((case, bits)).encode()))
+    if M + N - 1 > 64:
+        keys, vals = wide_vector(M, N, n, rng, "int" if style == "integer" else "float", scale)
+    else:
+        from test_gpu_parity import random_vector
+        keys, vals = random_vector(M, N, n, rng, "int" if style == "integer" else "float", scale)
+    seed = 0xF1 + bits
+    eng = _engine(gpu, kind, M, N, 4.0, style, seed)
+    eng.set_sort_bits(bits)
+    eng.upload(keys, vals)
+    om = oracle_model(kind, M, N, 4.0, 1.0)
+    ck, cv = keys, vals
+    for step in (1, 2):
+        st = eng.step(step, -2.0, 2e-3)
+        gk, gv = eng.download()
+        ok, ov, ostats = O.step(om, O.style(style), ck, cv, seed=seed, step=step, shift=-2.0,
+                                dtau=2e-3, order=1)
+        _compare(style, gk, gv, st, ok, ov, ostats)
+        ck, cv = gk, gv
+    eng.close()
+    with pytest.raises(gpu.RimuError):
+        eng2 = _engine(gpu, kind, M, N, 4.0, style, seed)
+        try:
+            eng2.set_sort_bits(4)
+        finally:
+            eng2.close()
