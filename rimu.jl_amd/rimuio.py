@@ -18,6 +18,12 @@ metadata)`` produces it (``RimuIO.jl:113-118``, the table of ``tables.jl``):
   ``save_state``, each as a string; ``load_state`` parses each back as Int, Float64,
   ComplexF64, Bool or leaves it a string (``RimuIO.jl:158-173``).
 
+FermiFS keys are the same with extension name ``Rimu.FermiFS.BitString`` and B = M
+(``arrowtypes.jl:76-81``).  A CompositeFS key (``arrowtypes.jl:107-148``) has extension
+name ``Rimu.CompositeFS``, metadata ``"name1:meta1;name2:meta2"`` of its components, and
+storage a Struct with fields ``1`` and ``2`` holding each component's chunk tuple (Arrow.jl
+stores a tuple of tuples as a named tuple with names 1, 2).
+
 The Arrow conventions follow Arrow.jl (Rimu's dependency, compat "1.5, 2"): an extension
 type is the field metadata pair ``ARROW:extension:name`` / ``ARROW:extension:metadata``,
 an ``NTuple{N,T}`` is a FixedSizeList<T>[N], and a field is nullable only for a
@@ -29,12 +35,14 @@ import re
 
 import numpy as np
 
-from .addresses import BoseFS, key_words, pack_keys, unpack_keys
+from .addresses import BoseFS, CompositeFS, FermiFS, key_words, pack_keys, unpack_keys
 from .dictvectors import DVec
 from .styles import IsDynamicSemistochastic, IsStochasticInteger
 
 RIMU_PACKAGE_VERSION = "0.14.0"   # the Rimu.jl release whose format this mirrors
 BOSEFS_BS = "Rimu.BoseFS.BitString"
+FERMIFS_BS = "Rimu.FermiFS.BitString"
+COMPOSITEFS = "Rimu.CompositeFS"
 
 _CHUNK_TYPES = ((8, np.uint8), (16, np.uint16), (32, np.uint32), (64, np.uint64))
 
@@ -98,27 +106,77 @@ def _parse_meta(k: str, v: str):
 
 
 def _vector_arrays(vector):
-    """(M, N, keys u64, values) of a DVec, a DeviceDVec or a simulation's state vector."""
+    """(address template, keys (word array), values, style) of a DVec, a DeviceDVec or a
+    simulation's state vector; the template is None for an empty host DVec."""
     if hasattr(vector, "state_vector"):
         vector = vector.state_vector
     if hasattr(vector, "engine"):
-        if not isinstance(vector.hamiltonian.address, BoseFS):
-            raise NotImplementedError("state files hold BoseFS addresses only")
         keys, vals = vector.engine.download()
-        return vector.hamiltonian.M, vector.hamiltonian.N, keys, vals, vector.style
+        return vector.hamiltonian.address, keys, vals, vector.style
     if isinstance(vector, DVec):
         items = sorted(vector.storage.items(), key=lambda kv: kv[0].bits)
-        if items and not isinstance(items[0][0], BoseFS):
-            raise NotImplementedError("state files hold BoseFS addresses only")
-        if items:
-            M, N = items[0][0].M, items[0][0].N
-        else:
-            M = N = None
-        keys = pack_keys([k.bits for k, _ in items], key_words(M, N) if items else 1)
+        template = items[0][0] if items else None
+        keys = pack_keys([k.bits for k, _ in items], template.key_words if items else 1)
         integer = isinstance(vector.style, IsStochasticInteger)
         vals = np.array([v for _, v in items], dtype=np.int64 if integer else np.float64)
-        return M, N, keys, vals, vector.style
+        return template, keys, vals, vector.style
     raise TypeError("save_state takes a DVec, a DeviceDVec or a PMCSimulation")
+
+
+def _single_spec(addr):
+    """(extension name, metadata, chunk dtype, chunks) of a BoseFS or FermiFS on a BitString."""
+    B = addr.key_bits
+    name = FERMIFS_BS if isinstance(addr, FermiFS) else BOSEFS_BS
+    return name, f"{addr.N}.{addr.M}.{B}", chunk_dtype(B), (2 if B > 64 else 1)
+
+
+def _chunk_array(pa, ints, B):
+    """FixedSizeList array of the BitString chunks (chunk 1 = most significant) of B-bit ints."""
+    dt = chunk_dtype(B)
+    nchunks = 2 if B > 64 else 1
+    for x in ints:
+        if x >> B:
+            raise ValueError("key outside the B-bit address space")
+    if nchunks == 1:
+        flat = np.array(ints, dtype=np.uint64).astype(dt)
+    else:
+        flat = np.array([[x >> 64, x & ((1 << 64) - 1)] for x in ints], dtype=np.uint64).reshape(-1)
+    ktype = pa.list_(pa.field("item", pa.from_numpy_dtype(dt), nullable=False), nchunks)
+    return pa.FixedSizeListArray.from_arrays(pa.array(flat), type=ktype), ktype
+
+
+def address_state_table(template, keys: np.ndarray, vals: np.ndarray, **metadata):
+    """The Arrow table of a state whose addresses have the type of ``template`` (a BoseFS,
+    FermiFS or two-component CompositeFS); keys are a word array of those addresses."""
+    pa, _ = _pa()
+    if isinstance(template, BoseFS) and not isinstance(template, CompositeFS):
+        return state_table(template.M, template.N, keys, vals, **metadata)
+    vals = np.ascontiguousarray(vals)
+    if vals.dtype not in (np.int64, np.float64):
+        raise TypeError("values must be int64 or float64")
+    ints = unpack_keys(np.ascontiguousarray(keys, dtype=np.uint64))
+    if isinstance(template, FermiFS):
+        name, meta_s, _, _ = _single_spec(template)
+        key_arr, ktype = _chunk_array(pa, ints, template.key_bits)
+    elif isinstance(template, CompositeFS):
+        a, b = template.components
+        wb = b.key_bits
+        arr_a, ta = _chunk_array(pa, [x >> wb for x in ints], a.key_bits)
+        arr_b, tb = _chunk_array(pa, [x & ((1 << wb) - 1) for x in ints], b.key_bits)
+        fields = [pa.field("1", ta, nullable=False), pa.field("2", tb, nullable=False)]
+        key_arr = pa.StructArray.from_arrays([arr_a, arr_b], fields=fields)
+        ktype = key_arr.type
+        name = COMPOSITEFS
+        meta_s = ";".join(f"{n}:{m}" for n, m, _, _ in (_single_spec(a), _single_spec(b)))
+    else:
+        raise TypeError(f"unsupported address type {type(template).__name__}")
+    key_field = pa.field("key", ktype, nullable=False, metadata={
+        b"ARROW:extension:name": name.encode(), b"ARROW:extension:metadata": meta_s.encode()})
+    val_field = pa.field("value", pa.from_numpy_dtype(vals.dtype), nullable=False)
+    meta = {"RIMU_PACKAGE_VERSION": RIMU_PACKAGE_VERSION}
+    meta.update({str(k): _julia_string(v) for k, v in metadata.items()})
+    schema = pa.schema([key_field, val_field], metadata=meta)
+    return pa.Table.from_arrays([key_arr, pa.array(vals)], schema=schema)
 
 
 def state_table(M: int, N: int, keys: np.ndarray, vals: np.ndarray, **metadata):
@@ -157,22 +215,81 @@ def state_table(M: int, N: int, keys: np.ndarray, vals: np.ndarray, **metadata):
 def save_state(filename, vector, **metadata):
     """``save_state(filename, vector; metadata...)`` (RimuIO.jl:92-118)."""
     pa, ipc = _pa()
-    M, N, keys, vals, _ = _vector_arrays(vector)
-    if M is None:
+    template, keys, vals, _ = _vector_arrays(vector)
+    if template is None:
         raise ValueError("an empty host DVec has no address type; save it from a DeviceDVec")
-    table = state_table(M, N, keys, vals, **metadata)
+    table = address_state_table(template, keys, vals, **metadata)
     opts = ipc.IpcWriteOptions(compression="zstd", use_threads=True)
     with ipc.new_file(str(filename), table.schema, options=opts) as w:
         w.write_table(table)
 
 
-def read_state_arrays(filename):
-    """(M, N, keys u64 sorted, values, metadata) of a state file."""
+def _read_table(filename):
     pa, ipc = _pa()
     with pa.memory_map(str(filename), "r") as src:
         table = ipc.open_file(src).read_all()
     if tuple(table.schema.names) != ("key", "value"):
         raise ValueError(f"`{filename}` is not a valid Rimu state file")
+    return table
+
+
+def _single_from_meta(name, meta_s):
+    N, M, B = (int(x) for x in meta_s.split("."))
+    if name == BOSEFS_BS and B == N + M - 1 and B <= 128:
+        return BoseFS(M=M, N=N, bits=0)
+    if name == FERMIFS_BS and B == M and M <= 128:
+        return FermiFS(M=M, N=N, bits=0)
+    raise NotImplementedError(f"key type {name} {meta_s} is not supported")
+
+
+def _chunk_ints(col, B):
+    nchunks = 2 if B > 64 else 1
+    if col.type.list_size != nchunks:
+        raise NotImplementedError("unexpected number of BitString chunks")
+    flat = col.flatten().to_numpy(zero_copy_only=False).astype(np.uint64)
+    if nchunks == 1:
+        return [int(x) for x in flat.tolist()]
+    return [(int(hi) << 64) | int(lo) for hi, lo in flat.reshape(-1, 2).tolist()]
+
+
+def read_state(filename):
+    """(address template, keys as a sorted word array, values, metadata) of a state file of
+    BoseFS, FermiFS or two-component CompositeFS addresses."""
+    table = _read_table(filename)
+    kf = table.schema.field("key")
+    fm = kf.metadata or {}
+    name = fm.get(b"ARROW:extension:name", b"").decode()
+    meta_s = fm.get(b"ARROW:extension:metadata", b"").decode()
+    col = table.column("key").combine_chunks()
+    if name in (BOSEFS_BS, FERMIFS_BS):
+        template = _single_from_meta(name, meta_s)
+        ints = _chunk_ints(col, template.key_bits)
+    elif name == COMPOSITEFS:
+        parts = [p.split(":", 1) for p in meta_s.split(";")]
+        if len(parts) != 2:
+            raise NotImplementedError("CompositeFS state files of two components are supported")
+        a, b = (_single_from_meta(n, m) for n, m in parts)
+        if a.M != b.M:
+            raise ValueError("components with different numbers of modes")
+        ia = _chunk_ints(col.field(0), a.key_bits)
+        ib = _chunk_ints(col.field(1), b.key_bits)
+        template = CompositeFS(a, b)
+        ints = [(x << b.key_bits) | y for x, y in zip(ia, ib)]
+    else:
+        raise NotImplementedError(f"key type {name or kf.type} is not supported")
+    vals = table.column("value").combine_chunks().to_numpy(zero_copy_only=False)
+    order = sorted(range(len(ints)), key=ints.__getitem__)   # files from PDVecs are unsorted
+    keys = pack_keys([ints[i] for i in order], template.key_words)
+    vals = vals[np.array(order, dtype=np.int64)] if len(order) else vals
+    raw = table.schema.metadata or {}
+    meta = {k.decode(): _parse_meta(k.decode(), v.decode()) for k, v in raw.items()}
+    return template, keys, vals, meta
+
+
+def read_state_arrays(filename):
+    """(M, N, keys u64 sorted, values, metadata) of a BoseFS state file."""
+    pa, ipc = _pa()
+    table = _read_table(filename)
     kf = table.schema.field("key")
     fm = kf.metadata or {}
     name = fm.get(b"ARROW:extension:name", b"").decode()
@@ -204,12 +321,12 @@ def load_state(filename, style=None):
     """``load_state(filename)`` (RimuIO.jl:130-180): a DVec and the parsed metadata.  The
     default style is IsDynamicSemistochastic for floating values and the default style of
     the value type otherwise."""
-    M, N, keys, vals, meta = read_state_arrays(filename)
+    template, keys, vals, meta = read_state(filename)
     if style is None:
         style = IsStochasticInteger() if vals.dtype.kind in "iu" else IsDynamicSemistochastic()
     out = DVec(style=style)
     for k, v in zip(unpack_keys(keys), vals.tolist()):
-        out[BoseFS(M=M, N=N, bits=k)] = v
+        out[template.with_bits(k)] = v
     return out, meta
 
 

@@ -248,3 +248,34 @@ def test_upload_rejects_wrong_components(gpu):
     with pytest.raises(gpu.RimuError):
         eng.upload(np.array([bad], np.uint64), np.array([3]))
     eng.close()
+
+
+def test_state_file_round_trip_two_component(gpu, tmp_path):
+    """save_state / load_state of a FermiFS2C run (Rimu.CompositeFS keys): the loaded vector
+    equals the device vector, and a run resumed from it continues bit for bit."""
+    a = gpu.FermiFS2C(16, (1, 1), (2, 1), (1, -1), (2, -1))
+    H = gpu.HubbardRealSpace(a, geometry=gpu.PeriodicBoundaries(4, 4), u=[[0, 4], [4, 0]])
+
+    def problem(**kw):
+        return gpu.ProjectorMonteCarloProblem(
+            H, style=gpu.IsDynamicSemistochastic(), time_step=1e-3, random_seed=5,
+            shift_strategy=gpu.DoubleLogUpdate(target_walkers=2000), **kw)
+
+    full = gpu.solve(problem(last_step=80))
+    fk, fv = full.engine.download()
+    full.close()
+    part = gpu.solve(problem(last_step=40))
+    pk, pv = part.engine.download()
+    f = tmp_path / "fermi2c.arrow"
+    gpu.save_state(f, part, step=40, shift=part.shift_parameters.shift)
+    part.close()
+    dv, meta = gpu.load_state(f)
+    assert meta["step"] == 40
+    got = sorted((k.bits, v) for k, v in dv.pairs())
+    assert [k for k, _ in got] == O.unpack_keys(pk)
+    assert np.array_equal(np.array([v for _, v in got]).view(np.uint64), pv.view(np.uint64))
+    resumed = gpu.solve(problem(start_at=dv, shift=meta["shift"], starting_step=40, last_step=80))
+    rk, rv = resumed.engine.download()
+    resumed.close()
+    assert len(rk) > 100
+    assert np.array_equal(fk, rk) and np.array_equal(fv.view(np.uint64), rv.view(np.uint64))

@@ -155,3 +155,67 @@ def test_float_bits_survive(R, tmp_path):
     M, N, k, v, _ = R.RimuIO.read_state_arrays(f)
     assert (M, N) == (3, 3) and np.array_equal(k, keys)
     assert np.array_equal(v.view(np.uint64), vals.view(np.uint64))
+
+
+# ---------------------------------------------------------------------------------------
+# FermiFS and CompositeFS keys (arrowtypes.jl:76-148)
+# ---------------------------------------------------------------------------------------
+def _random_fermi(M, N, rng):
+    onr = [0] * M
+    for i in rng.choice(M, N, replace=False):
+        onr[i] = 1
+    return tuple(onr)
+
+
+@pytest.mark.parametrize("M,N", [(12, 5), (16, 8), (64, 20), (100, 40)])
+def test_fermifs_state_round_trip(rimu, tmp_path, M, N):
+    R = rimu
+    rng = np.random.default_rng(M + N)
+    dv = R.DVec(style=R.IsDynamicSemistochastic())
+    for _ in range(200):
+        dv[R.FermiFS(_random_fermi(M, N, rng))] = float(rng.normal())
+    f = tmp_path / "f.arrow"
+    R.save_state(f, dv, step=3)
+    import pyarrow.ipc as ipc
+    t = ipc.open_file(str(f)).read_all()
+    kf = t.schema.field("key")
+    assert kf.metadata[b"ARROW:extension:name"] == b"Rimu.FermiFS.BitString"
+    assert kf.metadata[b"ARROW:extension:metadata"] == f"{N}.{M}.{M}".encode()
+    assert kf.type.list_size == (2 if M > 64 else 1) and not kf.nullable
+    out, meta = R.load_state(f)
+    assert meta["step"] == 3 and dict(out.pairs()) == dict(dv.pairs())
+
+
+@pytest.mark.parametrize("kinds,M,N1,N2", [(("fermi", "fermi"), 16, 2, 2), (("bose", "fermi"), 12, 5, 3),
+                                           (("fermi", "fermi"), 64, 20, 20), (("bose", "bose"), 40, 30, 20)])
+def test_compositefs_state_round_trip(rimu, tmp_path, kinds, M, N1, N2):
+    R = rimu
+    rng = np.random.default_rng(M * 7 + N1)
+
+    def comp(kind, n):
+        if kind == "fermi":
+            return R.FermiFS(_random_fermi(M, n, rng))
+        cuts = np.sort(rng.choice(n + M - 1, M - 1, replace=False))
+        return R.BoseFS(tuple(int(x) for x in np.diff(np.concatenate([[-1], cuts, [n + M - 1]])) - 1))
+
+    dv = R.DVec(style=R.IsStochasticInteger())
+    for _ in range(150):
+        dv[R.CompositeFS(comp(kinds[0], N1), comp(kinds[1], N2))] = int(rng.integers(1, 9))
+    f = tmp_path / "c.arrow"
+    R.save_state(f, dv)
+    import pyarrow.ipc as ipc
+    t = ipc.open_file(str(f)).read_all()
+    kf = t.schema.field("key")
+    names = {"fermi": "Rimu.FermiFS.BitString", "bose": "Rimu.BoseFS.BitString"}
+    bits = [M if k == "fermi" else n + M - 1 for k, n in zip(kinds, (N1, N2))]
+    assert kf.metadata[b"ARROW:extension:name"] == b"Rimu.CompositeFS"
+    assert kf.metadata[b"ARROW:extension:metadata"].decode() == ";".join(
+        f"{names[k]}:{n}.{M}.{b}" for k, n, b in zip(kinds, (N1, N2), bits))
+    assert [kf.type.field(i).name for i in range(2)] == ["1", "2"]
+    assert [kf.type.field(i).type.list_size for i in range(2)] == [2 if b > 64 else 1 for b in bits]
+    out, _ = R.load_state(f)
+    assert dict(out.pairs()) == dict(dv.pairs())
+    template, keys, vals, _ = R.RimuIO.read_state(f)
+    assert template.signature == next(iter(dv.pairs()))[0].signature
+    ints = R.addresses.unpack_keys(keys)
+    assert ints == sorted(ints)
