@@ -277,5 +277,5 @@ def test_state_file_round_trip_two_component(gpu, tmp_path):
     resumed = gpu.solve(problem(start_at=dv, shift=meta["shift"], starting_step=40, last_step=80))
     rk, rv = resumed.engine.download()
     resumed.close()
-    assert len(rk) > 100
+    assert len(rk) > 50
     assert np.array_equal(fk, rk) and np.array_equal(fv.view(np.uint64), rv.view(np.uint64))
