@@ -287,3 +287,36 @@ def test_top_bit_sort_matches_oracle(gpu, case, bits):
             eng2.set_sort_bits(4)
         finally:
             eng2.close()
+
+
+@pytest.mark.parametrize("kind,M,N", [("real", 50, 50), ("real", 32, 32)])
+def test_projection_on_mixed_keys(gpu, kind, M, N):
+    """ProjectedEnergy terms on keys stored mixed (above 40 bits): vproj = sum pc_i v[pk_i]
+    and hproj = sum hc_i v[hk_i] from the device equal the same sums over the downloaded
+    vector."""
+    rng = np.random.default_rng(M)
+    if M + N - 1 > 64:
+        keys, vals = wide_vector(M, N, 3000, rng, "float", 5.0)
+        ints = O.unpack_keys(keys)
+    else:
+        from test_gpu_parity import random_vector
+        keys, vals = random_vector(M, N, 3000, rng, "float", 5.0)
+        ints = [int(x) for x in keys.tolist()]
+    kw = 2 if M + N - 1 > 64 else 1
+    eng = _engine(gpu, kind, M, N, 4.0, "dynamic", 3)
+    eng.upload(keys, vals)
+    # projector: some stored addresses and some that are not
+    pk = ints[::50] + wide_addresses(M, N, 20, rng)
+    pc = rng.normal(size=len(pk))
+    hk = ints[7::40]
+    hc = rng.normal(size=len(hk))
+    eng.set_projector(O.pack_keys(pk, kw), pc, O.pack_keys(hk, kw), hc)
+    st = eng.step(1, -2.0, 1e-3)
+    gk, gv = eng.download()
+    eng.close()
+    v = dict(zip(O.unpack_keys(gk), gv.tolist()))
+    want_v = sum(c * v.get(k, 0.0) for k, c in zip(pk, pc))
+    want_h = sum(c * v.get(k, 0.0) for k, c in zip(hk, hc))
+    assert st.vproj == pytest.approx(want_v, rel=1e-12, abs=1e-12)
+    assert st.hproj == pytest.approx(want_h, rel=1e-12, abs=1e-12)
+    assert want_v != 0 and want_h != 0
