@@ -364,7 +364,7 @@ int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int6
     constexpr int KW = sizeof(K) / 8;
     const bool init = a.init_rule != 0;
     if (init && !bo) return fail(RIMU_ERR_INTERNAL, "initiator merge without the old values");
-    constexpr int64_t TILE = NT * MERGE_IPT;
+    constexpr int64_t TILE = NT * merge_ipt<K>::value;
     const int64_t nt = std::max<int64_t>(ceil_div(S + L, TILE), 1);
     *tiles = nt;
     HIP_TRY(e->merge_p.alloc((size_t)nt * 3));
@@ -390,7 +390,7 @@ int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int6
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
     if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nt + 1)) return rc;
-    compact_kernel<K, V, MERGE_IPT, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
+    compact_kernel<K, V, merge_ipt<K>::value, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
         (const K*)e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok, ov, cap, &dm->L_new, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());

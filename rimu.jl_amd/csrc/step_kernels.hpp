@@ -19,7 +19,12 @@
 namespace rimu {
 
 constexpr int NT = 256;          // workgroup size of every step kernel
-constexpr int MERGE_IPT = 8;     // merged positions per thread
+constexpr int MERGE_IPT = 8;     // merged positions per thread (one-chunk keys)
+// Merged positions per thread for key type K: two-chunk keys take 12 (3072-position tiles,
+// 75 KiB of LDS).  Measured at 1e8 walkers: the 128-bit merge takes 3.22 ms with 12
+// against 3.67 ms with 8 and 5.36 ms with 16; the 64-bit merge is fastest with 8 (1.09 ms
+// against 1.82 ms with 12).
+template <class K> struct merge_ipt { static constexpr int value = sizeof(K) == 8 ? MERGE_IPT : 12; };
 constexpr int SPAWN_TILE_SIZE = 1024;  // attempts per spawn tile (NT * SPAWN_ITEMS)
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__
 // len_before, len, norm.
 // ---------------------------------------------------------------------------------------
 constexpr int MERGE_EXTRA = 64;  // spawn records staged past the tile for runs that cross it
-static_assert(MERGE_IPT <= 32, "merge candidate masks are 32-bit");
+static_assert(MERGE_IPT <= 32 && merge_ipt<unsigned __int128>::value <= 32, "merge candidate masks are 32-bit");
 static_assert(MERGE_EXTRA <= NT, "one extra record per thread at most");
 
 // from_initiator_value of the rule for the components (safe, unsafe, initiator):
@@ -849,7 +854,8 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
                                                    const int64_t* __restrict__ splits,
                                                    const K* __restrict__ prevk,
                                                    const V* __restrict__ bo) {
-    constexpr int TILE = NT * MERGE_IPT;
+    constexpr int IPT = merge_ipt<K>::value;
+    constexpr int TILE = NT * IPT;
     constexpr K KM = INIT ? (K)~class_bit<K>() : (K)~(K)0;
     // the tile's A range, its B range, then up to MERGE_EXTRA records of A past the tile
     __shared__ K s_key[TILE + MERGE_EXTRA];
@@ -867,10 +873,10 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
     // stage A, B and the extra records: all loads of a thread are issued before the first
     // LDS store, so a wave waits for one memory round trip instead of one per element
     {
-        K rk[MERGE_IPT];
-        V rv[MERGE_IPT];
+        K rk[IPT];
+        V rv[IPT];
 #pragma unroll
-        for (int r = 0; r < MERGE_IPT; ++r) {
+        for (int r = 0; r < IPT; ++r) {
             const int i = (int)threadIdx.x + r * NT;
             if (i < na + nb) {
                 const bool in_a = i < na;
@@ -888,7 +894,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
         const int x = (int)threadIdx.x;
         if (x < ne) { ek = ak[a1 + x]; ev = av[a1 + x]; }
 #pragma unroll
-        for (int r = 0; r < MERGE_IPT; ++r) {
+        for (int r = 0; r < IPT; ++r) {
             const int i = (int)threadIdx.x + r * NT;
             if (i < na + nb) { s_key[i] = rk[r]; s_val[i] = rv[r]; }
         }
@@ -899,8 +905,8 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
     const K* s_b = s_key + na;
 
     // this thread's positions within the tile
-    const int dd = (int)threadIdx.x * MERGE_IPT;
-    const int n_here = (int)max((int64_t)0, min((int64_t)MERGE_IPT, (d1 - d0) - dd));
+    const int dd = (int)threadIdx.x * IPT;
+    const int n_here = (int)max((int64_t)0, min((int64_t)IPT, (d1 - d0) - dd));
     int ia = 0;
     {
         int lo = max(0, dd - nb), hi = min(dd, na);
@@ -926,12 +932,12 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
 
     // candidates stay in the slot of their position j (static register indices); bit j of
     // cvalid marks a segment head with a nonzero folded value
-    K ck[MERGE_IPT];
-    V cv[MERGE_IPT];
+    K ck[IPT];
+    V cv[IPT];
     uint32_t cvalid = 0;
     double lb = 0, ln = 0, nrm = 0;
 #pragma unroll
-    for (int j = 0; j < MERGE_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         ck[j] = 0;
         cv[j] = (V)0;
         if (j < n_here) {
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
             const double th = a.comp_threshold;
             uint32_t need = 0;
 #pragma unroll
-            for (int j = 0; j < MERGE_IPT; ++j)
+            for (int j = 0; j < IPT; ++j)
                 if (((cvalid >> j) & 1u) && fabs((double)cv[j]) / th < 1) need |= 1u << j;
             const int lane = threadIdx.x & (WAVE - 1);
             const int wid = threadIdx.x / WAVE;
@@ -1011,11 +1017,11 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
             const int incl = wave_inclusive_scan(nneed);
             const int wtot = __shfl(incl, WAVE - 1, WAVE);
             const int mybase = incl - nneed;
-            K* wk = s_key + wid * (WAVE * MERGE_IPT);
-            V* wv = s_val + wid * (WAVE * MERGE_IPT);
+            K* wk = s_key + wid * (WAVE * IPT);
+            V* wv = s_val + wid * (WAVE * IPT);
             __syncthreads();  // the tile's inputs are no longer read
 #pragma unroll
-            for (int j = 0; j < MERGE_IPT; ++j) {
+            for (int j = 0; j < IPT; ++j) {
                 if ((need >> j) & 1u) {
                     const int q = mybase + __popc(need & ((1u << j) - 1u));
                     wk[q] = ck[j];
@@ -1031,7 +1037,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
             }
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < MERGE_IPT; ++j) {
+            for (int j = 0; j < IPT; ++j) {
                 if ((need >> j) & 1u) {
                     const int q = mybase + __popc(need & ((1u << j) - 1u));
                     cv[j] = wv[q];
@@ -1041,7 +1047,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
         }
     }
 #pragma unroll
-    for (int j = 0; j < MERGE_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         if ((cvalid >> j) & 1u) {
             ln += 1;
             nrm += fabs((double)cv[j]);
@@ -1052,7 +1058,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
     const int ex = block_exclusive_scan<int, NT>(count, &btotal);  // ends with a barrier
     // stage the survivors in LDS (the input keys are no longer read) for coalesced writes
 #pragma unroll
-    for (int j = 0; j < MERGE_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         if ((cvalid >> j) & 1u) {
             const int q = ex + __popc(cvalid & ((1u << j) - 1u));
             s_key[q] = ck[j];
