@@ -30,12 +30,13 @@ constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SORT_IPT = 16;     // keys per thread per radix tile
 constexpr int SORT_TILE = NT * SORT_IPT;
-// Keys per thread of a radix tile for key type K.  Two-chunk keys use the same 4096-record
-// tiles (96 KiB of LDS, one workgroup per CU): with their top bits mixed, every digit is
+// Keys per thread of a radix tile for key type K.  Two-chunk keys take 24 (6144-record
+// tiles, 144 KiB of LDS, one workgroup per CU): with their top bits mixed, every digit is
 // spread evenly, and longer digit runs per tile measured faster than smaller tiles at
-// higher occupancy (1.43 ms per scatter pass of 1e8 records against 1.82 ms with 3072-record
-// and 2.16 ms with 2048-record tiles).
-template <class K> struct sort_ipt { static constexpr int value = SORT_IPT; };
+// higher occupancy (per scatter pass of 1e8 records: 1.36 ms at 6144 records, 1.41 at
+// 5120, 1.43 at 4096, 1.82 at 3072, 2.16 at 2048).  One-chunk keys are fastest at 4096
+// (0.56 ms per pass of 7e7 records against 0.74 at 6144 and 0.71 at 8192).
+template <class K> struct sort_ipt { static constexpr int value = sizeof(K) == 8 ? SORT_IPT : 24; };
 template <class K> struct sort_tile { static constexpr int value = NT * sort_ipt<K>::value; };
 static_assert(RADIX == NT, "radix kernels use one thread per digit");
 
