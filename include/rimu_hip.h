@@ -211,6 +211,12 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
  * setting gives the same step results; fewer bits exercise the run ordering. */
 int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits);
 
+/* (No counterpart in the reference; for tests and tuning.)  Radix sort of the spawn records:
+ * 1 (default) takes every digit histogram in one pass over the keys and gives each scatter
+ * tile its offsets by a look-back over the tiles before it; 0 counts digits per pass and
+ * scans the counts between passes.  Both give the same order. */
+int rimu_engine_set_radix_mode(rimu_engine* e, int32_t mode);
+
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on
  * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled);

@@ -26,7 +26,7 @@ EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
     "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
-    "rimu_engine_set_sort_bits", "rimu_engine_set_timing",
+    "rimu_engine_set_sort_bits", "rimu_engine_set_radix_mode", "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
@@ -106,6 +106,7 @@ def lib():
         L.rimu_step.argtypes = [vp, i64, dbl, dbl, dbl, P(StepStats)]
         L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
         L.rimu_engine_set_sort_bits.argtypes = [vp, i32]
+        L.rimu_engine_set_radix_mode.argtypes = [vp, i32]
         L.rimu_overlaps.argtypes = [vp, vp, P(Operator), i32, P(dbl)]
         L.rimu_engine_set_timing.argtypes = [vp, i32]
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
@@ -300,6 +301,10 @@ class Engine:
     def set_sort_bits(self, bits: int):
         """Top bits of mixed keys that the radix passes cover (tests and tuning)."""
         check(lib().rimu_engine_set_sort_bits(self._h, int(bits)))
+
+    def set_radix_mode(self, mode: int):
+        """Radix sort of the spawn records: 1 look-back scatters (default), 0 per-pass counts."""
+        check(lib().rimu_engine_set_radix_mode(self._h, int(mode)))
 
     def set_initiator(self, rule: int, threshold: float = 1.0):
         """Initiator rule of the vector (INITIATOR_* codes) and its threshold."""

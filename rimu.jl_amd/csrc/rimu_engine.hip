@@ -80,6 +80,7 @@ struct rimu_engine {
     ModelDev hdev{};
     KeyMix mix{};                   // stored keys are mix(address) above MIX_MIN_BITS bits
     int sort_bits = MIX_SORT_BITS;  // top bits of mixed keys the radix passes cover
+    int radix_mode = 1;             // 1: one histogram pass + look-back scatters; 0: counts per pass
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
@@ -114,6 +115,9 @@ struct rimu_engine {
     DevBuf<double> ov_part, ov_out;
     DevBuf<int64_t> ov_ranges;
     DevBuf<int64_t> fix_runs;       // runs to reorder after a sort on the top key bits
+    DevBuf<uint64_t> lb_status;     // look-back radix: status words [pass][tile][digit]
+    uint32_t lb_epoch = 0;          // sort calls since lb_status was last cleared
+    DevBuf<uint32_t> lb_hist;       // look-back radix: digit bases [pass][digit], tile counters
     int64_t np = 0, nh = 0;
 
     // host staging
@@ -290,9 +294,13 @@ int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
 // (the initiator class bit).  Input in buffer `in`; returns the buffer index holding the
 // result.
 template <class K, class V>
+int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo);
+
+template <class K, class V>
 int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo = 0) {
     int src = in;
     if (n <= 1) { *result = src; return RIMU_OK; }
+    if (e->radix_mode == 1) return radix_sort_lb<K, V>(e, n, nbits, varying, in, result, lo);
     const int64_t tiles = ceil_div(n, sort_tile<K>::value);
     const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
@@ -313,6 +321,58 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* 
         e->timer.begin("radix_scatter", e->stream, &ev);
         radix_scatter_kernel<K, V><<<tiles, NT, 0, e->stream>>>(
             (const K*)e->skeys[src].p, (const V*)e->svals[src].p, n, shift, dmask, e->rcounts.p, tiles,
+            (K*)e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
+        e->timer.end(e->stream);
+        HIP_TRY(hipGetLastError());
+        src = 1 - src;
+    }
+    *result = src;
+    return RIMU_OK;
+}
+
+// The same sort with one histogram pass over the keys for every digit and one look-back
+// scatter per pass (no per-pass count or scan kernels).  The output is that of the per-pass
+// path: the ranks within a tile and the digit offsets are the same numbers.  FLAG_LOOKBACK
+// in the step's flags reports a look-back that did not complete.
+template <class K, class V>
+int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo) {
+    int src = in;
+    PassSet ps{};
+    for (int shift = lo; shift < nbits; shift += RADIX_BITS) {
+        if ((uint32_t)((varying >> shift) & (RADIX - 1)) == 0) continue;  // digit constant: pass is identity
+        if (ps.n == MAX_PASSES) return fail(RIMU_ERR_INTERNAL, "too many radix passes");
+        ps.shift[ps.n] = shift;
+        ps.dmask[ps.n] = shift + RADIX_BITS > nbits ? (1u << (nbits - shift)) - 1u : (uint32_t)(RADIX - 1);
+        ++ps.n;
+    }
+    if (ps.n == 0) { *result = src; return RIMU_OK; }
+    if (n > (int64_t)RLB_VALUE) return fail(RIMU_ERR_CAPACITY, "radix sort of 2^32 or more records");
+    const int64_t tiles = ceil_div(n, sort_tile<K>::value);
+    const size_t need = (size_t)ps.n * tiles * RADIX;
+    HIP_TRY(e->lb_hist.alloc((size_t)ps.n * RADIX + MAX_PASSES));
+    uint32_t* counters = e->lb_hist.p + (int64_t)ps.n * RADIX;  // tile ids (two-word keys)
+    int* flags = &((rimu_engine::Misc*)e->misc.p)->flags;
+    hipEvent_t ev{};
+    if (need > e->lb_status.n || e->lb_epoch >= RLB_EPOCH_MAX) {  // epoch 0: cleared words
+        HIP_TRY(e->lb_status.alloc(need));
+        HIP_TRY(hipMemsetAsync(e->lb_status.p, 0, sizeof(uint64_t) * e->lb_status.n, e->stream));
+        e->lb_epoch = 0;
+    }
+    const uint32_t epoch = ++e->lb_epoch;
+    e->timer.begin("radix_count", e->stream, &ev);
+    HIP_TRY(hipMemsetAsync(e->lb_hist.p, 0, sizeof(uint32_t) * (ps.n * RADIX + MAX_PASSES), e->stream));
+    const int copies = hist_all_copies(ps.n);
+    radix_hist_all_kernel<K><<<(int)std::min<int64_t>(tiles, 2048), NT,
+                               sizeof(uint32_t) * ps.n * copies * HIST_STRIDE, e->stream>>>(
+        (const K*)e->skeys[src].p, n, ps, copies, e->lb_hist.p);
+    radix_hist_base_kernel<<<ps.n, RADIX, 0, e->stream>>>(e->lb_hist.p);
+    e->timer.end(e->stream);
+    for (int p = 0; p < ps.n; ++p) {
+        e->timer.begin("radix_scatter", e->stream, &ev);
+        radix_scatter_lb_kernel<K, V><<<tiles, NT, 0, e->stream>>>(
+            (const K*)e->skeys[src].p, (const V*)e->svals[src].p, n, ps.shift[p], ps.dmask[p],
+            e->lb_hist.p + (int64_t)p * RADIX, e->lb_status.p + (int64_t)p * tiles * RADIX, epoch, counters + p,
+            flags,
             (K*)e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
@@ -558,6 +618,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->timer.collect();
+    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
     if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) {
         out->overflow = 1;
         return fail(RIMU_ERR_CAPACITY, "vector capacity (max_entries) exceeded");
@@ -919,6 +980,13 @@ int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits) {
     return RIMU_OK;
 }
 
+int rimu_engine_set_radix_mode(rimu_engine* e, int32_t mode) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (mode != 0 && mode != 1) return fail(RIMU_ERR_ARG, "radix mode must be 0 or 1");
+    e->radix_mode = mode;
+    return RIMU_OK;
+}
+
 int rimu_engine_set_timing(rimu_engine* e, int32_t enabled) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
     e->timer.enabled = enabled != 0;
@@ -988,11 +1056,11 @@ int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int6
     rimu_engine::Misc* dm = (rimu_engine::Misc*)e->misc.p;
     HIP_TRY(hipMemcpyAsync(e->skeys[0].p, keys, n * 8, hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->svals[0].p, vals, n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
     int sorted = 0;
     if (int rc = radix_sort<uint64_t, V>(e, n, nbits, varying, 0, &sorted)) return rc;
     StepArgs a = make_args(e, 0, 0.0, 1.0, 1.0);
     a.compression = 0;
-    HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
     int64_t tiles = 0;
     if (int rc = merge_into<uint64_t, V>(e, a, e->skeys[sorted].p, (const V*)e->svals[sorted].p, n, e->vkeys[0].p,
                                (const V*)e->dvals.p, 0, e->vkeys[1].p, (V*)e->vvals[1].p, e->max_entries,
@@ -1001,6 +1069,7 @@ int annihilate_impl(rimu_engine* e, const uint64_t* keys, const void* vals, int6
     HIP_TRY(hipStreamSynchronize(e->stream));
     const int64_t m = e->h_misc->L_new;
     *n_out = m;
+    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
     if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) return fail(RIMU_ERR_CAPACITY, "annihilation buffer too small");
     if (m > cap) return fail(RIMU_ERR_CAPACITY, "output capacity too small");
     if (m > 0) {

@@ -653,15 +653,22 @@ __global__ __launch_bounds__(1024) void key_bits_kernel(const K* __restrict__ ti
 // ---------------------------------------------------------------------------------------
 // dmask selects the digit bits of the pass: RADIX - 1, or fewer bits for a top digit that
 // would reach the initiator class bit.
+// Digit counts go to HIST_COPIES copies of the histogram, lane l adding to copy
+// l % HIST_COPIES: lanes of a wavefront with the same digit (the top digits of the keys take
+// few values) then hit different words, and the odd stride puts the copies of one digit in
+// different LDS banks.
+constexpr int HIST_COPIES = 8;
+constexpr int HIST_STRIDE = RADIX + 1;
 template <class K>
 __global__ __launch_bounds__(NT) void radix_count_kernel(const K* __restrict__ keys, int64_t n,
                                                          int shift, uint32_t dmask,
                                                          uint32_t* __restrict__ counts,
                                                          int64_t num_tiles) {
     constexpr int IPT = sort_ipt<K>::value;
-    __shared__ uint32_t hist[RADIX];
-    hist[threadIdx.x] = 0;
+    __shared__ uint32_t hist[HIST_COPIES * HIST_STRIDE];
+    for (int j = threadIdx.x; j < HIST_COPIES * HIST_STRIDE; j += NT) hist[j] = 0;
     __syncthreads();
+    uint32_t* mine = hist + (threadIdx.x % HIST_COPIES) * HIST_STRIDE;
     const int64_t base = (int64_t)blockIdx.x * sort_tile<K>::value;
     K k[IPT];
 #pragma unroll
@@ -672,10 +679,13 @@ __global__ __launch_bounds__(NT) void radix_count_kernel(const K* __restrict__ k
 #pragma unroll
     for (int r = 0; r < IPT; ++r) {
         const int64_t idx = base + (int64_t)r * NT + threadIdx.x;
-        if (idx < n) atomicAdd(&hist[(uint32_t)(k[r] >> shift) & dmask], 1u);
+        if (idx < n) atomicAdd(&mine[(uint32_t)(k[r] >> shift) & dmask], 1u);
     }
     __syncthreads();
-    counts[(int64_t)blockIdx.x * RADIX + threadIdx.x] = hist[threadIdx.x];  // tile-major row
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < HIST_COPIES; ++q) c += hist[q * HIST_STRIDE + threadIdx.x];
+    counts[(int64_t)blockIdx.x * RADIX + threadIdx.x] = c;  // tile-major row
 }
 
 // Offsets of every (tile, digit) for the stable scatter, from tile-major counts:
@@ -802,6 +812,214 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__
             s_k[pos] = k[r];
             s_v[pos] = v[r];
         }
+    }
+    __syncthreads();
+    const int64_t cnt_tile = min((int64_t)TILE, n - base);
+    for (int i = threadIdx.x; i < cnt_tile; i += NT) {
+        const K kk = s_k[i];
+        const uint32_t digit = (uint32_t)(kk >> shift) & dmask;
+        const int64_t g = (int64_t)s_goff[digit] + (i - (int64_t)s_start[digit]);
+        keys_out[g] = kk;
+        vals_out[g] = s_v[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Radix sort without per-pass count kernels.  One pass over the keys histograms every
+// digit the sort will use (radix_hist_all_kernel, radix_hist_base_kernel: the global
+// exclusive prefix over digits of each pass).  Each scatter tile then publishes its digit
+// counts and looks back over its predecessors' status words, one digit per thread, for the
+// count of each digit in the tiles before it (decoupled look-back).  A tile waits only for
+// tiles of lower index: with tile ids taken from a counter those have started before it;
+// with tile = workgroup index, the in-order workgroup dispatch of each XCD has started them.
+// Every wait is bounded by LB_SPIN_LIMIT, past which FLAG_LOOKBACK is raised and the step
+// fails.  Ranks within the tile and the scatter are those of radix_scatter_kernel, so the
+// output is the same.
+// ---------------------------------------------------------------------------------------
+constexpr int MAX_PASSES = 16;
+constexpr int LB_WIN = 8;   // predecessor status words read per look-back round
+// Status words of the radix look-back: flag (2 bits, as LB_FLAG_*), the epoch of the sort
+// call that wrote them (30 bits; words of other calls read as empty, so the status array is
+// cleared only when it grows or the epochs wrap), and a count (32 bits).
+constexpr uint32_t RLB_EPOCH_MAX = (1u << 30) - 1;
+constexpr uint64_t RLB_EPOCH_FIELD = (uint64_t)RLB_EPOCH_MAX << 32;
+constexpr uint64_t RLB_VALUE = 0xffffffffull;
+struct PassSet {
+    int n;
+    int shift[MAX_PASSES];
+    uint32_t dmask[MAX_PASSES];
+};
+
+// hist[p * RADIX + d] += count of digit d of pass p over the keys (grid-stride over tiles).
+// Dynamic LDS: ps.n * copies * HIST_STRIDE words (copies as in radix_count_kernel; the
+// host takes hist_all_copies(ps.n), which keeps the array within 64 KiB).
+__host__ __device__ inline int hist_all_copies(int npasses) { return npasses <= 8 ? 4 : 2; }
+template <class K>
+__global__ __launch_bounds__(NT) void radix_hist_all_kernel(const K* __restrict__ keys, int64_t n,
+                                                            PassSet ps, int copies, uint32_t* __restrict__ hist) {
+    constexpr int IPT = sort_ipt<K>::value;
+    constexpr int TILE = sort_tile<K>::value;
+    extern __shared__ uint32_t h_all[];
+    const int words = ps.n * copies * HIST_STRIDE;
+    for (int j = threadIdx.x; j < words; j += NT) h_all[j] = 0;
+    __syncthreads();
+    const int copy = threadIdx.x % copies;
+    const int64_t tiles = (n + TILE - 1) / TILE;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base = t * TILE;
+        K k[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) k[r] = keys[min(base + (int64_t)r * NT + threadIdx.x, n - 1)];
+        for (int p = 0; p < ps.n; ++p) {
+            const int shift = ps.shift[p];
+            const uint32_t dmask = ps.dmask[p];
+            uint32_t* mine = h_all + (p * copies + copy) * HIST_STRIDE;
+#pragma unroll
+            for (int r = 0; r < IPT; ++r)
+                if (base + (int64_t)r * NT + threadIdx.x < n)
+                    atomicAdd(&mine[(uint32_t)(k[r] >> shift) & dmask], 1u);
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < ps.n; ++p) {
+        uint32_t c = 0;
+        for (int q = 0; q < copies; ++q) c += h_all[(p * copies + q) * HIST_STRIDE + threadIdx.x];
+        if (c) atomicAdd(&hist[p * RADIX + threadIdx.x], c);
+    }
+}
+
+// In place: hist[p][d] -> sum over d' < d of hist[p][d'] (one workgroup per pass)
+__global__ __launch_bounds__(RADIX) void radix_hist_base_kernel(uint32_t* __restrict__ hist) {
+    uint32_t* h = hist + (int64_t)blockIdx.x * RADIX;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<uint32_t, RADIX>(h[threadIdx.x], &tot);
+    h[threadIdx.x] = ex;
+}
+
+template <class K, class V>
+__global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restrict__ keys_in,
+                                                              const V* __restrict__ vals_in, int64_t n,
+                                                              int shift, uint32_t dmask,
+                                                              const uint32_t* __restrict__ gbase,
+                                                              uint64_t* status, uint32_t epoch,
+                                                              uint32_t* tile_counter, int* flags,
+                                                              K* __restrict__ keys_out,
+                                                              V* __restrict__ vals_out) {
+    constexpr int NW = NT / WAVE;
+    constexpr int IPT = sort_ipt<K>::value;
+    constexpr int TILE = sort_tile<K>::value;
+    __shared__ K s_k[TILE];
+    __shared__ V s_v[TILE];
+    __shared__ uint32_t s_wcnt[NW][RADIX];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_run[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+
+    // one-word keys: tile = workgroup index; two-word keys (one workgroup per CU): tile ids in
+    // start order from a counter (measured faster for each)
+    const uint32_t tile = sizeof(K) == 16 ? next_tile_id(tile_counter) : blockIdx.x;
+    const uint64_t tag = (uint64_t)epoch << 32;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int64_t base = (int64_t)tile * TILE;
+    const int64_t wbase = base + (int64_t)wid * IPT * WAVE;
+
+    for (int j = threadIdx.x; j < NW * RADIX; j += NT) (&s_wcnt[0][0])[j] = 0;
+    __syncthreads();
+
+    K k[IPT];
+    V v[IPT];
+    uint32_t rank[IPT];
+    uint32_t* cnt = s_wcnt[wid];
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
+        const bool valid = idx < n;
+        k[r] = valid ? keys_in[idx] : (K)0;
+        v[r] = valid ? vals_in[idx] : (V)0;
+    }
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
+        const bool valid = idx < n;
+        const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; ++b) {
+            const uint64_t bb = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = valid ? cnt[digit] : 0u;
+        const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
+        rank[r] = before + in_wave;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && in_wave == 0) cnt[digit] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = s_wcnt[w][d];
+            s_wcnt[w][d] = run;
+            run += c;
+        }
+        uint32_t tot;
+        s_start[d] = block_exclusive_scan<uint32_t, NT>(run, &tot);
+        s_run[d] = run;
+        lb_store(&status[(int64_t)tile * RADIX + d], (tile == 0 ? LB_FLAG_PFX : LB_FLAG_AGG) | tag | run);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int64_t idx = wbase + (int64_t)r * WAVE + lane;
+        if (idx < n) {
+            const uint32_t digit = (uint32_t)(k[r] >> shift) & dmask;
+            const uint32_t pos = s_start[digit] + s_wcnt[wid][digit] + rank[r];
+            s_k[pos] = k[r];
+            s_v[pos] = v[r];
+        }
+    }
+    // look-back for digit d: the count of d in all tiles before this one.  Each round reads
+    // LB_WIN predecessors at once (the nearest one seldom holds its inclusive prefix yet) and
+    // adds them up to the nearest inclusive prefix.  A word of another epoch or an empty one
+    // ends the round and is read again.
+    {
+        const int d = threadIdx.x;
+        uint64_t pre = 0;
+        if (tile > 0) {
+            int64_t j = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (true) {
+                uint64_t w[LB_WIN];
+#pragma unroll
+                for (int q = 0; q < LB_WIN; ++q) {
+                    w[q] = j - q >= 0 ? lb_load(&status[(j - q) * RADIX + d]) : (LB_FLAG_PFX | tag);
+                }
+                int used = 0;
+                bool done = false, empty = false;
+#pragma unroll
+                for (int q = 0; q < LB_WIN; ++q) {
+                    if (done || empty) continue;
+                    const uint64_t f = w[q] >> 62;
+                    if (f == 0 || (w[q] & RLB_EPOCH_FIELD) != tag) { empty = true; continue; }
+                    pre += w[q] & RLB_VALUE;
+                    ++used;
+                    if (f == 2) done = true;
+                }
+                if (done) break;
+                j -= used;
+                if (empty) {
+                    if (++spins > LB_SPIN_LIMIT) { atomicOr(flags, FLAG_LOOKBACK); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            lb_store(&status[(int64_t)tile * RADIX + d], LB_FLAG_PFX | tag | (pre + s_run[d]));
+        }
+        s_goff[d] = gbase[d] + (uint32_t)pre;
     }
     __syncthreads();
     const int64_t cnt_tile = min((int64_t)TILE, n - base);

@@ -88,3 +88,83 @@ def test_annihilate_fifty_million_integer_records(gpu):
     keep = sums != 0
     assert np.array_equal(ok, uk[keep])
     assert np.array_equal(ov, sums[keep])
+
+
+def random_bose_keys(M, N, n, rng, chunk=250_000):
+    """About n distinct random BoseFS{N,M} addresses of N + M - 1 in 65..128 bits as (lo, hi)
+    word rows, ascending: any bit string with N ones and M - 1 zeros is one."""
+    B = N + M - 1
+    shifts = np.arange(64, dtype=np.uint64)
+    rows = []
+    for a in range(0, n, chunk):
+        m = min(chunk, n - a)
+        ones = np.argsort(rng.random((m, B)), axis=1)[:, :N]
+        bits = np.zeros((m, 128), dtype=np.uint64)
+        np.put_along_axis(bits, ones, np.uint64(1), axis=1)
+        lo = (bits[:, :64] << shifts).sum(axis=1, dtype=np.uint64)
+        hi = (bits[:, 64:] << shifts).sum(axis=1, dtype=np.uint64)
+        rows.append(np.stack([lo, hi], axis=1))
+    keys = np.unique(np.concatenate(rows), axis=0)      # rows in (lo, hi) lexicographic order
+    return keys[np.lexsort((keys[:, 0], keys[:, 1]))]
+
+
+def _step_both_modes(rimu, make_engine, keys, vals, steps, shift, dtau, sort_bits=None):
+    """The same state stepped by two engines, one per radix mode; returns both results."""
+    out = []
+    for mode in (0, 1):
+        eng = make_engine()
+        eng.set_radix_mode(mode)
+        if sort_bits is not None:
+            eng.set_sort_bits(sort_bits)
+        eng.upload(keys, vals)
+        stats = [eng.step(s, shift, dtau) for s in range(1, steps + 1)]
+        k, v = eng.download()
+        eng.close()
+        out.append((k, v, stats))
+    return out
+
+
+def _assert_same(a, b):
+    (k0, v0, s0), (k1, v1, s1) = a, b
+    assert k0.shape == k1.shape and np.array_equal(k0, k1)
+    assert np.array_equal(v0.view(np.uint64), v1.view(np.uint64))  # bit for bit
+    for x, y in zip(s0, s1):
+        for f in ("spawn_attempts", "spawns", "deaths", "clones", "zombies", "len", "norm"):
+            assert getattr(x, f) == getattr(y, f), f
+
+
+def test_radix_modes_agree_on_headline_state(gpu):
+    """The look-back radix sort (one histogram pass, tiles ordered by start) and the
+    per-pass count sort give the same steps, bit for bit, on the 1e8-walker state
+    (about 7e7 spawn records, 17000 radix tiles per pass)."""
+    eng, sp, st = grow_headline(gpu, 300)
+    keys, vals = eng.download()
+    eng.close()
+    assert len(keys) > 1e7
+    M = N = 20
+    H = gpu.HubbardReal1D(gpu.near_uniform(N, M), u=6.0, t=1.0)
+
+    def make():
+        return gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 91,
+                                  int(2e8 + 101), int(4e8 + 4096))
+
+    a, b = _step_both_modes(gpu, make, keys, vals, 2, sp.shift, 1e-3)
+    _assert_same(a, b)
+
+
+@pytest.mark.parametrize("sort_bits", [40, 128])
+def test_radix_modes_agree_on_two_word_keys(gpu, sort_bits):
+    """Both radix modes on 128-bit keys (HubbardReal1D M = N = 50, 3e6 entries): the top-bit
+    sort with its run ordering (40) and the full 16-pass sort (128)."""
+    rng = np.random.default_rng(5050 + sort_bits)
+    keys = random_bose_keys(50, 50, 3_000_000, rng)
+    vals = rng.normal(scale=6.0, size=len(keys))
+    H = gpu.HubbardReal1D(gpu.near_uniform(50, 50), u=4.0, t=1.0)
+
+    def make():
+        return gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 77,
+                                  40_000_000, 80_000_000)
+
+    a, b = _step_both_modes(gpu, make, keys, vals, 2, -2.0, 2e-3, sort_bits)
+    _assert_same(a, b)
+    assert len(a[0]) > 2_900_000
