@@ -118,6 +118,7 @@ struct rimu_engine {
     DevBuf<uint64_t> lb_status;     // look-back radix: status words [pass][tile][digit]
     uint32_t lb_epoch = 0;          // sort calls since lb_status was last cleared
     DevBuf<uint32_t> lb_hist;       // look-back radix: digit bases [pass][digit], tile counters
+    DevBuf<uint32_t> lb_part;       // look-back radix: digit counts [row][pass][digit]
     int64_t np = 0, nh = 0;
 
     // host staging
@@ -360,12 +361,14 @@ int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, in
     }
     const uint32_t epoch = ++e->lb_epoch;
     e->timer.begin("radix_count", e->stream, &ev);
-    HIP_TRY(hipMemsetAsync(e->lb_hist.p, 0, sizeof(uint32_t) * (ps.n * RADIX + MAX_PASSES), e->stream));
+    HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * MAX_PASSES, e->stream));
     const int copies = hist_all_copies(ps.n);
-    radix_hist_all_kernel<K><<<(int)std::min<int64_t>(tiles, 2048), NT,
-                               sizeof(uint32_t) * ps.n * copies * HIST_STRIDE, e->stream>>>(
-        (const K*)e->skeys[src].p, n, ps, copies, e->lb_hist.p);
-    radix_hist_base_kernel<<<ps.n, RADIX, 0, e->stream>>>(e->lb_hist.p);
+    const int hblocks = (int)std::min<int64_t>(tiles, HIST_ALL_BLOCKS);
+    HIP_TRY(e->lb_part.alloc((size_t)HIST_ROWS * ps.n * RADIX));
+    HIP_TRY(hipMemsetAsync(e->lb_part.p, 0, sizeof(uint32_t) * HIST_ROWS * ps.n * RADIX, e->stream));
+    radix_hist_all_kernel<K><<<hblocks, NT, sizeof(uint32_t) * ps.n * copies * HIST_STRIDE, e->stream>>>(
+        (const K*)e->skeys[src].p, n, ps, copies, e->lb_part.p);
+    radix_hist_base_kernel<<<ps.n, RADIX, 0, e->stream>>>(e->lb_part.p, ps.n, e->lb_hist.p);
     e->timer.end(e->stream);
     for (int p = 0; p < ps.n; ++p) {
         e->timer.begin("radix_scatter", e->stream, &ev);

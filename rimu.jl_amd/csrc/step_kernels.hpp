@@ -850,13 +850,17 @@ struct PassSet {
     uint32_t dmask[MAX_PASSES];
 };
 
-// hist[p * RADIX + d] += count of digit d of pass p over the keys (grid-stride over tiles).
+// part[(b % HIST_ROWS * ps.n + p) * RADIX + d] += count of digit d of pass p over the tiles
+// of workgroup b (grid-stride over tiles).  The workgroups add into HIST_ROWS rows, not
+// one: 2048 workgroups adding into the same 256 words per pass serialize on them.
 // Dynamic LDS: ps.n * copies * HIST_STRIDE words (copies as in radix_count_kernel; the
 // host takes hist_all_copies(ps.n), which keeps the array within 64 KiB).
 __host__ __device__ inline int hist_all_copies(int npasses) { return npasses <= 8 ? 4 : 2; }
+constexpr int HIST_ALL_BLOCKS = 2048;
+constexpr int HIST_ROWS = 64;
 template <class K>
 __global__ __launch_bounds__(NT) void radix_hist_all_kernel(const K* __restrict__ keys, int64_t n,
-                                                            PassSet ps, int copies, uint32_t* __restrict__ hist) {
+                                                            PassSet ps, int copies, uint32_t* __restrict__ part) {
     constexpr int IPT = sort_ipt<K>::value;
     constexpr int TILE = sort_tile<K>::value;
     extern __shared__ uint32_t h_all[];
@@ -884,16 +888,21 @@ __global__ __launch_bounds__(NT) void radix_hist_all_kernel(const K* __restrict_
     for (int p = 0; p < ps.n; ++p) {
         uint32_t c = 0;
         for (int q = 0; q < copies; ++q) c += h_all[(p * copies + q) * HIST_STRIDE + threadIdx.x];
-        if (c) atomicAdd(&hist[p * RADIX + threadIdx.x], c);
+        if (c) atomicAdd(&part[((int64_t)(blockIdx.x % HIST_ROWS) * ps.n + p) * RADIX + threadIdx.x], c);
     }
 }
 
-// In place: hist[p][d] -> sum over d' < d of hist[p][d'] (one workgroup per pass)
-__global__ __launch_bounds__(RADIX) void radix_hist_base_kernel(uint32_t* __restrict__ hist) {
-    uint32_t* h = hist + (int64_t)blockIdx.x * RADIX;
+// hist[p][d] = sum over d' < d of the digit counts of pass p, the counts summed over the
+// HIST_ROWS rows of part (one workgroup per pass, one thread per digit).
+__global__ __launch_bounds__(RADIX) void radix_hist_base_kernel(const uint32_t* __restrict__ part,
+                                                                int npasses, uint32_t* __restrict__ hist) {
+    const int p = blockIdx.x;
+    uint32_t c = 0;
+#pragma unroll 16
+    for (int b = 0; b < HIST_ROWS; ++b) c += part[((int64_t)b * npasses + p) * RADIX + threadIdx.x];
     uint32_t tot;
-    const uint32_t ex = block_exclusive_scan<uint32_t, RADIX>(h[threadIdx.x], &tot);
-    h[threadIdx.x] = ex;
+    const uint32_t ex = block_exclusive_scan<uint32_t, RADIX>(c, &tot);
+    hist[p * RADIX + threadIdx.x] = ex;
 }
 
 template <class K, class V>
