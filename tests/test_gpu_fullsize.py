@@ -168,3 +168,32 @@ def test_radix_modes_agree_on_two_word_keys(gpu, sort_bits):
     a, b = _step_both_modes(gpu, make, keys, vals, 2, -2.0, 2e-3, sort_bits)
     _assert_same(a, b)
     assert len(a[0]) > 2_900_000
+
+
+def test_radix_modes_agree_while_growing(gpu):
+    """Both radix modes from Rimu's default start (one address, population 10) through 300
+    steps of growth to 2e5 walkers: every step's statistics and the final state agree bit
+    for bit.  The look-back status array grows many times and is reused by sorts of other
+    sizes (words of earlier sorts carry older epochs)."""
+    M = N = 20
+    H = gpu.HubbardReal1D(gpu.near_uniform(N, M), u=6.0, t=1.0)
+    start = H.starting_address()
+    runs = []
+    for mode in (0, 1):
+        eng = gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 23, 400_101, 800_000)
+        eng.set_radix_mode(mode)
+        eng.upload(np.array([start.bits], dtype=np.uint64), np.array([10.0]))
+        strat = gpu.DoubleLogUpdate(target_walkers=2e5)
+        sp = gpu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, 1e-3)
+        stats = []
+        for step in range(1, 301):
+            st = eng.step(step, sp.shift, 1e-3)
+            strat.update(sp, st.norm)
+            stats.append((st.spawn_attempts, st.spawns, st.len, st.norm))
+        k, v = eng.download()
+        eng.close()
+        runs.append((k, v, stats))
+    (k0, v0, s0), (k1, v1, s1) = runs
+    assert s0 == s1
+    assert np.array_equal(k0, k1) and np.array_equal(v0.view(np.uint64), v1.view(np.uint64))
+    assert s0[-1][3] > 1e5
