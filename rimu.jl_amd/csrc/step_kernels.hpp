@@ -5,8 +5,10 @@
 //   spawn    : all spawn attempts, Philox-seeded, compacted in (entry, attempt) order
 //              with a workgroup scan and a decoupled look-back    (spawning.jl:174-243,
 //                                                                  :364-378)
-//   radix    : LSD radix sort of the spawn records on the address key, ranks staged in
-//              LDS, stable                                         (deposit!, dvec.jl:129-136)
+//   radix    : LSD radix sort of the spawn records on the address key, stable: one
+//              histogram pass for every digit, then per pass a scatter whose tiles rank in
+//              LDS and find their offsets by look-back (or, selectable, a count and scan
+//              per pass)                                           (deposit!, dvec.jl:129-136)
 //   merge    : merge-path union of the sorted spawns with the diagonal survivors, a
 //              sequential left fold per address (diagonal value first, then spawns in list
 //              order), ThresholdCompression, norm and length      (compression.jl:18-33)
