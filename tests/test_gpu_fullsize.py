@@ -197,3 +197,16 @@ def test_radix_modes_agree_while_growing(gpu):
     assert s0 == s1
     assert np.array_equal(k0, k1) and np.array_equal(v0.view(np.uint64), v1.view(np.uint64))
     assert s0[-1][3] > 1e5
+
+
+def test_radix_mode_rejects_unknown_values(gpu):
+    H = gpu.HubbardReal1D(gpu.near_uniform(4, 4), u=1.0, t=1.0)
+    eng = gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 1, 1000, 1000)
+    try:
+        for bad in (-1, 2, 7):
+            with pytest.raises(gpu.RimuError):
+                eng.set_radix_mode(bad)
+        eng.set_radix_mode(0)
+        eng.set_radix_mode(1)
+    finally:
+        eng.close()
