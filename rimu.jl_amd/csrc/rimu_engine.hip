@@ -460,6 +460,17 @@ int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int6
     return RIMU_OK;
 }
 
+// A HubbardRealSpace model has off-diagonal elements of value zero (which the reference
+// never deposits) when a component is fermionic (Pauli blocking) or a site has a hard wall
+// in some direction.
+bool rs_zero_channels(const ModelDev& h) {
+    if (h.fermi) return true;
+    for (int s = 0; s < h.M; ++s)
+        for (int d = 0; d < h.ndir; ++d)
+            if (h.nbr[s][d] == 255) return true;
+    return false;
+}
+
 template <class K, class V>
 int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
               rimu_step_stats* out) {
@@ -509,7 +520,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         // every attempt yields a record when values are floating and deposits unprojected,
         // unless the model has zero-valued channels (hard walls, Pauli blocking: the
         // reference never deposits their spawns)
-        const bool dense = !e->int_vals && e->style.proj_threshold == 0 && !rs;
+        const bool dense = !e->int_vals && e->style.proj_threshold == 0 && (!rs || !rs_zero_channels(e->hdev));
         if (dense && W > e->max_spawns) {
             out->overflow = 1;
             return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
@@ -532,7 +543,11 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
             sv = (V*)e->slot_v.p;
         }
         e->timer.begin("spawn", e->stream, &ev);
-        if (dense)
+        if (dense && rs)
+            spawn_kernel<K, V, true, true><<<spawn_tiles, NT, 0, e->stream>>>(
+                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
+                (K*)e->tile_or.p, (K*)e->tile_and.p);
+        else if (dense)
             spawn_kernel<K, V, true, false><<<spawn_tiles, NT, 0, e->stream>>>(
                 a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
                 (K*)e->tile_or.p, (K*)e->tile_and.p);

@@ -32,6 +32,7 @@ MODELS = {
     "hub2c_6x6": ("periodic", (6, 6), [("fermi", 12), ("fermi", 12)], [1.0, 1.0], [[0, 4], [4, 0]], None),   # 72 bits
     "hub2c_8x8": ("periodic", (8, 8), [("fermi", 20), ("fermi", 20)], [1.0, 1.0], [[0, 8], [8, 0]], None),   # 128 bits
     "bose_1d_100": ("hardwall", (64,), [("bose", 40)], [1.0], [[4]], None),                                     # 103 bits
+    "bose_8x8": ("periodic", (8, 8), [("bose", 64)], [1.0], [[6]], None),                                       # 127 bits
 }
 
 
@@ -125,6 +126,7 @@ CASES = [
     ("hub2c_8x8", "integer", 2000, 6, -20.0, 1e-3),
     ("hub2c_8x8", "dynamic", 2000, 10.0, -20.0, 1e-3),
     ("bose_1d_100", "dynamic", 2000, 10.0, -2.0, 2e-3),
+    ("bose_8x8", "dynamic", 2000, 10.0, -5.0, 1e-3),     # every spawn channel nonzero: dense records
 ]
 
 
