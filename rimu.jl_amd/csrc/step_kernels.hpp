@@ -23,9 +23,9 @@ namespace rimu {
 constexpr int NT = 256;          // workgroup size of every step kernel
 constexpr int MERGE_IPT = 8;     // merged positions per thread (one-chunk keys)
 // Merged positions per thread for key type K: two-chunk keys take 12 (3072-position tiles,
-// 75 KiB of LDS).  Measured at 1e8 walkers: the 128-bit merge takes 3.22 ms with 12
-// against 3.67 ms with 8 and 5.36 ms with 16; the 64-bit merge is fastest with 8 (1.09 ms
-// against 1.82 ms with 12).
+// 75 KiB of LDS).  Measured at 1e8 walkers: the 128-bit merge takes 3.16-3.22 ms with 12
+// against 3.67 ms with 8, 3.20 with 10, 3.17 with 14 and 5.36 ms with 16; the 64-bit merge
+// is fastest with 8 (1.09 ms against 1.82 ms with 12).
 template <class K> struct merge_ipt { static constexpr int value = sizeof(K) == 8 ? MERGE_IPT : 12; };
 constexpr int SPAWN_TILE_SIZE = 1024;  // attempts per spawn tile (NT * SPAWN_ITEMS)
 constexpr int RADIX_BITS = 8;
