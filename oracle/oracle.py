@@ -54,7 +54,10 @@ def lib():
     """Load ``liboracle.so`` (built by ``make -C oracle``)."""
     global _LIB
     if _LIB is None:
-        path = _HERE / "liboracle.so"
+        # RIMU_ORACLE_LIB selects another build of the same source (the sanitizer build of
+        # `make -C oracle asan`)
+        path = pathlib.Path(os.environ["RIMU_ORACLE_LIB"]).resolve() if os.environ.get("RIMU_ORACLE_LIB") \
+            else _HERE / "liboracle.so"
         if not path.exists():
             raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
         L = ctypes.CDLL(str(path))

@@ -167,4 +167,4 @@ def test_replica_g2_estimator(gpu):
     sim.close()
     for op, exact in (("Op1", 0.2207082), ("Op2", 0.9074473)):
         r = ST.rayleigh_replica_estimator(df, op_name=op, skip=skip)
-        assert abs(r.ratio - exact) < 3 * r.err + 2e-3, (op, r)
+        assert abs(r.pmedian() - exact) < 3 * r.err + 2e-3, (op, r)

@@ -135,7 +135,7 @@ def test_projected_energy_known_answer(gpu):
     df = gpu.solve(p).df
     pe = ST.projected_energy(df, skip=1000)
     se = ST.shift_estimator(df, skip=1000)
-    assert abs(pe.ratio - E0) < 3 * pe.err, pe
+    assert ST.within(pe, E0, n=3), pe
     assert abs(se.mean - E0) < 3 * se.err, se
 
 
@@ -155,10 +155,10 @@ def test_config1_energy_against_ed(gpu):
     df, _ = gpu.lomc(H, targetwalkers=1000, laststep=20_000, dtau=1e-3, random_seed=17,
                      post_step=gpu.ProjectedEnergy(H, address))
     pe = ST.projected_energy(df, skip=5000)
-    assert abs(pe.ratio - E0) < 2 * pe.err, pe
+    assert ST.within(pe, E0, n=2), pe
     df, _ = gpu.lomc(H, targetwalkers=10_000, laststep=20_000, dtau=1e-3, random_seed=17,
                      post_step=gpu.ProjectedEnergy(H, address))
     se = ST.shift_estimator(df, skip=5000)
     pe = ST.projected_energy(df, skip=5000)
     assert abs(se.mean - E0) < 2 * se.err, se
-    assert abs(pe.ratio - E0) < 2 * pe.err, pe
+    assert ST.within(pe, E0, n=2), pe

@@ -336,6 +336,89 @@ def test_spawning_unbiased_against_exact(model, onr, kind):
         assert acc.get(kk, 0.0) / n == pytest.approx(ev, rel=0.1, abs=0.02 * total_exact)
 
 
+def test_projected_deposits_unbiased():
+    """IsDynamicSemistochastic(late_compression=false) projects every deposit below the
+    threshold to +-threshold with probability |x|/threshold (projected_deposit!,
+    spawning.jl:13-31), so the mean over seeds equals the exact product.  With a huge
+    rel_spawning_threshold every column is exact, so the projection is the only noise."""
+    model = O.real1d(6, 6, 2.0, 1.0)
+    rng = np.random.default_rng(3)
+    basis = O.build_basis(6, 6)
+    keys = np.sort(rng.choice(basis, 60, replace=False))
+    vals = rng.uniform(-2.0, 2.0, size=60)
+    dk, dv, _ = O.step(model, O.style("deterministic"), keys, vals, seed=0, step=1,
+                       shift=0.3, dtau=0.05)
+    exact = dict(zip(dk.tolist(), dv.tolist()))
+    sty = O.style("dynamic", proj_threshold=1.0, rel_threshold=1e9, compression=0)
+    n = 2000
+    acc, acc2 = {}, {}
+    for seed in range(n):
+        k, v, st = O.step(model, sty, keys, vals, seed=seed, step=1, shift=0.3, dtau=0.05)
+        assert st["exact_steps"] == 60 and st["inexact_steps"] == 0
+        for kk, vv in zip(k.tolist(), v.tolist()):
+            acc[kk] = acc.get(kk, 0.0) + vv
+            acc2[kk] = acc2.get(kk, 0.0) + vv * vv
+    for kk, ev in exact.items():
+        mean = acc.get(kk, 0.0) / n
+        var = max(acc2.get(kk, 0.0) / n - mean * mean, 1e-12)
+        assert abs(mean - ev) < 5 * math.sqrt(var / n) + 1e-12, kk
+
+
+@pytest.mark.parametrize("val,rel,absth,boost,exact", [
+    # n_od = 12 for the six singly occupied modes of near_uniform(6, 6)
+    (12.0, 1.0, math.inf, 1.0, True),    # amount 12 >= min(Inf, 12)
+    (11.5, 1.0, math.inf, 1.0, False),
+    (6.5, 2.0, math.inf, 1.0, True),     # rel_threshold multiplies the walker number
+    (5.5, 2.0, math.inf, 1.0, False),
+    (3.5, 1.0, 3.0, 1.0, True),          # abs_threshold caps the threshold
+    (2.5, 1.0, 3.0, 1.0, False),
+    (6.5, 1.0, math.inf, 2.0, True),     # boost multiplies the amount (spawning.jl:366)
+    (1.6, 1.0, 3.0, 2.0, True),          # ... but not abs_threshold
+    (1.4, 1.0, 3.0, 2.0, False),
+])
+def test_dynamic_semistochastic_exact_rule(val, rel, absth, boost, exact):
+    """spawn!(DynamicSemistochastic) (spawning.jl:364-378) spawns exactly when
+    boost * |v| * rel_threshold >= min(abs_threshold, num_offdiagonals); the docstrings of
+    IsDynamicSemistochastic (styles.jl:158-164) and DynamicSemistochastic
+    (spawning.jl:343-349) state which thresholds boost scales.  An exact column equals the
+    deterministic product; a stochastic one draws max(floor(boost |v|), 1) attempts."""
+    model = O.real1d(6, 6, 2.0, 1.0)
+    k0 = O.from_onr((1, 1, 1, 1, 1, 1))
+    assert O.num_offdiagonals(model, k0) == 12
+    sty = O.style("dynamic", rel_threshold=rel, abs_threshold=absth, compression=0)
+    k, v, st = O.step(model, sty, [k0], [val], seed=9, step=1, shift=0.0, dtau=0.01, boost=boost)
+    dk, dv, _ = O.step(model, O.style("deterministic"), [k0], [val], seed=0, step=1,
+                       shift=0.0, dtau=0.01)
+    assert st["exact_steps"] == (1 if exact else 0)
+    assert st["inexact_steps"] == (0 if exact else 1)
+    if exact:
+        assert st["attempts"] == 12
+        assert np.array_equal(k, dk) and np.array_equal(v, dv)
+    else:
+        assert st["attempts"] == max(math.floor(boost * val), 1)
+
+
+def test_boost_unbiased_integer():
+    """boost multiplies the number of attempts of IsStochasticInteger
+    (spawn!(WithReplacement), spawning.jl:232-243) without changing the expected spawn."""
+    model = O.real1d(7, 10, 1.0)
+    k0 = O.from_onr((5, 0, 0, 5, 0, 0, 0))
+    dk, dv, _ = O.step(model, O.style("deterministic"), [k0], [7.0], seed=0, step=1,
+                       shift=0.5, dtau=0.01)
+    exact = dict(zip(dk.tolist(), dv.tolist()))
+    acc = {}
+    n = 3000
+    for seed in range(n):
+        k, v, st = O.step(model, O.style("integer"), [k0], [7], seed=seed, step=1, shift=0.5,
+                          dtau=0.01, boost=2.0)
+        assert st["attempts"] == 14
+        for kk, vv in zip(k.tolist(), v.tolist()):
+            acc[kk] = acc.get(kk, 0.0) + vv
+    total_exact = sum(abs(x) for x in exact.values())
+    for kk, ev in exact.items():
+        assert acc.get(kk, 0.0) / n == pytest.approx(ev, rel=0.1, abs=0.02 * total_exact)
+
+
 def test_threshold_compression_keeps_mean():
     model = O.real1d(6, 6, 1.0, 1.0)
     rng = np.random.default_rng(0)
@@ -425,6 +508,81 @@ def test_blocking_white_and_correlated_noise():
     bor = ST.blocking_analysis(np.ones(2000))
     assert bor.k == -1 and math.isnan(bor.err)
     assert ST.blocking_analysis([1.0]).k == -1
+
+
+def test_ratio_of_means_reference_cases():
+    """The set-ups of test/StatsTools.jl:65-153 with numpy draws: zero variance gives
+    (0, 0, 0); a ratio of two zero-mean series is Cauchy-like and its 95% interval holds 0;
+    a well-behaved correlated pair reproduces x_by_y_linear within 8/sqrt(n); blocking of a
+    smoothed pair finds k in 6..8 and keeps the quantiles."""
+    r = ST.ratio_of_means([0, 0, 0, 0, 0, 0], [1, 1, 1, 1, 1, 2])
+    assert ST.val_and_errs(r) == (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(17)
+    r = ST.ratio_of_means(rng.normal(size=2000), rng.normal(size=2000))
+    q95 = r.pquantile([0.025, 0.975])
+    assert q95[0] < 0 < q95[1]
+    # Measurements.jl's uncorrelated propagation: |f| sqrt((sa/ma)^2 + (sb/mb)^2)
+    f, sf = ST.x_by_y_linear(2.0, 3.0, 0.5, 0.1, 0.0)
+    assert f == 2.0 / 3.0
+    assert sf == pytest.approx(abs(f) * math.hypot(0.5 / 2.0, 0.1 / 3.0), rel=1e-14)
+    n = 2000
+    mu_a, mu_b, s_a, s_b, rho = 2.0, 3.0, 0.5, 0.1, 0.02
+    f, sf = ST.x_by_y_linear(mu_a, mu_b, s_a, s_b, rho)
+    rng = np.random.default_rng(1)  # a fixed seed, as the reference fixes its own
+    ab = rng.multivariate_normal([mu_a, mu_b], np.array([[s_a ** 2, rho], [rho, s_b ** 2]]) * n, size=n)
+    a, b = ab[:, 0], ab[:, 1]
+    r = ST.ratio_of_means(a, b)
+    assert 1 <= r.k <= 4 and r.success
+    assert r.f == a.mean() / b.mean()
+    assert abs(mu_a / mu_b - r.f) < 2 * sf
+    assert abs(mu_a / mu_b - r.pmedian()) < 2 * sf
+    q = r.pquantile([0.16, 0.5, 0.84])
+    assert q[2] - q[1] == pytest.approx(sf, rel=8 / math.sqrt(n))
+    assert q[1] - q[0] == pytest.approx(sf, rel=8 / math.sqrt(n))
+    assert r.sigma_f == pytest.approx(sf, rel=8 / math.sqrt(n))
+    assert abs(r.delta_y) < 0.1
+    rs = ST.ratio_of_means(ST.smoothen(a, 2 ** 3), ST.smoothen(b, 2 ** 6))
+    assert 6 <= rs.k <= 8
+    np.testing.assert_allclose(rs.pquantile([0.16, 0.5, 0.84]), q, atol=2 * sf)
+    assert abs(rs.sigma_f - sf) < 2 * sf
+
+
+def test_ratio_particles_follow_the_ratio_of_normals():
+    """The Monte Carlo ratio of a correlated normal pair against its exact distribution
+    (Hinkley 1969: the CDF of x/y for bivariate normal (x, y), by numerical integration of
+    P(x - z y <= 0) over y), at a noisy denominator (delta_y = 0.25) where linear
+    propagation fails.  A dropped covariance term or swapped variances moves the quantiles
+    by several times the tolerance."""
+    from scipy import integrate, stats
+    mx, my, sx, sy, c = 1.0, 2.0, 0.3, 0.5, 0.06
+    rng = np.random.default_rng(5)
+    xy = ST._particles_2d([mx, my], [[sx ** 2, c], [c, sy ** 2]], 400_000, rng)
+    z = ST.Particles(xy[0] / xy[1])
+    rho = c / (sx * sy)
+
+    def cdf(t):
+        # P(x/y <= t) = int over y of p(y) P(x <= t y | y) for y > 0, P(x >= t y | y) for y < 0
+        def integrand(y):
+            m = mx + rho * sx / sy * (y - my)
+            s = sx * math.sqrt(1 - rho ** 2)
+            p = stats.norm.cdf(t * y, loc=m, scale=s)
+            return stats.norm.pdf(y, loc=my, scale=sy) * (p if y > 0 else 1 - p)
+        return (integrate.quad(integrand, my - 12 * sy, 0, limit=200)[0]
+                + integrate.quad(integrand, 0, my + 12 * sy, limit=200)[0])
+
+    for q in (0.025, 0.16, 0.5, 0.84, 0.975):
+        assert cdf(float(z.pquantile(q))) == pytest.approx(q, abs=0.004), q
+    # delta_y = standard error of the denominator's mean over that mean
+    _, _, _, dy, _ = ST.ratio_estimators(np.array([mx + sx, mx - sx]), np.array([my + sy, my - sy]))
+    assert dy == pytest.approx(sy / my, rel=1e-12)
+
+
+def test_val_and_errs_of_a_blocking_result():
+    """convenience.jl:61-112: a BlockingResult gives (mean, n*err, n*err)."""
+    br = ST.BlockingResult(mean=1.5, err=0.25, err_err=0.0, p_cov=0.0, k=2, blocks=100)
+    v, lo, hi = ST.val_and_errs(br, n=2)
+    assert v == 1.5 and lo == pytest.approx(0.5, rel=1e-12) and hi == pytest.approx(0.5, rel=1e-12)
+    assert ST.within(br, 1.99) and not ST.within(br, 2.01)
 
 
 def test_mtest_quantiles_match_chi2_table():
