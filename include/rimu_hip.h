@@ -234,6 +234,16 @@ int rimu_engine_last_traffic(rimu_engine* e, double* alg_bytes, double* records)
 int rimu_annihilate(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
                     uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out);
 
+/* rimu_annihilate without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc).  Same arguments, result and errors as rimu_annihilate, for at most 2^20
+ * records (larger lists return RIMU_ERR_CAPACITY).  Its working memory is one
+ * library-owned device array of about 37 MiB per GPU, held until the process ends; it calls
+ * no hipMalloc/hipFree.  Calls that overlap on one GPU share that array and race: the caller
+ * serialises them. */
+int rimu_annihilate_noalloc(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
+                            uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out);
+
 /* Hamiltonian interface on the device for a batch of addresses (for tests and
  * ProjectedEnergy set-up): diagonal_element and num_offdiagonals
  * (Interfaces/hamiltonians.jl), and get_offdiagonal for one chosen index per address.

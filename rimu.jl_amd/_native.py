@@ -28,7 +28,7 @@ EXPORTS = (
     "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
     "rimu_engine_set_sort_bits", "rimu_engine_set_radix_mode", "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
-    "rimu_engine_last_traffic", "rimu_annihilate", "rimu_hamiltonian_eval",
+    "rimu_engine_last_traffic", "rimu_annihilate", "rimu_annihilate_noalloc", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
     "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
     "rimu_dense_ground_state",
@@ -112,6 +112,7 @@ def lib():
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
         L.rimu_engine_last_traffic.argtypes = [vp, P(dbl), P(dbl)]
         L.rimu_annihilate.argtypes = [i32, vp, vp, i64, vp, vp, i64, P(i64)]
+        L.rimu_annihilate_noalloc.argtypes = [i32, vp, vp, i64, vp, vp, i64, P(i64)]
         L.rimu_hamiltonian_eval.argtypes = [P(Model), vp, vp, i64, vp, vp, vp, vp]
         L.rimu_basis_size.argtypes = [i32, i32, P(i64)]
         L.rimu_dense_create.argtypes = [P(Model), i32, P(vp)]
@@ -155,8 +156,10 @@ def _p(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def annihilate(keys, vals):
-    """Device annihilation of a spawn list (``rimu_annihilate``)."""
+def annihilate(keys, vals, noalloc: bool = False):
+    """Device annihilation of a spawn list (``rimu_annihilate``; with ``noalloc``,
+    ``rimu_annihilate_noalloc``, which takes at most 2^20 records and allocates no device
+    memory)."""
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
     vals = np.ascontiguousarray(vals)
     vtype = 0 if vals.dtype.kind in "iu" else 1
@@ -165,8 +168,8 @@ def annihilate(keys, vals):
     ok = np.zeros(max(n, 1), dtype=np.uint64)
     ov = np.zeros(max(n, 1), dtype=vals.dtype)
     nout = ctypes.c_int64(0)
-    check(lib().rimu_annihilate(vtype, _p(keys), _p(vals), n, _p(ok), _p(ov), max(n, 1),
-                                ctypes.byref(nout)))
+    fn = lib().rimu_annihilate_noalloc if noalloc else lib().rimu_annihilate
+    check(fn(vtype, _p(keys), _p(vals), n, _p(ok), _p(ov), max(n, 1), ctypes.byref(nout)))
     m = nout.value
     return ok[:m].copy(), ov[:m].copy()
 

@@ -25,21 +25,27 @@ inline int fail(int code, const std::string& msg) {
             return ::rimu::fail(RIMU_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+// A device buffer that owns its memory, or (after borrow) views `count` elements of memory
+// owned elsewhere: a borrowed buffer never allocates or frees, and asking it to grow past
+// its size fails with hipErrorOutOfMemory.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool borrowed = false;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), borrowed(o.borrowed) { o.p = nullptr; o.n = 0; o.borrowed = false; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            borrowed = o.borrowed;
             o.p = nullptr;
             o.n = 0;
+            o.borrowed = false;
         }
         return *this;
     }
@@ -48,6 +54,7 @@ struct DevBuf {
     // asked, so sizes that creep upward from step to step do not reallocate every step.
     hipError_t alloc(size_t count) {
         if (count <= n) return hipSuccess;
+        if (borrowed) return hipErrorOutOfMemory;
         const size_t want = n == 0 ? count : std::max(count, count + count / 4);
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -56,10 +63,17 @@ struct DevBuf {
         if (e == hipSuccess) n = want;
         return e;
     }
+    void borrow(T* q, size_t count) {
+        release();
+        p = q;
+        n = count;
+        borrowed = true;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        borrowed = false;
     }
 };
 

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1175,6 +1176,146 @@ int rimu_hamiltonian_eval(const rimu_model* model, const uint64_t* addrs, const 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Annihilation without device allocation (rimu_annihilate_noalloc): the sort and merge of
+// rimu_annihilate on buffers carved from one file-scope device array.  Layout, in records
+// of NOALLOC_MAX (8-byte keys, 8-byte values) and the sort/merge bookkeeping for that many:
+//   keys[2], vals[2]   ping-pong buffers of the sort; after the sort the free pair holds the
+//                      merge's per-tile slots and the sorted pair receives the result
+//   status             look-back status words, passes x tiles x RADIX (cleared every call)
+//   part, hist         digit counts and bases of the histogram pass
+//   merge bookkeeping  partial sums, splits, preceding keys, tile counts, scan sums, flags
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int64_t NOALLOC_MAX = 1 << 20;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr int NOALLOC_PASSES = 8;  // 8-bit digits of a 64-bit key
+constexpr int64_t NOALLOC_SORT_TILES = (NOALLOC_MAX + 4095) / 4096;
+constexpr int64_t NOALLOC_MERGE_TILES = (NOALLOC_MAX + NT * MERGE_IPT - 1) / (NT * MERGE_IPT);
+constexpr size_t NA_REC = align256(NOALLOC_MAX * 8);
+constexpr size_t NA_KEYS0 = 0, NA_KEYS1 = NA_REC, NA_VALS0 = 2 * NA_REC, NA_VALS1 = 3 * NA_REC;
+constexpr size_t NA_STATUS = 4 * NA_REC;
+constexpr size_t NA_STATUS_BYTES = align256((size_t)NOALLOC_PASSES * NOALLOC_SORT_TILES * RADIX * 8);
+constexpr size_t NA_PART = NA_STATUS + NA_STATUS_BYTES;
+constexpr size_t NA_PART_BYTES = align256((size_t)HIST_ROWS * NOALLOC_PASSES * RADIX * 4);
+constexpr size_t NA_HIST = NA_PART + NA_PART_BYTES;
+constexpr size_t NA_HIST_BYTES = align256((size_t)(NOALLOC_PASSES * RADIX + MAX_PASSES) * 4);
+constexpr size_t NA_MERGE_P = NA_HIST + NA_HIST_BYTES;
+constexpr size_t NA_MERGE_P_BYTES = align256((size_t)NOALLOC_MERGE_TILES * 3 * 8);
+constexpr size_t NA_SPLITS = NA_MERGE_P + NA_MERGE_P_BYTES;
+constexpr size_t NA_TILE_BYTES = align256((size_t)(NOALLOC_MERGE_TILES + 1) * 8);
+constexpr size_t NA_PREVK = NA_SPLITS + NA_TILE_BYTES;
+constexpr size_t NA_TILE_CNT = NA_PREVK + NA_TILE_BYTES;
+constexpr size_t NA_SCAN = NA_TILE_CNT + NA_TILE_BYTES;
+constexpr size_t NA_SCAN_BYTES = align256((size_t)(NOALLOC_MERGE_TILES / SCAN_TILE + 2) * 8);
+constexpr size_t NA_MISC = NA_SCAN + NA_SCAN_BYTES;
+constexpr size_t NA_BYTES = NA_MISC + align256(sizeof(rimu_engine::Misc));
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.  Host
+// code uses only this address: the array's name in host code is not a device address.
+int scratch_address(unsigned char** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(RIMU_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return fail(RIMU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+        g_scratch[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch[dev];
+    return RIMU_OK;
+}
+
+template <class V>
+int annihilate_noalloc_impl(unsigned char* base, const uint64_t* keys, const void* vals, int64_t n, int nbits,
+                            uint64_t varying, uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out) {
+    rimu_engine e;  // views of the scratch array only; its stream is the null stream
+    e.stream = nullptr;
+    e.max_entries = NOALLOC_MAX;
+    e.max_spawns = NOALLOC_MAX;
+    e.skeys[0].borrow((uint64_t*)(base + NA_KEYS0), NOALLOC_MAX);
+    e.skeys[1].borrow((uint64_t*)(base + NA_KEYS1), NOALLOC_MAX);
+    e.svals[0].borrow(base + NA_VALS0, NOALLOC_MAX * 8);
+    e.svals[1].borrow(base + NA_VALS1, NOALLOC_MAX * 8);
+    e.lb_status.borrow((uint64_t*)(base + NA_STATUS), NA_STATUS_BYTES / 8);
+    e.lb_part.borrow((uint32_t*)(base + NA_PART), NA_PART_BYTES / 4);
+    e.lb_hist.borrow((uint32_t*)(base + NA_HIST), NA_HIST_BYTES / 4);
+    e.merge_p.borrow((double*)(base + NA_MERGE_P), NA_MERGE_P_BYTES / 8);
+    e.splits.borrow((int64_t*)(base + NA_SPLITS), NA_TILE_BYTES / 8);
+    e.prevk.borrow((uint64_t*)(base + NA_PREVK), NA_TILE_BYTES / 8);
+    e.tile_cnt.borrow((int64_t*)(base + NA_TILE_CNT), NA_TILE_BYTES / 8);
+    e.scan_tmp.borrow((int64_t*)(base + NA_SCAN), NA_SCAN_BYTES / 8);
+    e.misc.borrow(base + NA_MISC, sizeof(rimu_engine::Misc));
+    rimu_engine::Misc host_misc{};
+    e.h_misc = &host_misc;
+    rimu_engine::Misc* dm = (rimu_engine::Misc*)e.misc.p;
+
+    HIP_TRY(hipMemcpy(e.skeys[0].p, keys, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e.svals[0].p, vals, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e.stream));
+    // status words of an earlier call carry the same epoch as this call's: clear them
+    HIP_TRY(hipMemsetAsync(e.lb_status.p, 0, NA_STATUS_BYTES, e.stream));
+    e.lb_epoch = 0;
+    int sorted = 0;
+    if (int rc = radix_sort<uint64_t, V>(&e, n, nbits, varying, 0, &sorted)) return rc;
+    // the free pair of sort buffers holds the merge's tile slots; the result replaces the
+    // sorted records once the merge has read them (compaction runs after the merge)
+    e.slot_k.borrow(e.skeys[1 - sorted].p, NOALLOC_MAX);
+    e.slot_v.borrow((uint64_t*)e.svals[1 - sorted].p, NOALLOC_MAX);
+    StepArgs a = make_args(&e, 0, 0.0, 1.0, 1.0);
+    a.compression = 0;
+    int64_t tiles = 0;
+    uint64_t* ok = e.skeys[sorted].p;
+    V* ov = (V*)e.svals[sorted].p;
+    if (int rc = merge_into<uint64_t, V>(&e, a, e.skeys[sorted].p, (const V*)e.svals[sorted].p, n, ok, ov, 0, ok,
+                                         ov, NOALLOC_MAX, dm, &tiles)) return rc;
+    HIP_TRY(hipMemcpyAsync(e.h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e.stream));
+    HIP_TRY(hipStreamSynchronize(e.stream));
+    const int64_t m = host_misc.L_new;
+    *n_out = m;
+    if (host_misc.flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
+    if (host_misc.flags & FLAG_VECTOR_OVERFLOW) return fail(RIMU_ERR_INTERNAL, "annihilation scratch overflow");
+    if (m > cap) return fail(RIMU_ERR_CAPACITY, "output capacity too small");
+    if (m > 0) {
+        HIP_TRY(hipMemcpy(out_keys, ok, m * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(out_vals, ov, m * 8, hipMemcpyDeviceToHost));
+    }
+    return RIMU_OK;
+}
+}  // namespace
+
+extern "C" int rimu_annihilate_noalloc(int32_t vtype, const uint64_t* keys, const void* vals, int64_t n,
+                                       uint64_t* out_keys, void* out_vals, int64_t cap, int64_t* n_out) {
+    if (!n_out) return fail(RIMU_ERR_ARG, "n_out is null");
+    *n_out = 0;
+    if (vtype != 0 && vtype != 1) return fail(RIMU_ERR_ARG, "vtype must be 0 (int64) or 1 (double)");
+    if (n < 0 || (n > 0 && (!keys || !vals))) return fail(RIMU_ERR_ARG, "bad list arguments");
+    if (n > NOALLOC_MAX)
+        return fail(RIMU_ERR_CAPACITY, "rimu_annihilate_noalloc takes at most 2^20 records; use rimu_annihilate");
+    if (n == 0) return RIMU_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(RIMU_ERR_NODEVICE, "no HIP device");
+    unsigned char* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    uint64_t kor = 0, kand = ~0ull;
+    for (int64_t i = 0; i < n; ++i) { kor |= keys[i]; kand &= keys[i]; }
+    const int nbits = kor ? 64 - __builtin_clzll(kor) : 1;
+    return vtype == 0
+        ? annihilate_noalloc_impl<int64_t>(base, keys, vals, n, nbits, kor ^ kand, out_keys, out_vals, cap, n_out)
+        : annihilate_noalloc_impl<double>(base, keys, vals, n, nbits, kor ^ kand, out_keys, out_vals, cap, n_out);
+}
 
 extern "C" int rimu_engine_reserve(rimu_engine* e, int64_t max_entries, int64_t max_spawns) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
