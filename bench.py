@@ -83,7 +83,7 @@ def kernel_alg_bytes(name, S, L, Lnew, vsize=8, ksize=8):
         return 2 * S * rec
     if name == "radix_count":
         return S * ksize
-    if name == "merge":
+    if name in ("merge", "bucket_merge"):
         return S * rec + L * (ksize + vsize) + Lnew * rec
     if name == "compact":
         return 2 * Lnew * rec
@@ -259,24 +259,63 @@ def run_momentum_leg(rimu, args):
             break
     log(f"momentum: grown in {step} steps ({time.perf_counter() - t0:.1f} s), len {st.len:.3g} norm {st.norm:.3g}")
     eng.set_timing(True)
-    attempts = 0.0
+    attempts = alg_bytes = 0.0
+    S_tot = L_tot = Lnew_tot = 0.0
+    L_prev = len(eng)
     t1 = time.perf_counter()
     for _ in range(args.mom_steps):
         step += 1
         st = eng.step(step, sp.shift, dtau)
         strat.update(sp, st.norm)
         attempts += st.spawn_attempts
+        b, records = eng.last_traffic()
+        alg_bytes += b
+        S_tot += records - L_prev
+        L_tot += L_prev
+        Lnew_tot += st.len
+        L_prev = st.len
     elapsed = time.perf_counter() - t1
     times = eng.kernel_times()
     eng.close()
-    kern = {n: {"ms_avg": ms / max(c, 1), "launches_per_step": c / args.mom_steps}
-            for n, (ms, c) in times.items()}
+    K = args.mom_steps
+    roofline, kern = kernel_rooflines(times, S_tot / K, L_tot / K, Lnew_tot / K, K)
     return {"workload": "HubbardMom1D M=N=16 u=4 t=1, condensate start, IsStochasticInteger, "
                         "target_walkers=2e7, dtau=5e-4, seed 17",
             "metric": "spawn_attempts/s", "value": attempts / elapsed,
-            "ms_per_step": 1e3 * elapsed / args.mom_steps, "steps": args.mom_steps,
-            "growth_steps": step - args.mom_steps, "walkers": st.norm, "len": st.len,
-            "shift": sp.shift, "kernels": kern}
+            "ms_per_step": 1e3 * elapsed / K, "steps": K,
+            "growth_steps": step - K, "walkers": st.norm, "len": st.len,
+            "spawn_records_per_step": S_tot / K,
+            "step_alg_roofline_frac": alg_bytes / elapsed / HBM_PEAK,
+            "roofline": roofline, "shift": sp.shift, "kernels": kern}
+
+
+def kernel_rooflines(times, S, L, Lnew, steps, traffic_ratio=None):
+    """Per-kernel algorithmic GB/s from the engine's HIP-event times (S records, L entries in,
+    Lnew out per step), and the roofline of the kernel with the most device time among those
+    with an algorithmic byte count."""
+    kernels = {}
+    for name, (ms, launches) in times.items():
+        kernels[name] = {"ms_total": ms, "launches": launches, "ms_avg": ms / max(launches, 1),
+                         "launches_per_step": launches / steps}
+        bpl = kernel_alg_bytes(name, S, L, Lnew)
+        if bpl is not None:
+            kernels[name]["alg_GBps"] = bpl / (kernels[name]["ms_avg"] / 1e3) / 1e9
+    known = [n for n in times if kernel_alg_bytes(n, 1, 1, 1) is not None]
+    if not known:
+        return None, kernels
+    dom = max(known, key=lambda n: times[n][0])
+    ms, launches = times[dom]
+    bytes_per_launch = kernel_alg_bytes(dom, S, L, Lnew)
+    avg_s = ms / launches / 1e3
+    achieved = bytes_per_launch / avg_s
+    traffic = None
+    if traffic_ratio and dom in traffic_ratio:
+        traffic = traffic_ratio[dom] * bytes_per_launch
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
+                "launches_per_step": launches / steps}
+    return roofline, kernels
 
 
 def run_checkpoint_leg(rimu, eng, M, N, step, shift):
@@ -760,8 +799,9 @@ def main():
     ap.add_argument("--rs-steps", type=int, default=10)
     ap.add_argument("--rs-cpu-seconds", type=float, default=10.0)
     ap.add_argument("--rs-hubbard-steps", type=int, default=2000)
-    ap.add_argument("--radix-mode", type=int, default=1, choices=(0, 1),
-                    help="spawn record sort: 1 look-back scatters, 0 per-pass digit counts")
+    ap.add_argument("--radix-mode", type=int, default=2, choices=(0, 1, 2),
+                    help="spawn record sort and merge: 2 top-bit sort and bucket merge, 1 look-back "
+                         "scatters over every bit and the merge path, 0 per-pass digit counts")
     ap.add_argument("--state-file", default=None,
                     help="resume from this equilibrated state if it exists, else save it there")
     args = ap.parse_args()
@@ -852,26 +892,13 @@ def main():
     roofline = None
     kernels = {}
     if times:
-        for name, (ms, launches) in times.items():
-            kernels[name] = {"ms_total": ms, "launches": launches, "ms_avg": ms / max(launches, 1)}
-        dom = max((n for n in times if kernel_alg_bytes(n, 1, 1, 1) is not None),
-                  key=lambda n: times[n][0])
-        ms, launches = times[dom]
-        per_step_launches = launches / K
-        bytes_per_launch = kernel_alg_bytes(dom, S_tot / K, L_tot / K, Lnew_tot / K)
-        avg_s = ms / launches / 1e3
-        achieved = bytes_per_launch / avg_s
-        traffic = None
-        if dom in MEASURED_TRAFFIC_RATIO and args.target == 1e8:
-            traffic = MEASURED_TRAFFIC_RATIO[dom] * bytes_per_launch
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                    "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
-                    "launches_per_step": per_step_launches}
-        for name in kernels:
-            bpl = kernel_alg_bytes(name, S_tot / K, L_tot / K, Lnew_tot / K)
-            if bpl is not None:
-                kernels[name]["alg_GBps"] = bpl / (kernels[name]["ms_avg"] / 1e3) / 1e9
+        roofline, kernels = kernel_rooflines(times, S_tot / K, L_tot / K, Lnew_tot / K, K,
+                                             MEASURED_TRAFFIC_RATIO if args.target == 1e8 else None)
+        if roofline is not None:
+            roofline["traffic_source"] = (
+                "algorithmic bytes x the FETCH_SIZE + WRITE_SIZE ratio measured with rocprofv3 on "
+                "this workload (profiles/hbm_traffic_real20_dynamic_1e8.md)"
+                if roofline["traffic"] is not None else None)
 
     checkpoint = None
     if not args.no_checkpoint:

@@ -211,11 +211,22 @@ int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* op
  * setting gives the same step results; fewer bits exercise the run ordering. */
 int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits);
 
-/* (No counterpart in the reference; for tests and tuning.)  Radix sort of the spawn records:
- * 1 (default) takes every digit histogram in one pass over the keys and gives each scatter
- * tile its offsets by a look-back over the tiles before it; 0 counts digits per pass and
- * scans the counts between passes.  Both give the same order. */
+/* (No counterpart in the reference; for tests and tuning.)  Sort and merge of the spawn
+ * records.  2 (default): where eligible (one-word keys of more than 20 bits, no initiator
+ * rule, enough records), sort on the top bits of the stored keys only and merge bucket by
+ * bucket, each bucket's records sorted and folded in LDS; other steps as 1.  1: one histogram
+ * pass for every digit and scatters that find their offsets by a look-back over the tiles
+ * before them, then the merge path.  0: as 1 with counts per pass and a scan between passes.
+ * All give the same step results. */
 int rimu_engine_set_radix_mode(rimu_engine* e, int32_t mode);
+
+/* (No counterpart in the reference; for tests.)  Every look-back wait of the step kernels
+ * (the plan's attempt offsets, the radix scatters' digit offsets) is bounded: after
+ * spin_limit rounds (default 2^25) the waiting tile gives up, and the step fails with
+ * RIMU_ERR_INTERNAL, leaving the vector unchanged.  withhold_first = 1 makes the first tile
+ * of every radix pass keep its counts to itself, so that every later tile's wait runs into
+ * the bound: a host-forced test of the give-up path. */
+int rimu_engine_set_lookback(rimu_engine* e, uint32_t spin_limit, int32_t withhold_first);
 
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on
  * the engine's stream when timing is enabled.  names/ms/launches hold up to cap kernels. */

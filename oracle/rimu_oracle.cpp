@@ -775,13 +775,14 @@ void compress(const Params& p, Map& m) {
     }
 }
 
-// The GPU engine's list order.  Above 40 address bits the engine stores each address x as
+// The GPU engine's list order.  Above 20 address bits the engine stores each address x as
 // y = mix(x), a bijection of [0, 2^B) (multiply by an odd constant mod 2^B, then fold the
 // high half down with y ^= y >> ceil(B/2), twice), keeps its vector in ascending y and emits
-// deposits entry by entry in that order.  order = 1 replays this list; at or below 40 bits
+// deposits entry by entry in that order.  order = 1 replays this list; at or below 20 bits
 // y = x (ascending addresses).
+constexpr int ENGINE_MIX_MIN_BITS = 20;
 template <class A> A engine_order_key(int B, A x) {
-    if (B <= 40) return x;
+    if (B <= ENGINE_MIX_MIN_BITS) return x;
     const u128 c1 = ((u128)0x9E3779B97F4A7C15ull << 64) | 0xF39CC0605CEDC835ull;
     const u128 c2 = ((u128)0xD1B54A32D192ED03ull << 64) | 0xABC98388FB8FAC03ull;
     const int s = (B + 1) / 2;
@@ -799,7 +800,7 @@ template <class A> std::vector<int64_t> engine_entry_order(const Params& p, cons
     std::vector<int64_t> idx((size_t)n);
     for (int64_t i = 0; i < n; ++i) idx[i] = i;
     const int B = model_bits(p.h);
-    if (B > 40) {
+    if (B > ENGINE_MIX_MIN_BITS) {
         std::vector<A> y((size_t)n);
         for (int64_t i = 0; i < n; ++i) y[i] = engine_order_key<A>(B, load_key<A>(keys, i));
         std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return y[a] < y[b]; });

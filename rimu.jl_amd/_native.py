@@ -26,7 +26,8 @@ EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
     "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
-    "rimu_engine_set_sort_bits", "rimu_engine_set_radix_mode", "rimu_engine_set_timing",
+    "rimu_engine_set_sort_bits", "rimu_engine_set_radix_mode", "rimu_engine_set_lookback",
+    "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
     "rimu_engine_last_traffic", "rimu_annihilate", "rimu_annihilate_noalloc", "rimu_hamiltonian_eval",
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
@@ -107,6 +108,7 @@ def lib():
         L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
         L.rimu_engine_set_sort_bits.argtypes = [vp, i32]
         L.rimu_engine_set_radix_mode.argtypes = [vp, i32]
+        L.rimu_engine_set_lookback.argtypes = [vp, ctypes.c_uint32, i32]
         L.rimu_overlaps.argtypes = [vp, vp, P(Operator), i32, P(dbl)]
         L.rimu_engine_set_timing.argtypes = [vp, i32]
         L.rimu_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, i32, P(dbl), P(i64), i32, P(i32)]
@@ -306,8 +308,15 @@ class Engine:
         check(lib().rimu_engine_set_sort_bits(self._h, int(bits)))
 
     def set_radix_mode(self, mode: int):
-        """Radix sort of the spawn records: 1 look-back scatters (default), 0 per-pass counts."""
+        """Sort and merge of the spawn records (``rimu_engine_set_radix_mode``): 2 top-bit sort
+        and bucket merge where eligible (default), 1 look-back scatters over every bit and the
+        merge path, 0 per-pass counts and the merge path."""
         check(lib().rimu_engine_set_radix_mode(self._h, int(mode)))
+
+    def set_lookback(self, spin_limit: int, withhold_first: bool = False):
+        """Bound of the look-back waits and the host-forced give-up test
+        (``rimu_engine_set_lookback``)."""
+        check(lib().rimu_engine_set_lookback(self._h, int(spin_limit), int(bool(withhold_first))))
 
     def set_initiator(self, rule: int, threshold: float = 1.0):
         """Initiator rule of the vector (INITIATOR_* codes) and its threshold."""

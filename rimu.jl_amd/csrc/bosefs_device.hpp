@@ -148,12 +148,14 @@ template <class A> __device__ inline A insert_one(A x, int p) {
 // ---------------------------------------------------------------------------------------
 // Storage order of keys.  Above MIX_MIN_BITS address bits the engine stores every address
 // x as y = mix(x), a bijection of [0, 2^B) (multiply by an odd constant mod 2^B, fold the
-// high half down, twice).  The top bits of y then depend on every bit of x, so sorting the
-// spawn records on the top MIX_SORT_BITS bits of y leaves only rare groups to order on the
-// rest.  The vector is kept in y order; the step's results per address are unchanged (every
-// draw and every matrix element is a function of x).  At or below MIX_MIN_BITS, y = x.
+// high half down, twice).  The top bits of y then depend on every bit of x and are spread
+// evenly over the addresses of a vector: the spawn records split into buckets of about equal
+// size by their top bits (the bucket merge), and sorting them on the top MIX_SORT_BITS bits of
+// y leaves only rare groups to order on the rest.  The vector is kept in y order; the step's
+// results per address are unchanged (every draw and every matrix element is a function of x).
+// At or below MIX_MIN_BITS, y = x.
 // ---------------------------------------------------------------------------------------
-constexpr int MIX_MIN_BITS = 40;
+constexpr int MIX_MIN_BITS = 20;
 constexpr int MIX_SORT_BITS = 40;
 
 struct KeyMix {

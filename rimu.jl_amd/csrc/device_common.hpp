@@ -90,6 +90,8 @@ __device__ inline void block_sum3(double& a, double& b, double& c) {
 constexpr uint64_t LB_FLAG_AGG = 1ull << 62;
 constexpr uint64_t LB_FLAG_PFX = 2ull << 62;
 constexpr uint64_t LB_MASK = (1ull << 62) - 1;
+// Default bound of every look-back wait, in rounds of one s_sleep: far beyond any wait a
+// resident predecessor causes; past it the waiting tile gives up and raises an error flag.
 constexpr uint32_t LB_SPIN_LIMIT = 1u << 25;
 
 __device__ inline uint64_t lb_load(const uint64_t* p) {
@@ -102,11 +104,11 @@ __device__ inline void lb_store(uint64_t* p, uint64_t v) {
 // Wavefront-parallel look-back, called by all 64 lanes of one wavefront of tile `tile`.
 // Each probe reads the status words of 64 predecessors at once (lane l reads tile
 // window_end - l); the nearest inclusive prefix ends the walk, aggregates in front of it
-// are added.  Returns the exclusive prefix in every lane.  Every wait is bounded: if a
-// predecessor never publishes (which the tile-id ordering rules out) the error flag is raised
-// and the loop ends.
+// are added.  Returns the exclusive prefix in every lane.  Every wait is bounded by
+// spin_limit rounds: if a predecessor never publishes (which the tile-id ordering rules out)
+// the error flag is raised and the loop ends.
 __device__ inline uint64_t lookback_exclusive_wave(uint64_t* status, uint32_t tile, uint64_t agg,
-                                                   int* err_flag) {
+                                                   int* err_flag, uint32_t spin_limit) {
     const int lane = threadIdx.x & (WAVE - 1);
     if (tile == 0) {
         if (lane == 0) lb_store(&status[0], LB_FLAG_PFX | agg);
@@ -125,7 +127,7 @@ __device__ inline uint64_t lookback_exclusive_wave(uint64_t* status, uint32_t ti
         const int first = pfx ? (__ffsll((long long)pfx) - 1) : WAVE;  // nearest prefix lane
         const uint64_t needed = first >= WAVE - 1 ? ~0ull : ((2ull << first) - 1ull);
         if (empty & needed) {
-            if (++spins > LB_SPIN_LIMIT) {
+            if (++spins > spin_limit) {
                 if (lane == 0) atomicOr(err_flag, 2);
                 break;
             }

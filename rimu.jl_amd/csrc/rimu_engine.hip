@@ -20,6 +20,7 @@
 #include "../../include/rimu_hip.h"
 #include "capi_util.hpp"
 #include "overlap_kernels.hpp"
+#include "bucket_kernels.hpp"
 #include "step_kernels.hpp"
 
 using namespace rimu;
@@ -81,7 +82,9 @@ struct rimu_engine {
     ModelDev hdev{};
     KeyMix mix{};                   // stored keys are mix(address) above MIX_MIN_BITS bits
     int sort_bits = MIX_SORT_BITS;  // top bits of mixed keys the radix passes cover
-    int radix_mode = 1;             // 1: one histogram pass + look-back scatters; 0: counts per pass
+    // 2: sort on the top bits + bucket merge where eligible, else as 1; 1: one histogram pass
+    // + look-back scatters over every bit, merge path; 0: counts per pass, merge path
+    int radix_mode = 2;
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
@@ -116,8 +119,12 @@ struct rimu_engine {
     DevBuf<double> ov_part, ov_out;
     DevBuf<int64_t> ov_ranges;
     DevBuf<int64_t> fix_runs;       // runs to reorder after a sort on the top key bits
+    DevBuf<int64_t> bk_rstart, bk_vstart;  // bucket merge: first record / vector entry per bucket
     DevBuf<uint64_t> lb_status;     // look-back radix: status words [pass][tile][digit]
     uint32_t lb_epoch = 0;          // sort calls since lb_status was last cleared
+    int bucket_cooldown = 0;        // steps left on the merge path after a bucket overflow
+    uint32_t lb_spin_limit = LB_SPIN_LIMIT;  // bound of every look-back wait
+    int lb_withhold = 0;            // test of the give-up path (rimu_engine_set_lookback)
     DevBuf<uint32_t> lb_hist;       // look-back radix: digit bases [pass][digit], tile counters
     DevBuf<uint32_t> lb_part;       // look-back radix: digit counts [row][pass][digit]
     int64_t np = 0, nh = 0;
@@ -267,6 +274,7 @@ StepArgs make_args(rimu_engine* e, int64_t step, double shift, double dtau, doub
     a.init_rule = e->init_rule;
     a.init_threshold = e->init_threshold;
     a.mix = e->mix;
+    a.spin_limit = e->lb_spin_limit;
     return a;
 }
 
@@ -302,7 +310,7 @@ template <class K, class V>
 int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo = 0) {
     int src = in;
     if (n <= 1) { *result = src; return RIMU_OK; }
-    if (e->radix_mode == 1) return radix_sort_lb<K, V>(e, n, nbits, varying, in, result, lo);
+    if (e->radix_mode != 0) return radix_sort_lb<K, V>(e, n, nbits, varying, in, result, lo);
     const int64_t tiles = ceil_div(n, sort_tile<K>::value);
     const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
@@ -376,7 +384,7 @@ int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, in
         radix_scatter_lb_kernel<K, V><<<tiles, NT, 0, e->stream>>>(
             (const K*)e->skeys[src].p, (const V*)e->svals[src].p, n, ps.shift[p], ps.dmask[p],
             e->lb_hist.p + (int64_t)p * RADIX, e->lb_status.p + (int64_t)p * tiles * RADIX, epoch, counters + p,
-            flags,
+            flags, LookbackCtl{e->lb_spin_limit, e->lb_withhold},
             (K*)e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
@@ -445,17 +453,101 @@ int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int6
     if (init)
         merge_kernel<K, V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
                                                            (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
-                                                           e->splits.p, (const K*)e->prevk.p, bo);
+                                                           e->splits.p, (const K*)e->prevk.p, bo, &dm->flags);
     else
         merge_kernel<K, V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
                                                             (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
-                                                            e->splits.p, (const K*)e->prevk.p, nullptr);
+                                                            e->splits.p, (const K*)e->prevk.p, nullptr,
+                                                            &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
     if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nt + 1)) return rc;
     compact_kernel<K, V, merge_ipt<K>::value, FLAG_VECTOR_OVERFLOW><<<nt, NT, 0, e->stream>>>(
         (const K*)e->slot_k.p, (const V*)e->slot_v.p, e->tile_cnt.p, nt, ok, ov, cap, &dm->L_new, &dm->flags);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    return RIMU_OK;
+}
+
+// Stored keys from addresses (forward) or addresses from stored keys, in place.
+__global__ __launch_bounds__(NT) void mix_keys_kernel(uint64_t* __restrict__ keys, int64_t n, KeyMix m,
+                                                      int forward) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = forward ? mix_bits<uint64_t>(m, keys[i]) : unmix_bits<uint64_t>(m, keys[i]);
+}
+
+// Vectors of this many entries or more are reordered between address order and stored-key
+// order on the device (one-word keys, when the spawn buffers can hold them).
+constexpr int64_t DEVICE_REORDER_MIN = 1 << 16;
+
+bool device_reorder(const rimu_engine* e, int64_t n) {
+    return e->mix.on && e->kw == 1 && n >= DEVICE_REORDER_MIN && n <= e->max_spawns;
+}
+
+// n (key, value) pairs in spawn buffer 0: turn the keys into stored keys (forward) or back
+// into addresses, sort the pairs on the key and leave them in spawn buffer *result.
+int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result) {
+    mix_keys_kernel<<<ceil_div(n, NT), NT, 0, e->stream>>>(e->skeys[0].p, n, e->mix, forward);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(&((rimu_engine::Misc*)e->misc.p)->flags, 0, sizeof(int), e->stream));
+    if (int rc = radix_sort<uint64_t, uint64_t>(e, n, e->hdev.B, ~(u128)0, 0, result)) return rc;
+    rimu_engine::Misc m{};
+    HIP_TRY(hipMemcpyAsync(&m.flags, &((rimu_engine::Misc*)e->misc.p)->flags, sizeof(int), hipMemcpyDeviceToHost,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (m.flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
+    return RIMU_OK;
+}
+
+// Bucket merge (bucket_kernels.hpp) of S spawn records sorted on their top T key bits with
+// the vector (bk, bv: keys and diagonal survivors, L entries) into (ok, ov) of capacity cap.
+// `tiles` receives the number of buckets (rows of merge_p).
+constexpr double BK_FILL_A = 1100;  // mean records per bucket aimed at (LDS holds BK_CAP_A)
+constexpr double BK_FILL_B = 700;   // mean vector entries per bucket aimed at (BK_CAP_B)
+constexpr int64_t BK_MIN_RECORDS = 4096;
+constexpr int BK_MAX_LOWBITS = 48;   // LDS sort passes per bucket at most 6
+constexpr int BK_COOLDOWN = 16;      // steps on the merge path after a bucket overflowed
+
+// Top bits that split S records and L entries into buckets of about BK_FILL_A records and at
+// most about BK_FILL_B entries.
+int bucket_bits(int64_t S, int64_t L, int B) {
+    const double need = std::max((double)S / BK_FILL_A, (double)L / BK_FILL_B);
+    int T = (int)std::ceil(std::log2(std::max(need, 2.0)));
+    return std::min(std::max(T, 1), std::min(B, 24));
+}
+
+template <class V>
+int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
+                      const uint64_t* bk, const V* bv, int64_t L, int T, uint64_t* ok, V* ov, int64_t cap,
+                      rimu_engine::Misc* dm, int64_t* tiles) {
+    const int B = e->hdev.B;
+    const int64_t nbk = (int64_t)1 << T;
+    *tiles = nbk;
+    HIP_TRY(e->bk_rstart.alloc(nbk + 1));
+    HIP_TRY(e->bk_vstart.alloc(nbk + 1));
+    HIP_TRY(e->tile_cnt.alloc(nbk + 1));
+    HIP_TRY(e->merge_p.alloc((size_t)nbk * 3));
+    HIP_TRY(e->slot_k.alloc((size_t)(S + L)));
+    HIP_TRY(e->slot_v.alloc((size_t)(S + L)));
+    hipEvent_t ev{};
+    e->timer.begin("bucket_bounds", e->stream, &ev);
+    bucket_bounds_kernel<<<ceil_div(2 * nbk + 2, NT), NT, 0, e->stream>>>(ak, S, bk, L, B - T, nbk,
+                                                                          e->bk_rstart.p, e->bk_vstart.p);
+    e->timer.end(e->stream);
+    HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nbk, 0, sizeof(int64_t), e->stream));
+    e->timer.begin("bucket_merge", e->stream, &ev);
+    bucket_merge_kernel<V><<<nbk, BK_NT, 0, e->stream>>>(a, ak, av, bk, bv, e->bk_rstart.p, e->bk_vstart.p, B - T,
+                                                      e->slot_k.p, (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
+                                                      &dm->flags);
+    e->timer.end(e->stream);
+    HIP_TRY(hipGetLastError());
+    e->timer.begin("compact", e->stream, &ev);
+    if (int rc = scan_inplace_i64(e, e->tile_cnt.p, nbk + 1)) return rc;
+    bucket_compact_kernel<V><<<nbk, NT, 0, e->stream>>>(e->slot_k.p, (const V*)e->slot_v.p, e->bk_rstart.p,
+                                                        e->bk_vstart.p, e->tile_cnt.p, nbk, ok, ov, cap,
+                                                        &dm->L_new, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     return RIMU_OK;
@@ -586,32 +678,51 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         if (KW == 1) { kor = (uint64_t)kor; kand = (uint64_t)kand; }
     }
 
-    // 3. sort the spawn records
-    int sorted = 0;
-    if (S > 1) {
-        const int nbits = e->hdev.B;
-        if (!e->mix.on || nbits <= e->sort_bits) {
-            if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
-        } else if (int rc = sort_mixed<K, V>(e, S, nbits, kor ^ kand, a.init_rule != 0, dm, &sorted)) {
-            return rc;
-        }
-    }
-
-    // 4. merge with the diagonal survivors, fold, compress, count
+    // 3-4. sort the spawn records, merge them with the diagonal survivors, fold, compress,
+    // count.  Eligible steps sort on the top T bits only and merge bucket by bucket; if a
+    // bucket outgrows its LDS tile, the records are sorted on the remaining bits and merged
+    // along the merge path instead (same result).
     const int nxt = 1 - e->cur;
+    const int nbits = e->hdev.B;
     int64_t merge_tiles = 0;
-    if (int rc = merge_into<K, V>(e, a, (const K*)e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
-                                  (const V*)e->dvals.p, L, (K*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
-                                  e->max_entries, dm, &merge_tiles, vv)) return rc;
+    int sorted = 0;
+    int T = 0;
+    if (KW == 1 && e->radix_mode == 2 && a.init_rule == 0 && e->mix.on && S >= BK_MIN_RECORDS &&
+        e->bucket_cooldown == 0) {
+        T = bucket_bits(S, L, nbits);
+        if (nbits - T > BK_MAX_LOWBITS) T = 0;
+    }
+    if (e->bucket_cooldown > 0) --e->bucket_cooldown;
+    auto merge_path = [&]() -> int {
+        return merge_into<K, V>(e, a, (const K*)e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
+                                (const V*)e->dvals.p, L, (K*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
+                                e->max_entries, dm, &merge_tiles, vv);
+    };
+    if (T > 0) {
+        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted, nbits - T)) return rc;
+        if (int rc = bucket_merge_into<V>(e, a, (const uint64_t*)e->skeys[sorted].p, (const V*)e->svals[sorted].p,
+                                          S, (const uint64_t*)vk, (const V*)e->dvals.p, L, T,
+                                          (uint64_t*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, dm,
+                                          &merge_tiles)) return rc;
+    } else {
+        if (S > 1) {
+            if (!e->mix.on || nbits <= e->sort_bits) {
+                if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
+            } else if (int rc = sort_mixed<K, V>(e, S, nbits, kor ^ kand, a.init_rule != 0, dm, &sorted)) {
+                return rc;
+            }
+        }
+        if (int rc = merge_path()) return rc;
+    }
 
     // 5. projections and statistics
-    if (e->np + e->nh > 0) {
-        project_kernel<K, V><<<ceil_div(e->np + e->nh, NT), NT, 0, e->stream>>>(
-            (const K*)e->vkeys[nxt].p, (const V*)e->vvals[nxt].p, &dm->L_new, (const K*)e->pkeys.p, e->pcoef.p,
-            e->np + e->nh, e->terms.p);
-        HIP_TRY(hipGetLastError());
-    }
-    {
+    auto finish = [&]() -> int {
+        if (e->np + e->nh > 0) {
+            project_kernel<K, V><<<ceil_div(e->np + e->nh, NT), NT, 0, e->stream>>>(
+                (const K*)e->vkeys[nxt].p, (const V*)e->vvals[nxt].p, &dm->L_new, (const K*)e->pkeys.p, e->pcoef.p,
+                e->np + e->nh, e->terms.p);
+            HIP_TRY(hipGetLastError());
+        }
         PartialSet ps{};
         // plan partials rows: deaths, clones, zombies, exact, inexact, attempts
         const int plan_map[6] = {4, 5, 6, 1, 2, 0};
@@ -631,13 +742,29 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         ps.n[11] = e->nh;
         partial_sums_kernel<<<dim3(PARTIAL_BLOCKS, NSTAT), NT, 0, e->stream>>>(ps, e->chunk_sums.p);
         finalize_kernel<<<1, 64, 0, e->stream>>>(e->chunk_sums.p, PARTIAL_BLOCKS, e->stats_dev.p);
-    }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, 12 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    e->timer.collect();
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, 12 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        e->timer.collect();
+        return RIMU_OK;
+    };
+    if (int rc = finish()) return rc;
     if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
+    if (e->h_misc->flags & FLAG_BUCKET_OVERFLOW) {
+        // sort the bucketed records on every bit (stable passes keep their list order within
+        // an address) and merge along the merge path
+        e->bucket_cooldown = BK_COOLDOWN;
+        e->timer.begin("bucket_fallback", e->stream, &ev);
+        e->timer.end(e->stream);
+        HIP_TRY(hipMemsetAsync(&dm->flags, 0, sizeof(int), e->stream));
+        int full = sorted;
+        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, sorted, &full)) return rc;
+        sorted = full;
+        if (int rc = merge_path()) return rc;
+        if (int rc = finish()) return rc;
+        if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
+    }
     if (e->h_misc->flags & FLAG_VECTOR_OVERFLOW) {
         out->overflow = 1;
         return fail(RIMU_ERR_CAPACITY, "vector capacity (max_entries) exceeded");
@@ -771,6 +898,7 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
     std::vector<int64_t> vi;
     std::vector<double> vd;
     std::vector<u128> stored;  // mix(address) of the kept entries
+    const bool on_device = device_reorder(e, n);  // mix and sort on the device
     k.reserve((size_t)n * kw);
     u128 prev = 0;
     for (int64_t i = 0; i < n; ++i) {
@@ -788,13 +916,13 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
             if (v == 0) continue;
             vd.push_back(v);
         }
-        if (e->mix.on) {
+        if (e->mix.on && !on_device) {
             stored.push_back(kw == 1 ? (u128)mix_bits<uint64_t>(e->mix, (uint64_t)key) : mix_bits<u128>(e->mix, key));
         } else {
             for (int w = 0; w < kw; ++w) k.push_back(keys[(size_t)kw * i + w]);
         }
     }
-    if (e->mix.on) {  // the device vector is kept in the order of the stored keys
+    if (e->mix.on && !on_device) {  // the device vector is kept in the order of the stored keys
         std::vector<int64_t> ord(stored.size());
         for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int64_t)i;
         std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return stored[a] < stored[b]; });
@@ -810,7 +938,15 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
     }
     const int64_t m = (int64_t)(k.size() / kw);
     if (m > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
-    if (m > 0) {
+    if (on_device && m > 0) {
+        HIP_TRY(hipMemcpyAsync(e->skeys[0].p, k.data(), m * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
+                               m * 8, hipMemcpyHostToDevice, e->stream));
+        int sorted = 0;
+        if (int rc = reorder_on_device(e, m, 1, &sorted)) return rc;
+        HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, e->skeys[sorted].p, m * 8, hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->vvals[e->cur].p, e->svals[sorted].p, m * 8, hipMemcpyDeviceToDevice, e->stream));
+    } else if (m > 0) {
         HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, k.data(), m * 8 * kw, hipMemcpyHostToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->vvals[e->cur].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
                                m * 8, hipMemcpyHostToDevice, e->stream));
@@ -830,6 +966,16 @@ int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap
     if (!e || !n) return fail(RIMU_ERR_ARG, "null argument");
     *n = e->len;
     if (e->len > cap) return fail(RIMU_ERR_CAPACITY, "download capacity too small");
+    if (device_reorder(e, e->len)) {  // addresses back and address order on the device
+        HIP_TRY(hipMemcpyAsync(e->skeys[0].p, e->vkeys[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
+        int sorted = 0;
+        if (int rc = reorder_on_device(e, e->len, 0, &sorted)) return rc;
+        HIP_TRY(hipMemcpyAsync(keys, e->skeys[sorted].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(vals, e->svals[sorted].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        return RIMU_OK;
+    }
     if (e->len > 0) {
         HIP_TRY(hipMemcpyAsync(keys, e->vkeys[e->cur].p, e->len * 8 * e->kw, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipMemcpyAsync(vals, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
@@ -1001,8 +1147,15 @@ int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits) {
 
 int rimu_engine_set_radix_mode(rimu_engine* e, int32_t mode) {
     if (!e) return fail(RIMU_ERR_ARG, "engine is null");
-    if (mode != 0 && mode != 1) return fail(RIMU_ERR_ARG, "radix mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return fail(RIMU_ERR_ARG, "radix mode must be 0, 1 or 2");
     e->radix_mode = mode;
+    return RIMU_OK;
+}
+
+int rimu_engine_set_lookback(rimu_engine* e, uint32_t spin_limit, int32_t withhold_first) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    e->lb_spin_limit = spin_limit;
+    e->lb_withhold = withhold_first != 0;
     return RIMU_OK;
 }
 

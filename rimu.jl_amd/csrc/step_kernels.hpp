@@ -55,6 +55,15 @@ struct StepArgs {
     int init_rule;    // 0 NonInitiator, 1 Initiator, 2 SimpleInitiator, 3 CoherentInitiator
     double init_threshold;
     KeyMix mix;       // stored keys are mix(address)
+    uint32_t spin_limit;  // bound of the plan kernel's look-back waits (LB_SPIN_LIMIT)
+};
+
+// Bounded waits of the radix look-back: spin_limit rounds per wait; withhold_first makes
+// tile 0 of every pass keep its status words to itself, so that every later tile's wait
+// runs into the bound (a host-forced test of the give-up path).
+struct LookbackCtl {
+    uint32_t spin_limit;
+    int withhold_first;
 };
 
 // Initiator class of a record, carried in key bit 63 (addresses then have B <= 63 bits).
@@ -195,7 +204,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
     int64_t total;
     const int64_t ex = block_exclusive_scan<int64_t, NT>(sum, &total);  // ends with a barrier
     if (threadIdx.x < WAVE) {
-        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags);
+        const uint64_t b = lookback_exclusive_wave(status, tile, (uint64_t)total, flags, a.spin_limit);
         if (threadIdx.x == 0) s_prefix = (int64_t)b;
     }
     __syncthreads();
@@ -831,12 +840,13 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__
 // digit the sort will use (radix_hist_all_kernel, radix_hist_base_kernel: the global
 // exclusive prefix over digits of each pass).  Each scatter tile then publishes its digit
 // counts and looks back over its predecessors' status words, one digit per thread, for the
-// count of each digit in the tiles before it (decoupled look-back).  A tile waits only for
-// tiles of lower index: with tile ids taken from a counter those have started before it;
-// with tile = workgroup index, the in-order workgroup dispatch of each XCD has started them.
-// Every wait is bounded by LB_SPIN_LIMIT, past which FLAG_LOOKBACK is raised and the step
-// fails.  Ranks within the tile and the scatter are those of radix_scatter_kernel, so the
-// output is the same.
+// count of each digit in the tiles before it (decoupled look-back).  Tile ids come from a
+// counter in start order, so a tile waits only for tiles that have already started, whatever
+// order the hardware dispatches workgroups in.  Every wait is bounded (LookbackCtl), past
+// which FLAG_LOOKBACK is raised and the step fails; a tile that gave up publishes a prefix no
+// larger than the true one, so every write of the pass stays inside the output.  Ranks
+// within the tile and the scatter are those of radix_scatter_kernel, so the output is the
+// same.
 // ---------------------------------------------------------------------------------------
 constexpr int MAX_PASSES = 16;
 constexpr int LB_WIN = 8;   // predecessor status words read per look-back round
@@ -914,6 +924,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restric
                                                               const uint32_t* __restrict__ gbase,
                                                               uint64_t* status, uint32_t epoch,
                                                               uint32_t* tile_counter, int* flags,
+                                                              LookbackCtl ctl,
                                                               K* __restrict__ keys_out,
                                                               V* __restrict__ vals_out) {
     constexpr int NW = NT / WAVE;
@@ -926,9 +937,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restric
     __shared__ uint32_t s_run[RADIX];
     __shared__ uint32_t s_goff[RADIX];
 
-    // one-word keys: tile = workgroup index; two-word keys (one workgroup per CU): tile ids in
-    // start order from a counter (measured faster for each)
-    const uint32_t tile = sizeof(K) == 16 ? next_tile_id(tile_counter) : blockIdx.x;
+    const uint32_t tile = next_tile_id(tile_counter);  // start order
     const uint64_t tag = (uint64_t)epoch << 32;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
@@ -981,7 +990,8 @@ __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restric
         uint32_t tot;
         s_start[d] = block_exclusive_scan<uint32_t, NT>(run, &tot);
         s_run[d] = run;
-        lb_store(&status[(int64_t)tile * RADIX + d], (tile == 0 ? LB_FLAG_PFX : LB_FLAG_AGG) | tag | run);
+        if (tile != 0 || !ctl.withhold_first)
+            lb_store(&status[(int64_t)tile * RADIX + d], (tile == 0 ? LB_FLAG_PFX : LB_FLAG_AGG) | tag | run);
     }
     __syncthreads();
 #pragma unroll
@@ -1024,7 +1034,7 @@ __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restric
                 if (done) break;
                 j -= used;
                 if (empty) {
-                    if (++spins > LB_SPIN_LIMIT) { atomicOr(flags, FLAG_LOOKBACK); break; }
+                    if (++spins > ctl.spin_limit) { atomicOr(flags, FLAG_LOOKBACK); break; }
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
@@ -1083,7 +1093,7 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
                                                    double* __restrict__ partials,
                                                    const int64_t* __restrict__ splits,
                                                    const K* __restrict__ prevk,
-                                                   const V* __restrict__ bo) {
+                                                   const V* __restrict__ bo, const int* flags) {
     constexpr int IPT = merge_ipt<K>::value;
     constexpr int TILE = NT * IPT;
     constexpr K KM = INIT ? (K)~class_bit<K>() : (K)~(K)0;
@@ -1092,6 +1102,13 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
     __shared__ V s_val[TILE + MERGE_EXTRA];
 
     const int64_t tile = blockIdx.x;
+    if (*flags & FLAG_LOOKBACK) {  // the sort failed: its output is not an order to merge
+        if (threadIdx.x == 0) {
+            tile_cnt[tile] = 0;
+            for (int j = 0; j < 3; ++j) partials[(int64_t)j * gridDim.x + tile] = 0;
+        }
+        return;
+    }
     const int64_t total = S + L;
     const int64_t d0 = tile * TILE;
     const int64_t d1 = min(d0 + TILE, total);

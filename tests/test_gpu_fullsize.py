@@ -108,10 +108,10 @@ def random_bose_keys(M, N, n, rng, chunk=250_000):
     return keys[np.lexsort((keys[:, 0], keys[:, 1]))]
 
 
-def _step_both_modes(rimu, make_engine, keys, vals, steps, shift, dtau, sort_bits=None):
-    """The same state stepped by two engines, one per radix mode; returns both results."""
+def _step_both_modes(rimu, make_engine, keys, vals, steps, shift, dtau, sort_bits=None, modes=(0, 1)):
+    """The same state stepped by one engine per sort mode; returns every result."""
     out = []
-    for mode in (0, 1):
+    for mode in modes:
         eng = make_engine()
         eng.set_radix_mode(mode)
         if sort_bits is not None:
@@ -124,19 +124,23 @@ def _step_both_modes(rimu, make_engine, keys, vals, steps, shift, dtau, sort_bit
     return out
 
 
-def _assert_same(a, b):
+def _assert_same(a, b, norm_rel=0.0):
+    """Same vectors bit for bit and the same statistics; norm_rel > 0 allows the norm's sum
+    to differ in association (the bucket merge sums per bucket, the merge path per tile)."""
     (k0, v0, s0), (k1, v1, s1) = a, b
     assert k0.shape == k1.shape and np.array_equal(k0, k1)
     assert np.array_equal(v0.view(np.uint64), v1.view(np.uint64))  # bit for bit
     for x, y in zip(s0, s1):
-        for f in ("spawn_attempts", "spawns", "deaths", "clones", "zombies", "len", "norm"):
+        for f in ("spawn_attempts", "spawns", "deaths", "clones", "zombies", "len", "len_before"):
             assert getattr(x, f) == getattr(y, f), f
+        assert x.norm == pytest.approx(y.norm, rel=norm_rel, abs=0)
 
 
 def test_radix_modes_agree_on_headline_state(gpu):
-    """The look-back radix sort (one histogram pass, tiles ordered by start) and the
-    per-pass count sort give the same steps, bit for bit, on the 1e8-walker state
-    (about 7e7 spawn records, 17000 radix tiles per pass)."""
+    """The bucket merge (sort on the top 16 bits, 65536 buckets sorted and folded in LDS), the
+    look-back radix sort (one histogram pass, tiles ordered by start) and the per-pass count
+    sort give the same steps, bit for bit, on the 1e8-walker state (about 7e7 spawn records,
+    17000 radix tiles per pass)."""
     eng, sp, st = grow_headline(gpu, 300)
     keys, vals = eng.download()
     eng.close()
@@ -148,8 +152,9 @@ def test_radix_modes_agree_on_headline_state(gpu):
         return gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 91,
                                   int(2e8 + 101), int(4e8 + 4096))
 
-    a, b = _step_both_modes(gpu, make, keys, vals, 2, sp.shift, 1e-3)
+    a, b, c = _step_both_modes(gpu, make, keys, vals, 2, sp.shift, 1e-3, modes=(0, 1, 2))
     _assert_same(a, b)
+    _assert_same(b, c, norm_rel=1e-13)
 
 
 @pytest.mark.parametrize("sort_bits", [40, 128])
@@ -179,7 +184,8 @@ def test_radix_modes_agree_while_growing(gpu):
     H = gpu.HubbardReal1D(gpu.near_uniform(N, M), u=6.0, t=1.0)
     start = H.starting_address()
     runs = []
-    for mode in (0, 1):
+    shifts = []
+    for mode in (0, 1, 2):
         eng = gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 23, 400_101, 800_000)
         eng.set_radix_mode(mode)
         eng.upload(np.array([start.bits], dtype=np.uint64), np.array([10.0]))
@@ -187,15 +193,24 @@ def test_radix_modes_agree_while_growing(gpu):
         sp = gpu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, 1e-3)
         stats = []
         for step in range(1, 301):
-            st = eng.step(step, sp.shift, 1e-3)
+            # the bucket merge sums the norm in another association: it is given the shifts
+            # of the merge path, so that its vectors can be compared bit for bit
+            shift = shifts[step - 1] if mode == 2 else sp.shift
+            if mode == 1:
+                shifts.append(sp.shift)
+            st = eng.step(step, shift, 1e-3)
             strat.update(sp, st.norm)
             stats.append((st.spawn_attempts, st.spawns, st.len, st.norm))
         k, v = eng.download()
         eng.close()
         runs.append((k, v, stats))
-    (k0, v0, s0), (k1, v1, s1) = runs
+    (k0, v0, s0), (k1, v1, s1), (k2, v2, s2) = runs
     assert s0 == s1
     assert np.array_equal(k0, k1) and np.array_equal(v0.view(np.uint64), v1.view(np.uint64))
+    assert np.array_equal(k0, k2) and np.array_equal(v0.view(np.uint64), v2.view(np.uint64))
+    # the shift follows the norms, whose sums differ in association on the bucket path
+    for a, b in zip(s1, s2):
+        assert a[:3] == b[:3] and a[3] == pytest.approx(b[3], rel=1e-13)
     assert s0[-1][3] > 1e5
 
 
@@ -203,10 +218,11 @@ def test_radix_mode_rejects_unknown_values(gpu):
     H = gpu.HubbardReal1D(gpu.near_uniform(4, 4), u=1.0, t=1.0)
     eng = gpu._native.Engine(H.descriptor(), gpu.IsDynamicSemistochastic().descriptor(), 1, 1000, 1000)
     try:
-        for bad in (-1, 2, 7):
+        for bad in (-1, 3, 7):
             with pytest.raises(gpu.RimuError):
                 eng.set_radix_mode(bad)
         eng.set_radix_mode(0)
         eng.set_radix_mode(1)
+        eng.set_radix_mode(2)
     finally:
         eng.close()

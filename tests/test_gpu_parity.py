@@ -96,7 +96,7 @@ def test_annihilate_bit_exact(gpu, vtype, n, nkeys):
 
 
 STEP_CASES = [
-    # model, M, N, u, style, vector length, value scale, shift, dtau
+    # model, M, N, u, style, vector length, value scale, shift, dtau[, options]
     ("real", 10, 10, 6.0, "integer", 2000, 6, -2.0, 0.01),
     ("real", 20, 20, 6.0, "integer", 5000, 40, 0.5, 0.001),
     ("mom", 8, 8, 4.0, "integer", 1500, 8, -3.0, 0.005),
@@ -106,31 +106,69 @@ STEP_CASES = [
     ("mom", 8, 8, 4.0, "dynamic", 1500, 10.0, -3.0, 0.005),
     ("real", 8, 8, 2.0, "deterministic", 1000, 1.0, 0.3, 0.01),
     ("mom", 7, 6, 3.0, "deterministic", 600, 1.0, 0.3, 0.01),
+    # IsDynamicSemistochastic(late_compression=false): deposits projected to +-1 when
+    # smaller (styles.jl:182-194, spawning.jl:13-31), no ThresholdCompression afterwards
+    ("real", 10, 10, 6.0, "dynamic", 2000, 4.0, -2.0, 0.01, {"late_compression": False}),
+    ("mom", 8, 8, 4.0, "dynamic", 1500, 3.0, -3.0, 0.005, {"late_compression": False}),
+    # non-default spawning thresholds of DynamicSemistochastic (spawning.jl:358-378)
+    ("real", 12, 12, 6.0, "dynamic", 3000, 8.0, -1.0, 0.01, {"rel_spawning_threshold": 2.0}),
+    ("mom", 8, 8, 4.0, "dynamic", 1500, 10.0, -3.0, 0.005, {"abs_spawning_threshold": 3.0}),
+    ("real", 10, 10, 6.0, "dynamic", 2000, 6.0, -2.0, 0.01,
+     {"rel_spawning_threshold": 0.5, "abs_spawning_threshold": 5.0, "threshold": 0.7}),
+    # boost of apply_operator! (Interfaces/dictvectors.jl:112): more attempts per walker
+    ("real", 10, 10, 6.0, "integer", 2000, 6, -2.0, 0.01, {"boost": 2.0}),
+    ("mom", 8, 8, 4.0, "dynamic", 1500, 10.0, -3.0, 0.005, {"boost": 2.0}),
+    ("real", 12, 12, 6.0, "dynamic", 3000, 3.0, -1.0, 0.01, {"boost": 3.0, "late_compression": False}),
 ]
 
 
-def _engine(rimu, kind, M, N, u, style, seed):
+def _case_id(c):
+    opts = c[9] if len(c) > 9 else {}
+    extra = "".join(f"-{k}={v}" for k, v in opts.items())
+    return f"{c[0]}{c[1]}x{c[2]}-{c[4]}{extra}"
+
+
+def _styles(rimu, style, opts):
+    """The device style and the oracle style of a case (options as Rimu's keywords)."""
+    if style == "integer":
+        return rimu.IsStochasticInteger(), O.style("integer")
+    if style == "deterministic":
+        return rimu.IsDeterministic(), O.style("deterministic")
+    kw = {k: opts[k] for k in ("threshold", "rel_spawning_threshold", "abs_spawning_threshold",
+                               "late_compression") if k in opts}
+    dev = rimu.IsDynamicSemistochastic(**kw)
+    th = kw.get("threshold", 1.0)
+    late = kw.get("late_compression", True)
+    osty = O.style("dynamic", compression_threshold=th, proj_threshold=0.0 if late else th,
+                   rel_threshold=kw.get("rel_spawning_threshold", 1.0),
+                   abs_threshold=kw.get("abs_spawning_threshold", float("inf")),
+                   compression=1 if late else 0)
+    return dev, osty
+
+
+def _engine(rimu, kind, M, N, u, style, seed, opts=None):
     h = device_model(rimu, kind, M, N, u, 1.0)
-    st = {"integer": rimu.IsStochasticInteger(), "dynamic": rimu.IsDynamicSemistochastic(),
-          "deterministic": rimu.IsDeterministic()}[style]
+    st, _ = _styles(rimu, style, opts or {})
     return rimu._native.Engine(h.descriptor(), st.descriptor(), seed, 4_000_000, 1 << 16)
 
 
-@pytest.mark.parametrize("case", STEP_CASES, ids=lambda c: f"{c[0]}{c[1]}x{c[2]}-{c[4]}")
+@pytest.mark.parametrize("case", STEP_CASES, ids=_case_id)
 def test_step_matches_oracle(gpu, case):
-    kind, M, N, u, style, n, scale, shift, dtau = case
+    kind, M, N, u, style, n, scale, shift, dtau = case[:9]
+    opts = case[9] if len(case) > 9 else {}
+    boost = opts.get("boost", 1.0)
     rng = np.random.default_rng(zlib.crc32(repr(case).encode()))
     keys, vals = random_vector(M, N, n, rng, "int" if style == "integer" else "float", scale)
     seed, step = 0xC0FFEE + n, 37
-    eng = _engine(gpu, kind, M, N, u, style, seed)
+    eng = _engine(gpu, kind, M, N, u, style, seed, opts)
     eng.upload(keys, vals)
-    st = eng.step(step, shift, dtau)
+    st = eng.step(step, shift, dtau, boost)
     gk, gv = eng.download()
     eng.close()
     om = oracle_model(kind, M, N, u, 1.0)
-    osty = O.style(style)
+    _, osty = _styles(gpu, style, opts)
     ok, ov, ostats = O.step(om, osty, keys, vals, seed=seed, step=step, shift=shift,
-                            dtau=dtau, order=1)
+                            dtau=dtau, boost=boost, order=1)
     assert np.array_equal(gk, ok)
     if style == "integer":
         assert np.array_equal(gv, ov)
@@ -147,7 +185,8 @@ def test_step_matches_oracle(gpu, case):
         assert st.norm == pytest.approx(ostats["norm"], rel=1e-13)
         assert st.spawns == pytest.approx(ostats["spawns"], rel=1e-13)
     # the reference's own column order differs only in floating-point association
-    rk, rv, _ = O.step(om, osty, keys, vals, seed=seed, step=step, shift=shift, dtau=dtau, order=0)
+    rk, rv, _ = O.step(om, osty, keys, vals, seed=seed, step=step, shift=shift, dtau=dtau,
+                       boost=boost, order=0)
     if style == "deterministic":
         assert np.array_equal(rk, gk)
         np.testing.assert_allclose(gv, rv, rtol=1e-12, atol=1e-12 * np.abs(rv).max())
