@@ -124,3 +124,51 @@ def u_sweep_specs(nruns: int = 256, dtau: float = 1e-3, rng_seed: int = 2024):
         h = HubbardReal1D(near_uniform(N, M), u=u, t=1.0)
         specs.append(RunSpec(h, target_walkers=tw, dtau=dtau, seed=i))
     return specs
+
+
+def ground_state_energies(specs):
+    """Exact ground-state energy of every run's Hamiltonian: Lanczos on the device over the
+    ranked basis (``rimu_dense_ground_state``), runs of equal M, N grouped eight to a call."""
+    from .dense import DenseHamiltonians
+    e0 = np.full(len(specs), np.nan)
+    groups = {}
+    for r, sp in enumerate(specs):
+        a = sp.hamiltonian.starting_address()
+        groups.setdefault((a.M, a.N), []).append(r)
+    for _, runs in groups.items():
+        for c in range(0, len(runs), 8):
+            chunk = runs[c:c + 8]
+            d = DenseHamiltonians([specs[r].hamiltonian for r in chunk])
+            e, _ = d.ground_state()
+            e0[chunk] = e
+            d.close()
+    return e0
+
+
+def energy_check(specs, hist, status, e0, skip):
+    """Per run: the shift estimator (blocking analysis) and the projected energy
+    (``ratio_of_means`` with Monte Carlo error propagation) after ``skip`` steps, and whether
+    each holds the exact energy ``e0`` within 2 and 3 standard errors (``val_and_errs``
+    intervals, StatsTools/convenience.jl:61-112).  Runs that stopped are left out."""
+    from . import statstools as ST
+    rows = []
+    for r, sp in enumerate(specs):
+        if status[r] != 0:
+            continue
+        se = ST.blocking_analysis(hist["shift"][:, r], skip=skip)
+        pe = ST.ratio_of_means(hist["hproj"][:, r], hist["vproj"][:, r], skip=skip)
+        a = sp.hamiltonian.starting_address()
+        ok_s = se.k > 0 and np.isfinite(se.err)
+        rows.append({
+            "run": r, "M": a.M, "N": a.N, "u": sp.hamiltonian.u, "target_walkers": sp.target_walkers,
+            "mean_walkers": float(hist["norm"][skip:, r].mean()), "E0": float(e0[r]),
+            "shift": se.mean, "shift_err": se.err, "shift_k": se.k,
+            "shift_2sigma": bool(ok_s and ST.within(se, e0[r], n=2)),
+            "shift_3sigma": bool(ok_s and ST.within(se, e0[r], n=3)),
+            "projected": pe.pmedian(), "projected_f": pe.f, "projected_sigma_f": pe.sigma_f,
+            "projected_err": pe.err, "delta_y": pe.delta_y, "projected_k": pe.k,
+            "projected_success": pe.success,
+            "projected_2sigma": bool(ST.within(pe, e0[r], n=2)),
+            "projected_3sigma": bool(ST.within(pe, e0[r], n=3)),
+        })
+    return rows

@@ -108,7 +108,8 @@ class Particles:
         return float(np.median(self.samples))
 
     def pquantile(self, q):
-        return np.quantile(self.samples, q)
+        with np.errstate(invalid="ignore"):  # samples of a zero-mean denominator hold inf
+            return np.quantile(self.samples, q)
 
     def pmean(self) -> float:
         return float(self.samples.mean())
@@ -171,9 +172,11 @@ def _particles_2d(mu, cov, samples, rng):
 def x_by_y_linear(mu_x, mu_y, sigma_x, sigma_y, rho):
     """``x_by_y_linear`` (ratio_of_means.jl:167-172): f = x/y and its standard error by
     linear propagation for correlated normal x, y (covariance rho)."""
-    f = mu_x / mu_y
-    v = (sigma_x / mu_y) ** 2 + (mu_x * sigma_y / mu_y ** 2) ** 2 - 2 * rho * mu_x / mu_y ** 3
-    return f, float(np.sqrt(v)) if v >= 0 else float("nan")
+    mu_x, mu_y, sigma_x, sigma_y, rho = (np.float64(x) for x in (mu_x, mu_y, sigma_x, sigma_y, rho))
+    with np.errstate(divide="ignore", invalid="ignore"):  # a zero mean gives inf/nan, as in Julia
+        f = mu_x / mu_y
+        v = (sigma_x / mu_y) ** 2 + (mu_x * sigma_y / mu_y ** 2) ** 2 - 2 * rho * mu_x / mu_y ** 3
+    return float(f), float(np.sqrt(v)) if v >= 0 else float("nan")
 
 
 def ratio_estimators(x, y, k=None, corrected=True, mc_samples=2000, rng=None):
@@ -214,8 +217,9 @@ def ratio_of_means(num, denom, alpha=0.01, corrected=True, skip=0, mc_samples=20
     success = kn >= 0 and kd >= 0
     k = max(kn, kd)
     r, _, sigma_f, delta_y, blocks = ratio_estimators(num, denom, k, corrected, mc_samples, rng)
-    return RatioBlockingResult(r, float(num.mean() / denom.mean()), sigma_f, delta_y, k, blocks,
-                               success)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = float(np.float64(num.mean()) / np.float64(denom.mean()))
+    return RatioBlockingResult(r, f, sigma_f, delta_y, k, blocks, success)
 
 
 def val_and_errs(r, n=1, p=None):

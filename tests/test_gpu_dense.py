@@ -105,6 +105,43 @@ def test_projector_gradient_against_finite_differences(gpu, ckpt):
             assert res["grad_u"][i] == pytest.approx(g_cpu, rel=1e-6)
 
 
+def test_config5_gradients_full_size(gpu):
+    """The differentiable projector at its own size: HubbardReal1D M = N = 12 (dimension
+    1,352,078), u in {2, 4, 6, 8}, t = 1, K = 2000 steps of dtau 1e-3 from the near-uniform
+    address, checkpoints every 45 steps (45 recomputed segments).  dE/du and dE/dt of the
+    Rayleigh quotient (Hamiltonians/abstract.jl:153) against central differences of the
+    forward projector at h = 1e-5 (rtol 1e-6); another checkpoint spacing (200 steps) gives
+    the same energies and gradients to 1e-12."""
+    M = N = 12
+    us = np.array([2.0, 4.0, 6.0, 8.0])
+    ts = np.ones(4)
+    K, dtau, h = 2000, 1e-3, 1e-5
+    d, hs = make(gpu, M, N, us, ts)
+    assert d.dim == 1352078
+    v0 = d.unit_vector(hs[0].starting_address())
+    shift = np.zeros(4)
+    print("config5 adjoint K=2000 ckpt=45", flush=True)
+    res = d.projector_energy(K, dtau, shift, v0, ckpt=45)
+    res2 = d.projector_energy(K, dtau, shift, v0, ckpt=200)
+    d.close()
+    for f in ("energy", "grad_u", "grad_t"):
+        np.testing.assert_allclose(res2[f], res[f], rtol=1e-12)
+
+    def energy(u, t):
+        e, _ = make(gpu, M, N, u, t)
+        out = e.projector_energy(K, dtau, shift, v0, ckpt=45)["energy"]
+        e.close()
+        return out
+
+    print("config5 central differences", flush=True)
+    fd_u = (energy(us + h, ts) - energy(us - h, ts)) / (2 * h)
+    fd_t = (energy(us, ts + h) - energy(us, ts - h)) / (2 * h)
+    np.testing.assert_allclose(res["grad_u"], fd_u, rtol=1e-6)
+    np.testing.assert_allclose(res["grad_t"], fd_t, rtol=1e-6)
+    # E = u <D> + t <K> of the final vector (H is linear in u and t)
+    np.testing.assert_allclose(res["energy"], us * res["hf_u"] + ts * res["hf_t"], rtol=1e-12)
+
+
 def test_projector_converges_to_hellmann_feynman(gpu):
     """At convergence dE/du is <psi0|D|psi0> and E is the ED ground state energy."""
     M = N = 5
