@@ -221,11 +221,12 @@ int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits);
 int rimu_engine_set_radix_mode(rimu_engine* e, int32_t mode);
 
 /* (No counterpart in the reference; for tests.)  Every look-back wait of the step kernels
- * (the plan's attempt offsets, the radix scatters' digit offsets) is bounded: after
- * spin_limit rounds (default 2^25) the waiting tile gives up, and the step fails with
- * RIMU_ERR_INTERNAL, leaving the vector unchanged.  withhold_first = 1 makes the first tile
- * of every radix pass keep its counts to itself, so that every later tile's wait runs into
- * the bound: a host-forced test of the give-up path. */
+ * (the plan's attempt offsets, the radix scatters' digit offsets) is bounded by spin_limit
+ * rounds (default 2^25).  A radix wait that gives up makes the engine repeat the step with
+ * the per-pass sort (radix mode 0, no look-back; same result); a plan wait that gives up
+ * fails the step with RIMU_ERR_INTERNAL, leaving the vector unchanged.  withhold_first = 1
+ * makes the first tile of every radix pass keep its counts to itself, so that every later
+ * tile's wait runs into the bound: a host-forced test of the give-up path. */
 int rimu_engine_set_lookback(rimu_engine* e, uint32_t spin_limit, int32_t withhold_first);
 
 /* Per-kernel device time accumulated since the last reset, measured with HIP events on

@@ -7,8 +7,10 @@
 // key >> (B - T) -- and the vector, kept in stored-key order, holds each bucket's entries as
 // one contiguous segment.  One workgroup per bucket then
 //   1. loads the bucket's records and the vector segment (keys, diagonal survivors),
-//   2. sorts the records stably on the key bits below the bucket bits, in LDS (8-bit digits,
-//      ranks from wavefront ballots, as the global scatter passes do),
+//   2. orders the records by key in LDS, equal keys in list order: one stable radix pass on the
+//      8 key bits below the bucket bits (ranks from wavefront ballots, as the global scatter
+//      passes do) makes 256 sub-buckets of a few records each, and every record then counts
+//      the smaller (key, list index) pairs of its sub-bucket,
 //   3. merges them with the segment along the merge path and folds every address left to
 //      right -- diagonal value first, then its spawns in list order, the order of deposit!
 //      over the list (Interfaces/dictvectors.jl:49-51, dvec.jl:129-136) -- applies
@@ -68,7 +70,7 @@ struct BucketLds {
     V vb[BK_CAP_B];                 // diagonal survivors of the bucket's vector entries
     uint64_t kb[BK_CAP_B];          // the entries' keys (ascending)
     uint32_t wcnt[BK_NT / WAVE][RADIX];
-    uint32_t start[RADIX];
+    uint32_t start[RADIX + 1];
     double red[3][BK_NT / WAVE];
 };
 static_assert(2 * BK_CAP_A >= BK_CAP_A + BK_CAP_B, "staged keys fit the sort arrays");
@@ -140,24 +142,27 @@ __global__ __launch_bounds__(BK_NT) void bucket_merge_kernel(StepArgs a, const u
     }
     __syncthreads();
 
-    // ---- LSD radix sort of the packed records on their key bits, in LDS -----------------
-    // Wavefront w ranks the contiguous records [w * BK_RPT * 64, (w + 1) * BK_RPT * 64) with
-    // its own digit counters, round r taking record w * BK_RPT * 64 + r * 64 + lane, so the
-    // ranks are stable.  (Equal keys never reorder: their packed words differ in the index.)
-    int cur = 0;
+    // ---- order the packed records by key bits, in LDS: one stable radix pass on the top 8
+    // key bits below the bucket bits (wavefront w ranks the contiguous records
+    // [w * BK_RPT * 64, (w + 1) * BK_RPT * 64) with its own digit counters, round r taking
+    // record w * BK_RPT * 64 + r * 64 + lane), which splits the bucket into 256 sub-buckets
+    // of a few records each; then every record finds its place inside its sub-bucket by
+    // counting the smaller packed words there (equal keys: smaller list index first).
     const int wbase = wid * BK_RPT * WAVE;
-    for (int shift = 0; shift < lowbits; shift += RADIX_BITS) {
-        const int sh = shift + BK_IDX_BITS;
-        const uint32_t dmask = shift + RADIX_BITS > lowbits ? (1u << (lowbits - shift)) - 1u : (uint32_t)(RADIX - 1);
+    const int dshift = lowbits > RADIX_BITS ? lowbits - RADIX_BITS : 0;
+    const int sh = dshift + BK_IDX_BITS;
+    const uint32_t dmask = lowbits >= RADIX_BITS ? (uint32_t)(RADIX - 1) : (1u << lowbits) - 1u;
+    uint32_t dig[BK_RPT];
+    uint64_t w[BK_RPT];
+    {
         for (int j = threadIdx.x; j < NW * RADIX; j += BK_NT) (&sm.wcnt[0][0])[j] = 0;
         __syncthreads();
         uint32_t* cnt = sm.wcnt[wid];
-        uint32_t rank[BK_RPT], dig[BK_RPT];
-        uint64_t w[BK_RPT];
+        uint32_t rank[BK_RPT];
 #pragma unroll
         for (int r = 0; r < BK_RPT; ++r) {
             const int idx = wbase + r * WAVE + lane;
-            w[r] = idx < na ? sm.rk[cur][idx] : 0ull;
+            w[r] = idx < na ? sm.rk[0][idx] : 0ull;
         }
 #pragma unroll
         for (int r = 0; r < BK_RPT; ++r) {
@@ -200,16 +205,30 @@ __global__ __launch_bounds__(BK_NT) void bucket_merge_kernel(StepArgs a, const u
             uint32_t run = incl - sum;
 #pragma unroll
             for (int q = 0; q < RADIX / WAVE; ++q) { sm.start[lane * (RADIX / WAVE) + q] = run; run += x[q]; }
+            if (lane == WAVE - 1) sm.start[RADIX] = run;
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < BK_RPT; ++r) {
             const int idx = wbase + r * WAVE + lane;
-            if (idx < na) sm.rk[1 - cur][sm.start[dig[r]] + sm.wcnt[wid][dig[r]] + rank[r]] = w[r];
+            if (idx < na) sm.rk[1][sm.start[dig[r]] + sm.wcnt[wid][dig[r]] + rank[r]] = w[r];
         }
         __syncthreads();
-        cur = 1 - cur;
     }
+    if (dshift > 0) {  // bits below the sub-bucket digit: rank inside the sub-bucket
+#pragma unroll
+        for (int r = 0; r < BK_RPT; ++r) {
+            const int idx = wbase + r * WAVE + lane;
+            if (idx < na) {
+                const int s0 = (int)sm.start[dig[r]], s1 = (int)sm.start[dig[r] + 1];
+                int less = 0;
+                for (int q = s0; q < s1; ++q) less += sm.rk[1][q] < w[r];
+                sm.rk[0][s0 + less] = w[r];
+            }
+        }
+        __syncthreads();
+    }
+    const int cur = dshift > 0 ? 0 : 1;
     const uint64_t* s_a = sm.rk[cur];
     auto akey = [&](int i) { return bucket_key | (s_a[i] >> BK_IDX_BITS); };
     auto aval = [&](int i) { return sm.rv[(int)(s_a[i] & ((1u << BK_IDX_BITS) - 1u))]; };

@@ -486,19 +486,30 @@ bool device_reorder(const rimu_engine* e, int64_t n) {
     return e->mix.on && e->kw == 1 && n >= DEVICE_REORDER_MIN && n <= e->max_spawns;
 }
 
-// n (key, value) pairs in spawn buffer 0: turn the keys into stored keys (forward) or back
-// into addresses, sort the pairs on the key and leave them in spawn buffer *result.
-int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result) {
-    mix_keys_kernel<<<ceil_div(n, NT), NT, 0, e->stream>>>(e->skeys[0].p, n, e->mix, forward);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemsetAsync(&((rimu_engine::Misc*)e->misc.p)->flags, 0, sizeof(int), e->stream));
-    if (int rc = radix_sort<uint64_t, uint64_t>(e, n, e->hdev.B, ~(u128)0, 0, result)) return rc;
-    rimu_engine::Misc m{};
-    HIP_TRY(hipMemcpyAsync(&m.flags, &((rimu_engine::Misc*)e->misc.p)->flags, sizeof(int), hipMemcpyDeviceToHost,
-                           e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    if (m.flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
-    return RIMU_OK;
+// n (key, value) pairs that fill() puts into spawn buffer 0: turn the keys into stored keys
+// (forward) or back into addresses, sort the pairs on the key and leave them in spawn buffer
+// *result.  A radix look-back that gives up (the sort buffers then hold no list) makes it
+// fill and sort again with the per-pass sort.
+template <class Fill>
+int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result, Fill&& fill) {
+    const int saved = e->radix_mode;
+    int rc = RIMU_OK;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((rc = fill())) break;
+        mix_keys_kernel<<<ceil_div(n, NT), NT, 0, e->stream>>>(e->skeys[0].p, n, e->mix, forward);
+        if (hipGetLastError() != hipSuccess) { rc = fail(RIMU_ERR_HIP, "mix_keys_kernel launch"); break; }
+        int* flags = &((rimu_engine::Misc*)e->misc.p)->flags;
+        if (hipMemsetAsync(flags, 0, sizeof(int), e->stream) != hipSuccess) { rc = fail(RIMU_ERR_HIP, "memset"); break; }
+        if ((rc = radix_sort<uint64_t, uint64_t>(e, n, e->hdev.B, ~(u128)0, 0, result))) break;
+        int h_flags = 0;
+        if (hipMemcpyAsync(&h_flags, flags, sizeof(int), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess) { rc = fail(RIMU_ERR_HIP, "flags"); break; }
+        if (!(h_flags & FLAG_LOOKBACK)) break;
+        if (e->radix_mode == 0) { rc = fail(RIMU_ERR_INTERNAL, "radix look-back did not complete"); break; }
+        e->radix_mode = 0;
+    }
+    e->radix_mode = saved;
+    return rc;
 }
 
 // Bucket merge (bucket_kernels.hpp) of S spawn records sorted on their top T key bits with
@@ -750,6 +761,18 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         return RIMU_OK;
     };
     if (int rc = finish()) return rc;
+    if ((e->h_misc->flags & FLAG_LOOKBACK) && e->radix_mode != 0) {
+        // a radix look-back wait gave up (only forced from the host: every tile waited on has
+        // started): the sort buffers hold no list any more, the vector is unchanged; repeat the
+        // whole step with the per-pass sort, which waits on no other tile (same result)
+        e->timer.begin("lookback_fallback", e->stream, &ev);
+        e->timer.end(e->stream);
+        const int saved = e->radix_mode;
+        e->radix_mode = 0;
+        const int rc = step_impl<K, V>(e, step, shift, dtau, boost, out);
+        e->radix_mode = saved;
+        return rc;
+    }
     if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "radix look-back did not complete");
     if (e->h_misc->flags & FLAG_BUCKET_OVERFLOW) {
         // sort the bucketed records on every bit (stable passes keep their list order within
@@ -939,11 +962,13 @@ int rimu_vector_upload(rimu_engine* e, const uint64_t* keys, const void* vals, i
     const int64_t m = (int64_t)(k.size() / kw);
     if (m > e->max_entries) return fail(RIMU_ERR_CAPACITY, "vector longer than max_entries");
     if (on_device && m > 0) {
-        HIP_TRY(hipMemcpyAsync(e->skeys[0].p, k.data(), m * 8, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
-                               m * 8, hipMemcpyHostToDevice, e->stream));
         int sorted = 0;
-        if (int rc = reorder_on_device(e, m, 1, &sorted)) return rc;
+        if (int rc = reorder_on_device(e, m, 1, &sorted, [&]() -> int {
+                HIP_TRY(hipMemcpyAsync(e->skeys[0].p, k.data(), m * 8, hipMemcpyHostToDevice, e->stream));
+                HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->int_vals ? (const void*)vi.data() : (const void*)vd.data(),
+                                       m * 8, hipMemcpyHostToDevice, e->stream));
+                return RIMU_OK;
+            })) return rc;
         HIP_TRY(hipMemcpyAsync(e->vkeys[e->cur].p, e->skeys[sorted].p, m * 8, hipMemcpyDeviceToDevice, e->stream));
         HIP_TRY(hipMemcpyAsync(e->vvals[e->cur].p, e->svals[sorted].p, m * 8, hipMemcpyDeviceToDevice, e->stream));
     } else if (m > 0) {
@@ -967,10 +992,14 @@ int rimu_vector_download(rimu_engine* e, uint64_t* keys, void* vals, int64_t cap
     *n = e->len;
     if (e->len > cap) return fail(RIMU_ERR_CAPACITY, "download capacity too small");
     if (device_reorder(e, e->len)) {  // addresses back and address order on the device
-        HIP_TRY(hipMemcpyAsync(e->skeys[0].p, e->vkeys[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
-        HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice, e->stream));
         int sorted = 0;
-        if (int rc = reorder_on_device(e, e->len, 0, &sorted)) return rc;
+        if (int rc = reorder_on_device(e, e->len, 0, &sorted, [&]() -> int {
+                HIP_TRY(hipMemcpyAsync(e->skeys[0].p, e->vkeys[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice,
+                                       e->stream));
+                HIP_TRY(hipMemcpyAsync(e->svals[0].p, e->vvals[e->cur].p, e->len * 8, hipMemcpyDeviceToDevice,
+                                       e->stream));
+                return RIMU_OK;
+            })) return rc;
         HIP_TRY(hipMemcpyAsync(keys, e->skeys[sorted].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipMemcpyAsync(vals, e->svals[sorted].p, e->len * 8, hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
