@@ -186,3 +186,54 @@ def test_batch_energy_against_ed(gpu, M, N, u):
         assert se.k > 0 and pe.k > 0
         assert abs(se.mean - E0) < 3 * se.err + 1e-3 * abs(E0), (r, se, E0)
         assert abs(pe.pmedian() - E0) < 3 * pe.err + 1e-3 * abs(E0), (r, pe, E0)
+
+
+def _sweep(gpu, specs, nsteps, chunk):
+    b = gpu.BatchedRuns(specs, history_steps=nsteps)
+    for c in range(nsteps // chunk):
+        b.step(chunk, steps_per_graph=100)
+        print(f"  sweep: {(c + 1) * chunk} of {nsteps} steps", flush=True)
+    st, _ = b.status()
+    h = b.history()
+    b.close()
+    return st, h
+
+
+def test_usweep_energies_against_ed(gpu):
+    """The batched U-sweep workload (256 HubbardReal1D runs: M ~ U{6..14}, N ~ U{ceil(M/2)..M},
+    u ~ U[1, 10], target walkers log-uniform in [1e3, 1e5], dtau = 1e-3) against each run's own
+    exact energy (Lanczos on the device).
+
+    * Projected energy (ratio_of_means with Monte Carlo error propagation, the reference's
+      2-sigma interval) over 100,000 steps after 20,000: at least 234 of 256 runs hold E0,
+      the binomial expectation 0.9545 * 256 = 244.4 less three standard deviations (3.3
+      each); 236 do (240 at four times the walkers).  Over the config's own 10,000 steps the
+      runs have not relaxed (about 220 pass).  The runs are seeded, so the count is exact.
+    * Shift: its population-control bias (shift - E0 > 0 on average) falls as 1 / N_w: at a
+      quarter of the walkers the mean relative bias is 2 to 8 times larger.
+    """
+    B = gpu.batched
+    base = gpu.u_sweep_specs(256)
+    e0 = B.ground_state_energies(base)
+    nsteps, skip = 100_000, 20_000
+    print("u-sweep at its target walkers", flush=True)
+    st, h = _sweep(gpu, base, nsteps, 20_000)
+    assert (st == 0).all()
+    rows = B.energy_check(base, h, st, e0, skip=skip)
+    short = B.energy_check(base, {k: v[:10_000] for k, v in h.items()}, st, e0, skip=2_500)
+    del h
+    proj = sum(r["projected_2sigma"] for r in rows)
+    print(f"projected within 2 sigma: {proj}/256 over 1e5 steps, "
+          f"{sum(r['projected_2sigma'] for r in short)}/256 over the first 1e4", flush=True)
+    assert proj >= 234
+    bias1 = np.mean([(r["shift"] - r["E0"]) / abs(r["E0"]) for r in rows])
+    quarter = [gpu.RunSpec(s.hamiltonian, target_walkers=s.target_walkers / 4, dtau=s.dtau, seed=s.seed)
+               for s in base]
+    print("u-sweep at a quarter of the walkers", flush=True)
+    st4, h4 = _sweep(gpu, quarter, nsteps, 20_000)
+    assert (st4 == 0).all()
+    rows4 = B.energy_check(quarter, h4, st4, e0, skip=skip)
+    bias4 = np.mean([(r["shift"] - r["E0"]) / abs(r["E0"]) for r in rows4])
+    print(f"mean relative shift bias: {bias1:.3e} at the targets, {bias4:.3e} at a quarter", flush=True)
+    assert 0 < bias1 < 2e-3
+    assert 2 < bias4 / bias1 < 8
