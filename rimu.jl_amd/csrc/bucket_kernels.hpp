@@ -41,13 +41,14 @@ constexpr int FLAG_BUCKET_OVERFLOW = 16;
 // First record and first vector entry of every bucket: rstart[b] = lower bound of b among
 // the records' top T bits (records sorted on them), vstart[b] likewise in the vector;
 // b = 0 .. 2^T (the last is S, resp. L).
-__global__ __launch_bounds__(NT) void bucket_bounds_kernel(const uint64_t* __restrict__ ak, int64_t S,
+__global__ __launch_bounds__(NT) void bucket_bounds_kernel(const uint64_t* __restrict__ ak, Count cS,
                                                            const uint64_t* __restrict__ bk, int64_t L,
                                                            int shift, int64_t nbuckets,
                                                            int64_t* __restrict__ rstart,
                                                            int64_t* __restrict__ vstart) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i > 2 * nbuckets + 1) return;
+    const int64_t S = min(count_value(cS), cS.cap);  // more records raise FLAG_SPEC earlier
     const bool rec = i <= nbuckets;
     const int64_t b = rec ? i : i - nbuckets - 1;
     const uint64_t* k = rec ? ak : bk;
@@ -97,7 +98,8 @@ __global__ __launch_bounds__(BK_NT) void bucket_merge_kernel(StepArgs a, const u
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t r0 = rstart[b], v0 = vstart[b];
     const int na = (int)(rstart[b + 1] - r0), nb = (int)(vstart[b + 1] - v0);
-    const bool failed_sort = (*flags & FLAG_LOOKBACK) != 0;  // its output is not an order to merge
+    // no sorted list to merge (raised by earlier kernels only: every wave reads the same bits)
+    const bool failed_sort = (*flags & (FLAG_LOOKBACK | FLAG_SPEC)) != 0;
     if (failed_sort || na > BK_CAP_A || nb > BK_CAP_B) {     // uniform over the workgroup
         if (threadIdx.x == 0) {
             if (!failed_sort) atomicOr(flags, FLAG_BUCKET_OVERFLOW);

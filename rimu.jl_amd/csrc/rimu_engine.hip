@@ -123,6 +123,8 @@ struct rimu_engine {
     DevBuf<uint64_t> lb_status;     // look-back radix: status words [pass][tile][digit]
     uint32_t lb_epoch = 0;          // sort calls since lb_status was last cleared
     int bucket_cooldown = 0;        // steps left on the merge path after a bucket overflow
+    bool spec_ok = false;           // the grids of the next step may be sized from W_prev, S_prev
+    int64_t W_prev = 0, S_prev = 0; // spawn attempts and records of the last step
     uint32_t lb_spin_limit = LB_SPIN_LIMIT;  // bound of every look-back wait
     int lb_withhold = 0;            // test of the give-up path (rimu_engine_set_lookback)
     DevBuf<uint32_t> lb_hist;       // look-back radix: digit bases [pass][digit], tile counters
@@ -304,13 +306,15 @@ int scan_u32(rimu_engine* e, const uint32_t* in, uint32_t* out, int64_t n) {
 // (the initiator class bit).  Input in buffer `in`; returns the buffer index holding the
 // result.
 template <class K, class V>
-int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo);
+int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo,
+                  const int64_t* n_dev);
 
 template <class K, class V>
-int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo = 0) {
+int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo = 0,
+               const int64_t* n_dev = nullptr) {
     int src = in;
-    if (n <= 1) { *result = src; return RIMU_OK; }
-    if (e->radix_mode != 0) return radix_sort_lb<K, V>(e, n, nbits, varying, in, result, lo);
+    if (n <= 1 && !n_dev) { *result = src; return RIMU_OK; }
+    if (e->radix_mode != 0 || n_dev) return radix_sort_lb<K, V>(e, n, nbits, varying, in, result, lo, n_dev);
     const int64_t tiles = ceil_div(n, sort_tile<K>::value);
     const int64_t nchunks = ceil_div(tiles, RADIX_CHUNK);
     HIP_TRY(e->rcounts.alloc((size_t)tiles * RADIX));
@@ -343,9 +347,11 @@ int radix_sort(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* 
 // The same sort with one histogram pass over the keys for every digit and one look-back
 // scatter per pass (no per-pass count or scan kernels).  The output is that of the per-pass
 // path: the ranks within a tile and the digit offsets are the same numbers.  FLAG_LOOKBACK
-// in the step's flags reports a look-back that did not complete.
+// in the step's flags reports a look-back that did not complete.  With n_dev the record
+// count is in device memory and n is the most the grids are sized for.
 template <class K, class V>
-int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo) {
+int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, int* result, int lo,
+                  const int64_t* n_dev) {
     int src = in;
     PassSet ps{};
     for (int shift = lo; shift < nbits; shift += RADIX_BITS) {
@@ -376,13 +382,13 @@ int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, in
     HIP_TRY(e->lb_part.alloc((size_t)HIST_ROWS * ps.n * RADIX));
     HIP_TRY(hipMemsetAsync(e->lb_part.p, 0, sizeof(uint32_t) * HIST_ROWS * ps.n * RADIX, e->stream));
     radix_hist_all_kernel<K><<<hblocks, NT, sizeof(uint32_t) * ps.n * copies * HIST_STRIDE, e->stream>>>(
-        (const K*)e->skeys[src].p, n, ps, copies, e->lb_part.p);
+        (const K*)e->skeys[src].p, Count{n, n_dev, n}, flags, ps, copies, e->lb_part.p);
     radix_hist_base_kernel<<<ps.n, RADIX, 0, e->stream>>>(e->lb_part.p, ps.n, e->lb_hist.p);
     e->timer.end(e->stream);
     for (int p = 0; p < ps.n; ++p) {
         e->timer.begin("radix_scatter", e->stream, &ev);
         radix_scatter_lb_kernel<K, V><<<tiles, NT, 0, e->stream>>>(
-            (const K*)e->skeys[src].p, (const V*)e->svals[src].p, n, ps.shift[p], ps.dmask[p],
+            (const K*)e->skeys[src].p, (const V*)e->svals[src].p, Count{n, n_dev, n}, ps.shift[p], ps.dmask[p],
             e->lb_hist.p + (int64_t)p * RADIX, e->lb_status.p + (int64_t)p * tiles * RADIX, epoch, counters + p,
             flags, LookbackCtl{e->lb_spin_limit, e->lb_withhold},
             (K*)e->skeys[1 - src].p, (V*)e->svals[1 - src].p);
@@ -394,28 +400,39 @@ int radix_sort_lb(rimu_engine* e, int64_t n, int nbits, u128 varying, int in, in
     return RIMU_OK;
 }
 
+__global__ void clear_flags_kernel(int* flags, int bits) { atomicAnd(flags, ~bits); }
+
 // Sort of mixed keys: radix passes on the top MIX_SORT_BITS bits, then the runs that agree
 // there but not below are put in order (fixup kernels).  If a run is too long or there are
 // too many, the passes below the top bits and the top passes again make the order complete
 // (stable LSD passes compose, so equal keys keep their list order either way).
 template <class K, class V>
-int sort_mixed(rimu_engine* e, int64_t n, int nbits, u128 varying, bool init, rimu_engine::Misc* dm, int* result) {
+int sort_mixed(rimu_engine* e, int64_t n, int nbits, u128 varying, bool init, rimu_engine::Misc* dm, int* result,
+               const int64_t* n_dev = nullptr) {
     const int lo = nbits - e->sort_bits;
     int sorted = 0;
-    if (int rc = radix_sort<K, V>(e, n, nbits, varying, 0, &sorted, lo)) return rc;
+    if (int rc = radix_sort<K, V>(e, n, nbits, varying, 0, &sorted, lo, n_dev)) return rc;
     const K kmask = init ? (K)~class_bit<K>() : (K)~(K)0;
     HIP_TRY(e->fix_runs.alloc(FIXUP_LIST));
     hipEvent_t ev{};
     e->timer.begin("sort_fixup", e->stream, &ev);
-    fixup_find_kernel<K><<<ceil_div(n, NT), NT, 0, e->stream>>>((const K*)e->skeys[sorted].p, n, lo, kmask,
-                                                                e->fix_runs.p, &dm->fix_count, &dm->flags);
-    fixup_sort_kernel<K, V><<<64, NT, 0, e->stream>>>((K*)e->skeys[sorted].p, (V*)e->svals[sorted].p, n, lo,
-                                                      kmask, e->fix_runs.p, &dm->fix_count, &dm->flags);
+    fixup_find_kernel<K><<<ceil_div(n, NT), NT, 0, e->stream>>>((const K*)e->skeys[sorted].p, Count{n, n_dev, n},
+                                                                lo, kmask, e->fix_runs.p, &dm->fix_count, &dm->flags);
+    fixup_sort_kernel<K, V><<<64, NT, 0, e->stream>>>((K*)e->skeys[sorted].p, (V*)e->svals[sorted].p,
+                                                      Count{n, n_dev, n}, lo, kmask, e->fix_runs.p,
+                                                      &dm->fix_count, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
+    if (n_dev) {  // counts in device memory: a fallback is taken by repeating the step
+        *result = sorted;
+        return RIMU_OK;
+    }
     HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->h_misc->flags & FLAG_FIXUP_FALLBACK) {
+        // the full sort below orders the runs: the merge must not read the flag as a failure
+        clear_flags_kernel<<<1, 1, 0, e->stream>>>(&dm->flags, FLAG_FIXUP_FALLBACK);
+        HIP_TRY(hipGetLastError());
         int low = 0;
         if (int rc = radix_sort<K, V>(e, n, lo, varying, sorted, &low)) return rc;
         if (int rc = radix_sort<K, V>(e, n, nbits, varying, low, &sorted, lo)) return rc;
@@ -432,7 +449,7 @@ int sort_mixed(rimu_engine* e, int64_t n, int nbits, u128 varying, bool init, ri
 template <class K, class V>
 int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int64_t S,
                const K* bk, const V* bv, int64_t L, K* ok, V* ov, int64_t cap,
-               rimu_engine::Misc* dm, int64_t* tiles, const V* bo = nullptr) {
+               rimu_engine::Misc* dm, int64_t* tiles, const V* bo = nullptr, const int64_t* S_dev = nullptr) {
     constexpr int KW = sizeof(K) / 8;
     const bool init = a.init_rule != 0;
     if (init && !bo) return fail(RIMU_ERR_INTERNAL, "initiator merge without the old values");
@@ -447,15 +464,15 @@ int merge_into(rimu_engine* e, const StepArgs& a, const K* ak, const V* av, int6
     HIP_TRY(e->slot_v.alloc((size_t)nt * TILE));
     HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nt, 0, sizeof(int64_t), e->stream));
     merge_split_kernel<K><<<ceil_div(nt + 1, NT), NT, 0, e->stream>>>(
-        ak, S, bk, L, TILE, nt, init ? (K)~class_bit<K>() : (K)~(K)0, e->splits.p, (K*)e->prevk.p);
+        ak, Count{S, S_dev, S}, bk, L, TILE, nt, init ? (K)~class_bit<K>() : (K)~(K)0, e->splits.p, (K*)e->prevk.p);
     hipEvent_t ev{};
     e->timer.begin("merge", e->stream, &ev);
     if (init)
-        merge_kernel<K, V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
+        merge_kernel<K, V, true><<<nt, NT, 0, e->stream>>>(a, ak, av, Count{S, S_dev, S}, bk, bv, L, (K*)e->slot_k.p,
                                                            (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
                                                            e->splits.p, (const K*)e->prevk.p, bo, &dm->flags);
     else
-        merge_kernel<K, V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, S, bk, bv, L, (K*)e->slot_k.p,
+        merge_kernel<K, V, false><<<nt, NT, 0, e->stream>>>(a, ak, av, Count{S, S_dev, S}, bk, bv, L, (K*)e->slot_k.p,
                                                             (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
                                                             e->splits.p, (const K*)e->prevk.p, nullptr,
                                                             &dm->flags);
@@ -532,7 +549,7 @@ int bucket_bits(int64_t S, int64_t L, int B) {
 template <class V>
 int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
                       const uint64_t* bk, const V* bv, int64_t L, int T, uint64_t* ok, V* ov, int64_t cap,
-                      rimu_engine::Misc* dm, int64_t* tiles) {
+                      rimu_engine::Misc* dm, int64_t* tiles, const int64_t* S_dev = nullptr) {
     const int B = e->hdev.B;
     const int64_t nbk = (int64_t)1 << T;
     *tiles = nbk;
@@ -544,7 +561,7 @@ int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, con
     HIP_TRY(e->slot_v.alloc((size_t)(S + L)));
     hipEvent_t ev{};
     e->timer.begin("bucket_bounds", e->stream, &ev);
-    bucket_bounds_kernel<<<ceil_div(2 * nbk + 2, NT), NT, 0, e->stream>>>(ak, S, bk, L, B - T, nbk,
+    bucket_bounds_kernel<<<ceil_div(2 * nbk + 2, NT), NT, 0, e->stream>>>(ak, Count{S, S_dev, S}, bk, L, B - T, nbk,
                                                                           e->bk_rstart.p, e->bk_vstart.p);
     e->timer.end(e->stream);
     HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nbk, 0, sizeof(int64_t), e->stream));
@@ -589,14 +606,33 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
 
     HIP_TRY(hipMemsetAsync(dm, 0, sizeof(rimu_engine::Misc), e->stream));
 
+    // Grids of the spawn, sort and merge kernels: sized from the counts of this step, read back
+    // after the plan and after the spawn (synchronous step), or -- once a step has run -- from
+    // the last step's counts with headroom, the kernels reading this step's counts from device
+    // memory (one host synchronisation per step).  A count beyond its grid raises FLAG_SPEC
+    // and the step is repeated synchronously.  The sorts with a look-back are needed for it.
+    const bool rs = e->hdev.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
+    const bool dense = !e->int_vals && e->style.proj_threshold == 0 && (!rs || !rs_zero_channels(e->hdev));
+    const int nbits = e->hdev.B;
+    const bool bucket_ok = KW == 1 && e->radix_mode == 2 && a.init_rule == 0 && e->mix.on && e->bucket_cooldown == 0;
+    bool spec = e->spec_ok && e->radix_mode != 0 && e->W_prev > 0;
+    int T_spec = 0;
+    if (spec) {  // the path the counts of the last step choose; sorts with a host check stay synchronous
+        if (bucket_ok && e->S_prev >= BK_MIN_RECORDS) {
+            T_spec = bucket_bits(e->S_prev, L, nbits);
+            if (nbits - T_spec > BK_MAX_LOWBITS) T_spec = 0;
+        }
+    }
+    const int64_t W_cap = spec ? e->W_prev + e->W_prev / 4 + 16 * SPAWN_TILE : 0;
+
     // 1. plan: diagonal step, attempt offsets and the first entry of each spawn tile (as
-    // many as the current tile_first buffer holds; the rest below if W needs more)
+    // many as the tile_first buffer holds; the rest below if W needs more)
     const int64_t plan_tiles = ceil_div(L + 1, PLAN_TILE);
     HIP_TRY(e->plan_p.alloc((size_t)plan_tiles * 6));
     HIP_TRY(e->plan_status.alloc(plan_tiles));
     HIP_TRY(hipMemsetAsync(e->plan_status.p, 0, sizeof(uint64_t) * plan_tiles, e->stream));
+    if (spec) HIP_TRY(e->tile_first.alloc(ceil_div(W_cap, SPAWN_TILE) + 1));
     const int64_t tf_cap = (int64_t)e->tile_first.n;
-    const bool rs = e->hdev.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
     e->timer.begin("plan", e->stream, &ev);
     if (rs)
         plan_kernel<K, V, true><<<plan_tiles, NT, 0, e->stream>>>(a, vk, vv, L, (V*)e->dvals.p, e->woff.p,
@@ -608,10 +644,13 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
                                                                 &dm->plan_counter, &dm->flags, e->plan_p.p, &dm->W);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "plan look-back did not complete");
-    const int64_t W = e->h_misc->W;
+    int64_t W = W_cap;  // synchronous: the count; speculative: the most the grids are sized for
+    if (!spec) {
+        HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (e->h_misc->flags & FLAG_LOOKBACK) return fail(RIMU_ERR_INTERNAL, "plan look-back did not complete");
+        W = e->h_misc->W;
+    }
 
     // 2. spawn (W = number of attempts).  Dense spawning (every attempt yields a record)
     // writes the record of attempt a at position a; otherwise tiles write to their own slot
@@ -619,13 +658,13 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     int64_t S = 0;
     int64_t spawn_tiles = 0;
     u128 kor = 0, kand = ~(u128)0;
+    const int64_t* S_dev = nullptr;  // speculative: the record count in device memory
     if (W > 0) {
         spawn_tiles = ceil_div(W, SPAWN_TILE);
         // every attempt yields a record when values are floating and deposits unprojected,
         // unless the model has zero-valued channels (hard walls, Pauli blocking: the
         // reference never deposits their spawns)
-        const bool dense = !e->int_vals && e->style.proj_threshold == 0 && (!rs || !rs_zero_channels(e->hdev));
-        if (dense && W > e->max_spawns) {
+        if (!spec && dense && W > e->max_spawns) {
             out->overflow = 1;
             return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
         }
@@ -646,23 +685,24 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
             sk = (K*)e->slot_k.p;
             sv = (V*)e->slot_v.p;
         }
+        const Count cA{W, spec ? &dm->W : nullptr, spawn_tiles * SPAWN_TILE};
         e->timer.begin("spawn", e->stream, &ev);
         if (dense && rs)
             spawn_kernel<K, V, true, true><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                (K*)e->tile_or.p, (K*)e->tile_and.p);
+                a, vk, vv, L, e->woff.p, cA, e->max_spawns, &dm->flags, sk, sv, e->tile_cnt.p, e->spawn_p.p,
+                e->tile_first.p, (K*)e->tile_or.p, (K*)e->tile_and.p);
         else if (dense)
             spawn_kernel<K, V, true, false><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                (K*)e->tile_or.p, (K*)e->tile_and.p);
+                a, vk, vv, L, e->woff.p, cA, e->max_spawns, &dm->flags, sk, sv, e->tile_cnt.p, e->spawn_p.p,
+                e->tile_first.p, (K*)e->tile_or.p, (K*)e->tile_and.p);
         else if (rs)
             spawn_kernel<K, V, false, true><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                (K*)e->tile_or.p, (K*)e->tile_and.p);
+                a, vk, vv, L, e->woff.p, cA, e->max_spawns, &dm->flags, sk, sv, e->tile_cnt.p, e->spawn_p.p,
+                e->tile_first.p, (K*)e->tile_or.p, (K*)e->tile_and.p);
         else
             spawn_kernel<K, V, false, false><<<spawn_tiles, NT, 0, e->stream>>>(
-                a, vk, vv, L, e->woff.p, W, sk, sv, e->tile_cnt.p, e->spawn_p.p, e->tile_first.p,
-                (K*)e->tile_or.p, (K*)e->tile_and.p);
+                a, vk, vv, L, e->woff.p, cA, e->max_spawns, &dm->flags, sk, sv, e->tile_cnt.p, e->spawn_p.p,
+                e->tile_first.p, (K*)e->tile_or.p, (K*)e->tile_and.p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
         key_bits_kernel<K><<<1, 1024, 0, e->stream>>>((const K*)e->tile_or.p, (const K*)e->tile_and.p,
@@ -676,17 +716,26 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
             e->timer.end(e->stream);
         }
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
-        HIP_TRY(hipStreamSynchronize(e->stream));
-        if (e->h_misc->flags & FLAG_SPAWN_OVERFLOW) {
-            e->timer.collect();
-            out->overflow = 1;
-            return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
+        if (spec) {
+            // at most W_cap records (one per attempt), at most max_spawns in the buffer; every
+            // digit over the bucket or key bits is sorted (no OR/AND read back)
+            S = std::min(W, e->max_spawns);
+            S_dev = dense ? &dm->W : &dm->S;
+            kor = nbits >= 128 ? ~(u128)0 : (((u128)1 << nbits) - 1);
+            kand = 0;
+        } else {
+            HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            if (e->h_misc->flags & FLAG_SPAWN_OVERFLOW) {
+                e->timer.collect();
+                out->overflow = 1;
+                return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
+            }
+            S = dense ? W : e->h_misc->S;
+            kor = e->h_misc->key_or;
+            kand = e->h_misc->key_and;
+            if (KW == 1) { kor = (uint64_t)kor; kand = (uint64_t)kand; }
         }
-        S = dense ? W : e->h_misc->S;
-        kor = e->h_misc->key_or;
-        kand = e->h_misc->key_and;
-        if (KW == 1) { kor = (uint64_t)kor; kand = (uint64_t)kand; }
     }
 
     // 3-4. sort the spawn records, merge them with the diagonal survivors, fold, compress,
@@ -694,32 +743,31 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     // bucket outgrows its LDS tile, the records are sorted on the remaining bits and merged
     // along the merge path instead (same result).
     const int nxt = 1 - e->cur;
-    const int nbits = e->hdev.B;
     int64_t merge_tiles = 0;
     int sorted = 0;
-    int T = 0;
-    if (KW == 1 && e->radix_mode == 2 && a.init_rule == 0 && e->mix.on && S >= BK_MIN_RECORDS &&
-        e->bucket_cooldown == 0) {
+    int T = T_spec;
+    if (!spec && bucket_ok && S >= BK_MIN_RECORDS) {
         T = bucket_bits(S, L, nbits);
         if (nbits - T > BK_MAX_LOWBITS) T = 0;
     }
     if (e->bucket_cooldown > 0) --e->bucket_cooldown;
+    const int64_t* merge_S_dev = S_dev;  // the merge path after the fallback sort reads S from the host
     auto merge_path = [&]() -> int {
         return merge_into<K, V>(e, a, (const K*)e->skeys[sorted].p, (const V*)e->svals[sorted].p, S, vk,
                                 (const V*)e->dvals.p, L, (K*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p,
-                                e->max_entries, dm, &merge_tiles, vv);
+                                e->max_entries, dm, &merge_tiles, vv, merge_S_dev);
     };
     if (T > 0) {
-        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted, nbits - T)) return rc;
+        if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted, nbits - T, S_dev)) return rc;
         if (int rc = bucket_merge_into<V>(e, a, (const uint64_t*)e->skeys[sorted].p, (const V*)e->svals[sorted].p,
                                           S, (const uint64_t*)vk, (const V*)e->dvals.p, L, T,
                                           (uint64_t*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, dm,
-                                          &merge_tiles)) return rc;
+                                          &merge_tiles, S_dev)) return rc;
     } else {
-        if (S > 1) {
+        if (S > 1 || S_dev) {
             if (!e->mix.on || nbits <= e->sort_bits) {
-                if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted)) return rc;
-            } else if (int rc = sort_mixed<K, V>(e, S, nbits, kor ^ kand, a.init_rule != 0, dm, &sorted)) {
+                if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted, 0, S_dev)) return rc;
+            } else if (int rc = sort_mixed<K, V>(e, S, nbits, kor ^ kand, a.init_rule != 0, dm, &sorted, S_dev)) {
                 return rc;
             }
         }
@@ -761,6 +809,20 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         return RIMU_OK;
     };
     if (int rc = finish()) return rc;
+    if (spec && (e->h_misc->flags & (FLAG_SPEC | FLAG_FIXUP_FALLBACK))) {
+        // a count outgrew the grids sized from the last step, or the runs left after a sort on
+        // the top bits need the full sort: repeat the step synchronously (the vector is
+        // unchanged; same result)
+        e->timer.begin("spec_repeat", e->stream, &ev);
+        e->timer.end(e->stream);
+        e->spec_ok = false;
+        return step_impl<K, V>(e, step, shift, dtau, boost, out);
+    }
+    if (e->h_misc->flags & FLAG_SPAWN_OVERFLOW) {
+        out->overflow = 1;
+        e->spec_ok = false;
+        return fail(RIMU_ERR_CAPACITY, "spawn buffer capacity exceeded");
+    }
     if ((e->h_misc->flags & FLAG_LOOKBACK) && e->radix_mode != 0) {
         // a radix look-back wait gave up (only forced from the host: every tile waited on has
         // started): the sort buffers hold no list any more, the vector is unchanged; repeat the
@@ -781,6 +843,10 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         e->timer.begin("bucket_fallback", e->stream, &ev);
         e->timer.end(e->stream);
         HIP_TRY(hipMemsetAsync(&dm->flags, 0, sizeof(int), e->stream));
+        if (spec) {  // the counts are known now
+            S = dense ? e->h_misc->W : e->h_misc->S;
+            merge_S_dev = nullptr;
+        }
         int full = sorted;
         if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, sorted, &full)) return rc;
         sorted = full;
@@ -808,9 +874,14 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     const int64_t L_new = e->h_misc->L_new;
     e->cur = nxt;
     e->len = L_new;
+    // the counts that size the next step's grids
+    e->W_prev = e->h_misc->W;
+    e->S_prev = dense ? e->h_misc->W : e->h_misc->S;
+    e->spec_ok = true;
     // algorithmic bytes: read v, write S records, read S records into the merge, write v'
     const double rec = 8.0 * KW + (double)e->vsize;
-    const double records = (double)(S + L);  // diagonal survivors included
+    const int64_t S_act = dense ? e->h_misc->W : e->h_misc->S;  // records this step emitted
+    const double records = (double)(S_act + L);  // diagonal survivors included
     e->last_records = records;
     e->last_alg_bytes = (double)L * rec + 2.0 * records * rec + (double)L_new * rec;
     return RIMU_OK;

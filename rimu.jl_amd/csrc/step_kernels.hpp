@@ -42,7 +42,19 @@ template <class K> struct sort_ipt { static constexpr int value = sizeof(K) == 8
 template <class K> struct sort_tile { static constexpr int value = NT * sort_ipt<K>::value; };
 static_assert(RADIX == NT, "radix kernels use one thread per digit");
 
-enum { FLAG_SPAWN_OVERFLOW = 1, FLAG_LOOKBACK = 2, FLAG_VECTOR_OVERFLOW = 4, FLAG_FIXUP_FALLBACK = 8 };
+enum { FLAG_SPAWN_OVERFLOW = 1, FLAG_LOOKBACK = 2, FLAG_VECTOR_OVERFLOW = 4, FLAG_FIXUP_FALLBACK = 8,
+       FLAG_SPEC = 32 };
+
+// A count a kernel works on: known on the host (dev == nullptr), or written to device memory
+// by an earlier kernel of the same step, with the grid sized for at most cap (a larger count
+// raises FLAG_SPEC: the kernels after it do nothing and the host repeats the step with grids
+// sized from the counts it has read back).
+struct Count {
+    int64_t n;
+    const int64_t* dev;
+    int64_t cap;
+};
+__device__ inline int64_t count_value(const Count& c) { return c.dev ? *c.dev : c.n; }
 
 struct StepArgs {
     ModelDev h;
@@ -318,7 +330,8 @@ static_assert(SPAWN_TILE == SPAWN_TILE_SIZE, "plan and spawn agree on the spawn 
 template <class K, class V, bool DENSE, bool RS>
 __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restrict__ keys,
                                                    const V* __restrict__ vals, int64_t L,
-                                                   const int64_t* __restrict__ aoff, int64_t A,
+                                                   const int64_t* __restrict__ aoff, Count cA,
+                                                   int64_t out_cap, int* flags,
                                                    K* __restrict__ out_keys,
                                                    V* __restrict__ out_vals,
                                                    int64_t* __restrict__ tile_cnt,
@@ -340,6 +353,22 @@ __global__ __launch_bounds__(NT) void spawn_kernel(StepArgs a, const K* __restri
     const int wid = threadIdx.x / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t a0 = (int64_t)tile * SPAWN_TILE;
+    const int64_t A = count_value(cA);
+    // attempts beyond the grid (FLAG_SPEC) or, dense, beyond the spawn buffer: nothing is
+    // written; tiles past the attempts leave neutral partials (uniform branches)
+    const bool spec_miss = A > cA.cap;
+    const bool no_room = DENSE && A > out_cap;
+    if (spec_miss || no_room || a0 >= A) {
+        if (threadIdx.x == 0) {
+            if (tile == 0 && spec_miss) atomicOr(flags, FLAG_SPEC);
+            if (tile == 0 && no_room) atomicOr(flags, FLAG_SPAWN_OVERFLOW);
+            partials[tile] = 0;
+            tile_or[tile] = 0;
+            tile_and[tile] = (K)~(K)0;
+            if (!DENSE) tile_cnt[tile] = 0;
+        }
+        return;
+    }
     const int64_t e0 = tile_first[tile];
     {
         constexpr int WR = (WIN + NT - 1) / NT;
@@ -541,10 +570,12 @@ constexpr int64_t FIXUP_LIST = 1 << 20;
 // Adjacent records i, i+1 that share bits [lo, B) but differ below: the start of the run
 // holding them is listed once (by the first such pair of the run).
 template <class K>
-__global__ __launch_bounds__(NT) void fixup_find_kernel(const K* __restrict__ keys, int64_t n, int lo,
+__global__ __launch_bounds__(NT) void fixup_find_kernel(const K* __restrict__ keys, Count cn, int lo,
                                                         K kmask, int64_t* __restrict__ runs,
                                                         uint32_t* count, int* flags) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (*flags & FLAG_SPEC) return;  // raised by earlier kernels only
+    const int64_t n = count_value(cn);
     if (i + 1 >= n) return;
     const K a = keys[i] & kmask, b = keys[i + 1] & kmask;
     if (a == b || ((a ^ b) >> lo) != 0) return;
@@ -561,9 +592,11 @@ __global__ __launch_bounds__(NT) void fixup_find_kernel(const K* __restrict__ ke
 
 // One thread per listed run: stable insertion sort of [s, end of run) on the masked key.
 template <class K, class V>
-__global__ __launch_bounds__(NT) void fixup_sort_kernel(K* __restrict__ keys, V* __restrict__ vals, int64_t n,
+__global__ __launch_bounds__(NT) void fixup_sort_kernel(K* __restrict__ keys, V* __restrict__ vals, Count cn,
                                                         int lo, K kmask, const int64_t* __restrict__ runs,
                                                         const uint32_t* count, int* flags) {
+    if (*flags & FLAG_SPEC) return;  // raised by earlier kernels only
+    const int64_t n = count_value(cn);
     const uint32_t nr = min(*count, (uint32_t)FIXUP_LIST);
     for (uint32_t r = blockIdx.x * NT + threadIdx.x; r < nr; r += gridDim.x * NT) {
         const int64_t s = runs[r];
@@ -600,7 +633,7 @@ __global__ __launch_bounds__(NT) void tile_first_kernel(const int64_t* __restric
 // elements among the first d merged elements (B before A on equal keys).
 // kmask clears the initiator class bit of the spawn keys (all ones without initiators).
 template <class K>
-__global__ __launch_bounds__(NT) void merge_split_kernel(const K* __restrict__ ak, int64_t S,
+__global__ __launch_bounds__(NT) void merge_split_kernel(const K* __restrict__ ak, Count cS,
                                                          const K* __restrict__ bk, int64_t L,
                                                          int64_t tile_size, int64_t ntiles,
                                                          K kmask,
@@ -608,6 +641,7 @@ __global__ __launch_bounds__(NT) void merge_split_kernel(const K* __restrict__ a
                                                          K* __restrict__ prevk) {
     const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (t > ntiles) return;
+    const int64_t S = min(count_value(cS), cS.cap);  // more records raise FLAG_SPEC earlier
     const int64_t d = min(t * tile_size, S + L);
     int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
     while (lo < hi) {
@@ -871,11 +905,17 @@ __host__ __device__ inline int hist_all_copies(int npasses) { return npasses <= 
 constexpr int HIST_ALL_BLOCKS = 2048;
 constexpr int HIST_ROWS = 64;
 template <class K>
-__global__ __launch_bounds__(NT) void radix_hist_all_kernel(const K* __restrict__ keys, int64_t n,
+__global__ __launch_bounds__(NT) void radix_hist_all_kernel(const K* __restrict__ keys, Count cn, int* flags,
                                                             PassSet ps, int copies, uint32_t* __restrict__ part) {
     constexpr int IPT = sort_ipt<K>::value;
     constexpr int TILE = sort_tile<K>::value;
     extern __shared__ uint32_t h_all[];
+    // every thread computes the same n: the early return is uniform
+    const int64_t n = count_value(cn);
+    if (n > cn.cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, FLAG_SPEC);
+        return;
+    }
     const int words = ps.n * copies * HIST_STRIDE;
     for (int j = threadIdx.x; j < words; j += NT) h_all[j] = 0;
     __syncthreads();
@@ -919,7 +959,7 @@ __global__ __launch_bounds__(RADIX) void radix_hist_base_kernel(const uint32_t* 
 
 template <class K, class V>
 __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restrict__ keys_in,
-                                                              const V* __restrict__ vals_in, int64_t n,
+                                                              const V* __restrict__ vals_in, Count cn,
                                                               int shift, uint32_t dmask,
                                                               const uint32_t* __restrict__ gbase,
                                                               uint64_t* status, uint32_t epoch,
@@ -940,7 +980,11 @@ __global__ __launch_bounds__(NT) void radix_scatter_lb_kernel(const K* __restric
     // tile ids in start order from a counter: every tile a tile waits for has started,
     // whatever order the workgroups are dispatched in (also measured faster than tile =
     // workgroup index: 0.70 against 0.755 ms per pass of 7e7 one-word records, same GPU)
+    // FLAG_SPEC is raised only by earlier kernels, so every wave reads the same bit here
+    if (*flags & FLAG_SPEC) return;
+    const int64_t n = count_value(cn);
     const uint32_t tile = next_tile_id(tile_counter);
+    if ((int64_t)tile * TILE >= n) return;  // past the records (ids of later tiles are larger)
     const uint64_t tag = (uint64_t)epoch << 32;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
@@ -1088,7 +1132,7 @@ __device__ inline V initiator_value(const StepArgs& a, V s, V u, V i) {
 // NoCompression keeps even when it is zero (stochasticstyles.jl:116-121).
 template <class K, class V, bool INIT>
 __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restrict__ ak,
-                                                   const V* __restrict__ av, int64_t S,
+                                                   const V* __restrict__ av, Count cS,
                                                    const K* __restrict__ bk,
                                                    const V* __restrict__ bv, int64_t L,
                                                    K* __restrict__ slot_k, V* __restrict__ slot_v,
@@ -1105,13 +1149,15 @@ __global__ __launch_bounds__(NT) void merge_kernel(StepArgs a, const K* __restri
     __shared__ V s_val[TILE + MERGE_EXTRA];
 
     const int64_t tile = blockIdx.x;
-    if (*flags & FLAG_LOOKBACK) {  // the sort failed: its output is not an order to merge
+    // the flags are raised by earlier kernels only: every wave reads the same bits
+    if (*flags & (FLAG_LOOKBACK | FLAG_SPEC | FLAG_FIXUP_FALLBACK)) {  // no sorted list to merge
         if (threadIdx.x == 0) {
             tile_cnt[tile] = 0;
             for (int j = 0; j < 3; ++j) partials[(int64_t)j * gridDim.x + tile] = 0;
         }
         return;
     }
+    const int64_t S = count_value(cS);
     const int64_t total = S + L;
     const int64_t d0 = tile * TILE;
     const int64_t d1 = min(d0 + TILE, total);
@@ -1391,8 +1437,11 @@ __global__ __launch_bounds__(NT) void project_kernel(const K* __restrict__ keys,
 
 // ---------------------------------------------------------------------------------------
 // Statistics: per-tile partial sums are reduced in two deterministic levels.  Level one
-// (grid PARTIAL_BLOCKS x 12) sums fixed contiguous chunks of each partial array; level
-// two (one workgroup) adds the chunk sums in order.
+// (grid PARTIAL_BLOCKS x 12) deals each partial array out in rows of NT: row r goes to
+// workgroup r mod PARTIAL_BLOCKS, element r*NT+i to its thread i; level two (one workgroup)
+// adds the chunk sums in order.  Where each element lands does not depend on the array's
+// length, so trailing zero partials (tiles of a grid sized for more work than came) add
+// exact zeros and leave every sum bit-identical to the one over the exact tile count.
 // out: [0] attempts [1] exact [2] inexact [3] spawns [4] deaths [5] clones [6] zombies
 //      [7] len_before [8] len [9] norm [10] vproj [11] hproj
 // ---------------------------------------------------------------------------------------
@@ -1407,9 +1456,9 @@ struct PartialSet {
 __global__ __launch_bounds__(NT) void partial_sums_kernel(PartialSet ps, double* __restrict__ chunk_sums) {
     const int j = blockIdx.y;
     const int64_t n = ps.n[j];
-    const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
+    const int64_t stride = (int64_t)gridDim.x * NT;
     double acc = 0;
-    for (int64_t t = lo + threadIdx.x; t < hi; t += NT) acc += ps.p[j][t];
+    for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < n; t += stride) acc += ps.p[j][t];
     const double s = block_sum<NT>(acc);
     if (threadIdx.x == 0) chunk_sums[(int64_t)j * gridDim.x + blockIdx.x] = s;
 }
