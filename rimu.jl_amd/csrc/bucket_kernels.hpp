@@ -5,16 +5,21 @@
 // The spawn records are radix-sorted (stable, look-back scatters of step_kernels.hpp) on the
 // top T bits only, which puts every address's records, in list order, into bucket
 // key >> (B - T) -- and the vector, kept in stored-key order, holds each bucket's entries as
-// one contiguous segment.  One workgroup per bucket then
-//   1. loads the bucket's records and the vector segment (keys, diagonal survivors),
-//   2. orders the records by key in LDS, equal keys in list order: one stable radix pass on the
-//      8 key bits below the bucket bits (ranks from wavefront ballots, as the global scatter
-//      passes do) makes 256 sub-buckets of a few records each, and every record then counts
-//      the smaller (key, list index) pairs of its sub-bucket,
-//   3. merges them with the segment along the merge path and folds every address left to
-//      right -- diagonal value first, then its spawns in list order, the order of deposit!
-//      over the list (Interfaces/dictvectors.jl:49-51, dvec.jl:129-136) -- applies
-//      ThresholdCompression (compression.jl:18-33) and counts len_before, len and the norm,
+// one contiguous segment.  A persistent grid of workgroups takes the buckets in a fixed
+// stride; for each bucket a workgroup
+//   1. holds the bucket's records and vector entries in registers (loaded while the previous
+//      bucket was being processed) and packs every item as (key bits below the bucket bits,
+//      tag): tag 0 for the vector entry, 1 + list index for a record -- so that sorting the
+//      packed words puts each address's diagonal survivor first and its spawns after it in
+//      list order, the order of deposit! over the list (Interfaces/dictvectors.jl:49-51,
+//      dvec.jl:129-136),
+//   2. sorts the packed words in LDS: LDS atomics drop every item into one of 2048
+//      sub-buckets (the top 11 key bits; arrival order inside a sub-bucket is arbitrary), and
+//      each item's place inside its sub-bucket is the number of smaller packed words there,
+//      which makes the order exact whatever the arrival order was,
+//   3. folds each address left to right (the first item of a run of equal keys walks the
+//      run), applies ThresholdCompression (compression.jl:18-33) and counts len_before, len
+//      and the norm,
 //   4. writes its survivors to its own slot range with their count; bucket_compact_kernel
 //      places them after a scan of the counts.
 // The result is that of the full sort and merge: same addresses, same fold order, same
@@ -27,14 +32,16 @@
 namespace rimu {
 
 constexpr int BK_NT = 512;                    // threads of a bucket workgroup
-constexpr int BK_CAP_A = 1536;                // spawn records per bucket in LDS
-constexpr int BK_CAP_B = 1024;                // vector entries per bucket in LDS
-constexpr int BK_RPT = BK_CAP_A / BK_NT;      // records per thread in the LDS sort
-constexpr int BK_IPT = (BK_CAP_A + BK_CAP_B) / BK_NT;  // merged positions per thread
-constexpr int BK_IDX_BITS = 11;               // list index packed below the key bits
-static_assert(BK_CAP_A <= (1 << BK_IDX_BITS), "list indices fit their bits");
-static_assert(BK_CAP_A % BK_NT == 0 && BK_CAP_B % BK_NT == 0 && (BK_CAP_A + BK_CAP_B) % BK_NT == 0,
-              "whole rounds per thread");
+constexpr int BK_CAP = 2560;                  // records + vector entries per bucket in LDS
+constexpr int BK_IPT = BK_CAP / BK_NT;        // items per thread
+constexpr int BK_TAG_BITS = 12;               // 0: vector entry; 1 + list index: record
+constexpr int BK_SUB_BITS = 11;               // sub-buckets of the LDS sort
+constexpr int BK_NSUB = 1 << BK_SUB_BITS;
+constexpr int BK_WG_PER_CU = 2;               // resident bucket workgroups per CU (LDS-bound)
+constexpr int BK_RANK_WIN = 2;                // sub-bucket words read at once
+constexpr int BK_FOLD_EXTRA = 4;              // positions read past a thread's own for its runs
+static_assert(BK_CAP < (1 << BK_TAG_BITS), "tags fit their bits");
+static_assert(BK_CAP % BK_NT == 0 && BK_NSUB % BK_NT == 0, "whole rounds per thread");
 static_assert(BK_IPT <= 32, "survivor masks are 32-bit");
 constexpr int FLAG_BUCKET_OVERFLOW = 16;
 
@@ -60,30 +67,30 @@ __global__ __launch_bounds__(NT) void bucket_bounds_kernel(const uint64_t* __res
     (rec ? rstart : vstart)[b] = lo;
 }
 
-// LDS of one bucket workgroup.  Records are sorted as packed words
-// (key bits below the bucket bits) << BK_IDX_BITS | list index, so that equal keys stay in
-// list order; their values stay in list order in rv.  After the merge walk, rk stages keys
-// and rv..kb stage values (contiguous: 2 * BK_CAP_A keys, BK_CAP_A + BK_CAP_B values).
+// LDS of one bucket workgroup.  `place` receives the packed words sub-bucket by sub-bucket,
+// `sk`/`sv` the packed words and values in order; after the fold `place` and `sv` stage the
+// survivors' keys and values.
 template <class V>
 struct BucketLds {
-    uint64_t rk[2][BK_CAP_A];
-    V rv[BK_CAP_A];
-    V vb[BK_CAP_B];                 // diagonal survivors of the bucket's vector entries
-    uint64_t kb[BK_CAP_B];          // the entries' keys (ascending)
-    uint32_t wcnt[BK_NT / WAVE][RADIX];
-    uint32_t start[RADIX + 1];
+    uint64_t place[BK_CAP];
+    uint64_t sk[BK_CAP];
+    V sv[BK_CAP];
+    uint32_t sub[BK_NSUB + 1];      // sub-bucket counts, then their starts
     double red[3][BK_NT / WAVE];
 };
-static_assert(2 * BK_CAP_A >= BK_CAP_A + BK_CAP_B, "staged keys fit the sort arrays");
 
+// Bucket b = blockIdx.x, blockIdx.x + gridDim.x, ...; nbk buckets of which rstart/vstart give
+// the record and vector ranges.  Outputs per bucket: survivors at slot[rstart[b] + vstart[b]
+// + i], their count in bucket_cnt[b], and partials[b], [nbk + b], [2 nbk + b] = len_before,
+// len, norm.
 template <class V>
-__global__ __launch_bounds__(BK_NT) void bucket_merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
+__global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
                                                              const V* __restrict__ av,
                                                              const uint64_t* __restrict__ bk,
                                                              const V* __restrict__ bv,
                                                              const int64_t* __restrict__ rstart,
                                                              const int64_t* __restrict__ vstart,
-                                                             int lowbits,
+                                                             int64_t nbk, int lowbits,
                                                              uint64_t* __restrict__ slot_k,
                                                              V* __restrict__ slot_v,
                                                              int64_t* __restrict__ bucket_cnt,
@@ -91,289 +98,281 @@ __global__ __launch_bounds__(BK_NT) void bucket_merge_kernel(StepArgs a, const u
                                                              int* flags) {
     constexpr int NW = BK_NT / WAVE;
     __shared__ BucketLds<V> sm;
-    const int64_t b = blockIdx.x;
-    const int64_t nbk = gridDim.x;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const int64_t r0 = rstart[b], v0 = vstart[b];
-    const int na = (int)(rstart[b + 1] - r0), nb = (int)(vstart[b + 1] - v0);
     // no sorted list to merge (raised by earlier kernels only: every wave reads the same bits)
-    const bool failed_sort = (*flags & (FLAG_LOOKBACK | FLAG_SPEC)) != 0;
-    if (failed_sort || na > BK_CAP_A || nb > BK_CAP_B) {     // uniform over the workgroup
-        if (threadIdx.x == 0) {
-            if (!failed_sort) atomicOr(flags, FLAG_BUCKET_OVERFLOW);
-            bucket_cnt[b] = 0;
-            partials[b] = 0;
-            partials[nbk + b] = 0;
-            partials[2 * nbk + b] = 0;
+    if (*flags & (FLAG_LOOKBACK | FLAG_SPEC)) {
+        for (int64_t b = blockIdx.x; b < nbk; b += gridDim.x) {
+            if (threadIdx.x == 0) {
+                bucket_cnt[b] = 0;
+                partials[b] = 0;
+                partials[nbk + b] = 0;
+                partials[2 * nbk + b] = 0;
+            }
         }
         return;
     }
     const uint64_t lowmask = lowbits >= 64 ? ~0ull : (1ull << lowbits) - 1ull;
-    const uint64_t bucket_key = lowbits >= 64 ? 0ull : (uint64_t)b << lowbits;
-    // ---- load: records (packed) and values, list order; the vector segment --------------
-    {
-        uint64_t rk[BK_RPT];
-        V rv[BK_RPT];
-        uint64_t qk[BK_CAP_B / BK_NT];
-        V qv[BK_CAP_B / BK_NT];
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int i = (int)threadIdx.x + r * BK_NT;
-            if (i < na) { rk[r] = ak[r0 + i]; rv[r] = av[r0 + i]; }
-        }
-#pragma unroll
-        for (int r = 0; r < BK_CAP_B / BK_NT; ++r) {
-            const int i = (int)threadIdx.x + r * BK_NT;
-            if (i < nb) { qk[r] = bk[v0 + i]; qv[r] = bv[v0 + i]; }
-        }
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int i = (int)threadIdx.x + r * BK_NT;
-            if (i < na) {
-                sm.rk[0][i] = ((rk[r] & lowmask) << BK_IDX_BITS) | (uint64_t)i;
-                sm.rv[i] = rv[r];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < BK_CAP_B / BK_NT; ++r) {
-            const int i = (int)threadIdx.x + r * BK_NT;
-            if (i < nb) { sm.kb[i] = qk[r]; sm.vb[i] = qv[r]; }
-        }
-    }
-    __syncthreads();
+    // sub-bucket digit: the top BK_SUB_BITS key bits below the bucket bits (all of them when
+    // fewer remain)
+    const int dsh = BK_TAG_BITS + (lowbits > BK_SUB_BITS ? lowbits - BK_SUB_BITS : 0);
+    const uint32_t dmask = lowbits >= BK_SUB_BITS ? (uint32_t)(BK_NSUB - 1) : (1u << lowbits) - 1u;
+    for (int j = threadIdx.x; j < BK_NSUB + 1; j += BK_NT) sm.sub[j] = 0;
 
-    // ---- order the packed records by key bits, in LDS: one stable radix pass on the top 8
-    // key bits below the bucket bits (wavefront w ranks the contiguous records
-    // [w * BK_RPT * 64, (w + 1) * BK_RPT * 64) with its own digit counters, round r taking
-    // record w * BK_RPT * 64 + r * 64 + lane), which splits the bucket into 256 sub-buckets
-    // of a few records each; then every record finds its place inside its sub-bucket by
-    // counting the smaller packed words there (equal keys: smaller list index first).
-    const int wbase = wid * BK_RPT * WAVE;
-    const int dshift = lowbits > RADIX_BITS ? lowbits - RADIX_BITS : 0;
-    const int sh = dshift + BK_IDX_BITS;
-    const uint32_t dmask = lowbits >= RADIX_BITS ? (uint32_t)(RADIX - 1) : (1u << lowbits) - 1u;
-    uint32_t dig[BK_RPT];
-    uint64_t w[BK_RPT];
-    {
-        for (int j = threadIdx.x; j < NW * RADIX; j += BK_NT) (&sm.wcnt[0][0])[j] = 0;
-        __syncthreads();
-        uint32_t* cnt = sm.wcnt[wid];
-        uint32_t rank[BK_RPT];
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int idx = wbase + r * WAVE + lane;
-            w[r] = idx < na ? sm.rk[0][idx] : 0ull;
-        }
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int idx = wbase + r * WAVE + lane;
-            const bool valid = idx < na;
-            const uint32_t digit = valid ? (uint32_t)(w[r] >> sh) & dmask : 0u;
-            dig[r] = digit;
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < RADIX_BITS; ++bit) {
-                const uint64_t bb = __ballot((digit >> bit) & 1u);
-                peers &= ((digit >> bit) & 1u) ? bb : ~bb;
-            }
-            const uint32_t before = valid ? cnt[digit] : 0u;
-            const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
-            rank[r] = before + in_wave;
-            __builtin_amdgcn_wave_barrier();
-            if (valid && in_wave == 0) cnt[digit] = before + (uint32_t)__popcll(peers);
-            __builtin_amdgcn_wave_barrier();
-        }
-        __syncthreads();
-        if (threadIdx.x < RADIX) {
-            const int d = threadIdx.x;
-            uint32_t run = 0;
-#pragma unroll
-            for (int q = 0; q < NW; ++q) {
-                const uint32_t c = sm.wcnt[q][d];
-                sm.wcnt[q][d] = run;
-                run += c;
-            }
-            sm.start[d] = run;
-        }
-        __syncthreads();
-        if (threadIdx.x < WAVE) {  // exclusive scan of the 256 digit totals by one wavefront
-            uint32_t x[RADIX / WAVE];
-            uint32_t sum = 0;
-#pragma unroll
-            for (int q = 0; q < RADIX / WAVE; ++q) { x[q] = sm.start[lane * (RADIX / WAVE) + q]; sum += x[q]; }
-            const uint32_t incl = wave_inclusive_scan(sum);
-            uint32_t run = incl - sum;
-#pragma unroll
-            for (int q = 0; q < RADIX / WAVE; ++q) { sm.start[lane * (RADIX / WAVE) + q] = run; run += x[q]; }
-            if (lane == WAVE - 1) sm.start[RADIX] = run;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int idx = wbase + r * WAVE + lane;
-            if (idx < na) sm.rk[1][sm.start[dig[r]] + sm.wcnt[wid][dig[r]] + rank[r]] = w[r];
-        }
-        __syncthreads();
-    }
-    if (dshift > 0) {  // bits below the sub-bucket digit: rank inside the sub-bucket
-#pragma unroll
-        for (int r = 0; r < BK_RPT; ++r) {
-            const int idx = wbase + r * WAVE + lane;
-            if (idx < na) {
-                const int s0 = (int)sm.start[dig[r]], s1 = (int)sm.start[dig[r] + 1];
-                int less = 0;
-                for (int q = s0; q < s1; ++q) less += sm.rk[1][q] < w[r];
-                sm.rk[0][s0 + less] = w[r];
-            }
-        }
-        __syncthreads();
-    }
-    const int cur = dshift > 0 ? 0 : 1;
-    const uint64_t* s_a = sm.rk[cur];
-    auto akey = [&](int i) { return bucket_key | (s_a[i] >> BK_IDX_BITS); };
-    auto aval = [&](int i) { return sm.rv[(int)(s_a[i] & ((1u << BK_IDX_BITS) - 1u))]; };
-
-    // ---- merge path inside the bucket, a left fold per address ---------------------------
-    const int total = na + nb;
-    const int dd = (int)threadIdx.x * BK_IPT;
-    const int n_here = max(0, min(BK_IPT, total - dd));
-    int ia = 0;
-    {
-        int lo = max(0, dd - nb), hi = min(dd, na);
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (akey(mid) < sm.kb[dd - mid - 1]) lo = mid + 1; else hi = mid;
-        }
-        ia = lo;
-    }
-    int ib = dd - ia;
-    bool have_prev = false;
-    uint64_t prev = 0;
-    if (dd > 0 && dd < total) {  // keys of other buckets differ from this bucket's
-        have_prev = true;
-        const uint64_t pa = ia > 0 ? akey(ia - 1) : 0;
-        const uint64_t pb = ib > 0 ? sm.kb[ib - 1] : 0;
-        prev = (ia > 0 && ib > 0) ? max(pa, pb) : (ia > 0 ? pa : pb);
-    }
+    // items of the first bucket; later ones are loaded one bucket ahead
+    int64_t b = blockIdx.x;
     uint64_t ck[BK_IPT];
     V cv[BK_IPT];
-    uint32_t cvalid = 0;
-    double lb = 0, ln = 0, nrm = 0;
+    int64_t r0 = 0, v0 = 0;
+    int na = 0, nb = 0;
+    auto load = [&](int t, int64_t bb, uint64_t* qk, V* qv, int64_t& q_r0, int64_t& q_v0, int& q_na, int& q_nb) {
+        q_r0 = rstart[bb];
+        q_v0 = vstart[bb];
+        q_na = (int)(rstart[bb + 1] - q_r0);
+        q_nb = (int)(vstart[bb + 1] - q_v0);
+        const int q_n = q_na + q_nb;
+        if (q_n > BK_CAP) return;  // overflow: nothing is read
+        // item i < q_na: record r0 + i; else vector entry v0 + i - q_na (uniform bases,
+        // 32-bit offsets)
+        const uint64_t* rk_b = ak + q_r0;
+        const V* rv_b = av + q_r0;
+        const uint64_t* ek_b = bk + q_v0;
+        const V* ev_b = bv + q_v0;
 #pragma unroll
-    for (int j = 0; j < BK_IPT; ++j) {
-        ck[j] = 0;
-        cv[j] = (V)0;
-        if (j < n_here) {
-            const uint64_t ka_ = ia < na ? akey(ia) : ~0ull;
-            const bool take_b = ib < nb && (ia >= na || sm.kb[ib] <= ka_);
-            const uint64_t key = take_b ? sm.kb[ib] : ka_;
-            if (!have_prev || key != prev) {
-                // segment head: the diagonal value (if any), then the spawns in list order
-                V acc = (V)0;
-                if (take_b) acc = acc + sm.vb[ib];
-                for (int la = ia; la < na && akey(la) == key; ++la) acc = acc + aval(la);
-                if (acc != (V)0) {
-                    lb += 1;
-                    ck[j] = key;
-                    cv[j] = acc;
-                    cvalid |= 1u << j;
+        for (int j = 0; j < BK_IPT; ++j) {
+            const int i = t + j * BK_NT;
+            const bool rec = i < q_na;
+            const int o = rec ? i : i - q_na;
+            if (i < q_n) {
+                qk[j] = rec ? rk_b[o] : ek_b[o];
+                qv[j] = rec ? rv_b[o] : ev_b[o];
+            }
+        }
+    };
+    if (b < nbk) load((int)threadIdx.x, b, ck, cv, r0, v0, na, nb);
+    __syncthreads();  // sub-bucket counters cleared
+
+    for (; b < nbk; b += gridDim.x) {
+        // the thread index, re-read every bucket: values derived from it stay inside the loop
+        // instead of being kept in registers across buckets
+        int tid;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+        // the next bucket's items travel while this one is processed
+        uint64_t nk[BK_IPT];
+        V nv[BK_IPT];
+        int64_t n_r0 = 0, n_v0 = 0;
+        int n_na = 0, n_nb = 0;
+        const int64_t bn = b + gridDim.x;
+        if (bn < nbk) load(tid, bn, nk, nv, n_r0, n_v0, n_na, n_nb);
+
+        const int n = na + nb;
+        if (n > BK_CAP) {  // uniform over the workgroup
+            if (tid == 0) {
+                atomicOr(flags, FLAG_BUCKET_OVERFLOW);
+                bucket_cnt[b] = 0;
+                partials[b] = 0;
+                partials[nbk + b] = 0;
+                partials[2 * nbk + b] = 0;
+            }
+        } else {
+            const uint64_t bucket_key = lowbits >= 64 ? 0ull : (uint64_t)b << lowbits;
+            // ---- 1-2. pack, drop into sub-buckets, rank inside the sub-bucket ---------------
+            uint64_t w[BK_IPT];
+            uint32_t dg[BK_IPT], slot[BK_IPT];
+#pragma unroll
+            for (int j = 0; j < BK_IPT; ++j) {
+                const int i = tid + j * BK_NT;
+                if (i < n) {
+                    const uint32_t tag = i < na ? (uint32_t)(i + 1) : 0u;
+                    w[j] = ((ck[j] & lowmask) << BK_TAG_BITS) | tag;
+                    dg[j] = (uint32_t)(w[j] >> dsh) & dmask;
+                    slot[j] = atomicAdd(&sm.sub[dg[j]], 1u);
                 }
             }
-            prev = key;
-            have_prev = true;
-            if (take_b) ++ib; else ++ia;
-        }
-    }
-    __syncthreads();  // the sorted records and the segment are no longer read
-    uint64_t* stage_k = &sm.rk[0][0];                 // 2 * BK_CAP_A keys
-    V* stage_v = &sm.rv[0];                          // rv, vb (and kb): BK_CAP_A + BK_CAP_B values
-    if constexpr (!is_int_v<V>::value) {
-        if (a.compression == 1) {
-            // ThresholdCompression: each wavefront lists its values below the threshold and
-            // draws them with every lane busy (a draw depends only on the address)
-            const double th = a.comp_threshold;
-            uint32_t need = 0;
+            __syncthreads();
+            {   // exclusive scan of the sub-bucket counts; sub[BK_NSUB] = n
+                constexpr int SPT = BK_NSUB / BK_NT;
+                uint32_t x[SPT];
+                uint32_t sum = 0;
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) { x[q] = sm.sub[tid * SPT + q]; sum += x[q]; }
+                uint32_t tot;
+                uint32_t run = block_exclusive_scan<uint32_t, BK_NT>(sum, &tot);
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) { sm.sub[tid * SPT + q] = run; run += x[q]; }
+                if (tid == 0) sm.sub[BK_NSUB] = tot;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < BK_IPT; ++j) {
+                const int i = tid + j * BK_NT;
+                if (i < n) sm.place[sm.sub[dg[j]] + slot[j]] = w[j];
+            }
+            __syncthreads();
+            // an item's place: the smaller packed words of its sub-bucket, whose first
+            // BK_RANK_WIN words are read at once (sub-buckets hold about one item)
+#pragma unroll
+            for (int j = 0; j < BK_IPT; ++j) {
+                const int i = tid + j * BK_NT;
+                if (i < n) {
+                    const int s0 = (int)sm.sub[dg[j]], s1 = (int)sm.sub[dg[j] + 1];
+                    uint64_t q[BK_RANK_WIN];
+#pragma unroll
+                    for (int k = 0; k < BK_RANK_WIN; ++k) q[k] = sm.place[min(s0 + k, BK_CAP - 1)];
+                    int less = 0;
+#pragma unroll
+                    for (int k = 0; k < BK_RANK_WIN; ++k) less += (s0 + k < s1 && q[k] < w[j]) ? 1 : 0;
+                    for (int k = s0 + BK_RANK_WIN; k < s1; ++k) less += sm.place[k] < w[j];
+                    sm.sk[s0 + less] = w[j];
+                    sm.sv[s0 + less] = cv[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // one item's window in registers at a time
+            }
+            __syncthreads();
+            // counters of the next bucket start at zero (their last reader was the rank above)
+            for (int j = tid; j < BK_NSUB + 1; j += BK_NT) sm.sub[j] = 0;
+
+            // ---- 3. fold: thread t owns the runs of equal keys whose first position lies in
+            // [t * BK_IPT, (t + 1) * BK_IPT).  It reads those positions and BK_FOLD_EXTRA more
+            // at once and adds each run left to right -- (0 + the diagonal value, tag 0, if
+            // any) + the spawns in list order -- in one pass over the window; a run's result
+            // is taken at its last position k, into slot min(k, BK_IPT - 1) (only one owned
+            // run can end at or after BK_IPT - 1), so the slots keep the runs in key order.
+            // A run going past the window reads on from LDS.
+            const int p0 = tid * BK_IPT;
+            constexpr int NWIN = BK_IPT + BK_FOLD_EXTRA;
+            constexpr uint64_t NOKEY = ~0ull;  // no position: differs from every key (< 2^52)
+            uint64_t kk[NWIN + 1];
+            V vv[NWIN];
+#pragma unroll
+            for (int k = 0; k <= NWIN; ++k) {
+                const int p = p0 + k;
+                kk[k] = p < n ? sm.sk[p] >> BK_TAG_BITS : NOKEY;
+                if (k < NWIN) vv[k] = p < n ? sm.sv[p] : (V)0;
+            }
+            const uint64_t kprev = p0 > 0 && p0 < n ? sm.sk[p0 - 1] >> BK_TAG_BITS : NOKEY;
+            uint64_t okey[BK_IPT];
+            V oval[BK_IPT];
+            uint32_t cvalid = 0;
+            bool own = false;
+            V acc = (V)0;
+#pragma unroll
+            for (int k = 0; k < NWIN; ++k) {
+                const bool valid = p0 + k < n;
+                const bool start = valid && kk[k] != (k > 0 ? kk[k - 1] : kprev);
+                if (start) own = k < BK_IPT;
+                acc = (start ? (V)0 : acc) + vv[k];
+                const bool end = valid && kk[k + 1] != kk[k];
+                const int slot_k = k < BK_IPT - 1 ? k : BK_IPT - 1;
+                if (own && end) {
+                    okey[slot_k] = bucket_key | kk[k];
+                    oval[slot_k] = acc;
+                    if (acc != (V)0) cvalid |= 1u << slot_k;
+                    own = false;
+                }
+            }
+            if (own) {  // the owned run goes on past the window: it ends in slot BK_IPT - 1
+                const uint64_t key = kk[NWIN - 1];
+                for (int q = p0 + NWIN; q < n && (sm.sk[q] >> BK_TAG_BITS) == key; ++q) acc = acc + sm.sv[q];
+                okey[BK_IPT - 1] = bucket_key | key;
+                oval[BK_IPT - 1] = acc;
+                if (acc != (V)0) cvalid |= 1u << (BK_IPT - 1);
+            }
+            const int n_before = __popc(cvalid);  // len_before: folded values that are not zero
+            __syncthreads();  // the sorted words and values are no longer read
+            uint64_t* stage_k = sm.place;
+            V* stage_v = sm.sv;
+            if constexpr (!is_int_v<V>::value) {
+                if (a.compression == 1) {
+                    // ThresholdCompression: each wavefront lists its values below the threshold
+                    // and draws them with every lane busy (a draw depends only on the address)
+                    const double th = a.comp_threshold;
+                    uint32_t need = 0;
+#pragma unroll
+                    for (int j = 0; j < BK_IPT; ++j)
+                        if (((cvalid >> j) & 1u) && fabs((double)oval[j]) / th < 1) need |= 1u << j;
+                    const int nneed = __popc(need);
+                    const int incl = wave_inclusive_scan(nneed);
+                    const int wtot = __shfl(incl, WAVE - 1, WAVE);
+                    const int mybase = incl - nneed;
+                    uint64_t* wk = stage_k + wid * (WAVE * BK_IPT);
+                    V* wv = stage_v + wid * (WAVE * BK_IPT);
+#pragma unroll
+                    for (int j = 0; j < BK_IPT; ++j) {
+                        if ((need >> j) & 1u) {
+                            const int q = mybase + __popc(need & ((1u << j) - 1u));
+                            wk[q] = okey[j];
+                            wv[q] = oval[j];
+                        }
+                    }
+                    __syncthreads();
+                    for (int q = lane; q < wtot; q += WAVE) {
+                        const uint64_t key = wk[q];
+                        const double x = (double)wv[q];
+                        const U4 r = draw(a.seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
+                        wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < BK_IPT; ++j) {
+                        if ((need >> j) & 1u) {
+                            const int q = mybase + __popc(need & ((1u << j) - 1u));
+                            oval[j] = wv[q];
+                            if (oval[j] == (V)0) cvalid &= ~(1u << j);
+                        }
+                    }
+                }
+            }
+            double nrm = 0;
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j)
-                if (((cvalid >> j) & 1u) && fabs((double)cv[j]) / th < 1) need |= 1u << j;
-            const int nneed = __popc(need);
-            const int incl = wave_inclusive_scan(nneed);
-            const int wtot = __shfl(incl, WAVE - 1, WAVE);
-            const int mybase = incl - nneed;
-            uint64_t* wk = stage_k + wid * (WAVE * BK_IPT);
-            V* wv = stage_v + wid * (WAVE * BK_IPT);
+                if ((cvalid >> j) & 1u) nrm += fabs((double)oval[j]);
+            // ---- 4. survivors in order, staged for coalesced writes ------------------------
+            const int count = __popc(cvalid);
+            int btotal;
+            const int ex = block_exclusive_scan<int, BK_NT>(count, &btotal);  // ends with a barrier
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j) {
-                if ((need >> j) & 1u) {
-                    const int q = mybase + __popc(need & ((1u << j) - 1u));
-                    wk[q] = ck[j];
-                    wv[q] = cv[j];
+                if ((cvalid >> j) & 1u) {
+                    const int q = ex + __popc(cvalid & ((1u << j) - 1u));
+                    stage_k[q] = okey[j];
+                    stage_v[q] = oval[j];
                 }
             }
-            __syncthreads();
-            for (int q = lane; q < wtot; q += WAVE) {
-                const uint64_t key = wk[q];
-                const double x = (double)wv[q];
-                const U4 r = draw(a.seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
-                wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
+            // per-bucket sums: counts (exact in any order) and the norm (fixed order:
+            // wavefront trees, then the wavefronts in order)
+            int nb_w = n_before;
+#pragma unroll
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                nb_w += __shfl_xor(nb_w, off, WAVE);
+                nrm += __shfl_xor(nrm, off, WAVE);
+            }
+            if (lane == 0) {
+                sm.red[0][wid] = (double)nb_w;
+                sm.red[2][wid] = nrm;
             }
             __syncthreads();
-#pragma unroll
-            for (int j = 0; j < BK_IPT; ++j) {
-                if ((need >> j) & 1u) {
-                    const int q = mybase + __popc(need & ((1u << j) - 1u));
-                    cv[j] = wv[q];
-                    if (cv[j] == (V)0) cvalid &= ~(1u << j);
-                }
+            const int64_t base = r0 + v0;  // the bucket's slot range: its merged positions
+            for (int i = tid; i < btotal; i += BK_NT) {
+                slot_k[base + i] = stage_k[i];
+                slot_v[base + i] = stage_v[i];
             }
+            if (tid == 0) {
+                double s0 = 0, s2 = 0;
+                for (int q = 0; q < NW; ++q) { s0 += sm.red[0][q]; s2 += sm.red[2][q]; }
+                bucket_cnt[b] = btotal;
+                partials[b] = s0;
+                partials[nbk + b] = (double)btotal;
+                partials[2 * nbk + b] = s2;
+            }
+            __syncthreads();  // staging and sums read before the next bucket writes LDS
         }
-    }
 #pragma unroll
-    for (int j = 0; j < BK_IPT; ++j) {
-        if ((cvalid >> j) & 1u) {
-            ln += 1;
-            nrm += fabs((double)cv[j]);
-        }
-    }
-    // survivors in merged order, staged for coalesced writes
-    const int count = __popc(cvalid);
-    int btotal;
-    const int ex = block_exclusive_scan<int, BK_NT>(count, &btotal);  // ends with a barrier
-#pragma unroll
-    for (int j = 0; j < BK_IPT; ++j) {
-        if ((cvalid >> j) & 1u) {
-            const int q = ex + __popc(cvalid & ((1u << j) - 1u));
-            stage_k[q] = ck[j];
-            stage_v[q] = cv[j];
-        }
-    }
-    // per-bucket sums (fixed order: wavefront trees, then the wavefronts in order)
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        lb += __shfl_xor(lb, off, WAVE);
-        ln += __shfl_xor(ln, off, WAVE);
-        nrm += __shfl_xor(nrm, off, WAVE);
-    }
-    if (lane == 0) {
-        sm.red[0][wid] = lb;
-        sm.red[1][wid] = ln;
-        sm.red[2][wid] = nrm;
-    }
-    __syncthreads();
-    const int64_t base = r0 + v0;  // the bucket's slot range: its merged positions
-    for (int i = threadIdx.x; i < btotal; i += BK_NT) {
-        slot_k[base + i] = stage_k[i];
-        slot_v[base + i] = stage_v[i];
-    }
-    if (threadIdx.x == 0) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int q = 0; q < NW; ++q) { s0 += sm.red[0][q]; s1 += sm.red[1][q]; s2 += sm.red[2][q]; }
-        bucket_cnt[b] = btotal;
-        partials[b] = s0;
-        partials[nbk + b] = s1;
-        partials[2 * nbk + b] = s2;
+        for (int j = 0; j < BK_IPT; ++j) { ck[j] = nk[j]; cv[j] = nv[j]; }
+        r0 = n_r0;
+        v0 = n_v0;
+        na = n_na;
+        nb = n_nb;
     }
 }
 
@@ -401,7 +400,7 @@ __global__ __launch_bounds__(NT) void bucket_compact_kernel(const uint64_t* __re
     const int64_t off = offs[b];
     const int n = (int)(offs[b + 1] - off);
     const int64_t src = rstart[b] + vstart[b];
-    constexpr int CPT = (BK_CAP_A + BK_CAP_B + NT - 1) / NT;
+    constexpr int CPT = (BK_CAP + NT - 1) / NT;
     uint64_t k[CPT];
     V v[CPT];
 #pragma unroll

@@ -85,6 +85,7 @@ struct rimu_engine {
     // 2: sort on the top bits + bucket merge where eligible, else as 1; 1: one histogram pass
     // + look-back scatters over every bit, merge path; 0: counts per pass, merge path
     int radix_mode = 2;
+    int n_cu = 0;                   // compute units of the device (queried on first use)
     uint64_t seed = 0;
     bool int_vals = false;
     size_t vsize = 8;
@@ -532,17 +533,18 @@ int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result, Fill&
 // Bucket merge (bucket_kernels.hpp) of S spawn records sorted on their top T key bits with
 // the vector (bk, bv: keys and diagonal survivors, L entries) into (ok, ov) of capacity cap.
 // `tiles` receives the number of buckets (rows of merge_p).
-constexpr double BK_FILL_A = 1100;  // mean records per bucket aimed at (LDS holds BK_CAP_A)
-constexpr double BK_FILL_B = 700;   // mean vector entries per bucket aimed at (BK_CAP_B)
+constexpr double BK_FILL = 1800;    // mean items (records + entries) per bucket aimed at
+constexpr double BK_FILL_MAX = 2050; // highest mean kept to save a radix pass (LDS holds BK_CAP)
 constexpr int64_t BK_MIN_RECORDS = 4096;
-constexpr int BK_MAX_LOWBITS = 48;   // LDS sort passes per bucket at most 6
+constexpr int BK_MAX_LOWBITS = 48;   // packed words: key bits below the bucket bits + a 12-bit tag
 constexpr int BK_COOLDOWN = 16;      // steps on the merge path after a bucket overflowed
 
-// Top bits that split S records and L entries into buckets of about BK_FILL_A records and at
-// most about BK_FILL_B entries.
+// Top bits that split S records and L entries into buckets of about BK_FILL items; one bit
+// fewer when that saves a radix pass (8 bits each) and keeps the mean below BK_FILL_MAX.
 int bucket_bits(int64_t S, int64_t L, int B) {
-    const double need = std::max((double)S / BK_FILL_A, (double)L / BK_FILL_B);
-    int T = (int)std::ceil(std::log2(std::max(need, 2.0)));
+    const double items = (double)S + (double)L;
+    int T = (int)std::ceil(std::log2(std::max(items / BK_FILL, 2.0)));
+    if (T > 1 && T % RADIX_BITS == 1 && items / std::ldexp(1.0, T - 1) <= BK_FILL_MAX) --T;
     return std::min(std::max(T, 1), std::min(B, 24));
 }
 
@@ -565,10 +567,15 @@ int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, con
                                                                           e->bk_rstart.p, e->bk_vstart.p);
     e->timer.end(e->stream);
     HIP_TRY(hipMemsetAsync(e->tile_cnt.p + nbk, 0, sizeof(int64_t), e->stream));
+    if (e->n_cu == 0) {
+        HIP_TRY(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, e->device));
+        if (e->n_cu <= 0) e->n_cu = 1;
+    }
+    const int64_t grid = std::min(nbk, (int64_t)e->n_cu * BK_WG_PER_CU);
     e->timer.begin("bucket_merge", e->stream, &ev);
-    bucket_merge_kernel<V><<<nbk, BK_NT, 0, e->stream>>>(a, ak, av, bk, bv, e->bk_rstart.p, e->bk_vstart.p, B - T,
-                                                      e->slot_k.p, (V*)e->slot_v.p, e->tile_cnt.p, e->merge_p.p,
-                                                      &dm->flags);
+    bucket_merge_kernel<V><<<grid, BK_NT, 0, e->stream>>>(a, ak, av, bk, bv, e->bk_rstart.p, e->bk_vstart.p, nbk,
+                                                       B - T, e->slot_k.p, (V*)e->slot_v.p, e->tile_cnt.p,
+                                                       e->merge_p.p, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
