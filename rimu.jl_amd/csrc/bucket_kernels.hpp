@@ -140,12 +140,14 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
         const V* ev_b = bv + q_v0;
 #pragma unroll
         for (int j = 0; j < BK_IPT; ++j) {
-            const int i = t + j * BK_NT;
-            const bool rec = i < q_na;
-            const int o = rec ? i : i - q_na;
-            if (i < q_n) {
-                qk[j] = rec ? rk_b[o] : ek_b[o];
-                qv[j] = rec ? rv_b[o] : ev_b[o];
+            const uint32_t i = (uint32_t)t + (uint32_t)(j * BK_NT);  // unsigned: 32-bit offsets
+            if (i < (uint32_t)q_na) {
+                qk[j] = rk_b[i];
+                qv[j] = rv_b[i];
+            }
+            if (i >= (uint32_t)q_na && i < (uint32_t)q_n) {
+                qk[j] = ek_b[i - (uint32_t)q_na];
+                qv[j] = ev_b[i - (uint32_t)q_na];
             }
         }
     };
@@ -157,24 +159,20 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
         // instead of being kept in registers across buckets
         int tid;
         asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
-        // the next bucket's items travel while this one is processed
+        // the next bucket's items travel while this one is compressed and written out
         uint64_t nk[BK_IPT];
         V nv[BK_IPT];
         int64_t n_r0 = 0, n_v0 = 0;
         int n_na = 0, n_nb = 0;
         const int64_t bn = b + gridDim.x;
-        if (bn < nbk) load(tid, bn, nk, nv, n_r0, n_v0, n_na, n_nb);
 
         const int n = na + nb;
-        if (n > BK_CAP) {  // uniform over the workgroup
-            if (tid == 0) {
-                atomicOr(flags, FLAG_BUCKET_OVERFLOW);
-                bucket_cnt[b] = 0;
-                partials[b] = 0;
-                partials[nbk + b] = 0;
-                partials[2 * nbk + b] = 0;
-            }
-        } else {
+        const bool fits = n <= BK_CAP;  // uniform over the workgroup
+        uint64_t okey[BK_IPT];
+        V oval[BK_IPT];
+        uint32_t cvalid = 0;
+        int n_before = 0;  // len_before: folded values that are not zero
+        if (fits) {
             const uint64_t bucket_key = lowbits >= 64 ? 0ull : (uint64_t)b << lowbits;
             // ---- 1-2. pack, drop into sub-buckets, rank inside the sub-bucket ---------------
             uint64_t w[BK_IPT];
@@ -251,9 +249,6 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
                 if (k < NWIN) vv[k] = p < n ? sm.sv[p] : (V)0;
             }
             const uint64_t kprev = p0 > 0 && p0 < n ? sm.sk[p0 - 1] >> BK_TAG_BITS : NOKEY;
-            uint64_t okey[BK_IPT];
-            V oval[BK_IPT];
-            uint32_t cvalid = 0;
             bool own = false;
             V acc = (V)0;
 #pragma unroll
@@ -278,8 +273,19 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
                 oval[BK_IPT - 1] = acc;
                 if (acc != (V)0) cvalid |= 1u << (BK_IPT - 1);
             }
-            const int n_before = __popc(cvalid);  // len_before: folded values that are not zero
+            n_before = __popc(cvalid);
             __syncthreads();  // the sorted words and values are no longer read
+        }
+        if (bn < nbk) load(tid, bn, nk, nv, n_r0, n_v0, n_na, n_nb);
+        if (!fits) {
+            if (tid == 0) {
+                atomicOr(flags, FLAG_BUCKET_OVERFLOW);
+                bucket_cnt[b] = 0;
+                partials[b] = 0;
+                partials[nbk + b] = 0;
+                partials[2 * nbk + b] = 0;
+            }
+        } else {
             uint64_t* stage_k = sm.place;
             V* stage_v = sm.sv;
             if constexpr (!is_int_v<V>::value) {
