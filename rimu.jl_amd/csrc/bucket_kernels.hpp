@@ -7,19 +7,19 @@
 // key >> (B - T) -- and the vector, kept in stored-key order, holds each bucket's entries as
 // one contiguous segment.  A persistent grid of workgroups takes the buckets in a fixed
 // stride; for each bucket a workgroup
-//   1. holds the bucket's records and vector entries in registers (loaded while the previous
-//      bucket was being processed) and packs every item as (key bits below the bucket bits,
-//      tag): tag 0 for the vector entry, 1 + list index for a record -- so that sorting the
-//      packed words puts each address's diagonal survivor first and its spawns after it in
-//      list order, the order of deposit! over the list (Interfaces/dictvectors.jl:49-51,
-//      dvec.jl:129-136),
-//   2. sorts the packed words in LDS: LDS atomics drop every item into one of 2048
+//   1. holds the bucket's vector entries and records in registers (loaded while the previous
+//      bucket was being finished), numbers them entries first, then records in list order,
+//      stores the values in LDS by that item index and packs every item as (key bits below
+//      the bucket bits, item index) -- so that sorting the packed words puts each address's
+//      diagonal survivor first and its spawns after it in list order, the order of deposit!
+//      over the list (Interfaces/dictvectors.jl:49-51, dvec.jl:129-136),
+//   2. sorts the packed words in LDS: LDS atomics drop every word into one of 2048
 //      sub-buckets (the top 11 key bits; arrival order inside a sub-bucket is arbitrary), and
-//      each item's place inside its sub-bucket is the number of smaller packed words there,
-//      which makes the order exact whatever the arrival order was,
-//   3. folds each address left to right (the first item of a run of equal keys walks the
-//      run), applies ThresholdCompression (compression.jl:18-33) and counts len_before, len
-//      and the norm,
+//      each word's place inside its sub-bucket is the number of smaller words there -- exact
+//      whatever the arrival order was, since the item indices make all words distinct,
+//   3. folds each address left to right in one pass over a window of the sorted words,
+//      applies ThresholdCompression (compression.jl:18-33) and counts len_before, len and
+//      the norm,
 //   4. writes its survivors to its own slot range with their count; bucket_compact_kernel
 //      places them after a scan of the counts.
 // The result is that of the full sort and merge: same addresses, same fold order, same
@@ -34,15 +34,15 @@ namespace rimu {
 constexpr int BK_NT = 512;                    // threads of a bucket workgroup
 constexpr int BK_CAP = 2560;                  // records + vector entries per bucket in LDS
 constexpr int BK_IPT = BK_CAP / BK_NT;        // items per thread
-constexpr int BK_TAG_BITS = 12;               // 0: vector entry; 1 + list index: record
+constexpr int BK_IDX_BITS = 12;               // item index below the key bits of a packed word
 constexpr int BK_SUB_BITS = 11;               // sub-buckets of the LDS sort
 constexpr int BK_NSUB = 1 << BK_SUB_BITS;
 constexpr int BK_WG_PER_CU = 2;               // resident bucket workgroups per CU (LDS-bound)
-constexpr int BK_RANK_WIN = 2;                // sub-bucket words read at once
+constexpr int BK_SORT_WIN = 2;                // sub-bucket words read at once by the ranking
 constexpr int BK_FOLD_EXTRA = 4;              // positions read past a thread's own for its runs
-static_assert(BK_CAP < (1 << BK_TAG_BITS), "tags fit their bits");
+static_assert(BK_CAP <= (1 << BK_IDX_BITS), "item indices fit their bits");
 static_assert(BK_CAP % BK_NT == 0 && BK_NSUB % BK_NT == 0, "whole rounds per thread");
-static_assert(BK_IPT <= 32, "survivor masks are 32-bit");
+static_assert(BK_IPT < 8, "survivor masks are 32-bit; per-thread counts fit three bits");
 constexpr int FLAG_BUCKET_OVERFLOW = 16;
 
 // First record and first vector entry of every bucket: rstart[b] = lower bound of b among
@@ -67,16 +67,49 @@ __global__ __launch_bounds__(NT) void bucket_bounds_kernel(const uint64_t* __res
     (rec ? rstart : vstart)[b] = lo;
 }
 
-// LDS of one bucket workgroup.  `place` receives the packed words sub-bucket by sub-bucket,
-// `sk`/`sv` the packed words and values in order; after the fold `place` and `sv` stage the
-// survivors' keys and values.
+// Scans of the bucket workgroup, written with the lane and (uniform) wavefront index the
+// caller derives from its per-bucket thread index, so that no lane masks are held in
+// registers across buckets.
+template <class T>
+__device__ inline T bk_wave_inclusive_scan(T v, int lane) {
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const T n = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += n;
+    }
+    return v;
+}
+
+// Exclusive prefix of this thread over the workgroup; *total receives the sum.
+template <class T>
+__device__ inline T bk_block_exclusive_scan(T v, int lane, int wid, T* s_wave, T* total) {
+    const T inc = bk_wave_inclusive_scan(v, lane);
+    if (lane == WAVE - 1) s_wave[wid] = inc;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BK_NT / WAVE; ++w) {
+        const T x = s_wave[w];
+        if (w < wid) off += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// LDS of one bucket workgroup.  `vin` holds the items' values by item index and `place`
+// their packed words sub-bucket by sub-bucket; `sk` receives the words in order.  After the
+// fold `place` and `vin` stage the values that ThresholdCompression draws for.
 template <class V>
 struct BucketLds {
     uint64_t place[BK_CAP];
     uint64_t sk[BK_CAP];
-    V sv[BK_CAP];
+    V vin[BK_CAP];
     uint32_t sub[BK_NSUB + 1];      // sub-bucket counts, then their starts
-    double red[3][BK_NT / WAVE];
+    double red[2][BK_NT / WAVE];
+    uint32_t scan_u[BK_NT / WAVE];
+    int scan_i[BK_NT / WAVE];
 };
 
 // Bucket b = blockIdx.x, blockIdx.x + gridDim.x, ...; nbk buckets of which rstart/vstart give
@@ -84,22 +117,14 @@ struct BucketLds {
 // + i], their count in bucket_cnt[b], and partials[b], [nbk + b], [2 nbk + b] = len_before,
 // len, norm.
 template <class V>
-__global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(StepArgs a, const uint64_t* __restrict__ ak,
-                                                             const V* __restrict__ av,
-                                                             const uint64_t* __restrict__ bk,
-                                                             const V* __restrict__ bv,
-                                                             const int64_t* __restrict__ rstart,
-                                                             const int64_t* __restrict__ vstart,
-                                                             int64_t nbk, int lowbits,
-                                                             uint64_t* __restrict__ slot_k,
-                                                             V* __restrict__ slot_v,
-                                                             int64_t* __restrict__ bucket_cnt,
-                                                             double* __restrict__ partials,
-                                                             int* flags) {
+__global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
+    StepArgs a, const uint64_t* __restrict__ ak, const V* __restrict__ av, const uint64_t* __restrict__ bk,
+    const V* __restrict__ bv, const int64_t* __restrict__ rstart, const int64_t* __restrict__ vstart, int64_t nbk,
+    int lowbits, uint64_t* __restrict__ slot_k, V* __restrict__ slot_v, int64_t* __restrict__ bucket_cnt,
+    double* __restrict__ partials, int* flags) {
     constexpr int NW = BK_NT / WAVE;
+    constexpr uint64_t IDX_MASK = (1ull << BK_IDX_BITS) - 1ull;
     __shared__ BucketLds<V> sm;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = threadIdx.x / WAVE;
     // no sorted list to merge (raised by earlier kernels only: every wave reads the same bits)
     if (*flags & (FLAG_LOOKBACK | FLAG_SPEC)) {
         for (int64_t b = blockIdx.x; b < nbk; b += gridDim.x) {
@@ -115,11 +140,13 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
     const uint64_t lowmask = lowbits >= 64 ? ~0ull : (1ull << lowbits) - 1ull;
     // sub-bucket digit: the top BK_SUB_BITS key bits below the bucket bits (all of them when
     // fewer remain)
-    const int dsh = BK_TAG_BITS + (lowbits > BK_SUB_BITS ? lowbits - BK_SUB_BITS : 0);
+    const int dsh = BK_IDX_BITS + (lowbits > BK_SUB_BITS ? lowbits - BK_SUB_BITS : 0);
     const uint32_t dmask = lowbits >= BK_SUB_BITS ? (uint32_t)(BK_NSUB - 1) : (1u << lowbits) - 1u;
     for (int j = threadIdx.x; j < BK_NSUB + 1; j += BK_NT) sm.sub[j] = 0;
 
-    // items of the first bucket; later ones are loaded one bucket ahead
+    // items of the first bucket; later ones are loaded during the bucket before them.  Item
+    // i < nb is vector entry v0 + i, item nb + r is record r0 + r: the item index orders each
+    // address's items as the fold takes them (diagonal value first, spawns in list order).
     int64_t b = blockIdx.x;
     uint64_t ck[BK_IPT];
     V cv[BK_IPT];
@@ -132,22 +159,20 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
         q_nb = (int)(vstart[bb + 1] - q_v0);
         const int q_n = q_na + q_nb;
         if (q_n > BK_CAP) return;  // overflow: nothing is read
-        // item i < q_na: record r0 + i; else vector entry v0 + i - q_na (uniform bases,
-        // 32-bit offsets)
+        const uint64_t* ek_b = bk + q_v0;  // uniform bases, 32-bit offsets
+        const V* ev_b = bv + q_v0;
         const uint64_t* rk_b = ak + q_r0;
         const V* rv_b = av + q_r0;
-        const uint64_t* ek_b = bk + q_v0;
-        const V* ev_b = bv + q_v0;
 #pragma unroll
         for (int j = 0; j < BK_IPT; ++j) {
-            const uint32_t i = (uint32_t)t + (uint32_t)(j * BK_NT);  // unsigned: 32-bit offsets
-            if (i < (uint32_t)q_na) {
-                qk[j] = rk_b[i];
-                qv[j] = rv_b[i];
+            const uint32_t i = (uint32_t)t + (uint32_t)(j * BK_NT);
+            if (i < (uint32_t)q_nb) {
+                qk[j] = ek_b[i];
+                qv[j] = ev_b[i];
             }
-            if (i >= (uint32_t)q_na && i < (uint32_t)q_n) {
-                qk[j] = ek_b[i - (uint32_t)q_na];
-                qv[j] = ev_b[i - (uint32_t)q_na];
+            if (i >= (uint32_t)q_nb && i < (uint32_t)q_n) {
+                qk[j] = rk_b[i - (uint32_t)q_nb];
+                qv[j] = rv_b[i - (uint32_t)q_nb];
             }
         }
     };
@@ -159,7 +184,8 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
         // instead of being kept in registers across buckets
         int tid;
         asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
-        // the next bucket's items travel while this one is compressed and written out
+        const int lane = tid & (WAVE - 1);
+        const int wid = __builtin_amdgcn_readfirstlane(tid / WAVE);
         uint64_t nk[BK_IPT];
         V nv[BK_IPT];
         int64_t n_r0 = 0, n_v0 = 0;
@@ -174,69 +200,77 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
         int n_before = 0;  // len_before: folded values that are not zero
         if (fits) {
             const uint64_t bucket_key = lowbits >= 64 ? 0ull : (uint64_t)b << lowbits;
-            // ---- 1-2. pack, drop into sub-buckets, rank inside the sub-bucket ---------------
+            // ---- 1. pack (key bits below the bucket bits, item index), values by item
+            // index, drop every word into its sub-bucket (arrival order is arbitrary) ---------
             uint64_t w[BK_IPT];
             uint32_t dg[BK_IPT], slot[BK_IPT];
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j) {
                 const int i = tid + j * BK_NT;
                 if (i < n) {
-                    const uint32_t tag = i < na ? (uint32_t)(i + 1) : 0u;
-                    w[j] = ((ck[j] & lowmask) << BK_TAG_BITS) | tag;
+                    w[j] = ((ck[j] & lowmask) << BK_IDX_BITS) | (uint64_t)i;
+                    sm.vin[i] = cv[j];
                     dg[j] = (uint32_t)(w[j] >> dsh) & dmask;
                     slot[j] = atomicAdd(&sm.sub[dg[j]], 1u);
                 }
             }
             __syncthreads();
-            {   // exclusive scan of the sub-bucket counts; sub[BK_NSUB] = n
+            // ---- 2. sub-bucket starts (exclusive scan of the counts; sub[BK_NSUB] = n) and
+            // placement; every item keeps the bounds of its sub-bucket -----------------------
+            {
                 constexpr int SPT = BK_NSUB / BK_NT;
                 uint32_t x[SPT];
                 uint32_t sum = 0;
 #pragma unroll
                 for (int q = 0; q < SPT; ++q) { x[q] = sm.sub[tid * SPT + q]; sum += x[q]; }
                 uint32_t tot;
-                uint32_t run = block_exclusive_scan<uint32_t, BK_NT>(sum, &tot);
+                uint32_t run = bk_block_exclusive_scan<uint32_t>(sum, lane, wid, sm.scan_u, &tot);
 #pragma unroll
                 for (int q = 0; q < SPT; ++q) { sm.sub[tid * SPT + q] = run; run += x[q]; }
                 if (tid == 0) sm.sub[BK_NSUB] = tot;
             }
             __syncthreads();
-#pragma unroll
-            for (int j = 0; j < BK_IPT; ++j) {
-                const int i = tid + j * BK_NT;
-                if (i < n) sm.place[sm.sub[dg[j]] + slot[j]] = w[j];
-            }
-            __syncthreads();
-            // an item's place: the smaller packed words of its sub-bucket, whose first
-            // BK_RANK_WIN words are read at once (sub-buckets hold about one item)
+            int s0[BK_IPT], s1[BK_IPT];
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j) {
                 const int i = tid + j * BK_NT;
                 if (i < n) {
-                    const int s0 = (int)sm.sub[dg[j]], s1 = (int)sm.sub[dg[j] + 1];
-                    uint64_t q[BK_RANK_WIN];
+                    s0[j] = (int)sm.sub[dg[j]];
+                    s1[j] = (int)sm.sub[dg[j] + 1];
+                    sm.place[s0[j] + slot[j]] = w[j];
+                }
+            }
+            __syncthreads();
+            // counters of the next bucket start at zero (their last reader was the placement)
+            for (int j = tid; j < BK_NSUB + 1; j += BK_NT) sm.sub[j] = 0;
+            // ---- 3. an item's place in its sub-bucket is the number of smaller words there
+            // (the words are distinct: item indices), read BK_SORT_WIN words at a time (wider
+            // windows measured no faster) --------------------------------------------------
 #pragma unroll
-                    for (int k = 0; k < BK_RANK_WIN; ++k) q[k] = sm.place[min(s0 + k, BK_CAP - 1)];
+            for (int j = 0; j < BK_IPT; ++j) {
+                const int i = tid + j * BK_NT;
+                if (i < n) {
                     int less = 0;
+                    for (int c = s0[j]; c < s1[j]; c += BK_SORT_WIN) {
+                        uint64_t q[BK_SORT_WIN];
 #pragma unroll
-                    for (int k = 0; k < BK_RANK_WIN; ++k) less += (s0 + k < s1 && q[k] < w[j]) ? 1 : 0;
-                    for (int k = s0 + BK_RANK_WIN; k < s1; ++k) less += sm.place[k] < w[j];
-                    sm.sk[s0 + less] = w[j];
-                    sm.sv[s0 + less] = cv[j];
+                        for (int k = 0; k < BK_SORT_WIN; ++k) q[k] = c + k < s1[j] ? sm.place[c + k] : ~0ull;
+#pragma unroll
+                        for (int k = 0; k < BK_SORT_WIN; ++k) less += q[k] < w[j] ? 1 : 0;
+                    }
+                    sm.sk[s0[j] + less] = w[j];
                 }
                 __builtin_amdgcn_sched_barrier(0);  // one item's window in registers at a time
             }
             __syncthreads();
-            // counters of the next bucket start at zero (their last reader was the rank above)
-            for (int j = tid; j < BK_NSUB + 1; j += BK_NT) sm.sub[j] = 0;
 
-            // ---- 3. fold: thread t owns the runs of equal keys whose first position lies in
+            // ---- 4. fold: thread t owns the runs of equal keys whose first position lies in
             // [t * BK_IPT, (t + 1) * BK_IPT).  It reads those positions and BK_FOLD_EXTRA more
-            // at once and adds each run left to right -- (0 + the diagonal value, tag 0, if
-            // any) + the spawns in list order -- in one pass over the window; a run's result
-            // is taken at its last position k, into slot min(k, BK_IPT - 1) (only one owned
-            // run can end at or after BK_IPT - 1), so the slots keep the runs in key order.
-            // A run going past the window reads on from LDS.
+            // at once and adds each run left to right -- (0 + the diagonal value, if any) +
+            // the spawns in list order -- in one pass over the window; a run's result is taken
+            // at its last position k, into slot min(k, BK_IPT - 1) (only one owned run can end
+            // at or after BK_IPT - 1), so the slots keep the runs in key order.  A run going
+            // past the window reads on from LDS.
             const int p0 = tid * BK_IPT;
             constexpr int NWIN = BK_IPT + BK_FOLD_EXTRA;
             constexpr uint64_t NOKEY = ~0ull;  // no position: differs from every key (< 2^52)
@@ -245,10 +279,11 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
 #pragma unroll
             for (int k = 0; k <= NWIN; ++k) {
                 const int p = p0 + k;
-                kk[k] = p < n ? sm.sk[p] >> BK_TAG_BITS : NOKEY;
-                if (k < NWIN) vv[k] = p < n ? sm.sv[p] : (V)0;
+                const uint64_t wd = p < n ? sm.sk[p] : NOKEY;
+                kk[k] = p < n ? wd >> BK_IDX_BITS : NOKEY;
+                if (k < NWIN) vv[k] = p < n ? sm.vin[(int)(wd & IDX_MASK)] : (V)0;
             }
-            const uint64_t kprev = p0 > 0 && p0 < n ? sm.sk[p0 - 1] >> BK_TAG_BITS : NOKEY;
+            const uint64_t kprev = p0 > 0 && p0 < n ? sm.sk[p0 - 1] >> BK_IDX_BITS : NOKEY;
             bool own = false;
             V acc = (V)0;
 #pragma unroll
@@ -268,14 +303,19 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
             }
             if (own) {  // the owned run goes on past the window: it ends in slot BK_IPT - 1
                 const uint64_t key = kk[NWIN - 1];
-                for (int q = p0 + NWIN; q < n && (sm.sk[q] >> BK_TAG_BITS) == key; ++q) acc = acc + sm.sv[q];
+                for (int q = p0 + NWIN; q < n; ++q) {
+                    const uint64_t wd = sm.sk[q];
+                    if ((wd >> BK_IDX_BITS) != key) break;
+                    acc = acc + sm.vin[(int)(wd & IDX_MASK)];
+                }
                 okey[BK_IPT - 1] = bucket_key | key;
                 oval[BK_IPT - 1] = acc;
                 if (acc != (V)0) cvalid |= 1u << (BK_IPT - 1);
             }
             n_before = __popc(cvalid);
-            __syncthreads();  // the sorted words and values are no longer read
+            __syncthreads();  // the sorted words and the values are no longer read
         }
+        // the next bucket's items travel while this one is compressed and written out
         if (bn < nbk) load(tid, bn, nk, nv, n_r0, n_v0, n_na, n_nb);
         if (!fits) {
             if (tid == 0) {
@@ -286,8 +326,6 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
                 partials[2 * nbk + b] = 0;
             }
         } else {
-            uint64_t* stage_k = sm.place;
-            V* stage_v = sm.sv;
             if constexpr (!is_int_v<V>::value) {
                 if (a.compression == 1) {
                     // ThresholdCompression: each wavefront lists its values below the threshold
@@ -298,11 +336,11 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
                     for (int j = 0; j < BK_IPT; ++j)
                         if (((cvalid >> j) & 1u) && fabs((double)oval[j]) / th < 1) need |= 1u << j;
                     const int nneed = __popc(need);
-                    const int incl = wave_inclusive_scan(nneed);
+                    const int incl = bk_wave_inclusive_scan(nneed, lane);
                     const int wtot = __shfl(incl, WAVE - 1, WAVE);
                     const int mybase = incl - nneed;
-                    uint64_t* wk = stage_k + wid * (WAVE * BK_IPT);
-                    V* wv = stage_v + wid * (WAVE * BK_IPT);
+                    uint64_t* wk = sm.place + wid * (WAVE * BK_IPT);
+                    V* wv = sm.vin + wid * (WAVE * BK_IPT);
 #pragma unroll
                     for (int j = 0; j < BK_IPT; ++j) {
                         if ((need >> j) & 1u) {
@@ -312,10 +350,13 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
                         }
                     }
                     __syncthreads();
+                    // the seed re-read here: its round keys are not held in registers across buckets
+                    uint64_t seed;
+                    asm volatile("s_mov_b64 %0, %1" : "=s"(seed) : "s"(a.seed));
                     for (int q = lane; q < wtot; q += WAVE) {
                         const uint64_t key = wk[q];
                         const double x = (double)wv[q];
-                        const U4 r = draw(a.seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
+                        const U4 r = draw(seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
                         wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
                     }
                     __syncthreads();
@@ -333,45 +374,40 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(S
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j)
                 if ((cvalid >> j) & 1u) nrm += fabs((double)oval[j]);
-            // ---- 4. survivors in order, staged for coalesced writes ------------------------
+            // ---- 5. survivors in order: each thread writes its own after those of the threads
+            // before it (a wavefront's writes cover one contiguous range) ---------------------
             const int count = __popc(cvalid);
             int btotal;
-            const int ex = block_exclusive_scan<int, BK_NT>(count, &btotal);  // ends with a barrier
+            const int ex = bk_block_exclusive_scan<int>(count, lane, wid, sm.scan_i, &btotal);
+            const int64_t base = r0 + v0 + ex;  // the bucket's slot range: its merged positions
 #pragma unroll
             for (int j = 0; j < BK_IPT; ++j) {
                 if ((cvalid >> j) & 1u) {
-                    const int q = ex + __popc(cvalid & ((1u << j) - 1u));
-                    stage_k[q] = okey[j];
-                    stage_v[q] = oval[j];
+                    const int q = __popc(cvalid & ((1u << j) - 1u));
+                    slot_k[base + q] = okey[j];
+                    slot_v[base + q] = oval[j];
                 }
             }
-            // per-bucket sums: counts (exact in any order) and the norm (fixed order:
-            // wavefront trees, then the wavefronts in order)
-            int nb_w = n_before;
+            // per-bucket sums: counts from ballots (n_before <= BK_IPT < 8: three bits), the
+            // norm in a fixed order (wavefront trees, then the wavefronts in order)
+            const int nb_w = __popcll(__ballot(n_before & 1)) + 2 * __popcll(__ballot(n_before & 2)) +
+                             4 * __popcll(__ballot(n_before & 4));
 #pragma unroll
-            for (int off = WAVE / 2; off > 0; off >>= 1) {
-                nb_w += __shfl_xor(nb_w, off, WAVE);
-                nrm += __shfl_xor(nrm, off, WAVE);
-            }
+            for (int off = WAVE / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off, WAVE);
             if (lane == 0) {
                 sm.red[0][wid] = (double)nb_w;
-                sm.red[2][wid] = nrm;
+                sm.red[1][wid] = nrm;
             }
             __syncthreads();
-            const int64_t base = r0 + v0;  // the bucket's slot range: its merged positions
-            for (int i = tid; i < btotal; i += BK_NT) {
-                slot_k[base + i] = stage_k[i];
-                slot_v[base + i] = stage_v[i];
-            }
             if (tid == 0) {
-                double s0 = 0, s2 = 0;
-                for (int q = 0; q < NW; ++q) { s0 += sm.red[0][q]; s2 += sm.red[2][q]; }
+                double s0 = 0, s1 = 0;
+                for (int q = 0; q < NW; ++q) { s0 += sm.red[0][q]; s1 += sm.red[1][q]; }
                 bucket_cnt[b] = btotal;
                 partials[b] = s0;
                 partials[nbk + b] = (double)btotal;
-                partials[2 * nbk + b] = s2;
+                partials[2 * nbk + b] = s1;
             }
-            __syncthreads();  // staging and sums read before the next bucket writes LDS
+            __syncthreads();  // sums read before the next bucket writes LDS
         }
 #pragma unroll
         for (int j = 0; j < BK_IPT; ++j) { ck[j] = nk[j]; cv[j] = nv[j]; }

@@ -536,7 +536,7 @@ int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result, Fill&
 constexpr double BK_FILL = 1800;    // mean items (records + entries) per bucket aimed at
 constexpr double BK_FILL_MAX = 2050; // highest mean kept to save a radix pass (LDS holds BK_CAP)
 constexpr int64_t BK_MIN_RECORDS = 4096;
-constexpr int BK_MAX_LOWBITS = 48;   // packed words: key bits below the bucket bits + a 12-bit tag
+constexpr int BK_MAX_LOWBITS = 48;   // packed words: key bits below the bucket bits + a 12-bit item index
 constexpr int BK_COOLDOWN = 16;      // steps on the merge path after a bucket overflowed
 
 // Top bits that split S records and L entries into buckets of about BK_FILL items; one bit
