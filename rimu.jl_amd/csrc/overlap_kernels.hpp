@@ -8,8 +8,9 @@
 // searches in t's sorted keys: t[k] for u's own keys within the range of t that the tile's
 // first and last key bound, staged in LDS when it holds at most OV_LDS keys; t[k'] for
 // neighbours over all of t, the first levels taken from OV_SPLIT evenly spaced keys of t
-// staged in LDS.  Per-workgroup partial sums are reduced in a
-// fixed order (deterministic).
+// staged in LDS -- or, when the keys are stored mixed (spread evenly over their B bits), from
+// an index of t by the top P key bits, which leaves a range of a few dozen keys to search.
+// Per-workgroup partial sums are reduced in a fixed order (deterministic).
 #pragma once
 
 #include "step_kernels.hpp"
@@ -21,6 +22,8 @@ constexpr int OV_TILE = NT * OV_IPT;
 constexpr int OV_SPLIT = 1024;
 constexpr int OV_MAX_OPS = 8;
 constexpr int OV_LDS = 3072;   // t's keys of a tile's range staged in LDS when they fit
+constexpr int OV_INDEX_MAX_BITS = 22;   // top-bit index of t for mixed keys: at most 2^22 + 1 entries
+constexpr int64_t OV_INDEX_MIN = 1 << 16;  // smaller t: the LDS splitters suffice
 template <class K> struct ov_lds { static constexpr int value = sizeof(K) == 8 ? OV_LDS : 2048; };
 
 enum { OP_HAMILTONIAN = 1, OP_G2_REAL = 2, OP_DENSITY = 3 };
@@ -84,18 +87,41 @@ __device__ inline int64_t find_split(const K* __restrict__ keys, int64_t L, cons
     return find_in(keys, a, b, key);
 }
 
+// idx[b] = the first position in keys[0, L) whose top P of the B key bits are at least b,
+// b = 0 .. 2^P (idx[2^P] = L).  Position i writes idx[b] for b in (top(i - 1), top(i)]
+// (i = 0 from b = 0; i = L up to 2^P), one coalesced pass over the keys.
+template <class K>
+__global__ __launch_bounds__(NT) void overlap_index_kernel(const K* __restrict__ keys, int64_t L, int B, int P,
+                                                           int64_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i > L) return;
+    const int sh = B - P;
+    const int64_t nb = (int64_t)1 << P;
+    const int64_t lo = i == 0 ? 0 : (int64_t)(keys[i - 1] >> sh) + 1;
+    const int64_t hi = i == L ? nb : (int64_t)(keys[i] >> sh);
+    for (int64_t b = lo; b <= hi; ++b) idx[b] = i;
+}
+
 // t[key] * diag * v plus the off-diagonal terms of a Hamiltonian (RS: HubbardRealSpace).
+// With an index (tidx != nullptr) a neighbour is searched in its top-bit range only.
 template <bool RS, class K>
 __device__ inline double ham_term(const ModelDev& h, const KeyMix& mix, K key, double tval, double v,
                                   const K* __restrict__ tk, const double* tvd,
-                                  const int64_t* tvi, int64_t Lt, const K* s_split) {
+                                  const int64_t* tvi, int64_t Lt, const K* s_split,
+                                  const int64_t* __restrict__ tidx, int tsh) {
     double res = tval * model_diagonal_element<RS>(h, key) * v;
     const int64_t n = model_num_offdiagonals<RS>(h, key);
     for (int64_t c = 1; c <= n; ++c) {
         double hv;
         const K nb = mix_key(mix, model_offdiagonal<RS>(h, key, c, &hv));
         if (hv == 0.0) continue;
-        const int64_t q = find_split(tk, Lt, s_split, nb);
+        int64_t q;
+        if (tidx) {
+            const int64_t b = (int64_t)(nb >> tsh);
+            q = find_in(tk, tidx[b], tidx[b + 1], nb);
+        } else {
+            q = find_split(tk, Lt, s_split, nb);
+        }
         if (q >= 0) res += (tvd ? tvd[q] : (double)tvi[q]) * hv * v;
     }
     return res;
@@ -106,12 +132,13 @@ __device__ inline double ham_term(const ModelDev& h, const KeyMix& mix, K key, d
 template <class K>
 __device__ __noinline__ double op_term(const OpDev& op, int M, K skey, double tval, double v,
                                        const K* __restrict__ tk, const double* tvd,
-                                       const int64_t* tvi, int64_t Lt, const K* s_split) {
+                                       const int64_t* tvi, int64_t Lt, const K* s_split,
+                                       const int64_t* __restrict__ tidx, int tsh) {
     const K key = unmix_key(op.mix, skey);  // the address
     if (op.kind == OP_G2_REAL) return tval * g2_real(key, M, op.param) * v;
     if (op.kind == OP_DENSITY) return tval * (double)occupation(key, op.param) * v;
-    if (op.h.kind == 3) return ham_term<true>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split);
-    return ham_term<false>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split);
+    if (op.h.kind == 3) return ham_term<true>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split, tidx, tsh);
+    return ham_term<false>(op.h, op.mix, key, tval, v, tk, tvd, tvi, Lt, s_split, tidx, tsh);
 }
 
 // ranges[t] = lower bound in t of u's key at t * OV_TILE (t < ntiles), ranges[ntiles] = Lt.
@@ -133,7 +160,8 @@ __global__ __launch_bounds__(NT) void overlap_ranges_kernel(const K* __restrict_
 }
 
 // partials[o * gridDim.x + block], o = 0 (dot) .. nops (operators).  Lt >= 1, Lu >= 1.
-// need_split: an operator has off-diagonals (the splitters are only read then).
+// need_split: an operator has off-diagonals (the splitters are only read then, and not at
+// all when the top-bit index tidx is given: neighbour b's range is tidx[b >> tsh ...]).
 template <class K, class VT, class VU>
 __global__ __launch_bounds__(NT) void overlap_kernel(const K* __restrict__ tk,
                                                      const VT* __restrict__ tv, int64_t Lt,
@@ -141,13 +169,14 @@ __global__ __launch_bounds__(NT) void overlap_kernel(const K* __restrict__ tk,
                                                      const VU* __restrict__ uv, int64_t Lu,
                                                      const OpDev* __restrict__ ops, int nops, int M,
                                                      const int64_t* __restrict__ ranges, int need_split,
+                                                     const int64_t* __restrict__ tidx, int tsh,
                                                      double* __restrict__ partials) {
     constexpr int LDSN = ov_lds<K>::value;
     __shared__ K s_split[OV_SPLIT];
     __shared__ OpDev s_ops[OV_MAX_OPS];
     __shared__ K s_lk[LDSN];
     const int64_t t0 = (int64_t)blockIdx.x * OV_TILE;
-    if (need_split)
+    if (need_split && !tidx)
         for (int j = threadIdx.x; j < OV_SPLIT; j += NT) s_split[j] = tk[(int64_t)j * Lt / OV_SPLIT];
     for (int j = threadIdx.x; j < nops * (int)(sizeof(OpDev) / 8); j += NT)
         ((uint64_t*)s_ops)[j] = ((const uint64_t*)ops)[j];
@@ -193,7 +222,8 @@ __global__ __launch_bounds__(NT) void overlap_kernel(const K* __restrict__ tk,
         acc[0] += tval * v[j];
 #pragma unroll
         for (int o = 0; o < OV_MAX_OPS; ++o)
-            if (o < nops) acc[1 + o] += op_term<K>(s_ops[o], M, k[j], tval, v[j], tk, tvd, tvi, Lt, s_split);
+            if (o < nops)
+                acc[1 + o] += op_term<K>(s_ops[o], M, k[j], tval, v[j], tk, tvd, tvi, Lt, s_split, tidx, tsh);
     }
 #pragma unroll
     for (int o = 0; o <= OV_MAX_OPS; ++o) {

@@ -118,7 +118,7 @@ struct rimu_engine {
     // overlaps (this engine as the right-hand vector)
     DevBuf<unsigned char> ov_ops;
     DevBuf<double> ov_part, ov_out;
-    DevBuf<int64_t> ov_ranges;
+    DevBuf<int64_t> ov_ranges, ov_index;
     DevBuf<int64_t> fix_runs;       // runs to reorder after a sort on the top key bits
     DevBuf<int64_t> bk_rstart, bk_vstart;  // bucket merge: first record / vector entry per bucket
     DevBuf<uint64_t> lb_status;     // look-back radix: status words [pass][tile][digit]
@@ -980,7 +980,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->rtmp.release(); e->plan_status.release(); e->tile_or.release(); e->tile_and.release(); e->plan_p.release(); e->spawn_p.release();
     e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
-    e->ov_ops.release(); e->ov_part.release(); e->ov_out.release(); e->ov_ranges.release();
+    e->ov_ops.release(); e->ov_part.release(); e->ov_out.release(); e->ov_ranges.release(); e->ov_index.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);
     if (e->h_stats) (void)hipHostFree(e->h_stats);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1205,21 +1205,34 @@ int overlaps_impl(rimu_engine* left, rimu_engine* right, const rimu_operator* op
     const OpDev* dp = (const OpDev*)e->ov_ops.p;
     int need_split = 0;
     for (int o = 0; o < nops; ++o) need_split |= dops[o].kind == RIMU_OP_HAMILTONIAN;
+    // mixed keys are spread evenly over their B bits: neighbours are looked up through an
+    // index of t by the top P bits, about 16 keys per index entry
+    const int64_t* tidx = nullptr;
+    int tsh = 0;
     hipEvent_t ev{};
     e->timer.begin("overlap", e->stream, &ev);
+    if (need_split && right->mix.on && Lt >= OV_INDEX_MIN) {
+        const int B = right->hdev.B;
+        int P = (int)std::floor(std::log2((double)Lt / 16.0));
+        P = std::max(1, std::min({P, OV_INDEX_MAX_BITS, B}));
+        tsh = B - P;
+        HIP_TRY(e->ov_index.alloc(((size_t)1 << P) + 1));
+        overlap_index_kernel<K><<<ceil_div(Lt + 1, NT), NT, 0, e->stream>>>(tk, Lt, B, P, e->ov_index.p);
+        tidx = e->ov_index.p;
+    }
     overlap_ranges_kernel<K><<<ceil_div(nb + 1, NT), NT, 0, e->stream>>>(tk, Lt, uk, nb, e->ov_ranges.p);
     if (left->int_vals && right->int_vals)
         overlap_kernel<K, int64_t, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
-            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, tidx, tsh, e->ov_part.p);
     else if (left->int_vals)
         overlap_kernel<K, int64_t, double><<<nb, NT, 0, e->stream>>>(tk, (const int64_t*)tv, Lt, uk,
-            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, tidx, tsh, e->ov_part.p);
     else if (right->int_vals)
         overlap_kernel<K, double, int64_t><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
-            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+            (const int64_t*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, tidx, tsh, e->ov_part.p);
     else
         overlap_kernel<K, double, double><<<nb, NT, 0, e->stream>>>(tk, (const double*)tv, Lt, uk,
-            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, e->ov_part.p);
+            (const double*)uv, Lu, dp, nops, M, e->ov_ranges.p, need_split, tidx, tsh, e->ov_part.p);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     overlap_final_kernel<<<nops + 1, NT, 0, e->stream>>>(e->ov_part.p, nb, e->ov_out.p);

@@ -52,6 +52,10 @@ CASES = [
     ("mom", 8, 6, 1500, 1200, "int", "float", True),
     ("mom", 12, 12, 4000, 4000, "float", "float", False),
     ("real", 32, 32, 3000, 3000, "int", "int", False),
+    # mixed keys (23 bits) and a left vector of more than 2^16 entries: neighbours are looked
+    # up through the top-bit index of the left keys (6% of the space: ~1.4 hits per entry)
+    ("real", 12, 12, 80000, 70000, "float", "float", False),
+    ("mom", 12, 12, 90000, 50000, "int", "float", False),
 ]
 
 
