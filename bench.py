@@ -744,6 +744,8 @@ def main():
     ap.add_argument("--cpu-sample-entries", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    ap.add_argument("--kernel-steps", type=int, default=10,
+                    help="steps after the timed ones with per-kernel HIP events (kernel times, roofline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for HBM byte counters)")
     ap.add_argument("--batched-runs", type=int, default=256)
@@ -850,33 +852,45 @@ def main():
         st = eng.step(step, sp.shift, dtau)
         strat.update(sp, st.norm)
 
-    eng.set_timing(not args.no_timing)
+    # the timed steps run without per-kernel events (`value`, `ms_per_step`); the kernel
+    # breakdown and the roofline come from the steps after them, with events on the stream
+    eng.set_timing(False)
     attempts = 0.0
-    S_tot = L_tot = Lnew_tot = 0.0
-    alg_bytes = 0.0
-    L_prev = len(eng)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step += 1
         st = eng.step(step, sp.shift, dtau)
         strat.update(sp, st.norm)
         attempts += st.spawn_attempts
-        b, records = eng.last_traffic()
-        alg_bytes += b
-        S_tot += records - L_prev
-        L_tot += L_prev
-        Lnew_tot += st.len
-        L_prev = st.len
     elapsed = time.perf_counter() - t0
-    times = eng.kernel_times() if not args.no_timing else {}
     K = args.steps
     value = attempts / elapsed
     ms_per_step = 1e3 * elapsed / K
 
+    times = {}
+    S_tot = L_tot = Lnew_tot = 0.0
+    alg_bytes = 0.0
+    KT = 0 if args.no_timing else args.kernel_steps
+    if KT > 0:
+        eng.set_timing(True)
+        L_prev = len(eng)
+        for _ in range(KT):
+            step += 1
+            st = eng.step(step, sp.shift, dtau)
+            strat.update(sp, st.norm)
+            b, records = eng.last_traffic()
+            alg_bytes += b
+            S_tot += records - L_prev
+            L_tot += L_prev
+            Lnew_tot += st.len
+            L_prev = st.len
+        times = eng.kernel_times()
+        eng.set_timing(False)
+
     roofline = None
     kernels = {}
     if times:
-        roofline, kernels = kernel_rooflines(times, S_tot / K, L_tot / K, Lnew_tot / K, K,
+        roofline, kernels = kernel_rooflines(times, S_tot / KT, L_tot / KT, Lnew_tot / KT, KT,
                                              MEASURED_TRAFFIC_RATIO if args.target == 1e8 else None)
         if roofline is not None:
             roofline["traffic_source"] = (
@@ -967,10 +981,11 @@ def main():
         "config": {"workload": "HubbardReal1D M=N=20 u=6 t=1, IsDynamicSemistochastic FP64, "
                                "target_walkers=1e8, dtau=1e-3, seed 17",
                    "equilibration_steps": args.equilibration, "parallelism": "single GPU"},
-        "step_alg_roofline_frac": alg_bytes / elapsed / HBM_PEAK,
+        "step_alg_roofline_frac": (alg_bytes / KT) / (ms_per_step / 1e3) / HBM_PEAK if KT else None,
         "walkers": final_norm,
         "len": final_len,
-        "spawn_records_per_step": S_tot / K,
+        "spawn_records_per_step": S_tot / KT if KT else None,
+        "kernel_steps": KT,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

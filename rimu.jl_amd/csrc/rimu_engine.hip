@@ -712,8 +712,9 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
                 e->tile_first.p, (K*)e->tile_or.p, (K*)e->tile_and.p);
         e->timer.end(e->stream);
         HIP_TRY(hipGetLastError());
-        key_bits_kernel<K><<<1, 1024, 0, e->stream>>>((const K*)e->tile_or.p, (const K*)e->tile_and.p,
-                                                      spawn_tiles, (K*)&dm->key_or, (K*)&dm->key_and);
+        if (!spec)  // a speculative step sorts every digit and reads no OR/AND back
+            key_bits_kernel<K><<<1, 1024, 0, e->stream>>>((const K*)e->tile_or.p, (const K*)e->tile_and.p,
+                                                          spawn_tiles, (K*)&dm->key_or, (K*)&dm->key_and);
         if (!dense) {
             e->timer.begin("spawn_compact", e->stream, &ev);
             if (int rc = scan_inplace_i64(e, e->tile_cnt.p, spawn_tiles + 1)) return rc;
