@@ -152,6 +152,7 @@ def run_batched_leg(rimu, args):
     ed = batched_ed_check(rimu, specs, hist, st, skip=nsteps // 4, details=args.batched_details)
 
     plain_s = 0.0
+    plain_per_run = {}
     sample_steps = args.plain_sample_steps
     for r, sp in enumerate(specs):
         d = sp.descriptor()
@@ -167,9 +168,31 @@ def run_batched_leg(rimu, args):
         t1 = time.perf_counter()
         for i in range(sample_steps):
             eng.step(nsteps + 4 + i, shift, d.dtau)
-        plain_s += (time.perf_counter() - t1) / sample_steps * nsteps
+        plain_per_run[r] = (time.perf_counter() - t1) / sample_steps * nsteps
+        plain_s += plain_per_run[r]
         eng.close()
     b.close()
+    # the extrapolation checked end to end on a sample of the runs: each sampled run from its
+    # start address through all nsteps on the plain engine (host DoubleLogUpdate), timed whole
+    sample = list(range(0, len(specs), max(1, len(specs) // args.plain_e2e_runs)))[:args.plain_e2e_runs]
+    e2e_s = est_s = 0.0
+    for r in sample:
+        sp = specs[r]
+        d = sp.descriptor()
+        if st[r] != 0:
+            continue
+        eng = rimu._native.Engine(sp.hamiltonian.descriptor(), rimu.IsStochasticInteger().descriptor(),
+                                  d.seed, d.max_length + 1, int(min(4 * d.target_walkers + 4096, 2 ** 32 - 2)))
+        eng.upload(np.array([d.start_address], np.uint64), np.array([d.start_population]))
+        strat = rimu.DoubleLogUpdate(target_walkers=d.target_walkers, zeta=d.zeta, xi=d.xi)
+        shp = rimu.strategies.DefaultShiftParameters(d.shift, float(abs(d.start_population)), d.dtau)
+        t1 = time.perf_counter()
+        for i in range(1, nsteps + 1):
+            stp = eng.step(i, shp.shift, d.dtau)
+            strat.update(shp, stp.norm)
+        e2e_s += time.perf_counter() - t1
+        est_s += plain_per_run[r]
+        eng.close()
     solves = len(specs) / elapsed
     plain = len(specs) / plain_s
     return {"metric": "solves/s", "value": solves, "plain_value": plain, "speedup": solves / plain,
@@ -181,7 +204,11 @@ def run_batched_leg(rimu, args):
                          "ms_per_step": {f"{k[0]}x{k[1]}": v for k, v in table.items()}},
             "plain_estimate": (f"each run's steady-state step time on the plain engine "
                                f"({sample_steps} steps from the sweep's final vector) x {nsteps} steps, "
-                               f"summed over runs: {plain_s:.1f} s")}
+                               f"summed over runs: {plain_s:.1f} s"),
+            "plain_end_to_end_check": {"runs": sample, "measured_s": e2e_s, "estimated_s": est_s,
+                                       "measured_over_estimated": e2e_s / est_s if est_s else None,
+                                       "speedup_scaled_to_measured": (solves / plain) * (e2e_s / est_s)
+                                       if est_s else None}}
 
 
 def batched_ed_check(rimu, specs, hist, status, skip, details=None):
@@ -721,6 +748,15 @@ def run_adjoint_leg(rimu, args):
     t_step = (time.perf_counter() - t1) / args.plain_sample_steps
     eng.close()
     plain_s = len(us) * 4 * K * t_step
+    # one of those runs end to end (from the start address, K steps) against K x t_step
+    eng = rimu._native.Engine(h.descriptor(), rimu.IsDeterministic().descriptor(), 1,
+                              len(basis) + 1, int(len(basis) * (2 * N + 1) + 4096))
+    eng.upload(np.array([h.starting_address().bits], np.uint64), np.array([1.0]))
+    t2 = time.perf_counter()
+    for i in range(K):
+        eng.step(i + 1, 0.0, dtau)
+    e2e = time.perf_counter() - t2
+    eng.close()
     return {"metric": "grads/s", "value": grads, "plain_value": len(us) / plain_s,
             "speedup": grads / (len(us) / plain_s), "seconds": elapsed, "dim": len(basis),
             "K": K, "checkpoint_every": ckpt, "u": us,
@@ -728,7 +764,10 @@ def run_adjoint_leg(rimu, args):
             "dE_dt": res["grad_t"].tolist(),
             "plain_estimate": (f"central differences on the plain engine: 4 runs x {K} "
                                f"IsDeterministic steps per u at {1e3 * t_step:.3f} ms/step "
-                               f"(measured on the full vector), {plain_s:.1f} s")}
+                               f"(measured on the full vector), {plain_s:.1f} s"),
+            "plain_end_to_end_check": {"measured_s": e2e, "estimated_s": K * t_step,
+                                       "measured_over_estimated": e2e / (K * t_step),
+                                       "speedup_scaled_to_measured": grads / (len(us) / plain_s) * e2e / (K * t_step)}}
 
 
 def main():
@@ -753,6 +792,8 @@ def main():
     ap.add_argument("--batched-tune-at", type=int, default=1000)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--plain-sample-steps", type=int, default=20)
+    ap.add_argument("--plain-e2e-runs", type=int, default=8,
+                    help="sweep runs also run end to end on the plain engine to check the estimate")
     ap.add_argument("--mom-target", type=float, default=2e7)
     ap.add_argument("--mom-equilibration", type=int, default=12_000,
                     help="cap on growth steps of the momentum-space run")
@@ -930,6 +971,11 @@ def main():
     if batched:
         log(f"batched: {batched['value']:.3g} solves/s vs plain {batched['plain_value']:.3g} "
             f"({batched['speedup']:.1f}x)")
+        chk = batched["plain_end_to_end_check"]
+        if chk["measured_over_estimated"] is not None:
+            log(f"batched: plain end to end on {len(chk['runs'])} runs {chk['measured_s']:.1f} s against "
+                f"{chk['estimated_s']:.1f} s estimated ({chk['measured_over_estimated']:.2f}); speedup at the "
+                f"measured rate {chk['speedup_scaled_to_measured']:.1f}x")
     momentum = None if args.no_momentum else run_momentum_leg(rimu, args)
     if momentum:
         log(f"momentum: {momentum['value']:.3g} attempts/s, {momentum['ms_per_step']:.2f} ms/step")
@@ -964,6 +1010,10 @@ def main():
     if adjoint:
         log(f"adjoint: {adjoint['value']:.3g} grads/s vs plain {adjoint['plain_value']:.3g} "
             f"({adjoint['speedup']:.1f}x)")
+        chk = adjoint["plain_end_to_end_check"]
+        log(f"adjoint: one plain run end to end {chk['measured_s']:.2f} s against {chk['estimated_s']:.2f} s "
+            f"estimated ({chk['measured_over_estimated']:.2f}); speedup at the measured rate "
+            f"{chk['speedup_scaled_to_measured']:.1f}x")
 
     out = {
         "metric": "spawned walkers/s & HBM % roofline; batched-U solves/s + grads/s vs plain",
