@@ -101,11 +101,12 @@ __device__ inline T bk_block_exclusive_scan(T v, int lane, int wid, T* s_wave, T
 // LDS of one bucket workgroup.  `vin` holds the items' values by item index and `place`
 // their packed words sub-bucket by sub-bucket; `sk` receives the words in order.  After the
 // fold `place` and `vin` stage the values that ThresholdCompression draws for.
-template <class V>
+template <class V, bool INIT>
 struct BucketLds {
     uint64_t place[BK_CAP];
     uint64_t sk[BK_CAP];
     V vin[BK_CAP];
+    uint8_t cls[INIT ? BK_CAP : 1];  // initiator runs: the class of every item (see below)
     uint32_t sub[BK_NSUB + 1];      // sub-bucket counts, then their starts
     double red[2][BK_NT / WAVE];
     uint32_t scan_u[BK_NT / WAVE];
@@ -116,15 +117,20 @@ struct BucketLds {
 // the record and vector ranges.  Outputs per bucket: survivors at slot[rstart[b] + vstart[b]
 // + i], their count in bucket_cnt[b], and partials[b], [nbk + b], [2 nbk + b] = len_before,
 // len, norm.
-template <class V>
+// INIT (initiator rules): a record's key carries CLASS_BIT when its parent was not above the
+// initiator threshold, a vector entry has the class when its old value bo is not above it;
+// each address folds three components in list order -- safe (entries with the class, records
+// without), unsafe (records with the class), initiator (entries without) -- exists when one
+// is nonzero and takes the rule's value (initiator_value), as merge_kernel<K, V, true> does.
+template <class V, bool INIT>
 __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
     StepArgs a, const uint64_t* __restrict__ ak, const V* __restrict__ av, const uint64_t* __restrict__ bk,
-    const V* __restrict__ bv, const int64_t* __restrict__ rstart, const int64_t* __restrict__ vstart, int64_t nbk,
+    const V* __restrict__ bv, const V* __restrict__ bo, const int64_t* __restrict__ rstart, const int64_t* __restrict__ vstart, int64_t nbk,
     int lowbits, uint64_t* __restrict__ slot_k, V* __restrict__ slot_v, int64_t* __restrict__ bucket_cnt,
     double* __restrict__ partials, int* flags) {
     constexpr int NW = BK_NT / WAVE;
     constexpr uint64_t IDX_MASK = (1ull << BK_IDX_BITS) - 1ull;
-    __shared__ BucketLds<V> sm;
+    __shared__ BucketLds<V, INIT> sm;
     // no sorted list to merge (raised by earlier kernels only: every wave reads the same bits)
     if (*flags & (FLAG_LOOKBACK | FLAG_SPEC)) {
         for (int64_t b = blockIdx.x; b < nbk; b += gridDim.x) {
@@ -150,9 +156,11 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
     int64_t b = blockIdx.x;
     uint64_t ck[BK_IPT];
     V cv[BK_IPT];
+    uint32_t ccls = 0;  // INIT: bit j = class of item slot j
     int64_t r0 = 0, v0 = 0;
     int na = 0, nb = 0;
-    auto load = [&](int t, int64_t bb, uint64_t* qk, V* qv, int64_t& q_r0, int64_t& q_v0, int& q_na, int& q_nb) {
+    auto load = [&](int t, int64_t bb, uint64_t* qk, V* qv, uint32_t& q_cls, int64_t& q_r0, int64_t& q_v0,
+                    int& q_na, int& q_nb) {
         q_r0 = rstart[bb];
         q_v0 = vstart[bb];
         q_na = (int)(rstart[bb + 1] - q_r0);
@@ -163,20 +171,27 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
         const V* ev_b = bv + q_v0;
         const uint64_t* rk_b = ak + q_r0;
         const V* rv_b = av + q_r0;
+        q_cls = 0;
 #pragma unroll
         for (int j = 0; j < BK_IPT; ++j) {
             const uint32_t i = (uint32_t)t + (uint32_t)(j * BK_NT);
             if (i < (uint32_t)q_nb) {
                 qk[j] = ek_b[i];
                 qv[j] = ev_b[i];
+                if constexpr (INIT) {
+                    if (below_initiator_threshold(a, (double)bo[q_v0 + i])) q_cls |= 1u << j;
+                }
             }
             if (i >= (uint32_t)q_nb && i < (uint32_t)q_n) {
                 qk[j] = rk_b[i - (uint32_t)q_nb];
                 qv[j] = rv_b[i - (uint32_t)q_nb];
+                if constexpr (INIT) {
+                    if (qk[j] & CLASS_BIT) q_cls |= 1u << j;
+                }
             }
         }
     };
-    if (b < nbk) load((int)threadIdx.x, b, ck, cv, r0, v0, na, nb);
+    if (b < nbk) load((int)threadIdx.x, b, ck, cv, ccls, r0, v0, na, nb);
     __syncthreads();  // sub-bucket counters cleared
 
     for (; b < nbk; b += gridDim.x) {
@@ -188,6 +203,7 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
         const int wid = __builtin_amdgcn_readfirstlane(tid / WAVE);
         uint64_t nk[BK_IPT];
         V nv[BK_IPT];
+        uint32_t ncls = 0;
         int64_t n_r0 = 0, n_v0 = 0;
         int n_na = 0, n_nb = 0;
         const int64_t bn = b + gridDim.x;
@@ -210,6 +226,7 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
                 if (i < n) {
                     w[j] = ((ck[j] & lowmask) << BK_IDX_BITS) | (uint64_t)i;
                     sm.vin[i] = cv[j];
+                    if constexpr (INIT) sm.cls[i] = (uint8_t)((ccls >> j) & 1u);
                     dg[j] = (uint32_t)(w[j] >> dsh) & dmask;
                     slot[j] = atomicAdd(&sm.sub[dg[j]], 1u);
                 }
@@ -276,47 +293,111 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
             constexpr uint64_t NOKEY = ~0ull;  // no position: differs from every key (< 2^52)
             uint64_t kk[NWIN + 1];
             V vv[NWIN];
+            uint32_t emask = 0, cmask = 0;  // INIT: window positions holding a vector entry / the class
 #pragma unroll
             for (int k = 0; k <= NWIN; ++k) {
                 const int p = p0 + k;
                 const uint64_t wd = p < n ? sm.sk[p] : NOKEY;
                 kk[k] = p < n ? wd >> BK_IDX_BITS : NOKEY;
-                if (k < NWIN) vv[k] = p < n ? sm.vin[(int)(wd & IDX_MASK)] : (V)0;
+                if (k < NWIN) {
+                    const int idx = (int)(wd & IDX_MASK);
+                    vv[k] = p < n ? sm.vin[idx] : (V)0;
+                    if constexpr (INIT) {
+                        if (p < n) {
+                            emask |= (uint32_t)(idx < nb) << k;
+                            cmask |= (uint32_t)sm.cls[idx] << k;
+                        }
+                    }
+                }
             }
             const uint64_t kprev = p0 > 0 && p0 < n ? sm.sk[p0 - 1] >> BK_IDX_BITS : NOKEY;
-            bool own = false;
-            V acc = (V)0;
+            if constexpr (!INIT) {
+                bool own = false;
+                V acc = (V)0;
 #pragma unroll
-            for (int k = 0; k < NWIN; ++k) {
-                const bool valid = p0 + k < n;
-                const bool start = valid && kk[k] != (k > 0 ? kk[k - 1] : kprev);
-                if (start) own = k < BK_IPT;
-                acc = (start ? (V)0 : acc) + vv[k];
-                const bool end = valid && kk[k + 1] != kk[k];
-                const int slot_k = k < BK_IPT - 1 ? k : BK_IPT - 1;
-                if (own && end) {
-                    okey[slot_k] = bucket_key | kk[k];
-                    oval[slot_k] = acc;
-                    if (acc != (V)0) cvalid |= 1u << slot_k;
-                    own = false;
+                for (int k = 0; k < NWIN; ++k) {
+                    const bool valid = p0 + k < n;
+                    const bool start = valid && kk[k] != (k > 0 ? kk[k - 1] : kprev);
+                    if (start) own = k < BK_IPT;
+                    acc = (start ? (V)0 : acc) + vv[k];
+                    const bool end = valid && kk[k + 1] != kk[k];
+                    const int slot_k = k < BK_IPT - 1 ? k : BK_IPT - 1;
+                    if (own && end) {
+                        okey[slot_k] = bucket_key | kk[k];
+                        oval[slot_k] = acc;
+                        if (acc != (V)0) cvalid |= 1u << slot_k;
+                        own = false;
+                    }
                 }
-            }
-            if (own) {  // the owned run goes on past the window: it ends in slot BK_IPT - 1
-                const uint64_t key = kk[NWIN - 1];
-                for (int q = p0 + NWIN; q < n; ++q) {
-                    const uint64_t wd = sm.sk[q];
-                    if ((wd >> BK_IDX_BITS) != key) break;
-                    acc = acc + sm.vin[(int)(wd & IDX_MASK)];
+                if (own) {  // the owned run goes on past the window: it ends in slot BK_IPT - 1
+                    const uint64_t key = kk[NWIN - 1];
+                    for (int q = p0 + NWIN; q < n; ++q) {
+                        const uint64_t wd = sm.sk[q];
+                        if ((wd >> BK_IDX_BITS) != key) break;
+                        acc = acc + sm.vin[(int)(wd & IDX_MASK)];
+                    }
+                    okey[BK_IPT - 1] = bucket_key | key;
+                    oval[BK_IPT - 1] = acc;
+                    if (acc != (V)0) cvalid |= 1u << (BK_IPT - 1);
                 }
-                okey[BK_IPT - 1] = bucket_key | key;
-                oval[BK_IPT - 1] = acc;
-                if (acc != (V)0) cvalid |= 1u << (BK_IPT - 1);
+                n_before = __popc(cvalid);
+            } else {
+                // three components per address (safe, unsafe, initiator), each its own left fold
+                V cs = (V)0, cu = (V)0, ci = (V)0;
+                bool own = false;
+                uint32_t cpres = 0;  // addresses with a nonzero component (len_before)
+                const bool keep_zero = is_int_v<V>::value || a.compression != 1;
+                // entries with the class and records without go to safe, records with it to
+                // unsafe, entries without it to initiator (selects: no divergent branches)
+                auto dep = [&](bool entry, bool c, V x) {
+                    const bool to_s = entry == c, to_u = !entry && c, to_i = entry && !c;
+                    cs = to_s ? cs + x : cs;
+                    cu = to_u ? cu + x : cu;
+                    ci = to_i ? ci + x : ci;
+                };
+                auto close_run = [&](int slot_k, uint64_t key) {
+                    if (cs != (V)0 || cu != (V)0 || ci != (V)0) {
+                        cpres |= 1u << slot_k;
+                        const V val = initiator_value<V>(a, cs, cu, ci);
+                        if (val != (V)0 || keep_zero) {
+                            okey[slot_k] = bucket_key | key;
+                            oval[slot_k] = val;
+                            cvalid |= 1u << slot_k;
+                        }
+                    }
+                };
+#pragma unroll
+                for (int k = 0; k < NWIN; ++k) {
+                    const int p = p0 + k;
+                    const bool valid = p < n;
+                    const bool start = valid && kk[k] != (k > 0 ? kk[k - 1] : kprev);
+                    if (start) {
+                        own = k < BK_IPT;
+                        cs = cu = ci = (V)0;
+                    }
+                    if (valid && own) dep((emask >> k) & 1u, (cmask >> k) & 1u, vv[k]);
+                    const bool end = valid && kk[k + 1] != kk[k];
+                    if (own && end) {
+                        close_run(k < BK_IPT - 1 ? k : BK_IPT - 1, kk[k]);
+                        own = false;
+                    }
+                }
+                if (own) {  // the owned run goes on past the window: it ends in slot BK_IPT - 1
+                    const uint64_t key = kk[NWIN - 1];
+                    for (int q = p0 + NWIN; q < n; ++q) {
+                        const uint64_t wd = sm.sk[q];
+                        if ((wd >> BK_IDX_BITS) != key) break;
+                        const int idx = (int)(wd & IDX_MASK);
+                        dep(idx < nb, sm.cls[idx] != 0, sm.vin[idx]);
+                    }
+                    close_run(BK_IPT - 1, key);
+                }
+                n_before = __popc(cpres);
             }
-            n_before = __popc(cvalid);
             __syncthreads();  // the sorted words and the values are no longer read
         }
         // the next bucket's items travel while this one is compressed and written out
-        if (bn < nbk) load(tid, bn, nk, nv, n_r0, n_v0, n_na, n_nb);
+        if (bn < nbk) load(tid, bn, nk, nv, ncls, n_r0, n_v0, n_na, n_nb);
         if (!fits) {
             if (tid == 0) {
                 atomicOr(flags, FLAG_BUCKET_OVERFLOW);
@@ -411,6 +492,7 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
         }
 #pragma unroll
         for (int j = 0; j < BK_IPT; ++j) { ck[j] = nk[j]; cv[j] = nv[j]; }
+        ccls = ncls;
         r0 = n_r0;
         v0 = n_v0;
         na = n_na;

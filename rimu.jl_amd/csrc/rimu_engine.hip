@@ -531,7 +531,8 @@ int reorder_on_device(rimu_engine* e, int64_t n, int forward, int* result, Fill&
 }
 
 // Bucket merge (bucket_kernels.hpp) of S spawn records sorted on their top T key bits with
-// the vector (bk, bv: keys and diagonal survivors, L entries) into (ok, ov) of capacity cap.
+// the vector (bk, bv: keys and diagonal survivors, L entries; bo: the old values, which give
+// the entries' initiator class when an initiator rule is set) into (ok, ov) of capacity cap.
 // `tiles` receives the number of buckets (rows of merge_p).
 constexpr double BK_FILL = 1800;    // mean items (records + entries) per bucket aimed at
 constexpr double BK_FILL_MAX = 2050; // highest mean kept to save a radix pass (LDS holds BK_CAP)
@@ -550,8 +551,8 @@ int bucket_bits(int64_t S, int64_t L, int B) {
 
 template <class V>
 int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, const V* av, int64_t S,
-                      const uint64_t* bk, const V* bv, int64_t L, int T, uint64_t* ok, V* ov, int64_t cap,
-                      rimu_engine::Misc* dm, int64_t* tiles, const int64_t* S_dev = nullptr) {
+                      const uint64_t* bk, const V* bv, const V* bo, int64_t L, int T, uint64_t* ok, V* ov,
+                      int64_t cap, rimu_engine::Misc* dm, int64_t* tiles, const int64_t* S_dev = nullptr) {
     const int B = e->hdev.B;
     const int64_t nbk = (int64_t)1 << T;
     *tiles = nbk;
@@ -573,9 +574,14 @@ int bucket_merge_into(rimu_engine* e, const StepArgs& a, const uint64_t* ak, con
     }
     const int64_t grid = std::min(nbk, (int64_t)e->n_cu * BK_WG_PER_CU);
     e->timer.begin("bucket_merge", e->stream, &ev);
-    bucket_merge_kernel<V><<<grid, BK_NT, 0, e->stream>>>(a, ak, av, bk, bv, e->bk_rstart.p, e->bk_vstart.p, nbk,
-                                                       B - T, e->slot_k.p, (V*)e->slot_v.p, e->tile_cnt.p,
-                                                       e->merge_p.p, &dm->flags);
+    if (a.init_rule != 0)
+        bucket_merge_kernel<V, true><<<grid, BK_NT, 0, e->stream>>>(
+            a, ak, av, bk, bv, bo, e->bk_rstart.p, e->bk_vstart.p, nbk, B - T, e->slot_k.p, (V*)e->slot_v.p,
+            e->tile_cnt.p, e->merge_p.p, &dm->flags);
+    else
+        bucket_merge_kernel<V, false><<<grid, BK_NT, 0, e->stream>>>(
+            a, ak, av, bk, bv, nullptr, e->bk_rstart.p, e->bk_vstart.p, nbk, B - T, e->slot_k.p, (V*)e->slot_v.p,
+            e->tile_cnt.p, e->merge_p.p, &dm->flags);
     e->timer.end(e->stream);
     HIP_TRY(hipGetLastError());
     e->timer.begin("compact", e->stream, &ev);
@@ -621,7 +627,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     const bool rs = e->hdev.kind == RIMU_MODEL_HUBBARD_REAL_SPACE;
     const bool dense = !e->int_vals && e->style.proj_threshold == 0 && (!rs || !rs_zero_channels(e->hdev));
     const int nbits = e->hdev.B;
-    const bool bucket_ok = KW == 1 && e->radix_mode == 2 && a.init_rule == 0 && e->mix.on && e->bucket_cooldown == 0;
+    const bool bucket_ok = KW == 1 && e->radix_mode == 2 && e->mix.on && e->bucket_cooldown == 0;
     bool spec = e->spec_ok && e->radix_mode != 0 && e->W_prev > 0;
     int T_spec = 0;
     if (spec) {  // the path the counts of the last step choose; sorts with a host check stay synchronous
@@ -768,7 +774,7 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     if (T > 0) {
         if (int rc = radix_sort<K, V>(e, S, nbits, kor ^ kand, 0, &sorted, nbits - T, S_dev)) return rc;
         if (int rc = bucket_merge_into<V>(e, a, (const uint64_t*)e->skeys[sorted].p, (const V*)e->svals[sorted].p,
-                                          S, (const uint64_t*)vk, (const V*)e->dvals.p, L, T,
+                                          S, (const uint64_t*)vk, (const V*)e->dvals.p, vv, L, T,
                                           (uint64_t*)e->vkeys[nxt].p, (V*)e->vvals[nxt].p, e->max_entries, dm,
                                           &merge_tiles, S_dev)) return rc;
     } else {

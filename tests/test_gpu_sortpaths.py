@@ -11,11 +11,13 @@ from test_gpu_parity import device_model, oracle_model, random_vector
 pytestmark = pytest.mark.gpu
 
 
-def _engine(rimu, kind, M, N, u, style, seed, mode):
+def _engine(rimu, kind, M, N, u, style, seed, mode, rule=0, thr=0.0):
     h = device_model(rimu, kind, M, N, u, 1.0)
     st = {"integer": rimu.IsStochasticInteger(), "dynamic": rimu.IsDynamicSemistochastic()}[style]
     e = rimu._native.Engine(h.descriptor(), st.descriptor(), seed, 8_000_000, 1 << 20)
     e.set_radix_mode(mode)
+    if rule:
+        e.set_initiator(rule, thr)
     e.set_timing(True)
     return e
 
@@ -102,3 +104,35 @@ def test_lookback_give_up_repeats_step(gpu, mode):
     f.close()
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1].view(np.uint64), want[1].view(np.uint64))
     assert st.len == sf.len and st.norm == pytest.approx(sf.norm, rel=1e-13)
+
+
+@pytest.mark.parametrize("rule", [1, 2, 3])
+@pytest.mark.parametrize("kind,M,N,style,n,scale,thr", [
+    ("real", 20, 20, "dynamic", 150_000, 60.0, 20.0),
+    ("real", 20, 20, "integer", 80_000, 200, 100.0),
+    ("mom", 16, 16, "dynamic", 50_000, 40.0, 10.0),
+])
+def test_initiator_bucket_merge_equals_merge_path(gpu, kind, M, N, style, n, scale, thr, rule):
+    """Initiator rules (Initiator, SimpleInitiator, CoherentInitiator) on the bucket path
+    (three components folded per address in LDS) and on the merge path: five steps from the
+    same vector and seed give bit-identical vectors and statistics."""
+    rng = np.random.default_rng(n + M + rule)
+    keys, vals = random_vector(M, N, n, rng, "int" if style == "integer" else "float", scale)
+    print(f"initiator bucket vs merge path {kind} {M}x{N} {style} rule {rule} n={len(keys)}", flush=True)
+    out = []
+    for mode in (2, 1):
+        e = _engine(gpu, kind, M, N, 4.0, style, 73, mode, rule, thr)
+        e.upload(keys, vals)
+        stats = [e.step(s, -2.0, 1e-3) for s in range(1, 6)]
+        names = _kernels(e)
+        out.append((e.download(), [(st.len, st.norm, st.spawns, st.len_before) for st in stats], names))
+        e.close()
+    (k2, v2), s2, names2 = out[0]
+    (k1, v1), s1, names1 = out[1]
+    assert "bucket_merge" in names2 and "bucket_fallback" not in names2
+    assert "bucket_merge" not in names1
+    assert np.array_equal(k2, k1)
+    assert np.array_equal(v2.view(np.uint64), v1.view(np.uint64))
+    for a, b in zip(s2, s1):
+        assert a[0] == b[0] and a[3] == b[3]
+        assert a[1] == pytest.approx(b[1], rel=1e-13) and a[2] == pytest.approx(b[2], rel=1e-13)
