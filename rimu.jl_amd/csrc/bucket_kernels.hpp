@@ -14,7 +14,8 @@
 //      diagonal survivor first and its spawns after it in list order, the order of deposit!
 //      over the list (Interfaces/dictvectors.jl:49-51, dvec.jl:129-136),
 //   2. sorts the packed words in LDS: LDS atomics drop every word into one of 2048
-//      sub-buckets (the top 11 key bits; arrival order inside a sub-bucket is arbitrary), and
+//      sub-buckets (the 11 key bits below the bucket bits; arrival order inside a sub-bucket
+//      is arbitrary), and
 //      each word's place inside its sub-bucket is the number of smaller words there -- exact
 //      whatever the arrival order was, since the item indices make all words distinct,
 //   3. folds each address left to right in one pass over a window of the sorted words,
@@ -23,7 +24,8 @@
 //   4. writes its survivors to its own slot range with their count; bucket_compact_kernel
 //      places them after a scan of the counts.
 // The result is that of the full sort and merge: same addresses, same fold order, same
-// draws.  A bucket larger than its LDS arrays raises FLAG_BUCKET_OVERFLOW and writes
+// draws; with an initiator rule each address folds three components instead of one, as on
+// the merge path.  A bucket larger than its LDS arrays raises FLAG_BUCKET_OVERFLOW and writes
 // nothing; the host then sorts on every bit and merges along the merge path.
 #pragma once
 
