@@ -69,6 +69,15 @@ __global__ __launch_bounds__(NT) void bucket_bounds_kernel(const uint64_t* __res
     (rec ? rstart : vstart)[b] = lo;
 }
 
+// LDS stores of this wavefront's lanes are seen by all its lanes after this point (the LDS
+// performs one wavefront's operations in order; the fences keep the compiler from moving
+// loads or stores across it).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Scans of the bucket workgroup, written with the lane and (uniform) wavefront index the
 // caller derives from its per-bucket thread index, so that no lane masks are held in
 // registers across buckets.
@@ -432,7 +441,7 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
                             wv[q] = oval[j];
                         }
                     }
-                    __syncthreads();
+                    wave_lds_sync();  // the staging area is this wavefront's own
                     // the seed re-read here: its round keys are not held in registers across buckets
                     uint64_t seed;
                     asm volatile("s_mov_b64 %0, %1" : "=s"(seed) : "s"(a.seed));
@@ -442,7 +451,7 @@ __global__ __launch_bounds__(BK_NT, 2 * BK_WG_PER_CU) void bucket_merge_kernel(
                         const U4 r = draw(seed, unmix_key(a.mix, key), 0, a.step, TAG_COMPRESS);
                         wv[q] = (fabs(x) / th > u53(r.x, r.y)) ? th * sgn(x) : 0.0;
                     }
-                    __syncthreads();
+                    wave_lds_sync();
 #pragma unroll
                     for (int j = 0; j < BK_IPT; ++j) {
                         if ((need >> j) & 1u) {
