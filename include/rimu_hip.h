@@ -204,17 +204,18 @@ typedef struct rimu_operator {
 int rimu_overlaps(rimu_engine* left, rimu_engine* right, const rimu_operator* ops, int32_t nops,
                   double* out);
 
-/* (No counterpart in the reference; for tests and tuning.)  Addresses of more than 40 bits
- * are stored as a bijective mix of their bits, and a step's spawn records are radix-sorted
- * on the top `bits` bits of the mixed keys only (default 40); runs that agree there are then
- * ordered one by one, or, when such runs are long or many, by the remaining passes.  Any
- * setting gives the same step results; fewer bits exercise the run ordering. */
+/* (No counterpart in the reference; for tests and tuning.)  Addresses of more than 20 bits
+ * are stored as a bijective mix of their bits.  Where the bucket merge does not apply (see
+ * rimu_engine_set_radix_mode), a step's spawn records with more than `bits` key bits are
+ * radix-sorted on the top `bits` bits of the mixed keys only (default 40); runs that agree
+ * there are then ordered one by one, or, when such runs are long or many, by the remaining
+ * passes.  Any setting gives the same step results; fewer bits exercise the run ordering. */
 int rimu_engine_set_sort_bits(rimu_engine* e, int32_t bits);
 
 /* (No counterpart in the reference; for tests and tuning.)  Sort and merge of the spawn
- * records.  2 (default): where eligible (one-word keys of more than 20 bits, no initiator
- * rule, enough records), sort on the top bits of the stored keys only and merge bucket by
- * bucket, each bucket's records sorted and folded in LDS; other steps as 1.  1: one histogram
+ * records.  2 (default): where eligible (one-word keys of more than 20 bits, at least 4096
+ * records), sort on the top bits of the stored keys only and merge bucket by bucket, each
+ * bucket's records and vector entries sorted and folded in LDS; other steps as 1.  1: one histogram
  * pass for every digit and scatters that find their offsets by a look-back over the tiles
  * before them, then the merge path.  0: as 1 with counts per pass and a scan between passes.
  * All give the same step results. */
