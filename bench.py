@@ -865,19 +865,24 @@ def main():
     sp = rimu.strategies.DefaultShiftParameters(H.diagonal_element(start), 10.0, dtau)
 
     step = 0
-    if args.state_file and os.path.exists(args.state_file):
+    resumed = bool(args.state_file) and os.path.exists(args.state_file)
+    st = None
+    if resumed:
         z = np.load(args.state_file)
         eng.upload(z["keys"], z["vals"])
         sp.shift, sp.pnorm, step = float(z["shift"]), float(z["pnorm"]), int(z["step"])
         log(f"resumed equilibrated state at step {step}: len {len(z['keys'])} shift {sp.shift:.4f}")
-        st = None
-    else:
+    # DoubleLogUpdate runs on the device (the step's shift argument is ignored); sp mirrors
+    # the device's shift parameters after every step
+    eng.set_shift_update(rimu._native.SHIFT_DOUBLE_LOG, sp.shift, sp.pnorm, target, strat.zeta,
+                         strat.xi_value)
+    if not resumed:
         t_eq = time.perf_counter()
         last_log = t_eq
         while step < args.equilibration:
             step += 1
             st = eng.step(step, sp.shift, dtau)
-            strat.update(sp, st.norm)
+            sp.shift, sp.pnorm = eng.shift_parameters()
             if time.perf_counter() - last_log > 10:
                 log(f"equilibration step {step}: len {st.len:.3g} norm {st.norm:.3g} shift {sp.shift:.4f}")
                 last_log = time.perf_counter()
@@ -891,7 +896,7 @@ def main():
     for _ in range(args.warmup):
         step += 1
         st = eng.step(step, sp.shift, dtau)
-        strat.update(sp, st.norm)
+        sp.shift, sp.pnorm = eng.shift_parameters()
 
     # the timed steps run without per-kernel events (`value`, `ms_per_step`); the kernel
     # breakdown and the roofline come from the steps after them, with events on the stream
@@ -901,7 +906,7 @@ def main():
     for _ in range(args.steps):
         step += 1
         st = eng.step(step, sp.shift, dtau)
-        strat.update(sp, st.norm)
+        sp.shift, sp.pnorm = eng.shift_parameters()
         attempts += st.spawn_attempts
     elapsed = time.perf_counter() - t0
     K = args.steps
@@ -918,7 +923,7 @@ def main():
         for _ in range(KT):
             step += 1
             st = eng.step(step, sp.shift, dtau)
-            strat.update(sp, st.norm)
+            sp.shift, sp.pnorm = eng.shift_parameters()
             b, records = eng.last_traffic()
             alg_bytes += b
             S_tot += records - L_prev
@@ -1030,6 +1035,7 @@ def main():
         "data": "synthetic (seeded FCIQMC run from the near-uniform start address)",
         "config": {"workload": "HubbardReal1D M=N=20 u=6 t=1, IsDynamicSemistochastic FP64, "
                                "target_walkers=1e8, dtau=1e-3, seed 17",
+                   "shift_update": "DoubleLogUpdate on the device",
                    "equilibration_steps": args.equilibration, "parallelism": "single GPU"},
         "step_alg_roofline_frac": (alg_bytes / KT) / (ms_per_step / 1e3) / HBM_PEAK if KT else None,
         "walkers": final_norm,

@@ -161,9 +161,30 @@ int rimu_set_projector(rimu_engine* e, const uint64_t* pk, const double* pc, int
 /* One step v <- compress(T v) with T = 1 + dtau (shift - H) (fciqmc.jl:78-112, :139),
  * apply_operator! with boost (Interfaces/dictvectors.jl:112), then len, norm and the
  * projections.  step numbers the Philox streams.  On overflow (stats->overflow = 1,
- * return RIMU_ERR_CAPACITY) the vector is left unchanged. */
+ * return RIMU_ERR_CAPACITY) the vector is left unchanged.  With a device shift update
+ * (rimu_engine_set_shift_update) the step uses the engine's shift and ignores the argument. */
 int rimu_step(rimu_engine* e, int64_t step, double shift, double dtau, double boost,
               rimu_step_stats* stats);
+
+/* Shift updates: the caller's (the shift argument of rimu_step), or DoubleLogUpdate
+ * (shiftstrategy.jl:161-181) on the device. */
+enum { RIMU_SHIFT_HOST = 0, RIMU_SHIFT_DOUBLE_LOG = 1 };
+
+/* Keep the shift in device memory and update it there after every step, as
+ * update_shift_parameters! of DoubleLogUpdate (shiftstrategy.jl:174-181) does with the
+ * step's norm N: shift <- shift - (xi/dtau ln(N/target_walkers) + zeta/dtau ln(N/pnorm)),
+ * pnorm <- N, with the dtau of the step (ConstantTimeStep).  shift and pnorm are the
+ * initial shift parameters (DefaultShiftParameters, shiftstrategy.jl:32-38; pnorm is the
+ * walker number of the start vector); xi = zeta^2/4 is the reference's default.  The next
+ * step reads the shift from device memory, so no host value enters the step.  A step that
+ * fails or is repeated leaves the shift parameters unchanged, like the vector.
+ * RIMU_SHIFT_HOST returns the shift to the caller. */
+int rimu_engine_set_shift_update(rimu_engine* e, int32_t mode, double shift, double pnorm,
+                                 double target_walkers, double zeta, double xi);
+/* Shift parameters after the last step (read back with the step's statistics, no device
+ * access here): the shift the next step uses, and pnorm.  RIMU_ERR_ARG without a device
+ * shift update. */
+int rimu_engine_get_shift(rimu_engine* e, double* shift, double* pnorm);
 
 /* Initiator rules: NonInitiator, Initiator, SimpleInitiator, CoherentInitiator
  * (DictVectors/initiators.jl:132-236). */

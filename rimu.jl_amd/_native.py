@@ -19,13 +19,15 @@ MODEL_REAL1D, MODEL_MOM1D, MODEL_REALSPACE = 1, 2, 3
 STYLE_INTEGER, STYLE_DETERMINISTIC, STYLE_DYNAMIC = 1, 2, 3
 INITIATOR_NONE, INITIATOR_DEFAULT, INITIATOR_SIMPLE, INITIATOR_COHERENT = 0, 1, 2, 3
 OP_HAMILTONIAN, OP_G2_REAL, OP_DENSITY_DIAGONAL = 1, 2, 3
+SHIFT_HOST, SHIFT_DOUBLE_LOG = 0, 1
 MAX_OPERATORS = 8
 
 # Every symbol declared in include/rimu_hip.h (checked by the CPU test-suite).
 EXPORTS = (
     "rimu_last_error", "rimu_device_count", "rimu_engine_create", "rimu_engine_destroy",
     "rimu_engine_reserve", "rimu_vector_upload", "rimu_vector_length", "rimu_vector_download",
-    "rimu_set_projector", "rimu_step", "rimu_engine_set_initiator", "rimu_overlaps",
+    "rimu_set_projector", "rimu_step", "rimu_engine_set_shift_update", "rimu_engine_get_shift",
+    "rimu_engine_set_initiator", "rimu_overlaps",
     "rimu_engine_set_sort_bits", "rimu_engine_set_radix_mode", "rimu_engine_set_lookback",
     "rimu_engine_set_timing",
     "rimu_engine_kernel_times",
@@ -105,6 +107,8 @@ def lib():
         L.rimu_vector_download.argtypes = [vp, vp, vp, i64, P(i64)]
         L.rimu_set_projector.argtypes = [vp, vp, vp, i64, vp, vp, i64]
         L.rimu_step.argtypes = [vp, i64, dbl, dbl, dbl, P(StepStats)]
+        L.rimu_engine_set_shift_update.argtypes = [vp, i32, dbl, dbl, dbl, dbl, dbl]
+        L.rimu_engine_get_shift.argtypes = [vp, P(dbl), P(dbl)]
         L.rimu_engine_set_initiator.argtypes = [vp, i32, dbl]
         L.rimu_engine_set_sort_bits.argtypes = [vp, i32]
         L.rimu_engine_set_radix_mode.argtypes = [vp, i32]
@@ -302,6 +306,21 @@ class Engine:
                 continue
             check(rc)
             return st
+
+    def set_shift_update(self, mode: int, shift: float = 0.0, pnorm: float = 1.0,
+                         target_walkers: float = 1000.0, zeta: float = 0.08, xi: float | None = None):
+        """Keep the shift on the device and update it there after every step
+        (``SHIFT_DOUBLE_LOG``: DoubleLogUpdate from the initial shift and pnorm), or return it to
+        the caller (``SHIFT_HOST``).  ``step`` then ignores its shift argument."""
+        xi = zeta ** 2 / 4 if xi is None else xi
+        check(lib().rimu_engine_set_shift_update(self._h, int(mode), float(shift), float(pnorm),
+                                                 float(target_walkers), float(zeta), float(xi)))
+
+    def shift_parameters(self):
+        """(shift, pnorm) of the device shift update after the last step."""
+        s, n = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().rimu_engine_get_shift(self._h, ctypes.byref(s), ctypes.byref(n)))
+        return float(s.value), float(n.value)
 
     def set_sort_bits(self, bits: int):
         """Top bits of mixed keys that the radix passes cover (tests and tuning)."""

@@ -131,6 +131,13 @@ struct rimu_engine {
     DevBuf<uint32_t> lb_hist;       // look-back radix: digit bases [pass][digit], tile counters
     DevBuf<uint32_t> lb_part;       // look-back radix: digit counts [row][pass][digit]
     int64_t np = 0, nh = 0;
+    // device shift update (rimu_engine_set_shift_update): shift_state holds (shift, pnorm)
+    // twice; a step reads half shift_cur and writes the other, and a kept step flips shift_cur
+    int shift_mode = 0;
+    double dlu_target = 0, dlu_zeta = 0, dlu_xi = 0;
+    DevBuf<double> shift_state;
+    int shift_cur = 0;
+    double h_shift = 0, h_pnorm = 0;  // the device state after the last kept step
 
     // host staging
     struct Misc {
@@ -144,7 +151,7 @@ struct rimu_engine {
         uint32_t fix_count;  // runs listed by fixup_find_kernel
     };
     Misc* h_misc = nullptr;    // pinned
-    double* h_stats = nullptr; // pinned, 12 doubles
+    double* h_stats = nullptr; // pinned: 12 statistics, then shift and pnorm
 
     KernelTimer timer;
     double last_alg_bytes = 0, last_records = 0;
@@ -266,6 +273,7 @@ StepArgs make_args(rimu_engine* e, int64_t step, double shift, double dtau, doub
     a.seed = e->seed;
     a.step = step;
     a.shift = shift;
+    a.shift_dev = e->shift_mode ? e->shift_state.p + 2 * e->shift_cur : nullptr;
     a.dtau = dtau;
     a.boost = boost;
     a.style = e->style.kind;
@@ -814,9 +822,19 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
         ps.p[11] = e->terms.p + e->np;
         ps.n[11] = e->nh;
         partial_sums_kernel<<<dim3(PARTIAL_BLOCKS, NSTAT), NT, 0, e->stream>>>(ps, e->chunk_sums.p);
-        finalize_kernel<<<1, 64, 0, e->stream>>>(e->chunk_sums.p, PARTIAL_BLOCKS, e->stats_dev.p);
+        ShiftUpdate su{};
+        if (e->shift_mode) {
+            su.in = a.shift_dev;
+            su.out = e->shift_state.p + 2 * (1 - e->shift_cur);
+            su.dtau = dtau;
+            su.target = e->dlu_target;
+            su.zeta = e->dlu_zeta;
+            su.xi = e->dlu_xi;
+        }
+        finalize_kernel<<<1, 64, 0, e->stream>>>(e->chunk_sums.p, PARTIAL_BLOCKS, e->stats_dev.p, su);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, 12 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->h_stats, e->stats_dev.p, (NSTAT + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                               e->stream));
         HIP_TRY(hipMemcpyAsync(e->h_misc, dm, sizeof(rimu_engine::Misc), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
         e->timer.collect();
@@ -888,6 +906,11 @@ int step_impl(rimu_engine* e, int64_t step, double shift, double dtau, double bo
     const int64_t L_new = e->h_misc->L_new;
     e->cur = nxt;
     e->len = L_new;
+    if (e->shift_mode) {
+        e->shift_cur = 1 - e->shift_cur;
+        e->h_shift = s[NSTAT];
+        e->h_pnorm = s[NSTAT + 1];
+    }
     // the counts that size the next step's grids
     e->W_prev = e->h_misc->W;
     e->S_prev = dense ? e->h_misc->W : e->h_misc->S;
@@ -985,7 +1008,7 @@ int rimu_engine_destroy(rimu_engine* e) {
     e->prevk.release(); e->slot_k.release(); e->slot_v.release();
     e->dvals.release(); e->woff.release(); e->scan_tmp.release(); e->rcounts.release(); e->rchunks.release();
     e->rtmp.release(); e->plan_status.release(); e->tile_or.release(); e->tile_and.release(); e->plan_p.release(); e->spawn_p.release();
-    e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->misc.release();
+    e->merge_p.release(); e->terms.release(); e->chunk_sums.release(); e->stats_dev.release(); e->shift_state.release(); e->misc.release();
     e->pkeys.release(); e->pcoef.release();
     e->ov_ops.release(); e->ov_part.release(); e->ov_out.release(); e->ov_ranges.release(); e->ov_index.release();
     if (e->h_misc) (void)hipHostFree(e->h_misc);
@@ -1262,6 +1285,40 @@ int rimu_engine_set_initiator(rimu_engine* e, int32_t rule, double threshold) {
         return fail(RIMU_ERR_ARG, "initiator rules need N + M - 1 <= 63 or 65..127 (the top key bit holds the class)");
     e->init_rule = rule;
     e->init_threshold = rule == RIMU_INITIATOR_NONE ? 0.0 : threshold;
+    return RIMU_OK;
+}
+
+int rimu_engine_set_shift_update(rimu_engine* e, int32_t mode, double shift, double pnorm,
+                                 double target_walkers, double zeta, double xi) {
+    if (!e) return fail(RIMU_ERR_ARG, "engine is null");
+    if (mode != RIMU_SHIFT_HOST && mode != RIMU_SHIFT_DOUBLE_LOG) return fail(RIMU_ERR_ARG, "unknown shift update");
+    if (mode == RIMU_SHIFT_HOST) {
+        e->shift_mode = mode;
+        return RIMU_OK;
+    }
+    if (!std::isfinite(shift) || !(pnorm > 0) || !std::isfinite(pnorm))
+        return fail(RIMU_ERR_ARG, "shift must be finite and pnorm positive");
+    if (!(target_walkers > 0) || !std::isfinite(target_walkers) || !std::isfinite(zeta) || !std::isfinite(xi))
+        return fail(RIMU_ERR_ARG, "target_walkers must be positive, zeta and xi finite");
+    HIP_TRY(e->shift_state.alloc(4));
+    const double init[2] = {shift, pnorm};
+    HIP_TRY(hipMemcpyAsync(e->shift_state.p, init, sizeof(init), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->shift_cur = 0;
+    e->h_shift = shift;
+    e->h_pnorm = pnorm;
+    e->dlu_target = target_walkers;
+    e->dlu_zeta = zeta;
+    e->dlu_xi = xi;
+    e->shift_mode = mode;
+    return RIMU_OK;
+}
+
+int rimu_engine_get_shift(rimu_engine* e, double* shift, double* pnorm) {
+    if (!e || !shift || !pnorm) return fail(RIMU_ERR_ARG, "null argument");
+    if (e->shift_mode == RIMU_SHIFT_HOST) return fail(RIMU_ERR_ARG, "the engine has no device shift update");
+    *shift = e->h_shift;
+    *pnorm = e->h_pnorm;
     return RIMU_OK;
 }
 

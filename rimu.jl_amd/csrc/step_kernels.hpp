@@ -61,6 +61,7 @@ struct StepArgs {
     uint64_t seed;
     int64_t step;
     double shift, dtau, boost;
+    const double* shift_dev;  // the shift in device memory (device shift update), or null: shift
     int style;        // 1 integer, 2 deterministic, 3 dynamic semistochastic
     int compression;  // 0 none, 1 threshold
     double comp_threshold, proj_threshold, rel_threshold, abs_threshold;
@@ -165,6 +166,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
     __shared__ int64_t s_prefix;
     const uint32_t tile = next_tile_id(tile_counter);
     const int64_t t0 = (int64_t)tile * PLAN_TILE;
+    const double shift = a.shift_dev ? a.shift_dev[0] : a.shift;
     double st[6] = {0, 0, 0, 0, 0, 0};
     // striped: entry t0 + j*NT + x (coalesced loads and stores)
     K kk[PLAN_IPT];
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
             const V v = vv[j];
             // diagonal_step!, spawning.jl:73-77 with diagonal_element(T) fciqmc.jl:93-96
             const double hd = model_diagonal_element<RS>(a.h, k);
-            const double d = 1 - a.dtau * (hd - a.shift);
+            const double d = 1 - a.dtau * (hd - shift);
             const double x = d * (double)v;
             double U = 0;
             if (is_int_v<V>::value || a.proj_threshold > 0) {
@@ -1463,13 +1465,41 @@ __global__ __launch_bounds__(NT) void partial_sums_kernel(PartialSet ps, double*
     if (threadIdx.x == 0) chunk_sums[(int64_t)j * gridDim.x + blockIdx.x] = s;
 }
 
+// DoubleLogUpdate (shiftstrategy.jl:174-181) for an engine that keeps its shift in device
+// memory: from the step's norm N, S' = S - (xi/dtau ln(N/N_target) + zeta/dtau ln(N/N_prev))
+// and N_prev' = N.  in holds (S, N_prev) of this step, out receives (S', N_prev'): two halves
+// of one buffer, so that a step repeated from the same vector updates from the same state.
+// ln of a quotient <= 0 is -inf, as on the host.
+struct ShiftUpdate {
+    const double* in;  // null: the host keeps the shift
+    double* out;
+    double dtau, target, zeta, xi;
+};
+
+__device__ inline double log_or_minf(double x) { return x > 0 ? log(x) : -INFINITY; }
+
+__device__ inline double double_log_update(const ShiftUpdate& su, double norm) {
+#pragma clang fp contract(off)  // the host's operation order and roundings
+    return su.in[0] - (su.xi / su.dtau * log_or_minf(norm / su.target) +
+                       su.zeta / su.dtau * log_or_minf(norm / su.in[1]));
+}
+
+// out[0..NSTAT-1]: the statistics; with a device shift update also out[NSTAT] = S' and
+// out[NSTAT + 1] = N_prev' (the norm is statistic 9).
 __global__ __launch_bounds__(64) void finalize_kernel(const double* __restrict__ chunk_sums, int nchunks,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, ShiftUpdate su) {
     const int j = threadIdx.x;
     if (j >= NSTAT) return;
     double acc = 0;
     for (int b = 0; b < nchunks; ++b) acc += chunk_sums[j * nchunks + b];
     out[j] = acc;
+    if (j == 9 && su.in) {
+        const double s = double_log_update(su, acc);
+        su.out[0] = s;
+        su.out[1] = acc;
+        out[NSTAT] = s;
+        out[NSTAT + 1] = acc;
+    }
 }
 
 }  // namespace rimu

@@ -2,10 +2,11 @@
 device engine.
 
 The host keeps Rimu's step loop (``advance!``, ``src/fciqmc.jl:126-181``; ``step!`` and
-``solve!``, ``src/pmc_simulation.jl:265-310``, ``:350-452``), the shift strategy and the
-report; one ``rimu_step`` call replaces ``apply_operator!`` + ``walkernumber_and_length``
-+ the ProjectedEnergy post-step.  Keyword defaults follow
-``src/projector_monte_carlo_problem.jl:150-290`` and ``src/lomc.jl:95-195``.
+``solve!``, ``src/pmc_simulation.jl:265-310``, ``:350-452``) and the report; one
+``rimu_step`` call replaces ``apply_operator!`` + ``walkernumber_and_length`` + the
+ProjectedEnergy post-step.  DoubleLogUpdate (the default shift strategy) runs on the
+device after every step; the other shift strategies update on the host.  Keyword defaults
+follow ``src/projector_monte_carlo_problem.jl:150-290`` and ``src/lomc.jl:95-195``.
 """
 from __future__ import annotations
 
@@ -172,6 +173,14 @@ class PMCSimulation:
         for r in self.replicas:
             r.shift_parameters = DefaultShiftParameters(float(shift), v0.walkernumber(),
                                                         problem.time_step)
+        # DoubleLogUpdate runs on the device (update_shift_parameters!, shiftstrategy.jl:174-181,
+        # after every step from the step's norm); the other strategies update on the host
+        s = problem.shift_strategy
+        self.device_shift = (type(s) is DoubleLogUpdate and s.target_walkers > 0
+                             and v0.walkernumber() > 0)
+        if self.device_shift:
+            for r in self.replicas:
+                self._set_device_shift(r)
         # projections (only ProjectedEnergy is evaluated on the device)
         self.projected = [p for p in problem.post_step_strategy if isinstance(p, ProjectedEnergy)]
         if len(self.projected) > 1:
@@ -231,12 +240,19 @@ class PMCSimulation:
         return self.df
 
     # stepping -------------------------------------------------------------------------
+    def _set_device_shift(self, r: _Replica):
+        s, sp = self.problem.shift_strategy, r.shift_parameters
+        r.engine.set_shift_update(_native.SHIFT_DOUBLE_LOG, sp.shift, sp.pnorm, s.target_walkers,
+                                  s.zeta, s.xi_value)
+
     def advance(self, r: _Replica | None = None) -> bool:
         """``advance!`` (fciqmc.jl:126-181) of one replica."""
         r = self.replicas[0] if r is None else r
         p = self.problem
         sp = r.shift_parameters
         st = None
+        if self.device_shift and (sp.shift, sp.pnorm) != r.engine.shift_parameters():
+            self._set_device_shift(r)   # the shift parameters were set from outside
         try:
             st = r.engine.step(self.step, sp.shift, sp.time_step)
         except _native.CapacityError:
@@ -244,7 +260,11 @@ class PMCSimulation:
             return False
         tnorm, length = st.norm, int(st.len)
         r.v._norm = tnorm
-        shift_stats, proceed = p.shift_strategy.update(sp, tnorm)
+        if self.device_shift:
+            sp.shift, sp.pnorm = r.engine.shift_parameters()
+            shift_stats, proceed = {"shift": sp.shift, "norm": tnorm}, True
+        else:
+            shift_stats, proceed = p.shift_strategy.update(sp, tnorm)
         if self.step % p.reporting_interval == 0:
             i = r.id
             self._report("len" + i, length)
