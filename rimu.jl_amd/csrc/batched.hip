@@ -1,19 +1,25 @@
-// batched.hip -- many independent IsStochasticInteger FCIQMC runs of HubbardReal1D, one
-// workgroup per run and one kernel launch per step for all runs (C ABI rimu_batch_*).
+// batched.hip -- many independent IsStochasticInteger FCIQMC runs of HubbardReal1D advanced
+// together (C ABI rimu_batch_*).
 //
-// A workgroup performs a whole step of its run with the same algorithm as the plain engine,
-// scaled down to one workgroup: diagonal step and spawn attempts (attempt-parallel, ballot
-// compaction into the run's spawn buffer), a stable LSD radix sort of the spawns
-// (histogram in LDS, ranks from wavefront ballots, RB-bit digits), a merge of the sorted
-// spawns with the diagonal survivors (sequential tiles, so output offsets need no
-// look-back), the norm, the ProjectedEnergy values and the DoubleLogUpdate shift update
-// (shiftstrategy.jl:174-181).  The step number, shift and lengths live in device memory, so
-// a sequence of launches can be captured once in a HIP graph and replayed.
+// A step of every run has four phases, the plain engine's algorithm at the size of a run:
+// spawn slices (a run's entries split over K workgroups, one per 2048 target walkers:
+// diagonal step and spawn attempts, ballot compaction into the slice's spawn buffer), the
+// run's sort (one workgroup: gather of the slices in order, stable LSD radix sort with LDS
+// histograms and ballot ranks, merge tile boundaries), merge slices (the run's merge tiles
+// over K workgroups) and the run's final phase (one workgroup: the new vector, norm,
+// ProjectedEnergy values, history row and the DoubleLogUpdate shift update,
+// shiftstrategy.jl:174-181).  The phases run as four launches covering all runs, or as one
+// persistent launch in which workgroups take the phases' items from a work queue
+// (batch_step_kernel).  The step number, shift and lengths live in device memory, so a
+// sequence of steps is captured once in a HIP graph and replayed.
 //
 // Random draws are addressed by (seed, address, attempt, step) and integer walker sums are
-// exact, so the result of a step does not depend on the workgroup size or radix width.
+// exact, so the result of a step does not depend on the workgroup size, radix width or
+// launch kind (except the vector an overflowing step leaves: a prefix of whole merge tiles,
+// whose size follows the workgroup size).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -28,7 +34,7 @@ using namespace rimu;
 namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
-constexpr int EPT = 4;           // entries per thread per spawn chunk
+constexpr int SPAWN_CH = 1024;   // entries per spawn chunk (SPAWN_CH / NT per thread)
 constexpr int AIT = 4;           // attempts per thread per spawn group
 constexpr double WALKERS_PER_SLICE = 2048;  // target walkers per spawn workgroup of a run
 constexpr int64_t MAX_SLICES = 64;
@@ -78,28 +84,43 @@ struct Pools {
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
 
+// The thread index, read afresh: values derived from it then stay inside the phase that
+// computes them instead of being hoisted out of the persistent step's loop and held in
+// registers across all phases.
+__device__ inline int thread_index() {
+    int t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+    return t;
+}
+
 // Diagonal step and spawning of one slice of a run: entries [L*s/K, L*(s+1)/K).  Records go,
 // in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
 // region; batch_step_kernel gathers the slices in order.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ runs, Pools P) {
-    constexpr int NW = NT / WAVE;
-    constexpr int CH = NT * EPT;          // entries per spawn chunk
-    __shared__ int s_cnt[AIT * NW];
-    __shared__ int s_tot;
-    __shared__ int64_t s_S;
-    __shared__ struct {
-        uint64_t k[CH];        // the chunk's keys
-        int64_t v[CH];         // and values
-        int64_t aoff[CH + 1];  // attempt offsets within the chunk (counts first)
-    } sp;
+struct SpawnLds {
+    int cnt[AIT * (NT / WAVE)];
+    int tot;
+    int64_t S;
+    uint64_t k[SPAWN_CH];        // the chunk's keys
+    int64_t v[SPAWN_CH];         // and values
+    int64_t aoff[SPAWN_CH + 1];  // attempt offsets within the chunk (counts first)
+};
 
-    const int run = P.wg_run[blockIdx.x];
+template <int NT>
+__device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, SpawnLds<NT>& sp, int tid) {
+    constexpr int NW = NT / WAVE;
+    constexpr int EPT = SPAWN_CH / NT;    // entries per thread per chunk
+    constexpr int CH = SPAWN_CH;
+    int* s_cnt = sp.cnt;
+    int& s_tot = sp.tot;
+    int64_t& s_S = sp.S;
+
+    const int run = P.wg_run[gslice];
     const RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
-    const int64_t slice = (int64_t)blockIdx.x - st.slice0;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = threadIdx.x / WAVE;
+    const int64_t slice = gslice - st.slice0;
+    const int lane = tid & (WAVE - 1);
+    const int wid = tid / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 
     const int M = st.M, B = st.B;
@@ -116,7 +137,7 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
     uint64_t* sk0 = slice == 0 ? P.sk[0] + st.offS : P.xk + st.offX + (slice - 1) * capS;
     int64_t* sv0 = slice == 0 ? P.sv[0] + st.offS : P.xv + st.offX + (slice - 1) * capS;
 
-    if (threadIdx.x == 0) s_S = 0;
+    if (tid == 0) s_S = 0;
     __syncthreads();
 
     // ---- diagonal step and spawning, CH entries at a time -------------------------------
@@ -129,13 +150,13 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
         int64_t vq[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {  // all loads in flight before any is used
-            const int64_t e = min(c0 + j * NT + (int64_t)threadIdx.x, L - 1);
+            const int64_t e = min(c0 + j * NT + (int64_t)tid, L - 1);
             kq[j] = vk[e];
             vq[j] = vv[e];
         }
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            const int loc = j * NT + (int)threadIdx.x;
+            const int loc = j * NT + tid;
             if (loc < n_ch) {
                 const uint64_t k = kq[j];
                 const int64_t v = vq[j];
@@ -158,7 +179,7 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
         int64_t sum = 0;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            const int loc = (int)threadIdx.x * EPT + j;
+            const int loc = tid * EPT + j;
             c[j] = loc < n_ch ? sp.aoff[loc] : 0;
             sum += c[j];
         }
@@ -166,10 +187,10 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
         int64_t run_ = block_exclusive_scan<int64_t, NT>(sum, &T);  // ends with a barrier
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            sp.aoff[(int)threadIdx.x * EPT + j] = run_;
+            sp.aoff[tid * EPT + j] = run_;
             run_ += c[j];
         }
-        if (threadIdx.x == 0) sp.aoff[CH] = T;
+        if (tid == 0) sp.aoff[CH] = T;
         st_att += (double)sum;
         __syncthreads();
         // attempts in groups of NT * AIT: one ballot per attempt, then a scan of the
@@ -181,7 +202,7 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
             int rpos[AIT];
 #pragma unroll
             for (int it = 0; it < AIT; ++it) {
-                const int64_t att = a0 + it * NT + threadIdx.x;
+                const int64_t att = a0 + it * NT + tid;
                 ok[it] = false;
                 nbk[it] = 0;
                 nbv[it] = 0;
@@ -212,7 +233,7 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
                 if (lane == 0) s_cnt[it * NW + wid] = __popcll(bal);
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
+            if (tid == 0) {
                 int64_t run = s_S;
                 for (int q = 0; q < AIT * NW; ++q) { const int c = s_cnt[q]; s_cnt[q] = (int)(run - s_S); run += c; }
                 s_tot = (int)(run - s_S);
@@ -227,37 +248,49 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) s_S += s_tot;
+            if (tid == 0) s_S += s_tot;
             __syncthreads();
         }
     }
     __syncthreads();
     // record count (it may exceed capS: the gather then reports the overflow) and attempts
     const double att_sum = block_sum<NT>(st_att);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         P.xcnt[st.slice0 + slice] = s_S;
         P.xatt[st.slice0 + slice] = att_sum;
     }
     (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie; (void)u;
+}
 
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ runs, Pools P) {
+    __shared__ SpawnLds<NT> sp;
+    spawn_phase<NT>(runs, P, blockIdx.x, sp, threadIdx.x);
 }
 
 // Gather of the slices' spawn records, their radix sort, and the boundaries of every merge
 // tile (one workgroup per run).  Hands S, the sorted buffer and the tile count to the next
 // kernels through the run state.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ runs, Pools P, int radix_bits) {
+struct SortLds {
+    uint32_t hist[256];
+    uint32_t wcnt[NT / WAVE][256];
+    int flag;
+};
+
+template <int NT>
+__device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run, int radix_bits, SortLds<NT>& so,
+                           int tid) {
     constexpr int NW = NT / WAVE;
     constexpr int TILE = NT * MIPT;
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_wcnt[NW][256];
-    __shared__ int s_flag;
+    uint32_t* s_hist = so.hist;
+    uint32_t(*s_wcnt)[256] = so.wcnt;
+    int& s_flag = so.flag;
 
-    const int run = blockIdx.x;
     RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = threadIdx.x / WAVE;
+    const int lane = tid & (WAVE - 1);
+    const int wid = tid / WAVE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int B = st.B;
     const int64_t L = st.len, capS = st.capS;
@@ -268,7 +301,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
     uint64_t* sk1 = P.sk[1] + st.offS;
     int64_t* sv1 = P.sv[1] + st.offS;
 
-    if (threadIdx.x == 0) s_flag = 0;
+    if (tid == 0) s_flag = 0;
     __syncthreads();
 
     // ---- the slices' spawn records, gathered after slice 0's in slice order -------------
@@ -279,7 +312,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
         att_sum += P.xatt[st.slice0 + q];
     }
     if (S > capS) {
-        if (threadIdx.x == 0) s_flag = RUN_SPAWN_OVERFLOW;
+        if (tid == 0) s_flag = RUN_SPAWN_OVERFLOW;
         S = capS;
     } else {
         int64_t at = P.xcnt[st.slice0];
@@ -287,7 +320,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
             const int64_t n = P.xcnt[st.slice0 + q];
             const uint64_t* xk = P.xk + st.offX + (q - 1) * capS;
             const int64_t* xv = P.xv + st.offX + (q - 1) * capS;
-            for (int64_t i = threadIdx.x; i < n; i += NT) {
+            for (int64_t i = tid; i < n; i += NT) {
                 sk0[at + i] = xk[i];
                 sv0[at + i] = xv[i];
             }
@@ -304,18 +337,18 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
     int64_t* dstv = sv1;
     for (int sh = 0; sh < B; sh += radix_bits) {
         const uint64_t dmask = (uint64_t)(nbins - 1);
-        for (int d = threadIdx.x; d < nbins; d += NT) s_hist[d] = 0;
-        for (int j = threadIdx.x; j < NW * nbins; j += NT) s_wcnt[j / nbins][j % nbins] = 0;
+        for (int d = tid; d < nbins; d += NT) s_hist[d] = 0;
+        for (int j = tid; j < NW * nbins; j += NT) s_wcnt[j / nbins][j % nbins] = 0;
         __syncthreads();
-        for (int64_t i = threadIdx.x; i < S; i += NT) atomicAdd(&s_hist[(srck[i] >> sh) & dmask], 1u);
+        for (int64_t i = tid; i < S; i += NT) atomicAdd(&s_hist[(srck[i] >> sh) & dmask], 1u);
         __syncthreads();
-        if (threadIdx.x == 0) {  // exclusive digit offsets (at most 256 bins)
+        if (tid == 0) {  // exclusive digit offsets (at most 256 bins)
             uint32_t run_ = 0;
             for (int d = 0; d < nbins; ++d) { const uint32_t c = s_hist[d]; s_hist[d] = run_; run_ += c; }
         }
         __syncthreads();
         for (int64_t t0 = 0; t0 < S; t0 += NT) {
-            const int64_t i = t0 + threadIdx.x;
+            const int64_t i = t0 + tid;
             const bool valid = i < S;
             const uint64_t key = valid ? srck[i] : 0;
             const int64_t val = valid ? srcv[i] : 0;
@@ -335,7 +368,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
                 dstv[pos] = val;
             }
             __syncthreads();
-            for (int d = threadIdx.x; d < nbins; d += NT) {
+            for (int d = tid; d < nbins; d += NT) {
                 uint32_t add = 0;
                 for (int w = 0; w < NW; ++w) { add += s_wcnt[w][d]; s_wcnt[w][d] = 0; }
                 s_hist[d] += add;
@@ -349,7 +382,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
     // ---- merge tile boundaries: spawns before each tile start and the key before it -------
     const uint64_t* ak = srck;
     const int64_t ntl = (S + L + TILE - 1) / TILE;
-    for (int64_t q = threadIdx.x; q <= ntl; q += NT) {
+    for (int64_t q = tid; q <= ntl; q += NT) {
         const int64_t d = min(q * TILE, S + L);
         int64_t lo = max((int64_t)0, d - L), hi = min(d, S);
         while (lo < hi) {
@@ -361,7 +394,7 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
         const uint64_t pa = lo > 0 ? ak[lo - 1] : 0, pb = bi > 0 ? vk[bi - 1] : 0;
         P.mprev[st.offT + q] = (lo > 0 && bi > 0) ? max(pa, pb) : (lo > 0 ? pa : pb);
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         st.S_cur = S;
         st.ntl = ntl;
         st.sbuf = srck == sk0 ? 0 : 1;
@@ -370,19 +403,33 @@ __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ r
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ runs, Pools P, int radix_bits) {
+    __shared__ SortLds<NT> so;
+    sort_phase<NT>(runs, P, blockIdx.x, radix_bits, so, threadIdx.x);
+}
+
 // Merge of the sorted spawns (A) with the diagonal survivors (B): workgroup w of a run takes
 // the tiles [ntl*w/K, ntl*(w+1)/K).  Merged order: by key, B before A on equal keys; each
 // address folds diag + spawns.  Survivors go to the tile's slots with their count and norm.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ runs, Pools P) {
-    constexpr int TILE = NT * MIPT;
-    __shared__ uint64_t s_a[TILE], s_b[TILE];   // tile keys: spawns, vector entries
-    __shared__ int64_t s_av[TILE], s_bv[TILE];  // their values (diagonal survivors for b)
+struct MergeLds {
+    uint64_t a[NT * MIPT], b[NT * MIPT];   // tile keys: spawns, vector entries
+    int64_t av[NT * MIPT], bv[NT * MIPT];  // their values (diagonal survivors for b)
+};
 
-    const int run = P.wg_run[blockIdx.x];
+template <int NT>
+__device__ void merge_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, MergeLds<NT>& me, int tid) {
+    constexpr int TILE = NT * MIPT;
+    uint64_t* s_a = me.a;
+    uint64_t* s_b = me.b;
+    int64_t* s_av = me.av;
+    int64_t* s_bv = me.bv;
+
+    const int run = P.wg_run[gslice];
     const RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
-    const int64_t w = (int64_t)blockIdx.x - st.slice0;
+    const int64_t w = gslice - st.slice0;
     const int64_t ntl = st.ntl, S = st.S_cur, L = st.len;
     const int64_t t_lo = ntl * w / st.K, t_hi = ntl * (w + 1) / st.K;
     const int cur = (int)st.cur;
@@ -405,7 +452,7 @@ __global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ 
             int64_t rv[MIPT];
 #pragma unroll
             for (int r = 0; r < MIPT; ++r) {  // all loads in flight before any is used
-                const int i = (int)threadIdx.x + r * NT;
+                const int i = tid + r * NT;
                 if (i < na + nb) {
                     const bool in_a = i < na;
                     const uint64_t* kp = in_a ? ak + (a0 + i) : vk + (b0 + (i - na));
@@ -416,13 +463,13 @@ __global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ 
             }
 #pragma unroll
             for (int r = 0; r < MIPT; ++r) {
-                const int i = (int)threadIdx.x + r * NT;
+                const int i = tid + r * NT;
                 if (i < na) { s_a[i] = rk[r]; s_av[i] = rv[r]; }
                 else if (i < na + nb) { s_b[i - na] = rk[r]; s_bv[i - na] = rv[r]; }
             }
         }
         __syncthreads();
-        const int dd = threadIdx.x * MIPT;
+        const int dd = tid * MIPT;
         const int n_here = (int)max((int64_t)0, min((int64_t)MIPT, (d1 - d0) - dd));
         int ia;
         {
@@ -481,20 +528,25 @@ __global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ 
             mv[d0 + ex + j] = ok_v[j];
         }
         const double tn = block_sum<NT>(nrm);
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             P.mcnt[st.offT + tl] = tot;
             P.mnrm[st.offT + tl] = tn;
         }
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ runs, Pools P) {
+    __shared__ MergeLds<NT> me;
+    merge_phase<NT>(runs, P, blockIdx.x, me, threadIdx.x);
+}
+
 // The new vector (the tiles' survivors in tile order), ProjectedEnergy, statistics, the
 // history row and the DoubleLogUpdate shift (one workgroup per run).  Tile t is copied only
 // if all of it fits max_length, so an overflowing step leaves a prefix of whole tiles.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ runs, int nruns, Pools P) {
+__device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools& P, int run, int tid) {
     constexpr int TILE = NT * MIPT;
-    const int run = blockIdx.x;
     RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
     const double dtau = st.dtau, shift = st.shift;
@@ -510,7 +562,7 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     // exclusive scan of the tile counts, NT tiles at a time
     int64_t carry = 0;
     for (int64_t t0 = 0; t0 < ntl; t0 += NT) {
-        const int64_t t = t0 + threadIdx.x;
+        const int64_t t = t0 + tid;
         const int64_t c = t < ntl ? P.mcnt[st.offT + t] : 0;
         int64_t tot;
         const int64_t ex = block_exclusive_scan<int64_t, NT>(c, &tot);
@@ -523,8 +575,8 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     // later one, since the offsets grow)
     {
         constexpr int NW = NT / WAVE;
-        const int lane = threadIdx.x & (WAVE - 1);
-        for (int64_t t = threadIdx.x / WAVE; t < ntl; t += NW) {
+        const int lane = tid & (WAVE - 1);
+        for (int64_t t = tid / WAVE; t < ntl; t += NW) {
             const int64_t o = offs[t], n = P.mcnt[st.offT + t];
             if (o + n > capV) continue;
             for (int64_t i = lane; i < n; i += WAVE) {
@@ -535,7 +587,7 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     }
     __syncthreads();
     double nrm = 0;  // integer walkers: the sum of the tile norms is exact in any order
-    for (int64_t t = threadIdx.x; t < ntl; t += NT) nrm += P.mnrm[st.offT + t];
+    for (int64_t t = tid; t < ntl; t += NT) nrm += P.mnrm[st.offT + t];
     const int64_t Lnew = min(out, capV);
     int flag = (int)st.flag;
     if (out > capV && flag == 0) flag = RUN_VEC_OVERFLOW;
@@ -545,7 +597,7 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     double vp = 0, hp = 0;
     {
         const int64_t np = st.nproj;
-        for (int64_t q = threadIdx.x; q < np; q += NT) {
+        for (int64_t q = tid; q < np; q += NT) {
             const uint64_t key = P.pk[st.offP + q];
             int64_t lo = 0, hi = Lnew;
             while (lo < hi) {
@@ -558,7 +610,7 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
         }
     }
     block_sum3<NT>(nrm, vp, hp);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         const int64_t row = st.nrec;
         if (row < P.hist_cap) {
             const int64_t base = row * nruns + run;
@@ -584,6 +636,135 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
             st.cur = 1 - cur;
         }
         st.step = step;
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ runs, int nruns, Pools P) {
+    final_phase<NT>(runs, nruns, P, blockIdx.x, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------------------
+// The whole step in one persistent launch.  Workgroups take work items from a ticket
+// counter in the order: every spawn slice, every run's sort, every merge slice, every run's
+// final phase (slices of the runs with the most slices first, runs in the same order).  An
+// item waits until the items of its run's phase before have finished: a sort for the run's
+// K spawn slices, a merge for the sort, the final phase for the K merge slices.  Those items
+// hold lower tickets, so they were taken by running workgroups and wait only on lower
+// tickets themselves: every wait ends, whatever the grid size and dispatch order.  Between
+// phases the producer's writes are released and the consumer acquires them at agent scope
+// (the workgroups run on different XCDs, whose L2 caches are written back and invalidated
+// by these fences).  Every wait is bounded (spin_limit rounds of s_sleep); a wait that gives
+// up raises the abort word, after which every item is skipped and the host reports the
+// error.  The last workgroup to run out of tickets clears the ticket counters for the next
+// launch; a run's final phase clears its phase counters.
+// ---------------------------------------------------------------------------------------
+struct StepSync {
+    uint32_t* words;        // [0] tickets, [1] workgroups done, [2] abort; from [4], 3 per run:
+                            // spawn slices done, sort done, merge slices done
+    const int* run_order;   // runs by decreasing slice count
+    uint32_t spin_limit;
+    int withhold;           // test of the give-up path: run run_order[0] never reports its spawns
+};
+
+template <int NT>
+union StepLds {
+    SpawnLds<NT> sp;
+    SortLds<NT> so;
+    MergeLds<NT> me;
+};
+
+__device__ inline uint32_t sync_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void sync_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void sync_add(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict__ runs, int nruns, int64_t K,
+                                                           Pools P, int radix_bits, StepSync C) {
+    __shared__ StepLds<NT> sm;
+    __shared__ uint32_t s_ticket;
+    __shared__ int s_go;
+    uint32_t* const tickets = C.words;
+    uint32_t* const done = C.words + 1;
+    uint32_t* const abort_word = C.words + 2;
+    uint32_t* const rs = C.words + 4;
+    const int64_t total = 2 * K + 2 * (int64_t)nruns;
+    while (true) {
+        if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        // uniform over the workgroup: kept in scalar registers, as blockIdx.x is
+        const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane(s_ticket);
+        if (t >= total) {
+            if (threadIdx.x == 0 &&
+                __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+                sync_store(tickets, 0);  // every workgroup has drawn its last ticket
+                sync_store(done, 0);
+            }
+            return;
+        }
+        int phase, run;
+        int64_t item = 0;
+        if (t < K) { phase = 0; item = t; run = P.wg_run[item]; }
+        else if (t < K + nruns) { phase = 1; run = C.run_order[t - K]; }
+        else if (t < 2 * K + nruns) { phase = 2; item = t - K - nruns; run = P.wg_run[item]; }
+        else { phase = 3; run = C.run_order[t - 2 * K - nruns]; }
+        uint32_t* const word = rs + 3 * run;  // spawn slices done, sort done, merge slices done
+        if (threadIdx.x == 0) {
+            // the run's status is that of the last launch until its final phase (this item's
+            // run ends its step only in the final phase, after every other item of it)
+            int go = runs[run].status == RUN_OK && sync_load(abort_word) == 0 ? 1 : 0;
+            if (go && phase > 0) {
+                const uint32_t need = phase == 2 ? 1u : (uint32_t)runs[run].K;
+                uint32_t* const w = word + (phase - 1);
+                uint32_t spins = 0;
+                while (sync_load(w) < need) {
+                    if (sync_load(abort_word) != 0) { go = 0; break; }
+                    if (++spins > C.spin_limit) {
+                        __hip_atomic_fetch_or(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        go = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            // the earlier phase's writes: this CU's L1 and this XCD's L2 are invalidated for
+            // every wave of the workgroup, which waits at the barrier below
+            if (go && phase > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_go = go;
+        }
+        __syncthreads();
+        const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
+        __syncthreads();  // s_ticket and s_go are written again by the next round
+        if (!go) continue;
+        // the thread index is read afresh for every item (thread_index)
+        if (phase == 0) spawn_phase<NT>(runs, P, item, sm.sp, thread_index());
+        else if (phase == 1) sort_phase<NT>(runs, P, run, radix_bits, sm.so, thread_index());
+        else if (phase == 2) merge_phase<NT>(runs, P, item, sm.me, thread_index());
+        else final_phase<NT>(runs, nruns, P, run, thread_index());
+        // this item's writes: every wave's stores have reached this XCD's L2 before the
+        // barrier; thread 0 then writes the L2 back for the other XCDs
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (phase == 0) {
+                if (!(C.withhold && run == C.run_order[0])) sync_add(word, 1u);
+            } else if (phase == 1) {
+                sync_store(word + 1, 1u);
+            } else if (phase == 2) {
+                sync_add(word + 2, 1u);
+            } else {  // every item of the run has finished: its counters start the next step at zero
+                sync_store(word, 0u);
+                sync_store(word + 1, 0u);
+                sync_store(word + 2, 0u);
+            }
+        }
     }
 }
 
@@ -617,6 +798,13 @@ __global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, c
 struct rimu_batch {
     int nruns = 0;
     int block = 512, radix = 8;
+    int persistent = 0;             // 1: the step is one launch of batch_step_kernel
+    uint32_t spin_limit = LB_SPIN_LIMIT;
+    int withhold = 0;               // test of the give-up path (rimu_batch_set_wait)
+    int n_cu = 0;
+    int occupancy[3] = {1, 1, 1};   // resident batch_step_kernel workgroups per CU, by block size
+    DevBuf<uint32_t> sync;          // StepSync words
+    DevBuf<int> run_order;
     hipStream_t stream = nullptr;
     std::vector<RunState> host_init;
     DevBuf<RunState> runs;
@@ -631,7 +819,8 @@ struct rimu_batch {
     DevBuf<uint64_t> mprev, mk;
     DevBuf<double> mnrm;
     int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, hist_cap = 0;
-    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
+    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix)); cleared
+                                                                           // when the launch kind changes
     Pools pools() {
         Pools P{};
         for (int b = 0; b < 2; ++b) {
@@ -662,9 +851,26 @@ struct rimu_batch {
 
 namespace {
 
+template <int NT>
+void launch_persistent(rimu_batch* b, int occ) {
+    const int64_t total = 2 * b->totK + 2 * (int64_t)b->nruns;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)b->n_cu * occ));
+    const StepSync C{b->sync.p, b->run_order.p, b->spin_limit, b->withhold};
+    batch_step_kernel<NT><<<grid, NT, 0, b->stream>>>(b->runs.p, b->nruns, b->totK, b->pools(), b->radix, C);
+}
+
 int launch_step(rimu_batch* b) {
     const Pools P = b->pools();
     const int K = (int)b->totK;
+    if (b->persistent) {
+        switch (b->block) {
+            case 256: launch_persistent<256>(b, b->occupancy[0]); break;
+            case 1024: launch_persistent<1024>(b, b->occupancy[2]); break;
+            default: launch_persistent<512>(b, b->occupancy[1]); break;
+        }
+        HIP_TRY(hipGetLastError());
+        return RIMU_OK;
+    }
     batch_spawn_kernel<SPAWN_NT><<<K, SPAWN_NT, 0, b->stream>>>(b->runs.p, P);
     switch (b->block) {
         case 256:
@@ -764,9 +970,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         s.offV = totV; s.offS = totS; s.offP = totP; s.nproj = 0;
         // spawning of a run is split over K workgroups, one per WALKERS_PER_SLICE target walkers
         s.K = std::min<int64_t>(MAX_SLICES, std::max<int64_t>(1, (int64_t)std::ceil(q.target_walkers / WALKERS_PER_SLICE)));
-        s.slice0 = (int64_t)wg_run.size();
         s.offX = totX;
-        for (int64_t k = 0; k < s.K; ++k) wg_run.push_back(r);
         // merge tiles: boundaries for the smallest tile, slots for the largest
         s.offT = totT;
         s.offM = totM;
@@ -774,6 +978,15 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         totM += s.capS + s.capV + MAX_TILE;
         totV += s.capV; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
         start[r] = q.start_address;
+    }
+    // slices numbered run by run, runs with the most slices first (the order in which the
+    // persistent step takes them)
+    std::vector<int> order(nruns);
+    for (int r = 0; r < nruns; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return st[x].K > st[y].K; });
+    for (int r : order) {
+        st[r].slice0 = (int64_t)wg_run.size();
+        for (int64_t k = 0; k < st[r].K; ++k) wg_run.push_back(r);
     }
     rimu_batch* b = new rimu_batch();
     b->nruns = nruns;
@@ -811,6 +1024,21 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
     if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
     if ((e = hipMemcpy(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    if ((e = b->run_order.alloc(nruns)) != hipSuccess) return bad("run order", e);
+    if ((e = hipMemcpy(b->run_order.p, order.data(), nruns * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    if ((e = b->sync.alloc(4 + 3 * (int64_t)nruns)) != hipSuccess) return bad("step sync", e);
+
+    if ((e = hipMemset(b->sync.p, 0, (4 + 3 * (int64_t)nruns) * sizeof(uint32_t))) != hipSuccess) return bad("step sync", e);
+    {
+        int dev = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return bad("device", e);
+        if ((e = hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return bad("device", e);
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[0], batch_step_kernel<256>, 256, 0)) != hipSuccess ||
+            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[1], batch_step_kernel<512>, 512, 0)) != hipSuccess ||
+            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[2], batch_step_kernel<1024>, 1024, 0)) != hipSuccess)
+            return bad("occupancy", e);
+        for (int& o : b->occupancy) o = std::max(o, 1);
+    }
     if ((e = hipMemcpy(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpy(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
     for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
@@ -834,6 +1062,7 @@ int rimu_batch_destroy(rimu_batch* b) {
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
     b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
     b->msplit.release(); b->mprev.release(); b->mcnt.release(); b->mnrm.release(); b->mk.release(); b->mv.release();
+    b->sync.release(); b->run_order.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return RIMU_OK;
@@ -849,11 +1078,33 @@ int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits)
     return RIMU_OK;
 }
 
+int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (persistent != 0 && persistent != 1) return fail(RIMU_ERR_ARG, "persistent must be 0 or 1");
+    if (persistent != b->persistent) clear_graphs(b);
+    b->persistent = persistent;
+    return RIMU_OK;
+}
+
+int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_first) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (spin_limit < 1) return fail(RIMU_ERR_ARG, "spin_limit must be positive");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemset(b->sync.p, 0, (4 + 3 * (int64_t)b->nruns) * sizeof(uint32_t)));
+    clear_graphs(b);  // the bound and the flag are arguments of the captured launches
+    b->spin_limit = spin_limit;
+    b->withhold = withhold_first != 0;
+    return RIMU_OK;
+}
+
 int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (nsteps < 0 || steps_per_graph < 1) return fail(RIMU_ERR_ARG, "need nsteps >= 0 and steps_per_graph >= 1");
     if (int rc = run_steps(b, nsteps, steps_per_graph)) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
+    uint32_t abort_word = 0;
+    HIP_TRY(hipMemcpy(&abort_word, b->sync.p + 2, sizeof(abort_word), hipMemcpyDeviceToHost));
+    if (abort_word) return fail(RIMU_ERR_INTERNAL, "a wait of the persistent batched step gave up");
     return RIMU_OK;
 }
 
@@ -896,7 +1147,8 @@ int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals,
     return RIMU_OK;
 }
 
-int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix, double* ms) {
+int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix,
+                        int32_t* best_persistent, double* ms) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (trial_steps < 1) return fail(RIMU_ERR_ARG, "trial_steps must be >= 1");
     // snapshot: run states and both vector buffers (the step reads one, writes the other)
@@ -925,13 +1177,16 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         const int blocks[3] = {256, 512, 1024};
         const int radixes[3] = {4, 6, 8};
         double best = 1e300;
-        int bb = b->block, br = b->radix;
+        const int p0 = b->persistent;
+        int bb = b->block, br = b->radix, bp = b->persistent;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int idx = 0;
+        for (int pi = 0; pi < 2; ++pi)
         for (int bi = 0; bi < 3; ++bi)
             for (int ri = 0; ri < 3; ++ri, ++idx) {
+                b->persistent = pi;
                 b->block = blocks[bi];
                 b->radix = radixes[ri];
                 if (int rc = launch_step(b)) return rc;  // first launch outside the timing
@@ -946,15 +1201,21 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
                 HIP_TRY(hipEventElapsedTime(&t, e0, e1));
                 const double per = t / (double)trial_steps;
                 if (ms) ms[idx] = per;
-                if (per < best) { best = per; bb = b->block; br = b->radix; }
+                if (per < best) { best = per; bb = b->block; br = b->radix; bp = b->persistent; }
                 if (int rc = restore()) return rc;
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        uint32_t abort_word = 0;
+        HIP_TRY(hipMemcpy(&abort_word, b->sync.p + 2, sizeof(abort_word), hipMemcpyDeviceToHost));
+        if (abort_word) return fail(RIMU_ERR_INTERNAL, "a wait of the persistent batched step gave up");
+        if (bp != p0) clear_graphs(b);
         b->block = bb;
         b->radix = br;
+        b->persistent = bp;
         if (best_block) *best_block = bb;
         if (best_radix) *best_radix = br;
+        if (best_persistent) *best_persistent = bp;
         return RIMU_OK;
     };
     const int rc = body();
