@@ -275,6 +275,8 @@ template <int NT>
 struct SortLds {
     uint32_t hist[256];
     uint32_t wcnt[NT / WAVE][256];
+    int64_t xoff[MAX_SLICES + 1];  // first record of each slice in the gathered list
+    double att;
     int flag;
 };
 
@@ -305,26 +307,53 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
     __syncthreads();
 
     // ---- the slices' spawn records, gathered after slice 0's in slice order -------------
-    int64_t S = 0;
-    double att_sum = 0;
-    for (int64_t q = 0; q < st.K; ++q) {
-        S += P.xcnt[st.slice0 + q];
-        att_sum += P.xatt[st.slice0 + q];
+    // slice counts and attempts: lane q of the first wavefront reads slice q (K <= 64)
+    const int K = (int)st.K;
+    if (tid < WAVE) {
+        const int64_t c = tid < K ? P.xcnt[st.slice0 + tid] : 0;
+        double a = tid < K ? P.xatt[st.slice0 + tid] : 0.0;
+        const int64_t inc = wave_inclusive_scan(c);
+        so.xoff[tid + 1] = inc;
+        if (tid == 0) so.xoff[0] = 0;
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, WAVE);  // integers: exact
+        if (tid == 0) so.att = a;
     }
+    __syncthreads();
+    int64_t S = so.xoff[K];
+    const double att_sum = so.att;
     if (S > capS) {
         if (tid == 0) s_flag = RUN_SPAWN_OVERFLOW;
         S = capS;
     } else {
-        int64_t at = P.xcnt[st.slice0];
-        for (int64_t q = 1; q < st.K; ++q) {
-            const int64_t n = P.xcnt[st.slice0 + q];
-            const uint64_t* xk = P.xk + st.offX + (q - 1) * capS;
-            const int64_t* xv = P.xv + st.offX + (q - 1) * capS;
-            for (int64_t i = tid; i < n; i += NT) {
-                sk0[at + i] = xk[i];
-                sv0[at + i] = xv[i];
+        // record j >= (slice 0's count) comes from slice q, xoff[q] <= j < xoff[q + 1]; four
+        // records per thread at a time, all loads in flight before the stores
+        constexpr int CU = 4;
+        for (int64_t j0 = so.xoff[1]; j0 < S; j0 += CU * NT) {
+            uint64_t kq[CU];
+            int64_t vq[CU];
+#pragma unroll
+            for (int c = 0; c < CU; ++c) {
+                const int64_t j = j0 + c * NT + tid;
+                if (j < S) {
+                    int lo = 1, hi = K;  // xoff[lo] <= j < xoff[hi]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (so.xoff[mid] <= j) lo = mid; else hi = mid;
+                    }
+                    const int64_t src = st.offX + (int64_t)(lo - 1) * capS + (j - so.xoff[lo]);
+                    kq[c] = P.xk[src];
+                    vq[c] = P.xv[src];
+                }
             }
-            at += n;
+#pragma unroll
+            for (int c = 0; c < CU; ++c) {
+                const int64_t j = j0 + c * NT + tid;
+                if (j < S) {
+                    sk0[j] = kq[c];
+                    sv0[j] = vq[c];
+                }
+            }
         }
     }
     __syncthreads();
@@ -572,16 +601,34 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
     const int64_t out = carry;
     __syncthreads();
     // one wavefront per tile; tiles that do not fit entirely are skipped (and so is every
-    // later one, since the offsets grow)
+    // later one, since the offsets grow).  CU elements per lane at a time, all loads in
+    // flight before the stores.
     {
         constexpr int NW = NT / WAVE;
+        constexpr int CU = 4;
         const int lane = tid & (WAVE - 1);
         for (int64_t t = tid / WAVE; t < ntl; t += NW) {
             const int64_t o = offs[t], n = P.mcnt[st.offT + t];
             if (o + n > capV) continue;
-            for (int64_t i = lane; i < n; i += WAVE) {
-                nk[o + i] = mk[t * TILE + i];
-                nv[o + i] = mv[t * TILE + i];
+            for (int64_t i0 = 0; i0 < n; i0 += CU * WAVE) {
+                uint64_t kq[CU];
+                int64_t vq[CU];
+#pragma unroll
+                for (int c = 0; c < CU; ++c) {
+                    const int64_t i = i0 + c * WAVE + lane;
+                    if (i < n) {
+                        kq[c] = mk[t * TILE + i];
+                        vq[c] = mv[t * TILE + i];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < CU; ++c) {
+                    const int64_t i = i0 + c * WAVE + lane;
+                    if (i < n) {
+                        nk[o + i] = kq[c];
+                        nv[o + i] = vq[c];
+                    }
+                }
             }
         }
     }
