@@ -1,22 +1,21 @@
 // batched.hip -- many independent IsStochasticInteger FCIQMC runs of HubbardReal1D advanced
 // together (C ABI rimu_batch_*).
 //
-// A step of every run has four phases, the plain engine's algorithm at the size of a run:
+// A step of every run has five phases, the plain engine's algorithm at the size of a run:
 // spawn slices (a run's entries split over K workgroups, one per 2048 target walkers:
 // diagonal step and spawn attempts, ballot compaction into the slice's spawn buffer), the
 // run's sort (one workgroup: gather of the slices in order, stable LSD radix sort with LDS
 // histograms and ballot ranks, merge tile boundaries), merge slices (the run's merge tiles
-// over K workgroups) and the run's final phase (one workgroup: the new vector, norm,
-// ProjectedEnergy values, history row and the DoubleLogUpdate shift update,
-// shiftstrategy.jl:174-181).  The phases run as four launches covering all runs, or as one
-// persistent launch in which workgroups take the phases' items from a work queue
-// (batch_step_kernel).  The step number, shift and lengths live in device memory, so a
-// sequence of steps is captured once in a HIP graph and replayed.
+// over K workgroups), copy slices (the tiles' survivors into the new vector, over K
+// workgroups) and the run's final phase (one workgroup: norm, ProjectedEnergy values,
+// history row and the DoubleLogUpdate shift update, shiftstrategy.jl:174-181), each phase
+// one launch covering all runs.  The step number, shift and lengths live in device memory,
+// so a sequence of steps is captured once in a HIP graph and replayed.
 //
 // Random draws are addressed by (seed, address, attempt, step) and integer walker sums are
-// exact, so the result of a step does not depend on the workgroup size, radix width or
-// launch kind (except the vector an overflowing step leaves: a prefix of whole merge tiles,
-// whose size follows the workgroup size).
+// exact, so the result of a step does not depend on the workgroup size or radix width
+// (except the vector an overflowing step leaves: a prefix of whole merge tiles, whose size
+// follows the workgroup size).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,15 +82,6 @@ struct Pools {
 };
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
-
-// The thread index, read afresh: values derived from it then stay inside the phase that
-// computes them instead of being hoisted out of the persistent step's loop and held in
-// registers across all phases.
-__device__ inline int thread_index() {
-    int t;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
-    return t;
-}
 
 // Diagonal step and spawning of one slice of a run: entries [L*s/K, L*(s+1)/K).  Records go,
 // in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
@@ -570,69 +560,84 @@ __global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ 
     merge_phase<NT>(runs, P, blockIdx.x, me, threadIdx.x);
 }
 
-// The new vector (the tiles' survivors in tile order), ProjectedEnergy, statistics, the
-// history row and the DoubleLogUpdate shift (one workgroup per run).  Tile t is copied only
-// if all of it fits max_length, so an overflowing step leaves a prefix of whole tiles.
+// The tiles' survivors into the new vector: copy slice w of a run takes the tiles
+// [ntl*w/K, ntl*(w+1)/K), their offset the survivors of the tiles before.  A tile that does
+// not fit entirely is skipped (and so is every later one, since the offsets grow), so an
+// overflowing step leaves a prefix of whole tiles.
+template <int NT>
+__device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, int tid) {
+    constexpr int TILE = NT * MIPT;
+    constexpr int CU = 4;  // elements per thread at a time, all loads in flight before the stores
+    const int run = P.wg_run[gslice];
+    const RunState& st = runs[run];
+    if (st.status != RUN_OK) return;  // uniform across the workgroup
+    const int64_t w = gslice - st.slice0;
+    const int64_t ntl = st.ntl, capV = st.capV;
+    const int64_t t_lo = ntl * w / st.K, t_hi = ntl * (w + 1) / st.K;
+    if (t_lo >= t_hi) return;
+    const int cur = (int)st.cur;
+    uint64_t* nk = P.vk[1 - cur] + st.offV;
+    int64_t* nv = P.vv[1 - cur] + st.offV;
+    const uint64_t* mk = P.mk + st.offM;
+    const int64_t* mv = P.mv + st.offM;
+    int64_t part = 0;
+    for (int64_t t = tid; t < t_lo; t += NT) part += P.mcnt[st.offT + t];
+    int64_t o;
+    (void)block_exclusive_scan<int64_t, NT>(part, &o);
+    for (int64_t t = t_lo; t < t_hi; ++t) {
+        const int64_t n = P.mcnt[st.offT + t];
+        if (o + n > capV) break;
+        for (int64_t i0 = 0; i0 < n; i0 += CU * NT) {
+            uint64_t kq[CU];
+            int64_t vq[CU];
+#pragma unroll
+            for (int c = 0; c < CU; ++c) {
+                const int64_t i = i0 + c * NT + tid;
+                if (i < n) {
+                    kq[c] = mk[t * TILE + i];
+                    vq[c] = mv[t * TILE + i];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CU; ++c) {
+                const int64_t i = i0 + c * NT + tid;
+                if (i < n) {
+                    nk[o + i] = kq[c];
+                    nv[o + i] = vq[c];
+                }
+            }
+        }
+        o += n;
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_copy_kernel(RunState* __restrict__ runs, Pools P) {
+    copy_phase<NT>(runs, P, blockIdx.x, threadIdx.x);
+}
+
+// ProjectedEnergy, statistics, the history row and the DoubleLogUpdate shift (one
+// workgroup per run), from the new vector that copy_phase wrote.
 template <int NT>
 __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools& P, int run, int tid) {
-    constexpr int TILE = NT * MIPT;
     RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
     const double dtau = st.dtau, shift = st.shift;
     const int64_t capV = st.capV, ntl = st.ntl;
     const int cur = (int)st.cur;
     const int64_t step = st.step + 1;
-    uint64_t* nk = P.vk[1 - cur] + st.offV;
-    int64_t* nv = P.vv[1 - cur] + st.offV;
-    const uint64_t* mk = P.mk + st.offM;
-    const int64_t* mv = P.mv + st.offM;
-    int64_t* offs = P.msplit + st.offT;   // the splits are spent: offsets of the survivors
+    const uint64_t* nk = P.vk[1 - cur] + st.offV;  // the new vector (copy_phase)
+    const int64_t* nv = P.vv[1 - cur] + st.offV;
 
-    // exclusive scan of the tile counts, NT tiles at a time
-    int64_t carry = 0;
+    // survivors of all tiles
+    int64_t out = 0;
     for (int64_t t0 = 0; t0 < ntl; t0 += NT) {
         const int64_t t = t0 + tid;
         const int64_t c = t < ntl ? P.mcnt[st.offT + t] : 0;
         int64_t tot;
-        const int64_t ex = block_exclusive_scan<int64_t, NT>(c, &tot);
-        if (t < ntl) offs[t] = carry + ex;
-        carry += tot;
+        (void)block_exclusive_scan<int64_t, NT>(c, &tot);
+        out += tot;
     }
-    const int64_t out = carry;
-    __syncthreads();
-    // one wavefront per tile; tiles that do not fit entirely are skipped (and so is every
-    // later one, since the offsets grow).  CU elements per lane at a time, all loads in
-    // flight before the stores.
-    {
-        constexpr int NW = NT / WAVE;
-        constexpr int CU = 4;
-        const int lane = tid & (WAVE - 1);
-        for (int64_t t = tid / WAVE; t < ntl; t += NW) {
-            const int64_t o = offs[t], n = P.mcnt[st.offT + t];
-            if (o + n > capV) continue;
-            for (int64_t i0 = 0; i0 < n; i0 += CU * WAVE) {
-                uint64_t kq[CU];
-                int64_t vq[CU];
-#pragma unroll
-                for (int c = 0; c < CU; ++c) {
-                    const int64_t i = i0 + c * WAVE + lane;
-                    if (i < n) {
-                        kq[c] = mk[t * TILE + i];
-                        vq[c] = mv[t * TILE + i];
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < CU; ++c) {
-                    const int64_t i = i0 + c * WAVE + lane;
-                    if (i < n) {
-                        nk[o + i] = kq[c];
-                        nv[o + i] = vq[c];
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
     double nrm = 0;  // integer walkers: the sum of the tile norms is exact in any order
     for (int64_t t = tid; t < ntl; t += NT) nrm += P.mnrm[st.offT + t];
     const int64_t Lnew = min(out, capV);
@@ -691,130 +696,6 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     final_phase<NT>(runs, nruns, P, blockIdx.x, threadIdx.x);
 }
 
-// ---------------------------------------------------------------------------------------
-// The whole step in one persistent launch.  Workgroups take work items from a ticket
-// counter in the order: every spawn slice, every run's sort, every merge slice, every run's
-// final phase (slices of the runs with the most slices first, runs in the same order).  An
-// item waits until the items of its run's phase before have finished: a sort for the run's
-// K spawn slices, a merge for the sort, the final phase for the K merge slices.  Those items
-// hold lower tickets, so they were taken by running workgroups and wait only on lower
-// tickets themselves: every wait ends, whatever the grid size and dispatch order.  Between
-// phases the producer's writes are released and the consumer acquires them at agent scope
-// (the workgroups run on different XCDs, whose L2 caches are written back and invalidated
-// by these fences).  Every wait is bounded (spin_limit rounds of s_sleep); a wait that gives
-// up raises the abort word, after which every item is skipped and the host reports the
-// error.  The last workgroup to run out of tickets clears the ticket counters for the next
-// launch; a run's final phase clears its phase counters.
-// ---------------------------------------------------------------------------------------
-struct StepSync {
-    uint32_t* words;        // [0] tickets, [1] workgroups done, [2] abort; from [4], 3 per run:
-                            // spawn slices done, sort done, merge slices done
-    const int* run_order;   // runs by decreasing slice count
-    uint32_t spin_limit;
-    int withhold;           // test of the give-up path: run run_order[0] never reports its spawns
-};
-
-template <int NT>
-union StepLds {
-    SpawnLds<NT> sp;
-    SortLds<NT> so;
-    MergeLds<NT> me;
-};
-
-__device__ inline uint32_t sync_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void sync_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void sync_add(uint32_t* p, uint32_t v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int NT>
-__global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict__ runs, int nruns, int64_t K,
-                                                           Pools P, int radix_bits, StepSync C) {
-    __shared__ StepLds<NT> sm;
-    __shared__ uint32_t s_ticket;
-    __shared__ int s_go;
-    uint32_t* const tickets = C.words;
-    uint32_t* const done = C.words + 1;
-    uint32_t* const abort_word = C.words + 2;
-    uint32_t* const rs = C.words + 4;
-    const int64_t total = 2 * K + 2 * (int64_t)nruns;
-    while (true) {
-        if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        // uniform over the workgroup: kept in scalar registers, as blockIdx.x is
-        const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane(s_ticket);
-        if (t >= total) {
-            if (threadIdx.x == 0 &&
-                __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
-                sync_store(tickets, 0);  // every workgroup has drawn its last ticket
-                sync_store(done, 0);
-            }
-            return;
-        }
-        int phase, run;
-        int64_t item = 0;
-        if (t < K) { phase = 0; item = t; run = P.wg_run[item]; }
-        else if (t < K + nruns) { phase = 1; run = C.run_order[t - K]; }
-        else if (t < 2 * K + nruns) { phase = 2; item = t - K - nruns; run = P.wg_run[item]; }
-        else { phase = 3; run = C.run_order[t - 2 * K - nruns]; }
-        uint32_t* const word = rs + 3 * run;  // spawn slices done, sort done, merge slices done
-        if (threadIdx.x == 0) {
-            // the run's status is that of the last launch until its final phase (this item's
-            // run ends its step only in the final phase, after every other item of it)
-            int go = runs[run].status == RUN_OK && sync_load(abort_word) == 0 ? 1 : 0;
-            if (go && phase > 0) {
-                const uint32_t need = phase == 2 ? 1u : (uint32_t)runs[run].K;
-                uint32_t* const w = word + (phase - 1);
-                uint32_t spins = 0;
-                while (sync_load(w) < need) {
-                    if (sync_load(abort_word) != 0) { go = 0; break; }
-                    if (++spins > C.spin_limit) {
-                        __hip_atomic_fetch_or(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        go = 0;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-            // the earlier phase's writes: this CU's L1 and this XCD's L2 are invalidated for
-            // every wave of the workgroup, which waits at the barrier below
-            if (go && phase > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            s_go = go;
-        }
-        __syncthreads();
-        const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
-        __syncthreads();  // s_ticket and s_go are written again by the next round
-        if (!go) continue;
-        // the thread index is read afresh for every item (thread_index)
-        if (phase == 0) spawn_phase<NT>(runs, P, item, sm.sp, thread_index());
-        else if (phase == 1) sort_phase<NT>(runs, P, run, radix_bits, sm.so, thread_index());
-        else if (phase == 2) merge_phase<NT>(runs, P, item, sm.me, thread_index());
-        else final_phase<NT>(runs, nruns, P, run, thread_index());
-        // this item's writes: every wave's stores have reached this XCD's L2 before the
-        // barrier; thread 0 then writes the L2 back for the other XCDs
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (phase == 0) {
-                if (!(C.withhold && run == C.run_order[0])) sync_add(word, 1u);
-            } else if (phase == 1) {
-                sync_store(word + 1, 1u);
-            } else if (phase == 2) {
-                sync_add(word + 2, 1u);
-            } else {  // every item of the run has finished: its counters start the next step at zero
-                sync_store(word, 0u);
-                sync_store(word + 1, 0u);
-                sync_store(word + 2, 0u);
-            }
-        }
-    }
-}
-
 // Projector of every run: the start address (coefficient diag) and its neighbours.
 __global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, const uint64_t* __restrict__ start,
                                        uint64_t* __restrict__ pk, double* __restrict__ pc) {
@@ -845,13 +726,6 @@ __global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, c
 struct rimu_batch {
     int nruns = 0;
     int block = 512, radix = 8;
-    int persistent = 0;             // 1: the step is one launch of batch_step_kernel
-    uint32_t spin_limit = LB_SPIN_LIMIT;
-    int withhold = 0;               // test of the give-up path (rimu_batch_set_wait)
-    int n_cu = 0;
-    int occupancy[3] = {1, 1, 1};   // resident batch_step_kernel workgroups per CU, by block size
-    DevBuf<uint32_t> sync;          // StepSync words
-    DevBuf<int> run_order;
     hipStream_t stream = nullptr;
     std::vector<RunState> host_init;
     DevBuf<RunState> runs;
@@ -866,8 +740,7 @@ struct rimu_batch {
     DevBuf<uint64_t> mprev, mk;
     DevBuf<double> mnrm;
     int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, hist_cap = 0;
-    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix)); cleared
-                                                                           // when the launch kind changes
+    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
     Pools pools() {
         Pools P{};
         for (int b = 0; b < 2; ++b) {
@@ -898,41 +771,27 @@ struct rimu_batch {
 
 namespace {
 
-template <int NT>
-void launch_persistent(rimu_batch* b, int occ) {
-    const int64_t total = 2 * b->totK + 2 * (int64_t)b->nruns;
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)b->n_cu * occ));
-    const StepSync C{b->sync.p, b->run_order.p, b->spin_limit, b->withhold};
-    batch_step_kernel<NT><<<grid, NT, 0, b->stream>>>(b->runs.p, b->nruns, b->totK, b->pools(), b->radix, C);
-}
-
 int launch_step(rimu_batch* b) {
     const Pools P = b->pools();
     const int K = (int)b->totK;
-    if (b->persistent) {
-        switch (b->block) {
-            case 256: launch_persistent<256>(b, b->occupancy[0]); break;
-            case 1024: launch_persistent<1024>(b, b->occupancy[2]); break;
-            default: launch_persistent<512>(b, b->occupancy[1]); break;
-        }
-        HIP_TRY(hipGetLastError());
-        return RIMU_OK;
-    }
     batch_spawn_kernel<SPAWN_NT><<<K, SPAWN_NT, 0, b->stream>>>(b->runs.p, P);
     switch (b->block) {
         case 256:
             batch_sort_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, P, b->radix);
             batch_merge_kernel<256><<<K, 256, 0, b->stream>>>(b->runs.p, P);
+            batch_copy_kernel<256><<<K, 256, 0, b->stream>>>(b->runs.p, P);
             batch_final_kernel<256><<<b->nruns, 256, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
         case 1024:
             batch_sort_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, P, b->radix);
             batch_merge_kernel<1024><<<K, 1024, 0, b->stream>>>(b->runs.p, P);
+            batch_copy_kernel<1024><<<K, 1024, 0, b->stream>>>(b->runs.p, P);
             batch_final_kernel<1024><<<b->nruns, 1024, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
         default:
             batch_sort_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, P, b->radix);
             batch_merge_kernel<512><<<K, 512, 0, b->stream>>>(b->runs.p, P);
+            batch_copy_kernel<512><<<K, 512, 0, b->stream>>>(b->runs.p, P);
             batch_final_kernel<512><<<b->nruns, 512, 0, b->stream>>>(b->runs.p, b->nruns, P);
             break;
     }
@@ -1026,8 +885,8 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         totV += s.capV; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
         start[r] = q.start_address;
     }
-    // slices numbered run by run, runs with the most slices first (the order in which the
-    // persistent step takes them)
+    // slices numbered run by run, runs with the most slices first (their workgroups are
+    // dispatched first)
     std::vector<int> order(nruns);
     for (int r = 0; r < nruns; ++r) order[r] = r;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return st[x].K > st[y].K; });
@@ -1070,28 +929,14 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->mnrm.alloc(totT)) != hipSuccess) return bad("merge norms", e);
     if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
     if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
-    if ((e = hipMemcpy(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
-    if ((e = b->run_order.alloc(nruns)) != hipSuccess) return bad("run order", e);
-    if ((e = hipMemcpy(b->run_order.p, order.data(), nruns * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
-    if ((e = b->sync.alloc(4 + 3 * (int64_t)nruns)) != hipSuccess) return bad("step sync", e);
-
-    if ((e = hipMemset(b->sync.p, 0, (4 + 3 * (int64_t)nruns) * sizeof(uint32_t))) != hipSuccess) return bad("step sync", e);
-    {
-        int dev = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return bad("device", e);
-        if ((e = hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return bad("device", e);
-        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[0], batch_step_kernel<256>, 256, 0)) != hipSuccess ||
-            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[1], batch_step_kernel<512>, 512, 0)) != hipSuccess ||
-            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[2], batch_step_kernel<1024>, 1024, 0)) != hipSuccess)
-            return bad("occupancy", e);
-        for (int& o : b->occupancy) o = std::max(o, 1);
-    }
-    if ((e = hipMemcpy(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
-    if ((e = hipMemcpy(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    std::vector<int64_t> pops(nruns);  // host buffers live until the stream is synchronized below
     for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
-        const int64_t pop = runs[r].start_population;
-        if ((e = hipMemcpy(b->vk[0].p + st[r].offV, &start[r], 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
-        if ((e = hipMemcpy(b->vv[0].p + st[r].offV, &pop, 8, hipMemcpyHostToDevice)) != hipSuccess) return bad("upload", e);
+        pops[r] = runs[r].start_population;
+        if ((e = hipMemcpyAsync(b->vk[0].p + st[r].offV, &start[r], 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+        if ((e = hipMemcpyAsync(b->vv[0].p + st[r].offV, &pops[r], 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     }
     batch_projector_kernel<<<ceil_div(nruns, 64), 64, 0, b->stream>>>(b->runs.p, nruns, b->start.p, b->pk.p, b->pc.p);
     if ((e = hipGetLastError()) != hipSuccess) return bad("projector", e);
@@ -1109,7 +954,6 @@ int rimu_batch_destroy(rimu_batch* b) {
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
     b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
     b->msplit.release(); b->mprev.release(); b->mcnt.release(); b->mnrm.release(); b->mk.release(); b->mv.release();
-    b->sync.release(); b->run_order.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return RIMU_OK;
@@ -1125,33 +969,11 @@ int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits)
     return RIMU_OK;
 }
 
-int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent) {
-    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
-    if (persistent != 0 && persistent != 1) return fail(RIMU_ERR_ARG, "persistent must be 0 or 1");
-    if (persistent != b->persistent) clear_graphs(b);
-    b->persistent = persistent;
-    return RIMU_OK;
-}
-
-int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_first) {
-    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
-    if (spin_limit < 1) return fail(RIMU_ERR_ARG, "spin_limit must be positive");
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(hipMemset(b->sync.p, 0, (4 + 3 * (int64_t)b->nruns) * sizeof(uint32_t)));
-    clear_graphs(b);  // the bound and the flag are arguments of the captured launches
-    b->spin_limit = spin_limit;
-    b->withhold = withhold_first != 0;
-    return RIMU_OK;
-}
-
 int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (nsteps < 0 || steps_per_graph < 1) return fail(RIMU_ERR_ARG, "need nsteps >= 0 and steps_per_graph >= 1");
     if (int rc = run_steps(b, nsteps, steps_per_graph)) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    uint32_t abort_word = 0;
-    HIP_TRY(hipMemcpy(&abort_word, b->sync.p + 2, sizeof(abort_word), hipMemcpyDeviceToHost));
-    if (abort_word) return fail(RIMU_ERR_INTERNAL, "a wait of the persistent batched step gave up");
     return RIMU_OK;
 }
 
@@ -1194,8 +1016,7 @@ int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals,
     return RIMU_OK;
 }
 
-int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix,
-                        int32_t* best_persistent, double* ms) {
+int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix, double* ms) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (trial_steps < 1) return fail(RIMU_ERR_ARG, "trial_steps must be >= 1");
     // snapshot: run states and both vector buffers (the step reads one, writes the other)
@@ -1206,34 +1027,32 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
     auto body = [&]() -> int {
         HIP_TRY(hipStreamSynchronize(b->stream));
         HIP_TRY(s_runs.alloc(b->nruns));
-        HIP_TRY(hipMemcpy(s_runs.p, b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice));
+        // copies on the batch's stream, ordered with its kernels
+        HIP_TRY(hipMemcpyAsync(s_runs.p, b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
         for (int k = 0; k < 2; ++k) {
             HIP_TRY(s_vk[k].alloc(b->totV));
             HIP_TRY(s_vv[k].alloc(b->totV));
-            HIP_TRY(hipMemcpy(s_vk[k].p, b->vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
-            HIP_TRY(hipMemcpy(s_vv[k].p, b->vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpyAsync(s_vk[k].p, b->vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipMemcpyAsync(s_vv[k].p, b->vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
         }
         auto restore = [&]() -> int {
-            HIP_TRY(hipMemcpy(b->runs.p, s_runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpyAsync(b->runs.p, s_runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
             for (int k = 0; k < 2; ++k) {
-                HIP_TRY(hipMemcpy(b->vk[k].p, s_vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
-                HIP_TRY(hipMemcpy(b->vv[k].p, s_vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice));
+                HIP_TRY(hipMemcpyAsync(b->vk[k].p, s_vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
+                HIP_TRY(hipMemcpyAsync(b->vv[k].p, s_vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
             }
             return RIMU_OK;
         };
         const int blocks[3] = {256, 512, 1024};
         const int radixes[3] = {4, 6, 8};
         double best = 1e300;
-        const int p0 = b->persistent;
-        int bb = b->block, br = b->radix, bp = b->persistent;
+        int bb = b->block, br = b->radix;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int idx = 0;
-        for (int pi = 0; pi < 2; ++pi)
         for (int bi = 0; bi < 3; ++bi)
             for (int ri = 0; ri < 3; ++ri, ++idx) {
-                b->persistent = pi;
                 b->block = blocks[bi];
                 b->radix = radixes[ri];
                 if (int rc = launch_step(b)) return rc;  // first launch outside the timing
@@ -1248,21 +1067,16 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
                 HIP_TRY(hipEventElapsedTime(&t, e0, e1));
                 const double per = t / (double)trial_steps;
                 if (ms) ms[idx] = per;
-                if (per < best) { best = per; bb = b->block; br = b->radix; bp = b->persistent; }
+                if (per < best) { best = per; bb = b->block; br = b->radix; }
                 if (int rc = restore()) return rc;
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        uint32_t abort_word = 0;
-        HIP_TRY(hipMemcpy(&abort_word, b->sync.p + 2, sizeof(abort_word), hipMemcpyDeviceToHost));
-        if (abort_word) return fail(RIMU_ERR_INTERNAL, "a wait of the persistent batched step gave up");
-        if (bp != p0) clear_graphs(b);
+        HIP_TRY(hipStreamSynchronize(b->stream));  // the last restore
         b->block = bb;
         b->radix = br;
-        b->persistent = bp;
         if (best_block) *best_block = bb;
         if (best_radix) *best_radix = br;
-        if (best_persistent) *best_persistent = bp;
         return RIMU_OK;
     };
     const int rc = body();
