@@ -62,6 +62,27 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
+// N deterministic workgroup sums at once in scratch (N * NT doubles of LDS that the caller
+// no longer needs), each in the tree order of block_sum (the same bits).  Results valid in
+// thread 0.
+template <int NT, int N>
+__device__ inline void block_sum_n(double (&v)[N], double* scratch) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) scratch[j * NT + threadIdx.x] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) scratch[j * NT + threadIdx.x] += scratch[j * NT + threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = scratch[j * NT];
+    __syncthreads();
+}
+
 // Three deterministic workgroup sums at once.  Results valid in thread 0.
 template <int NT>
 __device__ inline void block_sum3(double& a, double& b, double& c) {

@@ -244,10 +244,13 @@ __global__ __launch_bounds__(NT) void plan_kernel(StepArgs a, const K* __restric
         const int64_t i = t0 + j * NT + threadIdx.x;
         if (i < L) aoff[i] = s_cnt[j * NT + threadIdx.x];
     }
+    // the six tile sums at once, in s_cnt once every offset has been read
+    static_assert(sizeof(s_cnt) >= 6 * NT * sizeof(double), "s_cnt holds the six sums");
+    __syncthreads();
+    block_sum_n<NT, 6>(st, reinterpret_cast<double*>(s_cnt));
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        const double s = block_sum<NT>(st[j]);
-        if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + tile] = s;
+        for (int j = 0; j < 6; ++j) partials[(int64_t)j * gridDim.x + tile] = st[j];
     }
 }
 
