@@ -88,9 +88,8 @@ __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); 
 // region; batch_step_kernel gathers the slices in order.
 template <int NT>
 struct SpawnLds {
-    int cnt[AIT * (NT / WAVE)];
-    int tot;
-    int64_t S;
+    int cnt[2][AIT * (NT / WAVE)];  // double-buffered by attempt group
+    int tot[2];
     uint64_t k[SPAWN_CH];        // the chunk's keys
     int64_t v[SPAWN_CH];         // and values
     int64_t aoff[SPAWN_CH + 1];  // attempt offsets within the chunk (counts first)
@@ -101,9 +100,8 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     constexpr int NW = NT / WAVE;
     constexpr int EPT = SPAWN_CH / NT;    // entries per thread per chunk
     constexpr int CH = SPAWN_CH;
-    int* s_cnt = sp.cnt;
-    int& s_tot = sp.tot;
-    int64_t& s_S = sp.S;
+    int64_t S_run = 0;  // records of the slice so far (the same in every thread)
+    int grp = 0;        // attempt groups so far: their parity picks the count buffers
 
     const int run = P.wg_run[gslice];
     const RunState& st = runs[run];
@@ -127,8 +125,6 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     uint64_t* sk0 = slice == 0 ? P.sk[0] + st.offS : P.xk + st.offX + (slice - 1) * capS;
     int64_t* sv0 = slice == 0 ? P.sv[0] + st.offS : P.xv + st.offX + (slice - 1) * capS;
 
-    if (tid == 0) s_S = 0;
-    __syncthreads();
 
     // ---- diagonal step and spawning, CH entries at a time -------------------------------
     // Entries are loaded striped (coalesced) and kept in LDS; attempts are numbered entry by
@@ -185,7 +181,9 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
         __syncthreads();
         // attempts in groups of NT * AIT: one ballot per attempt, then a scan of the
         // (attempt item, wavefront) counts gives every record its place in attempt order
-        for (int64_t a0 = 0; a0 < T; a0 += NT * AIT) {
+        for (int64_t a0 = 0; a0 < T; a0 += NT * AIT, ++grp) {
+            // counts of this group; the other buffer is still read by the group before
+            int* s_cnt = sp.cnt[grp & 1];
             bool ok[AIT];
             uint64_t nbk[AIT];
             int64_t nbv[AIT];
@@ -223,13 +221,16 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                 if (lane == 0) s_cnt[it * NW + wid] = __popcll(bal);
             }
             __syncthreads();
-            if (tid == 0) {
-                int64_t run = s_S;
-                for (int q = 0; q < AIT * NW; ++q) { const int c = s_cnt[q]; s_cnt[q] = (int)(run - s_S); run += c; }
-                s_tot = (int)(run - s_S);
+            // exclusive prefix of the (attempt item, wavefront) counts in one wavefront
+            static_assert(AIT * NW <= WAVE, "one wavefront scans the counts");
+            if (tid < WAVE) {
+                const int c = tid < AIT * NW ? s_cnt[tid] : 0;
+                const int inc = wave_inclusive_scan(c);
+                if (tid < AIT * NW) s_cnt[tid] = inc - c;
+                if (tid == AIT * NW - 1) sp.tot[grp & 1] = inc;
             }
             __syncthreads();
-            const int64_t base = s_S;
+            const int64_t base = S_run;
 #pragma unroll
             for (int it = 0; it < AIT; ++it) {
                 if (ok[it]) {
@@ -237,16 +238,15 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                     if (pos < capS) { sk0[pos] = nbk[it]; sv0[pos] = nbv[it]; }
                 }
             }
-            __syncthreads();
-            if (tid == 0) s_S += s_tot;
-            __syncthreads();
+            S_run += sp.tot[grp & 1];
         }
+        __syncthreads();  // the chunk's entries in LDS are read until here
     }
     __syncthreads();
     // record count (it may exceed capS: the gather then reports the overflow) and attempts
     const double att_sum = block_sum<NT>(st_att);
     if (tid == 0) {
-        P.xcnt[st.slice0 + slice] = s_S;
+        P.xcnt[st.slice0 + slice] = S_run;
         P.xatt[st.slice0 + slice] = att_sum;
     }
     (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie; (void)u;
