@@ -5,7 +5,8 @@
  * (apply_operator!, Interfaces/dictvectors.jl:112-154) together with the reductions that
  * advance! performs right after it (walkernumber_and_length, fciqmc.jl:144; the
  * ProjectedEnergy post-step, poststepstrategy.jl:110-121).  Vectors live in device memory
- * (HBM) between calls; the host keeps the step loop and the shift strategy.
+ * (HBM) between calls; the host keeps the step loop, and the shift strategy unless
+ * DoubleLogUpdate runs on the device (rimu_engine_set_shift_update).
  *
  * Conventions: every function returns RIMU_OK (0) or an error code; rimu_last_error()
  * gives a message for the calling thread.  Host pointers are borrowed for the duration of
