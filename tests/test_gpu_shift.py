@@ -143,3 +143,26 @@ def test_pmc_device_shift_matches_host_strategy(gpu):
     s0 = H.diagonal_element(H.starting_address())
     _check_trajectory(dev["shift"].to_numpy(), dev["norm"].to_numpy(), s0, 10.0, 1000, 0.08,
                       0.08 ** 2 / 4, 0.01)
+
+
+@pytest.mark.parametrize("case", ["wide", "initiator"])
+def test_pmc_device_shift_two_chunks_and_initiator(gpu, case):
+    """solve() on two-chunk addresses (HubbardReal1D 50x50, 99 bits) and with Initiator(1)
+    (HubbardMom1D): the report's shifts follow DoubleLogUpdate of its norms step by step."""
+    if case == "wide":
+        H = gpu.HubbardReal1D(gpu.near_uniform(50, 50), u=6.0)
+        kw = dict(style=gpu.IsDynamicSemistochastic(), initiator=False)
+    else:
+        H = gpu.HubbardMom1D(gpu.near_uniform(10, 12), u=2.0)
+        kw = dict(style=gpu.IsDynamicSemistochastic(), initiator=True)
+    strat = gpu.DoubleLogUpdate(target_walkers=2000, zeta=0.1)
+    p = gpu.ProjectorMonteCarloProblem(H, time_step=1e-3, last_step=400, random_seed=11,
+                                       shift_strategy=strat, **kw)
+    sim = gpu.solve(p)
+    assert sim.device_shift and sim.success
+    df = sim.df
+    sim.close()
+    s0 = H.diagonal_element(H.starting_address())   # one-address start: its Rayleigh quotient
+    _check_trajectory(df["shift"].to_numpy(), df["norm"].to_numpy(), s0, 10.0, 2000, 0.1,
+                      0.1 ** 2 / 4, 1e-3)
+    assert df["norm"].to_numpy()[-1] > 200
