@@ -136,11 +136,11 @@ def run_batched_leg(rimu, args):
     # autotune on a copy that has grown to steady state (snapshot/restore inside)
     tune = rimu.BatchedRuns(specs, history_steps=args.batched_tune_at)
     tune.step(args.batched_tune_at, steps_per_graph=100)
-    blk, rad, table = tune.autotune(trial_steps=20)
+    blk, rad, pers, table = tune.autotune(trial_steps=20)
     tune.close()
-    log(f"batched autotune: block {blk} radix {rad}")
+    log(f"batched autotune: block {blk} radix {rad} {'persistent launch' if pers else 'five launches'} per step")
     b = rimu.BatchedRuns(specs, history_steps=nsteps)
-    b.set_config(blk, rad)
+    b.set_config(blk, rad, persistent=pers)
     t0 = time.perf_counter()
     b.step(nsteps, steps_per_graph=100)
     elapsed = time.perf_counter() - t0
@@ -200,8 +200,10 @@ def run_batched_leg(rimu, args):
             "spawn_attempts_per_s": attempts / elapsed,
             "status_counts": {rimu.batched.STATUS[c]: int((st == c).sum()) for c in np.unique(st)},
             "ed_check": ed,
-            "autotune": {"block": blk, "radix_bits": rad, "tuned_after_steps": args.batched_tune_at,
-                         "ms_per_step": {f"{k[0]}x{k[1]}": v for k, v in table.items()}},
+            "autotune": {"block": blk, "radix_bits": rad, "persistent": pers,
+                         "tuned_after_steps": args.batched_tune_at,
+                         "ms_per_step": {f"{'persistent' if k[0] else 'five launches'} {k[1]}x{k[2]}": v
+                                         for k, v in table.items()}},
             "plain_estimate": (f"each run's steady-state step time on the plain engine "
                                f"({sample_steps} steps from the sweep's final vector) x {nsteps} steps, "
                                f"summed over runs: {plain_s:.1f} s"),

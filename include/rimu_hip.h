@@ -345,7 +345,9 @@ int rimu_dense_ground_state(rimu_dense* d, const double* v0, int64_t max_iters, 
  * Batched engine: many independent IsStochasticInteger FCIQMC runs of HubbardReal1D
  * advanced together, every run's spawning split over workgroups by its target walker
  * number; a step is five launches covering all runs (spawn slices, per-run sort, merge
- * slices, copy slices, per-run final phase), captured in a HIP graph.
+ * slices, copy slices, per-run final phase) or one persistent launch that runs the same
+ * phases from one work queue per XCD (rimu_batch_set_persistent).  Steps are captured in a
+ * HIP graph.
  * Each run is Rimu's lomc!/ProjectorMonteCarloProblem loop with DoubleLogUpdate
  * (shiftstrategy.jl:174-181) and a ProjectedEnergy onto its start address
  * (poststepstrategy.jl:82-121); the shift update happens on the device.
@@ -375,6 +377,16 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
 int rimu_batch_destroy(rimu_batch* b);
 /* Launch configuration: workgroup size (256, 512 or 1024) and radix digit width (4..8). */
 int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits);
+/* 1: each step is one persistent launch (every run on one XCD; workgroups of the
+ * configured size take the items of their XCD's runs from a queue, an item waiting for its
+ * run's previous phase); 0: five launches per step.  Results are the same either way. */
+int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent);
+/* Bound of the persistent step's waits (rounds of one s_sleep; default 2^25) and a
+ * host-forced test of its give-up path: with withhold_first, the first run of the first
+ * queue never reports its spawn slices, so its sort's wait runs into the bound, every
+ * remaining item is skipped and rimu_batch_step returns RIMU_ERR_INTERNAL.  Also clears the
+ * persistent step's counters. */
+int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_first);
 /* Advance every live run by nsteps; steps_per_graph launches are captured once in a HIP
  * graph and replayed.  A run whose population dies or whose vector or spawn buffer would
  * overflow stops there (its status says why); the others continue. */
@@ -385,13 +397,14 @@ int rimu_batch_history(rimu_batch* b, int64_t* nsteps, double* shift, double* no
 /* Status of every run: 0 running, 1 population dead, 2 vector overflow (max_length),
  * 3 spawn buffer overflow; and the current length of each vector. */
 int rimu_batch_status(rimu_batch* b, int32_t* status, int64_t* len);
-/* Time `trial_steps` steps for every (block size, radix width) candidate from a snapshot of
- * the current state (restored after each trial; results do not depend on the choice),
- * select the fastest, which the batch keeps.  ms receives the per-step time of each
- * candidate in the order block in {256, 512, 1024} x radix in {4, 6, 8}; best_block/best_radix
- * the choice. */
+/* Time `trial_steps` steps for every (launch kind, block size, radix width) candidate from a
+ * snapshot of the current state (restored after each trial; results do not depend on the
+ * choice), select the fastest, which the batch keeps.  ms receives the per-step time of each
+ * of the 18 candidates in the order persistent in {0, 1} x block in {256, 512, 1024} x radix
+ * in {4, 6, 8} (a persistent trial whose wait gave up is given HUGE_VAL and its counters
+ * are cleared); best_block/best_radix/best_persistent the choice. */
 int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
-                        int32_t* best_radix, double* ms);
+                        int32_t* best_radix, int32_t* best_persistent, double* ms);
 /* The vector of one run (ascending keys, int64 values). */
 int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals, int64_t cap,
                       int64_t* n);

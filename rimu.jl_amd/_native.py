@@ -35,7 +35,8 @@ EXPORTS = (
     "rimu_basis_size", "rimu_dense_create", "rimu_dense_destroy", "rimu_dense_dim",
     "rimu_dense_basis", "rimu_dense_apply", "rimu_dense_pullback", "rimu_dense_projector",
     "rimu_dense_ground_state",
-    "rimu_batch_create", "rimu_batch_destroy", "rimu_batch_set_config", "rimu_batch_step",
+    "rimu_batch_create", "rimu_batch_destroy", "rimu_batch_set_config", "rimu_batch_set_persistent",
+    "rimu_batch_set_wait", "rimu_batch_step",
     "rimu_batch_history", "rimu_batch_status", "rimu_batch_autotune", "rimu_batch_vector",
 )
 
@@ -132,10 +133,12 @@ def lib():
         L.rimu_batch_create.argtypes = [P(BatchRun), i32, i64, P(vp)]
         L.rimu_batch_destroy.argtypes = [vp]
         L.rimu_batch_set_config.argtypes = [vp, i32, i32]
+        L.rimu_batch_set_persistent.argtypes = [vp, i32]
+        L.rimu_batch_set_wait.argtypes = [vp, ctypes.c_uint32, i32]
         L.rimu_batch_step.argtypes = [vp, i64, i32]
         L.rimu_batch_history.argtypes = [vp, P(i64), vp, vp, vp, vp, vp, vp]
         L.rimu_batch_status.argtypes = [vp, vp, vp]
-        L.rimu_batch_autotune.argtypes = [vp, i64, P(i32), P(i32), vp]
+        L.rimu_batch_autotune.argtypes = [vp, i64, P(i32), P(i32), P(i32), vp]
         L.rimu_batch_vector.argtypes = [vp, i32, vp, vp, i64, P(i64)]
         for name in EXPORTS:
             if name != "rimu_last_error":

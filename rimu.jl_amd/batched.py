@@ -1,6 +1,7 @@
 """Many independent FCIQMC runs advanced together (C ABI ``rimu_batch_*``): each run's
 spawning is split over workgroups by its target walker number; a step is five launches
-covering all runs, and steps are captured in a HIP graph.
+covering all runs or one persistent launch (``set_persistent``), and steps are captured in
+a HIP graph.
 
 Each run is ``lomc!``/``ProjectorMonteCarloProblem`` with ``IsStochasticInteger``
 walkers on a ``HubbardReal1D`` (``src/lomc.jl:185-195``): start vector
@@ -75,22 +76,33 @@ class BatchedRuns:
         except Exception:
             pass
 
-    def set_config(self, block_size: int, radix_bits: int):
+    def set_config(self, block_size: int, radix_bits: int, persistent: bool | None = None):
         _native.check(_native.lib().rimu_batch_set_config(self._h, block_size, radix_bits))
+        if persistent is not None:
+            self.set_persistent(persistent)
+
+    def set_persistent(self, persistent: bool):
+        """One persistent launch per step (True) or five launches (False); same results."""
+        _native.check(_native.lib().rimu_batch_set_persistent(self._h, int(bool(persistent))))
+
+    def set_wait(self, spin_limit: int, withhold_first: bool = False):
+        """Bound of the persistent step's waits and the host-forced give-up test
+        (``rimu_batch_set_wait``)."""
+        _native.check(_native.lib().rimu_batch_set_wait(self._h, int(spin_limit), int(bool(withhold_first))))
 
     def step(self, nsteps: int, steps_per_graph: int = 100):
         _native.check(_native.lib().rimu_batch_step(self._h, int(nsteps), int(steps_per_graph)))
 
     def autotune(self, trial_steps: int = 20):
-        """Time every (block, radix) candidate; the batch keeps the fastest.
-        Returns (block, radix, {(block, radix): ms per step})."""
-        ms = np.zeros(9)
-        bb, br = ctypes.c_int32(0), ctypes.c_int32(0)
+        """Time every (persistent, block, radix) candidate; the batch keeps the fastest.
+        Returns (block, radix, persistent, {(persistent, block, radix): ms per step})."""
+        ms = np.zeros(18)
+        bb, br, bp = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
         _native.check(_native.lib().rimu_batch_autotune(self._h, int(trial_steps), ctypes.byref(bb),
-                                                        ctypes.byref(br), _p(ms)))
-        table = {(blk, rad): ms[i * 3 + j] for i, blk in enumerate((256, 512, 1024))
-                 for j, rad in enumerate((4, 6, 8))}
-        return int(bb.value), int(br.value), table
+                                                        ctypes.byref(br), ctypes.byref(bp), _p(ms)))
+        table = {(bool(p), blk, rad): ms[(p * 3 + i) * 3 + j] for p in (0, 1)
+                 for i, blk in enumerate((256, 512, 1024)) for j, rad in enumerate((4, 6, 8))}
+        return int(bb.value), int(br.value), bool(bp.value), table
 
     def history(self):
         n = ctypes.c_int64(0)

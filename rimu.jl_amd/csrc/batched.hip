@@ -696,6 +696,151 @@ __global__ __launch_bounds__(NT) void batch_final_kernel(RunState* __restrict__ 
     final_phase<NT>(runs, nruns, P, blockIdx.x, threadIdx.x);
 }
 
+// ---------------------------------------------------------------------------------------
+// The whole step in one persistent launch, each run on one XCD.  The runs are dealt to the
+// XCDs (largest first, to the least loaded); a workgroup reads the id of the XCD it runs on
+// and takes work items from that XCD's queue, in the order: the spawn slices of its runs,
+// their sorts, merge slices, copy slices and final phases.  An item waits until the items
+// of its run's phase before have finished (a sort for the K spawn slices, a merge for the
+// sort, a copy for the K merge slices, the final phase for the K copy slices).  Those items
+// hold lower tickets of the same queue, so they were taken by running workgroups of the
+// same XCD that wait only on lower tickets themselves: every wait ends, whatever the grid
+// size.  All hand-overs of a run stay inside one XCD, whose L2 every CU of it shares: a
+// producing workgroup waits for its stores to reach the L2 before it signals, a consuming
+// one invalidates its CU's L1 before it reads.  Every wait is bounded (spin_limit rounds of
+// s_sleep); a wait that gives up raises the abort word, after which every item is skipped
+// and the host reports the error.  The last workgroup to run out of work clears the queue
+// counters for the next launch; a run's final phase clears its phase counters.
+// ---------------------------------------------------------------------------------------
+constexpr int MAX_XCD = 16;
+struct StepSync {
+    uint32_t* words;         // [0, MAX_XCD) queue tickets, [MAX_XCD] workgroups done,
+                             // [MAX_XCD + 1] abort; from [MAX_XCD + 4], 4 per run: spawn,
+                             // sort, merge and copy items done
+    const int* items;        // per queue, in ticket order: phase << 24 | slice or run
+    const int* queue_start;  // queue q holds items[queue_start[q], queue_start[q + 1])
+    int nq;                  // queues (XCDs)
+    uint32_t spin_limit;
+    int withhold;            // test of the give-up path: the first run of queue 0 never
+                             // reports its spawn slices
+};
+
+template <int NT>
+union StepLds {
+    SpawnLds<NT> sp;
+    SortLds<NT> so;
+    MergeLds<NT> me;
+};
+
+__device__ inline uint32_t sync_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void sync_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t sync_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// id of the XCD this wavefront runs on
+__device__ inline int xcd_id() {
+    int x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x & 15;
+}
+// The thread index, read afresh: values derived from it stay inside the phase that
+// computes them instead of being hoisted out of the persistent loop and held in registers
+// across all phases.
+__device__ inline int thread_index() {
+    int t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+    return t;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict__ runs, int nruns, Pools P,
+                                                           int radix_bits, StepSync C) {
+    __shared__ StepLds<NT> sm;
+    __shared__ uint32_t s_ticket;
+    __shared__ int s_go;
+    uint32_t* const done = C.words + MAX_XCD;
+    uint32_t* const abort_word = C.words + MAX_XCD + 1;
+    uint32_t* const rs = C.words + MAX_XCD + 4;
+    const int q = __builtin_amdgcn_readfirstlane(xcd_id() % C.nq);
+    const int q0 = C.queue_start[q], qn = C.queue_start[q + 1] - q0;
+    while (true) {
+        if (threadIdx.x == 0) s_ticket = sync_add(C.words + q, 1u);
+        __syncthreads();
+        const int t = (int)__builtin_amdgcn_readfirstlane(s_ticket);
+        if (t >= qn) {
+            if (threadIdx.x == 0 && sync_add(done, 1u) == gridDim.x - 1) {
+                // every workgroup has drawn past its queue: the counters start the next launch at zero
+                for (int j = 0; j < C.nq; ++j) sync_store(C.words + j, 0u);
+                sync_store(done, 0u);
+            }
+            return;
+        }
+        const int code = C.items[q0 + t];
+        const int phase = code >> 24, idx = code & 0xffffff;
+        const int run = (phase == 1 || phase == 4) ? idx : P.wg_run[idx];
+        uint32_t* const word = rs + 4 * run;
+        if (threadIdx.x == 0) {
+            // the run's status is that of the last launch until its final phase, which runs
+            // after every other item of the run
+            int go = runs[run].status == RUN_OK && sync_load(abort_word) == 0 ? 1 : 0;
+            if (go && phase > 0) {
+                const uint32_t need = phase == 2 ? 1u : (uint32_t)runs[run].K;
+                const uint32_t* w = word + (phase - 1);
+                uint32_t spins = 0;
+                while (sync_load(w) < need) {
+                    if (sync_load(abort_word) != 0) { go = 0; break; }
+                    if (++spins > C.spin_limit) {
+                        __hip_atomic_fetch_or(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        go = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
+        __syncthreads();  // s_ticket and s_go are written again by the next round
+        if (!go) continue;
+        // the earlier phase's writes are in this XCD's L2: drop this CU's L1 copies (every
+        // wavefront, so that its own loads are ordered after the invalidation)
+        if (phase > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (phase == 0) spawn_phase<NT>(runs, P, idx, sm.sp, thread_index());
+        else if (phase == 1) sort_phase<NT>(runs, P, run, radix_bits, sm.so, thread_index());
+        else if (phase == 2) merge_phase<NT>(runs, P, idx, sm.me, thread_index());
+        else if (phase == 3) copy_phase<NT>(runs, P, idx, thread_index());
+        else final_phase<NT>(runs, nruns, P, run, thread_index());
+        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's stores have reached the L2
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (phase == 0) {
+                if (!(C.withhold && q == 0 && run == P.wg_run[C.items[C.queue_start[0]] & 0xffffff])) sync_add(word, 1u);
+            } else if (phase == 1) {
+                sync_store(word + 1, 1u);
+            } else if (phase == 2) {
+                sync_add(word + 2, 1u);
+            } else if (phase == 3) {
+                sync_add(word + 3, 1u);
+            } else {  // every item of the run has finished: its counters start the next step at zero
+                sync_store(word, 0u);
+                sync_store(word + 1, 0u);
+                sync_store(word + 2, 0u);
+                sync_store(word + 3, 0u);
+            }
+        }
+    }
+}
+
+// Largest XCD id a workgroup reports (the host takes that + 1 as the number of queues).
+__global__ void xcd_count_kernel(int* out) {
+    if (threadIdx.x == 0) atomicMax(out, xcd_id());
+}
+
 // Projector of every run: the start address (coefficient diag) and its neighbours.
 __global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, const uint64_t* __restrict__ start,
                                        uint64_t* __restrict__ pk, double* __restrict__ pc) {
@@ -726,6 +871,13 @@ __global__ void batch_projector_kernel(RunState* __restrict__ runs, int nruns, c
 struct rimu_batch {
     int nruns = 0;
     int block = 512, radix = 8;
+    int persistent = 0;             // 1: the step is one launch of batch_step_kernel
+    uint32_t spin_limit = LB_SPIN_LIMIT;
+    int withhold = 0;               // test of the give-up path (rimu_batch_set_wait)
+    int n_cu = 0, nq = 1;           // compute units, XCDs (work queues of the persistent step)
+    int occupancy[3] = {1, 1, 1};   // resident batch_step_kernel workgroups per CU, by block size
+    DevBuf<uint32_t> sync;          // StepSync words
+    DevBuf<int> items, queue_start; // the persistent step's work queues
     hipStream_t stream = nullptr;
     std::vector<RunState> host_init;
     DevBuf<RunState> runs;
@@ -740,7 +892,8 @@ struct rimu_batch {
     DevBuf<uint64_t> mprev, mk;
     DevBuf<double> mnrm;
     int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, hist_cap = 0;
-    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix))
+    std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix)); cleared
+                                                                           // when the launch kind changes
     Pools pools() {
         Pools P{};
         for (int b = 0; b < 2; ++b) {
@@ -771,9 +924,28 @@ struct rimu_batch {
 
 namespace {
 
+int sync_words(const rimu_batch* b) { return MAX_XCD + 4 + 4 * b->nruns; }
+
+template <int NT>
+void launch_persistent(rimu_batch* b, int occ) {
+    const int64_t total = 3 * b->totK + 2 * (int64_t)b->nruns;
+    const int64_t grid = std::max<int64_t>(b->nq, std::min<int64_t>(total, (int64_t)b->n_cu * occ));
+    const StepSync C{b->sync.p, b->items.p, b->queue_start.p, b->nq, b->spin_limit, b->withhold};
+    batch_step_kernel<NT><<<grid, NT, 0, b->stream>>>(b->runs.p, b->nruns, b->pools(), b->radix, C);
+}
+
 int launch_step(rimu_batch* b) {
     const Pools P = b->pools();
     const int K = (int)b->totK;
+    if (b->persistent) {
+        switch (b->block) {
+            case 256: launch_persistent<256>(b, b->occupancy[0]); break;
+            case 1024: launch_persistent<1024>(b, b->occupancy[2]); break;
+            default: launch_persistent<512>(b, b->occupancy[1]); break;
+        }
+        HIP_TRY(hipGetLastError());
+        return RIMU_OK;
+    }
     batch_spawn_kernel<SPAWN_NT><<<K, SPAWN_NT, 0, b->stream>>>(b->runs.p, P);
     switch (b->block) {
         case 256:
@@ -930,6 +1102,50 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
     if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
     if ((e = hipMemcpyAsync(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    // the persistent step: one work queue per XCD, the runs dealt to the least loaded queue
+    // (by spawn slices), largest first
+    std::vector<int> items, qstart;
+    {
+        int dev = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return bad("device", e);
+        if ((e = hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return bad("device", e);
+        DevBuf<int> xmax;
+        if ((e = xmax.alloc(1)) != hipSuccess) return bad("xcd count", e);
+        int zero = 0, xm = 0;
+        if ((e = hipMemcpyAsync(xmax.p, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("xcd count", e);
+        xcd_count_kernel<<<std::max(1, b->n_cu), 64, 0, b->stream>>>(xmax.p);
+        if ((e = hipMemcpyAsync(&xm, xmax.p, sizeof(int), hipMemcpyDeviceToHost, b->stream)) != hipSuccess) return bad("xcd count", e);
+        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bad("xcd count", e);
+        xmax.release();
+        b->nq = std::min(MAX_XCD, xm + 1);
+        std::vector<std::vector<int>> qruns(b->nq);
+        std::vector<int64_t> load(b->nq, 0);
+        for (int r : order) {
+            const int qq = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            qruns[qq].push_back(r);
+            load[qq] += st[r].K;
+        }
+        for (int qq = 0; qq < b->nq; ++qq) {
+            qstart.push_back((int)items.size());
+            for (int ph = 0; ph < 5; ++ph)
+                for (int r : qruns[qq]) {
+                    if (ph == 1 || ph == 4) items.push_back(ph << 24 | r);
+                    else for (int64_t k = 0; k < st[r].K; ++k) items.push_back(ph << 24 | (int)(st[r].slice0 + k));
+                }
+        }
+        qstart.push_back((int)items.size());
+        if ((e = b->items.alloc(items.size())) != hipSuccess) return bad("work queues", e);
+        if ((e = b->queue_start.alloc(qstart.size())) != hipSuccess) return bad("work queues", e);
+        if ((e = hipMemcpyAsync(b->items.p, items.data(), items.size() * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+        if ((e = hipMemcpyAsync(b->queue_start.p, qstart.data(), qstart.size() * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+        if ((e = b->sync.alloc(sync_words(b))) != hipSuccess) return bad("step sync", e);
+        if ((e = hipMemsetAsync(b->sync.p, 0, sync_words(b) * sizeof(uint32_t), b->stream)) != hipSuccess) return bad("step sync", e);
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[0], batch_step_kernel<256>, 256, 0)) != hipSuccess ||
+            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[1], batch_step_kernel<512>, 512, 0)) != hipSuccess ||
+            (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occupancy[2], batch_step_kernel<1024>, 1024, 0)) != hipSuccess)
+            return bad("occupancy", e);
+        for (int& o : b->occupancy) o = std::max(o, 1);
+    }
     if ((e = hipMemcpyAsync(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpyAsync(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     std::vector<int64_t> pops(nruns);  // host buffers live until the stream is synchronized below
@@ -954,6 +1170,7 @@ int rimu_batch_destroy(rimu_batch* b) {
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
     b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
     b->msplit.release(); b->mprev.release(); b->mcnt.release(); b->mnrm.release(); b->mk.release(); b->mv.release();
+    b->sync.release(); b->items.release(); b->queue_start.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return RIMU_OK;
@@ -969,11 +1186,33 @@ int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits)
     return RIMU_OK;
 }
 
+int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (persistent != 0 && persistent != 1) return fail(RIMU_ERR_ARG, "persistent must be 0 or 1");
+    if (persistent != b->persistent) clear_graphs(b);
+    b->persistent = persistent;
+    return RIMU_OK;
+}
+
+int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_first) {
+    if (!b) return fail(RIMU_ERR_ARG, "batch is null");
+    if (spin_limit < 1) return fail(RIMU_ERR_ARG, "spin_limit must be positive");
+    HIP_TRY(hipMemsetAsync(b->sync.p, 0, sync_words(b) * sizeof(uint32_t), b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    clear_graphs(b);  // the bound and the flag are arguments of the captured launches
+    b->spin_limit = spin_limit;
+    b->withhold = withhold_first != 0;
+    return RIMU_OK;
+}
+
 int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (nsteps < 0 || steps_per_graph < 1) return fail(RIMU_ERR_ARG, "need nsteps >= 0 and steps_per_graph >= 1");
     if (int rc = run_steps(b, nsteps, steps_per_graph)) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
+    uint32_t abort_word = 0;
+    HIP_TRY(hipMemcpy(&abort_word, b->sync.p + MAX_XCD + 1, sizeof(abort_word), hipMemcpyDeviceToHost));
+    if (abort_word) return fail(RIMU_ERR_INTERNAL, "a wait of the persistent batched step gave up");
     return RIMU_OK;
 }
 
@@ -1016,7 +1255,8 @@ int rimu_batch_vector(rimu_batch* b, int32_t run, uint64_t* keys, int64_t* vals,
     return RIMU_OK;
 }
 
-int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix, double* ms) {
+int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block, int32_t* best_radix,
+                        int32_t* best_persistent, double* ms) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (trial_steps < 1) return fail(RIMU_ERR_ARG, "trial_steps must be >= 1");
     // snapshot: run states and both vector buffers (the step reads one, writes the other)
@@ -1046,13 +1286,16 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         const int blocks[3] = {256, 512, 1024};
         const int radixes[3] = {4, 6, 8};
         double best = 1e300;
-        int bb = b->block, br = b->radix;
+        const int p0 = b->persistent;
+        int bb = b->block, br = b->radix, bp = b->persistent;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int idx = 0;
+        for (int pi = 0; pi < 2; ++pi)
         for (int bi = 0; bi < 3; ++bi)
             for (int ri = 0; ri < 3; ++ri, ++idx) {
+                b->persistent = pi;
                 b->block = blocks[bi];
                 b->radix = radixes[ri];
                 if (int rc = launch_step(b)) return rc;  // first launch outside the timing
@@ -1065,18 +1308,30 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
                 HIP_TRY(hipEventSynchronize(e1));
                 float t = 0;
                 HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-                const double per = t / (double)trial_steps;
-                if (ms) ms[idx] = per;
-                if (per < best) { best = per; bb = b->block; br = b->radix; }
+                double per = t / (double)trial_steps;
                 if (int rc = restore()) return rc;
+                if (pi) {  // a persistent trial whose wait gave up is not a candidate
+                    uint32_t abort_word = 0;
+                    HIP_TRY(hipStreamSynchronize(b->stream));
+                    HIP_TRY(hipMemcpy(&abort_word, b->sync.p + MAX_XCD + 1, sizeof(abort_word), hipMemcpyDeviceToHost));
+                    if (abort_word) {
+                        per = HUGE_VAL;
+                        HIP_TRY(hipMemsetAsync(b->sync.p, 0, sync_words(b) * sizeof(uint32_t), b->stream));
+                    }
+                }
+                if (ms) ms[idx] = per;
+                if (per < best) { best = per; bb = b->block; br = b->radix; bp = b->persistent; }
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIP_TRY(hipStreamSynchronize(b->stream));  // the last restore
+        if (bp != p0) clear_graphs(b);
         b->block = bb;
         b->radix = br;
+        b->persistent = bp;
         if (best_block) *best_block = bb;
         if (best_radix) *best_radix = br;
+        if (best_persistent) *best_persistent = bp;
         return RIMU_OK;
     };
     const int rc = body();

@@ -139,9 +139,9 @@ def test_autotune_keeps_state(gpu):
     sp = specs(gpu, CASES, target=2000)
     a = gpu.BatchedRuns(sp, history_steps=300)
     a.step(150, steps_per_graph=50)
-    blk, rad, table = a.autotune(trial_steps=5)
-    assert blk in (256, 512, 1024) and rad in (4, 6, 8)
-    assert len(table) == 9 and all(t > 0 for t in table.values())
+    blk, rad, pers, table = a.autotune(trial_steps=5)
+    assert blk in (256, 512, 1024) and rad in (4, 6, 8) and pers in (False, True)
+    assert len(table) == 18 and all(t > 0 for t in table.values())
     a.step(150, steps_per_graph=50)
     b = gpu.BatchedRuns(sp, history_steps=300)
     b.step(300, steps_per_graph=50)
