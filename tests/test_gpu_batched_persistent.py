@@ -39,7 +39,7 @@ def _run(rimu, sp, nsteps, persistent, block, radix, spg):
 
 
 @pytest.mark.parametrize("block,radix,spg", [(512, 6, 50), (1024, 8, 1), (256, 4, 25)])
-def test_persistent_step_equals_four_launches(gpu, block, radix, spg):
+def test_persistent_step_equals_five_launches(gpu, block, radix, spg):
     sp = _mixed(gpu)
     nsteps = 300
     print(f"persistent step: block {block}, radix {radix}, {spg} steps per graph", flush=True)
