@@ -801,15 +801,18 @@ __global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict_
                     __builtin_amdgcn_s_sleep(2);
                 }
             }
+            if (go && phase > 0) {
+                // the earlier phase's writes: invalidate this CU's L1 (and this XCD's L2) once,
+                // and wait for the invalidation before the barrier releases the other waves
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __builtin_amdgcn_s_waitcnt(0);
+            }
             s_go = go;
         }
         __syncthreads();
         const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
         __syncthreads();  // s_ticket and s_go are written again by the next round
         if (!go) continue;
-        // the earlier phase's writes are in this XCD's L2: drop this CU's L1 copies (every
-        // wavefront, so that its own loads are ordered after the invalidation)
-        if (phase > 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (phase == 0) spawn_phase<NT>(runs, P, idx, sm.sp, thread_index());
         else if (phase == 1) sort_phase<NT>(runs, P, run, radix_bits, sm.so, thread_index());
         else if (phase == 2) merge_phase<NT>(runs, P, idx, sm.me, thread_index());
