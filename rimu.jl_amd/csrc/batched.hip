@@ -878,6 +878,7 @@ struct rimu_batch {
     uint32_t spin_limit = LB_SPIN_LIMIT;
     int withhold = 0;               // test of the give-up path (rimu_batch_set_wait)
     int n_cu = 0, nq = 1;           // compute units, XCDs (work queues of the persistent step)
+    bool persistent_ok = true;      // every XCD has its own queue
     int occupancy[3] = {1, 1, 1};   // resident batch_step_kernel workgroups per CU, by block size
     DevBuf<uint32_t> sync;          // StepSync words
     DevBuf<int> items, queue_start; // the persistent step's work queues
@@ -1120,6 +1121,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         if ((e = hipMemcpyAsync(&xm, xmax.p, sizeof(int), hipMemcpyDeviceToHost, b->stream)) != hipSuccess) return bad("xcd count", e);
         if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bad("xcd count", e);
         xmax.release();
+        b->persistent_ok = xm + 1 <= MAX_XCD;  // one queue per XCD, or no persistent step
         b->nq = std::min(MAX_XCD, xm + 1);
         std::vector<std::vector<int>> qruns(b->nq);
         std::vector<int64_t> load(b->nq, 0);
@@ -1192,6 +1194,7 @@ int rimu_batch_set_config(rimu_batch* b, int32_t block_size, int32_t radix_bits)
 int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (persistent != 0 && persistent != 1) return fail(RIMU_ERR_ARG, "persistent must be 0 or 1");
+    if (persistent && !b->persistent_ok) return fail(RIMU_ERR_ARG, "more XCDs than the persistent step has queues");
     if (persistent != b->persistent) clear_graphs(b);
     b->persistent = persistent;
     return RIMU_OK;
@@ -1295,7 +1298,9 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int idx = 0;
-        for (int pi = 0; pi < 2; ++pi)
+        if (ms)
+            for (int i = 0; i < 18; ++i) ms[i] = HUGE_VAL;  // candidates that are not timed
+        for (int pi = 0; pi < (b->persistent_ok ? 2 : 1); ++pi)
         for (int bi = 0; bi < 3; ++bi)
             for (int ri = 0; ri < 3; ++ri, ++idx) {
                 b->persistent = pi;
