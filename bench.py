@@ -202,8 +202,8 @@ def run_batched_leg(rimu, args):
             "ed_check": ed,
             "autotune": {"block": blk, "radix_bits": rad, "persistent": pers,
                          "tuned_after_steps": args.batched_tune_at,
-                         "ms_per_step": {f"{'persistent' if k[0] else 'five launches'} {k[1]}x{k[2]}": v
-                                         for k, v in table.items()}},
+                         "ms_per_step": {f"{'persistent' if k[0] else 'five launches'} {k[1]}x{k[2]}":
+                                         (float(v) if np.isfinite(v) else None) for k, v in table.items()}},
             "plain_estimate": (f"each run's steady-state step time on the plain engine "
                                f"({sample_steps} steps from the sweep's final vector) x {nsteps} steps, "
                                f"summed over runs: {plain_s:.1f} s"),
