@@ -124,6 +124,19 @@ def run_cpu_baseline(rimu_keys, rimu_vals, args, step_no, shift):
             "seconds": dt}
 
 
+def _finite(x):
+    """The result with every non-finite float replaced by None (strict JSON)."""
+    if isinstance(x, dict):
+        return {k: _finite(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_finite(v) for v in x]
+    if isinstance(x, (float, np.floating)):
+        return float(x) if math.isfinite(float(x)) else None
+    if isinstance(x, np.integer):
+        return int(x)
+    return x
+
+
 def run_batched_leg(rimu, args):
     """The batched U-sweep: 256 HubbardReal1D runs (M ~ U{6..14}, N ~ U{ceil(M/2)..M},
     u ~ U[1,10], target walkers log-uniform in [1e3, 1e5], IsStochasticInteger, dtau 1e-3,
@@ -1056,7 +1069,7 @@ def main():
         "wide": wide,
         "realspace": realspace,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_finite(out), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":
