@@ -361,9 +361,11 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
         __syncthreads();
         for (int64_t i = tid; i < S; i += NT) atomicAdd(&s_hist[(srck[i] >> sh) & dmask], 1u);
         __syncthreads();
-        if (tid == 0) {  // exclusive digit offsets (at most 256 bins)
-            uint32_t run_ = 0;
-            for (int d = 0; d < nbins; ++d) { const uint32_t c = s_hist[d]; s_hist[d] = run_; run_ += c; }
+        {  // exclusive digit offsets (at most 256 bins <= NT threads)
+            const uint32_t c = tid < nbins ? s_hist[tid] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan<uint32_t, NT>(c, &tot);  // ends with a barrier
+            if (tid < nbins) s_hist[tid] = ex;
         }
         __syncthreads();
         for (int64_t t0 = 0; t0 < S; t0 += NT) {
