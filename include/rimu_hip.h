@@ -367,7 +367,7 @@ typedef struct rimu_batch_run {
     double zeta;                /* DoubleLogUpdate damping (0.08) */
     double xi;                  /* DoubleLogUpdate restoring force (zeta^2/4) */
     double dtau;
-    int64_t max_length;         /* vector capacity; exceeding it aborts the run */
+    int64_t max_length;         /* a vector longer than this stops the run */
 } rimu_batch_run;
 
 /* Per-step history of every run, row-major [steps][nruns]: shift, norm (walkers), len,
@@ -388,8 +388,11 @@ int rimu_batch_set_persistent(rimu_batch* b, int32_t persistent);
  * persistent step's counters. */
 int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_first);
 /* Advance every live run by nsteps; steps_per_graph launches are captured once in a HIP
- * graph and replayed.  A run whose population dies or whose vector or spawn buffer would
- * overflow stops there (its status says why); the others continue. */
+ * graph and replayed.  A run stops (its status says why) and the others continue when: its
+ * population dies; its vector outgrows max_length (as fciqmc.jl:155-179, that step is
+ * completed, recorded and kept, so the run's vector is longer than max_length); or a step's
+ * spawns overflow the run's spawn buffer (4 target_walkers + 4096 records: that step is
+ * void and leaves no history row). */
 int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph);
 /* Steps recorded so far and the history arrays (each [steps][nruns], may be null). */
 int rimu_batch_history(rimu_batch* b, int64_t* nsteps, double* shift, double* norm, double* len,

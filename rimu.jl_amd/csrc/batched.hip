@@ -261,17 +261,23 @@ __global__ __launch_bounds__(NT) void batch_spawn_kernel(RunState* __restrict__ 
 // Gather of the slices' spawn records, their radix sort, and the boundaries of every merge
 // tile (one workgroup per run).  Hands S, the sorted buffer and the tile count to the next
 // kernels through the run state.
-template <int NT>
+constexpr int SORT_LDS_CAP = 4096;  // records a run's sort keeps in LDS for all its passes
+
+template <int NT, bool LS>
 struct SortLds {
     uint32_t hist[256];
     uint32_t wcnt[NT / WAVE][256];
     int64_t xoff[MAX_SLICES + 1];  // first record of each slice in the gathered list
     double att;
     int flag;
+    uint64_t lk[LS ? 2 : 1][LS ? SORT_LDS_CAP : 1];  // LS: both buffers of the passes
+    int64_t lv[LS ? 2 : 1][LS ? SORT_LDS_CAP : 1];
 };
 
-template <int NT>
-__device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run, int radix_bits, SortLds<NT>& so,
+// LS: a run of at most SORT_LDS_CAP records is gathered into LDS, sorted there and written
+// once to its spawn buffer; larger runs pass through their global buffers.
+template <int NT, bool LS>
+__device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run, int radix_bits, SortLds<NT, LS>& so,
                            int tid) {
     constexpr int NW = NT / WAVE;
     constexpr int TILE = NT * MIPT;
@@ -312,6 +318,16 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
     __syncthreads();
     int64_t S = so.xoff[K];
     const double att_sum = so.att;
+    // LS: a run of at most SORT_LDS_CAP records is gathered into LDS and sorted there
+    const bool in_lds = LS && S <= SORT_LDS_CAP && S <= capS;  // uniform
+    uint64_t* gk = in_lds ? so.lk[0] : sk0;
+    int64_t* gv = in_lds ? so.lv[0] : sv0;
+    if (in_lds) {  // slice 0's records, written in place by its spawn workgroup
+        for (int64_t i = tid; i < min(so.xoff[1], S); i += NT) {
+            gk[i] = sk0[i];
+            gv[i] = sv0[i];
+        }
+    }
     if (S > capS) {
         if (tid == 0) s_flag = RUN_SPAWN_OVERFLOW;
         S = capS;
@@ -340,8 +356,8 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
             for (int c = 0; c < CU; ++c) {
                 const int64_t j = j0 + c * NT + tid;
                 if (j < S) {
-                    sk0[j] = kq[c];
-                    sv0[j] = vq[c];
+                    gk[j] = kq[c];
+                    gv[j] = vq[c];
                 }
             }
         }
@@ -350,10 +366,10 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
 
     // ---- stable LSD radix sort of the spawn records on bits [0, B) ------------------------
     const int nbins = 1 << radix_bits;
-    uint64_t* srck = sk0;
-    int64_t* srcv = sv0;
-    uint64_t* dstk = sk1;
-    int64_t* dstv = sv1;
+    uint64_t* srck = gk;
+    int64_t* srcv = gv;
+    uint64_t* dstk = in_lds ? so.lk[LS ? 1 : 0] : sk1;
+    int64_t* dstv = in_lds ? so.lv[LS ? 1 : 0] : sv1;
     for (int sh = 0; sh < B; sh += radix_bits) {
         const uint64_t dmask = (uint64_t)(nbins - 1);
         for (int d = tid; d < nbins; d += NT) s_hist[d] = 0;
@@ -399,6 +415,12 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
         uint64_t* tk = srck; srck = dstk; dstk = tk;
         int64_t* tv = srcv; srcv = dstv; dstv = tv;
     }
+    if (in_lds) {  // the sorted records to the run's spawn buffer, where the merge reads them
+        for (int64_t i = tid; i < S; i += NT) {
+            sk0[i] = srck[i];
+            sv0[i] = srcv[i];
+        }
+    }
 
     // ---- merge tile boundaries: spawns before each tile start and the key before it -------
     const uint64_t* ak = srck;
@@ -418,7 +440,7 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
     if (tid == 0) {
         st.S_cur = S;
         st.ntl = ntl;
-        st.sbuf = srck == sk0 ? 0 : 1;
+        st.sbuf = (in_lds || srck == sk0) ? 0 : 1;
         st.flag = s_flag;
         st.att = att_sum;
     }
@@ -426,8 +448,8 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
 
 template <int NT>
 __global__ __launch_bounds__(NT) void batch_sort_kernel(RunState* __restrict__ runs, Pools P, int radix_bits) {
-    __shared__ SortLds<NT> so;
-    sort_phase<NT>(runs, P, blockIdx.x, radix_bits, so, threadIdx.x);
+    __shared__ SortLds<NT, true> so;
+    sort_phase<NT, true>(runs, P, blockIdx.x, radix_bits, so, threadIdx.x);
 }
 
 // Merge of the sorted spawns (A) with the diagonal survivors (B): workgroup w of a run takes
@@ -574,7 +596,7 @@ __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t 
     const RunState& st = runs[run];
     if (st.status != RUN_OK) return;  // uniform across the workgroup
     const int64_t w = gslice - st.slice0;
-    const int64_t ntl = st.ntl, capV = st.capV;
+    const int64_t ntl = st.ntl, capB = st.capV + st.capS;  // the vector buffer holds L + S
     const int64_t t_lo = ntl * w / st.K, t_hi = ntl * (w + 1) / st.K;
     if (t_lo >= t_hi) return;
     const int cur = (int)st.cur;
@@ -588,7 +610,7 @@ __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t 
     (void)block_exclusive_scan<int64_t, NT>(part, &o);
     for (int64_t t = t_lo; t < t_hi; ++t) {
         const int64_t n = P.mcnt[st.offT + t];
-        if (o + n > capV) break;
+        if (o + n > capB) break;  // not reached: L <= capV and S <= capS
         for (int64_t i0 = 0; i0 < n; i0 += CU * NT) {
             uint64_t kq[CU];
             int64_t vq[CU];
@@ -626,6 +648,10 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
     if (st.status != RUN_OK) return;  // uniform across the workgroup
     const double dtau = st.dtau, shift = st.shift;
     const int64_t capV = st.capV, ntl = st.ntl;
+    if (st.flag == RUN_SPAWN_OVERFLOW) {  // uniform: the step is void, no history row
+        if (tid == 0) st.status = RUN_SPAWN_OVERFLOW;
+        return;
+    }
     const int cur = (int)st.cur;
     const int64_t step = st.step + 1;
     const uint64_t* nk = P.vk[1 - cur] + st.offV;  // the new vector (copy_phase)
@@ -642,9 +668,9 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
     }
     double nrm = 0;  // integer walkers: the sum of the tile norms is exact in any order
     for (int64_t t = tid; t < ntl; t += NT) nrm += P.mnrm[st.offT + t];
-    const int64_t Lnew = min(out, capV);
-    int flag = (int)st.flag;
-    if (out > capV && flag == 0) flag = RUN_VEC_OVERFLOW;
+    // the whole new vector: a step that outgrows max_length (capV) is completed, reported
+    // and kept, and then the run stops
+    const int64_t Lnew = out;
     const double att_sum = st.att;
 
     // ---- ProjectedEnergy, statistics and the shift update -----------------------------
@@ -665,12 +691,12 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
     }
     block_sum3<NT>(nrm, vp, hp);
     if (tid == 0) {
+        double sh = shift;
+        if (Lnew > 0) sh = shift - (st.xi / dtau * log(nrm / st.target) + st.zeta / dtau * log(nrm / st.pnorm));
         const int64_t row = st.nrec;
         if (row < P.hist_cap) {
             const int64_t base = row * nruns + run;
             const int64_t plane = P.hist_cap * nruns;
-            double sh = shift;
-            if (Lnew > 0) sh = shift - (st.xi / dtau * log(nrm / st.target) + st.zeta / dtau * log(nrm / st.pnorm));
             P.hist[0 * plane + base] = sh;
             P.hist[1 * plane + base] = nrm;
             P.hist[2 * plane + base] = (double)Lnew;
@@ -679,15 +705,14 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
             P.hist[5 * plane + base] = att_sum;
             st.nrec = row + 1;
         }
-        if (flag != 0) {
-            st.status = flag;
-        } else if (Lnew == 0) {
+        if (Lnew == 0) {
             st.status = RUN_DEAD;
         } else {
-            st.shift = shift - (st.xi / dtau * log(nrm / st.target) + st.zeta / dtau * log(nrm / st.pnorm));
+            st.shift = sh;
             st.pnorm = nrm;
             st.len = Lnew;
             st.cur = 1 - cur;
+            if (Lnew > capV) st.status = RUN_VEC_OVERFLOW;
         }
         st.step = step;
     }
@@ -730,7 +755,7 @@ struct StepSync {
 template <int NT>
 union StepLds {
     SpawnLds<NT> sp;
-    SortLds<NT> so;
+    SortLds<NT, false> so;
     MergeLds<NT> me;
 };
 
@@ -816,7 +841,7 @@ __global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict_
         __syncthreads();  // s_ticket and s_go are written again by the next round
         if (!go) continue;
         if (phase == 0) spawn_phase<NT>(runs, P, idx, sm.sp, thread_index());
-        else if (phase == 1) sort_phase<NT>(runs, P, run, radix_bits, sm.so, thread_index());
+        else if (phase == 1) sort_phase<NT, false>(runs, P, run, radix_bits, sm.so, thread_index());
         else if (phase == 2) merge_phase<NT>(runs, P, idx, sm.me, thread_index());
         else if (phase == 3) copy_phase<NT>(runs, P, idx, thread_index());
         else final_phase<NT>(runs, nruns, P, run, thread_index());
@@ -1060,7 +1085,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         s.offM = totM;
         totT += (s.capS + s.capV) / MIN_TILE + 3;
         totM += s.capS + s.capV + MAX_TILE;
-        totV += s.capV; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
+        totV += s.capV + s.capS; totS += s.capS; totP += 2 * q.M + 1; totX += (s.K - 1) * s.capS;
         start[r] = q.start_address;
     }
     // slices numbered run by run, runs with the most slices first (their workgroups are

@@ -167,6 +167,16 @@ def test_batch_status_dead_population_and_max_length(gpu):
     assert st[0] == 1 and st[1] == 2 and st[2] == 0 and ln[2] > 0
     h = b.history()
     assert h["norm"].shape[0] == 100
+    # as fciqmc.jl:155-179: the step that outgrows max_length (2 * 100 + 100) is completed
+    # and reported, the run keeps its new vector, and then stops
+    last = int(np.nonzero(h["len"][:, 1])[0][-1])
+    assert h["len"][last, 1] > 300 and ln[1] == h["len"][last, 1]
+    keys, vals = b.vector(1)
+    assert len(keys) == ln[1] and (keys[1:] > keys[:-1]).all()
+    assert np.abs(vals).sum() == h["norm"][last, 1]
+    start = gpu.BoseFS(1, 3, 5, 2, 1).bits
+    at = np.nonzero(keys == start)[0]
+    assert h["vproj"][last, 1] == (vals[at[0]] if len(at) else 0)
 
 
 @pytest.mark.parametrize("M,N,u", [(6, 6, 6.0), (8, 5, 3.0), (7, 7, 2.0)])

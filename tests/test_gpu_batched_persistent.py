@@ -43,9 +43,8 @@ def test_persistent_step_equals_five_launches(gpu, block, radix, spg):
     sp = _mixed(gpu)
     nsteps = 300
     print(f"persistent step: block {block}, radix {radix}, {spg} steps per graph", flush=True)
-    # the same block size: the step at which a vector overflows keeps a prefix of whole
-    # merge tiles, whose size follows the block size
-    ref = _run(gpu, sp, nsteps, False, block, radix, 50)
+    # one five-launch configuration for all: results do not depend on the launch shape
+    ref = _run(gpu, sp, nsteps, False, 512, 6, 50)
     got = _run(gpu, sp, nsteps, True, block, radix, spg)
     for f in FIELDS:
         np.testing.assert_array_equal(got[0][f], ref[0][f], err_msg=f)
