@@ -195,11 +195,14 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                 nbk[it] = 0;
                 nbv[it] = 0;
                 if (att < T) {
-                    int lo = 0, hi = CH;  // aoff[lo] <= att < aoff[hi]
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (sp.aoff[mid] <= att) lo = mid; else hi = mid;
-                    }
+                    // the entry of the attempt: the last lo with aoff[lo] <= att (offsets
+                    // rise strictly over the chunk's entries and equal T past them); a fixed
+                    // number of halvings, so the AIT searches of a thread overlap
+                    static_assert((CH & (CH - 1)) == 0, "power-of-two chunk");
+                    int lo = 0;
+#pragma unroll
+                    for (int s = CH / 2; s >= 1; s >>= 1)
+                        if (sp.aoff[lo + s] <= att) lo += s;
                     const uint64_t k = sp.k[lo];
                     const int64_t v = sp.v[lo];
                     const int64_t j = att - sp.aoff[lo];
