@@ -34,7 +34,10 @@ namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
 constexpr int SPAWN_CH = 1024;   // entries per spawn chunk (SPAWN_CH / NT per thread)
-constexpr int AIT = 4;           // attempts per thread per spawn group
+// attempts per thread per spawn group: 8, fewer where the (attempt item, wavefront) counts
+// would not fit one wavefront
+template <int NT>
+constexpr int spawn_ait() { return 8 * (NT / WAVE) <= WAVE ? 8 : WAVE / (NT / WAVE); }
 constexpr double WALKERS_PER_SLICE = 2048;  // target walkers per spawn workgroup of a run
 constexpr int64_t MAX_SLICES = 64;
 constexpr int SPAWN_NT = 256;                // spawn workgroup size (several per CU)
@@ -86,7 +89,7 @@ __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); 
 // Diagonal step and spawning of one slice of a run: entries [L*s/K, L*(s+1)/K).  Records go,
 // in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
 // region; batch_step_kernel gathers the slices in order.
-template <int NT>
+template <int NT, int AIT = spawn_ait<NT>()>
 struct SpawnLds {
     int cnt[2][AIT * (NT / WAVE)];  // double-buffered by attempt group
     int tot[2];
@@ -95,8 +98,9 @@ struct SpawnLds {
     int64_t aoff[SPAWN_CH + 1];  // attempt offsets within the chunk (counts first)
 };
 
-template <int NT>
-__device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, SpawnLds<NT>& sp, int tid) {
+template <int NT, int AIT = spawn_ait<NT>()>
+__device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, SpawnLds<NT, AIT>& sp,
+                            int tid) {
     constexpr int NW = NT / WAVE;
     constexpr int EPT = SPAWN_CH / NT;    // entries per thread per chunk
     constexpr int CH = SPAWN_CH;
@@ -345,11 +349,13 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
             for (int c = 0; c < CU; ++c) {
                 const int64_t j = j0 + c * NT + tid;
                 if (j < S) {
-                    int lo = 1, hi = K;  // xoff[lo] <= j < xoff[hi]
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (so.xoff[mid] <= j) lo = mid; else hi = mid;
-                    }
+                    // the last slice lo with xoff[lo] <= j (xoff[q] = S for q >= K): six
+                    // halvings, so the CU searches of a thread overlap
+                    static_assert(MAX_SLICES == 64, "six halvings");
+                    int lo = 0;
+#pragma unroll
+                    for (int s = 32; s >= 1; s >>= 1)
+                        if (so.xoff[lo + s] <= j) lo += s;
                     const int64_t src = st.offX + (int64_t)(lo - 1) * capS + (j - so.xoff[lo]);
                     kq[c] = P.xk[src];
                     vq[c] = P.xv[src];
@@ -757,7 +763,7 @@ struct StepSync {
 
 template <int NT>
 union StepLds {
-    SpawnLds<NT> sp;
+    SpawnLds<NT, 4> sp;  // four attempts per thread: eight spill in the fused kernel
     SortLds<NT, false> so;
     MergeLds<NT> me;
 };
@@ -843,7 +849,7 @@ __global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict_
         const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
         __syncthreads();  // s_ticket and s_go are written again by the next round
         if (!go) continue;
-        if (phase == 0) spawn_phase<NT>(runs, P, idx, sm.sp, thread_index());
+        if (phase == 0) spawn_phase<NT, 4>(runs, P, idx, sm.sp, thread_index());
         else if (phase == 1) sort_phase<NT, false>(runs, P, run, radix_bits, sm.so, thread_index());
         else if (phase == 2) merge_phase<NT>(runs, P, idx, sm.me, thread_index());
         else if (phase == 3) copy_phase<NT>(runs, P, idx, thread_index());
