@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -55,6 +56,7 @@ struct RunState {
     int64_t len, cur, step, nrec;     // nrec: history rows written
     int64_t capV, capS, offV, offS, offP, nproj;
     int64_t K, slice0, offX;          // spawn slices, first slice's global index, slice pool offset
+    int64_t offB;                     // the slices' bounds (K + 1 per run)
     int64_t offT, offM;               // merge tile arrays, merge slot pool
     // handed from the sort kernel to the merge and final kernels of the same step
     int64_t S_cur, ntl, sbuf, flag;
@@ -76,12 +78,18 @@ struct Pools {
     int64_t* xcnt;       // records of every slice
     double* xatt;        // spawn attempts of every slice
     const int* wg_run;   // run of every spawn / merge workgroup
+    // Spawn slice s of a run takes the attempts [A s / K, A (s + 1) / K) of its vector (A =
+    // walkers, attempt numbers run entry by entry): sbe[s] is the entry holding attempt
+    // A s / K and sbp[s] the walkers before that entry; sbe[K] = length, sbp[K] = A.
+    int64_t* sbe;
+    int64_t* sbp;
     int64_t* msplit;     // per merge tile: spawns before its start, then survivor offsets
     uint64_t* mprev;     // per merge tile: merged key just before it
     int64_t* mcnt;       // per merge tile: survivors
     double* mnrm;        // per merge tile: norm of the survivors
     uint64_t* mk;        // per merge tile: survivor slots (TILE each)
     int64_t* mv;
+    int64_t* mw;         // per merge slot: walkers before the survivor in its tile
 };
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
@@ -121,7 +129,17 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     const int64_t Lrun = st.len, capS = st.capS;
     const int cur = (int)st.cur;
     const int64_t step = st.step + 1;
-    const int64_t e_lo = Lrun * slice / st.K, e_hi = Lrun * (slice + 1) / st.K;
+    // the slice's attempts [q_lo, q_hi) and the entries holding them: e_lo (whose first d0
+    // attempts belong to the slice before) up to the entry holding attempt q_hi - 1
+    const int64_t K = st.K;
+    const int64_t* sbe = P.sbe + st.offB;
+    const int64_t* sbp = P.sbp + st.offB;
+    const int64_t A = sbp[K];
+    const int64_t q_lo = A * slice / K, q_hi = A * (slice + 1) / K;
+    const int64_t e_lo = sbe[slice], d0 = q_lo - sbp[slice];
+    const int64_t e_nx = sbe[slice + 1];
+    const int64_t e_hi = (e_nx < Lrun && q_hi > sbp[slice + 1]) ? e_nx + 1 : e_nx;
+    int64_t left = q_hi - q_lo;  // attempts of the slice not yet taken (uniform)
     const uint64_t* vk = P.vk[cur] + st.offV + e_lo;
     const int64_t* vv = P.vv[cur] + st.offV + e_lo;
     int64_t* dv = P.dv + st.offV + e_lo;
@@ -133,7 +151,7 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     // ---- diagonal step and spawning, CH entries at a time -------------------------------
     // Entries are loaded striped (coalesced) and kept in LDS; attempts are numbered entry by
     // entry, so the records come out in (entry, attempt) order.
-    double st_att = 0, st_spawn = 0, st_death = 0, st_clone = 0, st_zombie = 0;
+    double st_spawn = 0, st_death = 0, st_clone = 0, st_zombie = 0;
     for (int64_t c0 = 0; c0 < L; c0 += CH) {
         const int n_ch = (int)min((int64_t)CH, L - c0);
         uint64_t kq[EPT];
@@ -150,17 +168,21 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
             if (loc < n_ch) {
                 const uint64_t k = kq[j];
                 const int64_t v = vq[j];
-                const double hd = u * (double)bose_hubbard_interaction(k) / 2;
-                const double x = (1 - dtau * (hd - shift)) * (double)v;
-                const U4 r = draw(seed, k, 0, step, TAG_DIAG);
-                const int64_t res = isgn(x) * (int64_t)floor(fabs(x) + u53(r.x, r.y));
-                dv[c0 + loc] = res;
-                if (res > v) st_clone += (double)(res - v);
-                else if (isgn((double)res) != isgn((double)v)) { st_death += fabs((double)v); st_zombie += fabs((double)res); }
-                else st_death += fabs((double)(res - v));
+                const bool first = c0 + loc == 0;  // the slice's first entry
+                if (!first || d0 == 0) {  // the entry's diagonal step: the slice of its first attempt
+                    const double hd = u * (double)bose_hubbard_interaction(k) / 2;
+                    const double x = (1 - dtau * (hd - shift)) * (double)v;
+                    const U4 r = draw(seed, k, 0, step, TAG_DIAG);
+                    const int64_t res = isgn(x) * (int64_t)floor(fabs(x) + u53(r.x, r.y));
+                    dv[c0 + loc] = res;
+                    if (res > v) st_clone += (double)(res - v);
+                    else if (isgn((double)res) != isgn((double)v)) { st_death += fabs((double)v); st_zombie += fabs((double)res); }
+                    else st_death += fabs((double)(res - v));
+                }
                 sp.k[loc] = k;
                 sp.v[loc] = v;
-                sp.aoff[loc] = v < 0 ? -v : v;   // attempts: |v| (every entry has one at least)
+                // attempts: |v| (every entry has one at least), less those of the slice before
+                sp.aoff[loc] = (v < 0 ? -v : v) - (first ? d0 : 0);
             }
         }
         __syncthreads();
@@ -181,7 +203,9 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
             run_ += c[j];
         }
         if (tid == 0) sp.aoff[CH] = T;
-        st_att += (double)sum;
+        // the last entry's attempts past q_hi belong to the next slice
+        T = min(T, left);
+        left -= T;
         __syncthreads();
         // attempts in groups of NT * AIT: one ballot per attempt, then a scan of the
         // (attempt item, wavefront) counts gives every record its place in attempt order
@@ -209,7 +233,7 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                         if (sp.aoff[lo + s] <= att) lo += s;
                     const uint64_t k = sp.k[lo];
                     const int64_t v = sp.v[lo];
-                    const int64_t j = att - sp.aoff[lo];
+                    const int64_t j = att - sp.aoff[lo] + (c0 + lo == 0 ? d0 : 0);  // attempt of the entry
                     const int64_t n_od = 2 * (int64_t)num_occupied(k);
                     const int64_t na = v < 0 ? -v : v;
                     const double magnitude = (double)v / (double)na;
@@ -249,12 +273,10 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
         }
         __syncthreads();  // the chunk's entries in LDS are read until here
     }
-    __syncthreads();
     // record count (it may exceed capS: the gather then reports the overflow) and attempts
-    const double att_sum = block_sum<NT>(st_att);
     if (tid == 0) {
         P.xcnt[st.slice0 + slice] = S_run;
-        P.xatt[st.slice0 + slice] = att_sum;
+        P.xatt[st.slice0 + slice] = (double)(q_hi - q_lo);
     }
     (void)st_spawn; (void)st_death; (void)st_clone; (void)st_zombie; (void)u;
 }
@@ -491,6 +513,7 @@ __device__ void merge_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     const int64_t* av = P.sv[st.sbuf] + st.offS;
     uint64_t* mk = P.mk + st.offM;
     int64_t* mv = P.mv + st.offM;
+    int64_t* mw = P.mw + st.offM;
 
     for (int64_t tl = t_lo; tl < t_hi; ++tl) {
         const int64_t d0 = tl * TILE;
@@ -546,7 +569,7 @@ __device__ void merge_phase(RunState* __restrict__ runs, const Pools& P, int64_t
         uint64_t ok_k[MIPT];
         int64_t ok_v[MIPT];
         int cnt = 0;
-        double nrm = 0;
+        int64_t wk = 0;  // walkers of the thread's survivors
 #pragma unroll
         for (int j = 0; j < MIPT; ++j) {
             if (j < n_here) {
@@ -565,7 +588,7 @@ __device__ void merge_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                         ok_k[cnt] = key;
                         ok_v[cnt] = acc;
                         ++cnt;
-                        nrm += fabs((double)acc);
+                        wk += acc < 0 ? -acc : acc;
                     }
                 }
                 prev = key;
@@ -573,16 +596,22 @@ __device__ void merge_phase(RunState* __restrict__ runs, const Pools& P, int64_t
                 if (take_b) ++ib; else ++ia;
             }
         }
-        int tot;
-        const int ex = block_exclusive_scan<int, NT>(cnt, &tot);  // ends with a barrier
+        // one scan of (walkers << 16 | survivors): places, the walkers before every survivor
+        // in the tile, and the tile's count and norm (at most 2 NT < 2^16 survivors, so the
+        // count never carries into the walkers)
+        int64_t ptot;
+        const int64_t pex = block_exclusive_scan<int64_t, NT>(wk << 16 | cnt, &ptot);  // ends with a barrier
+        const int ex = (int)(pex & 0xffff);
+        int64_t wx = pex >> 16;
         for (int j = 0; j < cnt; ++j) {
             mk[d0 + ex + j] = ok_k[j];
             mv[d0 + ex + j] = ok_v[j];
+            mw[d0 + ex + j] = wx;
+            wx += ok_v[j] < 0 ? -ok_v[j] : ok_v[j];
         }
-        const double tn = block_sum<NT>(nrm);
         if (tid == 0) {
-            P.mcnt[st.offT + tl] = tot;
-            P.mnrm[st.offT + tl] = tn;
+            P.mcnt[st.offT + tl] = ptot & 0xffff;
+            P.mnrm[st.offT + tl] = (double)(ptot >> 16);
         }
     }
 }
@@ -594,9 +623,9 @@ __global__ __launch_bounds__(NT) void batch_merge_kernel(RunState* __restrict__ 
 }
 
 // The tiles' survivors into the new vector: copy slice w of a run takes the tiles
-// [ntl*w/K, ntl*(w+1)/K), their offset the survivors of the tiles before.  A tile that does
-// not fit entirely is skipped (and so is every later one, since the offsets grow), so an
-// overflowing step leaves a prefix of whole tiles.
+// [ntl*w/K, ntl*(w+1)/K), their offset the survivors of the tiles before.  With the walkers
+// before every entry (those of the tiles before, and before it in its tile from the merge)
+// it writes the next step's spawn slice bounds that fall in its entries.
 template <int NT>
 __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, int tid) {
     constexpr int TILE = NT * MIPT;
@@ -613,22 +642,48 @@ __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t 
     int64_t* nv = P.vv[1 - cur] + st.offV;
     const uint64_t* mk = P.mk + st.offM;
     const int64_t* mv = P.mv + st.offM;
-    int64_t part = 0;
-    for (int64_t t = tid; t < t_lo; t += NT) part += P.mcnt[st.offT + t];
-    int64_t o;
-    (void)block_exclusive_scan<int64_t, NT>(part, &o);
+    const int64_t* mw = P.mw + st.offM;
+    // survivors and walkers before the range, and all walkers (tile norms: exact integers),
+    // summed by the first wavefront
+    __shared__ int64_t s_pre[3];
+    if (tid < WAVE) {
+        int64_t part = 0, wpart = 0, wall = 0;
+        for (int64_t t = tid; t < ntl; t += WAVE) {
+            const int64_t m = (int64_t)P.mnrm[st.offT + t];
+            if (t < t_lo) {
+                part += P.mcnt[st.offT + t];
+                wpart += m;
+            }
+            wall += m;
+        }
+        part = wave_inclusive_scan(part);
+        wpart = wave_inclusive_scan(wpart);
+        wall = wave_inclusive_scan(wall);
+        if (tid == WAVE - 1) {
+            s_pre[0] = part;
+            s_pre[1] = wpart;
+            s_pre[2] = wall;
+        }
+    }
+    __syncthreads();
+    int64_t o = s_pre[0], W = s_pre[1];
+    const int64_t A = s_pre[2];
+    const int64_t K = st.K;
+    int64_t* sbe = P.sbe + st.offB;
+    int64_t* sbp = P.sbp + st.offB;
     for (int64_t t = t_lo; t < t_hi; ++t) {
         const int64_t n = P.mcnt[st.offT + t];
         if (o + n > capB) break;  // not reached: L <= capV and S <= capS
         for (int64_t i0 = 0; i0 < n; i0 += CU * NT) {
             uint64_t kq[CU];
-            int64_t vq[CU];
+            int64_t vq[CU], wq[CU];
 #pragma unroll
             for (int c = 0; c < CU; ++c) {
                 const int64_t i = i0 + c * NT + tid;
                 if (i < n) {
                     kq[c] = mk[t * TILE + i];
                     vq[c] = mv[t * TILE + i];
+                    wq[c] = mw[t * TILE + i];
                 }
             }
 #pragma unroll
@@ -637,9 +692,21 @@ __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t 
                 if (i < n) {
                     nk[o + i] = kq[c];
                     nv[o + i] = vq[c];
+                    const int64_t Pg = W + wq[c];  // walkers before the entry
+                    const int64_t a = vq[c] < 0 ? -vq[c] : vq[c];
+                    // slices q whose first attempt floor(A q / K) lies in [Pg, Pg + a), i.e.
+                    // Pg K <= A q < (Pg + a) K.  The smallest is ceil(Pg K / A), exact in
+                    // doubles while A K < 2^53: a quotient that is not an integer lies at
+                    // least 1 / A from one, far beyond the rounding of the division.
+                    int64_t q = max((int64_t)1, (int64_t)ceil((double)(Pg * K) / (double)A));
+                    for (; q < K && A * q < (Pg + a) * K; ++q) {
+                        sbe[q] = o + i;
+                        sbp[q] = Pg;
+                    }
                 }
             }
         }
+        W += (int64_t)P.mnrm[st.offT + t];
         o += n;
     }
 }
@@ -721,6 +788,8 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
             st.pnorm = nrm;
             st.len = Lnew;
             st.cur = 1 - cur;
+            P.sbe[st.offB + st.K] = Lnew;  // the bounds of the next step's last spawn slice
+            P.sbp[st.offB + st.K] = (int64_t)nrm;
             if (Lnew > capV) st.status = RUN_VEC_OVERFLOW;
         }
         st.step = step;
@@ -928,10 +997,11 @@ struct rimu_batch {
     DevBuf<int64_t> xv, xcnt;
     DevBuf<double> xatt;
     DevBuf<int> wg_run;
-    DevBuf<int64_t> msplit, mcnt, mv;
+    DevBuf<int64_t> msplit, mcnt, mv, mw;
     DevBuf<uint64_t> mprev, mk;
     DevBuf<double> mnrm;
-    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, hist_cap = 0;
+    DevBuf<int64_t> sbe, sbp;       // spawn slice bounds
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totK = 0, totT = 0, totM = 0, totB = 0, hist_cap = 0;
     std::map<std::pair<int, std::pair<int, int>>, hipGraphExec_t> graphs;  // (steps, (block, radix)); cleared
                                                                            // when the launch kind changes
     Pools pools() {
@@ -952,6 +1022,9 @@ struct rimu_batch {
         P.xcnt = xcnt.p;
         P.xatt = xatt.p;
         P.wg_run = wg_run.p;
+        P.sbe = sbe.p;
+        P.sbp = sbp.p;
+        P.mw = mw.p;
         P.msplit = msplit.p;
         P.mprev = mprev.p;
         P.mcnt = mcnt.p;
@@ -1066,7 +1139,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     std::vector<RunState> st(nruns);
     std::vector<uint64_t> start(nruns);
     std::vector<int> wg_run;
-    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totT = 0, totM = 0;
+    int64_t totV = 0, totS = 0, totP = 0, totX = 0, totT = 0, totM = 0, totB = 0;
     for (int r = 0; r < nruns; ++r) {
         const rimu_batch_run& q = runs[r];
         if (q.M < 2 || q.N < 1 || q.M + q.N - 1 > 64) return fail(RIMU_ERR_ARG, "run: need M >= 2, N >= 1, N + M - 1 <= 64");
@@ -1089,6 +1162,8 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
         // spawning of a run is split over K workgroups, one per WALKERS_PER_SLICE target walkers
         s.K = std::min<int64_t>(MAX_SLICES, std::max<int64_t>(1, (int64_t)std::ceil(q.target_walkers / WALKERS_PER_SLICE)));
         s.offX = totX;
+        s.offB = totB;
+        totB += s.K + 1;
         // merge tiles: boundaries for the smallest tile, slots for the largest
         s.offT = totT;
         s.offM = totM;
@@ -1110,7 +1185,7 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     b->nruns = nruns;
     b->host_init = st;
     b->totV = totV; b->totS = totS; b->totP = totP; b->hist_cap = history_steps;
-    b->totX = totX; b->totK = (int64_t)wg_run.size(); b->totT = totT; b->totM = totM;
+    b->totX = totX; b->totK = (int64_t)wg_run.size(); b->totT = totT; b->totM = totM; b->totB = totB;
     auto bad = [&](const char* what, hipError_t e) {
         std::string m = std::string(what) + ": " + hipGetErrorString(e);
         rimu_batch_destroy(b);
@@ -1141,6 +1216,9 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->mnrm.alloc(totT)) != hipSuccess) return bad("merge norms", e);
     if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
     if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
+    if ((e = b->mw.alloc(totM)) != hipSuccess) return bad("merge slot walker offsets", e);
+    if ((e = b->sbe.alloc(totB)) != hipSuccess) return bad("spawn slice bounds", e);
+    if ((e = b->sbp.alloc(totB)) != hipSuccess) return bad("spawn slice bounds", e);
     if ((e = hipMemcpyAsync(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     // the persistent step: one work queue per XCD, the runs dealt to the least loaded queue
     // (by spawn slices), largest first
@@ -1190,6 +1268,14 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = hipMemcpyAsync(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpyAsync(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     std::vector<int64_t> pops(nruns);  // host buffers live until the stream is synchronized below
+    // slice bounds of the start vector (one entry holding every attempt)
+    std::vector<int64_t> sb_e(totB, 0), sb_p(totB, 0);
+    for (int r = 0; r < nruns; ++r) {
+        sb_e[st[r].offB + st[r].K] = 1;
+        sb_p[st[r].offB + st[r].K] = std::llabs(runs[r].start_population);
+    }
+    if ((e = hipMemcpyAsync(b->sbe.p, sb_e.data(), totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->sbp.p, sb_p.data(), totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
         pops[r] = runs[r].start_population;
         if ((e = hipMemcpyAsync(b->vk[0].p + st[r].offV, &start[r], 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
@@ -1211,6 +1297,7 @@ int rimu_batch_destroy(rimu_batch* b) {
     b->dv.release(); b->pk.release(); b->pc.release(); b->start.release(); b->hist.release();
     b->xk.release(); b->xv.release(); b->xcnt.release(); b->xatt.release(); b->wg_run.release();
     b->msplit.release(); b->mprev.release(); b->mcnt.release(); b->mnrm.release(); b->mk.release(); b->mv.release();
+    b->sbe.release(); b->sbp.release(); b->mw.release();
     b->sync.release(); b->items.release(); b->queue_start.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1301,16 +1388,26 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
                         int32_t* best_persistent, double* ms) {
     if (!b) return fail(RIMU_ERR_ARG, "batch is null");
     if (trial_steps < 1) return fail(RIMU_ERR_ARG, "trial_steps must be >= 1");
-    // snapshot: run states and both vector buffers (the step reads one, writes the other)
+    // snapshot: run states, both vector buffers (the step reads one, writes the other) and
+    // the spawn slice bounds
     DevBuf<RunState> s_runs;
     DevBuf<uint64_t> s_vk[2];
-    DevBuf<int64_t> s_vv[2];
-    auto cleanup = [&]() { s_runs.release(); for (int k = 0; k < 2; ++k) { s_vk[k].release(); s_vv[k].release(); } };
+    DevBuf<int64_t> s_vv[2], s_sbe, s_sbp;
+    auto cleanup = [&]() {
+        s_runs.release();
+        s_sbe.release();
+        s_sbp.release();
+        for (int k = 0; k < 2; ++k) { s_vk[k].release(); s_vv[k].release(); }
+    };
     auto body = [&]() -> int {
         HIP_TRY(hipStreamSynchronize(b->stream));
         HIP_TRY(s_runs.alloc(b->nruns));
         // copies on the batch's stream, ordered with its kernels
         HIP_TRY(hipMemcpyAsync(s_runs.p, b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(s_sbe.alloc(b->totB));
+        HIP_TRY(s_sbp.alloc(b->totB));
+        HIP_TRY(hipMemcpyAsync(s_sbe.p, b->sbe.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(s_sbp.p, b->sbp.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
         for (int k = 0; k < 2; ++k) {
             HIP_TRY(s_vk[k].alloc(b->totV));
             HIP_TRY(s_vv[k].alloc(b->totV));
@@ -1319,6 +1416,8 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         }
         auto restore = [&]() -> int {
             HIP_TRY(hipMemcpyAsync(b->runs.p, s_runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipMemcpyAsync(b->sbe.p, s_sbe.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipMemcpyAsync(b->sbp.p, s_sbp.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
             for (int k = 0; k < 2; ++k) {
                 HIP_TRY(hipMemcpyAsync(b->vk[k].p, s_vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
                 HIP_TRY(hipMemcpyAsync(b->vv[k].p, s_vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
