@@ -2,8 +2,9 @@
 // together (C ABI rimu_batch_*).
 //
 // A step of every run has five phases, the plain engine's algorithm at the size of a run:
-// spawn slices (a run's entries split over K workgroups, one per 2048 target walkers:
-// diagonal step and spawn attempts, ballot compaction into the slice's spawn buffer), the
+// spawn slices (a run's spawn attempts split evenly over K workgroups, one per 2048 target
+// walkers: diagonal step and spawn attempts, ballot compaction into the slice's spawn
+// buffer), the
 // run's sort (one workgroup: gather of the slices in order, stable LSD radix sort with LDS
 // histograms and ballot ranks, merge tile boundaries), merge slices (the run's merge tiles
 // over K workgroups), copy slices (the tiles' survivors into the new vector, over K
@@ -13,9 +14,7 @@
 // so a sequence of steps is captured once in a HIP graph and replayed.
 //
 // Random draws are addressed by (seed, address, attempt, step) and integer walker sums are
-// exact, so the result of a step does not depend on the workgroup size or radix width
-// (except the vector an overflowing step leaves: a prefix of whole merge tiles, whose size
-// follows the workgroup size).
+// exact, so the result of a step does not depend on the workgroup size or radix width.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -94,9 +93,10 @@ struct Pools {
 
 __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); }
 
-// Diagonal step and spawning of one slice of a run: entries [L*s/K, L*(s+1)/K).  Records go,
+// Diagonal step and spawning of one slice of a run: its attempts [A s / K, A (s + 1) / K)
+// (A walkers; attempts numbered entry by entry) and the entries holding them.  Records go,
 // in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
-// region; batch_step_kernel gathers the slices in order.
+// region; the run's sort gathers the slices in order.
 template <int NT, int AIT = spawn_ait<NT>()>
 struct SpawnLds {
     int cnt[2][AIT * (NT / WAVE)];  // double-buffered by attempt group
