@@ -294,8 +294,8 @@ constexpr int SORT_LDS_CAP = 4096;  // records a run's sort keeps in LDS for all
 
 template <int NT, bool LS>
 struct SortLds {
-    uint32_t hist[256];
-    uint32_t wcnt[NT / WAVE][256];
+    uint32_t hist[2][256];              // digit offsets, double-buffered by chunk
+    uint16_t wcnt[2][NT / WAVE][256];   // (wavefront, digit) counts, double-buffered by chunk
     int64_t xoff[MAX_SLICES + 1];  // first record of each slice in the gathered list
     double att;
     int flag;
@@ -310,8 +310,7 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
                            int tid) {
     constexpr int NW = NT / WAVE;
     constexpr int TILE = NT * MIPT;
-    uint32_t* s_hist = so.hist;
-    uint32_t(*s_wcnt)[256] = so.wcnt;
+    uint32_t* s_hist = so.hist[0];
     int& s_flag = so.flag;
 
     RunState& st = runs[run];
@@ -404,7 +403,7 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
     for (int sh = 0; sh < B; sh += radix_bits) {
         const uint64_t dmask = (uint64_t)(nbins - 1);
         for (int d = tid; d < nbins; d += NT) s_hist[d] = 0;
-        for (int j = tid; j < NW * nbins; j += NT) s_wcnt[j / nbins][j % nbins] = 0;
+        for (int j = tid; j < NW * nbins; j += NT) so.wcnt[0][j / nbins][j % nbins] = 0;
         __syncthreads();
         for (int64_t i = tid; i < S; i += NT) atomicAdd(&s_hist[(srck[i] >> sh) & dmask], 1u);
         __syncthreads();
@@ -415,7 +414,11 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
             if (tid < nbins) s_hist[tid] = ex;
         }
         __syncthreads();
-        for (int64_t t0 = 0; t0 < S; t0 += NT) {
+        // chunk c reads offsets hist[c & 1] and counts wcnt[c & 1], writes the next chunk's
+        // offsets into hist[1 - (c & 1)] and clears wcnt[1 - (c & 1)] (last read by chunk
+        // c - 1 before this chunk's first barrier): two barriers per chunk
+        int p = 0;
+        for (int64_t t0 = 0; t0 < S; t0 += NT, p ^= 1) {
             const int64_t i = t0 + tid;
             const bool valid = i < S;
             const uint64_t key = valid ? srck[i] : 0;
@@ -427,19 +430,21 @@ __device__ void sort_phase(RunState* __restrict__ runs, const Pools& P, int run,
                 peers &= ((digit >> b) & 1u) ? bb : ~bb;
             }
             const uint32_t in_wave = (uint32_t)__popcll(peers & lt_mask);
-            if (valid && in_wave == 0) s_wcnt[wid][digit] = (uint32_t)__popcll(peers);
+            if (valid && in_wave == 0) so.wcnt[p][wid][digit] = (uint16_t)__popcll(peers);
             __syncthreads();
             if (valid) {
-                uint32_t pos = s_hist[digit] + in_wave;
-                for (int w = 0; w < wid; ++w) pos += s_wcnt[w][digit];
+                uint32_t pos = so.hist[p][digit] + in_wave;
+                for (int w = 0; w < wid; ++w) pos += so.wcnt[p][w][digit];
                 dstk[pos] = key;
                 dstv[pos] = val;
             }
-            __syncthreads();
             for (int d = tid; d < nbins; d += NT) {
                 uint32_t add = 0;
-                for (int w = 0; w < NW; ++w) { add += s_wcnt[w][d]; s_wcnt[w][d] = 0; }
-                s_hist[d] += add;
+                for (int w = 0; w < NW; ++w) {
+                    add += so.wcnt[p][w][d];
+                    so.wcnt[p ^ 1][w][d] = 0;
+                }
+                so.hist[p ^ 1][d] = so.hist[p][d] + add;
             }
             __syncthreads();
         }
