@@ -394,7 +394,8 @@ int rimu_batch_set_wait(rimu_batch* b, uint32_t spin_limit, int32_t withhold_fir
  * spawns overflow the run's spawn buffer (4 target_walkers + 4096 records: that step is
  * void and leaves no history row). */
 int rimu_batch_step(rimu_batch* b, int64_t nsteps, int32_t steps_per_graph);
-/* Steps recorded so far and the history arrays (each [steps][nruns], may be null). */
+/* Steps recorded so far (the most rows of any run) and the history arrays (each
+ * [steps][nruns], may be null); rows a run has not written (it stopped earlier) hold 0. */
 int rimu_batch_history(rimu_batch* b, int64_t* nsteps, double* shift, double* norm, double* len,
                        double* vproj, double* hproj, double* attempts);
 /* Status of every run: 0 running, 1 population dead, 2 vector overflow (max_length),

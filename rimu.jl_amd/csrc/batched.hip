@@ -79,9 +79,12 @@ struct Pools {
     const int* wg_run;   // run of every spawn / merge workgroup
     // Spawn slice s of a run takes the attempts [A s / K, A (s + 1) / K) of its vector (A =
     // walkers, attempt numbers run entry by entry): sbe[s] is the entry holding attempt
-    // A s / K and sbp[s] the walkers before that entry; sbe[K] = length, sbp[K] = A.
+    // A s / K and sbp[s] the walkers before that entry; sbe[K] = length, sbp[K] = A.  Two
+    // copies of totB each, like the vectors: the bounds of vector cur at cur * totB, so a
+    // step that is not completed leaves the current vector's bounds as they were.
     int64_t* sbe;
     int64_t* sbp;
+    int64_t totB;
     int64_t* msplit;     // per merge tile: spawns before its start, then survivor offsets
     uint64_t* mprev;     // per merge tile: merged key just before it
     int64_t* mcnt;       // per merge tile: survivors
@@ -132,8 +135,8 @@ __device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t
     // the slice's attempts [q_lo, q_hi) and the entries holding them: e_lo (whose first d0
     // attempts belong to the slice before) up to the entry holding attempt q_hi - 1
     const int64_t K = st.K;
-    const int64_t* sbe = P.sbe + st.offB;
-    const int64_t* sbp = P.sbp + st.offB;
+    const int64_t* sbe = P.sbe + cur * P.totB + st.offB;
+    const int64_t* sbp = P.sbp + cur * P.totB + st.offB;
     const int64_t A = sbp[K];
     const int64_t q_lo = A * slice / K, q_hi = A * (slice + 1) / K;
     const int64_t e_lo = sbe[slice], d0 = q_lo - sbp[slice];
@@ -674,8 +677,8 @@ __device__ void copy_phase(RunState* __restrict__ runs, const Pools& P, int64_t 
     int64_t o = s_pre[0], W = s_pre[1];
     const int64_t A = s_pre[2];
     const int64_t K = st.K;
-    int64_t* sbe = P.sbe + st.offB;
-    int64_t* sbp = P.sbp + st.offB;
+    int64_t* sbe = P.sbe + (1 - cur) * P.totB + st.offB;  // the new vector's bounds
+    int64_t* sbp = P.sbp + (1 - cur) * P.totB + st.offB;
     for (int64_t t = t_lo; t < t_hi; ++t) {
         const int64_t n = P.mcnt[st.offT + t];
         if (o + n > capB) break;  // not reached: L <= capV and S <= capS
@@ -793,8 +796,9 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
             st.pnorm = nrm;
             st.len = Lnew;
             st.cur = 1 - cur;
-            P.sbe[st.offB + st.K] = Lnew;  // the bounds of the next step's last spawn slice
-            P.sbp[st.offB + st.K] = (int64_t)nrm;
+            // the end bound of the new vector, whose other bounds the copy phase wrote
+            P.sbe[(1 - cur) * P.totB + st.offB + st.K] = Lnew;
+            P.sbp[(1 - cur) * P.totB + st.offB + st.K] = (int64_t)nrm;
             if (Lnew > capV) st.status = RUN_VEC_OVERFLOW;
         }
         st.step = step;
@@ -1029,6 +1033,7 @@ struct rimu_batch {
         P.wg_run = wg_run.p;
         P.sbe = sbe.p;
         P.sbp = sbp.p;
+        P.totB = totB;
         P.mw = mw.p;
         P.msplit = msplit.p;
         P.mprev = mprev.p;
@@ -1210,6 +1215,9 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->pc.alloc(totP)) != hipSuccess) return bad("projector coefficients", e);
     if ((e = b->start.alloc(nruns)) != hipSuccess) return bad("start addresses", e);
     if ((e = b->hist.alloc(std::max<int64_t>(1, NHIST * history_steps * nruns))) != hipSuccess) return bad("history", e);
+    // rows a run does not write (it stopped, or a step of it was cut off) read zero
+    if ((e = hipMemsetAsync(b->hist.p, 0, std::max<int64_t>(1, NHIST * history_steps * nruns) * sizeof(double), b->stream)) != hipSuccess)
+        return bad("history", e);
     if ((e = b->xk.alloc(std::max<int64_t>(1, totX))) != hipSuccess) return bad("slice spawn keys", e);
     if ((e = b->xv.alloc(std::max<int64_t>(1, totX))) != hipSuccess) return bad("slice spawn values", e);
     if ((e = b->xcnt.alloc(b->totK)) != hipSuccess) return bad("slice counts", e);
@@ -1222,8 +1230,8 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = b->mk.alloc(totM)) != hipSuccess) return bad("merge slot keys", e);
     if ((e = b->mv.alloc(totM)) != hipSuccess) return bad("merge slot values", e);
     if ((e = b->mw.alloc(totM)) != hipSuccess) return bad("merge slot walker offsets", e);
-    if ((e = b->sbe.alloc(totB)) != hipSuccess) return bad("spawn slice bounds", e);
-    if ((e = b->sbp.alloc(totB)) != hipSuccess) return bad("spawn slice bounds", e);
+    if ((e = b->sbe.alloc(2 * totB)) != hipSuccess) return bad("spawn slice bounds", e);
+    if ((e = b->sbp.alloc(2 * totB)) != hipSuccess) return bad("spawn slice bounds", e);
     if ((e = hipMemcpyAsync(b->wg_run.p, wg_run.data(), b->totK * sizeof(int), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     // the persistent step: one work queue per XCD, the runs dealt to the least loaded queue
     // (by spawn slices), largest first
@@ -1273,14 +1281,15 @@ int rimu_batch_create(const rimu_batch_run* runs, int32_t nruns, int64_t history
     if ((e = hipMemcpyAsync(b->runs.p, st.data(), nruns * sizeof(RunState), hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     if ((e = hipMemcpyAsync(b->start.p, start.data(), nruns * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     std::vector<int64_t> pops(nruns);  // host buffers live until the stream is synchronized below
-    // slice bounds of the start vector (one entry holding every attempt)
-    std::vector<int64_t> sb_e(totB, 0), sb_p(totB, 0);
+    // slice bounds of the start vector (one entry holding every attempt) in the first copy;
+    // both copies start with sbe[0] = sbp[0] = 0, which no step writes
+    std::vector<int64_t> sb_e(2 * totB, 0), sb_p(2 * totB, 0);
     for (int r = 0; r < nruns; ++r) {
         sb_e[st[r].offB + st[r].K] = 1;
         sb_p[st[r].offB + st[r].K] = std::llabs(runs[r].start_population);
     }
-    if ((e = hipMemcpyAsync(b->sbe.p, sb_e.data(), totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
-    if ((e = hipMemcpyAsync(b->sbp.p, sb_p.data(), totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->sbe.p, sb_e.data(), 2 * totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
+    if ((e = hipMemcpyAsync(b->sbp.p, sb_p.data(), 2 * totB * 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
     for (int r = 0; r < nruns; ++r) {  // the start vector: start_address => start_population
         pops[r] = runs[r].start_population;
         if ((e = hipMemcpyAsync(b->vk[0].p + st[r].offV, &start[r], 8, hipMemcpyHostToDevice, b->stream)) != hipSuccess) return bad("upload", e);
@@ -1409,10 +1418,10 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         HIP_TRY(s_runs.alloc(b->nruns));
         // copies on the batch's stream, ordered with its kernels
         HIP_TRY(hipMemcpyAsync(s_runs.p, b->runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(s_sbe.alloc(b->totB));
-        HIP_TRY(s_sbp.alloc(b->totB));
-        HIP_TRY(hipMemcpyAsync(s_sbe.p, b->sbe.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(s_sbp.p, b->sbp.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(s_sbe.alloc(2 * b->totB));
+        HIP_TRY(s_sbp.alloc(2 * b->totB));
+        HIP_TRY(hipMemcpyAsync(s_sbe.p, b->sbe.p, 2 * b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(s_sbp.p, b->sbp.p, 2 * b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
         for (int k = 0; k < 2; ++k) {
             HIP_TRY(s_vk[k].alloc(b->totV));
             HIP_TRY(s_vv[k].alloc(b->totV));
@@ -1421,8 +1430,8 @@ int rimu_batch_autotune(rimu_batch* b, int64_t trial_steps, int32_t* best_block,
         }
         auto restore = [&]() -> int {
             HIP_TRY(hipMemcpyAsync(b->runs.p, s_runs.p, b->nruns * sizeof(RunState), hipMemcpyDeviceToDevice, b->stream));
-            HIP_TRY(hipMemcpyAsync(b->sbe.p, s_sbe.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
-            HIP_TRY(hipMemcpyAsync(b->sbp.p, s_sbp.p, b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipMemcpyAsync(b->sbe.p, s_sbe.p, 2 * b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipMemcpyAsync(b->sbp.p, s_sbp.p, 2 * b->totB * 8, hipMemcpyDeviceToDevice, b->stream));
             for (int k = 0; k < 2; ++k) {
                 HIP_TRY(hipMemcpyAsync(b->vk[k].p, s_vk[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));
                 HIP_TRY(hipMemcpyAsync(b->vv[k].p, s_vv[k].p, b->totV * 8, hipMemcpyDeviceToDevice, b->stream));

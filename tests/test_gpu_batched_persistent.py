@@ -60,7 +60,7 @@ def test_persistent_wait_gives_up(gpu):
     """The first run of the first queue never reports its spawn slices (forced from the host):
     its sort's bounded wait gives up, every later item is skipped, every workgroup exits and
     the step reports the error; the runs keep a consistent state (a step is either complete
-    or not begun)."""
+    or not begun) and step on from it."""
     sp = _mixed(gpu)[:6]
     b = gpu.BatchedRuns(sp, history_steps=10)
     b.set_persistent(True)
@@ -70,6 +70,18 @@ def test_persistent_wait_gives_up(gpu):
         b.step(1, steps_per_graph=1)
     st, ln = b.status()
     assert (st == 0).all() and (ln > 0).all()
+    # the batch goes on from that state: every run's vector, length and norm agree with its
+    # last history row (a run whose step was cut off kept its vector and slice bounds)
+    b.set_wait(1 << 25, withhold_first=False)
+    b.step(5, steps_per_graph=1)
+    st, ln = b.status()
+    h = b.history()
+    assert (st == 0).all()
+    for r in range(len(sp)):
+        keys, vals = b.vector(r)
+        assert len(keys) == ln[r] and (keys[1:] > keys[:-1]).all()
+        last = h["norm"][:, r].nonzero()[0][-1]
+        assert h["len"][last, r] == ln[r] and h["norm"][last, r] == np.abs(vals).sum()
     b.close()
 
 
