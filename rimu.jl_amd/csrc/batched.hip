@@ -34,10 +34,9 @@ namespace {
 
 constexpr int MIPT = 2;          // merged positions per thread per merge tile
 constexpr int SPAWN_CH = 1024;   // entries per spawn chunk (SPAWN_CH / NT per thread)
-// attempts per thread per spawn group: 8, fewer where the (attempt item, wavefront) counts
-// would not fit one wavefront
-template <int NT>
-constexpr int spawn_ait() { return 8 * (NT / WAVE) <= WAVE ? 8 : WAVE / (NT / WAVE); }
+// attempts per thread per spawn group (eight measured slower: more registers, one
+// wavefront less per SIMD)
+constexpr int AIT = 4;
 constexpr double WALKERS_PER_SLICE = 2048;  // target walkers per spawn workgroup of a run
 constexpr int64_t MAX_SLICES = 64;
 constexpr int SPAWN_NT = 256;                // spawn workgroup size (several per CU)
@@ -100,7 +99,7 @@ __device__ inline int64_t isgn(double x) { return (int64_t)((x > 0) - (x < 0)); 
 // (A walkers; attempts numbered entry by entry) and the entries holding them.  Records go,
 // in (entry, attempt) order, to the run's spawn buffer (slice 0) or to the slice's own
 // region; the run's sort gathers the slices in order.
-template <int NT, int AIT = spawn_ait<NT>()>
+template <int NT>
 struct SpawnLds {
     int cnt[2][AIT * (NT / WAVE)];  // double-buffered by attempt group
     int tot[2];
@@ -109,8 +108,8 @@ struct SpawnLds {
     int64_t aoff[SPAWN_CH + 1];  // attempt offsets within the chunk (counts first)
 };
 
-template <int NT, int AIT = spawn_ait<NT>()>
-__device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, SpawnLds<NT, AIT>& sp,
+template <int NT>
+__device__ void spawn_phase(RunState* __restrict__ runs, const Pools& P, int64_t gslice, SpawnLds<NT>& sp,
                             int tid) {
     constexpr int NW = NT / WAVE;
     constexpr int EPT = SPAWN_CH / NT;    // entries per thread per chunk
@@ -841,7 +840,7 @@ struct StepSync {
 
 template <int NT>
 union StepLds {
-    SpawnLds<NT, 4> sp;  // four attempts per thread: eight spill in the fused kernel
+    SpawnLds<NT> sp;
     SortLds<NT, false> so;
     MergeLds<NT> me;
 };
@@ -927,7 +926,7 @@ __global__ __launch_bounds__(NT, 4) void batch_step_kernel(RunState* __restrict_
         const bool go = __builtin_amdgcn_readfirstlane(s_go) != 0;
         __syncthreads();  // s_ticket and s_go are written again by the next round
         if (!go) continue;
-        if (phase == 0) spawn_phase<NT, 4>(runs, P, idx, sm.sp, thread_index());
+        if (phase == 0) spawn_phase<NT>(runs, P, idx, sm.sp, thread_index());
         else if (phase == 1) sort_phase<NT, false>(runs, P, run, radix_bits, sm.so, thread_index());
         else if (phase == 2) merge_phase<NT>(runs, P, idx, sm.me, thread_index());
         else if (phase == 3) copy_phase<NT>(runs, P, idx, thread_index());
