@@ -759,15 +759,39 @@ __device__ void final_phase(RunState* __restrict__ runs, int nruns, const Pools&
     // ---- ProjectedEnergy, statistics and the shift update -----------------------------
     double vp = 0, hp = 0;
     {
+        // each projector key's first position >= key, found by a group of G lanes: every
+        // round samples G positions of [lo, hi), so a round of dependent loads divides it by
+        // G + 1 (about four rounds instead of a binary search's 15 on the sweep's largest runs)
+        constexpr int G = 16;
+        __shared__ double s_x[MAXPROJ];
         const int64_t np = st.nproj;
-        for (int64_t q = tid; q < np; q += NT) {
-            const uint64_t key = P.pk[st.offP + q];
-            int64_t lo = 0, hi = Lnew;
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (nk[mid] < key) lo = mid + 1; else hi = mid;
+        const int g = tid / G, l = tid % G;
+        const int gshift = (tid % WAVE) / G * G;  // the group's bits in a wavefront ballot
+        for (int64_t q0 = 0; q0 < np; q0 += NT / G) {
+            const int64_t q = q0 + g;
+            const bool act = q < np;  // uniform over the group
+            const uint64_t key = act ? P.pk[st.offP + q] : 0;
+            int64_t lo = 0, hi = act ? Lnew : 0;  // nk[lo - 1] < key <= nk[hi] (where they exist)
+            while (__ballot(hi > lo)) {
+                const int64_t span = hi - lo;
+                const int64_t sp_l = lo + span * (l + 1) / (G + 1);  // in [lo, hi) when span > 0
+                const bool ge = span > 0 && nk[sp_l] >= key;
+                const uint32_t m = (uint32_t)(__ballot(ge) >> gshift) & ((1u << G) - 1u);
+                if (span > 0) {
+                    if (m == 0) {
+                        lo = lo + span * G / (G + 1) + 1;  // past the last sample
+                    } else {
+                        const int f = __builtin_ctz(m);  // the first sample >= key
+                        hi = lo + span * (f + 1) / (G + 1);
+                        if (f > 0) lo = lo + span * f / (G + 1) + 1;
+                    }
+                }
             }
-            const double x = (lo < Lnew && nk[lo] == key) ? (double)nv[lo] : 0.0;
+            if (act && l == 0) s_x[q] = (lo < Lnew && nk[lo] == key) ? (double)nv[lo] : 0.0;
+        }
+        __syncthreads();
+        for (int64_t q = tid; q < np; q += NT) {
+            const double x = s_x[q];
             hp += P.pc[st.offP + q] * x;
             if (q == 0) vp = x;   // entry 0 is the start address itself
         }
