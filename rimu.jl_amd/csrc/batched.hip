@@ -4,13 +4,13 @@
 // A step of every run has five phases, the plain engine's algorithm at the size of a run:
 // spawn slices (a run's spawn attempts split evenly over K workgroups, one per 2048 target
 // walkers: diagonal step and spawn attempts, ballot compaction into the slice's spawn
-// buffer), the
-// run's sort (one workgroup: gather of the slices in order, stable LSD radix sort with LDS
-// histograms and ballot ranks, merge tile boundaries), merge slices (the run's merge tiles
-// over K workgroups), copy slices (the tiles' survivors into the new vector, over K
-// workgroups) and the run's final phase (one workgroup: norm, ProjectedEnergy values,
-// history row and the DoubleLogUpdate shift update, shiftstrategy.jl:174-181), each phase
-// one launch covering all runs.  The step number, shift and lengths live in device memory,
+// buffer), the run's sort (one workgroup: gather of the slices in order, stable LSD radix
+// sort with LDS histograms and ballot ranks, merge tile boundaries), merge slices (the
+// run's merge tiles over K workgroups), copy slices (the tiles' survivors into the new
+// vector and the next step's slice bounds, over K workgroups) and the run's final phase
+// (one workgroup: norm, ProjectedEnergy values, history row and the DoubleLogUpdate shift
+// update, shiftstrategy.jl:174-181), each phase one launch covering all runs, or all five
+// in one persistent launch.  The step number, shift and lengths live in device memory,
 // so a sequence of steps is captured once in a HIP graph and replayed.
 //
 // Random draws are addressed by (seed, address, attempt, step) and integer walker sums are
